@@ -1,0 +1,139 @@
+"""Benchmark: single-state QRAM interference at 2^20 entries on one MI355X.
+
+One step = one pass of the operator-apply path over the workload: H(addr,20) on
+|0> (1 -> 2^20 entries), QRAMLoad data ^= M[addr] (M = 2^20 uint32 words),
+32 Z phases on the data bits, the load's uncompute, and the inverse H(addr,20)
+(radix sort + 2^20-point Walsh-Hadamard + ClearZero).  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "sparse entries/s + HBM GB/s; 2^20 QDA solve time (batched+learned vs plain)"
+HBM_PEAK_GBS = 8000.0
+
+
+class ProfRecord(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64),
+                ("total_ms", ctypes.c_double), ("total_bytes", ctypes.c_double)]
+
+
+def kernel_profile(lib):
+    cap = 128
+    arr = (ProfRecord * cap)()
+    cnt = ctypes.c_uint32(0)
+    if lib.sparq_prof_report(arr, cap, ctypes.byref(cnt)) != 0:
+        raise RuntimeError("sparq_prof_report failed")
+    out = []
+    for i in range(min(cnt.value, cap)):
+        r = arr[i]
+        out.append({"name": r.name.decode(), "launches": int(r.launches), "ms": r.total_ms, "bytes": r.total_bytes})
+    return sorted(out, key=lambda r: -r["ms"])
+
+
+def cpu_baseline(address_bits, data_bits, mem_words):
+    """The oracle (single-thread restatement of the reference CPU path) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import _oracle as o
+    from sparqsim_amd.workloads import qram_interference
+
+    o.System.clear()
+    st0 = o.SparseState()
+    o.AddRegister("addr", o.UnsignedInteger, address_bits)(st0)
+    o.AddRegister("data", o.UnsignedInteger, data_bits)(st0)
+    q = o.QRAMCircuit_qutrit(address_bits, data_bits, mem_words)
+    t0 = time.perf_counter()
+    _, ops = qram_interference(o, q, address_bits, data_bits)
+    dt = time.perf_counter() - t0
+    o.System.clear()
+    return ops / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--address-bits", type=int, default=20)
+    ap.add_argument("--data-bits", type=int, default=32)
+    ap.add_argument("--cpu-address-bits", type=int, default=18, help="size of the bounded CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this engine has no multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import sparqsim_amd as ps
+    from sparqsim_amd.workloads import qram_interference, qram_memory
+
+    A, D = args.address_bits, args.data_bits
+    lib = ctypes.CDLL(ps.ENGINE_LIBRARY)
+    mem = qram_memory(A, D, config_index=1)
+    ps.System.clear()
+    st0 = ps.SparseState()
+    ps.AddRegister("addr", ps.UnsignedInteger, A)(st0)
+    ps.AddRegister("data", ps.UnsignedInteger, D)(st0)
+    q = ps.QRAMCircuit_qutrit(A, D, mem)
+    print(f"# device: {ps.device_name()}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        st, ops = qram_interference(ps, q, A, D)
+        st.size()
+    lib.sparq_prof_reset()
+    lib.sparq_prof_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st, ops = qram_interference(ps, q, A, D)
+        n_out = st.size()             # synchronizes the state's stream
+    t1 = time.perf_counter()
+    lib.sparq_prof_enable(0)
+    prof = kernel_profile(lib)
+    # a second, unprofiled timing of the same steps (event bracketing adds a little)
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        st, ops = qram_interference(ps, q, A, D)
+        n_out = st.size()
+    t3 = time.perf_counter()
+
+    ms_per_step = (t3 - t2) / args.steps * 1e3
+    value = ops / (ms_per_step / 1e3)
+    dom = prof[0]
+    avg_ms = dom["ms"] / dom["launches"]
+    bytes_per_launch = dom["bytes"] / dom["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    kernels_ms = sum(r["ms"] for r in prof) / args.steps
+    hbm_total = sum(r["bytes"] for r in prof) / args.steps
+    result = {
+        "metric": METRIC, "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (splitmix64 seed 0x5EED2026+1)",
+        "config": {"workload": f"single-state QRAM interference, 2^{A} entries ({A}-bit addr, {D}-bit data, "
+                               f"2^{A} uint32 memory words), H->QRAMLoad->{D}xZ->QRAMLoad->H",
+                   "entries": 1 << A, "entry_ops_per_step": ops, "final_entries": n_out, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
+        "step_hbm_GBs": hbm_total / (ms_per_step * 1e-3) / 1e9,
+        "kernel_ms_per_step": kernels_ms,
+        "kernels": [{"name": r["name"], "launches_per_step": r["launches"] / args.steps,
+                     "ms_per_step": r["ms"] / args.steps,
+                     "GBs": (r["bytes"] / (r["ms"] * 1e-3) / 1e9) if r["ms"] > 0 else None} for r in prof[:12]],
+    }
+    if not args.no_cpu_baseline:
+        ca = args.cpu_address_bits
+        eps, secs = cpu_baseline(ca, D, qram_memory(ca, D, config_index=1))
+        result["cpu_baseline"] = {"value": eps, "unit": "entries/s", "cores": 1, "kind": "port",
+                                  "sample": f"same circuit at 2^{ca} entries, one pass ({secs:.1f} s), oracle single-thread"}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,247 @@
+/*
+ * sparq_hip.h -- thin host->HIP C ABI of the MI355X sparse-state engine.
+ *
+ * The engine keeps a sparse quantum state as a structure of arrays on one GPU:
+ * W packed uint64 key words per entry (word 0 most significant, word-major in
+ * HBM) and one FP64 complex amplitude per entry.  Registers are bit fields of
+ * the key (sparq_field_t); the C++ host layer (sparqsim_amd/csrc/host) maps the
+ * reference's process-global register table onto fields so that unsigned
+ * comparison of the packed words equals the reference canonical order
+ * System::operator< (SparQ/src/basic_components.cpp:461-474).
+ *
+ * Every call returns a status code; no C++ exception crosses this boundary.
+ * All work for one state is ordered on that state's HIP stream.  Calls that
+ * report a size (sparq_state_size, ...) synchronize that stream.
+ *
+ * The reference has no counterpart of this layer (its CUDA path calls thrust
+ * directly in SparQ/src/cuda); each entry point below names the reference
+ * operator(s) whose semantics it implements.
+ */
+#ifndef SPARQ_HIP_H
+#define SPARQ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+enum sparq_status {
+    SPARQ_OK = 0,
+    SPARQ_OVERFLOW = 1,   /* a predicted capacity was too small; the exact path must be rerun */
+    SPARQ_OOM = 2,        /* device allocation failed; the state is unchanged */
+    SPARQ_INVALID = 3,    /* bad argument (maps to std::invalid_argument / ValueError) */
+    SPARQ_RUNTIME = 4,    /* HIP error or violated invariant (std::runtime_error / RuntimeError) */
+    SPARQ_NO_DEVICE = 5   /* no usable GPU */
+};
+
+#define SPARQ_MAX_WORDS 8          /* 512 key bits */
+#define SPARQ_MAX_PROGRAM_OPS 16   /* ops fused into one pointwise launch */
+#define SPARQ_MAX_PROGRAM_CONDS 32 /* condition predicates per pointwise launch */
+
+typedef struct sparq_state_s* sparq_state_t;
+typedef struct sparq_table_s* sparq_table_t;
+
+/* A register inside the packed key: bits [shift, shift+width) of key word `word`. */
+typedef struct {
+    uint8_t word;
+    uint8_t shift;
+    uint8_t width;
+    uint8_t flags; /* bit 0: register is SignedInteger (used by sign-extending ops) */
+} sparq_field_t;
+
+/* Condition predicate, AND-combined (global_macros.h:28-92). */
+enum sparq_cond_kind {
+    SPARQ_COND_NONZERO = 0,  /* field != 0                         (conditioned_by_nonzeros) */
+    SPARQ_COND_ALL_ONES = 1, /* field == 2^width - 1               (conditioned_by_all_ones) */
+    SPARQ_COND_BIT = 2,      /* bit `value` of field is 1          (conditioned_by_bit)      */
+    SPARQ_COND_VALUE = 3     /* field == value                     (conditioned_by_value)    */
+};
+typedef struct {
+    uint32_t kind;
+    sparq_field_t field;
+    uint64_t value;
+} sparq_cond_t;
+
+/* Per-entry operations (key rewrites and diagonal phases).  Operand fields are
+ * f[0..3]; integer constants c[0..2]; complex constants d[0..3] as (re,im)
+ * pairs.  Each op applies only to entries satisfying its conditions
+ * conds[cond_first .. cond_first+ncond). */
+enum sparq_pw_opcode {
+    SPARQ_PW_FLIP_BIT = 1,        /* f0 ^= 1<<c0                                  X_Bool (basic_gates.cpp:383-400) */
+    SPARQ_PW_FLIP_ALL,            /* f0 = ~f0                                     FlipBools (quantum_arithmetic.cpp:11-30) */
+    SPARQ_PW_SWAP_BITS,           /* swap bit c0 of f0 with bit c1 of f1          Swap_Bool_Bool (:32-61) */
+    SPARQ_PW_SWAP_FIELDS,         /* swap f0, f1                                  Swap_General_General (:982-999) */
+    SPARQ_PW_ROTL,                /* rotate f0 left by c0                         ShiftLeft_InPlace (:63-93) */
+    SPARQ_PW_ROTR,                /* rotate f0 right by c0                        ShiftRight_InPlace (:95-121) */
+    SPARQ_PW_XOR_MUL_CONST,       /* f1 ^= f0*c0                                  Mult_UInt_ConstUInt (:137-155) */
+    SPARQ_PW_ADD_MUL_CONST,       /* f1 += f0*c0  (c0 two's complement for dag)   Add_Mult_UInt_ConstUInt_InPlace (:157-195) */
+    SPARQ_PW_MODMUL_CONST,        /* f0 = (f0*c0) % c1                            Mod_Mult_UInt_ConstUInt_InPlace (:222-285) */
+    SPARQ_PW_XOR_ADD,             /* f2 ^= f0+f1                                  Add_UInt_UInt (:287-306) */
+    SPARQ_PW_ADD_FIELD,           /* f1 += f0                                     Add_UInt_UInt_InPlace (:308-327) */
+    SPARQ_PW_SUB_FIELD,           /* f1 -= f0                                     Add_UInt_UInt_InPlace::dag (:329-349) */
+    SPARQ_PW_XOR_ADD_CONST,       /* f1 ^= f0+c0                                  Add_UInt_ConstUInt (:351-369) */
+    SPARQ_PW_ADD_CONST,           /* f0 += c0                                     Add_ConstUInt_InPlace (:371-411) */
+    SPARQ_PW_XOR_SUB,             /* f2 ^= f0-f1                                  Sub_UInt_UInt (:490-509) */
+    SPARQ_PW_XOR_NEG,             /* f1 ^= -f0                                    Neg_UInt (:511-530) */
+    SPARQ_PW_XOR_ABS,             /* f1 ^= |sext(f0)|                             Abs_SInt (:532-554) */
+    SPARQ_PW_XOR_MUL,             /* f2 ^= f0*f1                                  Mul_UInt_UInt (:556-575) */
+    SPARQ_PW_XOR_DIV,             /* f2 ^= f1 ? f0/f1 : 0                         Div_UInt_UInt (:577-600) */
+    SPARQ_PW_XOR_ISQRT,           /* f1 ^= isqrt(f0)                              Sqrt_UInt (:602-621) */
+    SPARQ_PW_XOR_SELECT,          /* f3 ^= (f0&1) ? f1 : f2                       Select_Bool_UInt_UInt (:623-644) */
+    SPARQ_PW_XOR_AND,             /* f2 ^= f0&f1                                  And_UInt_UInt (:646-665) */
+    SPARQ_PW_XOR_OR,              /* f2 ^= f0|f1                                  Or_UInt_UInt (:667-686) */
+    SPARQ_PW_XOR_XOR,             /* f2 ^= f0^f1                                  Xor_UInt_UInt (:688-707) */
+    SPARQ_PW_FLAG_LESS_S,         /* f2 ^= sext(f0) < sext(f1)                    Less_SInt_SInt (:709-729) */
+    SPARQ_PW_FLAG_CARRY,          /* f3 ^= carry(f0+f1) at width(f2)              Carry_UInt_UInt (:731-754) */
+    SPARQ_PW_FLAG_OVERFLOW_S,     /* f3 ^= signed overflow at width(f2)           Overflow_SInt_SInt (:756-785) */
+    SPARQ_PW_FLAG_MULOVERFLOW,    /* f3 ^= f0*f1 overflows width(f2)              MulOverflow_UInt_UInt (:787-817) */
+    SPARQ_PW_FLAG_ISZERO,         /* f1 ^= f0 == 0                                IsZero_UInt (:819-837) */
+    SPARQ_PW_FLAG_NEGATIVE,       /* f1 ^= sext(f0) < 0                           Negative_SInt (:839-856) */
+    SPARQ_PW_ADD_ANY,             /* f0 += ext(f1)                                Add_AnyInt_AnyInt_InPlace (:869-887) */
+    SPARQ_PW_SUB_ANY,             /* f0 -= ext(f1)                                Add_AnyInt_AnyInt_InPlace::dag (:889-907) */
+    SPARQ_PW_XOR_ASSIGN,          /* f1 ^= f0                                     Assign (:909-930) */
+    SPARQ_PW_COMPARE,             /* f0==f1 ? f3^=1 : f2^=(f0<f1)                 Compare_UInt_UInt (:932-959) */
+    SPARQ_PW_FLAG_LESS_U,         /* f2 ^= f0 < f1                                Less_UInt_UInt (:961-980) */
+    SPARQ_PW_XOR_MID,             /* f2 ^= (f0+f1)/2                              GetMid_UInt_UInt (:1001-1020) */
+    SPARQ_PW_XOR_DIVSQRT_ACOS,    /* f2 ^= rational(acos(sqrt(f0/f1))/2pi)        Div_Sqrt_Arccos_UInt_UInt (:413-433) */
+    SPARQ_PW_XOR_SQRTDIV_ACOS,    /* f2 ^= rational(acos(sext(f0)/sqrt(f1))/2pi)  Sqrt_Div_Arccos_Int_UInt (:435-456) */
+    SPARQ_PW_XOR_ROTATE_ANGLE,    /* f2 ^= rational(atan2(sext f1, sext f0)/2pi)  GetRotateAngle_Int_Int (:458-488) */
+    SPARQ_PW_QRAM_LOAD,           /* f1 ^= table[f0 mod 2^c0]                     QRAMLoad::noise_free_impl (qram.cpp:15-35) */
+    SPARQ_PW_PHASE_BITS,          /* for each set bit b of (f0 & c0), ascending:
+                                     amp *= d0                                    Phase_Bool/Z/S/T (basic_gates.cpp:11-55) */
+    SPARQ_PW_RZ,                  /* bit c0 of f0 ? amp*=d0 : amp*=d1             RZ_Bool (basic_gates.cpp:486-520) */
+    SPARQ_PW_Y,                   /* flip bit c0; bit set ? amp*=i : amp*=-i      Y_Bool (basic_gates.cpp:402-420) */
+    SPARQ_PW_ROT_DIAG,            /* bit c0 of f0 ? amp*=d1 : amp*=d0            Rot_Bool diagonal (basic_gates.cpp:104-122) */
+    SPARQ_PW_ROT_OFFDIAG,         /* bit c0 set ? amp*=d1 : amp*=d0; flip bit     Rot_Bool off-diagonal (basic_gates.cpp:134-157) */
+    SPARQ_PW_NEG_AMP,             /* amp *= -1                                    ZeroConditionalPhaseFlip / Reflection_Bool
+                                                                                  (parallel_phase_operations.cpp:11-70) */
+    SPARQ_PW_MUL_AMP,             /* amp *= d0                                    GlobalPhase (parallel_phase_operations.cpp:72-108) */
+    SPARQ_PW_SET_FIELD,           /* f0 = c0                                      Init_Unsafe (dark_magic.cpp:35-41) */
+    SPARQ_PW_OPCODE_END
+};
+
+typedef struct {
+    uint32_t opcode;
+    uint16_t cond_first;
+    uint16_t ncond;
+    sparq_field_t f[4];
+    uint64_t c[3];
+    double d[8];            /* d[2k], d[2k+1] = re, im of complex constant k */
+    const void* table;      /* device table (sparq_table_device_ptr) for SPARQ_PW_QRAM_LOAD */
+    uint32_t table_word_bytes;
+    uint32_t pad;
+} sparq_pw_op_t;
+
+/* ------------------------------------------------------------ device & state */
+int sparq_device_init(int device);
+int sparq_device_name(char* buf, size_t len);
+
+/* Create an empty state with `words` key words.  (SparseState(), basic_components.h:611-624,
+ * is sparq_state_create + sparq_state_set_basis(0).) */
+int sparq_state_create(uint32_t words, uint64_t capacity, sparq_state_t* out);
+int sparq_state_destroy(sparq_state_t st);
+int sparq_state_clone(sparq_state_t src, sparq_state_t* out);
+int sparq_state_words(sparq_state_t st, uint32_t* words);
+int sparq_state_size(sparq_state_t st, uint64_t* n);
+/* Replace the content by the single entry |key> with amplitude 1. */
+int sparq_state_set_basis(sparq_state_t st, const uint64_t* key);
+/* keys: n*W words, entry-major (entry i word w at keys[i*W+w]); amps: 2n doubles (re,im). */
+int sparq_state_upload(sparq_state_t st, const uint64_t* keys, const double* amps, uint64_t n);
+int sparq_state_download(sparq_state_t st, uint64_t* keys, double* amps, uint64_t capacity, uint64_t* n);
+/* Move fields to a new layout (register add/remove/split/combine, system_operations.cpp:124-369).
+ * Bits of the new key not covered by any destination field are zero. */
+int sparq_state_relayout(sparq_state_t st, uint32_t new_words,
+                         const sparq_field_t* from, const sparq_field_t* to, uint32_t nfields);
+int sparq_state_sync(sparq_state_t st);
+/* Raw device pointers of the current arrays (valid until the next call that changes the size). */
+int sparq_state_device_arrays(sparq_state_t st, uint64_t** keys, double** amps, uint64_t* capacity);
+/* The HIP stream (hipStream_t) that all work of this state is ordered on. */
+int sparq_state_stream(sparq_state_t st, void** stream);
+
+/* Device lookup table for QRAMLoad (the QRAM memory, qram.cpp:29-31). word_bytes in {1,2,4,8}. */
+int sparq_table_create(const void* host_words, uint64_t nwords, uint32_t word_bytes, sparq_table_t* out);
+int sparq_table_destroy(sparq_table_t t);
+int sparq_table_device_ptr(sparq_table_t t, const void** ptr);
+
+/* ------------------------------------------------------------ operators */
+/* Fused per-entry program (all key-rewrite and diagonal operators). */
+int sparq_pointwise(sparq_state_t st, const sparq_pw_op_t* ops, uint32_t nops,
+                    const sparq_cond_t* conds, uint32_t nconds);
+
+/* ClearZero: stable removal of |amp|^2 < eps (system_operations.cpp:371-382). */
+int sparq_clear_zero(sparq_state_t st, double eps);
+
+/* Sort by the full key, unsigned lexicographic over the words (SortUnconditional,
+ * sort_state.cpp:84-88); when `last` is non-NULL the bits of that field are moved
+ * to the least significant position (SortExceptKey, sort_state.cpp:10-36). */
+int sparq_sort(sparq_state_t st, const sparq_field_t* last);
+
+/* sort_merge_unique_erase: sort, sum amplitudes of equal keys, drop |sum|^2 < eps
+ * (system_operations.h:157-181, basic_components.cpp:517-526). */
+int sparq_merge_duplicates(sparq_state_t st, double eps);
+/* add_systems(cur, add, coef) (system_operations.h:157-181). `add` is unchanged. */
+int sparq_add_systems(sparq_state_t cur, sparq_state_t add, double coef, double eps);
+/* CheckDuplicateKey (debugger.cpp:214-222): sorts; *has_dup = 1 when two entries share a key. */
+int sparq_check_duplicates(sparq_state_t st, int* has_dup);
+
+/* Hadamard_Int on the low n_digits bits of `target` (hadamard.cpp:18-162), followed by
+ * ClearZero(eps).  Conditions are evaluated per entry. */
+int sparq_hadamard_int(sparq_state_t st, sparq_field_t target, uint32_t n_digits,
+                       const sparq_cond_t* conds, uint32_t ncond, double eps);
+/* Hadamard_Int_Full (hadamard.cpp:181-338) incl. its CheckDuplicateKey, then ClearZero(eps). */
+int sparq_hadamard_int_full(sparq_state_t st, sparq_field_t target,
+                            const sparq_cond_t* conds, uint32_t ncond, double eps);
+
+/* Two-branch operators on bit `bit` of `target` (pairs of entries equal elsewhere). */
+enum sparq_pair_kind {
+    SPARQ_PAIR_HADAMARD_BOOL = 0,   /* Hadamard_Bool (hadamard.cpp:340-439) */
+    SPARQ_PAIR_CONDROT_RATIONAL = 1,/* CondRot_Rational_Bool (condrot.cpp:12-151); angle from `in`, sign in mat[0].re */
+    SPARQ_PAIR_CONDROT_TABLE = 2,   /* CondRot_General_Bool_Fast (condrot.h:149-332); 2x2 per value of `in` from table */
+    SPARQ_PAIR_ROT_GENERAL = 3      /* Rot_Bool general branch (basic_gates.cpp:159-199) with mat */
+};
+typedef struct {
+    uint32_t kind;
+    sparq_field_t in;          /* angle / table index register */
+    double mat[8];             /* 2x2 complex row-major for ROT_GENERAL; mat[0] = +1/-1 for CONDROT_RATIONAL */
+    const double* table;       /* device: 8 doubles per value of `in` (CONDROT_TABLE) */
+} sparq_pair_params_t;
+int sparq_pair_op(sparq_state_t st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t* params,
+                  const sparq_cond_t* conds, uint32_t ncond, double eps);
+
+/* AddRegisterWithHadamard (system_operations.cpp:264-290): every entry is copied
+ * 2^width(target) times with the (zero) target field set to each value, amplitudes unchanged. */
+int sparq_replicate(sparq_state_t st, sparq_field_t target);
+
+/* Normalize (dark_magic.cpp:11-33). *norm2 receives the sum of |amp|^2 before scaling. */
+int sparq_normalize(sparq_state_t st, double* norm2);
+/* TestRemovable helper (debugger.cpp:191-212): *uniform = 1 when every entry has the same value. */
+int sparq_field_uniform(sparq_state_t st, sparq_field_t f, int* uniform, uint64_t* value);
+
+/* ------------------------------------------------------------ diagnostics */
+/* Message of the last failing call on this thread. */
+const char* sparq_last_error(void);
+/* Event timing helpers on a state's stream (used by the benchmark): record, then read ms. */
+int sparq_timer_record(sparq_state_t st, int slot);
+int sparq_timer_elapsed(sparq_state_t st, int slot_begin, int slot_end, float* ms);
+
+/* Per-kernel profiling: when enabled, every engine kernel launch is bracketed by
+ * HIP events on the stream it is launched on and credited with its algorithmic
+ * HBM bytes.  sparq_prof_report synchronizes and returns one record per kernel
+ * name (accumulated since the last reset). */
+typedef struct {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+    double total_bytes;
+} sparq_prof_record_t;
+int sparq_prof_enable(int on);
+int sparq_prof_reset(void);
+int sparq_prof_report(sparq_prof_record_t* out, uint32_t capacity, uint32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPARQ_HIP_H */

@@ -1,0 +1,740 @@
+// CPU oracle: a restatement of the reference SparQ CPU semantics for the
+// sparse-state operator-apply path (array-of-structs entries, process-global
+// register table, comparison sorts and ordered bucket maps).
+//
+// THIS IS TEST INFRASTRUCTURE.  Only the test-suite, the smoke check and the
+// CPU-baseline leg of bench.py load it, and only as the checker / reported
+// CPU baseline.  The product (sparqsim_amd) never links, imports or calls it.
+//
+// Every function cites the reference file:line it follows (paths are relative
+// to the reference repository root).  The reference's base layer
+// (extern/qram-simulator) is not vendored, so its helpers are re-derived from
+// their call sites; see the L0 block below.
+//
+// Build knob: ORACLE_OMP enables the reference's own `#pragma omp parallel for`
+// loops on the per-entry operators (the reference ships with SINGLE_THREAD,
+// SparQ/include/global_macros.h:10); sorts stay sequential as in the
+// reference (std::execution::seq, SparQ/include/basic_components.h:833-839).
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+namespace oracle {
+
+using complex_t = std::complex<double>;
+using u22_t = std::array<complex_t, 4>;   // row-major [u00,u01,u10,u11] (SparQ/include/condrot.h:19-41)
+
+// ---------------------------------------------------------------- L0 helpers
+// Re-derived from call sites (the defining submodule is absent).
+constexpr double pi = 3.141592653589793238462643383279502884;
+constexpr double sqrt2 = 1.4142135623730950488016887242096981;
+constexpr double sqrt2inv = 0.7071067811865475244008443621048490;   // 1/sqrt(2), correctly rounded
+constexpr double epsilon = 1e-14;                                      // CHANGELOG.md:108
+
+inline uint64_t pow2(size_t n) { return uint64_t(1) << n; }
+// basic_components.h:106, system_operations.cpp:156-157
+inline uint64_t width_mask(size_t w) { return w >= 64 ? ~uint64_t(0) : pow2(w) - 1; }
+// two's complement sign extension of a w-bit value (cuda/basic_components.cuh:74-78)
+inline int64_t get_complement(uint64_t v, size_t w) {
+    if (w == 0) return 0;
+    if (w >= 64) return int64_t(v);
+    v &= width_mask(w);
+    return int64_t(v << (64 - w)) >> (64 - w);
+}
+inline uint64_t flip_digit(uint64_t v, size_t d) { return v ^ pow2(d); }
+inline bool digit1(uint64_t v, size_t d) { return (v >> d) & 1; }
+inline bool digit0(uint64_t v, size_t d) { return !((v >> d) & 1); }
+inline bool get_digit(uint64_t v, size_t d) { return (v >> d) & 1; }
+inline double abs_sqr(complex_t c) { return c.real() * c.real() + c.imag() * c.imag(); }
+inline int bitcount(uint64_t v) { return __builtin_popcountll(v); }
+inline bool bit_parity(uint64_t v) { return __builtin_popcountll(v) & 1; }
+// Fixed-point encoding of a fraction x in [0,1) into w bits.  The reference
+// only pins |decode(encode(x)) - x| <= 2^-w
+// (test/CPUTest/CommonTest/CorrectnessTest_Common.inl:171-195); truncation
+// satisfies that bound.  Exact rounding: parity unpinned.
+inline uint64_t get_rational(double x, size_t w) {
+    double scaled = (w >= 64) ? x * 2.0 * double(pow2(63)) : x * double(pow2(w));
+    uint64_t v = scaled <= 0 ? 0 : (uint64_t)scaled;
+    return v & width_mask(w);
+}
+inline uint64_t isqrt_u64(uint64_t v) {
+    uint64_t r = (uint64_t)std::sqrt((double)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+enum StateStorageType { General = 0, UnsignedInteger = 1, SignedInteger = 2, Boolean = 3, Rational = 4 };
+
+[[noreturn]] inline void throw_invalid_input(const char* m = "invalid input") { throw std::invalid_argument(m); }
+[[noreturn]] inline void throw_runtime(const char* m = "runtime error") { throw std::runtime_error(m); }
+
+// --------------------------------------------------------- register table
+// SparQ/include/basic_components.h:239-286, SparQ/src/basic_components.cpp:341-459
+struct RegInfo { std::string name; StateStorageType type; size_t size; bool status; };
+
+struct Registry {
+    static std::vector<RegInfo>& map() { static std::vector<RegInfo> m; return m; }
+    static std::unordered_map<std::string, size_t>& index() { static std::unordered_map<std::string, size_t> i; return i; }
+    static std::vector<size_t>& reusable() { static std::vector<size_t> r; return r; }
+    static std::vector<size_t>& temporal() { static std::vector<size_t> t; return t; }
+    static size_t& max_qubit_count() { static size_t v = 0; return v; }
+    static size_t& max_register_count() { static size_t v = 0; return v; }
+    static size_t& max_system_size() { static size_t v = 0; return v; }
+
+    // basic_components.cpp:150-161
+    static void clear() {
+        map().clear(); index().clear(); reusable().clear(); temporal().clear();
+        max_qubit_count() = max_register_count() = max_system_size() = 0;
+    }
+    static size_t get_qubit_count() {
+        size_t c = 0; for (auto& m : map()) if (m.status) c += m.size; return c;
+    }
+    static size_t get_activated_register_size() {
+        size_t c = 0; for (auto& m : map()) if (m.status) ++c; return c;
+    }
+    static void update_max_size(size_t n) { if (n > max_system_size()) max_system_size() = n; }
+    // basic_components.cpp:206-228 (name index, then reverse linear scan)
+    static size_t get(const std::string& name) {
+        auto it = index().find(name);
+        if (it != index().end() && map()[it->second].status) return it->second;
+        for (size_t i = map().size(); i-- > 0;)
+            if (map()[i].status && map()[i].name == name) { index()[name] = i; return i; }
+        return SIZE_MAX;
+    }
+    static const RegInfo& info(size_t id) {
+        if (id >= map().size()) throw_runtime("Register not found.");
+        return map()[id];
+    }
+    static size_t size_of(size_t id) { return info(id).size; }
+    static StateStorageType type_of(size_t id) { return info(id).type; }
+    static bool status_of(size_t id) { return info(id).status; }
+    static const std::string& name_of(size_t id) { return info(id).name; }
+    // basic_components.cpp:344-381: LIFO reuse of freed ids, else append
+    static size_t add_register(const std::string& name, StateStorageType type, size_t size) {
+        if (size > 64) throw_invalid_input("register wider than 64 bits");
+        size_t id;
+        if (!reusable().empty()) {
+            id = reusable().back(); reusable().pop_back();
+            map()[id] = RegInfo{name, type, size, true};
+        } else {
+            map().push_back(RegInfo{name, type, size, true});
+            id = map().size() - 1;
+        }
+        index()[name] = id;
+        max_qubit_count() = std::max(max_qubit_count(), get_qubit_count());
+        max_register_count() = std::max(max_register_count(), get_activated_register_size());
+        return id;
+    }
+    // basic_components.cpp:405-415
+    static void remove_register(size_t id) {
+        if (id >= map().size()) throw_runtime("Register not found.");
+        if (!map()[id].status) throw_runtime("Register is not activated.");
+        index().erase(map()[id].name);
+        map()[id].status = false;
+        reusable().push_back(id);
+    }
+};
+
+// ------------------------------------------------------------------ entries
+// One basis entry: amplitude + one stored uint64 per register id
+// (basic_components.h:221-297).  Values of inactive ids are kept but ignored.
+struct System {
+    complex_t amplitude = 1.0;
+    mutable std::vector<uint64_t> registers;
+    System() : registers(Registry::map().size(), 0) {}
+    uint64_t& get(size_t id) {
+        if (id >= Registry::map().size()) throw_runtime("Register not found.");
+        if (registers.size() < Registry::map().size()) registers.resize(Registry::map().size(), 0);
+        return registers[id];
+    }
+    uint64_t get(size_t id) const {
+        if (id >= Registry::map().size()) throw_runtime("Register not found.");
+        if (registers.size() < Registry::map().size()) registers.resize(Registry::map().size(), 0);
+        return registers[id];
+    }
+    // StateStorage::as<uint64_t>/as<bool>/as<int64_t>/as<double> (basic_components.h:102-131)
+    uint64_t as_uint(size_t id) const { return get(id) & width_mask(Registry::size_of(id)); }
+    bool as_bool(size_t id) const { return as_uint(id) != 0; }
+    int64_t as_int(size_t id) const { return get_complement(as_uint(id), Registry::size_of(id)); }
+    double as_double(size_t id) const {
+        size_t w = Registry::size_of(id);
+        uint64_t t = as_uint(id);
+        if (w == 64) return t * 1.0 / 2 / double(pow2(63));
+        return 1.0 * t / double(pow2(w));
+    }
+    // Canonical order: active ids ascending, full stored values (basic_components.cpp:461-490)
+    bool operator<(const System& r) const {
+        for (size_t i = 0; i < Registry::map().size(); ++i) {
+            if (!Registry::status_of(i)) continue;
+            uint64_t a = get(i), b = r.get(i);
+            if (a < b) return true;
+            if (b < a) return false;
+        }
+        return false;
+    }
+    bool operator==(const System& r) const {
+        for (size_t i = 0; i < Registry::map().size(); ++i) {
+            if (!Registry::status_of(i)) continue;
+            if (get(i) != r.get(i)) return false;
+        }
+        return true;
+    }
+};
+
+using State = std::vector<System>;
+
+// SparseState() holds |0...0> with amplitude 1 (basic_components.h:611-624)
+struct SparseState {
+    State basis_states;
+    SparseState() { basis_states.emplace_back(); }
+    explicit SparseState(size_t n) : basis_states(n) {}
+    size_t size() const { return basis_states.size(); }
+};
+
+// ------------------------------------------------------------- conditions
+// ClassControllable (global_macros.h:28-92, 104-284): AND of all predicates.
+struct Conditions {
+    std::vector<size_t> nonzeros;
+    std::vector<size_t> all_ones;
+    std::vector<std::pair<size_t, size_t>> by_bit;
+    std::vector<std::pair<size_t, uint64_t>> by_value;
+    bool any() const { return !nonzeros.empty() || !all_ones.empty() || !by_bit.empty() || !by_value.empty(); }
+    bool satisfied(const System& s) const {
+        for (size_t id : all_ones) if (s.get(id) != pow2(Registry::size_of(id)) - 1) return false;
+        for (auto& p : by_bit) if (!(s.get(p.first) & pow2(p.second))) return false;
+        for (size_t id : nonzeros) if (!s.as_bool(id)) return false;
+        for (auto& p : by_value) if (s.get(p.first) != p.second) return false;
+        return true;
+    }
+};
+
+// --------------------------------------------------------- sort primitives
+// SortExceptKey (sort_state.cpp:10-36): all active registers except id, then id.
+inline void SortExceptKey(State& st, size_t id) {
+    std::sort(st.begin(), st.end(), [id](const System& l, const System& r) {
+        for (size_t i = 0; i < Registry::map().size(); ++i) {
+            if (!Registry::status_of(i) || i == id) continue;
+            uint64_t a = l.get(i), b = r.get(i);
+            if (a < b) return true;
+            if (a > b) return false;
+        }
+        return l.get(id) < r.get(id);
+    });
+}
+// SortUnconditional (sort_state.cpp:84-88)
+inline void SortUnconditional(State& st) { std::sort(st.begin(), st.end()); }
+// SortExceptBit (sort_state.cpp:55-82): others, then id with bit `digit` masked out.
+inline void SortExceptBit(State& st, size_t id, size_t digit) {
+    uint64_t mask = ~pow2(digit);
+    std::sort(st.begin(), st.end(), [id, mask](const System& l, const System& r) {
+        for (size_t i = 0; i < Registry::map().size(); ++i) {
+            if (!Registry::status_of(i) || i == id) continue;
+            uint64_t a = l.get(i), b = r.get(i);
+            if (a < b) return true;
+            if (a > b) return false;
+        }
+        return (l.get(id) & mask) < (r.get(id) & mask);
+    });
+}
+// compare_equal (sort_state.cpp:173-188): equal on all active registers but out_id
+inline bool compare_equal(const System& a, const System& b, size_t out_id) {
+    for (size_t i = Registry::map().size(); i-- > 0;) {
+        if (!Registry::status_of(i) || i == out_id) continue;
+        if (a.get(i) != b.get(i)) return false;
+    }
+    return true;
+}
+// compare_equal_rot (sort_state.cpp:197-217)
+inline bool compare_equal_rot(const System& a, const System& b, size_t out_id, uint64_t mask) {
+    for (size_t i = Registry::map().size(); i-- > 0;) {
+        if (!Registry::status_of(i)) continue;
+        if (i == out_id) { if ((a.get(i) & mask) != (b.get(i) & mask)) return false; continue; }
+        if (a.get(i) != b.get(i)) return false;
+    }
+    return true;
+}
+// StateLessExceptKey (quantum_interfere_basic.cpp:10-40): order on all active
+// registers except `id`; used as the bucket-map comparator (SAFE_HASH path).
+struct StateLessExceptKey {
+    size_t id;
+    bool operator()(const System& l, const System& r) const {
+        for (size_t i = 0; i < Registry::map().size(); ++i) {
+            if (!Registry::status_of(i) || i == id) continue;
+            uint64_t a = l.get(i), b = r.get(i);
+            if (a < b) return true;
+            if (a > b) return false;
+        }
+        return false;
+    }
+};
+
+// ------------------------------------------------------------ structural
+// ClearZero (system_operations.cpp:371-382): stable removal of |a|^2 < eps.
+inline double& clearzero_default_eps() { static double e = epsilon; return e; }   // system_operations.h:577
+inline void ClearZero(State& st, double eps) {
+    st.erase(std::remove_if(st.begin(), st.end(),
+        [eps](const System& s) { return abs_sqr(s.amplitude) < eps; }), st.end());
+}
+inline void ClearZero(State& st) { ClearZero(st, clearzero_default_eps()); }
+
+// sort_merge_unique_erase semantics (documented at cuda/basic_components.cuh:549-560):
+// sort by the full key, fold each run of equal keys into its first element with
+// merge_system (amp sum, basic_components.cpp:517-521) and drop entries for which
+// remove_system (|a|^2 < epsilon, basic_components.cpp:523-526) holds.
+// Duplicates are folded in sorted order (the reference's std::sort is unstable,
+// so its summation order for >= 3 duplicates is unspecified: parity unpinned).
+inline void sort_merge_unique_erase(State& st) {
+    std::stable_sort(st.begin(), st.end());
+    State out;
+    out.reserve(st.size());
+    size_t i = 0;
+    while (i < st.size()) {
+        System head = st[i];
+        size_t j = i + 1;
+        while (j < st.size() && st[j] == head) { head.amplitude += st[j].amplitude; ++j; }
+        if (!(abs_sqr(head.amplitude) < epsilon)) out.push_back(std::move(head));
+        i = j;
+    }
+    st.swap(out);
+}
+// add_systems (system_operations.h:157-181)
+inline void add_systems(State& cur, const State& add, double coef) {
+    if (add.empty()) return;
+    size_t orig = cur.size();
+    cur.insert(cur.end(), add.begin(), add.end());
+    if (std::abs(coef - 1.0) > epsilon)
+        for (size_t i = orig; i < cur.size(); ++i) cur[i].amplitude *= coef;
+    if (orig != 0) sort_merge_unique_erase(cur);
+}
+
+// split_systems / combine_systems (system_operations.cpp:53-106):
+// partition by the predicate; unsatisfied entries are appended back afterwards.
+inline State split_systems(State& st, const Conditions& c) {
+    auto p = std::partition(st.begin(), st.end(), [&c](const System& s) { return c.satisfied(s); });
+    State rest(p, st.end());
+    st.erase(p, st.end());
+    return rest;
+}
+inline void combine_systems(State& to, const State& from) { to.insert(to.end(), from.begin(), from.end()); }
+
+// AddRegister (system_operations.cpp:251-262; basic_components.cpp:378-388)
+inline size_t AddRegister(State& st, const std::string& name, StateStorageType type, size_t size) {
+    if (size > 64) throw_invalid_input();
+    size_t id = Registry::add_register(name, type, size);
+    for (auto& s : st) s.get(id) = 0;
+    return id;
+}
+// AddRegisterWithHadamard (system_operations.cpp:264-290)
+inline size_t AddRegisterWithHadamard(State& st, const std::string& name, StateStorageType type, size_t size) {
+    size_t id = AddRegister(st, name, type, size);
+    size_t full = pow2(size), orig = st.size();
+    st.resize(orig * full);
+    for (size_t i = 1; i < full; ++i) {
+        std::copy(st.begin(), st.begin() + orig, st.begin() + i * orig);
+        for (size_t j = i * orig; j < (i + 1) * orig; ++j) st[j].get(id) = i;
+    }
+    return id;
+}
+// TestRemovable (debugger.cpp:191-212)
+inline void TestRemovable(const State& st, size_t id) {
+    if (st.empty()) return;
+    uint64_t v = st[0].as_uint(id);
+    for (size_t i = 1; i < st.size(); ++i)
+        if (st[i].as_uint(id) != v) throw_runtime("Removing failed: register is entangled");
+}
+// RemoveRegister (system_operations.cpp:303-322; basic_components.cpp:425-452)
+inline void RemoveRegister(State& st, size_t id) {
+    TestRemovable(st, id);
+    if (id >= Registry::map().size()) throw_runtime("Register not found.");
+    for (auto& s : st) s.get(id) = 0;
+    Registry::remove_register(id);
+}
+// SplitRegister (system_operations.cpp:134-166)
+inline size_t SplitRegister(State& st, const std::string& first, const std::string& second, size_t second_size) {
+    size_t fp = Registry::get(first);
+    size_t orig = Registry::size_of(fp);
+    if (orig < second_size) throw_invalid_input();
+    StateStorageType type = Registry::type_of(fp);
+    size_t sp = AddRegister(st, second, type, second_size);
+    Registry::map()[fp].size = orig - second_size;
+    uint64_t m2 = width_mask(second_size);
+    for (auto& s : st) {
+        uint64_t& v = s.get(fp);
+        s.get(sp) = v & m2;
+        v = second_size == 64 ? 0 : v >> second_size;
+    }
+    return sp;
+}
+// CombineRegister (system_operations.cpp:172-203)
+inline size_t CombineRegister(State& st, const std::string& first, const std::string& second) {
+    size_t fp = Registry::get(first), sp = Registry::get(second);
+    size_t w1 = Registry::size_of(fp), w2 = Registry::size_of(sp);
+    Registry::map()[fp].size = w1 + w2;
+    uint64_t m1 = width_mask(w1), m2 = width_mask(w2);
+    for (auto& s : st) {
+        uint64_t& v = s.get(fp);
+        uint64_t a = v & m1, b = s.get(sp) & m2;
+        v = w2 == 64 ? b : (a << w2) | b;
+    }
+    Registry::remove_register(sp);
+    return fp;
+}
+// Init_Unsafe (dark_magic.cpp:35-41)
+inline void Init_Unsafe(State& st, size_t id, uint64_t value) { for (auto& s : st) s.get(id) = value; }
+// Normalize (dark_magic.cpp:11-33): accumulate |a|^2 in entry order, scale by 1/sqrt.
+inline void Normalize(State& st) {
+    double sum = 0.0;
+    for (auto& s : st) sum += abs_sqr(s.amplitude);
+    sum = 1.0 / std::sqrt(sum);
+    if (std::isnan(sum)) throw_runtime("bad result");
+    for (auto& s : st) s.amplitude *= sum;
+}
+
+// ------------------------------------------------------------- branching
+// Hadamard_Int (hadamard.cpp:18-162).  Groups are runs of entries equal on all
+// other registers after SortExceptKey.  The reference compares the target
+// register's upper bits only implicitly (Appendix: n_digits < width); this
+// restatement groups by (other registers, upper bits of the target) like the
+// reference GPU path (cuda/hadamard.cu:195-210), which coincides with the CPU
+// path whenever the upper bits are uniform within a group.
+struct Hadamard_Int {
+    size_t id, n_digits; uint64_t mask; Conditions cond;
+    Hadamard_Int(size_t id_, size_t n) : id(id_), n_digits(n) { mask = ~(pow2(n) - 1); }
+    void operate(size_t l, size_t r, State& st) const {
+        size_t n = r - l;
+        size_t full = pow2(n_digits);
+        size_t orig = st.size();
+        double extra = 1.0 / std::sqrt((double)full);
+        if (n == 0) return;
+        if (!cond.satisfied(st[l])) return;
+        if (n == 1) {                                   // hadamard.cpp:26-40
+            uint64_t v = st[l].get(id);
+            st[l].amplitude *= extra;
+            st.insert(st.end(), full, st[l]);
+            for (size_t i = 0; i < full; ++i) {
+                System& e = st[orig + i];
+                e.get(id) = (e.get(id) & mask) + i;
+                if (bitcount(i & v) & 1) e.amplitude *= -1;
+            }
+            st[l].amplitude = 0;
+        } else if (n == full) {                         // hadamard.cpp:42-61
+            for (size_t qn = 0; qn < n_digits; ++qn)
+                for (size_t i = 0; i < full; ++i)
+                    if ((i >> qn) & 1) {
+                        complex_t a0 = st[i + l - pow2(qn)].amplitude, a1 = st[i + l].amplitude;
+                        st[i + l - pow2(qn)].amplitude = (a0 + a1) * sqrt2inv;
+                        st[i + l].amplitude = (a0 - a1) * sqrt2inv;
+                    }
+        } else {                                        // hadamard.cpp:79-131 (Version 2)
+            st.insert(st.end(), full, st[l]);
+            uint64_t vm = st[l].get(id) & mask;
+            for (size_t i = 0; i < full; ++i) { st[orig + i].amplitude = 0; st[orig + i].get(id) = vm + i; }
+            for (size_t i = l; i < r; ++i) {
+                st[st[i].get(id) - vm + orig].amplitude = st[i].amplitude;
+                st[i].amplitude = 0;
+            }
+            for (size_t qn = 0; qn < n_digits; ++qn)
+                for (size_t i = 0; i < full; ++i)
+                    if ((i >> qn) & 1) {
+                        complex_t a0 = st[i + orig - pow2(qn)].amplitude, a1 = st[i + orig].amplitude;
+                        st[i + orig - pow2(qn)].amplitude = (a0 + a1) * sqrt2inv;
+                        st[i + orig].amplitude = (a0 - a1) * sqrt2inv;
+                    }
+        }
+    }
+    bool same_group(const System& a, const System& b) const {
+        return compare_equal(a, b, id) && ((a.get(id) & mask) == (b.get(id) & mask));
+    }
+    // hadamard.cpp:134-162
+    void operator()(State& st) const {
+        if (st.empty()) return;
+        SortExceptKey(st, id);
+        size_t cur = st.size(), l = 0, r = 1;
+        while (true) {
+            if (r == cur) { operate(l, r, st); break; }
+            if (!same_group(st[l], st[r])) { operate(l, r, st); l = r; r = l + 1; }
+            else ++r;
+        }
+        ClearZero(st);
+        Registry::update_max_size(st.size());
+    }
+    void dag(State& st) const { (*this)(st); }
+};
+
+// CheckDuplicateKey (debugger.cpp:214-222): sorts the state as a side effect.
+inline void CheckDuplicateKey(State& st) {
+    SortUnconditional(st);
+    for (size_t i = 1; i < st.size(); ++i)
+        if (st[i] == st[i - 1]) throw_runtime("duplicate key");
+}
+
+// Hadamard_Int_Full (hadamard.cpp:181-338), SAFE_HASH bucket-map path.
+struct Hadamard_Int_Full {
+    size_t id, n_digits, few_threshold, full_size; Conditions cond;
+    explicit Hadamard_Int_Full(size_t id_) : id(id_) {
+        n_digits = Registry::size_of(id); few_threshold = n_digits - 1; full_size = pow2(n_digits);
+    }
+    void operate_bucket_sparse(const std::vector<size_t>& pos, State& st) const {   // :269-304
+        double extra = 1.0 / std::sqrt((double)full_size);
+        size_t orig = st.size();
+        bool inserted = false;
+        for (size_t i = 0; i < pos.size(); ++i) {
+            if (pos[i] == size_t(-1)) continue;
+            uint64_t v = st[pos[i]].get(id);
+            if (!inserted) {
+                st.insert(st.end(), full_size, st[pos[i]]);
+                for (size_t j = 0; j < full_size; ++j) { st[orig + j].get(id) = j; st[orig + j].amplitude = 0; }
+                inserted = true;
+            }
+            for (size_t j = 0; j < full_size; ++j)
+                st[orig + j].amplitude += (st[pos[i]].amplitude * (bit_parity(j & v) ? -1. : 1.) * extra);
+            st[pos[i]].amplitude = 0;
+        }
+    }
+    void operate_bucket_inplace(const std::vector<size_t>& pos, State& st) const {  // :306-338
+        std::vector<complex_t> amps(pos.size(), 0);
+        for (size_t i = 0; i < pos.size(); ++i) amps[i] = st[pos[i]].amplitude;
+        for (size_t qn = 0; qn < n_digits; ++qn)
+            for (size_t i = 0; i < full_size; ++i)
+                if ((i >> qn) & 1) {
+                    complex_t a0 = amps[i - pow2(qn)], a1 = amps[i];
+                    amps[i - pow2(qn)] = (a0 + a1) * sqrt2inv;
+                    amps[i] = (a0 - a1) * sqrt2inv;
+                }
+        for (size_t i = 0; i < pos.size(); ++i) st[pos[i]].amplitude = amps[i];
+    }
+    void operator()(State& st) const {
+        CheckDuplicateKey(st);
+        if (st.empty()) return;
+        size_t cur = st.size();
+        std::map<System, std::pair<size_t, std::vector<size_t>>, StateLessExceptKey> buckets(StateLessExceptKey{id});
+        for (size_t i = 0; i < cur; ++i) {
+            if (!cond.satisfied(st[i])) continue;
+            const System& s = st[i];
+            uint64_t v = s.as_uint(id);
+            auto res = buckets.insert({s, {0, std::vector<size_t>(full_size, size_t(-1))}});
+            auto it = res.first;
+            it->second.second[v] = i;
+            it->second.first += 1;
+            if (it->second.first == full_size) { operate_bucket_inplace(it->second.second, st); buckets.erase(it); }
+        }
+        for (auto& kv : buckets) {
+            auto& pos = kv.second.second;
+            if (kv.second.first <= few_threshold) { operate_bucket_sparse(pos, st); continue; }
+            for (size_t i = 0; i < pos.size(); ++i)
+                if (pos[i] == size_t(-1)) {
+                    st.push_back(kv.first);
+                    st.back().get(id) = i;
+                    st.back().amplitude = 0;
+                    pos[i] = st.size() - 1;
+                }
+            operate_bucket_inplace(pos, st);
+        }
+        ClearZero(st);
+        Registry::update_max_size(st.size());
+    }
+    void dag(State& st) const { (*this)(st); }
+};
+
+// Generic "pair" driver shared by Hadamard_Bool (hadamard.cpp:340-439),
+// CondRot_General_Bool_Fast (condrot.h:149-332) and CondRot_Rational_Bool
+// (condrot.cpp:12-151): bucket entries equal on all registers except `out`,
+// pair a 0/1 partner, process singletons after the scan.
+template <class PairFn, class AloneFn>
+inline void pair_scan(State& st, size_t out, const Conditions* cond, PairFn pairfn, AloneFn alonefn) {
+    std::map<System, size_t, StateLessExceptKey> buckets(StateLessExceptKey{out});
+    size_t cur = st.size();
+    for (size_t i = 0; i < cur; ++i) {
+        if (cond && !cond->satisfied(st[i])) continue;
+        auto it = buckets.find(st[i]);
+        if (it == buckets.end()) { buckets.insert({st[i], i}); continue; }
+        if (st[it->second].get(out) & 1) pairfn(i, it->second);   // stored 1, now 0
+        else pairfn(it->second, i);
+        buckets.erase(it);
+    }
+    std::vector<size_t> alone;
+    for (auto& kv : buckets) alone.push_back(kv.second);
+    for (size_t idx : alone) alonefn(idx, (st[idx].get(out) & 1) != 0);
+}
+
+struct Hadamard_Bool {
+    size_t out_id; Conditions cond;
+    explicit Hadamard_Bool(size_t id) : out_id(id) {
+        if (Registry::size_of(out_id) != 1) throw_invalid_input("Hadamard_Bool: size of output register must be 1");
+    }
+    void operator()(State& st) const {
+        if (st.empty()) return;
+        pair_scan(st, out_id, &cond,
+            [&](size_t z, size_t o) {          // operate_pair (hadamard.cpp:340-346)
+                complex_t a = st[z].amplitude, b = st[o].amplitude;
+                st[z].amplitude = a * sqrt2inv + b * sqrt2inv;
+                st[o].amplitude = a * sqrt2inv - b * sqrt2inv;
+            },
+            [&](size_t i, bool one) {
+                if (!one) {                    // operate_alone_zero (:348-355)
+                    st.push_back(st[i]);
+                    st.back().get(out_id) = 1;
+                    st[i].amplitude *= sqrt2inv;
+                    st.back().amplitude *= sqrt2inv;
+                } else {                       // operate_alone_one (:357-364)
+                    st.push_back(st[i]);
+                    st.back().get(out_id) = 0;
+                    st.back().amplitude *= sqrt2inv;
+                    st[i].amplitude *= (-sqrt2inv);
+                }
+            });
+        ClearZero(st);
+        Registry::update_max_size(st.size());
+    }
+    void dag(State& st) const { (*this)(st); }
+};
+
+// CondRot_Rational_Bool (condrot.cpp:12-151).  Not controllable.
+struct CondRot_Rational_Bool {
+    size_t in_id, out_id;
+    CondRot_Rational_Bool(size_t in, size_t out) : in_id(in), out_id(out) {}
+    void apply(State& st, double sign) const {
+        if (st.empty()) return;
+        auto mapping = [&](size_t z, size_t o, double c, double s) {
+            complex_t a0 = st[z].amplitude, a1 = st[o].amplitude;
+            st[z].amplitude = a0 * c - a1 * s;
+            st[o].amplitude = a0 * s + a1 * c;
+        };
+        pair_scan(st, out_id, nullptr,
+            [&](size_t z, size_t o) {
+                double rot = st[z].as_double(in_id) * (sign * 2 * pi);
+                mapping(z, o, std::cos(rot), std::sin(rot));
+            },
+            [&](size_t i, bool one) {
+                st.push_back(st[i]);
+                st.back().get(out_id) ^= 1;
+                st.back().amplitude = 0;
+                size_t z = one ? st.size() - 1 : i, o = one ? i : st.size() - 1;
+                double rot = st[z].as_double(in_id) * (sign * 2 * pi);
+                mapping(z, o, std::cos(rot), std::sin(rot));
+            });
+        ClearZero(st);
+    }
+    void operator()(State& st) const { apply(st, 1.0); }
+    void dag(State& st) const { apply(st, -1.0); }
+};
+
+// CondRot_General_Bool_Fast (condrot.h:149-332) with the 2x2 given per input value.
+struct CondRot_General_Bool {
+    size_t in_id, out_id; std::function<u22_t(uint64_t)> func;
+    CondRot_General_Bool(size_t in, size_t out, std::function<u22_t(uint64_t)> f) : in_id(in), out_id(out), func(f) {}
+    void operator()(State& st) const {
+        if (st.empty()) return;
+        pair_scan(st, out_id, nullptr,
+            [&](size_t z, size_t o) {
+                u22_t m = func(st[z].as_uint(in_id));
+                complex_t a = st[z].amplitude, b = st[o].amplitude;
+                st[z].amplitude = a * m[0] + b * m[1];
+                st[o].amplitude = a * m[2] + b * m[3];
+            },
+            [&](size_t i, bool one) {
+                u22_t m = func(st[i].as_uint(in_id));
+                st.push_back(st[i]);
+                st.back().get(out_id) = one ? 0 : 1;
+                if (!one) { st[i].amplitude *= m[0]; st.back().amplitude *= m[2]; }
+                else { st.back().amplitude *= m[1]; st[i].amplitude *= m[3]; }
+            });
+        ClearZero(st);
+    }
+};
+// make_func / make_func_inv (condrot.h:19-41)
+inline u22_t make_func(uint64_t value, size_t n) {
+    double th = (n == 64) ? value * 1.0 / 2 / double(pow2(63)) : value * 1.0 / double(pow2(n));
+    th *= (2 * pi);
+    return u22_t{complex_t(std::cos(th)), complex_t(-std::sin(th)), complex_t(std::sin(th)), complex_t(std::cos(th))};
+}
+inline u22_t make_func_inv(uint64_t value, size_t n) {
+    double th = (n == 64) ? value * 1.0 / 2 / double(pow2(63)) : value * 1.0 / double(pow2(n));
+    th *= (2 * pi);
+    return u22_t{complex_t(std::cos(th)), complex_t(std::sin(th)), complex_t(-std::sin(th)), complex_t(std::cos(th))};
+}
+
+// Rot_Bool (basic_gates.cpp:74-358), sort-based v1 path.
+struct Rot_Bool {
+    size_t id, digit; uint64_t mask; u22_t mat; Conditions cond;
+    Rot_Bool(size_t id_, size_t digit_, u22_t m) : id(id_), digit(digit_), mask(pow2(digit_)), mat(m) {
+        if (digit >= Registry::size_of(id)) throw_invalid_input();
+    }
+    static bool is_diagonal(const u22_t& d) { return abs_sqr(d[1]) < epsilon && abs_sqr(d[2]) < epsilon; }
+    static bool is_off_diagonal(const u22_t& d) { return abs_sqr(d[0]) < epsilon && abs_sqr(d[3]) < epsilon; }
+    void operate(size_t l, size_t r, State& st) const {
+        size_t n = r - l;
+        if (n == 0) return;
+        if (is_diagonal(mat)) {                         // :104-122
+            for (size_t i = l; i < r; ++i) {
+                System& s = st[i];
+                if (!cond.satisfied(s)) return;
+                if (s.get(id) & mask) s.amplitude *= mat[3]; else s.amplitude *= mat[0];
+            }
+        } else if (is_off_diagonal(mat)) {              // :134-157
+            for (size_t i = l; i < r; ++i) {
+                System& s = st[i];
+                if (!cond.satisfied(s)) return;
+                if (s.get(id) & mask) { s.amplitude *= mat[1]; } else { s.amplitude *= mat[2]; }
+                s.get(id) ^= mask;
+            }
+        } else if (n == 1) {                            // :159-183
+            if (!cond.satisfied(st[l])) return;
+            size_t np = st.size();
+            st.push_back(st[l]);
+            bool v = st[l].get(id) & mask;
+            st[np].get(id) ^= mask;
+            if (!v) { st[l].amplitude *= mat[0]; st[np].amplitude *= mat[2]; }
+            else { st[np].amplitude *= mat[1]; st[l].amplitude *= mat[3]; }
+        } else {                                        // :184-199
+            if (!cond.satisfied(st[l])) return;
+            bool bit = st[l].as_uint(id) & pow2(digit);
+            complex_t a = st[l].amplitude, b = st[l + 1].amplitude;
+            if (!bit) { st[l].amplitude = a * mat[0] + b * mat[1]; st[l + 1].amplitude = a * mat[2] + b * mat[3]; }
+            else { st[l + 1].amplitude = b * mat[0] + a * mat[1]; st[l].amplitude = b * mat[2] + a * mat[3]; }
+        }
+    }
+    void operator()(State& st) const {                 // :228-255
+        if (st.empty()) return;
+        SortExceptBit(st, id, digit);
+        size_t cur = st.size(), l = 0, r = 1;
+        while (true) {
+            if (r == cur) { operate(l, r, st); break; }
+            if (!compare_equal_rot(st[l], st[r], id, ~mask)) { operate(l, r, st); l = r; r = l + 1; }
+            else ++r;
+        }
+        ClearZero(st);
+        Registry::update_max_size(st.size());
+    }
+    void dag(State& st) const {                        // :310-315
+        u22_t d{std::conj(mat[0]), std::conj(mat[2]), std::conj(mat[1]), std::conj(mat[3])};
+        Rot_Bool inv(id, digit, d); inv.cond = cond; inv(st);
+    }
+};
+
+// ---------------------------------------------------------- per-entry ops
+// Helper: run f on every condition-satisfied entry.
+template <class F> inline void for_each_satisfied(State& st, const Conditions& c, F f) {
+#ifdef ORACLE_OMP
+#pragma omp parallel for
+    for (int64_t i = 0; i < (int64_t)st.size(); ++i) { if (c.satisfied(st[i])) f(st[i]); }
+#else
+    for (auto& s : st) if (c.satisfied(s)) f(s);
+#endif
+}
+
+}  // namespace oracle

@@ -1,0 +1,477 @@
+// extern "C" entry points of the engine (declared in include/sparq_hip.h).
+// Every call converts exceptions into status codes and records a message.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace sparq;
+
+struct sparq_state_s {
+    State st;
+};
+struct sparq_table_s {
+    void* dev = nullptr;
+    uint64_t nwords = 0;
+    uint32_t word_bytes = 0;
+};
+
+static thread_local std::string g_last_error;
+static int g_device = -1;
+
+template <class F>
+static int guarded(F&& f) {
+    try {
+        f();
+        return SPARQ_OK;
+    } catch (const Error& e) {
+        g_last_error = e.msg;
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return SPARQ_OOM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SPARQ_RUNTIME;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return SPARQ_RUNTIME;
+    }
+}
+
+static void ensure_device() {
+    if (g_device >= 0) return;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        throw Error(SPARQ_NO_DEVICE, "no HIP device available");
+    }
+    SPQ_HIP(hipSetDevice(0));
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, 0) == hipSuccess) {
+        uint64_t threshold = ~0ull;   // keep freed blocks in the pool
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+    }
+    g_device = 0;
+}
+
+static State& S(sparq_state_t h) {
+    if (!h) throw Error(SPARQ_INVALID, "null state handle");
+    return h->st;
+}
+
+// Copy entries [b, b+cnt) of src into dst at position at (same W).
+static void copy_entries(State& dst, uint64_t at, const State& src, uint64_t b, uint64_t cnt) {
+    if (!cnt) return;
+    for (uint32_t w = 0; w < src.W; ++w)
+        SPQ_HIP(hipMemcpyAsync(dst.keys + (uint64_t)w * dst.cap + at, src.keys + (uint64_t)w * src.cap + b, cnt * 8,
+                               hipMemcpyDeviceToDevice, dst.stream));
+    SPQ_HIP(hipMemcpyAsync(dst.amps + at, src.amps + b, cnt * 16, hipMemcpyDeviceToDevice, dst.stream));
+}
+
+static State sub_state(State& parent, uint64_t b, uint64_t cnt) {
+    State s;
+    s.W = parent.W;
+    s.stream = parent.stream;
+    s.pinned = parent.pinned;
+    s.device = parent.device;
+    alloc_arrays(s, s.W, std::max<uint64_t>(cnt, 1), &s.keys, &s.amps);
+    s.cap = std::max<uint64_t>(cnt, 1);
+    s.n = cnt;
+    copy_entries(s, 0, parent, b, cnt);
+    return s;
+}
+
+static void release(State& s) {
+    dfree(s.keys, s.stream);
+    dfree(s.amps, s.stream);
+    s.keys = nullptr;
+    s.amps = nullptr;
+}
+
+// Run `op` on the entries satisfying `conds`; the rest are kept (and ClearZero'd,
+// as the reference's ClearZero covers the whole state).
+template <class F>
+static void with_conditions(State& st, const sparq_cond_t* conds, uint32_t ncond, double eps, F op) {
+    if (ncond == 0) {
+        op(st);
+        return;
+    }
+    uint64_t S_ = partition_by_conditions(st, conds, ncond);
+    if (S_ == st.n) {
+        op(st);
+        return;
+    }
+    State act = sub_state(st, 0, S_);
+    State rest = sub_state(st, S_, st.n - S_);
+    if (act.n) op(act);
+    clear_zero(rest, eps);
+    uint64_t total = act.n + rest.n;
+    uint64_t* k;
+    double2* a;
+    alloc_arrays(st, st.W, std::max<uint64_t>(total, 1), &k, &a);
+    State out = st;
+    out.keys = k;
+    out.amps = a;
+    out.cap = std::max<uint64_t>(total, 1);
+    copy_entries(out, 0, act, 0, act.n);
+    copy_entries(out, act.n, rest, 0, rest.n);
+    install_arrays(st, st.W, k, a, total, out.cap);
+    release(act);
+    release(rest);
+}
+
+extern "C" {
+
+const char* sparq_last_error(void) { return g_last_error.c_str(); }
+
+int sparq_device_init(int device) {
+    return guarded([&] {
+        SPQ_REQUIRE(device == 0, "only device 0 is supported");
+        ensure_device();
+    });
+}
+
+int sparq_device_name(char* buf, size_t len) {
+    return guarded([&] {
+        ensure_device();
+        hipDeviceProp_t p;
+        SPQ_HIP(hipGetDeviceProperties(&p, 0));
+        std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    });
+}
+
+int sparq_state_create(uint32_t words, uint64_t capacity, sparq_state_t* out) {
+    return guarded([&] {
+        SPQ_REQUIRE(out != nullptr, "null output");
+        SPQ_REQUIRE(words >= 1 && words <= SPARQ_MAX_WORDS, "key width out of range");
+        ensure_device();
+        auto* h = new sparq_state_s();
+        State& st = h->st;
+        st.W = words;
+        try {
+            SPQ_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+            SPQ_HIP(hipHostMalloc((void**)&st.pinned, 64 * sizeof(uint64_t), hipHostMallocDefault));
+            for (auto& e : st.ev) SPQ_HIP(hipEventCreate(&e));
+            st.cap = std::max<uint64_t>(capacity, 1);
+            alloc_arrays(st, words, st.cap, &st.keys, &st.amps);
+            st.n = 0;
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int sparq_state_destroy(sparq_state_t h) {
+    return guarded([&] {
+        if (!h) return;
+        State& st = h->st;
+        release(st);
+        if (st.stream) (void)hipStreamSynchronize(st.stream);
+        for (auto& e : st.ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st.pinned) (void)hipHostFree(st.pinned);
+        if (st.stream) (void)hipStreamDestroy(st.stream);
+        delete h;
+    });
+}
+
+int sparq_state_clone(sparq_state_t src, sparq_state_t* out) {
+    return guarded([&] {
+        State& s = S(src);
+        sparq_state_t h = nullptr;
+        int rc = sparq_state_create(s.W, std::max<uint64_t>(s.n, 1), &h);
+        if (rc != SPARQ_OK) throw Error(rc, g_last_error);
+        State& d = h->st;
+        // order the copy after pending work on the source stream
+        SPQ_HIP(hipEventRecord(s.ev[7], s.stream));
+        SPQ_HIP(hipStreamWaitEvent(d.stream, s.ev[7], 0));
+        copy_entries(d, 0, s, 0, s.n);
+        d.n = s.n;
+        *out = h;
+    });
+}
+
+int sparq_state_words(sparq_state_t h, uint32_t* words) {
+    return guarded([&] { *words = S(h).W; });
+}
+
+int sparq_state_size(sparq_state_t h, uint64_t* n) {
+    return guarded([&] {
+        State& st = S(h);
+        SPQ_HIP(hipStreamSynchronize(st.stream));
+        *n = st.n;
+    });
+}
+
+int sparq_state_set_basis(sparq_state_t h, const uint64_t* key) {
+    return guarded([&] {
+        State& st = S(h);
+        std::vector<uint64_t> k(st.W, 0);
+        if (key) std::memcpy(k.data(), key, st.W * 8);
+        double amp[2] = {1.0, 0.0};
+        uint64_t* kd;
+        double2* ad;
+        alloc_arrays(st, st.W, 1, &kd, &ad);
+        SPQ_HIP(hipMemcpyAsync(kd, k.data(), st.W * 8, hipMemcpyHostToDevice, st.stream));
+        SPQ_HIP(hipMemcpyAsync(ad, amp, 16, hipMemcpyHostToDevice, st.stream));
+        SPQ_HIP(hipStreamSynchronize(st.stream));
+        install_arrays(st, st.W, kd, ad, 1, 1);
+    });
+}
+
+int sparq_state_upload(sparq_state_t h, const uint64_t* keys, const double* amps, uint64_t n) {
+    return guarded([&] {
+        State& st = S(h);
+        SPQ_REQUIRE(n == 0 || (keys && amps), "null host buffer");
+        uint64_t cap = std::max<uint64_t>(n, 1);
+        std::vector<uint64_t> planes((size_t)st.W * n);
+        for (uint64_t i = 0; i < n; ++i)
+            for (uint32_t w = 0; w < st.W; ++w) planes[(size_t)w * n + i] = keys[i * st.W + w];
+        uint64_t* kd;
+        double2* ad;
+        alloc_arrays(st, st.W, cap, &kd, &ad);
+        for (uint32_t w = 0; w < st.W && n; ++w)
+            SPQ_HIP(hipMemcpyAsync(kd + (uint64_t)w * cap, planes.data() + (size_t)w * n, n * 8, hipMemcpyHostToDevice, st.stream));
+        if (n) SPQ_HIP(hipMemcpyAsync(ad, amps, n * 16, hipMemcpyHostToDevice, st.stream));
+        SPQ_HIP(hipStreamSynchronize(st.stream));
+        install_arrays(st, st.W, kd, ad, n, cap);
+    });
+}
+
+int sparq_state_download(sparq_state_t h, uint64_t* keys, double* amps, uint64_t capacity, uint64_t* n) {
+    return guarded([&] {
+        State& st = S(h);
+        SPQ_HIP(hipStreamSynchronize(st.stream));
+        if (n) *n = st.n;
+        if (!keys && !amps) return;
+        SPQ_REQUIRE(capacity >= st.n, "download buffer too small");
+        if (st.n == 0) return;
+        std::vector<uint64_t> planes((size_t)st.W * st.n);
+        for (uint32_t w = 0; w < st.W; ++w)
+            SPQ_HIP(hipMemcpyAsync(planes.data() + (size_t)w * st.n, st.keys + (uint64_t)w * st.cap, st.n * 8, hipMemcpyDeviceToHost,
+                                   st.stream));
+        if (amps) SPQ_HIP(hipMemcpyAsync(amps, st.amps, st.n * 16, hipMemcpyDeviceToHost, st.stream));
+        SPQ_HIP(hipStreamSynchronize(st.stream));
+        if (keys)
+            for (uint64_t i = 0; i < st.n; ++i)
+                for (uint32_t w = 0; w < st.W; ++w) keys[i * st.W + w] = planes[(size_t)w * st.n + i];
+    });
+}
+
+int sparq_state_relayout(sparq_state_t h, uint32_t new_words, const sparq_field_t* from, const sparq_field_t* to, uint32_t nfields) {
+    return guarded([&] {
+        SPQ_REQUIRE(nfields == 0 || (from && to), "null field list");
+        relayout(S(h), new_words, from, to, nfields);
+    });
+}
+
+int sparq_state_sync(sparq_state_t h) {
+    return guarded([&] { SPQ_HIP(hipStreamSynchronize(S(h).stream)); });
+}
+
+int sparq_state_device_arrays(sparq_state_t h, uint64_t** keys, double** amps, uint64_t* capacity) {
+    return guarded([&] {
+        State& st = S(h);
+        if (keys) *keys = st.keys;
+        if (amps) *amps = reinterpret_cast<double*>(st.amps);
+        if (capacity) *capacity = st.cap;
+    });
+}
+
+int sparq_state_stream(sparq_state_t h, void** stream) {
+    return guarded([&] { *stream = (void*)S(h).stream; });
+}
+
+int sparq_table_create(const void* host_words, uint64_t nwords, uint32_t word_bytes, sparq_table_t* out) {
+    return guarded([&] {
+        SPQ_REQUIRE(out && (host_words || nwords == 0), "null argument");
+        SPQ_REQUIRE(word_bytes == 1 || word_bytes == 2 || word_bytes == 4 || word_bytes == 8, "word_bytes must be 1, 2, 4 or 8");
+        ensure_device();
+        auto* t = new sparq_table_s();
+        t->nwords = nwords;
+        t->word_bytes = word_bytes;
+        hipError_t e = hipMalloc(&t->dev, std::max<uint64_t>(nwords * word_bytes, 8));
+        if (e != hipSuccess) {
+            delete t;
+            throw Error(SPARQ_OOM, "table allocation failed");
+        }
+        if (nwords) {
+            e = hipMemcpy(t->dev, host_words, nwords * word_bytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                (void)hipFree(t->dev);
+                delete t;
+                throw Error(SPARQ_RUNTIME, "table upload failed");
+            }
+        }
+        *out = t;
+    });
+}
+
+int sparq_table_destroy(sparq_table_t t) {
+    return guarded([&] {
+        if (!t) return;
+        (void)hipDeviceSynchronize();
+        (void)hipFree(t->dev);
+        delete t;
+    });
+}
+
+int sparq_table_device_ptr(sparq_table_t t, const void** ptr) {
+    return guarded([&] {
+        SPQ_REQUIRE(t && ptr, "null argument");
+        *ptr = t->dev;
+    });
+}
+
+int sparq_pointwise(sparq_state_t h, const sparq_pw_op_t* ops, uint32_t nops, const sparq_cond_t* conds, uint32_t nconds) {
+    return guarded([&] {
+        SPQ_REQUIRE(nops == 0 || ops, "null op list");
+        SPQ_REQUIRE(nconds == 0 || conds, "null condition list");
+        pointwise(S(h), ops, nops, conds, nconds);
+    });
+}
+
+int sparq_clear_zero(sparq_state_t h, double eps) {
+    return guarded([&] { clear_zero(S(h), eps); });
+}
+
+int sparq_sort(sparq_state_t h, const sparq_field_t* last) {
+    return guarded([&] {
+        State& st = S(h);
+        if (last) SPQ_REQUIRE(last->word < st.W && last->width >= 1 && (uint32_t)last->shift + last->width <= 64, "bad field");
+        if (st.n > 1) sort_by_plan(st, make_plan(st, last, 8));
+    });
+}
+
+int sparq_merge_duplicates(sparq_state_t h, double eps) {
+    return guarded([&] {
+        State& st = S(h);
+        if (st.n > 1) sort_by_plan(st, make_plan(st, nullptr, 8));
+        merge_sorted_duplicates(st, eps);
+    });
+}
+
+int sparq_add_systems(sparq_state_t hc, sparq_state_t ha, double coef, double eps) {
+    return guarded([&] {
+        State& cur = S(hc);
+        State& add = S(ha);
+        SPQ_REQUIRE(cur.W == add.W, "states have different key widths");
+        if (add.n == 0) return;
+        SPQ_HIP(hipEventRecord(add.ev[7], add.stream));
+        SPQ_HIP(hipStreamWaitEvent(cur.stream, add.ev[7], 0));
+        const uint64_t orig = cur.n, total = cur.n + add.n;
+        uint64_t* k;
+        double2* a;
+        alloc_arrays(cur, cur.W, total, &k, &a);
+        State out = cur;
+        out.keys = k;
+        out.amps = a;
+        out.cap = total;
+        copy_entries(out, 0, cur, 0, cur.n);
+        copy_entries(out, cur.n, add, 0, add.n);
+        install_arrays(cur, cur.W, k, a, total, total);
+        if (std::abs(coef - 1.0) > 1e-14) {
+            State tail = cur;   // view of the appended entries
+            tail.amps = cur.amps + orig;
+            tail.n = add.n;
+            scale_amps(tail, coef);
+        }
+        if (orig != 0) {
+            sort_by_plan(cur, make_plan(cur, nullptr, 8));
+            merge_sorted_duplicates(cur, eps);
+        }
+        // keep `add` alive until the copy has run
+        SPQ_HIP(hipEventRecord(cur.ev[6], cur.stream));
+        SPQ_HIP(hipStreamWaitEvent(add.stream, cur.ev[6], 0));
+    });
+}
+
+int sparq_check_duplicates(sparq_state_t h, int* has_dup) {
+    return guarded([&] {
+        State& st = S(h);
+        if (st.n > 1) sort_by_plan(st, make_plan(st, nullptr, 8));
+        *has_dup = has_sorted_duplicates(st) ? 1 : 0;
+    });
+}
+
+int sparq_hadamard_int(sparq_state_t h, sparq_field_t target, uint32_t n_digits, const sparq_cond_t* conds, uint32_t ncond,
+                       double eps) {
+    return guarded([&] {
+        State& st = S(h);
+        with_conditions(st, conds, ncond, eps, [&](State& s) { hadamard_int(s, target, n_digits, false, eps, s.n); });
+    });
+}
+
+int sparq_hadamard_int_full(sparq_state_t h, sparq_field_t target, const sparq_cond_t* conds, uint32_t ncond, double eps) {
+    return guarded([&] {
+        State& st = S(h);
+        // CheckDuplicateKey precedes the transform (hadamard.cpp:183)
+        if (st.n > 1) {
+            sort_by_plan(st, make_plan(st, nullptr, 8));
+            if (has_sorted_duplicates(st)) throw Error(SPARQ_RUNTIME, "Hadamard_Int_Full: duplicate key in state");
+        }
+        with_conditions(st, conds, ncond, eps, [&](State& s) { hadamard_int(s, target, target.width, true, eps, s.n); });
+    });
+}
+
+int sparq_pair_op(sparq_state_t h, sparq_field_t target, uint32_t bit, const sparq_pair_params_t* params, const sparq_cond_t* conds,
+                  uint32_t ncond, double eps) {
+    return guarded([&] {
+        SPQ_REQUIRE(params != nullptr, "null params");
+        State& st = S(h);
+        with_conditions(st, conds, ncond, eps, [&](State& s) {
+            pair_op(s, target, bit, *params, s.n);
+            clear_zero(s, eps);
+        });
+    });
+}
+
+int sparq_replicate(sparq_state_t h, sparq_field_t target) {
+    return guarded([&] { replicate(S(h), target); });
+}
+
+int sparq_normalize(sparq_state_t h, double* norm2) {
+    return guarded([&] {
+        State& st = S(h);
+        double s = sum_abs2(st);
+        if (norm2) *norm2 = s;
+        double f = 1.0 / std::sqrt(s);
+        if (std::isnan(f)) throw Error(SPARQ_RUNTIME, "Normalize: bad result (NaN norm)");
+        scale_amps(st, f);
+    });
+}
+
+int sparq_field_uniform(sparq_state_t h, sparq_field_t f, int* uniform, uint64_t* value) {
+    return guarded([&] {
+        State& st = S(h);
+        SPQ_REQUIRE(f.word < st.W, "bad field");
+        uint64_t v = 0;
+        bool u = field_uniform(st, f, &v);
+        if (uniform) *uniform = u ? 1 : 0;
+        if (value) *value = v;
+    });
+}
+
+int sparq_timer_record(sparq_state_t h, int slot) {
+    return guarded([&] {
+        SPQ_REQUIRE(slot >= 0 && slot < 6, "timer slot out of range");
+        State& st = S(h);
+        SPQ_HIP(hipEventRecord(st.ev[slot], st.stream));
+    });
+}
+
+int sparq_timer_elapsed(sparq_state_t h, int b, int e, float* ms) {
+    return guarded([&] {
+        SPQ_REQUIRE(b >= 0 && b < 6 && e >= 0 && e < 6 && ms, "timer slot out of range");
+        State& st = S(h);
+        SPQ_HIP(hipEventSynchronize(st.ev[e]));
+        SPQ_HIP(hipEventElapsedTime(ms, st.ev[b], st.ev[e]));
+    });
+}
+
+}  // extern "C"

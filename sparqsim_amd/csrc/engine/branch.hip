@@ -1,0 +1,398 @@
+// Branching operators: Hadamard_Int / Hadamard_Int_Full (fan-out 2^n per
+// group) and the two-branch pair operators (Hadamard_Bool, CondRot_*,
+// Rot_Bool general).  Groups come from one LSD radix sort that places the
+// target bits least significant; every group then owns a contiguous output
+// block, so the expansion is a pure streaming write and the Walsh-Hadamard
+// butterflies run tile-by-tile in LDS.
+//
+// Bytes per output entry of the expansion: 8W+16 written (+16 read/written per
+// WHT pass); pair ops: (8W+16) read + (8W+16) written per output entry.
+#include <algorithm>
+#include <cmath>
+
+#include "blockscan.hpp"
+#include "engine.hpp"
+#include "keyops.hpp"
+
+namespace sparq {
+
+enum GroupClass : uint8_t { kDirect = 0, kSparseSum = 1, kWht = 2 };
+
+// ---------------------------------------------------------------- groups
+template <int W>
+__global__ void __launch_bounds__(kBlock) head_flag_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, sparq_field_t low,
+                                                      uint32_t* __restrict__ head) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0) { head[0] = 1; return; }
+    const uint64_t lm = fmask(low.width) << low.shift;
+    bool diff = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint64_t m = (w == low.word) ? ~lm : ~0ull;
+        diff = diff || ((keys[(uint64_t)w * cap + i] & m) != (keys[(uint64_t)w * cap + i - 1] & m));
+    }
+    head[i] = diff ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) group_start_kernel(const uint32_t* __restrict__ head, const uint64_t* __restrict__ pos, uint64_t n,
+                                                        uint64_t* __restrict__ start, const uint64_t* __restrict__ G) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && head[i]) start[pos[i]] = i;
+    if (i == 0) start[G[0]] = n;
+}
+
+__global__ void __launch_bounds__(kBlock) classify_kernel(const uint64_t* __restrict__ start, uint64_t G, uint64_t F, uint32_t nd, int full,
+                                                     uint8_t* __restrict__ cls) {
+    uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= G) return;
+    uint64_t cnt = start[g + 1] - start[g];
+    uint8_t c;
+    if (!full) c = (cnt == 1) ? kDirect : kWht;                        // hadamard.cpp:26-131
+    else c = (cnt == F) ? kWht : (cnt + 1 <= nd ? kSparseSum : kWht);  // hadamard.cpp:220-256 (few_threshold = n-1)
+    cls[g] = c;
+}
+
+// Output entry o = g*F + j: key of the group head with the low target bits = j.
+// DIRECT and SPARSE_SUM amplitudes are produced here; WHT blocks start at 0.
+template <int W>
+__global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                                        const uint64_t* __restrict__ start, const uint8_t* __restrict__ cls, uint32_t nd,
+                                                        sparq_field_t low, double extra, uint64_t M, uint64_t* __restrict__ kout,
+                                                        double2* __restrict__ aout, uint64_t cap_out) {
+    uint64_t o = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (o >= M) return;
+    const uint64_t g = o >> nd, j = o & ((uint64_t(1) << nd) - 1);
+    const uint64_t h = start[g];
+    uint64_t k[W];
+    load_key<W>(keys, cap, h, k);
+    const uint64_t vlow = fget<W>(k, low);
+    fput<W>(k, low, j);
+    store_key<W>(kout, cap_out, o, k);
+    const uint8_t c = cls[g];
+    double2 r = make_double2(0.0, 0.0);
+    if (c == kDirect) {
+        // amp * 2^{-n/2}, then * -1 on odd parity (hadamard.cpp:30-39)
+        double2 t = cscale(amps[h], extra);
+        r = (__popcll(j & vlow) & 1) ? make_double2(t.x * -1.0, t.y * -1.0) : t;
+    } else if (c == kSparseSum) {
+        // sum over members in ascending value of (amp * (+-1)) * 2^{-n/2} (hadamard.cpp:269-304)
+        const uint64_t e = start[g + 1];
+        for (uint64_t m = h; m < e; ++m) {
+            uint64_t km[W];
+            load_key<W>(keys, cap, m, km);
+            uint64_t v = fget<W>(km, low);
+            double sg = (__popcll(j & v) & 1) ? -1. : 1.;
+            r = cadd(r, cscale(cscale(amps[m], sg), extra));
+        }
+    }
+    aout[o] = r;
+}
+
+// Place the members of WHT groups at their slot of the zeroed block.
+template <int W>
+__global__ void __launch_bounds__(kBlock) expand_scatter_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                                           uint64_t n, const uint32_t* __restrict__ head, const uint64_t* __restrict__ pos,
+                                                           const uint8_t* __restrict__ cls, uint32_t nd, sparq_field_t low,
+                                                           double2* __restrict__ aout) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    // group id of entry i = inclusive count of heads up to i, minus one
+    uint64_t g = pos[i] + head[i] - 1;
+    if (cls[g] != kWht) return;
+    uint64_t k[W];
+    load_key<W>(keys, cap, i, k);
+    aout[(g << nd) + fget<W>(k, low)] = amps[i];
+}
+
+// WHT stages [lo, lo+cnt) over blocks of F=2^nd, on tiles of 2^cnt x 2^c2
+// elements (c2 contiguous low index bits, c2 <= lo).  Butterfly per stage:
+// (a0+a1)*s, (a0-a1)*s, stages in ascending order (hadamard.cpp:44-60, 118-130).
+constexpr int kWhtTileLog = 11;   // 2048 complex doubles = 32 KiB of LDS
+__global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ a, uint64_t M, uint32_t nd, uint32_t lo, uint32_t cnt,
+                                                     uint32_t c2, const uint8_t* __restrict__ cls) {
+    __shared__ double2 buf[1 << kWhtTileLog];
+    const uint32_t T = 1u << (cnt + c2);
+    // tile t -> (group g, high part, low-chunk)
+    const uint64_t tile = blockIdx.x;
+    const uint64_t per_group = (uint64_t(1) << nd) >> (cnt + c2);
+    const uint64_t g = tile / per_group;
+    const uint64_t r = tile % per_group;
+    if (cls[g] != kWht) return;
+    // r enumerates (high bits above lo+cnt) x (low chunk index below lo, in units of 2^c2)
+    const uint64_t nlowchunks = (uint64_t(1) << lo) >> c2;
+    const uint64_t lowchunk = r % nlowchunks, high = r / nlowchunks;
+    const uint64_t base = (g << nd) + (high << (lo + cnt)) + (lowchunk << c2);
+    // element e of tile: sub = e >> c2 (stage index bits), l = e & (2^c2-1)
+    for (uint32_t e = threadIdx.x; e < T; e += kBlock) {
+        uint64_t sub = e >> c2, l = e & ((1u << c2) - 1);
+        buf[e] = a[base + (sub << lo) + l];
+    }
+    __syncthreads();
+    for (uint32_t s = 0; s < cnt; ++s) {
+        for (uint32_t p = threadIdx.x; p < T / 2; p += kBlock) {
+            // insert a zero at bit (s + c2) of p
+            uint32_t bitpos = s + c2;
+            uint32_t lowp = p & ((1u << bitpos) - 1);
+            uint32_t i0 = ((p >> bitpos) << (bitpos + 1)) | lowp;
+            uint32_t i1 = i0 | (1u << bitpos);
+            double2 a0 = buf[i0], a1 = buf[i1];
+            buf[i0] = cscale(cadd(a0, a1), kSqrt2Inv);
+            buf[i1] = cscale(csub(a0, a1), kSqrt2Inv);
+        }
+        __syncthreads();
+    }
+    for (uint32_t e = threadIdx.x; e < T; e += kBlock) {
+        uint64_t sub = e >> c2, l = e & ((1u << c2) - 1);
+        a[base + (sub << lo) + l] = buf[e];
+    }
+}
+
+template <int W>
+static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps) {
+    const uint64_t n = st.n;
+    sparq_field_t low{target.word, target.shift, (uint8_t)nd, 0};
+    if (n > 1) sort_by_plan(st, make_plan(st, &low, 8));
+    uint32_t* head = dalloc<uint32_t>(n, st.stream);
+    uint64_t* pos = dalloc<uint64_t>(n + 1, st.stream);
+    {
+        SPQ_PROF(st.stream, "head_flag_kernel", n * (8.0 * W + 4));
+        head_flag_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, n, low, head);
+    }
+    SPQ_HIP(hipGetLastError());
+    exclusive_scan_u32(st, head, pos, n, pos + n);
+    const uint64_t G = read_u64(st, pos + n);
+    uint64_t* start = dalloc<uint64_t>(G + 1, st.stream);
+    {
+        SPQ_PROF(st.stream, "group_start_kernel", n * 12.0 + G * 8.0);
+        group_start_kernel<<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(head, pos, n, start, pos + n);
+    }
+    SPQ_HIP(hipGetLastError());
+    const uint64_t F = uint64_t(1) << nd;
+    uint8_t* cls = dalloc<uint8_t>(G, st.stream);
+    {
+        SPQ_PROF(st.stream, "classify_kernel", G * 17.0);
+        classify_kernel<<<grid_for(G, kBlock), kBlock, 0, st.stream>>>(start, G, F, nd, full ? 1 : 0, cls);
+    }
+    SPQ_HIP(hipGetLastError());
+    SPQ_REQUIRE(G <= (~0ull >> nd), "Hadamard output size overflows");
+    const uint64_t M = G << nd;
+    uint64_t* kout;
+    double2* aout;
+    alloc_arrays(st, st.W, M, &kout, &aout);
+    const double extra = 1.0 / std::sqrt((double)F);
+    {
+        SPQ_PROF(st.stream, "expand_init_kernel", M * (8.0 * W + 16) + G * (8.0 * W + 16));
+        expand_init_kernel<W><<<grid_for(M, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, start, cls, nd, low, extra, M, kout,
+                                                                         aout, M);
+    }
+    SPQ_HIP(hipGetLastError());
+    {
+        SPQ_PROF(st.stream, "expand_scatter_kernel", n * (8.0 * W + 44));
+        expand_scatter_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, head, pos, cls, nd, low, aout);
+    }
+    SPQ_HIP(hipGetLastError());
+    // Butterfly passes: first pass covers min(nd, 11) stages on contiguous tiles,
+    // later passes take up to 7 stages on 16-wide column tiles.
+    uint32_t lo = 0;
+    while (lo < nd) {
+        uint32_t c2 = (lo == 0) ? 0 : std::min<uint32_t>(4, lo);
+        uint32_t cnt = std::min<uint32_t>(nd - lo, kWhtTileLog - c2);
+        uint64_t tiles = M >> (cnt + c2);
+        {
+            SPQ_PROF(st.stream, "wht_pass_kernel", M * 32.0);
+            wht_pass_kernel<<<(unsigned)tiles, kBlock, 0, st.stream>>>(aout, M, nd, lo, cnt, c2, cls);
+        }
+        SPQ_HIP(hipGetLastError());
+        lo += cnt;
+    }
+    install_arrays(st, st.W, kout, aout, M, M);
+    dfree(head, st.stream);
+    dfree(pos, st.stream);
+    dfree(start, st.stream);
+    dfree(cls, st.stream);
+    clear_zero(st, eps);
+}
+
+void hadamard_int(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, uint64_t /*n_active*/) {
+    SPQ_REQUIRE(target.word < st.W && target.width >= 1 && (uint32_t)target.shift + target.width <= 64, "bad target field");
+    SPQ_REQUIRE(nd >= 1 && nd <= target.width, "n_digits must be in [1, width]");
+    SPQ_REQUIRE(nd <= 40, "n_digits too large for a sparse expansion");
+    if (st.n == 0) return;
+    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, nd, full, eps);
+}
+
+// ---------------------------------------------------------------- replicate
+template <int W>
+__global__ void __launch_bounds__(kBlock) replicate_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                                      uint32_t nd, sparq_field_t f, uint64_t M, uint64_t* __restrict__ kout,
+                                                      double2* __restrict__ aout) {
+    uint64_t o = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (o >= M) return;
+    const uint64_t g = o >> nd, j = o & ((uint64_t(1) << nd) - 1);
+    uint64_t k[W];
+    load_key<W>(keys, cap, g, k);
+    fput<W>(k, f, j);
+    store_key<W>(kout, M, o, k);
+    aout[o] = amps[g];
+}
+
+template <int W>
+static void replicate_impl(State& st, sparq_field_t f) {
+    const uint32_t nd = f.width;
+    SPQ_REQUIRE(st.n <= (~0ull >> nd), "replicated state too large");
+    const uint64_t M = st.n << nd;
+    uint64_t* kout;
+    double2* aout;
+    alloc_arrays(st, st.W, M, &kout, &aout);
+    {
+        SPQ_PROF(st.stream, "replicate_kernel", M * (8.0 * W + 16) + st.n * (8.0 * W + 16));
+        replicate_kernel<W><<<grid_for(M, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, nd, f, M, kout, aout);
+    }
+    SPQ_HIP(hipGetLastError());
+    install_arrays(st, st.W, kout, aout, M, M);
+}
+
+void replicate(State& st, sparq_field_t f) {
+    SPQ_REQUIRE(f.word < st.W && f.width >= 1 && f.width <= 40, "bad replicate field");
+    if (st.n == 0) return;
+    SPQ_DISPATCH_W(st.W, replicate_impl, st, f);
+}
+
+// ---------------------------------------------------------------- pair ops
+template <int W>
+__global__ void __launch_bounds__(kBlock) pair_count_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, uint32_t word,
+                                                       uint64_t bm, uint32_t* __restrict__ cnt) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k[W];
+    load_key<W>(keys, cap, i, k);
+    const bool one = kword<W>(k, word) & bm;
+    bool paired = false;
+    uint64_t j = one ? i - 1 : i + 1;
+    if ((one && i > 0) || (!one && i + 1 < n)) {
+        paired = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uint64_t expect = (w == (int)word) ? (k[w] ^ bm) : k[w];
+            paired = paired && (keys[(uint64_t)w * cap + j] == expect);
+        }
+    }
+    cnt[i] = paired ? 1 : 2;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) pair_apply_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                                       uint64_t n, uint32_t word, uint64_t bm, const uint32_t* __restrict__ cnt,
+                                                       const uint64_t* __restrict__ off, const sparq_pair_params_t p,
+                                                       uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k[W];
+    load_key<W>(keys, cap, i, k);
+    const bool one = kword<W>(k, word) & bm;
+    const bool paired = cnt[i] == 1;
+    if (paired && one) return;               // handled by the zero partner
+    const uint64_t o = off[i];
+    double2 z, on;                            // amplitudes of the |0> and |1> branches
+    const double2 zero = make_double2(0.0, 0.0);
+    if (paired) { z = amps[i]; on = amps[i + 1]; }
+    else if (one) { z = zero; on = amps[i]; }
+    else { z = amps[i]; on = zero; }
+    double2 z2, o2;
+    switch (p.kind) {
+        case SPARQ_PAIR_HADAMARD_BOOL:
+            if (paired) {
+                z2 = cadd(cscale(z, kSqrt2Inv), cscale(on, kSqrt2Inv));
+                o2 = csub(cscale(z, kSqrt2Inv), cscale(on, kSqrt2Inv));
+            } else if (!one) {
+                z2 = cscale(z, kSqrt2Inv);
+                o2 = cscale(z, kSqrt2Inv);
+            } else {
+                z2 = cscale(on, kSqrt2Inv);
+                o2 = cscale(on, -kSqrt2Inv);
+            }
+            break;
+        case SPARQ_PAIR_CONDROT_RATIONAL: {
+            uint64_t v = fget<W>(k, p.in);
+            double frac = (p.in.width == 64) ? v * 1.0 / 2 / 9223372036854775808.0 : 1.0 * v / (double)(uint64_t(1) << p.in.width);
+            double rot = frac * (p.mat[0] * 2 * 3.141592653589793238462643383279502884);
+            double c = cos(rot), s = sin(rot);
+            z2 = csub(cscale(z, c), cscale(on, s));
+            o2 = cadd(cscale(z, s), cscale(on, c));
+            break;
+        }
+        default: {
+            double2 m0, m1, m2, m3;
+            if (p.kind == SPARQ_PAIR_CONDROT_TABLE) {
+                const double* t = p.table + 8 * fget<W>(k, p.in);
+                m0 = make_double2(t[0], t[1]); m1 = make_double2(t[2], t[3]);
+                m2 = make_double2(t[4], t[5]); m3 = make_double2(t[6], t[7]);
+            } else {
+                m0 = make_double2(p.mat[0], p.mat[1]); m1 = make_double2(p.mat[2], p.mat[3]);
+                m2 = make_double2(p.mat[4], p.mat[5]); m3 = make_double2(p.mat[6], p.mat[7]);
+            }
+            if (paired) {
+                z2 = cadd(cmul(z, m0), cmul(on, m1));
+                o2 = cadd(cmul(z, m2), cmul(on, m3));
+            } else if (!one) {
+                z2 = cmul(z, m0);
+                o2 = cmul(z, m2);
+            } else {
+                z2 = cmul(on, m1);
+                o2 = cmul(on, m3);
+            }
+            break;
+        }
+    }
+    uint64_t kz[W], ko[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        kz[w] = (w == (int)word) ? (k[w] & ~bm) : k[w];
+        ko[w] = (w == (int)word) ? (k[w] | bm) : k[w];
+    }
+    store_key<W>(kout, cap_out, o, kz);
+    store_key<W>(kout, cap_out, o + 1, ko);
+    aout[o] = z2;
+    aout[o + 1] = o2;
+}
+
+template <int W>
+static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p) {
+    const uint64_t n = st.n;
+    sparq_field_t b{target.word, (uint8_t)(target.shift + bit), 1, 0};
+    if (n > 1) sort_by_plan(st, make_plan(st, &b, 8));
+    const uint64_t bm = uint64_t(1) << b.shift;
+    uint32_t* cnt = dalloc<uint32_t>(n, st.stream);
+    uint64_t* off = dalloc<uint64_t>(n + 1, st.stream);
+    {
+        SPQ_PROF(st.stream, "pair_count_kernel", n * (8.0 * W + 4));
+        pair_count_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, n, b.word, bm, cnt);
+    }
+    SPQ_HIP(hipGetLastError());
+    exclusive_scan_u32(st, cnt, off, n, off + n);
+    const uint64_t M = read_u64(st, off + n);
+    uint64_t* kout;
+    double2* aout;
+    alloc_arrays(st, st.W, M, &kout, &aout);
+    {
+        SPQ_PROF(st.stream, "pair_apply_kernel", n * (8.0 * W + 28) + M * (8.0 * W + 16));
+        pair_apply_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, b.word, bm, cnt, off, p, kout, aout, M);
+    }
+    SPQ_HIP(hipGetLastError());
+    install_arrays(st, st.W, kout, aout, M, M);
+    dfree(cnt, st.stream);
+    dfree(off, st.stream);
+}
+
+void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, uint64_t /*n_active*/) {
+    SPQ_REQUIRE(target.word < st.W && bit < target.width, "bad target bit");
+    SPQ_REQUIRE(p.kind <= SPARQ_PAIR_ROT_GENERAL, "unknown pair kind");
+    if (p.kind == SPARQ_PAIR_CONDROT_RATIONAL || p.kind == SPARQ_PAIR_CONDROT_TABLE)
+        SPQ_REQUIRE(p.in.word < st.W && p.in.width >= 1, "bad input register");
+    if (p.kind == SPARQ_PAIR_CONDROT_TABLE) SPQ_REQUIRE(p.table != nullptr && p.in.width <= 24, "table-driven rotation needs a table");
+    if (st.n == 0) return;
+    SPQ_DISPATCH_W(st.W, pair_impl, st, target, bit, p);
+}
+
+}  // namespace sparq

@@ -1,0 +1,219 @@
+// Stream compaction: ClearZero, condition partition, and the duplicate merge
+// of a sorted state.  Each is count -> scan of tile counts -> ordered scatter.
+// ClearZero bytes per entry: (8W+16) read + (8W+16) per kept entry written
+// (+16 for the amplitude re-read of the counting pass).
+#include <algorithm>
+
+#include "blockscan.hpp"
+#include "engine.hpp"
+#include "keyops.hpp"
+
+namespace sparq {
+
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;   // 2048 entries per workgroup, blocked per thread
+
+struct CondList {
+    uint32_t n;
+    sparq_cond_t c[SPARQ_MAX_PROGRAM_CONDS];
+};
+
+enum Mode { kClearZero = 0, kPartition = 1, kMerge = 2 };
+
+// Predicate of entry i for the given mode; for kMerge also returns the run sum.
+template <int W, int MODE>
+__device__ __forceinline__ bool keep_entry(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                           uint64_t n, uint64_t i, double eps, const CondList& cl, double2& sum) {
+    if (MODE == kClearZero) {
+        sum = amps[i];
+        return !(cabs2(sum) < eps);
+    } else if (MODE == kPartition) {
+        uint64_t k[W];
+        load_key<W>(keys, cap, i, k);
+        bool ok = true;
+        for (uint32_t c = 0; c < cl.n; ++c) ok = ok && cond_ok<W>(k, cl.c[c]);
+        return ok;
+    } else {
+        uint64_t k[W];
+        load_key<W>(keys, cap, i, k);
+        if (i > 0) {
+            bool same = true;
+#pragma unroll
+            for (int w = 0; w < W; ++w) same = same && (keys[(uint64_t)w * cap + i - 1] == k[w]);
+            if (same) return false;   // not the head of its run
+        }
+        double2 s = amps[i];
+        for (uint64_t j = i + 1; j < n; ++j) {
+            bool same = true;
+#pragma unroll
+            for (int w = 0; w < W; ++w) same = same && (keys[(uint64_t)w * cap + j] == k[w]);
+            if (!same) break;
+            s = cadd(s, amps[j]);
+        }
+        sum = s;
+        return !(cabs2(s) < eps);
+    }
+}
+
+template <int W, int MODE>
+__global__ void __launch_bounds__(kBlock) compact_count_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                                          uint64_t n, double eps, const CondList cl, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t lds[kBlock / 64];
+    uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+    uint32_t c = 0;
+    for (int j = 0; j < kItems; ++j) {
+        uint64_t i = base + j;
+        double2 s;
+        if (i < n && keep_entry<W, MODE>(keys, amps, cap, n, i, eps, cl, s)) ++c;
+    }
+    uint32_t tot;
+    (void)block_excl_scan<uint32_t>(c, lds, tot);
+    if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+// Kept entries go to kout/aout at offs[tile] + rank; for kPartition the others
+// go after all kept ones (at n_kept + tile_start - offs[tile] + rank).
+template <int W, int MODE>
+__global__ void __launch_bounds__(kBlock) compact_scatter_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps,
+                                                            uint64_t cap, uint64_t n, double eps, const CondList cl,
+                                                            const uint64_t* __restrict__ offs, const uint64_t* __restrict__ total,
+                                                            uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out) {
+    __shared__ uint32_t lds[kBlock / 64];
+    const uint64_t tile = (uint64_t)blockIdx.x * kTile;
+    uint64_t base = tile + (uint64_t)threadIdx.x * kItems;
+    uint32_t flags = 0, c = 0;
+    double2 sums[kItems];
+    for (int j = 0; j < kItems; ++j) {
+        uint64_t i = base + j;
+        sums[j] = make_double2(0.0, 0.0);
+        if (i < n && keep_entry<W, MODE>(keys, amps, cap, n, i, eps, cl, sums[j])) {
+            flags |= 1u << j;
+            ++c;
+        }
+    }
+    uint32_t tot;
+    uint32_t pre = block_excl_scan<uint32_t>(c, lds, tot);
+    uint64_t pos = offs[blockIdx.x] + pre;
+    uint64_t rej = 0;
+    if (MODE == kPartition) rej = total[0] + (base - tile) - pre + (tile - offs[blockIdx.x]);
+    for (int j = 0; j < kItems; ++j) {
+        uint64_t i = base + j;
+        if (i >= n) break;
+        bool kept = (flags >> j) & 1;
+        uint64_t dst;
+        if (kept) dst = pos++;
+        else if (MODE == kPartition) dst = rej++;
+        else continue;
+#pragma unroll
+        for (int w = 0; w < W; ++w) kout[(uint64_t)w * cap_out + dst] = keys[(uint64_t)w * cap + i];
+        aout[dst] = (MODE == kPartition) ? amps[i] : sums[j];
+    }
+}
+
+template <int W, int MODE>
+static uint64_t run_compact(State& st, double eps, const CondList& cl) {
+    const uint64_t n = st.n;
+    const unsigned nb = grid_for(n, kTile);
+    uint32_t* counts = dalloc<uint32_t>(nb, st.stream);
+    uint64_t* offs = dalloc<uint64_t>(nb + 1, st.stream);
+    {
+        SPQ_PROF(st.stream, "compact_count_kernel", n * (MODE == kClearZero ? 16.0 : MODE == kPartition ? 8.0 * W : 8.0 * W + 16));
+        compact_count_kernel<W, MODE><<<nb, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, eps, cl, counts);
+    }
+    SPQ_HIP(hipGetLastError());
+    exclusive_scan_u32(st, counts, offs, nb, offs + nb);
+    uint64_t kept = read_u64(st, offs + nb);
+    uint64_t out_n = (MODE == kPartition) ? n : kept;
+    if (MODE != kPartition && kept == n) {   // nothing removed
+        dfree(counts, st.stream);
+        dfree(offs, st.stream);
+        return kept;
+    }
+    uint64_t cap_out = std::max<uint64_t>(out_n, 1);
+    uint64_t* kout;
+    double2* aout;
+    alloc_arrays(st, st.W, cap_out, &kout, &aout);
+    {
+        SPQ_PROF(st.stream, "compact_scatter_kernel", n * (MODE == kClearZero ? 16.0 : MODE == kPartition ? 8.0 * W : 8.0 * W + 16) + (double)out_n * (8.0 * W + 16));
+        compact_scatter_kernel<W, MODE><<<nb, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, eps, cl, offs, offs + nb, kout,
+                                                                   aout, cap_out);
+    }
+    SPQ_HIP(hipGetLastError());
+    install_arrays(st, st.W, kout, aout, out_n, cap_out);
+    dfree(counts, st.stream);
+    dfree(offs, st.stream);
+    return kept;
+}
+
+template <int W>
+static void clear_zero_w(State& st, double eps, uint64_t* kept) {
+    *kept = run_compact<W, kClearZero>(st, eps, CondList{});
+}
+template <int W>
+static void partition_w(State& st, const CondList& cl, uint64_t* kept) {
+    *kept = run_compact<W, kPartition>(st, 0.0, cl);
+}
+template <int W>
+static void merge_w(State& st, double eps, uint64_t* kept) {
+    *kept = run_compact<W, kMerge>(st, eps, CondList{});
+}
+
+void clear_zero(State& st, double eps) {
+    if (st.n == 0) return;
+    uint64_t kept;
+    SPQ_DISPATCH_W(st.W, clear_zero_w, st, eps, &kept);
+}
+
+void merge_sorted_duplicates(State& st, double eps) {
+    if (st.n == 0) return;
+    uint64_t kept;
+    SPQ_DISPATCH_W(st.W, merge_w, st, eps, &kept);
+}
+
+uint64_t partition_by_conditions(State& st, const sparq_cond_t* conds, uint32_t ncond) {
+    if (ncond == 0) return st.n;
+    SPQ_REQUIRE(ncond <= SPARQ_MAX_PROGRAM_CONDS, "too many conditions");
+    CondList cl{};
+    cl.n = ncond;
+    for (uint32_t c = 0; c < ncond; ++c) {
+        SPQ_REQUIRE(conds[c].field.word < st.W, "condition field out of range");
+        cl.c[c] = conds[c];
+    }
+    if (st.n == 0) return 0;
+    uint64_t kept;
+    SPQ_DISPATCH_W(st.W, partition_w, st, cl, &kept);
+    return kept;
+}
+
+// ---- duplicate detection on a sorted state
+template <int W>
+__global__ void __launch_bounds__(kBlock) dup_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, unsigned int* flag) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i >= n) return;
+    bool same = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) same = same && (keys[(uint64_t)w * cap + i] == keys[(uint64_t)w * cap + i - 1]);
+    if (same) atomicOr(flag, 1u);
+}
+
+template <int W>
+static void launch_dup(State& st, unsigned int* flag) {
+    {
+        SPQ_PROF(st.stream, "dup_kernel", st.n * 8.0 * W);
+        dup_kernel<W><<<grid_for(st.n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, st.n, flag);
+    }
+    SPQ_HIP(hipGetLastError());
+}
+
+bool has_sorted_duplicates(State& st) {
+    if (st.n <= 1) return false;
+    unsigned int* flag = dalloc<unsigned int>(1, st.stream);
+    SPQ_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), st.stream));
+    SPQ_DISPATCH_W(st.W, launch_dup, st, flag);
+    unsigned int h = 0;
+    read_back(st, flag, sizeof(h), &h);
+    dfree(flag, st.stream);
+    return h != 0;
+}
+
+}  // namespace sparq

@@ -1,0 +1,149 @@
+// Internal declarations of the MI355X sparse-state engine (gfx950, wave64).
+//
+// Data layout in HBM (one state):
+//   keys : uint64 [W][cap]  word-major, word 0 most significant; entry i's
+//          word w at keys[w*cap + i], so every word plane is read coalesced.
+//   amps : double2 [cap]    FP64 complex amplitude (re, im).
+// Registers are bit fields of the key (sparq_field_t) assigned by the host
+// layer in register-id order, most significant first, so unsigned
+// lexicographic order of the words is the reference canonical order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/sparq_hip.h"
+
+namespace sparq {
+
+struct Error : std::exception {
+    int status;
+    std::string msg;
+    Error(int s, std::string m) : status(s), msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+#define SPQ_HIP(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::sparq::Error(e_ == hipErrorOutOfMemory ? SPARQ_OOM : SPARQ_RUNTIME,       \
+                                 std::string(#expr " failed: ") + hipGetErrorString(e_));     \
+    } while (0)
+
+#define SPQ_REQUIRE(cond, msg)                                  \
+    do {                                                        \
+        if (!(cond)) throw ::sparq::Error(SPARQ_INVALID, msg);  \
+    } while (0)
+
+constexpr int kBlock = 256;           // threads per workgroup (4 waves of 64)
+constexpr int kWave = 64;
+
+struct State {
+    uint32_t W = 1;
+    uint64_t n = 0, cap = 0;
+    uint64_t* keys = nullptr;   // [W][cap]
+    double2* amps = nullptr;    // [cap]
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    uint64_t* pinned = nullptr; // 64 words of pinned host memory for small readbacks
+    int device = 0;
+};
+
+// ---- memory (stream-ordered pool allocations)
+void* dmalloc(size_t bytes, hipStream_t s);
+void dfree(void* p, hipStream_t s);
+template <class T> T* dalloc(size_t count, hipStream_t s) { return static_cast<T*>(dmalloc(count * sizeof(T), s)); }
+
+// Allocate fresh key/amp arrays of capacity `cap` for `st` (content undefined).
+void alloc_arrays(State& st, uint32_t W, uint64_t cap, uint64_t** keys, double2** amps);
+// Install new arrays as the state's content, freeing the old ones.
+void install_arrays(State& st, uint32_t W, uint64_t* keys, double2* amps, uint64_t n, uint64_t cap);
+// Read `count` uint64 values from device memory (synchronizes the stream).
+void read_back(State& st, const void* dptr, size_t bytes, void* host);
+uint64_t read_u64(State& st, const uint64_t* dptr);
+
+inline unsigned grid_for(uint64_t n, unsigned per_block) {
+    uint64_t g = (n + per_block - 1) / per_block;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+// ---- device-wide primitives (scan.hip)
+// Exclusive scan of `n` uint32 values into uint64 offsets; returns the total (syncs).
+void exclusive_scan_u32(State& st, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev);
+void exclusive_scan_u64(State& st, const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev);
+
+// ---- sort (sort.hip)
+// A digit of the LSD plan: bits [shift, shift+bits) of key word `word`.
+struct Digit { uint8_t word, shift, bits, pad; };
+// Stable LSD radix sort of the state by the given digit plan (least significant
+// digit first).  Reorders keys and amps.
+void sort_by_plan(State& st, const std::vector<Digit>& plan);
+// Plan for the key with optional field moved to the least significant place,
+// restricted to bits that vary across the state.
+std::vector<Digit> make_plan(State& st, const sparq_field_t* last, int digit_bits);
+// Bitwise OR / AND over all keys, per word (for the varying-bit mask).
+void key_or_and(State& st, std::vector<uint64_t>& or_mask, std::vector<uint64_t>& and_mask);
+
+// ---- compaction (compact.hip)
+void clear_zero(State& st, double eps);
+void merge_sorted_duplicates(State& st, double eps);
+bool has_sorted_duplicates(State& st);
+
+// ---- condition split (compact.hip): stable partition so that entries that
+// satisfy `conds` come first; returns their count.
+uint64_t partition_by_conditions(State& st, const sparq_cond_t* conds, uint32_t ncond);
+
+// ---- branching (branch.hip)
+void hadamard_int(State& st, sparq_field_t target, uint32_t n_digits, bool full_variant, double eps,
+                  uint64_t n_active);
+void replicate(State& st, sparq_field_t f);
+void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, uint64_t n_active);
+
+// ---- pointwise (pointwise.hip)
+void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_cond_t* conds, uint32_t nconds);
+void relayout(State& st, uint32_t new_words, const sparq_field_t* from, const sparq_field_t* to, uint32_t nf);
+double sum_abs2(State& st);
+void scale_amps(State& st, double s);
+bool field_uniform(State& st, sparq_field_t f, uint64_t* value);
+
+// ---- profiling (prof.hip)
+bool prof_enabled();
+void prof_begin(hipStream_t s, const char* name, double bytes);
+void prof_end(hipStream_t s);
+struct ProfScope {
+    bool on;
+    hipStream_t s;
+    ProfScope(hipStream_t s_, const char* name, double bytes) : on(prof_enabled()), s(s_) {
+        if (on) prof_begin(s, name, bytes);
+    }
+    ~ProfScope() {
+        if (on) prof_end(s);
+    }
+};
+
+}  // namespace sparq
+
+#define SPQ_CAT2(a, b) a##b
+#define SPQ_CAT(a, b) SPQ_CAT2(a, b)
+// Bracket the next kernel launch with profiling events (no-op unless enabled).
+#define SPQ_PROF(stream, name, bytes) ::sparq::ProfScope SPQ_CAT(spq_prof_, __LINE__)((stream), (name), (double)(bytes))
+
+// Dispatch a templated launcher on the runtime key width W (1..8).
+#define SPQ_DISPATCH_W(Wv, FN, ...)                                         \
+    switch (Wv) {                                                           \
+        case 1: FN<1>(__VA_ARGS__); break;                                  \
+        case 2: FN<2>(__VA_ARGS__); break;                                  \
+        case 3: FN<3>(__VA_ARGS__); break;                                  \
+        case 4: FN<4>(__VA_ARGS__); break;                                  \
+        case 5: FN<5>(__VA_ARGS__); break;                                  \
+        case 6: FN<6>(__VA_ARGS__); break;                                  \
+        case 7: FN<7>(__VA_ARGS__); break;                                  \
+        case 8: FN<8>(__VA_ARGS__); break;                                  \
+        default: throw ::sparq::Error(SPARQ_INVALID, "key width out of range"); \
+    }

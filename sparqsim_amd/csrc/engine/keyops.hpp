@@ -1,0 +1,84 @@
+// Device helpers: register-resident packed keys, field access, conditions and
+// FP64 complex arithmetic written out explicitly (the library is compiled with
+// -ffp-contract=off so every product and sum rounds exactly like the host
+// reference's std::complex<double> arithmetic).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../../include/sparq_hip.h"
+
+namespace sparq {
+
+__device__ __forceinline__ uint64_t fmask(uint32_t width) {
+    return width >= 64 ? ~uint64_t(0) : ((uint64_t(1) << width) - 1);
+}
+
+// Select word w of a register-resident key without dynamic register indexing.
+template <int W>
+__device__ __forceinline__ uint64_t kword(const uint64_t (&k)[W], int w) {
+    uint64_t r = k[0];
+#pragma unroll
+    for (int i = 1; i < W; ++i) r = (w == i) ? k[i] : r;
+    return r;
+}
+template <int W>
+__device__ __forceinline__ void kset_word(uint64_t (&k)[W], int w, uint64_t v) {
+#pragma unroll
+    for (int i = 0; i < W; ++i) k[i] = (w == i) ? v : k[i];
+}
+template <int W>
+__device__ __forceinline__ uint64_t fget(const uint64_t (&k)[W], sparq_field_t f) {
+    return (kword<W>(k, f.word) >> f.shift) & fmask(f.width);
+}
+template <int W>
+__device__ __forceinline__ void fput(uint64_t (&k)[W], sparq_field_t f, uint64_t v) {
+    uint64_t m = fmask(f.width) << f.shift;
+    uint64_t w = kword<W>(k, f.word);
+    w = (w & ~m) | ((v << f.shift) & m);
+    kset_word<W>(k, f.word, w);
+}
+
+template <int W>
+__device__ __forceinline__ void load_key(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t i, uint64_t (&k)[W]) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) k[w] = keys[(uint64_t)w * cap + i];
+}
+template <int W>
+__device__ __forceinline__ void store_key(uint64_t* __restrict__ keys, uint64_t cap, uint64_t i, const uint64_t (&k)[W]) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) keys[(uint64_t)w * cap + i] = k[w];
+}
+
+__device__ __forceinline__ int64_t sext(uint64_t v, uint32_t w) {
+    if (w == 0) return 0;
+    if (w >= 64) return (int64_t)v;
+    return (int64_t)(v << (64 - w)) >> (64 - w);
+}
+
+template <int W>
+__device__ __forceinline__ bool cond_ok(const uint64_t (&k)[W], const sparq_cond_t& c) {
+    uint64_t v = fget<W>(k, c.field);
+    switch (c.kind) {
+        case SPARQ_COND_NONZERO: return v != 0;
+        case SPARQ_COND_ALL_ONES: return v == fmask(c.field.width);
+        case SPARQ_COND_BIT: return (v >> c.value) & 1;
+        default: return v == c.value;
+    }
+}
+
+// ---- complex FP64 (no contraction: see file header)
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+
+// 1/sqrt(2), correctly rounded.
+constexpr double kSqrt2Inv = 0.7071067811865475244008443621048490;
+
+}  // namespace sparq

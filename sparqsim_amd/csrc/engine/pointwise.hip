@@ -1,0 +1,299 @@
+// Fused per-entry program: key rewrites (arithmetic, QRAM XOR-load, bit ops)
+// and diagonal amplitude phases applied to every entry in one streaming pass.
+// HBM bytes per entry: 2*8*W (+32 when any op touches the amplitude)
+// (+ the table word for each QRAM load, served mostly from L2/MALL).
+#include <cmath>
+
+#include "engine.hpp"
+#include "keyops.hpp"
+
+namespace sparq {
+
+struct PwProgram {
+    uint32_t nops, nconds;
+    sparq_pw_op_t ops[SPARQ_MAX_PROGRAM_OPS];
+    sparq_cond_t conds[SPARQ_MAX_PROGRAM_CONDS];
+};
+
+// Fixed-point encoding of x in [0,1) into w bits by truncation; identical to the
+// host layer's rational encoding.
+__device__ __forceinline__ uint64_t rational_bits(double x, uint32_t w) {
+    double scaled = (w >= 64) ? x * 2.0 * 9223372036854775808.0 : x * (double)(uint64_t(1) << w);
+    uint64_t v = scaled <= 0 ? 0 : (uint64_t)scaled;
+    return v & fmask(w);
+}
+
+__device__ __forceinline__ uint64_t isqrt_u64(uint64_t v) {
+    uint64_t r = (uint64_t)sqrt((double)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+__device__ __forceinline__ uint64_t table_word(const void* t, uint32_t bytes, uint64_t i) {
+    switch (bytes) {
+        case 1: return ((const uint8_t*)t)[i];
+        case 2: return ((const uint16_t*)t)[i];
+        case 4: return ((const uint32_t*)t)[i];
+        default: return ((const uint64_t*)t)[i];
+    }
+}
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+template <int W>
+__device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W], double2& amp) {
+    const sparq_field_t f0 = op.f[0], f1 = op.f[1], f2 = op.f[2], f3 = op.f[3];
+    switch (op.opcode) {
+        case SPARQ_PW_FLIP_BIT: fput<W>(k, f0, fget<W>(k, f0) ^ (uint64_t(1) << op.c[0])); break;
+        case SPARQ_PW_FLIP_ALL: fput<W>(k, f0, ~fget<W>(k, f0)); break;
+        case SPARQ_PW_SWAP_BITS: {
+            uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
+            uint64_t ba = (a >> op.c[0]) & 1, bb = (b >> op.c[1]) & 1;
+            a = (a & ~(uint64_t(1) << op.c[0])) | (bb << op.c[0]);
+            b = (b & ~(uint64_t(1) << op.c[1])) | (ba << op.c[1]);
+            fput<W>(k, f0, a);
+            fput<W>(k, f1, b);
+            break;
+        }
+        case SPARQ_PW_SWAP_FIELDS: {
+            uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
+            fput<W>(k, f0, b);
+            fput<W>(k, f1, a);
+            break;
+        }
+        case SPARQ_PW_ROTL: {
+            uint32_t size = f0.width, d = (uint32_t)op.c[0];
+            uint64_t v = fget<W>(k, f0);
+            uint64_t high = (size - d) >= 64 ? 0 : v >> (size - d);
+            uint64_t low = v - ((size - d) >= 64 ? 0 : (high << (size - d)));
+            fput<W>(k, f0, (d >= 64 ? 0 : (low << d)) + high);
+            break;
+        }
+        case SPARQ_PW_ROTR: {
+            uint32_t size = f0.width, d = (uint32_t)op.c[0];
+            uint64_t v = fget<W>(k, f0);
+            uint64_t high = d >= 64 ? 0 : v >> d;
+            uint64_t low = v - (d >= 64 ? 0 : (high << d));
+            fput<W>(k, f0, ((size - d) >= 64 ? 0 : (low << (size - d))) + high);
+            break;
+        }
+        case SPARQ_PW_XOR_MUL_CONST: fput<W>(k, f1, fget<W>(k, f1) ^ (fget<W>(k, f0) * op.c[0])); break;
+        case SPARQ_PW_ADD_MUL_CONST: fput<W>(k, f1, fget<W>(k, f1) + op.c[0] * fget<W>(k, f0)); break;
+        case SPARQ_PW_MODMUL_CONST: fput<W>(k, f0, (fget<W>(k, f0) * op.c[0]) % op.c[1]); break;
+        case SPARQ_PW_XOR_ADD: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) + fget<W>(k, f1))); break;
+        case SPARQ_PW_ADD_FIELD: fput<W>(k, f1, fget<W>(k, f1) + fget<W>(k, f0)); break;
+        case SPARQ_PW_SUB_FIELD: fput<W>(k, f1, fget<W>(k, f1) - fget<W>(k, f0)); break;
+        case SPARQ_PW_XOR_ADD_CONST: fput<W>(k, f1, fget<W>(k, f1) ^ (fget<W>(k, f0) + op.c[0])); break;
+        case SPARQ_PW_ADD_CONST: fput<W>(k, f0, fget<W>(k, f0) + op.c[0]); break;
+        case SPARQ_PW_XOR_SUB: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) - fget<W>(k, f1))); break;
+        case SPARQ_PW_XOR_NEG: fput<W>(k, f1, fget<W>(k, f1) ^ (uint64_t(0) - fget<W>(k, f0))); break;
+        case SPARQ_PW_XOR_ABS: {
+            int64_t v = sext(fget<W>(k, f0), f0.width);
+            uint64_t mag = v < 0 ? (uint64_t)(-v) : (uint64_t)v;
+            fput<W>(k, f1, fget<W>(k, f1) ^ mag);
+            break;
+        }
+        case SPARQ_PW_XOR_MUL: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) * fget<W>(k, f1))); break;
+        case SPARQ_PW_XOR_DIV: {
+            uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
+            fput<W>(k, f2, fget<W>(k, f2) ^ (b == 0 ? 0 : a / b));
+            break;
+        }
+        case SPARQ_PW_XOR_ISQRT: fput<W>(k, f1, fget<W>(k, f1) ^ isqrt_u64(fget<W>(k, f0))); break;
+        case SPARQ_PW_XOR_SELECT: {
+            uint64_t sel = (fget<W>(k, f0) & 1) ? fget<W>(k, f1) : fget<W>(k, f2);
+            fput<W>(k, f3, fget<W>(k, f3) ^ sel);
+            break;
+        }
+        case SPARQ_PW_XOR_AND: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) & fget<W>(k, f1))); break;
+        case SPARQ_PW_XOR_OR: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) | fget<W>(k, f1))); break;
+        case SPARQ_PW_XOR_XOR: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) ^ fget<W>(k, f1))); break;
+        case SPARQ_PW_FLAG_LESS_S: {
+            bool p = sext(fget<W>(k, f0), f0.width) < sext(fget<W>(k, f1), f1.width);
+            fput<W>(k, f2, fget<W>(k, f2) ^ (p ? 1 : 0));
+            break;
+        }
+        case SPARQ_PW_FLAG_CARRY: {
+            uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
+            uint32_t w = f2.width;
+            bool p = w >= 64 ? (a + b < a) : (a >= (uint64_t(1) << w)) || (b >= (uint64_t(1) << w) - a);
+            fput<W>(k, f3, fget<W>(k, f3) ^ (p ? 1 : 0));
+            break;
+        }
+        case SPARQ_PW_FLAG_OVERFLOW_S: {
+            uint32_t w = f2.width;
+            uint64_t m = fmask(w);
+            uint64_t A = (uint64_t)sext(fget<W>(k, f0), f0.width) & m;
+            uint64_t B = (uint64_t)sext(fget<W>(k, f1), f1.width) & m;
+            uint64_t S = (A + B) & m;
+            uint64_t sa = (A >> (w - 1)) & 1, sb = (B >> (w - 1)) & 1, ss = (S >> (w - 1)) & 1;
+            fput<W>(k, f3, fget<W>(k, f3) ^ (((sa == sb) && (ss != sa)) ? 1 : 0));
+            break;
+        }
+        case SPARQ_PW_FLAG_MULOVERFLOW: {
+            uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
+            uint64_t lo = a * b, hi = __umul64hi(a, b);
+            uint32_t w = f2.width;
+            bool p = hi != 0 || (w < 64 && lo >= (uint64_t(1) << w));
+            fput<W>(k, f3, fget<W>(k, f3) ^ (p ? 1 : 0));
+            break;
+        }
+        case SPARQ_PW_FLAG_ISZERO: fput<W>(k, f1, fget<W>(k, f1) ^ (fget<W>(k, f0) == 0 ? 1 : 0)); break;
+        case SPARQ_PW_FLAG_NEGATIVE: fput<W>(k, f1, fget<W>(k, f1) ^ (sext(fget<W>(k, f0), f0.width) < 0 ? 1 : 0)); break;
+        case SPARQ_PW_ADD_ANY:
+        case SPARQ_PW_SUB_ANY: {
+            uint64_t r = fget<W>(k, f1);
+            uint64_t e = (f1.flags & 1) ? (uint64_t)sext(r, f1.width) : r;
+            uint64_t v = fget<W>(k, f0);
+            fput<W>(k, f0, op.opcode == SPARQ_PW_ADD_ANY ? v + e : v - e);
+            break;
+        }
+        case SPARQ_PW_XOR_ASSIGN: fput<W>(k, f1, fget<W>(k, f1) ^ fget<W>(k, f0)); break;
+        case SPARQ_PW_COMPARE: {
+            uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
+            if (a == b) fput<W>(k, f3, fget<W>(k, f3) ^ 1);
+            else fput<W>(k, f2, fget<W>(k, f2) ^ (a < b ? 1 : 0));
+            break;
+        }
+        case SPARQ_PW_FLAG_LESS_U: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) < fget<W>(k, f1) ? 1 : 0)); break;
+        case SPARQ_PW_XOR_MID: fput<W>(k, f2, fget<W>(k, f2) ^ ((fget<W>(k, f0) + fget<W>(k, f1)) / 2)); break;
+        case SPARQ_PW_XOR_DIVSQRT_ACOS: {
+            double o = acos(sqrt(1.0 * fget<W>(k, f0) / fget<W>(k, f1))) / kPi / 2;
+            fput<W>(k, f2, fget<W>(k, f2) ^ rational_bits(o, f2.width));
+            break;
+        }
+        case SPARQ_PW_XOR_SQRTDIV_ACOS: {
+            double o = acos(sext(fget<W>(k, f0), f0.width) / sqrt((double)fget<W>(k, f1))) / kPi / 2;
+            fput<W>(k, f2, fget<W>(k, f2) ^ rational_bits(o, f2.width));
+            break;
+        }
+        case SPARQ_PW_XOR_ROTATE_ANGLE: {
+            int64_t l = sext(fget<W>(k, f0), f0.width), r = sext(fget<W>(k, f1), f1.width);
+            double o;
+            if (l == 0 && r >= 0) o = 0.25;
+            else if (l == 0 && r < 0) o = 0.75;
+            else {
+                o = atan2((double)r, (double)l) / kPi / 2;
+                if (o < 0) o += 1;
+            }
+            fput<W>(k, f2, fget<W>(k, f2) ^ rational_bits(o, f2.width));
+            break;
+        }
+        case SPARQ_PW_QRAM_LOAD: {
+            uint64_t a = fget<W>(k, f0) & fmask((uint32_t)op.c[0]);
+            fput<W>(k, f1, fget<W>(k, f1) ^ table_word(op.table, op.table_word_bytes, a));
+            break;
+        }
+        case SPARQ_PW_PHASE_BITS: {
+            uint64_t bits = fget<W>(k, f0) & op.c[0];
+            const double2 ph = make_double2(op.d[0], op.d[1]);
+            while (bits) {
+                amp = cmul(amp, ph);
+                bits &= bits - 1;
+            }
+            break;
+        }
+        case SPARQ_PW_RZ: {
+            bool b = (fget<W>(k, f0) >> op.c[0]) & 1;
+            amp = cmul(amp, b ? make_double2(op.d[0], op.d[1]) : make_double2(op.d[2], op.d[3]));
+            break;
+        }
+        case SPARQ_PW_Y: {
+            uint64_t v = fget<W>(k, f0) ^ (uint64_t(1) << op.c[0]);
+            fput<W>(k, f0, v);
+            amp = cmul(amp, ((v >> op.c[0]) & 1) ? make_double2(0.0, 1.0) : make_double2(0.0, -1.0));
+            break;
+        }
+        case SPARQ_PW_ROT_DIAG: {
+            bool b = (fget<W>(k, f0) >> op.c[0]) & 1;
+            amp = cmul(amp, b ? make_double2(op.d[2], op.d[3]) : make_double2(op.d[0], op.d[1]));
+            break;
+        }
+        case SPARQ_PW_ROT_OFFDIAG: {
+            uint64_t v = fget<W>(k, f0);
+            bool b = (v >> op.c[0]) & 1;
+            amp = cmul(amp, b ? make_double2(op.d[2], op.d[3]) : make_double2(op.d[0], op.d[1]));
+            fput<W>(k, f0, v ^ (uint64_t(1) << op.c[0]));
+            break;
+        }
+        case SPARQ_PW_NEG_AMP: amp = make_double2(amp.x * -1.0, amp.y * -1.0); break;
+        case SPARQ_PW_MUL_AMP: amp = cmul(amp, make_double2(op.d[0], op.d[1])); break;
+        case SPARQ_PW_SET_FIELD: fput<W>(k, f0, op.c[0]); break;
+        default: break;
+    }
+}
+
+template <int W, bool AMP>
+__global__ void __launch_bounds__(kBlock) pointwise_kernel(uint64_t* __restrict__ keys, double2* __restrict__ amps, uint64_t cap,
+                                                      uint64_t n, const PwProgram prog) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k[W];
+    load_key<W>(keys, cap, i, k);
+    double2 a = make_double2(0.0, 0.0);
+    if (AMP) a = amps[i];
+    for (uint32_t o = 0; o < prog.nops; ++o) {
+        const sparq_pw_op_t& op = prog.ops[o];
+        bool ok = true;
+        for (uint32_t c = 0; c < op.ncond; ++c) ok = ok && cond_ok<W>(k, prog.conds[op.cond_first + c]);
+        if (ok) run_op<W>(op, k, a);
+    }
+    store_key<W>(keys, cap, i, k);
+    if (AMP) amps[i] = a;
+}
+
+static bool touches_amp(uint32_t opcode) {
+    switch (opcode) {
+        case SPARQ_PW_PHASE_BITS: case SPARQ_PW_RZ: case SPARQ_PW_Y: case SPARQ_PW_ROT_DIAG:
+        case SPARQ_PW_ROT_OFFDIAG: case SPARQ_PW_NEG_AMP: case SPARQ_PW_MUL_AMP:
+            return true;
+        default:
+            return false;
+    }
+}
+
+template <int W>
+static void launch_pointwise(State& st, const PwProgram& prog, bool amp) {
+    unsigned g = grid_for(st.n, kBlock);
+    SPQ_PROF(st.stream, "pointwise_kernel", st.n * (16.0 * W + (amp ? 32.0 : 0.0)));
+    if (amp) pointwise_kernel<W, true><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
+    else pointwise_kernel<W, false><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
+    SPQ_HIP(hipGetLastError());
+}
+
+static void check_field(const State& st, sparq_field_t f) {
+    SPQ_REQUIRE(f.word < st.W, "field word out of range");
+    SPQ_REQUIRE(f.width >= 1 && f.width <= 64 && (uint32_t)f.shift + f.width <= 64, "bad field");
+}
+
+void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_cond_t* conds, uint32_t nconds) {
+    SPQ_REQUIRE(nops <= SPARQ_MAX_PROGRAM_OPS, "too many ops in one pointwise program");
+    SPQ_REQUIRE(nconds <= SPARQ_MAX_PROGRAM_CONDS, "too many conditions in one pointwise program");
+    PwProgram prog{};
+    prog.nops = nops;
+    prog.nconds = nconds;
+    bool amp = false;
+    for (uint32_t o = 0; o < nops; ++o) {
+        const sparq_pw_op_t& op = ops[o];
+        SPQ_REQUIRE(op.opcode >= SPARQ_PW_FLIP_BIT && op.opcode < SPARQ_PW_OPCODE_END, "unknown pointwise opcode");
+        SPQ_REQUIRE((uint32_t)op.cond_first + op.ncond <= nconds, "condition range out of bounds");
+        for (int f = 0; f < 4; ++f)
+            if (op.f[f].width) check_field(st, op.f[f]);
+        if (op.opcode == SPARQ_PW_QRAM_LOAD) {
+            SPQ_REQUIRE(op.table != nullptr, "QRAM load without table");
+            SPQ_REQUIRE(op.table_word_bytes == 1 || op.table_word_bytes == 2 || op.table_word_bytes == 4 ||
+                            op.table_word_bytes == 8, "bad table word size");
+        }
+        amp = amp || touches_amp(op.opcode);
+        prog.ops[o] = op;
+    }
+    for (uint32_t c = 0; c < nconds; ++c) {
+        check_field(st, conds[c].field);
+        prog.conds[c] = conds[c];
+    }
+    if (st.n == 0 || nops == 0) return;
+    SPQ_DISPATCH_W(st.W, launch_pointwise, st, prog, amp);
+}
+
+}  // namespace sparq

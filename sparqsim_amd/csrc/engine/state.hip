@@ -1,0 +1,280 @@
+// State storage, device-wide scans, relayout and reductions.
+#include <algorithm>
+#include <cstring>
+
+#include "blockscan.hpp"
+#include "engine.hpp"
+#include "keyops.hpp"
+
+namespace sparq {
+
+// ------------------------------------------------------------------ memory
+void* dmalloc(size_t bytes, hipStream_t s) {
+    if (bytes == 0) bytes = 256;
+    void* p = nullptr;
+    hipError_t e = hipMallocAsync(&p, bytes, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(e == hipErrorOutOfMemory ? SPARQ_OOM : SPARQ_RUNTIME,
+                    std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    return p;
+}
+
+void dfree(void* p, hipStream_t s) {
+    if (p) (void)hipFreeAsync(p, s);
+}
+
+void alloc_arrays(State& st, uint32_t W, uint64_t cap, uint64_t** keys, double2** amps) {
+    if (cap == 0) cap = 1;
+    uint64_t* k = dalloc<uint64_t>((size_t)W * cap, st.stream);
+    double2* a = nullptr;
+    try {
+        a = dalloc<double2>(cap, st.stream);
+    } catch (...) {
+        dfree(k, st.stream);
+        throw;
+    }
+    *keys = k;
+    *amps = a;
+}
+
+void install_arrays(State& st, uint32_t W, uint64_t* keys, double2* amps, uint64_t n, uint64_t cap) {
+    if (st.keys != keys) dfree(st.keys, st.stream);
+    if (st.amps != amps) dfree(st.amps, st.stream);
+    st.keys = keys;
+    st.amps = amps;
+    st.W = W;
+    st.n = n;
+    st.cap = cap;
+}
+
+void read_back(State& st, const void* dptr, size_t bytes, void* host) {
+    SPQ_HIP(hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, st.stream));
+    SPQ_HIP(hipStreamSynchronize(st.stream));
+}
+
+uint64_t read_u64(State& st, const uint64_t* dptr) {
+    SPQ_HIP(hipMemcpyAsync(st.pinned, dptr, sizeof(uint64_t), hipMemcpyDeviceToHost, st.stream));
+    SPQ_HIP(hipStreamSynchronize(st.stream));
+    return st.pinned[0];
+}
+
+// ------------------------------------------------------------------ scans
+// 2048 elements per workgroup (256 threads x 8 items).
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+template <class TIn>
+__global__ void __launch_bounds__(kBlock) scan_reduce_kernel(const TIn* __restrict__ in, uint64_t n, uint64_t* __restrict__ sums) {
+    __shared__ uint64_t lds[kBlock / 64];
+    uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) s += (base + j < n) ? (uint64_t)in[base + j] : 0;
+    uint64_t tot;
+    (void)block_excl_scan<uint64_t>(s, lds, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <class TIn>
+__global__ void __launch_bounds__(kBlock) scan_apply_kernel(const TIn* __restrict__ in, uint64_t n, const uint64_t* __restrict__ offs,
+                                                       uint64_t* __restrict__ out, uint64_t* __restrict__ total) {
+    __shared__ uint64_t lds[kBlock / 64];
+    uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = (base + j < n) ? (uint64_t)in[base + j] : 0;
+        s += v[j];
+    }
+    uint64_t tot;
+    uint64_t pre = block_excl_scan<uint64_t>(s, lds, tot) + (offs ? offs[blockIdx.x] : 0);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < n) out[base + j] = pre;
+        pre += v[j];
+    }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) total[0] = pre;
+}
+
+template <class TIn>
+static void excl_scan_impl(State& st, const TIn* in, uint64_t* out, uint64_t n, uint64_t* total_dev) {
+    if (n == 0) {
+        if (total_dev) SPQ_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint64_t), st.stream));
+        return;
+    }
+    unsigned nb = grid_for(n, kScanTile);
+    if (nb == 1) {
+        {
+            SPQ_PROF(st.stream, "scan_apply_kernel", n * ((double)sizeof(TIn) + 8));
+            scan_apply_kernel<TIn><<<1, kBlock, 0, st.stream>>>(in, n, nullptr, out, total_dev);
+        }
+        SPQ_HIP(hipGetLastError());
+        return;
+    }
+    uint64_t* sums = dalloc<uint64_t>(nb, st.stream);
+    uint64_t* offs = dalloc<uint64_t>(nb, st.stream);
+    {
+        SPQ_PROF(st.stream, "scan_reduce_kernel", n * (double)sizeof(TIn));
+        scan_reduce_kernel<TIn><<<nb, kBlock, 0, st.stream>>>(in, n, sums);
+    }
+    SPQ_HIP(hipGetLastError());
+    excl_scan_impl<uint64_t>(st, sums, offs, nb, nullptr);
+    {
+        SPQ_PROF(st.stream, "scan_apply_kernel", n * ((double)sizeof(TIn) + 8));
+        scan_apply_kernel<TIn><<<nb, kBlock, 0, st.stream>>>(in, n, offs, out, total_dev);
+    }
+    SPQ_HIP(hipGetLastError());
+    dfree(sums, st.stream);
+    dfree(offs, st.stream);
+}
+
+void exclusive_scan_u32(State& st, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev) {
+    excl_scan_impl<uint32_t>(st, in, out, n, total_dev);
+}
+void exclusive_scan_u64(State& st, const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev) {
+    excl_scan_impl<uint64_t>(st, in, out, n, total_dev);
+}
+
+// ------------------------------------------------------------------ relayout
+struct FieldMoves {
+    uint32_t nf;
+    sparq_field_t from[64];
+    sparq_field_t to[64];
+};
+
+template <int WI, int WO>
+__global__ void __launch_bounds__(kBlock) relayout_kernel(const uint64_t* __restrict__ kin, uint64_t cap_in, uint64_t* __restrict__ kout,
+                                                     uint64_t cap_out, uint64_t n, FieldMoves mv) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t a[WI];
+    load_key<WI>(kin, cap_in, i, a);
+    uint64_t b[WO];
+#pragma unroll
+    for (int w = 0; w < WO; ++w) b[w] = 0;
+    for (uint32_t f = 0; f < mv.nf; ++f) fput<WO>(b, mv.to[f], fget<WI>(a, mv.from[f]));
+    store_key<WO>(kout, cap_out, i, b);
+}
+
+template <int WI>
+static void launch_relayout_in(State& st, uint32_t WO, uint64_t* kout, uint64_t cap_out, const FieldMoves& mv) {
+    unsigned g = grid_for(st.n, kBlock);
+#define SPQ_RL(WOV)                                                                                          \
+    case WOV: {                                                                                              \
+        SPQ_PROF(st.stream, "relayout_kernel", st.n * 8.0 * (WI + WOV));                                     \
+        relayout_kernel<WI, WOV><<<g, kBlock, 0, st.stream>>>(st.keys, st.cap, kout, cap_out, st.n, mv);     \
+    } break;
+    switch (WO) {
+        SPQ_RL(1) SPQ_RL(2) SPQ_RL(3) SPQ_RL(4) SPQ_RL(5) SPQ_RL(6) SPQ_RL(7) SPQ_RL(8)
+        default: throw Error(SPARQ_INVALID, "key width out of range");
+    }
+#undef SPQ_RL
+    SPQ_HIP(hipGetLastError());
+}
+
+void relayout(State& st, uint32_t new_words, const sparq_field_t* from, const sparq_field_t* to, uint32_t nf) {
+    SPQ_REQUIRE(new_words >= 1 && new_words <= SPARQ_MAX_WORDS, "key width out of range");
+    SPQ_REQUIRE(nf <= 64, "too many fields");
+    FieldMoves mv{};
+    mv.nf = nf;
+    for (uint32_t f = 0; f < nf; ++f) {
+        SPQ_REQUIRE(from[f].word < st.W && to[f].word < new_words, "field word out of range");
+        SPQ_REQUIRE(from[f].width == to[f].width && from[f].width >= 1 && from[f].width <= 64, "field width mismatch");
+        SPQ_REQUIRE((uint32_t)from[f].shift + from[f].width <= 64 && (uint32_t)to[f].shift + to[f].width <= 64, "field crosses a word");
+        mv.from[f] = from[f];
+        mv.to[f] = to[f];
+    }
+    uint64_t* kout = dalloc<uint64_t>((size_t)new_words * std::max<uint64_t>(st.cap, 1), st.stream);
+    if (st.n) SPQ_DISPATCH_W(st.W, launch_relayout_in, st, new_words, kout, st.cap, mv);
+    dfree(st.keys, st.stream);
+    st.keys = kout;
+    st.W = new_words;
+}
+
+// ------------------------------------------------------------------ reductions
+__global__ void __launch_bounds__(kBlock) sum_abs2_kernel(const double2* __restrict__ amps, uint64_t n, double* __restrict__ partial) {
+    __shared__ double lds[kBlock / 64];
+    double s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) s += cabs2(amps[i]);
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+double sum_abs2(State& st) {
+    if (st.n == 0) return 0.0;
+    unsigned g = std::min<unsigned>(grid_for(st.n, kBlock), 1024);
+    double* partial = dalloc<double>(g, st.stream);
+    {
+        SPQ_PROF(st.stream, "sum_abs2_kernel", st.n * 16.0);
+        sum_abs2_kernel<<<g, kBlock, 0, st.stream>>>(st.amps, st.n, partial);
+    }
+    SPQ_HIP(hipGetLastError());
+    std::vector<double> h(g);
+    read_back(st, partial, g * sizeof(double), h.data());
+    dfree(partial, st.stream);
+    double s = 0;
+    for (double v : h) s += v;
+    return s;
+}
+
+__global__ void __launch_bounds__(kBlock) scale_kernel(double2* __restrict__ amps, uint64_t n, double s) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) amps[i] = cscale(amps[i], s);
+}
+
+void scale_amps(State& st, double s) {
+    if (!st.n) return;
+    {
+        SPQ_PROF(st.stream, "scale_kernel", st.n * 32.0);
+        scale_kernel<<<grid_for(st.n, kBlock), kBlock, 0, st.stream>>>(st.amps, st.n, s);
+    }
+    SPQ_HIP(hipGetLastError());
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) field_minmax_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, sparq_field_t f,
+                                                         unsigned long long* mm) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k[W];
+    load_key<W>(keys, cap, i, k);
+    unsigned long long v = fget<W>(k, f);
+    atomicMin(&mm[0], v);
+    atomicMax(&mm[1], v);
+}
+
+template <int W>
+static void launch_minmax(State& st, sparq_field_t f, unsigned long long* mm) {
+    {
+        SPQ_PROF(st.stream, "field_minmax_kernel", st.n * 8.0 * W);
+        field_minmax_kernel<W><<<grid_for(st.n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, st.n, f, mm);
+    }
+    SPQ_HIP(hipGetLastError());
+}
+
+bool field_uniform(State& st, sparq_field_t f, uint64_t* value) {
+    if (st.n == 0) {
+        *value = 0;
+        return true;
+    }
+    unsigned long long* mm = dalloc<unsigned long long>(2, st.stream);
+    unsigned long long init[2] = {~0ull, 0ull};
+    SPQ_HIP(hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, st.stream));
+    SPQ_DISPATCH_W(st.W, launch_minmax, st, f, mm);
+    unsigned long long h[2];
+    read_back(st, mm, sizeof(h), h);
+    dfree(mm, st.stream);
+    *value = h[0];
+    return h[0] == h[1];
+}
+
+}  // namespace sparq

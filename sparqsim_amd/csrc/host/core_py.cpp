@@ -1,0 +1,285 @@
+// pybind11 module `sparqsim_amd._core`: the pysparq._core surface for the
+// sparse-state operator-apply path (PySparQ/core.cpp:21-180,
+// PySparQ/BindUtils.h:117-207).  Device work runs with the GIL released.
+#include <pybind11/complex.h>
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "sparq.hpp"
+
+namespace py = pybind11;
+using namespace sparqsim;
+
+namespace {
+
+using Release = py::call_guard<py::gil_scoped_release>;
+
+size_t resolve(const py::handle& h) {
+    if (py::isinstance<py::str>(h)) return System::get(h.cast<std::string>());
+    return h.cast<size_t>();
+}
+
+u22_t to_u22(const py::handle& m) {
+    auto v = m.cast<std::vector<complex_t>>();
+    if (v.size() != 4) throw std::invalid_argument("2x2 matrix must have 4 entries (row-major)");
+    return u22_t{v[0], v[1], v[2], v[3]};
+}
+
+// conditioned_by_* / clear_control_* (PySparQ/BindUtils.h:117-190)
+template <class T, class... Extra>
+void bind_controllable(py::class_<T, Extra...>& c) {
+    c.def_property_readonly("condition_variable_nonzeros", [](T& o) { return o.condition_variable_nonzeros; })
+        .def_property_readonly("condition_variable_all_ones", [](T& o) { return o.condition_variable_all_ones; })
+        .def_property_readonly("condition_variable_by_bit", [](T& o) { return o.condition_variable_by_bit; })
+        .def_property_readonly("condition_variable_by_value", [](T& o) { return o.condition_variable_by_value; })
+        .def("conditioned_by_nonzeros", [](T& o, py::object c) -> T& {
+            o.condition_variable_nonzeros.clear();
+            if (py::isinstance<py::str>(c) || py::isinstance<py::int_>(c)) o.condition_variable_nonzeros.push_back(resolve(c));
+            else for (auto it : c) o.condition_variable_nonzeros.push_back(resolve(it));
+            return o;
+        }, py::arg("cond"), py::return_value_policy::reference_internal)
+        .def("conditioned_by_all_ones", [](T& o, py::object c) -> T& {
+            o.condition_variable_all_ones.clear();
+            if (py::isinstance<py::str>(c) || py::isinstance<py::int_>(c)) o.condition_variable_all_ones.push_back(resolve(c));
+            else for (auto it : c) o.condition_variable_all_ones.push_back(resolve(it));
+            return o;
+        }, py::arg("cond"), py::return_value_policy::reference_internal)
+        .def("conditioned_by_bit", [](T& o, py::object c, py::object pos) -> T& {
+            o.condition_variable_by_bit.clear();
+            if (pos.is_none()) {
+                for (auto it : c) {
+                    auto t = it.cast<py::tuple>();
+                    o.condition_variable_by_bit.emplace_back(resolve(t[0]), t[1].cast<size_t>());
+                }
+            } else {
+                o.condition_variable_by_bit.emplace_back(resolve(c), pos.cast<size_t>());
+            }
+            return o;
+        }, py::arg("cond"), py::arg("pos") = py::none(), py::return_value_policy::reference_internal)
+        .def("conditioned_by_value", [](T& o, py::object c, py::object val) -> T& {
+            o.condition_variable_by_value.clear();
+            if (val.is_none()) {
+                for (auto it : c) {
+                    auto t = it.cast<py::tuple>();
+                    o.condition_variable_by_value.emplace_back(resolve(t[0]), t[1].cast<uint64_t>());
+                }
+            } else {
+                o.condition_variable_by_value.emplace_back(resolve(c), val.cast<uint64_t>());
+            }
+            return o;
+        }, py::arg("cond"), py::arg("value") = py::none(), py::return_value_policy::reference_internal)
+        .def("clear_control_nonzeros", [](T& o) { o.clear_control_nonzeros(); })
+        .def("clear_control_all_ones", [](T& o) { o.clear_control_all_ones(); })
+        .def("clear_control_by_bit", [](T& o) { o.clear_control_by_bit(); })
+        .def("clear_control_by_value", [](T& o) { o.clear_control_by_value(); });
+}
+
+template <class T>
+py::class_<T, PointwiseOperator, std::shared_ptr<T>> pw_class(py::module_& m, const char* name) {
+    return py::class_<T, PointwiseOperator, std::shared_ptr<T>>(m, name);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_core, m) {
+    m.doc() = "MI355X sparse-state engine: pysparq-compatible register-level API (HIP backend)";
+
+    py::enum_<StateStorageType>(m, "StateStorageType")
+        .value("General", General).value("UnsignedInteger", UnsignedInteger)
+        .value("SignedInteger", SignedInteger).value("Boolean", Boolean).value("Rational", Rational)
+        .export_values();
+
+    py::class_<System>(m, "System")
+        .def_static("clear", &System::clear)
+        .def_static("add_register", [](std::string_view n, StateStorageType t, size_t s) { return System::add_register(n, t, s); })
+        .def_static("get_id", [](std::string_view n) { return System::get(n); })
+        .def_static("size_of", [](py::object r) { return py::isinstance<py::str>(r) ? System::size_of(r.cast<std::string>()) : System::size_of(r.cast<size_t>()); })
+        .def_static("type_of", [](py::object r) { return py::isinstance<py::str>(r) ? System::type_of(r.cast<std::string>()) : System::type_of(r.cast<size_t>()); })
+        .def_static("status_of", [](py::object r) { return py::isinstance<py::str>(r) ? System::status_of(r.cast<std::string>()) : System::status_of(r.cast<size_t>()); })
+        .def_static("name_of", [](size_t id) { return System::name_of(id); })
+        .def_static("remove_register", [](py::object r) { System::remove_register(resolve(r)); })
+        .def_static("get_qubit_count", &System::get_qubit_count)
+        .def_static("get_activated_register_size", &System::get_activated_register_size)
+        .def_static("get_register_info", &System::get_register_info)
+        .def_static("update_max_size", &System::update_max_size)
+        .def_static("max_system_size", []() { return System::max_system_size; })
+        .def_static("max_qubit_count", []() { return System::max_qubit_count; })
+        .def_static("max_register_count", []() { return System::max_register_count; })
+        .def_static("reusable_registers", []() { return System::reusable_registers; })
+        .def_static("register_count", []() { return System::name_register_map.size(); });
+
+    py::class_<SparseState>(m, "SparseState")
+        .def(py::init<>())
+        .def(py::init<const SparseState&>(), py::arg("other"))
+        .def("clone", [](const SparseState& s) { return SparseState(s); })
+        .def("size", [](SparseState& s) { return s.size(); }, Release())
+        .def("empty", [](SparseState& s) { return s.empty(); }, Release())
+        .def("key_words", [](SparseState& s) { return s.sync().words; })
+        .def("_handle", [](SparseState& s) { return reinterpret_cast<std::uintptr_t>(s.handle()); })
+        .def("_cpp_ptr", [](SparseState& s) { return reinterpret_cast<std::uintptr_t>(&s); })
+        .def("flush", [](SparseState& s) { s.flush(); }, Release())
+        // export(): (active ids ascending, values[N, R] uint64, amps[N] complex128)
+        .def("export", [](SparseState& s) {
+            std::vector<size_t> ids;
+            std::vector<uint64_t> vals;
+            std::vector<complex_t> amps;
+            {
+                py::gil_scoped_release r;
+                s.export_entries(ids, vals, amps);
+            }
+            size_t n = amps.size(), R = ids.size();
+            py::array_t<uint64_t> v({n, R});
+            if (n * R) std::memcpy(v.mutable_data(), vals.data(), n * R * 8);
+            py::array_t<std::complex<double>> a(n);
+            if (n) std::memcpy(a.mutable_data(), amps.data(), n * 16);
+            return py::make_tuple(ids, v, a);
+        })
+        .def("load", [](SparseState& s, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> vals,
+                        py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> amps) {
+            std::vector<uint64_t> v(vals.data(), vals.data() + vals.size());
+            std::vector<complex_t> a(amps.data(), amps.data() + amps.size());
+            py::gil_scoped_release r;
+            s.import_entries(v, a);
+        })
+        .def("to_string", [](SparseState& s, int precision) { return s.to_string(precision); }, py::arg("precision") = 6)
+        .def("__str__", [](SparseState& s) { return s.to_string(6); })
+        .def("__repr__", [](SparseState& s) { return "<SparseState entries=" + std::to_string(s.size()) + ">"; });
+
+    py::class_<BaseOperator, std::shared_ptr<BaseOperator>>(m, "BaseOperator")
+        .def("__call__", [](const BaseOperator& o, SparseState& s) { o(s); }, py::arg("state"), Release())
+        .def("dag", [](const BaseOperator& o, SparseState& s) { o.dag(s); }, py::arg("state"), Release());
+    py::class_<SelfAdjointOperator, BaseOperator, std::shared_ptr<SelfAdjointOperator>>(m, "SelfAdjointOperator");
+
+    auto pwo = py::class_<PointwiseOperator, BaseOperator, std::shared_ptr<PointwiseOperator>>(m, "PointwiseOperator");
+    pwo.def_readonly("name", &PointwiseOperator::name);
+    bind_controllable(pwo);
+
+#define SPQ_BIND3(NAME) pw_class<NAME>(m, #NAME).def(py::init<std::string_view, std::string_view, std::string_view>()).def(py::init<size_t, size_t, size_t>());
+#define SPQ_BIND2(NAME) pw_class<NAME>(m, #NAME).def(py::init<std::string_view, std::string_view>()).def(py::init<size_t, size_t>());
+#define SPQ_BIND4(NAME) pw_class<NAME>(m, #NAME).def(py::init<std::string_view, std::string_view, std::string_view, std::string_view>()).def(py::init<size_t, size_t, size_t, size_t>());
+#define SPQ_BINDRCR(NAME) pw_class<NAME>(m, #NAME).def(py::init<std::string_view, uint64_t, std::string_view>()).def(py::init<size_t, uint64_t, size_t>());
+#define SPQ_BINDRC(NAME) pw_class<NAME>(m, #NAME).def(py::init<std::string_view, uint64_t>()).def(py::init<size_t, uint64_t>());
+    SPQ_BIND3(Add_UInt_UInt) SPQ_BIND3(Sub_UInt_UInt) SPQ_BIND3(Mul_UInt_UInt) SPQ_BIND3(Div_UInt_UInt)
+    SPQ_BIND3(And_UInt_UInt) SPQ_BIND3(Or_UInt_UInt) SPQ_BIND3(Xor_UInt_UInt) SPQ_BIND3(Less_SInt_SInt)
+    SPQ_BIND3(Less_UInt_UInt) SPQ_BIND3(GetMid_UInt_UInt) SPQ_BIND3(Div_Sqrt_Arccos_UInt_UInt)
+    SPQ_BIND3(Sqrt_Div_Arccos_Int_UInt) SPQ_BIND3(GetRotateAngle_Int_Int)
+    SPQ_BIND2(Add_UInt_UInt_InPlace) SPQ_BIND2(Neg_UInt) SPQ_BIND2(Abs_SInt) SPQ_BIND2(Sqrt_UInt) SPQ_BIND2(IsZero_UInt)
+    SPQ_BIND2(Negative_SInt) SPQ_BIND2(Add_AnyInt_AnyInt_InPlace) SPQ_BIND2(Assign) SPQ_BIND2(Swap_General_General)
+    SPQ_BIND4(Select_Bool_UInt_UInt) SPQ_BIND4(Carry_UInt_UInt) SPQ_BIND4(Overflow_SInt_SInt)
+    SPQ_BIND4(MulOverflow_UInt_UInt) SPQ_BIND4(Compare_UInt_UInt)
+    SPQ_BINDRCR(Mult_UInt_ConstUInt) SPQ_BINDRCR(Add_Mult_UInt_ConstUInt_InPlace) SPQ_BINDRCR(Add_UInt_ConstUInt)
+    SPQ_BINDRC(Add_ConstUInt_InPlace) SPQ_BINDRC(ShiftLeft_InPlace) SPQ_BINDRC(ShiftRight_InPlace) SPQ_BINDRC(Init_Unsafe)
+    pw_class<X_Bool>(m, "X_Bool").def(py::init<std::string_view, uint64_t>(), py::arg("reg"), py::arg("digit") = 0)
+        .def(py::init<size_t, uint64_t>(), py::arg("reg_id"), py::arg("digit") = 0);
+    pw_class<Y_Bool>(m, "Y_Bool").def(py::init<std::string_view, uint64_t>(), py::arg("reg"), py::arg("digit") = 0)
+        .def(py::init<size_t, uint64_t>(), py::arg("reg_id"), py::arg("digit") = 0);
+    pw_class<FlipBools>(m, "FlipBools").def(py::init<std::string_view>()).def(py::init<size_t>());
+    pw_class<Mod_Mult_UInt_ConstUInt_InPlace>(m, "Mod_Mult_UInt_ConstUInt_InPlace")
+        .def(py::init<std::string_view, uint64_t, uint64_t, uint64_t>()).def(py::init<size_t, uint64_t, uint64_t, uint64_t>());
+    pw_class<Swap_Bool_Bool>(m, "Swap_Bool_Bool")
+        .def(py::init<std::string_view, size_t, std::string_view, size_t>()).def(py::init<size_t, size_t, size_t, size_t>());
+    pw_class<Phase_Bool>(m, "Phase_Bool")
+        .def(py::init<std::string_view, size_t, double>(), py::arg("reg"), py::arg("digit"), py::arg("lambda_"))
+        .def(py::init<size_t, size_t, double>(), py::arg("reg_id"), py::arg("digit"), py::arg("lambda_"))
+        .def(py::init([](std::string_view r, double l) { return std::make_shared<Phase_Bool>(r, 0, l); }), py::arg("reg"), py::arg("lambda_"));
+    py::class_<Z_Bool, Phase_Bool, std::shared_ptr<Z_Bool>>(m, "Z_Bool")
+        .def(py::init<std::string_view, size_t>(), py::arg("reg"), py::arg("digit") = 0)
+        .def(py::init<size_t, size_t>(), py::arg("reg_id"), py::arg("digit") = 0);
+    py::class_<S_Bool, Phase_Bool, std::shared_ptr<S_Bool>>(m, "S_Bool")
+        .def(py::init<std::string_view, size_t>(), py::arg("reg"), py::arg("digit") = 0)
+        .def(py::init<size_t, size_t>(), py::arg("reg_id"), py::arg("digit") = 0);
+    py::class_<T_Bool, Phase_Bool, std::shared_ptr<T_Bool>>(m, "T_Bool")
+        .def(py::init<std::string_view, size_t>(), py::arg("reg"), py::arg("digit") = 0)
+        .def(py::init<size_t, size_t>(), py::arg("reg_id"), py::arg("digit") = 0);
+    pw_class<RZ_Bool>(m, "RZ_Bool").def(py::init<std::string_view, size_t, double>()).def(py::init<size_t, size_t, double>());
+    pw_class<ZeroConditionalPhaseFlip>(m, "ZeroConditionalPhaseFlip")
+        .def(py::init<const std::vector<std::string>&>(), py::arg("regs"))
+        .def(py::init<const std::vector<size_t>&>(), py::arg("reg_ids"));
+    pw_class<Reflection_Bool>(m, "Reflection_Bool")
+        .def(py::init<std::string_view, bool>(), py::arg("reg"), py::arg("inverse") = false)
+        .def(py::init<const std::vector<std::string>&, bool>(), py::arg("regs"), py::arg("inverse") = false)
+        .def(py::init<const std::vector<size_t>&, bool>(), py::arg("reg_ids"), py::arg("inverse") = false);
+    pw_class<GlobalPhase>(m, "GlobalPhase").def(py::init<complex_t>(), py::arg("phase"));
+
+    py::class_<QRAMCircuit_qutrit, std::shared_ptr<QRAMCircuit_qutrit>>(m, "QRAMCircuit_qutrit")
+        .def(py::init<size_t, size_t, std::vector<uint64_t>>(), py::arg("addr_size"), py::arg("data_size"), py::arg("memory"))
+        .def_readonly("address_size", &QRAMCircuit_qutrit::address_size)
+        .def_readonly("data_size", &QRAMCircuit_qutrit::data_size);
+    pw_class<QRAMLoad>(m, "QRAMLoad")
+        .def(py::init<const QRAMCircuit_qutrit*, std::string_view, std::string_view>(), py::keep_alive<1, 2>())
+        .def(py::init<const QRAMCircuit_qutrit*, size_t, size_t>(), py::keep_alive<1, 2>());
+    m.attr("QRAMLoadFast") = m.attr("QRAMLoad");
+
+    // branching
+    auto hi = py::class_<Hadamard_Int, SelfAdjointOperator, std::shared_ptr<Hadamard_Int>>(m, "Hadamard_Int")
+        .def(py::init<std::string_view, size_t>(), py::arg("reg_in"), py::arg("n_digits"))
+        .def(py::init<size_t, size_t>(), py::arg("reg_in"), py::arg("n_digits"));
+    bind_controllable(hi);
+    auto hf = py::class_<Hadamard_Int_Full, SelfAdjointOperator, std::shared_ptr<Hadamard_Int_Full>>(m, "Hadamard_Int_Full")
+        .def(py::init<std::string_view>(), py::arg("reg_in")).def(py::init<size_t>(), py::arg("reg_in"));
+    bind_controllable(hf);
+    auto hb = py::class_<Hadamard_Bool, SelfAdjointOperator, std::shared_ptr<Hadamard_Bool>>(m, "Hadamard_Bool")
+        .def(py::init<std::string_view>(), py::arg("reg_in")).def(py::init<size_t>(), py::arg("reg_in"));
+    bind_controllable(hb);
+    py::class_<CondRot_Rational_Bool, BaseOperator, std::shared_ptr<CondRot_Rational_Bool>>(m, "CondRot_Rational_Bool")
+        .def(py::init<std::string_view, std::string_view>()).def(py::init<size_t, size_t>());
+    py::class_<CondRot_Fixed_Bool, CondRot_Rational_Bool, std::shared_ptr<CondRot_Fixed_Bool>>(m, "CondRot_Fixed_Bool")
+        .def(py::init<std::string_view, std::string_view>()).def(py::init<size_t, size_t>());
+    py::class_<CondRot_General_Bool, BaseOperator, std::shared_ptr<CondRot_General_Bool>>(m, "CondRot_General_Bool")
+        .def(py::init([](py::object in, py::object out, py::function f) {
+            std::function<u22_t(uint64_t)> fn = [f](uint64_t v) { py::gil_scoped_acquire g; return to_u22(f(v)); };
+            return std::make_shared<CondRot_General_Bool>(resolve(in), resolve(out), fn);
+        }), py::arg("reg_in"), py::arg("reg_out"), py::arg("angle_function"));
+    m.def("make_func", [](uint64_t v, size_t n) { auto u = make_func(v, n); return std::vector<complex_t>(u.begin(), u.end()); });
+    m.def("make_func_inv", [](uint64_t v, size_t n) { auto u = make_func_inv(v, n); return std::vector<complex_t>(u.begin(), u.end()); });
+    auto rb = py::class_<Rot_Bool, BaseOperator, std::shared_ptr<Rot_Bool>>(m, "Rot_Bool")
+        .def(py::init([](py::object r, size_t d, py::object mat) { return std::make_shared<Rot_Bool>(resolve(r), d, to_u22(mat)); }),
+             py::arg("reg"), py::arg("digit"), py::arg("matrix"));
+    bind_controllable(rb);
+    py::class_<RX_Bool, Rot_Bool, std::shared_ptr<RX_Bool>>(m, "RX_Bool")
+        .def(py::init<std::string_view, size_t, double>()).def(py::init<size_t, size_t, double>());
+    py::class_<RY_Bool, Rot_Bool, std::shared_ptr<RY_Bool>>(m, "RY_Bool")
+        .def(py::init<std::string_view, size_t, double>()).def(py::init<size_t, size_t, double>());
+    py::class_<SX_Bool, Rot_Bool, std::shared_ptr<SX_Bool>>(m, "SX_Bool")
+        .def(py::init<std::string_view, size_t>(), py::arg("reg"), py::arg("digit") = 0)
+        .def(py::init<size_t, size_t>(), py::arg("reg_id"), py::arg("digit") = 0);
+    py::class_<U2_Bool, Rot_Bool, std::shared_ptr<U2_Bool>>(m, "U2_Bool")
+        .def(py::init<std::string_view, size_t, double, double>()).def(py::init<size_t, size_t, double, double>());
+    py::class_<U3_Bool, Rot_Bool, std::shared_ptr<U3_Bool>>(m, "U3_Bool")
+        .def(py::init<std::string_view, size_t, double, double, double>()).def(py::init<size_t, size_t, double, double, double>());
+
+    // structural
+    py::class_<ClearZero, SelfAdjointOperator, std::shared_ptr<ClearZero>>(m, "ClearZero")
+        .def(py::init<>()).def(py::init<double>(), py::arg("eps"));
+    m.def("set_clearzero_default_eps", [](double e) { clearzero_default_eps() = e; });
+    m.def("get_clearzero_default_eps", []() { return clearzero_default_eps(); });
+    py::class_<Normalize, SelfAdjointOperator, std::shared_ptr<Normalize>>(m, "Normalize").def(py::init<>());
+    py::class_<SortUnconditional, SelfAdjointOperator, std::shared_ptr<SortUnconditional>>(m, "SortUnconditional").def(py::init<>());
+    py::class_<CheckDuplicateKey, SelfAdjointOperator, std::shared_ptr<CheckDuplicateKey>>(m, "CheckDuplicateKey").def(py::init<>());
+    py::class_<TestRemovable, SelfAdjointOperator, std::shared_ptr<TestRemovable>>(m, "TestRemovable")
+        .def(py::init<std::string_view>()).def(py::init<size_t>());
+    py::class_<AddRegister>(m, "AddRegister")
+        .def(py::init<std::string_view, StateStorageType, size_t>())
+        .def("__call__", [](const AddRegister& a, SparseState& s) { return a(s); }, Release());
+    py::class_<AddRegisterWithHadamard>(m, "AddRegisterWithHadamard")
+        .def(py::init<std::string_view, StateStorageType, size_t>())
+        .def("__call__", [](const AddRegisterWithHadamard& a, SparseState& s) { return a(s); }, Release());
+    py::class_<RemoveRegister>(m, "RemoveRegister")
+        .def(py::init<std::string_view>()).def(py::init<size_t>())
+        .def("__call__", [](const RemoveRegister& a, SparseState& s) { a(s); }, Release());
+    py::class_<SplitRegister>(m, "SplitRegister")
+        .def(py::init<std::string_view, std::string_view, size_t>()).def(py::init<size_t, std::string_view, size_t>())
+        .def("__call__", [](const SplitRegister& a, SparseState& s) { return a(s); }, Release());
+    py::class_<CombineRegister>(m, "CombineRegister")
+        .def(py::init<std::string_view, std::string_view>()).def(py::init<size_t, size_t>())
+        .def("__call__", [](const CombineRegister& a, SparseState& s) { return a(s); }, Release());
+    m.def("add_systems", [](SparseState& cur, SparseState& add, double coef) { add_systems(cur, add, coef); }, Release());
+
+    m.def("device_name", []() {
+        char buf[256] = {0};
+        check(sparq_device_name(buf, sizeof(buf)));
+        return std::string(buf);
+    });
+}

@@ -1,0 +1,863 @@
+// Host layer implementation: register table, layout, SparseState and the
+// operator classes.  Semantics cite the reference files they follow.
+#include "sparq.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+
+namespace sparqsim {
+
+void check(int status) {
+    if (status == SPARQ_OK) return;
+    std::string msg = sparq_last_error();
+    if (status == SPARQ_INVALID) throw std::invalid_argument(msg);
+    if (status == SPARQ_OOM) throw std::runtime_error("out of device memory: " + msg);
+    throw std::runtime_error(msg);
+}
+
+double& clearzero_default_eps() {   // system_operations.h:577
+    static double eps = epsilon;
+    return eps;
+}
+
+// ------------------------------------------------------------ register table
+std::vector<StateInfoType> System::name_register_map;
+std::unordered_map<std::string, size_t> System::name_to_index;
+std::vector<size_t> System::temporal_registers;
+std::vector<size_t> System::reusable_registers;
+size_t System::max_qubit_count = 0, System::max_register_count = 0, System::max_system_size = 0;
+uint64_t System::layout_version = 0;
+
+static const StateInfoType& info(size_t id) {
+    if (id >= System::name_register_map.size()) throw std::runtime_error("Register not found.");
+    return System::name_register_map[id];
+}
+
+void System::clear() {   // basic_components.cpp:150-161
+    name_register_map.clear();
+    name_to_index.clear();
+    temporal_registers.clear();
+    reusable_registers.clear();
+    max_qubit_count = max_register_count = max_system_size = 0;
+    ++layout_version;
+}
+size_t System::get_qubit_count() {
+    size_t c = 0;
+    for (auto& m : name_register_map) if (std::get<3>(m)) c += std::get<2>(m);
+    return c;
+}
+size_t System::get_activated_register_size() {
+    size_t c = 0;
+    for (auto& m : name_register_map) if (std::get<3>(m)) ++c;
+    return c;
+}
+size_t System::get_last_activated_register() {   // basic_components.cpp:185-195
+    if (name_register_map.empty()) throw std::runtime_error("No register is activated.");
+    for (size_t i = name_register_map.size(); i-- > 0;) if (status_of(i)) return i;
+    return SIZE_MAX;
+}
+void System::update_max_size(size_t n) { if (n > max_system_size) max_system_size = n; }
+size_t System::get(std::string_view name) {   // basic_components.cpp:206-228
+    auto it = name_to_index.find(std::string(name));
+    if (it != name_to_index.end() && it->second < name_register_map.size() && status_of(it->second)) return it->second;
+    for (size_t i = name_register_map.size(); i-- > 0;) {
+        if (!status_of(i)) continue;
+        if (std::get<0>(name_register_map[i]) == name) {
+            name_to_index[std::string(name)] = i;
+            return i;
+        }
+    }
+    return SIZE_MAX;
+}
+StateInfoType System::get_register_info(std::string_view name) {
+    for (auto it = name_register_map.rbegin(); it != name_register_map.rend(); ++it)
+        if (std::get<0>(*it) == name) return *it;
+    throw std::runtime_error("Register not found.");
+}
+const std::string& System::name_of(size_t id) { return std::get<0>(info(id)); }
+size_t System::size_of(std::string_view name) { return std::get<2>(get_register_info(name)); }
+size_t System::size_of(size_t id) { return std::get<2>(info(id)); }
+StateStorageType System::type_of(std::string_view name) { return std::get<1>(get_register_info(name)); }
+StateStorageType System::type_of(size_t id) { return std::get<1>(info(id)); }
+bool System::status_of(std::string_view name) { return status_of(get(name)); }
+bool System::status_of(size_t id) { return std::get<3>(info(id)); }
+size_t System::add_register(std::string_view name, StateStorageType type, size_t size) {   // :344-381
+    size_t id;
+    if (!reusable_registers.empty()) {
+        id = reusable_registers.back();
+        reusable_registers.pop_back();
+        name_register_map[id] = StateInfoType{std::string(name), type, size, true};
+    } else {
+        name_register_map.emplace_back(std::string(name), type, size, true);
+        id = name_register_map.size() - 1;
+    }
+    name_to_index[std::string(name)] = id;
+    max_qubit_count = std::max(max_qubit_count, get_qubit_count());
+    max_register_count = std::max(max_register_count, get_activated_register_size());
+    ++layout_version;
+    return id;
+}
+void System::remove_register(size_t id) {   // :405-415
+    if (id >= name_register_map.size()) throw std::runtime_error("Register not found.");
+    if (!std::get<3>(name_register_map[id])) throw std::runtime_error("Register is not activated.");
+    name_to_index.erase(std::get<0>(name_register_map[id]));
+    std::get<3>(name_register_map[id]) = false;
+    reusable_registers.push_back(id);
+    ++layout_version;
+}
+void System::remove_register(std::string_view name) { remove_register(get(name)); }
+void System::set_size(size_t id, size_t size) {
+    if (id >= name_register_map.size()) throw std::runtime_error("Register not found.");
+    std::get<2>(name_register_map[id]) = size;
+    ++layout_version;
+}
+void System::set_type(size_t id, StateStorageType t) {
+    if (id >= name_register_map.size()) throw std::runtime_error("Register not found.");
+    std::get<1>(name_register_map[id]) = t;
+}
+
+// ------------------------------------------------------------ layout
+Layout Layout::from_registry() {
+    Layout L;
+    L.field.assign(System::name_register_map.size(), sparq_field_t{0, 0, 0, 0});
+    int word = 0, room = 64;
+    for (size_t id = 0; id < System::name_register_map.size(); ++id) {
+        const auto& m = System::name_register_map[id];
+        if (!std::get<3>(m)) continue;
+        size_t w = std::get<2>(m);
+        if (w == 0) continue;
+        if (w > 64) throw std::invalid_argument("register wider than 64 bits");
+        if ((int)w > room) {
+            ++word;
+            room = 64;
+        }
+        room -= (int)w;
+        L.field[id] = sparq_field_t{(uint8_t)word, (uint8_t)room, (uint8_t)w,
+                                    (uint8_t)(std::get<1>(m) == SignedInteger ? 1 : 0)};
+    }
+    L.words = word + 1;
+    if (L.words > SPARQ_MAX_WORDS) throw std::invalid_argument("active registers exceed 512 key bits");
+    return L;
+}
+bool Layout::operator==(const Layout& o) const {
+    if (words != o.words || field.size() != o.field.size()) return false;
+    for (size_t i = 0; i < field.size(); ++i)
+        if (std::memcmp(&field[i], &o.field[i], sizeof(sparq_field_t)) != 0) return false;
+    return true;
+}
+sparq_field_t Layout::at(size_t id) const {
+    if (id >= field.size() || field[id].width == 0) throw std::runtime_error("Register not found.");
+    return field[id];
+}
+
+// ------------------------------------------------------------ state
+SparseState::SparseState() {
+    layout_ = Layout::from_registry();
+    check(sparq_state_create(layout_.words, 1, &h_));
+    check(sparq_state_set_basis(h_, nullptr));
+}
+SparseState::SparseState(const SparseState& other) {
+    SparseState& o = const_cast<SparseState&>(other);
+    o.flush();
+    layout_ = o.layout_;
+    check(sparq_state_clone(o.h_, &h_));
+}
+SparseState& SparseState::operator=(const SparseState& other) {
+    if (this == &other) return *this;
+    SparseState tmp(other);
+    std::swap(h_, tmp.h_);
+    std::swap(layout_, tmp.layout_);
+    pend_ops_.clear();
+    pend_conds_.clear();
+    return *this;
+}
+SparseState::SparseState(SparseState&& other) noexcept
+    : h_(other.h_), layout_(std::move(other.layout_)), pend_ops_(std::move(other.pend_ops_)), pend_conds_(std::move(other.pend_conds_)) {
+    other.h_ = nullptr;
+}
+SparseState& SparseState::operator=(SparseState&& other) noexcept {
+    std::swap(h_, other.h_);
+    std::swap(layout_, other.layout_);
+    std::swap(pend_ops_, other.pend_ops_);
+    std::swap(pend_conds_, other.pend_conds_);
+    return *this;
+}
+SparseState::~SparseState() {
+    if (h_) sparq_state_destroy(h_);
+}
+
+size_t SparseState::size() {
+    flush();
+    uint64_t n = 0;
+    check(sparq_state_size(h_, &n));
+    return n;
+}
+
+void SparseState::flush() {
+    if (pend_ops_.empty()) return;
+    int rc = sparq_pointwise(h_, pend_ops_.data(), (uint32_t)pend_ops_.size(), pend_conds_.data(), (uint32_t)pend_conds_.size());
+    pend_ops_.clear();
+    pend_conds_.clear();
+    check(rc);
+}
+
+const Layout& SparseState::sync() {
+    Layout T = Layout::from_registry();
+    if (T == layout_) return layout_;
+    flush();
+    std::vector<sparq_field_t> from, to;
+    bool identity = T.words == layout_.words;
+    for (size_t id = 0; id < layout_.field.size(); ++id) {
+        sparq_field_t a = layout_.field[id];
+        if (a.width == 0) continue;
+        if (id < T.field.size() && T.field[id].width == a.width) {
+            sparq_field_t b = T.field[id];
+            from.push_back(a);
+            to.push_back(b);
+            if (a.word != b.word || a.shift != b.shift) identity = false;
+        } else {
+            identity = false;   // a dropped field must be cleared
+        }
+    }
+    if (!identity) check(sparq_state_relayout(h_, T.words, from.data(), to.data(), (uint32_t)from.size()));
+    layout_ = T;
+    return layout_;
+}
+
+void SparseState::relayout_with(const Layout& target, const std::vector<std::pair<sparq_field_t, sparq_field_t>>& moves) {
+    flush();
+    std::vector<sparq_field_t> from, to;
+    for (auto& m : moves) {
+        from.push_back(m.first);
+        to.push_back(m.second);
+    }
+    check(sparq_state_relayout(h_, target.words, from.data(), to.data(), (uint32_t)from.size()));
+    layout_ = target;
+}
+
+static bool same_conds(const sparq_cond_t* a, const sparq_cond_t* b, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (a[i].kind != b[i].kind || std::memcmp(&a[i].field, &b[i].field, sizeof(sparq_field_t)) != 0 || a[i].value != b[i].value)
+            return false;
+    return true;
+}
+
+void SparseState::push_pointwise(const sparq_pw_op_t& op_in, const std::vector<sparq_cond_t>& conds) {
+    if (conds.size() > SPARQ_MAX_PROGRAM_CONDS) throw std::invalid_argument("too many conditions on one operator");
+    // Consecutive phase ops on the same register with equal phase and conditions merge.
+    if (op_in.opcode == SPARQ_PW_PHASE_BITS && !pend_ops_.empty()) {
+        sparq_pw_op_t& last = pend_ops_.back();
+        if (last.opcode == SPARQ_PW_PHASE_BITS && std::memcmp(&last.f[0], &op_in.f[0], sizeof(sparq_field_t)) == 0 &&
+            last.d[0] == op_in.d[0] && last.d[1] == op_in.d[1] && last.ncond == conds.size() &&
+            (conds.empty() || same_conds(&pend_conds_[last.cond_first], conds.data(), conds.size())) && (last.c[0] & op_in.c[0]) == 0) {
+            last.c[0] |= op_in.c[0];
+            return;
+        }
+    }
+    if (pend_ops_.size() >= SPARQ_MAX_PROGRAM_OPS || pend_conds_.size() + conds.size() > SPARQ_MAX_PROGRAM_CONDS) flush();
+    sparq_pw_op_t op = op_in;
+    op.cond_first = (uint16_t)pend_conds_.size();
+    op.ncond = (uint16_t)conds.size();
+    pend_conds_.insert(pend_conds_.end(), conds.begin(), conds.end());
+    pend_ops_.push_back(op);
+}
+
+static uint64_t field_value(const std::vector<uint64_t>& key, sparq_field_t f) {
+    return (key[f.word] >> f.shift) & width_mask(f.width);
+}
+
+void SparseState::export_entries(std::vector<size_t>& ids, std::vector<uint64_t>& values, std::vector<complex_t>& amps) {
+    sync();
+    flush();
+    uint64_t n = 0;
+    check(sparq_state_size(h_, &n));
+    std::vector<uint64_t> keys((size_t)n * layout_.words);
+    std::vector<double> a((size_t)n * 2);
+    check(sparq_state_download(h_, keys.data(), a.data(), n, &n));
+    ids.clear();
+    for (size_t id = 0; id < layout_.field.size(); ++id) if (layout_.field[id].width) ids.push_back(id);
+    values.assign((size_t)n * ids.size(), 0);
+    amps.resize(n);
+    std::vector<uint64_t> k(layout_.words);
+    for (uint64_t i = 0; i < n; ++i) {
+        for (uint32_t w = 0; w < layout_.words; ++w) k[w] = keys[i * layout_.words + w];
+        for (size_t j = 0; j < ids.size(); ++j) values[i * ids.size() + j] = field_value(k, layout_.field[ids[j]]);
+        amps[i] = complex_t(a[2 * i], a[2 * i + 1]);
+    }
+}
+
+void SparseState::import_entries(const std::vector<uint64_t>& values, const std::vector<complex_t>& amps) {
+    sync();
+    flush();
+    std::vector<size_t> ids;
+    for (size_t id = 0; id < layout_.field.size(); ++id) if (layout_.field[id].width) ids.push_back(id);
+    size_t n = amps.size();
+    if (values.size() != n * ids.size()) throw std::invalid_argument("value array does not match the active registers");
+    std::vector<uint64_t> keys(n * layout_.words, 0);
+    std::vector<double> a(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t j = 0; j < ids.size(); ++j) {
+            sparq_field_t f = layout_.field[ids[j]];
+            uint64_t v = values[i * ids.size() + j];
+            if (v & ~width_mask(f.width)) throw std::invalid_argument("register value exceeds its width");
+            keys[i * layout_.words + f.word] |= v << f.shift;
+        }
+        a[2 * i] = amps[i].real();
+        a[2 * i + 1] = amps[i].imag();
+    }
+    check(sparq_state_upload(h_, keys.data(), a.data(), n));
+}
+
+std::string SparseState::to_string(int precision) {
+    std::vector<size_t> ids;
+    std::vector<uint64_t> vals;
+    std::vector<complex_t> amps;
+    export_entries(ids, vals, amps);
+    std::ostringstream os;
+    os.precision(precision);
+    for (size_t i = 0; i < amps.size(); ++i) {
+        os << amps[i].real() << (amps[i].imag() < 0 ? " - " : " + ") << std::abs(amps[i].imag()) << "i";
+        for (size_t j = 0; j < ids.size(); ++j) os << " |" << System::name_of(ids[j]) << "=" << vals[i * ids.size() + j] << ">";
+        os << "\n";
+    }
+    return os.str();
+}
+
+// ------------------------------------------------------------ conditions
+std::vector<sparq_cond_t> Controllable::conditions(const Layout& L) const {   // global_macros.h:28-92
+    std::vector<sparq_cond_t> out;
+    for (size_t id : condition_variable_all_ones) out.push_back(sparq_cond_t{SPARQ_COND_ALL_ONES, L.at(id), 0});
+    for (auto& p : condition_variable_by_bit) {
+        if (p.second >= 64) throw std::invalid_argument("condition bit out of range");
+        out.push_back(sparq_cond_t{SPARQ_COND_BIT, L.at(p.first), p.second});
+    }
+    for (size_t id : condition_variable_nonzeros) out.push_back(sparq_cond_t{SPARQ_COND_NONZERO, L.at(id), 0});
+    for (auto& p : condition_variable_by_value) out.push_back(sparq_cond_t{SPARQ_COND_VALUE, L.at(p.first), p.second});
+    return out;
+}
+
+// ------------------------------------------------------------ pointwise operators
+void PointwiseOperator::emit(SparseState& state, bool dagger) const {
+    const Layout& L = state.sync();
+    sparq_pw_op_t op{};
+    op.opcode = dagger && dag_opcode ? dag_opcode : opcode;
+    for (size_t i = 0; i < regs.size() && i < 4; ++i) op.f[i] = L.at(regs[i]);
+    const uint64_t* cc = dagger && dag_opcode ? dag_c : c;
+    const double* dd = dagger && dag_opcode ? dag_d : d;
+    for (int i = 0; i < 3; ++i) op.c[i] = cc[i];
+    for (int i = 0; i < 8; ++i) op.d[i] = dd[i];
+    op.table = table;
+    op.table_word_bytes = table_word_bytes;
+    std::vector<sparq_cond_t> conds = conditions(L);
+    for (auto& z : extra_zero_regs) conds.push_back(sparq_cond_t{SPARQ_COND_VALUE, L.at(z.first), z.second});
+    state.push_pointwise(op, conds);
+}
+void PointwiseOperator::dag(SparseState& state) const {
+    if (!has_dag) throw std::runtime_error("Dagger is not implemented.");
+    emit(state, true);
+}
+
+static void set_op(PointwiseOperator& p, const char* name, uint32_t opcode, std::vector<size_t> regs, uint32_t dag_opcode = 0) {
+    p.name = name;
+    p.opcode = opcode;
+    p.dag_opcode = dag_opcode;
+    p.regs = std::move(regs);
+}
+static void require_ids(const std::vector<size_t>& ids) {
+    for (size_t id : ids)
+        if (id >= System::name_register_map.size() || !System::status_of(id)) throw std::runtime_error("Register not found.");
+}
+
+#define SPQ_DEF3(NAME, OPC)                                   \
+    NAME::NAME(size_t a, size_t b, size_t c) {                \
+        require_ids({a, b, c});                               \
+        set_op(*this, #NAME, OPC, {a, b, c});                 \
+    }
+#define SPQ_DEF2(NAME, OPC)                                   \
+    NAME::NAME(size_t a, size_t b) {                          \
+        require_ids({a, b});                                  \
+        set_op(*this, #NAME, OPC, {a, b});                    \
+    }
+#define SPQ_DEF4(NAME, OPC)                                   \
+    NAME::NAME(size_t a, size_t b, size_t c, size_t d) {      \
+        require_ids({a, b, c, d});                            \
+        set_op(*this, #NAME, OPC, {a, b, c, d});              \
+    }
+
+// quantum_arithmetic.cpp line ranges are in include/sparq_hip.h next to each opcode.
+SPQ_DEF3(Add_UInt_UInt, SPARQ_PW_XOR_ADD)
+SPQ_DEF3(Sub_UInt_UInt, SPARQ_PW_XOR_SUB)
+SPQ_DEF3(Mul_UInt_UInt, SPARQ_PW_XOR_MUL)
+SPQ_DEF3(Div_UInt_UInt, SPARQ_PW_XOR_DIV)
+SPQ_DEF3(And_UInt_UInt, SPARQ_PW_XOR_AND)
+SPQ_DEF3(Or_UInt_UInt, SPARQ_PW_XOR_OR)
+SPQ_DEF3(Xor_UInt_UInt, SPARQ_PW_XOR_XOR)
+SPQ_DEF3(Less_SInt_SInt, SPARQ_PW_FLAG_LESS_S)
+SPQ_DEF3(Less_UInt_UInt, SPARQ_PW_FLAG_LESS_U)
+SPQ_DEF3(GetMid_UInt_UInt, SPARQ_PW_XOR_MID)
+SPQ_DEF3(Div_Sqrt_Arccos_UInt_UInt, SPARQ_PW_XOR_DIVSQRT_ACOS)
+SPQ_DEF3(Sqrt_Div_Arccos_Int_UInt, SPARQ_PW_XOR_SQRTDIV_ACOS)
+SPQ_DEF3(GetRotateAngle_Int_Int, SPARQ_PW_XOR_ROTATE_ANGLE)
+SPQ_DEF2(Neg_UInt, SPARQ_PW_XOR_NEG)
+SPQ_DEF2(Abs_SInt, SPARQ_PW_XOR_ABS)
+SPQ_DEF2(Sqrt_UInt, SPARQ_PW_XOR_ISQRT)
+SPQ_DEF2(IsZero_UInt, SPARQ_PW_FLAG_ISZERO)
+SPQ_DEF2(Negative_SInt, SPARQ_PW_FLAG_NEGATIVE)
+SPQ_DEF4(Select_Bool_UInt_UInt, SPARQ_PW_XOR_SELECT)
+SPQ_DEF4(Carry_UInt_UInt, SPARQ_PW_FLAG_CARRY)
+SPQ_DEF4(Overflow_SInt_SInt, SPARQ_PW_FLAG_OVERFLOW_S)
+SPQ_DEF4(MulOverflow_UInt_UInt, SPARQ_PW_FLAG_MULOVERFLOW)
+
+Add_UInt_UInt_InPlace::Add_UInt_UInt_InPlace(size_t a, size_t b) {   // quantum_arithmetic.cpp:308-349
+    require_ids({a, b});
+    set_op(*this, "Add_UInt_UInt_InPlace", SPARQ_PW_ADD_FIELD, {a, b}, SPARQ_PW_SUB_FIELD);
+}
+Add_AnyInt_AnyInt_InPlace::Add_AnyInt_AnyInt_InPlace(size_t a, size_t b) {   // :869-907
+    require_ids({a, b});
+    set_op(*this, "Add_AnyInt_AnyInt_InPlace", SPARQ_PW_ADD_ANY, {a, b}, SPARQ_PW_SUB_ANY);
+}
+Assign::Assign(size_t a, size_t b) {   // :909-930 (quantum_arithmetic.h:2777-2783 rejects a == b)
+    if (a == b) throw std::invalid_argument("Assign: registers must differ");
+    require_ids({a, b});
+    set_op(*this, "Assign", SPARQ_PW_XOR_ASSIGN, {a, b});
+}
+Swap_General_General::Swap_General_General(size_t a, size_t b) {   // :982-999
+    require_ids({a, b});
+    if (System::size_of(a) != System::size_of(b))
+        throw std::invalid_argument("Swap_General_General: registers of different widths");
+    set_op(*this, "Swap_General_General", SPARQ_PW_SWAP_FIELDS, {a, b});
+}
+Compare_UInt_UInt::Compare_UInt_UInt(size_t l, size_t r, size_t lt, size_t eq) {   // :932-959
+    if (lt == eq || lt == l || lt == r || eq == l || eq == r) throw std::invalid_argument("Compare_UInt_UInt: bad registers");
+    require_ids({l, r, lt, eq});
+    set_op(*this, "Compare_UInt_UInt", SPARQ_PW_COMPARE, {l, r, lt, eq});
+}
+Mult_UInt_ConstUInt::Mult_UInt_ConstUInt(size_t a, uint64_t k, size_t b) {   // :137-155
+    require_ids({a, b});
+    set_op(*this, "Mult_UInt_ConstUInt", SPARQ_PW_XOR_MUL_CONST, {a, b});
+    c[0] = k;
+}
+Add_Mult_UInt_ConstUInt_InPlace::Add_Mult_UInt_ConstUInt_InPlace(size_t a, uint64_t k, size_t b) {   // :157-195
+    require_ids({a, b});
+    set_op(*this, "Add_Mult_UInt_ConstUInt_InPlace", SPARQ_PW_ADD_MUL_CONST, {a, b}, SPARQ_PW_ADD_MUL_CONST);
+    c[0] = k;
+    dag_c[0] = uint64_t(0) - k;   // (r - l*k) mod 2^64 == (r + l*(-k)) mod 2^64
+}
+Add_UInt_ConstUInt::Add_UInt_ConstUInt(size_t a, uint64_t k, size_t b) {   // :351-369
+    require_ids({a, b});
+    set_op(*this, "Add_UInt_ConstUInt", SPARQ_PW_XOR_ADD_CONST, {a, b});
+    c[0] = k;
+}
+Add_ConstUInt_InPlace::Add_ConstUInt_InPlace(size_t a, uint64_t k) {   // :371-411
+    require_ids({a});
+    set_op(*this, "Add_ConstUInt_InPlace", SPARQ_PW_ADD_CONST, {a}, SPARQ_PW_ADD_CONST);
+    c[0] = k;
+    dag_c[0] = uint64_t(0) - k;
+}
+ShiftLeft_InPlace::ShiftLeft_InPlace(size_t a, uint64_t k) {   // :63-135
+    require_ids({a});
+    if (k > System::size_of(a)) throw std::invalid_argument("shift larger than the register");
+    set_op(*this, "ShiftLeft_InPlace", SPARQ_PW_ROTL, {a}, SPARQ_PW_ROTR);
+    c[0] = dag_c[0] = k;
+}
+ShiftRight_InPlace::ShiftRight_InPlace(size_t a, uint64_t k) {
+    require_ids({a});
+    if (k > System::size_of(a)) throw std::invalid_argument("shift larger than the register");
+    set_op(*this, "ShiftRight_InPlace", SPARQ_PW_ROTR, {a}, SPARQ_PW_ROTL);
+    c[0] = dag_c[0] = k;
+}
+X_Bool::X_Bool(size_t a, uint64_t digit) {   // basic_gates.cpp:383-400; GateBase digit check basic_gates.h:20-40
+    require_ids({a});
+    if (digit >= System::size_of(a)) throw std::invalid_argument("digit out of range");
+    set_op(*this, "X_Bool", SPARQ_PW_FLIP_BIT, {a});
+    c[0] = digit;
+}
+Y_Bool::Y_Bool(size_t a, uint64_t digit) {   // basic_gates.cpp:402-420
+    require_ids({a});
+    if (digit >= System::size_of(a)) throw std::invalid_argument("digit out of range");
+    set_op(*this, "Y_Bool", SPARQ_PW_Y, {a});
+    c[0] = digit;
+}
+Init_Unsafe::Init_Unsafe(size_t a, uint64_t value) {   // dark_magic.cpp:35-41
+    require_ids({a});
+    if (value & ~width_mask(System::size_of(a))) throw std::invalid_argument("Init_Unsafe: value exceeds the register width");
+    set_op(*this, "Init_Unsafe", SPARQ_PW_SET_FIELD, {a});
+    c[0] = value;
+}
+FlipBools::FlipBools(size_t a) {   // quantum_arithmetic.cpp:11-30
+    require_ids({a});
+    set_op(*this, "FlipBools", SPARQ_PW_FLIP_ALL, {a});
+}
+Mod_Mult_UInt_ConstUInt_InPlace::Mod_Mult_UInt_ConstUInt_InPlace(size_t r, uint64_t a, uint64_t x, uint64_t N) {   // :197-285
+    require_ids({r});
+    if (N == 0 || std::gcd(a, N) > 1) throw std::invalid_argument("a and N must be coprime");
+    uint64_t opnum = a % N;
+    for (uint64_t i = 0; i < x; ++i) opnum = (opnum * opnum) % N;
+    int64_t t = 0, nt = 1, rr = (int64_t)N, nr = (int64_t)opnum;
+    while (nr != 0) {
+        int64_t q = rr / nr, tt = t - q * nt, tr = rr - q * nr;
+        t = nt; rr = nr; nt = tt; nr = tr;
+    }
+    uint64_t inv = 1;
+    if (rr == 1) {
+        int64_t tn = t % (int64_t)N;
+        if (tn < 0) tn += (int64_t)N;
+        inv = (uint64_t)tn;
+    }
+    set_op(*this, "Mod_Mult_UInt_ConstUInt_InPlace", SPARQ_PW_MODMUL_CONST, {r}, SPARQ_PW_MODMUL_CONST);
+    c[0] = opnum; c[1] = N;
+    dag_c[0] = inv; dag_c[1] = N;
+}
+Swap_Bool_Bool::Swap_Bool_Bool(size_t a, size_t d1, size_t b, size_t d2) {   // :32-61
+    require_ids({a, b});
+    if (d1 >= System::size_of(a) || d2 >= System::size_of(b)) throw std::invalid_argument("digit out of range");
+    set_op(*this, "Swap_Bool_Bool", SPARQ_PW_SWAP_BITS, {a, b});
+    c[0] = d1; c[1] = d2;
+}
+Phase_Bool::Phase_Bool(size_t r, size_t digit, double lambda) {   // basic_gates.cpp:11-55
+    require_ids({r});
+    if (digit >= System::size_of(r)) throw std::invalid_argument("digit out of range");
+    set_op(*this, "Phase_Bool", SPARQ_PW_PHASE_BITS, {r}, SPARQ_PW_PHASE_BITS);
+    c[0] = dag_c[0] = uint64_t(1) << digit;
+    d[0] = std::cos(lambda); d[1] = std::sin(lambda);
+    dag_d[0] = std::cos(lambda); dag_d[1] = -std::sin(lambda);
+}
+RZ_Bool::RZ_Bool(size_t r, size_t digit, double angle) {   // basic_gates.cpp:486-520
+    require_ids({r});
+    if (digit >= System::size_of(r)) throw std::invalid_argument("digit out of range");
+    set_op(*this, "RZ_Bool", SPARQ_PW_RZ, {r}, SPARQ_PW_RZ);
+    c[0] = dag_c[0] = digit;
+    d[0] = std::cos(angle / 2); d[1] = std::sin(angle / 2); d[2] = std::cos(angle / 2); d[3] = -std::sin(angle / 2);
+    double na = -angle;
+    dag_d[0] = std::cos(na / 2); dag_d[1] = std::sin(na / 2); dag_d[2] = std::cos(na / 2); dag_d[3] = -std::sin(na / 2);
+}
+
+QRAMCircuit_qutrit::QRAMCircuit_qutrit(size_t addr, size_t data, std::vector<uint64_t> mem)
+    : address_size(addr), data_size(data), memory(std::move(mem)) {
+    if (addr > 40) throw std::invalid_argument("address size too large");
+    if (memory.size() < (size_t(1) << addr)) throw std::invalid_argument("memory smaller than 2^address_size");
+    for (uint64_t v : memory) max_word = std::max(max_word, v);
+    word_bytes = max_word < (1ull << 8) ? 1 : max_word < (1ull << 16) ? 2 : max_word < (1ull << 32) ? 4 : 8;
+    std::vector<uint8_t> packed((size_t(1) << addr) * word_bytes);
+    for (size_t i = 0; i < (size_t(1) << addr); ++i) {
+        uint64_t v = memory[i];
+        std::memcpy(packed.data() + i * word_bytes, &v, word_bytes);   // little endian
+    }
+    sparq_table_t t = nullptr;
+    check(sparq_table_create(packed.data(), size_t(1) << addr, word_bytes, &t));
+    table = std::shared_ptr<sparq_table_s>(t, [](sparq_table_s* p) { sparq_table_destroy(p); });
+}
+
+QRAMLoad::QRAMLoad(const QRAMCircuit_qutrit* q, size_t a, size_t dreg) {   // qram.cpp:15-35, qram.h:49-71
+    if (q == nullptr || a == dreg) throw std::invalid_argument("QRAMLoad: bad arguments");
+    require_ids({a, dreg});
+    if (q->max_word & ~width_mask(System::size_of(dreg)))
+        throw std::invalid_argument("QRAMLoad: memory word wider than the data register");
+    set_op(*this, "QRAMLoad", SPARQ_PW_QRAM_LOAD, {a, dreg});
+    c[0] = q->address_size;
+    const void* ptr = nullptr;
+    check(sparq_table_device_ptr(q->table.get(), &ptr));
+    table = ptr;
+    table_word_bytes = q->word_bytes;
+    table_owner = q->table;
+}
+
+ZeroConditionalPhaseFlip::ZeroConditionalPhaseFlip(const std::vector<size_t>& ids) {   // parallel_phase_operations.cpp:11-31
+    require_ids(ids);
+    set_op(*this, "ZeroConditionalPhaseFlip", SPARQ_PW_NEG_AMP, {});
+    for (size_t id : ids) extra_zero_regs.emplace_back(id, 0);
+}
+ZeroConditionalPhaseFlip::ZeroConditionalPhaseFlip(const std::vector<std::string>& names) {
+    std::vector<size_t> ids;
+    for (auto& n : names) ids.push_back(System::get(n));
+    *this = ZeroConditionalPhaseFlip(ids);
+}
+Reflection_Bool::Reflection_Bool(const std::vector<size_t>& ids, bool inv) : inverse(inv) {   // :46-70
+    require_ids(ids);
+    set_op(*this, "Reflection_Bool", SPARQ_PW_NEG_AMP, {});
+    for (size_t id : ids) extra_zero_regs.emplace_back(id, 0);
+}
+Reflection_Bool::Reflection_Bool(const std::vector<std::string>& names, bool inv) : inverse(inv) {
+    std::vector<size_t> ids;
+    for (auto& n : names) ids.push_back(System::get(n));
+    require_ids(ids);
+    set_op(*this, "Reflection_Bool", SPARQ_PW_NEG_AMP, {});
+    for (size_t id : ids) extra_zero_regs.emplace_back(id, 0);
+}
+void Reflection_Bool::operator()(SparseState& state) const {
+    // inverse: flip unless all zero == flip everything, then flip back the all-zero ones
+    // (two exact sign changes; bit-identical to the single conditional flip).
+    if (inverse) {
+        const Layout& L = state.sync();
+        sparq_pw_op_t op{};
+        op.opcode = SPARQ_PW_NEG_AMP;
+        state.push_pointwise(op, conditions(L));
+    }
+    emit(state, false);
+}
+GlobalPhase::GlobalPhase(complex_t cc) {   // parallel_phase_operations.cpp:72-108
+    set_op(*this, "GlobalPhase", SPARQ_PW_MUL_AMP, {}, SPARQ_PW_MUL_AMP);
+    d[0] = cc.real(); d[1] = cc.imag();
+    complex_t cj = std::conj(cc);
+    dag_d[0] = cj.real(); dag_d[1] = cj.imag();
+}
+
+// ------------------------------------------------------------ branching operators
+static std::vector<sparq_cond_t> conds_of(const Controllable* c, const Layout& L) { return c ? c->conditions(L) : std::vector<sparq_cond_t>{}; }
+
+void Hadamard_Int::operator()(SparseState& state) const {   // hadamard.cpp:134-162
+    const Layout& L = state.sync();
+    state.flush();
+    auto conds = conds_of(this, L);
+    check(sparq_hadamard_int(state.handle(), L.at(id), (uint32_t)n_digits, conds.data(), (uint32_t)conds.size(), clearzero_default_eps()));
+    System::update_max_size(state.size());
+}
+void Hadamard_Int_Full::operator()(SparseState& state) const {   // hadamard.cpp:181-267
+    const Layout& L = state.sync();
+    state.flush();
+    auto conds = conds_of(this, L);
+    check(sparq_hadamard_int_full(state.handle(), L.at(id), conds.data(), (uint32_t)conds.size(), clearzero_default_eps()));
+    System::update_max_size(state.size());
+}
+Hadamard_Bool::Hadamard_Bool(size_t reg) : out_id(reg) {   // hadamard.h:160-175
+    if (System::size_of(out_id) != 1) throw std::invalid_argument("Hadamard_Bool: size of output register must be 1");
+}
+void Hadamard_Bool::operator()(SparseState& state) const {   // hadamard.cpp:366-439
+    const Layout& L = state.sync();
+    state.flush();
+    auto conds = conds_of(this, L);
+    sparq_pair_params_t p{};
+    p.kind = SPARQ_PAIR_HADAMARD_BOOL;
+    check(sparq_pair_op(state.handle(), L.at(out_id), 0, &p, conds.data(), (uint32_t)conds.size(), clearzero_default_eps()));
+    System::update_max_size(state.size());
+}
+CondRot_Rational_Bool::CondRot_Rational_Bool(size_t in, size_t out) : register_in(in), register_out(out) {}
+void CondRot_Rational_Bool::apply(SparseState& state, double sign) const {   // condrot.cpp:12-151
+    const Layout& L = state.sync();
+    state.flush();
+    sparq_pair_params_t p{};
+    p.kind = SPARQ_PAIR_CONDROT_RATIONAL;
+    p.in = L.at(register_in);
+    p.mat[0] = sign;
+    check(sparq_pair_op(state.handle(), L.at(register_out), 0, &p, nullptr, 0, clearzero_default_eps()));
+}
+
+u22_t make_func(uint64_t value, size_t n) {   // condrot.h:19-30
+    double th = (n == 64) ? value * 1.0 / 2 / 9223372036854775808.0 : value * 1.0 / double(uint64_t(1) << n);
+    th *= (2 * pi);
+    return u22_t{complex_t(std::cos(th)), complex_t(-std::sin(th)), complex_t(std::sin(th)), complex_t(std::cos(th))};
+}
+u22_t make_func_inv(uint64_t value, size_t n) {   // condrot.h:32-41
+    double th = (n == 64) ? value * 1.0 / 2 / 9223372036854775808.0 : value * 1.0 / double(uint64_t(1) << n);
+    th *= (2 * pi);
+    return u22_t{complex_t(std::cos(th)), complex_t(std::sin(th)), complex_t(-std::sin(th)), complex_t(std::cos(th))};
+}
+
+CondRot_General_Bool::CondRot_General_Bool(size_t in, size_t out, std::function<u22_t(uint64_t)> f) : in_id(in), out_id(out), func(std::move(f)) {
+    if (System::size_of(out_id) != 1) throw std::invalid_argument("CondRot_General_Bool: size of output register must be 1");
+}
+namespace {
+struct DeviceTable {
+    double* ptr = nullptr;
+    sparq_state_t carrier = nullptr;   // tables live in a helper state's amplitude array
+};
+}  // namespace
+void CondRot_General_Bool::operator()(SparseState& state) const {   // condrot.h:149-332
+    const Layout& L = state.sync();
+    state.flush();
+    size_t w = System::size_of(in_id);
+    if (w > 24) throw std::invalid_argument("CondRot_General_Bool: input register wider than 24 bits");
+    size_t nv = size_t(1) << w;
+    // Tabulate the 2x2 for every input value and upload it as a temporary device state.
+    std::vector<double> tab(nv * 8);
+    for (size_t v = 0; v < nv; ++v) {
+        u22_t m = func(v);
+        for (int k = 0; k < 4; ++k) { tab[v * 8 + 2 * k] = m[k].real(); tab[v * 8 + 2 * k + 1] = m[k].imag(); }
+    }
+    sparq_state_t carrier = nullptr;
+    check(sparq_state_create(1, nv * 4, &carrier));
+    std::vector<uint64_t> zeros(nv * 4, 0);
+    int rc = sparq_state_upload(carrier, zeros.data(), tab.data(), nv * 4);
+    if (rc != SPARQ_OK) { sparq_state_destroy(carrier); check(rc); }
+    double* dev = nullptr;
+    sparq_state_device_arrays(carrier, nullptr, &dev, nullptr);
+    sparq_pair_params_t p{};
+    p.kind = SPARQ_PAIR_CONDROT_TABLE;
+    p.in = L.at(in_id);
+    p.table = dev;
+    rc = sparq_pair_op(state.handle(), L.at(out_id), 0, &p, nullptr, 0, clearzero_default_eps());
+    sparq_state_sync(state.handle());
+    sparq_state_destroy(carrier);
+    check(rc);
+}
+
+Rot_Bool::Rot_Bool(size_t reg, size_t dg, u22_t m) : id(reg), digit(dg), mat(m) {   // basic_gates.h:157-200
+    if (digit >= System::size_of(id)) throw std::invalid_argument("digit out of range");
+}
+static double abs_sqr(complex_t c) { return c.real() * c.real() + c.imag() * c.imag(); }
+void Rot_Bool::operator()(SparseState& state) const {   // basic_gates.cpp:74-255
+    const Layout& L = state.sync();
+    auto conds = conditions(L);
+    sparq_pw_op_t op{};
+    op.f[0] = L.at(id);
+    op.c[0] = digit;
+    if (abs_sqr(mat[1]) < epsilon && abs_sqr(mat[2]) < epsilon) {          // diagonal, in place
+        op.opcode = SPARQ_PW_ROT_DIAG;
+        op.d[0] = mat[0].real(); op.d[1] = mat[0].imag(); op.d[2] = mat[3].real(); op.d[3] = mat[3].imag();
+        state.push_pointwise(op, conds);
+        return;
+    }
+    if (abs_sqr(mat[0]) < epsilon && abs_sqr(mat[3]) < epsilon) {          // off-diagonal, flip in place
+        op.opcode = SPARQ_PW_ROT_OFFDIAG;
+        op.d[0] = mat[2].real(); op.d[1] = mat[2].imag(); op.d[2] = mat[1].real(); op.d[3] = mat[1].imag();
+        state.push_pointwise(op, conds);
+        return;
+    }
+    state.flush();
+    sparq_pair_params_t p{};
+    p.kind = SPARQ_PAIR_ROT_GENERAL;
+    for (int k = 0; k < 4; ++k) { p.mat[2 * k] = mat[k].real(); p.mat[2 * k + 1] = mat[k].imag(); }
+    check(sparq_pair_op(state.handle(), L.at(id), (uint32_t)digit, &p, conds.data(), (uint32_t)conds.size(), clearzero_default_eps()));
+    System::update_max_size(state.size());
+}
+void Rot_Bool::dag(SparseState& state) const {   // basic_gates.cpp:310-315
+    u22_t d{std::conj(mat[0]), std::conj(mat[2]), std::conj(mat[1]), std::conj(mat[3])};
+    Rot_Bool inv(id, digit, d);
+    copy_control_conditions_to(inv);
+    inv(state);
+}
+// basic_gates.cpp:422-520
+static u22_t rx(double a) { return u22_t{complex_t(std::cos(a / 2), 0), complex_t(0, -std::sin(a / 2)), complex_t(0, -std::sin(a / 2)), complex_t(std::cos(a / 2), 0)}; }
+static u22_t ry(double a) { return u22_t{complex_t(std::cos(a / 2), 0), complex_t(-std::sin(a / 2), 0), complex_t(std::sin(a / 2), 0), complex_t(std::cos(a / 2), 0)}; }
+static u22_t sx() { return u22_t{complex_t(1, 1) / 2.0, complex_t(1, -1) / 2.0, complex_t(1, -1) / 2.0, complex_t(1, 1) / 2.0}; }
+static const double kSqrt2 = 1.4142135623730950488016887242096981;
+static u22_t u2(double phi, double lambda) {
+    return u22_t{complex_t(1, 0) / kSqrt2, complex_t(-std::cos(lambda), -std::sin(lambda)) / kSqrt2,
+                 complex_t(std::cos(phi), std::sin(phi)) / kSqrt2, complex_t(std::cos(lambda + phi), std::sin(lambda + phi)) / kSqrt2};
+}
+static u22_t u3(double t, double phi, double lambda) {
+    return u22_t{complex_t(std::cos(t / 2), 0), complex_t(-std::cos(lambda) * std::sin(t / 2), -std::sin(lambda) * std::sin(t / 2)),
+                 complex_t(std::cos(phi) * std::sin(t / 2), std::sin(phi) * std::sin(t / 2)),
+                 complex_t(std::cos(lambda + phi) * std::cos(t / 2), std::sin(lambda + phi) * std::cos(t / 2))};
+}
+RX_Bool::RX_Bool(std::string_view r, size_t dg, double a) : Rot_Bool(r, dg, rx(a)) {}
+RX_Bool::RX_Bool(size_t r, size_t dg, double a) : Rot_Bool(r, dg, rx(a)) {}
+RY_Bool::RY_Bool(std::string_view r, size_t dg, double a) : Rot_Bool(r, dg, ry(a)) {}
+RY_Bool::RY_Bool(size_t r, size_t dg, double a) : Rot_Bool(r, dg, ry(a)) {}
+SX_Bool::SX_Bool(std::string_view r, size_t dg) : Rot_Bool(r, dg, sx()) {}
+SX_Bool::SX_Bool(size_t r, size_t dg) : Rot_Bool(r, dg, sx()) {}
+U2_Bool::U2_Bool(std::string_view r, size_t dg, double phi, double lambda) : Rot_Bool(r, dg, u2(phi, lambda)) {}
+U2_Bool::U2_Bool(size_t r, size_t dg, double phi, double lambda) : Rot_Bool(r, dg, u2(phi, lambda)) {}
+U3_Bool::U3_Bool(std::string_view r, size_t dg, double t, double phi, double lambda) : Rot_Bool(r, dg, u3(t, phi, lambda)) {}
+U3_Bool::U3_Bool(size_t r, size_t dg, double t, double phi, double lambda) : Rot_Bool(r, dg, u3(t, phi, lambda)) {}
+
+// ------------------------------------------------------------ structural operators
+void ClearZero::operator()(SparseState& state) const {   // system_operations.cpp:371-382
+    state.flush();
+    check(sparq_clear_zero(state.handle(), eps));
+}
+void Normalize::operator()(SparseState& state) const {   // dark_magic.cpp:11-33
+    state.flush();
+    double n2 = 0;
+    check(sparq_normalize(state.handle(), &n2));
+}
+void SortUnconditional::operator()(SparseState& state) const {   // sort_state.cpp:84-88
+    state.sync();
+    state.flush();
+    check(sparq_sort(state.handle(), nullptr));
+}
+void CheckDuplicateKey::operator()(SparseState& state) const {   // debugger.cpp:214-222
+    state.sync();
+    state.flush();
+    int dup = 0;
+    check(sparq_check_duplicates(state.handle(), &dup));
+    if (dup) throw std::runtime_error("CheckDuplicateKey: duplicate key");
+}
+void TestRemovable::operator()(SparseState& state) const {   // debugger.cpp:191-212
+    const Layout& L = state.sync();
+    state.flush();
+    int uni = 1;
+    uint64_t v = 0;
+    check(sparq_field_uniform(state.handle(), L.at(register_id), &uni, &v));
+    if (!uni) throw std::runtime_error("Removing failed: register " + System::name_of(register_id) + " is not uniform");
+}
+
+AddRegister::AddRegister(std::string_view n, StateStorageType t, size_t s) : register_name(n), type(t), size(s) {
+    if (size > 64) throw std::invalid_argument("register wider than 64 bits");   // system_operations.cpp:235-245
+}
+size_t AddRegister::operator()(SparseState& state) const {   // system_operations.cpp:251-262
+    state.sync();
+    size_t id = System::add_register(register_name, type, size);
+    state.sync();
+    return id;
+}
+size_t AddRegisterWithHadamard::operator()(SparseState& state) const {   // system_operations.cpp:264-290
+    size_t id = AddRegister(register_name, type, size)(state);
+    const Layout& L = state.sync();
+    state.flush();
+    check(sparq_replicate(state.handle(), L.at(id)));
+    return id;
+}
+void RemoveRegister::operator()(SparseState& state) const {   // system_operations.cpp:303-322
+    TestRemovable{register_id}(state);
+    if (register_id >= System::name_register_map.size()) throw std::runtime_error("Register not found.");
+    System::remove_register(register_id);
+    state.sync();
+}
+size_t SplitRegister::operator()(SparseState& state) const {   // system_operations.cpp:134-166
+    const Layout old = state.sync();
+    state.flush();
+    size_t fp = System::get(first_name);
+    if (fp == SIZE_MAX) throw std::runtime_error("Register not found.");
+    size_t orig = System::size_of(fp);
+    if (orig < second_size) throw std::invalid_argument("SplitRegister: second register larger than the first");
+    StateStorageType type = System::type_of(fp);
+    size_t sp = System::add_register(second_name, type, second_size);
+    System::set_size(fp, orig - second_size);
+    Layout T = Layout::from_registry();
+    std::vector<std::pair<sparq_field_t, sparq_field_t>> moves;
+    for (size_t id = 0; id < old.field.size(); ++id) {
+        if (old.field[id].width == 0 || id == fp || id == sp) continue;
+        if (id < T.field.size() && T.field[id].width == old.field[id].width) moves.emplace_back(old.field[id], T.field[id]);
+    }
+    sparq_field_t f = old.field[fp];
+    if (orig - second_size > 0)
+        moves.emplace_back(sparq_field_t{f.word, (uint8_t)(f.shift + second_size), (uint8_t)(orig - second_size), 0}, T.field[fp]);
+    if (second_size > 0) moves.emplace_back(sparq_field_t{f.word, f.shift, (uint8_t)second_size, 0}, T.field[sp]);
+    state.relayout_with(T, moves);
+    return sp;
+}
+size_t CombineRegister::operator()(SparseState& state) const {   // system_operations.cpp:172-203
+    const Layout old = state.sync();
+    state.flush();
+    size_t fp = System::get(first_name), sp = System::get(second_name);
+    if (fp == SIZE_MAX || sp == SIZE_MAX) throw std::runtime_error("Register not found.");
+    size_t w1 = System::size_of(fp), w2 = System::size_of(sp);
+    if (w1 + w2 > 64) throw std::invalid_argument("CombineRegister: combined register wider than 64 bits");
+    System::set_size(fp, w1 + w2);
+    System::remove_register(sp);
+    Layout T = Layout::from_registry();
+    std::vector<std::pair<sparq_field_t, sparq_field_t>> moves;
+    for (size_t id = 0; id < old.field.size(); ++id) {
+        if (old.field[id].width == 0 || id == fp || id == sp) continue;
+        if (id < T.field.size() && T.field[id].width == old.field[id].width) moves.emplace_back(old.field[id], T.field[id]);
+    }
+    sparq_field_t nf = T.field[fp];
+    moves.emplace_back(old.field[fp], sparq_field_t{nf.word, (uint8_t)(nf.shift + w2), (uint8_t)w1, 0});
+    moves.emplace_back(old.field[sp], sparq_field_t{nf.word, nf.shift, (uint8_t)w2, 0});
+    state.relayout_with(T, moves);
+    return fp;
+}
+
+void add_systems(SparseState& current, SparseState& new_state, double coef) {   // system_operations.h:157-181
+    current.sync();
+    new_state.sync();
+    current.flush();
+    new_state.flush();
+    check(sparq_add_systems(current.handle(), new_state.handle(), coef, epsilon));
+}
+
+}  // namespace sparqsim

@@ -1,0 +1,433 @@
+// Host C++ layer of sparqsim_amd: the reference's register table (System),
+// the device-resident SparseState and the BaseOperator hierarchy
+// (SparQ/include/basic_components.h:221-830), driving the MI355X engine
+// through the C ABI in include/sparq_hip.h.
+//
+// Registers are packed into the key most-significant-first in register-id
+// order, each field inside one 64-bit word, so that unsigned lexicographic
+// order of the packed words equals System::operator<
+// (SparQ/src/basic_components.cpp:461-474).
+#pragma once
+
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <tuple>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../../include/sparq_hip.h"
+
+namespace sparqsim {
+
+using complex_t = std::complex<double>;
+using u22_t = std::array<complex_t, 4>;   // row-major [u00,u01,u10,u11]
+
+enum StateStorageType { General = 0, UnsignedInteger = 1, SignedInteger = 2, Boolean = 3, Rational = 4 };
+
+constexpr double pi = 3.141592653589793238462643383279502884;
+constexpr double sqrt2inv = 0.7071067811865475244008443621048490;
+constexpr double epsilon = 1e-14;
+
+inline uint64_t width_mask(size_t w) { return w >= 64 ? ~uint64_t(0) : (uint64_t(1) << w) - 1; }
+
+// Throws the exception matching an engine status code.
+void check(int status);
+
+// ------------------------------------------------------------ register table
+using StateInfoType = std::tuple<std::string, StateStorageType, size_t, bool>;
+
+// Process-global register table with the reference's id-reuse rules
+// (SparQ/src/basic_components.cpp:341-459).
+struct System {
+    static std::vector<StateInfoType> name_register_map;
+    static std::unordered_map<std::string, size_t> name_to_index;
+    static std::vector<size_t> temporal_registers;
+    static std::vector<size_t> reusable_registers;
+    static size_t max_qubit_count, max_register_count, max_system_size;
+    static uint64_t layout_version;   // bumps on every table change
+
+    static void clear();
+    static size_t get_qubit_count();
+    static size_t get_activated_register_size();
+    static size_t get_last_activated_register();
+    static void update_max_size(size_t n);
+    static size_t get(std::string_view name);
+    static StateInfoType get_register_info(std::string_view name);
+    static const std::string& name_of(size_t id);
+    static size_t size_of(std::string_view name);
+    static size_t size_of(size_t id);
+    static StateStorageType type_of(std::string_view name);
+    static StateStorageType type_of(size_t id);
+    static bool status_of(std::string_view name);
+    static bool status_of(size_t id);
+    static size_t add_register(std::string_view name, StateStorageType type, size_t size);
+    static void remove_register(size_t id);
+    static void remove_register(std::string_view name);
+    static void set_size(size_t id, size_t size);
+    static void set_type(size_t id, StateStorageType t);
+};
+
+// Bit-field assignment of the active registers.
+struct Layout {
+    uint32_t words = 1;
+    std::vector<sparq_field_t> field;   // by register id; width 0 = inactive
+    static Layout from_registry();
+    bool operator==(const Layout& o) const;
+    sparq_field_t at(size_t id) const;
+};
+
+// ------------------------------------------------------------ state
+class SparseState {
+public:
+    SparseState();                                 // |0...0>, amplitude 1
+    explicit SparseState(size_t) = delete;
+    SparseState(const SparseState& other);
+    SparseState& operator=(const SparseState& other);
+    SparseState(SparseState&& other) noexcept;
+    SparseState& operator=(SparseState&& other) noexcept;
+    ~SparseState();
+
+    size_t size();
+    bool empty() { return size() == 0; }
+    sparq_state_t handle() { return h_; }
+
+    // Bring the device keys to the current register-table layout (new
+    // registers read as 0, removed ones are dropped).  Flushes pending work.
+    const Layout& sync();
+    const Layout& layout() const { return layout_; }
+    // Replace the layout after an explicit field move (Split/Combine).
+    void relayout_with(const Layout& target, const std::vector<std::pair<sparq_field_t, sparq_field_t>>& moves);
+
+    // Queue a per-entry op; consecutive ops run fused in one kernel.
+    void push_pointwise(const sparq_pw_op_t& op, const std::vector<sparq_cond_t>& conds);
+    void flush();
+
+    // Host copies: values[i][k] for the active register ids (ascending), amplitudes.
+    void export_entries(std::vector<size_t>& ids, std::vector<uint64_t>& values, std::vector<complex_t>& amps);
+    void import_entries(const std::vector<uint64_t>& values, const std::vector<complex_t>& amps);
+    std::string to_string(int precision = 6);
+
+private:
+    sparq_state_t h_ = nullptr;
+    Layout layout_;
+    std::vector<sparq_pw_op_t> pend_ops_;
+    std::vector<sparq_cond_t> pend_conds_;
+};
+
+// ------------------------------------------------------------ conditions
+struct Controllable {
+    std::vector<size_t> condition_variable_nonzeros;
+    std::vector<size_t> condition_variable_all_ones;
+    std::vector<std::pair<size_t, size_t>> condition_variable_by_bit;
+    std::vector<std::pair<size_t, uint64_t>> condition_variable_by_value;
+    bool has_condition() const {
+        return !condition_variable_nonzeros.empty() || !condition_variable_all_ones.empty() ||
+               !condition_variable_by_bit.empty() || !condition_variable_by_value.empty();
+    }
+    void clear_control_nonzeros() { condition_variable_nonzeros.clear(); }
+    void clear_control_all_ones() { condition_variable_all_ones.clear(); }
+    void clear_control_by_bit() { condition_variable_by_bit.clear(); }
+    void clear_control_by_value() { condition_variable_by_value.clear(); }
+    // Resolve to engine predicates for the given layout.
+    std::vector<sparq_cond_t> conditions(const Layout& L) const;
+    template <class T> void copy_control_conditions_to(T& t) const {
+        t.condition_variable_nonzeros = condition_variable_nonzeros;
+        t.condition_variable_all_ones = condition_variable_all_ones;
+        t.condition_variable_by_bit = condition_variable_by_bit;
+        t.condition_variable_by_value = condition_variable_by_value;
+    }
+};
+
+// ------------------------------------------------------------ operators
+struct BaseOperator {
+    virtual ~BaseOperator() = default;
+    virtual void operator()(SparseState& state) const = 0;
+    virtual void dag(SparseState& state) const { throw std::runtime_error("Dagger is not implemented."); }
+    virtual Controllable* controls() { return nullptr; }
+};
+struct SelfAdjointOperator : BaseOperator {
+    void dag(SparseState& state) const override { (*this)(state); }
+};
+
+double& clearzero_default_eps();
+
+// Per-entry operator described by an engine opcode over register ids.
+struct PointwiseOperator : BaseOperator, Controllable {
+    std::string name;
+    uint32_t opcode = 0, dag_opcode = 0;    // dag_opcode 0: self-adjoint
+    std::vector<size_t> regs;               // operand register ids (f0..f3)
+    uint64_t c[3] = {0, 0, 0};
+    uint64_t dag_c[3] = {0, 0, 0};
+    double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double dag_d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::shared_ptr<void> table_owner;      // keeps a QRAM table alive
+    const void* table = nullptr;
+    uint32_t table_word_bytes = 0;
+    std::vector<std::pair<size_t, uint64_t>> extra_zero_regs;   // implicit "== 0" predicates
+    bool has_dag = true;
+    Controllable* controls() override { return this; }
+    void operator()(SparseState& state) const override { emit(state, false); }
+    void dag(SparseState& state) const override;
+    void emit(SparseState& state, bool dagger) const;
+};
+
+struct QRAMCircuit_qutrit {
+    size_t address_size, data_size;
+    std::vector<uint64_t> memory;
+    std::shared_ptr<struct sparq_table_s> table;   // device copy
+    uint32_t word_bytes = 8;
+    uint64_t max_word = 0;
+    QRAMCircuit_qutrit(size_t addr, size_t data, std::vector<uint64_t> mem);
+};
+
+// ---- per-entry operator classes (names and constructor arguments follow the
+// reference classes in quantum_arithmetic.h, basic_gates.h, qram.h,
+// parallel_phase_operations.h and dark_magic.h)
+#define SPQ_PW_CLASS_3(NAME)                                                                        \
+    struct NAME : PointwiseOperator {                                                               \
+        NAME(std::string_view a, std::string_view b, std::string_view c)                            \
+            : NAME(System::get(a), System::get(b), System::get(c)) {}                               \
+        NAME(size_t a, size_t b, size_t c);                                                         \
+    };
+#define SPQ_PW_CLASS_2(NAME)                                                                        \
+    struct NAME : PointwiseOperator {                                                               \
+        NAME(std::string_view a, std::string_view b) : NAME(System::get(a), System::get(b)) {}      \
+        NAME(size_t a, size_t b);                                                                   \
+    };
+#define SPQ_PW_CLASS_4(NAME)                                                                        \
+    struct NAME : PointwiseOperator {                                                               \
+        NAME(std::string_view a, std::string_view b, std::string_view c, std::string_view d)        \
+            : NAME(System::get(a), System::get(b), System::get(c), System::get(d)) {}               \
+        NAME(size_t a, size_t b, size_t c, size_t d);                                               \
+    };
+#define SPQ_PW_CLASS_RCR(NAME)                                                                      \
+    struct NAME : PointwiseOperator {                                                               \
+        NAME(std::string_view a, uint64_t k, std::string_view b) : NAME(System::get(a), k, System::get(b)) {} \
+        NAME(size_t a, uint64_t k, size_t b);                                                       \
+    };
+#define SPQ_PW_CLASS_RC(NAME)                                                                       \
+    struct NAME : PointwiseOperator {                                                               \
+        NAME(std::string_view a, uint64_t k) : NAME(System::get(a), k) {}                           \
+        NAME(size_t a, uint64_t k);                                                                 \
+    };
+
+SPQ_PW_CLASS_3(Add_UInt_UInt)
+SPQ_PW_CLASS_3(Sub_UInt_UInt)
+SPQ_PW_CLASS_3(Mul_UInt_UInt)
+SPQ_PW_CLASS_3(Div_UInt_UInt)
+SPQ_PW_CLASS_3(And_UInt_UInt)
+SPQ_PW_CLASS_3(Or_UInt_UInt)
+SPQ_PW_CLASS_3(Xor_UInt_UInt)
+SPQ_PW_CLASS_3(Less_SInt_SInt)
+SPQ_PW_CLASS_3(Less_UInt_UInt)
+SPQ_PW_CLASS_3(GetMid_UInt_UInt)
+SPQ_PW_CLASS_3(Div_Sqrt_Arccos_UInt_UInt)
+SPQ_PW_CLASS_3(Sqrt_Div_Arccos_Int_UInt)
+SPQ_PW_CLASS_3(GetRotateAngle_Int_Int)
+SPQ_PW_CLASS_2(Add_UInt_UInt_InPlace)
+SPQ_PW_CLASS_2(Neg_UInt)
+SPQ_PW_CLASS_2(Abs_SInt)
+SPQ_PW_CLASS_2(Sqrt_UInt)
+SPQ_PW_CLASS_2(IsZero_UInt)
+SPQ_PW_CLASS_2(Negative_SInt)
+SPQ_PW_CLASS_2(Add_AnyInt_AnyInt_InPlace)
+SPQ_PW_CLASS_2(Assign)
+SPQ_PW_CLASS_2(Swap_General_General)
+SPQ_PW_CLASS_4(Select_Bool_UInt_UInt)
+SPQ_PW_CLASS_4(Carry_UInt_UInt)
+SPQ_PW_CLASS_4(Overflow_SInt_SInt)
+SPQ_PW_CLASS_4(MulOverflow_UInt_UInt)
+SPQ_PW_CLASS_4(Compare_UInt_UInt)
+SPQ_PW_CLASS_RCR(Mult_UInt_ConstUInt)
+SPQ_PW_CLASS_RCR(Add_Mult_UInt_ConstUInt_InPlace)
+SPQ_PW_CLASS_RCR(Add_UInt_ConstUInt)
+SPQ_PW_CLASS_RC(Add_ConstUInt_InPlace)
+SPQ_PW_CLASS_RC(ShiftLeft_InPlace)
+SPQ_PW_CLASS_RC(ShiftRight_InPlace)
+SPQ_PW_CLASS_RC(X_Bool)
+SPQ_PW_CLASS_RC(Y_Bool)
+SPQ_PW_CLASS_RC(Init_Unsafe)
+
+struct FlipBools : PointwiseOperator {
+    explicit FlipBools(std::string_view r) : FlipBools(System::get(r)) {}
+    explicit FlipBools(size_t r);
+};
+struct Mod_Mult_UInt_ConstUInt_InPlace : PointwiseOperator {
+    Mod_Mult_UInt_ConstUInt_InPlace(std::string_view r, uint64_t a, uint64_t x, uint64_t N)
+        : Mod_Mult_UInt_ConstUInt_InPlace(System::get(r), a, x, N) {}
+    Mod_Mult_UInt_ConstUInt_InPlace(size_t r, uint64_t a, uint64_t x, uint64_t N);
+};
+struct Swap_Bool_Bool : PointwiseOperator {
+    Swap_Bool_Bool(std::string_view a, size_t d1, std::string_view b, size_t d2)
+        : Swap_Bool_Bool(System::get(a), d1, System::get(b), d2) {}
+    Swap_Bool_Bool(size_t a, size_t d1, size_t b, size_t d2);
+};
+struct Phase_Bool : PointwiseOperator {
+    Phase_Bool(std::string_view r, size_t digit, double lambda) : Phase_Bool(System::get(r), digit, lambda) {}
+    Phase_Bool(size_t r, size_t digit, double lambda);
+};
+// Z/S/T: Phase_Bool with lambda = pi, pi/2, pi/4 on bit `digit` (basic_gates.h:390-450)
+struct Z_Bool : Phase_Bool {
+    explicit Z_Bool(std::string_view r, size_t digit = 0) : Phase_Bool(r, digit, pi) {}
+    explicit Z_Bool(size_t r, size_t digit = 0) : Phase_Bool(r, digit, pi) {}
+};
+struct S_Bool : Phase_Bool {
+    explicit S_Bool(std::string_view r, size_t digit = 0) : Phase_Bool(r, digit, pi / 2) {}
+    explicit S_Bool(size_t r, size_t digit = 0) : Phase_Bool(r, digit, pi / 2) {}
+};
+struct T_Bool : Phase_Bool {
+    explicit T_Bool(std::string_view r, size_t digit = 0) : Phase_Bool(r, digit, pi / 4) {}
+    explicit T_Bool(size_t r, size_t digit = 0) : Phase_Bool(r, digit, pi / 4) {}
+};
+struct RZ_Bool : PointwiseOperator {
+    RZ_Bool(std::string_view r, size_t digit, double angle) : RZ_Bool(System::get(r), digit, angle) {}
+    RZ_Bool(size_t r, size_t digit, double angle);
+};
+struct QRAMLoad : PointwiseOperator {
+    QRAMLoad(const QRAMCircuit_qutrit* q, std::string_view a, std::string_view d) : QRAMLoad(q, System::get(a), System::get(d)) {}
+    QRAMLoad(const QRAMCircuit_qutrit* q, size_t a, size_t d);
+};
+using QRAMLoadFast = QRAMLoad;
+struct ZeroConditionalPhaseFlip : PointwiseOperator {
+    explicit ZeroConditionalPhaseFlip(const std::vector<size_t>& ids);
+    explicit ZeroConditionalPhaseFlip(const std::vector<std::string>& names);
+};
+struct Reflection_Bool : PointwiseOperator {
+    bool inverse;
+    Reflection_Bool(const std::vector<size_t>& ids, bool inverse = false);
+    Reflection_Bool(const std::vector<std::string>& names, bool inverse = false);
+    Reflection_Bool(std::string_view name, bool inverse = false) : Reflection_Bool(std::vector<size_t>{System::get(name)}, inverse) {}
+    void operator()(SparseState& state) const override;
+    void dag(SparseState& state) const override { (*this)(state); }
+};
+struct GlobalPhase : PointwiseOperator {
+    explicit GlobalPhase(complex_t c);
+};
+
+// Branching / structural operators
+struct Hadamard_Int : SelfAdjointOperator, Controllable {
+    size_t id, n_digits;
+    Hadamard_Int(std::string_view reg, size_t n) : Hadamard_Int(System::get(reg), n) {}
+    Hadamard_Int(size_t reg, size_t n) : id(reg), n_digits(n) {}
+    Controllable* controls() override { return this; }
+    void operator()(SparseState& state) const override;
+};
+struct Hadamard_Int_Full : SelfAdjointOperator, Controllable {
+    size_t id;
+    explicit Hadamard_Int_Full(std::string_view reg) : Hadamard_Int_Full(System::get(reg)) {}
+    explicit Hadamard_Int_Full(size_t reg) : id(reg) {}
+    Controllable* controls() override { return this; }
+    void operator()(SparseState& state) const override;
+};
+struct Hadamard_Bool : SelfAdjointOperator, Controllable {
+    size_t out_id;
+    explicit Hadamard_Bool(std::string_view reg) : Hadamard_Bool(System::get(reg)) {}
+    explicit Hadamard_Bool(size_t reg);
+    Controllable* controls() override { return this; }
+    void operator()(SparseState& state) const override;
+};
+struct CondRot_Rational_Bool : BaseOperator {
+    size_t register_in, register_out;
+    CondRot_Rational_Bool(std::string_view in, std::string_view out) : CondRot_Rational_Bool(System::get(in), System::get(out)) {}
+    CondRot_Rational_Bool(size_t in, size_t out);
+    void operator()(SparseState& state) const override { apply(state, 1.0); }
+    void dag(SparseState& state) const override { apply(state, -1.0); }
+    void apply(SparseState& state, double sign) const;
+};
+struct CondRot_Fixed_Bool : CondRot_Rational_Bool {
+    using CondRot_Rational_Bool::CondRot_Rational_Bool;
+};
+// CondRot_General_Bool_Fast with the 2x2 given per value of the input register
+// (tabulated on the host for inputs of up to 24 bits).
+struct CondRot_General_Bool : BaseOperator {
+    size_t in_id, out_id;
+    std::function<u22_t(uint64_t)> func;
+    CondRot_General_Bool(size_t in, size_t out, std::function<u22_t(uint64_t)> f);
+    void operator()(SparseState& state) const override;
+};
+struct Rot_Bool : BaseOperator, Controllable {
+    size_t id, digit;
+    u22_t mat;
+    Rot_Bool(std::string_view reg, size_t digit, u22_t m) : Rot_Bool(System::get(reg), digit, m) {}
+    Rot_Bool(size_t reg, size_t digit, u22_t m);
+    Controllable* controls() override { return this; }
+    void operator()(SparseState& state) const override;
+    void dag(SparseState& state) const override;
+};
+// Named 2x2 gates (basic_gates.cpp:422-520): fixed matrices on Rot_Bool.
+struct RX_Bool : Rot_Bool { RX_Bool(std::string_view r, size_t digit, double theta); RX_Bool(size_t r, size_t digit, double theta); };
+struct RY_Bool : Rot_Bool { RY_Bool(std::string_view r, size_t digit, double theta); RY_Bool(size_t r, size_t digit, double theta); };
+struct SX_Bool : Rot_Bool { explicit SX_Bool(std::string_view r, size_t digit = 0); explicit SX_Bool(size_t r, size_t digit = 0); };
+struct U2_Bool : Rot_Bool { U2_Bool(std::string_view r, size_t digit, double phi, double lambda); U2_Bool(size_t r, size_t digit, double phi, double lambda); };
+struct U3_Bool : Rot_Bool { U3_Bool(std::string_view r, size_t digit, double theta, double phi, double lambda); U3_Bool(size_t r, size_t digit, double theta, double phi, double lambda); };
+u22_t make_func(uint64_t value, size_t n_digit);
+u22_t make_func_inv(uint64_t value, size_t n_digit);
+
+struct ClearZero : SelfAdjointOperator {
+    double eps;
+    ClearZero() : eps(clearzero_default_eps()) {}
+    explicit ClearZero(double e) : eps(e) {}
+    void operator()(SparseState& state) const override;
+};
+struct Normalize : SelfAdjointOperator {
+    void operator()(SparseState& state) const override;
+};
+struct SortUnconditional : SelfAdjointOperator {
+    void operator()(SparseState& state) const override;
+};
+struct CheckDuplicateKey : SelfAdjointOperator {
+    void operator()(SparseState& state) const override;
+};
+struct TestRemovable : SelfAdjointOperator {
+    size_t register_id;
+    explicit TestRemovable(std::string_view n) : register_id(System::get(n)) {}
+    explicit TestRemovable(size_t id) : register_id(id) {}
+    void operator()(SparseState& state) const override;
+};
+
+struct AddRegister {
+    std::string register_name;
+    StateStorageType type;
+    size_t size;
+    AddRegister(std::string_view n, StateStorageType t, size_t s);
+    size_t operator()(SparseState& state) const;
+};
+struct AddRegisterWithHadamard {
+    std::string register_name;
+    StateStorageType type;
+    size_t size;
+    AddRegisterWithHadamard(std::string_view n, StateStorageType t, size_t s) : register_name(n), type(t), size(s) {}
+    size_t operator()(SparseState& state) const;
+};
+struct RemoveRegister {
+    size_t register_id;
+    explicit RemoveRegister(std::string_view n) : register_id(System::get(n)) {}
+    explicit RemoveRegister(size_t id) : register_id(id) {}
+    void operator()(SparseState& state) const;
+};
+struct SplitRegister {
+    std::string first_name, second_name;
+    size_t second_size;
+    SplitRegister(std::string_view f, std::string_view s, size_t sz) : first_name(f), second_name(s), second_size(sz) {}
+    SplitRegister(size_t f, std::string_view s, size_t sz) : first_name(System::name_of(f)), second_name(s), second_size(sz) {}
+    size_t operator()(SparseState& state) const;
+};
+struct CombineRegister {
+    std::string first_name, second_name;
+    CombineRegister(std::string_view f, std::string_view s) : first_name(f), second_name(s) {}
+    CombineRegister(size_t f, size_t s) : first_name(System::name_of(f)), second_name(System::name_of(s)) {}
+    size_t operator()(SparseState& state) const;
+};
+
+// add_systems (system_operations.h:157-181)
+void add_systems(SparseState& current, SparseState& new_state, double coef);
+
+}  // namespace sparqsim

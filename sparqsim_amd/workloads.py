@@ -1,0 +1,48 @@
+"""Seeded synthetic workloads shared by the benchmark, the tests and the CPU baseline.
+
+All randomness comes from splitmix64 over a counter (base seed 0x5EED2026, plus
+the configuration index), so the CPU oracle and the GPU engine see identical
+inputs without depending on any library RNG.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BASE_SEED = 0x5EED2026
+
+_G = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n outputs of splitmix64 started at `seed` (outputs for states seed+G, seed+2G, ...)."""
+    with np.errstate(over="ignore"):
+        x = np.arange(1, n + 1, dtype=np.uint64) * _G + np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+        z = (x ^ (x >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def qram_memory(address_bits: int, data_bits: int, config_index: int) -> list[int]:
+    """2^address_bits words, uniform in [0, 2^data_bits)."""
+    words = splitmix64(BASE_SEED + config_index, 1 << address_bits)
+    if data_bits < 64:
+        words = words & np.uint64((1 << data_bits) - 1)
+    return words.tolist()
+
+
+def qram_interference(mod, qram, address_bits: int, data_bits: int):
+    """Single-state QRAM interference: H(addr); data ^= M[addr]; phase flip on the
+    parity of data (one Z per data bit); uncompute the load; H(addr).
+    Returns (state, entry-ops): entry-ops sums max(N_in, N_out) over the operators."""
+    st = mod.SparseState()
+    n = 1 << address_bits
+    mod.Hadamard_Int("addr", address_bits)(st)
+    load = mod.QRAMLoad(qram, "addr", "data")
+    load(st)
+    for i in range(data_bits):
+        mod.Phase_Bool("data", i, np.pi)(st)
+    load(st)
+    mod.Hadamard_Int("addr", address_bits)(st)
+    return st, n * (4 + data_bits)

@@ -1,0 +1,328 @@
+"""GPU parity: the HIP engine against the CPU oracle on the same seeded inputs.
+
+Keys are compared bit-exact after canonical sort; amplitudes within
+parity.RTOL (1e-12 relative to max(|a|, max|a| of the state)); per-entry ops
+that only move keys or multiply by host-computed constants are checked
+bit-identical.
+"""
+import numpy as np
+import pytest
+
+from parity import assert_same_state, random_state, run_both
+
+pytestmark = pytest.mark.gpu
+
+U = None  # set per module (UnsignedInteger etc. are identical names in both modules)
+
+
+def regs(mod, spec):
+    st = mod.SparseState()
+    for name, typ, w in spec:
+        mod.AddRegister(name, getattr(mod, typ), w)(st)
+    return st
+
+
+def loaded(mod, spec, n, seed):
+    st = regs(mod, spec)
+    vals, amps = random_state(mod, n, seed)
+    st.load(vals, amps)
+    return st
+
+
+# ---------------------------------------------------------------- configs (reduced sizes)
+@pytest.mark.parametrize("w", [4, 8])
+def test_config1_plain_circuit(ps, oracle, w):
+    """H(a); b += a; d ^= M[a]; H(a); ClearZero -- the plain CPU-runnable circuit at reduced width."""
+    rng = np.random.default_rng(0x5EED2026 + 0)
+    mem = [int(x) for x in rng.integers(0, 1 << w, size=1 << w)]
+
+    def build(mod):
+        st = regs(mod, [("a", "UnsignedInteger", w), ("b", "UnsignedInteger", w), ("d", "UnsignedInteger", w)])
+        q = mod.QRAMCircuit_qutrit(w, w, mem)
+        mod.Hadamard_Int("a", w)(st)
+        mod.Add_UInt_UInt_InPlace("a", "b")(st)
+        mod.QRAMLoad(q, "a", "d")(st)
+        mod.Hadamard_Int("a", w)(st)
+        mod.ClearZero(1e-14)(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert eg[1].shape[0] == (1 << (2 * w))
+    assert_same_state(eg, eo, exact_amps=True, what="config1")
+
+
+@pytest.mark.parametrize("A", [6, 12, 16])
+def test_config2_qram_interference(ps, oracle, A):
+    """H(addr); data ^= M[addr]; 32 Z on data bits; uncompute; H(addr)."""
+    D = 32
+    rng = np.random.default_rng(0x5EED2026 + 1)
+    mem = [int(x) for x in rng.integers(0, 1 << D, size=1 << A, dtype=np.uint64)]
+
+    def build(mod):
+        st = regs(mod, [("addr", "UnsignedInteger", A), ("data", "UnsignedInteger", D)])
+        q = mod.QRAMCircuit_qutrit(A, D, mem)
+        mod.Hadamard_Int("addr", A)(st)
+        mod.QRAMLoad(q, "addr", "data")(st)
+        for i in range(D):
+            mod.Phase_Bool("data", i, np.pi)(st)
+        mod.QRAMLoad(q, "addr", "data")(st)
+        mod.Hadamard_Int("addr", A)(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, what="config2")
+
+
+# ---------------------------------------------------------------- per-entry ops
+SPEC3 = [("a", "UnsignedInteger", 7), ("b", "UnsignedInteger", 9), ("c", "UnsignedInteger", 6),
+         ("s", "SignedInteger", 8), ("f", "Boolean", 1), ("g", "Boolean", 1)]
+
+ARITH = [
+    ("Add_UInt_UInt", ("a", "b", "c")), ("Sub_UInt_UInt", ("a", "b", "c")), ("Mul_UInt_UInt", ("a", "b", "c")),
+    ("Div_UInt_UInt", ("b", "a", "c")), ("And_UInt_UInt", ("a", "b", "c")), ("Or_UInt_UInt", ("a", "b", "c")),
+    ("Xor_UInt_UInt", ("a", "b", "c")), ("Less_SInt_SInt", ("s", "a", "f")), ("Less_UInt_UInt", ("a", "b", "f")),
+    ("GetMid_UInt_UInt", ("a", "c", "b")), ("Add_UInt_UInt_InPlace", ("a", "b")), ("Neg_UInt", ("a", "c")),
+    ("Abs_SInt", ("s", "c")), ("Sqrt_UInt", ("b", "c")), ("IsZero_UInt", ("c", "f")), ("Negative_SInt", ("s", "g")),
+    ("Add_AnyInt_AnyInt_InPlace", ("b", "s")), ("Assign", ("c", "a")),
+    ("Select_Bool_UInt_UInt", ("f", "a", "c", "b")), ("Carry_UInt_UInt", ("a", "c", "a", "f")),
+    ("Overflow_SInt_SInt", ("s", "a", "s", "g")), ("MulOverflow_UInt_UInt", ("a", "c", "a", "f")),
+    ("Compare_UInt_UInt", ("a", "c", "f", "g")), ("Mult_UInt_ConstUInt", ("a", 5, "b")),
+    ("Add_Mult_UInt_ConstUInt_InPlace", ("a", 3, "b")), ("Add_UInt_ConstUInt", ("a", 77, "b")),
+    ("Add_ConstUInt_InPlace", ("b", 300)), ("ShiftLeft_InPlace", ("b", 3)), ("ShiftRight_InPlace", ("a", 2)),
+    ("X_Bool", ("b", 4)), ("FlipBools", ("c",)), ("Mod_Mult_UInt_ConstUInt_InPlace", ("b", 7, 2, 509)),
+    ("Swap_Bool_Bool", ("a", 1, "b", 7)), ("Init_Unsafe", ("c", 13)),
+]
+
+
+@pytest.mark.parametrize("name,args", ARITH, ids=[a[0] for a in ARITH])
+def test_key_rewrite_ops(ps, oracle, name, args):
+    def build(mod):
+        st = loaded(mod, SPEC3, 3000, seed=1)
+        op = getattr(mod, name)(*args)
+        op(st)
+        op(st)              # twice: exercises XOR-into involution / in-place composition
+        if name not in ("Init_Unsafe", "Mod_Mult_UInt_ConstUInt_InPlace"):
+            try:
+                op.dag(st)
+            except RuntimeError:
+                pass
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, exact_amps=True, what=name)
+
+
+def test_conditions_on_pointwise(ps, oracle):
+    def build(mod):
+        st = loaded(mod, SPEC3, 4000, seed=2)
+        mod.Add_UInt_UInt("a", "b", "c").conditioned_by_nonzeros("f")(st)
+        mod.X_Bool("a", 3).conditioned_by_bit("b", 2)(st)
+        mod.Add_ConstUInt_InPlace("b", 11).conditioned_by_value("c", 5)(st)
+        mod.FlipBools("c").conditioned_by_all_ones("g")(st)
+        mod.Phase_Bool("a", 1, 0.37).conditioned_by_bit([("b", 0), ("c", 1)])(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, exact_amps=True, what="conditions")
+
+
+def test_diagonal_ops(ps, oracle):
+    def build(mod):
+        st = loaded(mod, SPEC3, 4000, seed=3)
+        mod.Phase_Bool("a", 2, 0.913)(st)
+        mod.Phase_Bool("a", 2, 0.913).dag(st)
+        mod.Phase_Bool("b", 5, 1.7)(st)
+        mod.Z_Bool("c", 1)(st)
+        mod.S_Bool("a", 0)(st)
+        mod.T_Bool("b", 3)(st)
+        mod.RZ_Bool("b", 1, 0.77)(st)
+        mod.Y_Bool("c", 2)(st)
+        mod.GlobalPhase(complex(0.6, 0.8))(st)
+        mod.GlobalPhase(complex(0.6, 0.8)).dag(st)
+        mod.ZeroConditionalPhaseFlip(["f", "g"])(st)
+        mod.Reflection_Bool(["f"], False)(st)
+        mod.Reflection_Bool(["g", "f"], True)(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, exact_amps=True, what="diagonal")
+
+
+def test_qram_load_conditioned(ps, oracle):
+    rng = np.random.default_rng(5)
+    mem = [int(x) for x in rng.integers(0, 1 << 9, size=1 << 7)]
+
+    def build(mod):
+        st = loaded(mod, SPEC3, 3000, seed=4)
+        q = mod.QRAMCircuit_qutrit(7, 9, mem)
+        mod.QRAMLoad(q, "a", "b").conditioned_by_nonzeros("f")(st)
+        mod.QRAMLoad(q, "c", "b")(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, exact_amps=True, what="qram")
+
+
+# ---------------------------------------------------------------- branching
+@pytest.mark.parametrize("nd", [1, 3, 6])
+def test_hadamard_int_mixed_groups(ps, oracle, nd):
+    """Random state: groups of size 1, partial and full occur together; n_digits <= width."""
+    spec = [("x", "UnsignedInteger", 6), ("y", "UnsignedInteger", 5)]
+
+    def build(mod):
+        st = loaded(mod, spec, 700, seed=10 + nd)
+        mod.Hadamard_Int("x", nd)(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, what=f"Hadamard_Int nd={nd}")
+
+
+def test_hadamard_int_conditioned(ps, oracle):
+    spec = [("x", "UnsignedInteger", 5), ("y", "UnsignedInteger", 4), ("c", "Boolean", 1)]
+
+    def build(mod):
+        st = loaded(mod, spec, 300, seed=21)
+        mod.Hadamard_Int("x", 5).conditioned_by_nonzeros("c")(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, what="Hadamard_Int conditioned")
+
+
+@pytest.mark.parametrize("n", [1, 3, 40, 256])
+def test_hadamard_int_full(ps, oracle, n):
+    spec = [("x", "UnsignedInteger", 8), ("y", "UnsignedInteger", 3)]
+
+    def build(mod):
+        st = regs(mod, spec)
+        rng = np.random.default_rng(n)
+        xs = rng.choice(256, size=n, replace=False)
+        vals = np.stack([xs, np.full(n, 5)], axis=1).astype(np.uint64)
+        amps = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        st.load(vals, amps)
+        mod.Hadamard_Int_Full("x")(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, what=f"Hadamard_Int_Full n={n}")
+
+
+def test_hadamard_bool_and_rotations(ps, oracle):
+    spec = [("a", "UnsignedInteger", 5), ("r", "Rational", 6), ("q", "Boolean", 1), ("m", "UnsignedInteger", 3)]
+    mat = [complex(0.6, 0.1), complex(-0.3, 0.2), complex(0.5, -0.4), complex(0.2, 0.7)]
+
+    def build(mod):
+        st = loaded(mod, spec, 1500, seed=30)
+        mod.Hadamard_Bool("q")(st)
+        mod.CondRot_Rational_Bool("r", "q")(st)
+        mod.CondRot_Rational_Bool("r", "q").dag(st)
+        mod.Rot_Bool("m", 1, mat)(st)
+        mod.Rot_Bool("m", 2, [1, 0, 0, complex(0, 1)])(st)            # diagonal, in place
+        mod.Rot_Bool("m", 0, [0, complex(0, -1), complex(0, 1), 0])(st)  # off-diagonal
+        mod.Hadamard_Bool("q").conditioned_by_nonzeros("a")(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, what="pair ops")
+
+
+def test_condrot_general(ps, oracle):
+    spec = [("r", "UnsignedInteger", 5), ("q", "Boolean", 1), ("z", "UnsignedInteger", 4)]
+
+    def build(mod):
+        st = loaded(mod, spec, 400, seed=31)
+        mod.CondRot_General_Bool("r", "q", lambda v: mod.make_func(v, 5))(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, what="CondRot_General_Bool")
+
+
+# ---------------------------------------------------------------- structural
+def test_register_structure(ps, oracle):
+    def build(mod):
+        st = loaded(mod, [("a", "UnsignedInteger", 10), ("b", "UnsignedInteger", 12)], 2000, seed=40)
+        mod.AddRegister("t", mod.UnsignedInteger, 9)(st)
+        mod.Add_UInt_UInt("a", "b", "t")(st)
+        mod.SplitRegister("b", "b_lo", 5)(st)
+        mod.Add_UInt_UInt_InPlace("b_lo", "a")(st)
+        mod.CombineRegister("b", "b_lo")(st)
+        mod.Add_UInt_UInt("a", "b", "t")(st)    # uncompute t ...
+        mod.Add_UInt_UInt("a", "b", "t")(st)
+        mod.Add_UInt_UInt("a", "b", "t")(st)    # ... t = a+b again
+        mod.Init_Unsafe("t", 0)(st)
+        mod.RemoveRegister("t")(st)
+        mod.AddRegister("u", mod.UnsignedInteger, 3)(st)      # reuses the freed id
+        mod.X_Bool("u", 1)(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, exact_amps=True, what="register structure")
+
+
+def test_remove_entangled_register_raises(ps):
+    st = ps.SparseState()
+    ps.AddRegister("a", ps.UnsignedInteger, 4)(st)
+    ps.Hadamard_Int("a", 4)(st)
+    with pytest.raises(RuntimeError):
+        ps.RemoveRegister("a")(st)
+
+
+def test_add_systems_and_normalize(ps, oracle):
+    spec = [("a", "UnsignedInteger", 9), ("b", "UnsignedInteger", 4)]
+
+    def build(mod):
+        s1 = loaded(mod, spec, 900, seed=50)
+        s2 = mod.SparseState()
+        vals, amps = random_state(mod, 900, seed=51)
+        s2.load(vals, amps)
+        mod.add_systems(s1, s2, 0.5)
+        mod.Normalize()(s1)
+        return s1
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, what="add_systems")
+
+
+def test_sort_unconditional_order(ps, oracle):
+    spec = [("a", "UnsignedInteger", 20), ("b", "UnsignedInteger", 30), ("c", "UnsignedInteger", 30)]
+
+    def build(mod):
+        st = loaded(mod, spec, 50000, seed=60)
+        mod.SortUnconditional()(st)
+        return st
+
+    ps.System.clear()
+    oracle.System.clear()
+    eg = build(ps).export()
+    eo = build(oracle).export()
+    # both are in canonical order already: compare row by row without re-sorting
+    assert np.array_equal(eg[1], eo[1])
+    assert np.array_equal(eg[2], eo[2])
+
+
+def test_clear_zero_threshold(ps, oracle):
+    def build(mod):
+        st = regs(mod, [("a", "UnsignedInteger", 10)])
+        n = 1024
+        vals = np.arange(n, dtype=np.uint64).reshape(n, 1)
+        amps = np.logspace(-9, 0, n) * np.exp(1j * np.arange(n))
+        st.load(vals, amps)
+        mod.ClearZero(1e-8)(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, exact_amps=True, what="ClearZero")
+
+
+def test_empty_state_ops(ps):
+    st = ps.SparseState()
+    ps.AddRegister("a", ps.UnsignedInteger, 4)(st)
+    ps.X_Bool("a", 0)(st)
+    ps.ClearZero(2.0)(st)      # removes the only entry
+    assert st.size() == 0
+    ps.Hadamard_Int("a", 4)(st)
+    ps.X_Bool("a", 1)(st)
+    ps.Hadamard_Bool  # noqa: B018
+    assert st.size() == 0

@@ -1,0 +1,262 @@
+"""CPU checks that pin the oracle (oracle/sparq_oracle*.hpp) before it is trusted.
+
+Each test names what fixes the expected value: a known answer from the
+reference's own tests, a closed form, an invariant, or brute force with a dense
+numpy model on small registers.  None of them re-types the oracle's formula.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+
+def reg_state(o, spec):
+    st = o.SparseState()
+    ids = [o.AddRegister(n, getattr(o, t), w)(st) for n, t, w in spec]
+    return st, ids
+
+
+def amp_map(st):
+    ids, vals, amps = st.export()
+    return {tuple(int(x) for x in v): a for v, a in zip(vals, amps)}
+
+
+# ---- known answers of SparQ/test/hadamard.cpp:14-111
+def test_hadamard_known_answers(oracle):
+    st, _ = reg_state(oracle, [("q", "Boolean", 1)])
+    oracle.Hadamard_Int_Full("q")(st)
+    m = amp_map(st)
+    assert abs(m[(0,)] - 2 ** -0.5) < 1e-9 and abs(m[(1,)] - 2 ** -0.5) < 1e-9
+    oracle.System.clear()
+    st, _ = reg_state(oracle, [("q", "Boolean", 1)])
+    oracle.Init_Unsafe("q", 1)(st)
+    oracle.Hadamard_Int_Full("q")(st)
+    m = amp_map(st)
+    assert abs(m[(0,)] - 2 ** -0.5) < 1e-9 and abs(m[(1,)] + 2 ** -0.5) < 1e-9
+    oracle.Hadamard_Int_Full("q")(st)
+    m = amp_map(st)
+    assert list(m) == [(1,)] and abs(m[(1,)] - 1) < 1e-9     # H^2 = I
+    oracle.System.clear()
+    st, _ = reg_state(oracle, [("q", "UnsignedInteger", 2)])
+    oracle.Hadamard_Int_Full("q")(st)
+    assert st.size() == 4 and all(abs(abs(a) - 0.5) < 1e-9 for a in amp_map(st).values())
+
+
+def dense_wht(n):
+    h = np.array([[1.0]])
+    for _ in range(n):
+        h = np.kron(h, np.array([[1, 1], [1, -1]]) / np.sqrt(2))
+    return h
+
+
+@pytest.mark.parametrize("variant", ["Int", "Full"])
+@pytest.mark.parametrize("k", [1, 2, 5, 16])
+def test_hadamard_closed_form(oracle, variant, k):
+    """H^{(x)n} on a random k-sparse vector equals the dense Walsh-Hadamard matrix product
+    (H|v> = 2^{-n/2} sum_j (-1)^{popcount(j&v)} |j>, hadamard.cpp:11-17)."""
+    n = 4
+    rng = np.random.default_rng(k)
+    st, _ = reg_state(oracle, [("x", "UnsignedInteger", n), ("y", "UnsignedInteger", 2)])
+    xs = rng.choice(1 << n, size=k, replace=False)
+    vec = np.zeros(1 << n, complex)
+    vec[xs] = rng.standard_normal(k) + 1j * rng.standard_normal(k)
+    st.load(np.stack([xs, np.full(k, 2)], 1).astype(np.uint64), vec[xs])
+    (oracle.Hadamard_Int("x", n) if variant == "Int" else oracle.Hadamard_Int_Full("x"))(st)
+    out = dense_wht(n) @ vec
+    m = amp_map(st)
+    for j in range(1 << n):
+        got = m.get((j, 2), 0)
+        assert abs(got - out[j]) < 1e-12
+    assert all(key[1] == 2 for key in m)
+
+
+def test_hadamard_partial_digits(oracle):
+    """n_digits < width acts on the low bits only; upper bits keep their value."""
+    st, _ = reg_state(oracle, [("x", "UnsignedInteger", 6)])
+    oracle.Init_Unsafe("x", 0b101100)(st)
+    oracle.Hadamard_Int("x", 2)(st)
+    m = amp_map(st)
+    assert sorted(m) == [(0b101100 + j,) for j in range(4)]
+    assert all(abs(a - 0.5) < 1e-15 for a in m.values())
+
+
+# ---- reference arithmetic expectations (SparQ/test/quantum_arithmetic.cpp:155-1181)
+def run_classical(o, spec, init, op_factory, dagger=False):
+    o.System.clear()
+    st, ids = reg_state(o, spec)
+    for name, v in init.items():
+        o.Init_Unsafe(name, v)(st)
+    op = op_factory()
+    (op.dag if dagger else op)(st)
+    ids_, vals, amps = st.export()
+    assert len(amps) == 1 and amps[0] == 1
+    return dict(zip([n for n, _, _ in spec], (int(x) for x in vals[0])))
+
+
+def test_add_inplace_truth_table(oracle):
+    # (lhs + rhs) & 0xf, quantum_arithmetic.cpp test :175-191
+    for a, b in itertools.product(range(16), repeat=2):
+        r = run_classical(oracle, [("l", "UnsignedInteger", 4), ("r", "UnsignedInteger", 4)], {"l": a, "r": b},
+                          lambda: oracle.Add_UInt_UInt_InPlace("l", "r"))
+        assert r["r"] == (a + b) & 0xF and r["l"] == a
+        r = run_classical(oracle, [("l", "UnsignedInteger", 4), ("r", "UnsignedInteger", 4)], {"l": a, "r": b},
+                          lambda: oracle.Add_UInt_UInt_InPlace("l", "r"), dagger=True)
+        assert r["r"] == (b - a) & 0xF
+
+
+def test_xor_into_truth_tables(oracle):
+    spec = [("a", "UnsignedInteger", 3), ("b", "UnsignedInteger", 3), ("c", "UnsignedInteger", 4)]
+    cases = {
+        "Add_UInt_UInt": lambda a, b: (a + b) & 0xF, "Sub_UInt_UInt": lambda a, b: (a - b) & 0xF,
+        "Mul_UInt_UInt": lambda a, b: (a * b) & 0xF, "Div_UInt_UInt": lambda a, b: 0 if b == 0 else a // b,
+        "And_UInt_UInt": lambda a, b: a & b, "Or_UInt_UInt": lambda a, b: a | b, "Xor_UInt_UInt": lambda a, b: a ^ b,
+        "GetMid_UInt_UInt": lambda a, b: (a + b) // 2,
+    }
+    for name, f in cases.items():
+        for a, b in itertools.product(range(8), repeat=2):
+            for c0 in (0, 9):
+                r = run_classical(oracle, spec, {"a": a, "b": b, "c": c0}, lambda: getattr(oracle, name)("a", "b", "c"))
+                assert r["c"] == c0 ^ f(a, b), (name, a, b, c0)
+
+
+def test_compare_and_flags(oracle):
+    spec = [("a", "UnsignedInteger", 3), ("b", "UnsignedInteger", 3), ("lt", "Boolean", 1), ("eq", "Boolean", 1)]
+    for a, b in itertools.product(range(8), repeat=2):
+        r = run_classical(oracle, spec, {"a": a, "b": b}, lambda: oracle.Compare_UInt_UInt("a", "b", "lt", "eq"))
+        assert (r["lt"], r["eq"]) == (int(a < b), int(a == b))
+    spec = [("s", "SignedInteger", 4), ("t", "SignedInteger", 4), ("f", "Boolean", 1)]
+    sx = lambda v: v - 16 if v >= 8 else v  # noqa: E731
+    for a, b in itertools.product(range(16), repeat=2):
+        r = run_classical(oracle, spec, {"s": a, "t": b}, lambda: oracle.Less_SInt_SInt("s", "t", "f"))
+        assert r["f"] == int(sx(a) < sx(b))
+
+
+def test_shift_rotations_roundtrip(oracle):
+    for v in range(32):
+        r = run_classical(oracle, [("x", "UnsignedInteger", 5)], {"x": v}, lambda: oracle.ShiftLeft_InPlace("x", 2))
+        assert r["x"] == ((v << 2) | (v >> 3)) & 31          # rotate left by 2 in 5 bits
+        r = run_classical(oracle, [("x", "UnsignedInteger", 5)], {"x": v}, lambda: oracle.ShiftRight_InPlace("x", 2))
+        assert r["x"] == ((v >> 2) | (v << 3)) & 31
+
+
+def test_modmul_inverse(oracle):
+    for v in range(21):
+        r = run_classical(oracle, [("x", "UnsignedInteger", 5)], {"x": v}, lambda: oracle.Mod_Mult_UInt_ConstUInt_InPlace("x", 2, 1, 21))
+        assert r["x"] == (v * 4) % 21      # a^(2^x) with a=2, x=1
+        r2 = run_classical(oracle, [("x", "UnsignedInteger", 5)], {"x": r["x"]}, lambda: oracle.Mod_Mult_UInt_ConstUInt_InPlace("x", 2, 1, 21), dagger=True)
+        assert r2["x"] == v
+
+
+def test_qram_xor_model(oracle):
+    """data ^= memory[address] (docs/verification/v1b-runtime.md:14)."""
+    mem = [(7 * i + 3) % 64 for i in range(16)]
+    q = oracle.QRAMCircuit_qutrit(4, 6, mem)
+    for a in range(16):
+        for d0 in (0, 21):
+            r = run_classical(oracle, [("a", "UnsignedInteger", 4), ("d", "UnsignedInteger", 6)], {"a": a, "d": d0},
+                              lambda: oracle.QRAMLoad(q, "a", "d"))
+            assert r["d"] == d0 ^ mem[a]
+
+
+# ---- rotations: CudaBenchmark analytic answer (test/GPUTest/Benchmark/CudaBenchmark.cu:13-18,188-217)
+def test_cudabenchmark_analytic(oracle):
+    n = 6
+    st, _ = reg_state(oracle, [("addr", "UnsignedInteger", n), ("sum", "UnsignedInteger", n + 1),
+                               ("sq", "UnsignedInteger", n // 2 + 1), ("b", "UnsignedInteger", 1)])
+    th = np.pi / 3
+    rot = [np.cos(th), -np.sin(th), np.sin(th), np.cos(th)]
+    phase = complex(np.cos(np.pi / 5), np.sin(np.pi / 5))
+    oracle.Hadamard_Int("addr", n)(st)
+    oracle.Add_UInt_UInt("addr", "addr", "sum")(st)
+    oracle.Sqrt_UInt("addr", "sq")(st)
+    oracle.X_Bool("b", 0)(st)
+    oracle.Rot_Bool("b", 0, rot)(st)
+    oracle.GlobalPhase(phase)(st)
+    oracle.Normalize()(st)
+    inv = 1 / np.sqrt(2 ** n)
+    m = amp_map(st)
+    assert len(m) == 2 * 2 ** n
+    for k in range(2 ** n):
+        r = int(np.floor(np.sqrt(k)))
+        assert abs(m[(k, 2 * k, r, 1)] - 0.5 * phase * inv) < 1e-12
+        assert abs(m[(k, 2 * k, r, 0)] + np.sqrt(3) / 2 * phase * inv) < 1e-12
+
+
+def test_condrot_closed_form(oracle):
+    """CondRot_Rational_Bool: |v>|0> -> cos(2 pi v/2^w)|v>|0> + sin(2 pi v/2^w)|v>|1> (condrot.cpp:27-33)."""
+    w = 5
+    st, _ = reg_state(oracle, [("r", "Rational", w), ("q", "Boolean", 1)])
+    oracle.Hadamard_Int("r", w)(st)
+    oracle.CondRot_Rational_Bool("r", "q")(st)
+    m = amp_map(st)
+    for v in range(2 ** w):
+        th = 2 * np.pi * v / 2 ** w
+        a = 2 ** (-w / 2)
+        assert abs(m.get((v, 0), 0) - a * np.cos(th)) < 1e-12
+        assert abs(m.get((v, 1), 0) - a * np.sin(th)) < 1e-12
+    oracle.CondRot_Rational_Bool("r", "q").dag(st)
+    m = amp_map(st)
+    assert all(k[1] == 0 for k in m)            # dag undoes the rotation (|1> branch pruned)
+
+
+def test_hadamard_bool_pairs(oracle):
+    """Dense check of H on a 1-bit register over a random superposition."""
+    st, _ = reg_state(oracle, [("x", "UnsignedInteger", 3), ("q", "Boolean", 1)])
+    rng = np.random.default_rng(3)
+    keys = [(x, q) for x in range(8) for q in range(2)]
+    pick = rng.choice(16, size=9, replace=False)
+    vals = np.array([keys[i] for i in pick], np.uint64)
+    amps = rng.standard_normal(9) + 1j * rng.standard_normal(9)
+    st.load(vals, amps)
+    dense = np.zeros(16, complex)
+    for (x, q), a in zip(vals, amps):
+        dense[int(x) * 2 + int(q)] = a
+    h = np.kron(np.eye(8), dense_wht(1))
+    out = h @ dense
+    m = amp_map(st)
+    oracle.Hadamard_Bool("q")(st)
+    m = amp_map(st)
+    for i in range(16):
+        assert abs(m.get((i // 2, i % 2), 0) - out[i]) < 1e-12
+
+
+def test_sort_merge_invariants(oracle):
+    """add_systems: result sorted, unique keys, duplicates summed, |a|^2 < 1e-14 dropped."""
+    spec = [("a", "UnsignedInteger", 6)]
+    s1, _ = reg_state(oracle, spec)
+    s1.load(np.array([[5], [1], [9], [3]], np.uint64), np.array([1, 2, 3, 1e-8]))
+    s2 = oracle.SparseState()
+    s2.load(np.array([[9], [4], [3]], np.uint64), np.array([-3, 5, -1e-8]))
+    oracle.add_systems(s1, s2, 1.0)
+    ids, vals, amps = s1.export()
+    assert [int(v) for v in vals[:, 0]] == [1, 4, 5]
+    assert list(amps) == [2, 5, 1]
+
+
+def test_clear_zero_is_strict(oracle):
+    st, _ = reg_state(oracle, [("a", "UnsignedInteger", 2)])
+    st.load(np.array([[0], [1], [2]], np.uint64), np.array([0.5, 0.49999, 0.5j]))
+    oracle.ClearZero(0.25)(st)                  # |0.5|^2 = 0.25 is not < 0.25: kept (strict <)
+    assert sorted(int(v[0]) for v in st.export()[1]) == [0, 2]
+
+
+def test_register_reuse_lifo(oracle):
+    """Freed ids are reused last-in-first-out (basic_components.cpp:344-356)."""
+    st = oracle.SparseState()
+    ids = [oracle.AddRegister(f"r{i}", oracle.UnsignedInteger, 3)(st) for i in range(4)]
+    oracle.RemoveRegister("r1")(st)
+    oracle.RemoveRegister("r3")(st)
+    assert oracle.AddRegister("x", oracle.UnsignedInteger, 2)(st) == ids[3]
+    assert oracle.AddRegister("y", oracle.UnsignedInteger, 2)(st) == ids[1]
+
+
+def test_split_combine_roundtrip(oracle):
+    for v in (0, 1, 0b1011011, 127):
+        oracle.System.clear()
+        st, _ = reg_state(oracle, [("a", "UnsignedInteger", 7)])
+        oracle.Init_Unsafe("a", v)(st)
+        oracle.SplitRegister("a", "lo", 3)(st)
+        m = amp_map(st)
+        assert list(m) == [(v >> 3, v & 7)]
+        oracle.CombineRegister("a", "lo")(st)
+        assert list(amp_map(st)) == [(v,)]
