@@ -182,6 +182,7 @@ PYBIND11_MODULE(_oracle, m) {
         return py::cpp_function([a, b](PyState& p) { return CombineRegister(p.s.basis_states, a, b); });
     });
     m.def("add_systems", [](PyState& cur, const PyState& add, double coef) { add_systems(cur.s.basis_states, add.s.basis_states, coef); });
+    m.def("reset_systems", [](PyState& p) { p.s.basis_states.clear(); p.s.basis_states.emplace_back(); });   // system_operations.cpp:108-117
     m.def("CheckDuplicateKey", [](PyState& p) { CheckDuplicateKey(p.s.basis_states); });
 
     // ---- per-entry ops

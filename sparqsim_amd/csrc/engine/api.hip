@@ -76,6 +76,7 @@ static State sub_state(State& parent, uint64_t b, uint64_t cnt) {
     s.W = parent.W;
     s.stream = parent.stream;
     s.pinned = parent.pinned;
+    s.counter = parent.counter;
     s.device = parent.device;
     alloc_arrays(s, s.W, std::max<uint64_t>(cnt, 1), &s.keys, &s.amps);
     s.cap = std::max<uint64_t>(cnt, 1);
@@ -155,6 +156,8 @@ int sparq_state_create(uint32_t words, uint64_t capacity, sparq_state_t* out) {
             SPQ_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
             SPQ_HIP(hipHostMalloc((void**)&st.pinned, 64 * sizeof(uint64_t), hipHostMallocDefault));
             for (auto& e : st.ev) SPQ_HIP(hipEventCreate(&e));
+            SPQ_HIP(hipMalloc(&st.counter, 256));
+            SPQ_HIP(hipMemset(st.counter, 0, 256));
             st.cap = std::max<uint64_t>(capacity, 1);
             alloc_arrays(st, words, st.cap, &st.keys, &st.amps);
             st.n = 0;
@@ -175,6 +178,7 @@ int sparq_state_destroy(sparq_state_t h) {
         for (auto& e : st.ev)
             if (e) (void)hipEventDestroy(e);
         if (st.pinned) (void)hipHostFree(st.pinned);
+        if (st.counter) (void)hipFree(st.counter);
         if (st.stream) (void)hipStreamDestroy(st.stream);
         delete h;
     });
@@ -211,16 +215,16 @@ int sparq_state_size(sparq_state_t h, uint64_t* n) {
 int sparq_state_set_basis(sparq_state_t h, const uint64_t* key) {
     return guarded([&] {
         State& st = S(h);
-        std::vector<uint64_t> k(st.W, 0);
-        if (key) std::memcpy(k.data(), key, st.W * 8);
-        double amp[2] = {1.0, 0.0};
-        uint64_t* kd;
-        double2* ad;
-        alloc_arrays(st, st.W, 1, &kd, &ad);
-        SPQ_HIP(hipMemcpyAsync(kd, k.data(), st.W * 8, hipMemcpyHostToDevice, st.stream));
-        SPQ_HIP(hipMemcpyAsync(ad, amp, 16, hipMemcpyHostToDevice, st.stream));
-        SPQ_HIP(hipStreamSynchronize(st.stream));
-        install_arrays(st, st.W, kd, ad, 1, 1);
+        BasisKey k{};
+        if (key) std::memcpy(k.w, key, st.W * 8);
+        if (st.cap < 1 || !st.keys) {
+            uint64_t* kd;
+            double2* ad;
+            alloc_arrays(st, st.W, 1, &kd, &ad);
+            install_arrays(st, st.W, kd, ad, 0, 1);
+        }
+        write_basis(st, k);   // stream-ordered; no host synchronization
+        st.n = 1;
     });
 }
 
@@ -344,14 +348,14 @@ int sparq_sort(sparq_state_t h, const sparq_field_t* last) {
     return guarded([&] {
         State& st = S(h);
         if (last) SPQ_REQUIRE(last->word < st.W && last->width >= 1 && (uint32_t)last->shift + last->width <= 64, "bad field");
-        if (st.n > 1) sort_by_plan(st, make_plan(st, last, 8));
+        sort_state(st, last);
     });
 }
 
 int sparq_merge_duplicates(sparq_state_t h, double eps) {
     return guarded([&] {
         State& st = S(h);
-        if (st.n > 1) sort_by_plan(st, make_plan(st, nullptr, 8));
+        sort_state(st, nullptr);
         merge_sorted_duplicates(st, eps);
     });
 }
@@ -382,7 +386,7 @@ int sparq_add_systems(sparq_state_t hc, sparq_state_t ha, double coef, double ep
             scale_amps(tail, coef);
         }
         if (orig != 0) {
-            sort_by_plan(cur, make_plan(cur, nullptr, 8));
+            sort_state(cur, nullptr);
             merge_sorted_duplicates(cur, eps);
         }
         // keep `add` alive until the copy has run
@@ -394,7 +398,7 @@ int sparq_add_systems(sparq_state_t hc, sparq_state_t ha, double coef, double ep
 int sparq_check_duplicates(sparq_state_t h, int* has_dup) {
     return guarded([&] {
         State& st = S(h);
-        if (st.n > 1) sort_by_plan(st, make_plan(st, nullptr, 8));
+        sort_state(st, nullptr);
         *has_dup = has_sorted_duplicates(st) ? 1 : 0;
     });
 }
@@ -412,7 +416,9 @@ int sparq_hadamard_int_full(sparq_state_t h, sparq_field_t target, const sparq_c
         State& st = S(h);
         // CheckDuplicateKey precedes the transform (hadamard.cpp:183)
         if (st.n > 1) {
-            sort_by_plan(st, make_plan(st, nullptr, 8));
+            // any total order of the keys makes duplicates adjacent; sorting with the
+            // target least significant also prepares the grouping sort below
+            sort_state(st, &target);
             if (has_sorted_duplicates(st)) throw Error(SPARQ_RUNTIME, "Hadamard_Int_Full: duplicate key in state");
         }
         with_conditions(st, conds, ncond, eps, [&](State& s) { hadamard_int(s, target, target.width, true, eps, s.n); });

@@ -42,15 +42,25 @@ __global__ void __launch_bounds__(kBlock) group_start_kernel(const uint32_t* __r
     if (i == 0) start[G[0]] = n;
 }
 
-__global__ void __launch_bounds__(kBlock) classify_kernel(const uint64_t* __restrict__ start, uint64_t G, uint64_t F, uint32_t nd, int full,
-                                                     uint8_t* __restrict__ cls) {
+// Class of each group (grid covers an upper bound of the group count; the
+// actual count is read from device memory).  counts[1] receives the number of
+// WHT groups so the butterfly passes can be skipped when there are none.
+__global__ void __launch_bounds__(kBlock) classify_kernel(const uint64_t* __restrict__ start, const uint64_t* __restrict__ G_dev, uint64_t F,
+                                                     uint32_t nd, int full, uint8_t* __restrict__ cls,
+                                                     unsigned long long* __restrict__ n_wht) {
+    const uint64_t G = G_dev[0];
     uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= G) return;
-    uint64_t cnt = start[g + 1] - start[g];
-    uint8_t c;
-    if (!full) c = (cnt == 1) ? kDirect : kWht;                        // hadamard.cpp:26-131
-    else c = (cnt == F) ? kWht : (cnt + 1 <= nd ? kSparseSum : kWht);  // hadamard.cpp:220-256 (few_threshold = n-1)
-    cls[g] = c;
+    bool wht = false;
+    if (g < G) {
+        uint64_t cnt = start[g + 1] - start[g];
+        uint8_t c;
+        if (!full) c = (cnt == 1) ? kDirect : kWht;                        // hadamard.cpp:26-131
+        else c = (cnt == F) ? kWht : (cnt + 1 <= nd ? kSparseSum : kWht);  // hadamard.cpp:220-256 (few_threshold = n-1)
+        cls[g] = c;
+        wht = c == kWht;
+    }
+    uint64_t b = __ballot(wht);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(n_wht, (unsigned long long)__popcll(b));
 }
 
 // Output entry o = g*F + j: key of the group head with the low target bits = j.
@@ -98,22 +108,80 @@ __global__ void __launch_bounds__(kBlock) expand_scatter_kernel(const uint64_t* 
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     // group id of entry i = inclusive count of heads up to i, minus one
-    uint64_t g = pos[i] + head[i] - 1;
+    uint64_t g = head ? pos[i] + head[i] - 1 : 0;
     if (cls[g] != kWht) return;
     uint64_t k[W];
     load_key<W>(keys, cap, i, k);
     aout[(g << nd) + fget<W>(k, low)] = amps[i];
 }
 
-// WHT stages [lo, lo+cnt) over blocks of F=2^nd, on tiles of 2^cnt x 2^c2
-// elements (c2 contiguous low index bits, c2 <= lo).  Butterfly per stage:
-// (a0+a1)*s, (a0-a1)*s, stages in ascending order (hadamard.cpp:44-60, 118-130).
-constexpr int kWhtTileLog = 11;   // 2048 complex doubles = 32 KiB of LDS
+// Walsh-Hadamard butterflies.  Stage q pairs indices i, i + 2^q of a group's
+// block and maps (a0, a1) -> ((a0+a1)*s, (a0-a1)*s), stages in ascending order
+// (hadamard.cpp:44-60, 118-130); every element sees the same operation
+// sequence as in the reference, so the results are bit-identical.
+//
+// A tile of up to 2^11 complex doubles (32 KiB) sits in LDS; each thread
+// applies up to 3 consecutive stages to 8 elements in registers per round.
+constexpr int kWhtTileLog = 11;
+
+// Stages [s0, s0+cnt) of tile bits, applied in rounds of <= 3 stages.  Groups
+// of 2^k elements differing only in the round's stage bits are handled by one
+// thread; `base_index` is the tile's first index in the output array (for the
+// class lookup when a tile holds several groups).
+__device__ __forceinline__ void lds_wht_rounds(double2* buf, uint32_t T, uint32_t s0, uint32_t cnt, bool check_class,
+                                               uint64_t base_index, uint32_t nd, const uint8_t* __restrict__ cls) {
+    for (uint32_t done = 0; done < cnt;) {
+        const uint32_t k = min(3u, cnt - done);
+        const uint32_t sb = s0 + done;
+        const uint32_t ngroups = T >> k;
+        for (uint32_t q = threadIdx.x; q < ngroups; q += kBlock) {
+            const uint32_t lowq = q & ((1u << sb) - 1);
+            const uint32_t first = ((q >> sb) << (sb + k)) | lowq;
+            if (check_class && cls[(base_index + first) >> nd] != 2) continue;
+            double2 v[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if (j < (1u << k)) v[j] = buf[first | (j << sb)];
+#pragma unroll
+            for (uint32_t r = 0; r < 3; ++r) {
+                if (r >= k) break;
+#pragma unroll
+                for (uint32_t j = 0; j < 8; ++j) {
+                    if (j >= (1u << k) || (j >> r) & 1) continue;
+                    const uint32_t j1 = j | (1u << r);
+                    double2 a0 = v[j], a1 = v[j1];
+                    v[j] = cscale(cadd(a0, a1), kSqrt2Inv);
+                    v[j1] = cscale(csub(a0, a1), kSqrt2Inv);
+                }
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if (j < (1u << k)) buf[first | (j << sb)] = v[j];
+        }
+        __syncthreads();
+        done += k;
+    }
+}
+
+// First pass: stages [0, min(nd, 11)) on contiguous tiles of 2^11 elements of the
+// output array (a tile may hold many small groups; non-WHT groups are skipped).
+__global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restrict__ a, uint64_t M, uint32_t nd, uint32_t cnt,
+                                                           const uint8_t* __restrict__ cls) {
+    __shared__ double2 buf[1 << kWhtTileLog];
+    const uint64_t base = (uint64_t)blockIdx.x << kWhtTileLog;
+    const uint32_t T = (uint32_t)min<uint64_t>(uint64_t(1) << kWhtTileLog, M - base);
+    for (uint32_t e = threadIdx.x; e < T; e += kBlock) buf[e] = a[base + e];
+    __syncthreads();
+    lds_wht_rounds(buf, T, 0, cnt, true, base, nd, cls);
+    for (uint32_t e = threadIdx.x; e < T; e += kBlock) a[base + e] = buf[e];
+}
+
+// Later passes: stages [lo, lo+cnt) of one group, on tiles of 2^cnt x 2^c2
+// elements (c2 contiguous low index bits, c2 <= lo).
 __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ a, uint64_t M, uint32_t nd, uint32_t lo, uint32_t cnt,
                                                      uint32_t c2, const uint8_t* __restrict__ cls) {
     __shared__ double2 buf[1 << kWhtTileLog];
     const uint32_t T = 1u << (cnt + c2);
-    // tile t -> (group g, high part, low-chunk)
     const uint64_t tile = blockIdx.x;
     const uint64_t per_group = (uint64_t(1) << nd) >> (cnt + c2);
     const uint64_t g = tile / per_group;
@@ -123,58 +191,86 @@ __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ 
     const uint64_t nlowchunks = (uint64_t(1) << lo) >> c2;
     const uint64_t lowchunk = r % nlowchunks, high = r / nlowchunks;
     const uint64_t base = (g << nd) + (high << (lo + cnt)) + (lowchunk << c2);
-    // element e of tile: sub = e >> c2 (stage index bits), l = e & (2^c2-1)
+    // element e of the tile: sub = e >> c2 (stage index bits), l = e & (2^c2-1)
     for (uint32_t e = threadIdx.x; e < T; e += kBlock) {
         uint64_t sub = e >> c2, l = e & ((1u << c2) - 1);
         buf[e] = a[base + (sub << lo) + l];
     }
     __syncthreads();
-    for (uint32_t s = 0; s < cnt; ++s) {
-        for (uint32_t p = threadIdx.x; p < T / 2; p += kBlock) {
-            // insert a zero at bit (s + c2) of p
-            uint32_t bitpos = s + c2;
-            uint32_t lowp = p & ((1u << bitpos) - 1);
-            uint32_t i0 = ((p >> bitpos) << (bitpos + 1)) | lowp;
-            uint32_t i1 = i0 | (1u << bitpos);
-            double2 a0 = buf[i0], a1 = buf[i1];
-            buf[i0] = cscale(cadd(a0, a1), kSqrt2Inv);
-            buf[i1] = cscale(csub(a0, a1), kSqrt2Inv);
-        }
-        __syncthreads();
-    }
+    lds_wht_rounds(buf, T, c2, cnt, false, 0, nd, cls);
     for (uint32_t e = threadIdx.x; e < T; e += kBlock) {
         uint64_t sub = e >> c2, l = e & ((1u << c2) - 1);
         a[base + (sub << lo) + l] = buf[e];
     }
 }
 
+__global__ void single_group_kernel(uint64_t* __restrict__ start, uint8_t* __restrict__ cls, uint64_t n, uint8_t c) {
+    start[0] = 0;
+    start[1] = n;
+    cls[0] = c;
+}
+
 template <int W>
 static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps) {
     const uint64_t n = st.n;
-    sparq_field_t low{target.word, target.shift, (uint8_t)nd, 0};
-    if (n > 1) sort_by_plan(st, make_plan(st, &low, 8));
-    uint32_t* head = dalloc<uint32_t>(n, st.stream);
-    uint64_t* pos = dalloc<uint64_t>(n + 1, st.stream);
-    {
-        SPQ_PROF(st.stream, "head_flag_kernel", n * (8.0 * W + 4));
-        head_flag_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, n, low, head);
-    }
-    SPQ_HIP(hipGetLastError());
-    exclusive_scan_u32(st, head, pos, n, pos + n);
-    const uint64_t G = read_u64(st, pos + n);
-    uint64_t* start = dalloc<uint64_t>(G + 1, st.stream);
-    {
-        SPQ_PROF(st.stream, "group_start_kernel", n * 12.0 + G * 8.0);
-        group_start_kernel<<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(head, pos, n, start, pos + n);
-    }
-    SPQ_HIP(hipGetLastError());
     const uint64_t F = uint64_t(1) << nd;
-    uint8_t* cls = dalloc<uint8_t>(G, st.stream);
-    {
-        SPQ_PROF(st.stream, "classify_kernel", G * 17.0);
-        classify_kernel<<<grid_for(G, kBlock), kBlock, 0, st.stream>>>(start, G, F, nd, full ? 1 : 0, cls);
+    sparq_field_t low{target.word, target.shift, (uint8_t)nd, 0};
+    // Sort by (everything else, low target bits) unless already in that order;
+    // when no bit outside the low target bits varies there is a single group.
+    bool single = (n == 1);
+    if (n > 1) {
+        KeyStats ks = key_stats(st, &low);
+        if (!ks.sorted) sort_by_plan(st, make_plan(ks, st.W, &low, 8));
+        single = true;
+        for (uint32_t w = 0; w < st.W; ++w) {
+            uint64_t v = ks.vary[w];
+            if (w == low.word) v &= ~(fmask_host(nd) << low.shift);
+            if (v) single = false;
+        }
     }
-    SPQ_HIP(hipGetLastError());
+    uint32_t* head = nullptr;
+    uint64_t* pos = nullptr;
+    uint64_t* start = nullptr;
+    uint8_t* cls = nullptr;
+    uint64_t G, n_wht;
+    if (single) {
+        uint8_t c;
+        if (!full) c = (n == 1) ? kDirect : kWht;
+        else c = (n == F) ? kWht : (n + 1 <= nd ? kSparseSum : kWht);
+        G = 1;
+        n_wht = (c == kWht) ? 1 : 0;
+        start = dalloc<uint64_t>(2, st.stream);
+        cls = dalloc<uint8_t>(1, st.stream);
+        single_group_kernel<<<1, 1, 0, st.stream>>>(start, cls, n, c);
+        SPQ_HIP(hipGetLastError());
+    } else {
+        head = dalloc<uint32_t>(n, st.stream);
+        pos = dalloc<uint64_t>(n + 2, st.stream);   // [n] = group count, [n+1] = WHT group count
+        {
+            SPQ_PROF(st.stream, "head_flag_kernel", n * (8.0 * W + 4));
+            head_flag_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, n, low, head);
+        }
+        SPQ_HIP(hipGetLastError());
+        exclusive_scan_u32(st, head, pos, n, pos + n);
+        start = dalloc<uint64_t>(n + 1, st.stream);
+        {
+            SPQ_PROF(st.stream, "group_start_kernel", n * 12.0);
+            group_start_kernel<<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(head, pos, n, start, pos + n);
+        }
+        SPQ_HIP(hipGetLastError());
+        cls = dalloc<uint8_t>(n, st.stream);
+        SPQ_HIP(hipMemsetAsync(pos + n + 1, 0, sizeof(uint64_t), st.stream));
+        {
+            SPQ_PROF(st.stream, "classify_kernel", n * 17.0);
+            classify_kernel<<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(start, pos + n, F, nd, full ? 1 : 0, cls,
+                                                                          (unsigned long long*)(pos + n + 1));
+        }
+        SPQ_HIP(hipGetLastError());
+        uint64_t h2[2];
+        read_back(st, pos + n, sizeof(h2), h2);
+        G = h2[0];
+        n_wht = h2[1];
+    }
     SPQ_REQUIRE(G <= (~0ull >> nd), "Hadamard output size overflows");
     const uint64_t M = G << nd;
     uint64_t* kout;
@@ -182,29 +278,37 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     alloc_arrays(st, st.W, M, &kout, &aout);
     const double extra = 1.0 / std::sqrt((double)F);
     {
-        SPQ_PROF(st.stream, "expand_init_kernel", M * (8.0 * W + 16) + G * (8.0 * W + 16));
+        SPQ_PROF(st.stream, "expand_init_kernel", M * (8.0 * W + 16));
         expand_init_kernel<W><<<grid_for(M, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, start, cls, nd, low, extra, M, kout,
                                                                          aout, M);
     }
     SPQ_HIP(hipGetLastError());
-    {
-        SPQ_PROF(st.stream, "expand_scatter_kernel", n * (8.0 * W + 44));
-        expand_scatter_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, head, pos, cls, nd, low, aout);
-    }
-    SPQ_HIP(hipGetLastError());
-    // Butterfly passes: first pass covers min(nd, 11) stages on contiguous tiles,
-    // later passes take up to 7 stages on 16-wide column tiles.
-    uint32_t lo = 0;
-    while (lo < nd) {
-        uint32_t c2 = (lo == 0) ? 0 : std::min<uint32_t>(4, lo);
-        uint32_t cnt = std::min<uint32_t>(nd - lo, kWhtTileLog - c2);
-        uint64_t tiles = M >> (cnt + c2);
+    if (n_wht) {
         {
-            SPQ_PROF(st.stream, "wht_pass_kernel", M * 32.0);
-            wht_pass_kernel<<<(unsigned)tiles, kBlock, 0, st.stream>>>(aout, M, nd, lo, cnt, c2, cls);
+            SPQ_PROF(st.stream, "expand_scatter_kernel", n * (8.0 * W + 16));
+            expand_scatter_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, head, pos, cls, nd, low,
+                                                                                     aout);
         }
         SPQ_HIP(hipGetLastError());
-        lo += cnt;
+        // Butterfly passes: the first covers min(nd, 11) stages on contiguous tiles,
+        // later ones up to 7 stages on 16-wide column tiles of one group.
+        uint32_t lo = std::min<uint32_t>(nd, kWhtTileLog);
+        {
+            SPQ_PROF(st.stream, "wht_first_pass_kernel", (double)n_wht * F * 32.0);
+            wht_first_pass_kernel<<<grid_for(M, 1u << kWhtTileLog), kBlock, 0, st.stream>>>(aout, M, nd, lo, cls);
+        }
+        SPQ_HIP(hipGetLastError());
+        while (lo < nd) {
+            uint32_t c2 = std::min<uint32_t>(4, lo);
+            uint32_t cnt = std::min<uint32_t>(nd - lo, kWhtTileLog - c2);
+            uint64_t tiles = M >> (cnt + c2);
+            {
+                SPQ_PROF(st.stream, "wht_pass_kernel", (double)n_wht * F * 32.0);
+                wht_pass_kernel<<<(unsigned)tiles, kBlock, 0, st.stream>>>(aout, M, nd, lo, cnt, c2, cls);
+            }
+            SPQ_HIP(hipGetLastError());
+            lo += cnt;
+        }
     }
     install_arrays(st, st.W, kout, aout, M, M);
     dfree(head, st.stream);
@@ -361,7 +465,7 @@ template <int W>
 static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p) {
     const uint64_t n = st.n;
     sparq_field_t b{target.word, (uint8_t)(target.shift + bit), 1, 0};
-    if (n > 1) sort_by_plan(st, make_plan(st, &b, 8));
+    sort_state(st, &b);
     const uint64_t bm = uint64_t(1) << b.shift;
     uint32_t* cnt = dalloc<uint32_t>(n, st.stream);
     uint64_t* off = dalloc<uint64_t>(n + 1, st.stream);

@@ -51,6 +51,7 @@ struct State {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     uint64_t* pinned = nullptr; // 64 words of pinned host memory for small readbacks
+    unsigned int* counter = nullptr;   // device word, zero between kernels (last-block reductions)
     int device = 0;
 };
 
@@ -66,6 +67,8 @@ void install_arrays(State& st, uint32_t W, uint64_t* keys, double2* amps, uint64
 // Read `count` uint64 values from device memory (synchronizes the stream).
 void read_back(State& st, const void* dptr, size_t bytes, void* host);
 uint64_t read_u64(State& st, const uint64_t* dptr);
+
+inline uint64_t fmask_host(uint32_t width) { return width >= 64 ? ~uint64_t(0) : ((uint64_t(1) << width) - 1); }
 
 inline unsigned grid_for(uint64_t n, unsigned per_block) {
     uint64_t g = (n + per_block - 1) / per_block;
@@ -84,11 +87,17 @@ struct Digit { uint8_t word, shift, bits, pad; };
 // Stable LSD radix sort of the state by the given digit plan (least significant
 // digit first).  Reorders keys and amps.
 void sort_by_plan(State& st, const std::vector<Digit>& plan);
-// Plan for the key with optional field moved to the least significant place,
-// restricted to bits that vary across the state.
-std::vector<Digit> make_plan(State& st, const sparq_field_t* last, int digit_bits);
-// Bitwise OR / AND over all keys, per word (for the varying-bit mask).
-void key_or_and(State& st, std::vector<uint64_t>& or_mask, std::vector<uint64_t>& and_mask);
+// One pass over the keys: bits that vary across the state, and whether the
+// state is already ordered with the optional field `last` least significant.
+struct KeyStats {
+    std::vector<uint64_t> vary;
+    bool sorted = true;
+};
+KeyStats key_stats(State& st, const sparq_field_t* last);
+// LSD plan over the varying bits with `last` moved to the least significant place.
+std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t* last, int digit_bits);
+// Sort unless already ordered; returns whether a sort ran.
+bool sort_state(State& st, const sparq_field_t* last);
 
 // ---- compaction (compact.hip)
 void clear_zero(State& st, double eps);
@@ -108,6 +117,9 @@ void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_par
 // ---- pointwise (pointwise.hip)
 void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_cond_t* conds, uint32_t nconds);
 void relayout(State& st, uint32_t new_words, const sparq_field_t* from, const sparq_field_t* to, uint32_t nf);
+struct BasisKey { uint64_t w[SPARQ_MAX_WORDS]; };
+// Entry 0 := |key>, amplitude 1 (the state's arrays must hold at least one entry).
+void write_basis(State& st, const BasisKey& k);
 double sum_abs2(State& st);
 void scale_amps(State& st, double s);
 bool field_uniform(State& st, sparq_field_t f, uint64_t* value);

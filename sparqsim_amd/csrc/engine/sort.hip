@@ -30,73 +30,162 @@ __device__ __forceinline__ uint64_t match_any(uint32_t d, int nbits) {
     return m;
 }
 
-// ---------------------------------------------------------------- OR / AND
+// ---------------------------------------------------------------- key statistics
+// One read pass over the keys: bitwise OR and AND per word (bits that vary
+// across the state) and whether the state is already ordered by
+// (all bits except `last`, then `last`).  Per-block partials are combined by
+// the last block to finish (no spin waits).
+struct StatsOrder {
+    uint64_t others[SPARQ_MAX_WORDS];   // per-word mask of the primary comparison bits
+    sparq_field_t last;                 // width 0: none
+};
+
 template <int W>
-__global__ void __launch_bounds__(kBlock) or_and_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n,
-                                                   unsigned long long* __restrict__ out /* [W] or, [W] and */) {
-    uint64_t o[W], a[W];
+__device__ __forceinline__ bool out_of_order(const uint64_t (&p)[W], const uint64_t (&c)[W], const StatsOrder& so) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint64_t a = p[w] & so.others[w], b = c[w] & so.others[w];
+        if (a != b) return a > b;
+    }
+    if (so.last.width) return fget<W>(p, so.last) > fget<W>(c, so.last);
+    return false;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) key_stats_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, StatsOrder so,
+                                                      uint64_t* __restrict__ partial, unsigned int* __restrict__ counter,
+                                                      uint64_t* __restrict__ out /* [W] or, [W] and, [1] unsorted */) {
+    __shared__ uint64_t lds[4][2 * W + 1];
+    __shared__ bool is_last;
+    uint64_t o[W], a[W], bad = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) { o[w] = 0; a[w] = ~0ull; }
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        uint64_t k[W];
+        load_key<W>(keys, cap, i, k);
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            uint64_t v = keys[(uint64_t)w * cap + i];
-            o[w] |= v;
-            a[w] &= v;
+        for (int w = 0; w < W; ++w) { o[w] |= k[w]; a[w] &= k[w]; }
+        if (i > 0 && !bad) {
+            uint64_t p[W];
+            load_key<W>(keys, cap, i - 1, p);
+            bad = out_of_order<W>(p, k, so) ? 1 : 0;
         }
     }
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
+    for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
+        for (int w = 0; w < W; ++w) {
             o[w] |= __shfl_xor(o[w], off, 64);
             a[w] &= __shfl_xor(a[w], off, 64);
         }
+        bad |= __shfl_xor(bad, off, 64);
     }
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            atomicOr(&out[w], (unsigned long long)o[w]);
-            atomicAnd(&out[W + w], (unsigned long long)a[w]);
+        for (int w = 0; w < W; ++w) { lds[wave][w] = o[w]; lds[wave][W + w] = a[w]; }
+        lds[wave][2 * W] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t r[2 * W + 1];
+#pragma unroll
+        for (int j = 0; j < W; ++j) { r[j] = 0; r[W + j] = ~0ull; }
+        r[2 * W] = 0;
+        for (int v = 0; v < 4; ++v) {
+            for (int j = 0; j < W; ++j) { r[j] |= lds[v][j]; r[W + j] &= lds[v][W + j]; }
+            r[2 * W] |= lds[v][2 * W];
+        }
+        for (int j = 0; j < 2 * W + 1; ++j) partial[(uint64_t)blockIdx.x * (2 * W + 1) + j] = r[j];
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        is_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (is_last) {
+        // the last block combines all partials, one block-strided slice per thread
+        __threadfence();
+        uint64_t r[2 * W + 1];
+#pragma unroll
+        for (int j = 0; j < W; ++j) { r[j] = 0; r[W + j] = ~0ull; }
+        r[2 * W] = 0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) {
+#pragma unroll
+            for (int j = 0; j < 2 * W + 1; ++j) {
+                uint64_t v = __hip_atomic_load(&partial[(uint64_t)b * (2 * W + 1) + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (j < W) r[j] |= v;
+                else if (j < 2 * W) r[j] &= v;
+                else r[j] |= v;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                r[j] |= __shfl_xor(r[j], off, 64);
+                r[W + j] &= __shfl_xor(r[W + j], off, 64);
+            }
+            r[2 * W] |= __shfl_xor(r[2 * W], off, 64);
+        }
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int j = 0; j < 2 * W + 1; ++j) lds[wave][j] = r[j];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int v = 1; v < 4; ++v)
+                for (int j = 0; j < 2 * W + 1; ++j) {
+                    if (j < W) r[j] |= lds[v][j];
+                    else if (j < 2 * W) r[j] &= lds[v][j];
+                    else r[j] |= lds[v][j];
+                }
+            for (int j = 0; j < 2 * W + 1; ++j) out[j] = r[j];
+            *counter = 0;   // ready for the next launch on this stream
         }
     }
 }
 
 template <int W>
-static void launch_or_and(State& st, unsigned long long* out) {
-    unsigned g = std::min<unsigned>(grid_for(st.n, kBlock * 4), 2048);
+static void launch_key_stats(State& st, const StatsOrder& so, uint64_t* partial, uint64_t* out) {
+    unsigned g = std::min<unsigned>(grid_for(st.n, kBlock * 8), 512);
     {
-        SPQ_PROF(st.stream, "or_and_kernel", st.n * 8.0 * W);
-        or_and_kernel<W><<<g, kBlock, 0, st.stream>>>(st.keys, st.cap, st.n, out);
+        SPQ_PROF(st.stream, "key_stats_kernel", st.n * 8.0 * W);
+        key_stats_kernel<W><<<g, kBlock, 0, st.stream>>>(st.keys, st.cap, st.n, so, partial, st.counter, out);
     }
     SPQ_HIP(hipGetLastError());
 }
 
-void key_or_and(State& st, std::vector<uint64_t>& or_mask, std::vector<uint64_t>& and_mask) {
-    or_mask.assign(st.W, 0);
-    and_mask.assign(st.W, ~0ull);
-    if (st.n == 0) return;
-    unsigned long long* d = dalloc<unsigned long long>(2 * st.W, st.stream);
-    std::vector<unsigned long long> init(2 * st.W);
-    for (uint32_t w = 0; w < st.W; ++w) { init[w] = 0; init[st.W + w] = ~0ull; }
-    SPQ_HIP(hipMemcpyAsync(d, init.data(), init.size() * 8, hipMemcpyHostToDevice, st.stream));
-    SPQ_DISPATCH_W(st.W, launch_or_and, st, d);
-    std::vector<unsigned long long> h(2 * st.W);
-    read_back(st, d, h.size() * 8, h.data());
-    dfree(d, st.stream);
-    for (uint32_t w = 0; w < st.W; ++w) { or_mask[w] = h[w]; and_mask[w] = h[st.W + w]; }
+KeyStats key_stats(State& st, const sparq_field_t* last) {
+    KeyStats ks;
+    ks.vary.assign(st.W, 0);
+    ks.sorted = true;
+    if (st.n == 0) return ks;
+    StatsOrder so{};
+    for (uint32_t w = 0; w < SPARQ_MAX_WORDS; ++w) so.others[w] = ~0ull;
+    if (last) {
+        so.last = *last;
+        so.others[last->word] &= ~(fmask_host(last->width) << last->shift);
+    }
+    uint64_t* partial = dalloc<uint64_t>((size_t)512 * (2 * st.W + 1), st.stream);
+    uint64_t* out = dalloc<uint64_t>(2 * st.W + 1, st.stream);
+    SPQ_DISPATCH_W(st.W, launch_key_stats, st, so, partial, out);
+    std::vector<uint64_t> h(2 * st.W + 1);
+    read_back(st, out, h.size() * 8, h.data());
+    dfree(partial, st.stream);
+    dfree(out, st.stream);
+    for (uint32_t w = 0; w < st.W; ++w) ks.vary[w] = h[w] ^ h[st.W + w];
+    ks.sorted = h[2 * st.W] == 0;
+    return ks;
 }
 
-std::vector<Digit> make_plan(State& st, const sparq_field_t* last, int digit_bits) {
-    std::vector<uint64_t> om, am;
-    key_or_and(st, om, am);
-    std::vector<uint64_t> vary(st.W);
-    for (uint32_t w = 0; w < st.W; ++w) vary[w] = om[w] ^ am[w];
+std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t* last, int digit_bits) {
+    const std::vector<uint64_t>& vary = ks.vary;
     // Segments of eligible bits in LSD order: (word, lo, hi).
     struct Seg { int word, lo, hi; };
     std::vector<Seg> segs;
     if (last) segs.push_back({last->word, last->shift, last->shift + last->width});
-    for (int w = (int)st.W - 1; w >= 0; --w) {
+    for (int w = (int)W - 1; w >= 0; --w) {
         if (last && last->word == w) {
             if (last->shift > 0) segs.push_back({w, 0, last->shift});
             if (last->shift + last->width < 64) segs.push_back({w, last->shift + last->width, 64});
@@ -115,6 +204,14 @@ std::vector<Digit> make_plan(State& st, const sparq_field_t* last, int digit_bit
         }
     }
     return plan;
+}
+
+bool sort_state(State& st, const sparq_field_t* last) {
+    if (st.n <= 1) return false;
+    KeyStats ks = key_stats(st, last);
+    if (ks.sorted) return false;
+    sort_by_plan(st, make_plan(ks, st.W, last, 8));
+    return true;
 }
 
 // ---------------------------------------------------------------- passes

@@ -194,6 +194,20 @@ void relayout(State& st, uint32_t new_words, const sparq_field_t* from, const sp
     st.W = new_words;
 }
 
+// ------------------------------------------------------------------ basis state
+__global__ void basis_kernel(uint64_t* __restrict__ keys, double2* __restrict__ amps, uint64_t cap, uint32_t W, BasisKey k) {
+    if (threadIdx.x < W) keys[(uint64_t)threadIdx.x * cap] = k.w[threadIdx.x];
+    if (threadIdx.x == 0) amps[0] = make_double2(1.0, 0.0);
+}
+
+void write_basis(State& st, const BasisKey& k) {
+    {
+        SPQ_PROF(st.stream, "basis_kernel", 8.0 * st.W + 16);
+        basis_kernel<<<1, 64, 0, st.stream>>>(st.keys, st.amps, st.cap, st.W, k);
+    }
+    SPQ_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ reductions
 __global__ void __launch_bounds__(kBlock) sum_abs2_kernel(const double2* __restrict__ amps, uint64_t n, double* __restrict__ partial) {
     __shared__ double lds[kBlock / 64];

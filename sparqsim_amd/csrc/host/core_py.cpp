@@ -275,6 +275,7 @@ PYBIND11_MODULE(_core, m) {
     py::class_<CombineRegister>(m, "CombineRegister")
         .def(py::init<std::string_view, std::string_view>()).def(py::init<size_t, size_t>())
         .def("__call__", [](const CombineRegister& a, SparseState& s) { return a(s); }, Release());
+    m.def("reset_systems", [](SparseState& s) { reset_systems(s); }, Release());
     m.def("add_systems", [](SparseState& cur, SparseState& add, double coef) { add_systems(cur, add, coef); }, Release());
 
     m.def("device_name", []() {

@@ -117,6 +117,7 @@ void System::set_size(size_t id, size_t size) {
 void System::set_type(size_t id, StateStorageType t) {
     if (id >= name_register_map.size()) throw std::runtime_error("Register not found.");
     std::get<1>(name_register_map[id]) = t;
+    ++layout_version;   // the signed flag of the field follows the type
 }
 
 // ------------------------------------------------------------ layout
@@ -205,7 +206,9 @@ void SparseState::flush() {
 }
 
 const Layout& SparseState::sync() {
+    if (layout_version_ == System::layout_version) return layout_;
     Layout T = Layout::from_registry();
+    layout_version_ = System::layout_version;
     if (T == layout_) return layout_;
     flush();
     std::vector<sparq_field_t> from, to;
@@ -236,6 +239,7 @@ void SparseState::relayout_with(const Layout& target, const std::vector<std::pai
     }
     check(sparq_state_relayout(h_, target.words, from.data(), to.data(), (uint32_t)from.size()));
     layout_ = target;
+    layout_version_ = System::layout_version;
 }
 
 static bool same_conds(const sparq_cond_t* a, const sparq_cond_t* b, size_t n) {
@@ -858,6 +862,12 @@ void add_systems(SparseState& current, SparseState& new_state, double coef) {   
     current.flush();
     new_state.flush();
     check(sparq_add_systems(current.handle(), new_state.handle(), coef, epsilon));
+}
+
+void reset_systems(SparseState& state) {
+    state.sync();
+    state.flush();
+    check(sparq_state_set_basis(state.handle(), nullptr));
 }
 
 }  // namespace sparqsim

@@ -118,6 +118,7 @@ public:
 private:
     sparq_state_t h_ = nullptr;
     Layout layout_;
+    uint64_t layout_version_ = ~0ull;   // System::layout_version the layout was checked against
     std::vector<sparq_pw_op_t> pend_ops_;
     std::vector<sparq_cond_t> pend_conds_;
 };
@@ -429,5 +430,7 @@ struct CombineRegister {
 
 // add_systems (system_operations.h:157-181)
 void add_systems(SparseState& current, SparseState& new_state, double coef);
+// reset_systems (system_operations.cpp:108-117): back to the single entry |0...0>, amplitude 1.
+void reset_systems(SparseState& state);
 
 }  // namespace sparqsim

@@ -32,17 +32,30 @@ def qram_memory(address_bits: int, data_bits: int, config_index: int) -> list[in
     return words.tolist()
 
 
+class QRAMInterference:
+    """Single-state QRAM interference on registers addr (A bits) and data (D bits):
+    H(addr); data ^= M[addr]; phase flip on the parity of data (one Z per data
+    bit); uncompute the load; H(addr).  The registers must already exist.
+
+    `entry_ops` sums max(N_in, N_out) over the reference operators of one pass
+    (2^A entries for each of the 4 + D operators)."""
+
+    def __init__(self, mod, qram, address_bits: int, data_bits: int):
+        self.mod = mod
+        self.ops = [mod.Hadamard_Int("addr", address_bits), mod.QRAMLoad(qram, "addr", "data")]
+        self.ops += [mod.Phase_Bool("data", i, np.pi) for i in range(data_bits)]
+        self.ops += [mod.QRAMLoad(qram, "addr", "data"), mod.Hadamard_Int("addr", address_bits)]
+        self.entry_ops = (1 << address_bits) * (4 + data_bits)
+
+    def __call__(self, state):
+        for op in self.ops:
+            op(state)
+        return state
+
+
 def qram_interference(mod, qram, address_bits: int, data_bits: int):
-    """Single-state QRAM interference: H(addr); data ^= M[addr]; phase flip on the
-    parity of data (one Z per data bit); uncompute the load; H(addr).
-    Returns (state, entry-ops): entry-ops sums max(N_in, N_out) over the operators."""
+    """Run one pass on a fresh |0> state; returns (state, entry-ops)."""
+    circ = QRAMInterference(mod, qram, address_bits, data_bits)
     st = mod.SparseState()
-    n = 1 << address_bits
-    mod.Hadamard_Int("addr", address_bits)(st)
-    load = mod.QRAMLoad(qram, "addr", "data")
-    load(st)
-    for i in range(data_bits):
-        mod.Phase_Bool("data", i, np.pi)(st)
-    load(st)
-    mod.Hadamard_Int("addr", address_bits)(st)
-    return st, n * (4 + data_bits)
+    circ(st)
+    return st, circ.entry_ops
