@@ -120,6 +120,9 @@ enum sparq_pw_opcode {
                                                                                   (parallel_phase_operations.cpp:11-70) */
     SPARQ_PW_MUL_AMP,             /* amp *= d0                                    GlobalPhase (parallel_phase_operations.cpp:72-108) */
     SPARQ_PW_SET_FIELD,           /* f0 = c0                                      Init_Unsafe (dark_magic.cpp:35-41) */
+    SPARQ_PW_INC_WRAP,            /* f0 += 1 mod 2^w; on wrap f1 = (f1+1)%2       PlusOneAndOverflow
+                                     (SparQ_Algorithm/src/BlockEncoding/block_encoding_tridiagonal.cpp:20-47) */
+    SPARQ_PW_DEC_WRAP,            /* f0 -= 1 mod 2^w; on wrap f1 = (f1+1)%2       PlusOneAndOverflow::dag (:55-84) */
     SPARQ_PW_OPCODE_END
 };
 
@@ -194,6 +197,13 @@ int sparq_hadamard_int(sparq_state_t st, sparq_field_t target, uint32_t n_digits
 /* Hadamard_Int_Full (hadamard.cpp:181-338) incl. its CheckDuplicateKey, then ClearZero(eps). */
 int sparq_hadamard_int_full(sparq_state_t st, sparq_field_t target,
                             const sparq_cond_t* conds, uint32_t ncond, double eps);
+
+/* Rot_GeneralUnitary (SparQ/src/rot.cpp:340-467): every group of entries equal on all
+ * registers but `target` is completed to 2^w entries and multiplied by the dense
+ * 2^w x 2^w complex matrix `mat` (row-major, 2 doubles per element; w <= 6), then
+ * ClearZero(eps).  Conditions are evaluated per entry. */
+int sparq_group_unitary(sparq_state_t st, sparq_field_t target, const double* mat,
+                        const sparq_cond_t* conds, uint32_t ncond, double eps);
 
 /* Two-branch operators on bit `bit` of `target` (pairs of entries equal elsewhere). */
 enum sparq_pair_kind {

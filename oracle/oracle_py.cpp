@@ -150,6 +150,18 @@ PYBIND11_MODULE(_oracle, m) {
     m.def("make_func_inv", &make_func_inv);
     m.def("Rot_Bool", [](py::object r, size_t d, py::object mat) { return wrap(Rot_Bool(rid(r), d, to_u22(mat)), "Rot_Bool"); },
           py::arg("reg"), py::arg("digit"), py::arg("matrix"));
+    m.def("Rot_GeneralUnitary", [](py::object r, std::vector<std::vector<complex_t>> m) {
+        std::vector<complex_t> flat;
+        for (auto& row : m) flat.insert(flat.end(), row.begin(), row.end());
+        return wrap(Rot_GeneralUnitary(rid(r), flat), "Rot_GeneralUnitary");
+    });
+    m.def("Rot_GeneralStatePrep", [](py::object r, std::vector<complex_t> vec) {
+        size_t id = rid(r);
+        if (vec.size() != pow2(Registry::size_of(id))) throw std::invalid_argument("State size does not match the register's size.");
+        return wrap(Rot_GeneralUnitary(id, stateprep_unitary(vec)), "Rot_GeneralStatePrep");
+    });
+    m.def("stateprep_unitary", [](std::vector<complex_t> vec) { return stateprep_unitary(vec); });
+    m.def("PlusOneAndOverflow", [](py::object a, py::object o) { return PlusOneAndOverflow(rid(a), rid(o)); });
     // ---- structural
     m.def("ClearZero", [](py::object eps) {
         double e = eps.is_none() ? clearzero_default_eps() : eps.cast<double>();

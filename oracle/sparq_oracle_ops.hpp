@@ -306,6 +306,103 @@ inline OpPtr GlobalPhase(complex_t c) {
     return pw("GlobalPhase", [=](System& s) { s.amplitude *= c; }, [=](System& s) { s.amplitude *= std::conj(c); });
 }
 
+// ---- PlusOneAndOverflow (SparQ_Algorithm/src/BlockEncoding/block_encoding_tridiagonal.cpp:20-84)
+inline OpPtr PlusOneAndOverflow(size_t main, size_t overflow) {
+    return pw("PlusOneAndOverflow",
+        [=](System& s) {
+            uint64_t& v = s.get(main);
+            uint64_t& o = s.get(overflow);
+            if (v == pow2(Registry::size_of(main)) - 1) { o = (o + 1) % 2; v = 0; }
+            else v += 1;
+        },
+        [=](System& s) {
+            uint64_t& v = s.get(main);
+            uint64_t& o = s.get(overflow);
+            if (v == 0) { o = (o + 1) % 2; v = pow2(Registry::size_of(main)) - 1; }
+            else v -= 1;
+        });
+}
+
+// ---- Rot_GeneralUnitary (SparQ/src/rot.cpp:340-467): every bucket (entries equal on
+// all registers but `id`, conditions per entry) is completed with zero-amplitude
+// clones and multiplied by the dense 2^n x 2^n matrix (row i: sum over j ascending).
+struct Rot_GeneralUnitary {
+    size_t id, full_size;
+    std::vector<complex_t> mat;   // row-major full_size x full_size
+    Conditions cond;
+    Rot_GeneralUnitary(size_t id_, std::vector<complex_t> m) : id(id_), mat(std::move(m)) {
+        full_size = pow2(Registry::size_of(id));
+        if (mat.size() != full_size * full_size) throw_invalid_input("Matrix size does not match the register's size.");
+    }
+    void bucket(const std::vector<size_t>& pos, State& st, bool dagger) const {
+        std::vector<complex_t> amps(pos.size()), out(pos.size(), 0);
+        for (size_t i = 0; i < pos.size(); ++i) amps[i] = st[pos[i]].amplitude;
+        for (size_t i = 0; i < full_size; ++i)
+            for (size_t j = 0; j < full_size; ++j)
+                out[i] += (dagger ? std::conj(mat[j * full_size + i]) : mat[i * full_size + j]) * amps[j];
+        for (size_t i = 0; i < pos.size(); ++i) st[pos[i]].amplitude = out[i];
+    }
+    void apply(State& st, bool dagger) const {
+        if (st.empty()) return;
+        size_t cur = st.size();
+        std::map<System, std::pair<size_t, std::vector<size_t>>, StateLessExceptKey> buckets(StateLessExceptKey{id});
+        for (size_t i = 0; i < cur; ++i) {
+            if (!cond.satisfied(st[i])) continue;
+            const System& s = st[i];
+            auto res = buckets.insert({s, {0, std::vector<size_t>(full_size, size_t(-1))}});
+            auto it = res.first;
+            it->second.second[s.as_uint(id)] = i;
+            it->second.first += 1;
+            if (it->second.first == full_size) { bucket(it->second.second, st, dagger); buckets.erase(it); }
+        }
+        for (auto& kv : buckets) {
+            auto& pos = kv.second.second;
+            for (size_t i = 0; i < pos.size(); ++i)
+                if (pos[i] == size_t(-1)) {
+                    st.push_back(kv.first);
+                    st.back().get(id) = i;
+                    st.back().amplitude = 0;
+                    pos[i] = st.size() - 1;
+                }
+            bucket(pos, st, dagger);
+        }
+        ClearZero(st);
+        Registry::update_max_size(st.size());
+    }
+    void operator()(State& st) const { apply(st, false); }
+    void dag(State& st) const { apply(st, true); }
+};
+
+// stateprep_unitary_build_schmidt (SparQ/src/rot.cpp:469-503): Householder
+// reflection taking |0> to the normalised target, with the first column rephased.
+inline std::vector<complex_t> stateprep_unitary(const std::vector<complex_t>& target) {
+    const size_t N = target.size();
+    if (N == 0) throw_invalid_input("Cannot prepare an empty vector.");
+    double scale = 0;
+    for (auto& v : target) {
+        if (!std::isfinite(v.real()) || !std::isfinite(v.imag())) throw_invalid_input("State amplitudes must be finite.");
+        scale = std::max(scale, std::max(std::abs(v.real()), std::abs(v.imag())));
+    }
+    if (scale == 0) throw_invalid_input("Cannot prepare the zero vector.");
+    std::vector<complex_t> v(N);
+    double n2 = 0;
+    for (size_t i = 0; i < N; ++i) { v[i] = target[i] / scale; n2 += abs_sqr(v[i]); }
+    const double norm = std::sqrt(n2);
+    for (auto& x : v) x /= norm;
+    const double a0 = std::abs(v[0]);
+    const complex_t phase = a0 == 0 ? complex_t(1) : v[0] / a0;
+    v[0] += phase;
+    double vn2 = 0;
+    for (auto& x : v) vn2 += abs_sqr(x);
+    std::vector<complex_t> U(N * N);
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) {
+            U[i * N + j] = complex_t(i == j ? 1 : 0) - (2.0 / vn2) * v[i] * std::conj(v[j]);
+            if (j == 0) U[i * N + j] *= -phase;
+        }
+    return U;
+}
+
 // ---- wrappers making the structural/branching operators uniform
 template <class T, class = void> struct has_cond : std::false_type {};
 template <class T> struct has_cond<T, std::void_t<decltype(std::declval<T&>().cond)>> : std::true_type {};

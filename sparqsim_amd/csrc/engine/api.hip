@@ -425,6 +425,15 @@ int sparq_hadamard_int_full(sparq_state_t h, sparq_field_t target, const sparq_c
     });
 }
 
+int sparq_group_unitary(sparq_state_t h, sparq_field_t target, const double* mat, const sparq_cond_t* conds, uint32_t ncond,
+                        double eps) {
+    return guarded([&] {
+        SPQ_REQUIRE(mat != nullptr, "null matrix");
+        State& st = S(h);
+        with_conditions(st, conds, ncond, eps, [&](State& s) { group_unitary(s, target, mat, eps); });
+    });
+}
+
 int sparq_pair_op(sparq_state_t h, sparq_field_t target, uint32_t bit, const sparq_pair_params_t* params, const sparq_cond_t* conds,
                   uint32_t ncond, double eps) {
     return guarded([&] {

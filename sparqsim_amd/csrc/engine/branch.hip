@@ -16,7 +16,7 @@
 
 namespace sparq {
 
-enum GroupClass : uint8_t { kDirect = 0, kSparseSum = 1, kWht = 2 };
+enum GroupClass : uint8_t { kDirect = 0, kSparseSum = 1, kWht = 2, kDense = 3 };
 
 // ---------------------------------------------------------------- groups
 template <int W>
@@ -54,7 +54,8 @@ __global__ void __launch_bounds__(kBlock) classify_kernel(const uint64_t* __rest
     if (g < G) {
         uint64_t cnt = start[g + 1] - start[g];
         uint8_t c;
-        if (!full) c = (cnt == 1) ? kDirect : kWht;                        // hadamard.cpp:26-131
+        if (full == 2) c = kDense;                                         // rot.cpp:395-455
+        else if (!full) c = (cnt == 1) ? kDirect : kWht;                   // hadamard.cpp:26-131
         else c = (cnt == F) ? kWht : (cnt + 1 <= nd ? kSparseSum : kWht);  // hadamard.cpp:220-256 (few_threshold = n-1)
         cls[g] = c;
         wht = c == kWht;
@@ -69,7 +70,7 @@ template <int W>
 __global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
                                                         const uint64_t* __restrict__ start, const uint8_t* __restrict__ cls, uint32_t nd,
                                                         sparq_field_t low, double extra, uint64_t M, uint64_t* __restrict__ kout,
-                                                        double2* __restrict__ aout, uint64_t cap_out) {
+                                                        double2* __restrict__ aout, uint64_t cap_out, const double2* __restrict__ dense) {
     uint64_t o = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (o >= M) return;
     const uint64_t g = o >> nd, j = o & ((uint64_t(1) << nd) - 1);
@@ -94,6 +95,15 @@ __global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __r
             uint64_t v = fget<W>(km, low);
             double sg = (__popcll(j & v) & 1) ? -1. : 1.;
             r = cadd(r, cscale(cscale(amps[m], sg), extra));
+        }
+    } else if (c == kDense) {
+        // row j of the dense matrix against the members, ascending column (rot.cpp:351-358)
+        const uint64_t e = start[g + 1];
+        const uint64_t F = uint64_t(1) << nd;
+        for (uint64_t m = h; m < e; ++m) {
+            uint64_t km[W];
+            load_key<W>(keys, cap, m, km);
+            r = cadd(r, cmul(dense[j * F + fget<W>(km, low)], amps[m]));
         }
     }
     aout[o] = r;
@@ -211,7 +221,7 @@ __global__ void single_group_kernel(uint64_t* __restrict__ start, uint8_t* __res
 }
 
 template <int W>
-static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps) {
+static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, const double2* dense) {
     const uint64_t n = st.n;
     const uint64_t F = uint64_t(1) << nd;
     sparq_field_t low{target.word, target.shift, (uint8_t)nd, 0};
@@ -235,7 +245,8 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     uint64_t G, n_wht;
     if (single) {
         uint8_t c;
-        if (!full) c = (n == 1) ? kDirect : kWht;
+        if (dense) c = kDense;
+        else if (!full) c = (n == 1) ? kDirect : kWht;
         else c = (n == F) ? kWht : (n + 1 <= nd ? kSparseSum : kWht);
         G = 1;
         n_wht = (c == kWht) ? 1 : 0;
@@ -262,7 +273,7 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
         SPQ_HIP(hipMemsetAsync(pos + n + 1, 0, sizeof(uint64_t), st.stream));
         {
             SPQ_PROF(st.stream, "classify_kernel", n * 17.0);
-            classify_kernel<<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(start, pos + n, F, nd, full ? 1 : 0, cls,
+            classify_kernel<<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(start, pos + n, F, nd, dense ? 2 : (full ? 1 : 0), cls,
                                                                           (unsigned long long*)(pos + n + 1));
         }
         SPQ_HIP(hipGetLastError());
@@ -280,7 +291,7 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     {
         SPQ_PROF(st.stream, "expand_init_kernel", M * (8.0 * W + 16));
         expand_init_kernel<W><<<grid_for(M, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, start, cls, nd, low, extra, M, kout,
-                                                                         aout, M);
+                                                                         aout, M, dense);
     }
     SPQ_HIP(hipGetLastError());
     if (n_wht) {
@@ -323,7 +334,19 @@ void hadamard_int(State& st, sparq_field_t target, uint32_t nd, bool full, doubl
     SPQ_REQUIRE(nd >= 1 && nd <= target.width, "n_digits must be in [1, width]");
     SPQ_REQUIRE(nd <= 40, "n_digits too large for a sparse expansion");
     if (st.n == 0) return;
-    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, nd, full, eps);
+    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, nd, full, eps, (const double2*)nullptr);
+}
+
+void group_unitary(State& st, sparq_field_t target, const double* mat, double eps) {
+    SPQ_REQUIRE(target.word < st.W && target.width >= 1 && target.width <= 6 && (uint32_t)target.shift + target.width <= 64,
+                "group unitary needs a register of 1..6 bits");
+    if (st.n == 0) return;
+    const uint64_t F = uint64_t(1) << target.width;
+    double2* dense = dalloc<double2>(F * F, st.stream);
+    SPQ_HIP(hipMemcpyAsync(dense, mat, F * F * sizeof(double2), hipMemcpyHostToDevice, st.stream));
+    SPQ_HIP(hipStreamSynchronize(st.stream));   // the host matrix may be released after return
+    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, (uint32_t)target.width, false, eps, (const double2*)dense);
+    dfree(dense, st.stream);
 }
 
 // ---------------------------------------------------------------- replicate

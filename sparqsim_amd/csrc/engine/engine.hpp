@@ -112,6 +112,7 @@ uint64_t partition_by_conditions(State& st, const sparq_cond_t* conds, uint32_t 
 void hadamard_int(State& st, sparq_field_t target, uint32_t n_digits, bool full_variant, double eps,
                   uint64_t n_active);
 void replicate(State& st, sparq_field_t f);
+void group_unitary(State& st, sparq_field_t target, const double* mat, double eps);
 void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, uint64_t n_active);
 
 // ---- pointwise (pointwise.hip)

@@ -220,6 +220,26 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
         case SPARQ_PW_NEG_AMP: amp = make_double2(amp.x * -1.0, amp.y * -1.0); break;
         case SPARQ_PW_MUL_AMP: amp = cmul(amp, make_double2(op.d[0], op.d[1])); break;
         case SPARQ_PW_SET_FIELD: fput<W>(k, f0, op.c[0]); break;
+        case SPARQ_PW_INC_WRAP: {
+            uint64_t v = fget<W>(k, f0);
+            if (v == fmask(f0.width)) {
+                fput<W>(k, f1, (fget<W>(k, f1) + 1) % 2);
+                fput<W>(k, f0, 0);
+            } else {
+                fput<W>(k, f0, v + 1);
+            }
+            break;
+        }
+        case SPARQ_PW_DEC_WRAP: {
+            uint64_t v = fget<W>(k, f0);
+            if (v == 0) {
+                fput<W>(k, f1, (fget<W>(k, f1) + 1) % 2);
+                fput<W>(k, f0, fmask(f0.width));
+            } else {
+                fput<W>(k, f0, v - 1);
+            }
+            break;
+        }
         default: break;
     }
 }

@@ -202,6 +202,20 @@ PYBIND11_MODULE(_core, m) {
         .def(py::init<const std::vector<std::string>&, bool>(), py::arg("regs"), py::arg("inverse") = false)
         .def(py::init<const std::vector<size_t>&, bool>(), py::arg("reg_ids"), py::arg("inverse") = false);
     pw_class<GlobalPhase>(m, "GlobalPhase").def(py::init<complex_t>(), py::arg("phase"));
+    pw_class<PlusOneAndOverflow>(m, "PlusOneAndOverflow")
+        .def(py::init<std::string_view, std::string_view>(), py::arg("main_reg"), py::arg("overflow"))
+        .def(py::init<size_t, size_t>(), py::arg("main_reg"), py::arg("overflow"));
+    auto rgu = py::class_<Rot_GeneralUnitary, BaseOperator, std::shared_ptr<Rot_GeneralUnitary>>(m, "Rot_GeneralUnitary")
+        .def(py::init([](py::object r, std::vector<std::vector<complex_t>> rows) {
+            std::vector<complex_t> flat;
+            for (auto& row : rows) flat.insert(flat.end(), row.begin(), row.end());
+            return std::make_shared<Rot_GeneralUnitary>(resolve(r), flat);
+        }), py::arg("reg"), py::arg("unitary_matrix"));
+    bind_controllable(rgu);
+    py::class_<Rot_GeneralStatePrep, Rot_GeneralUnitary, std::shared_ptr<Rot_GeneralStatePrep>>(m, "Rot_GeneralStatePrep")
+        .def(py::init([](py::object r, std::vector<complex_t> vec) { return std::make_shared<Rot_GeneralStatePrep>(resolve(r), vec); }),
+             py::arg("reg"), py::arg("state"));
+    m.def("stateprep_unitary", &stateprep_unitary);
 
     py::class_<QRAMCircuit_qutrit, std::shared_ptr<QRAMCircuit_qutrit>>(m, "QRAMCircuit_qutrit")
         .def(py::init<size_t, size_t, std::vector<uint64_t>>(), py::arg("addr_size"), py::arg("data_size"), py::arg("memory"))

@@ -592,9 +592,11 @@ Reflection_Bool::Reflection_Bool(const std::vector<std::string>& names, bool inv
     for (size_t id : ids) extra_zero_regs.emplace_back(id, 0);
 }
 void Reflection_Bool::operator()(SparseState& state) const {
-    // inverse: flip unless all zero == flip everything, then flip back the all-zero ones
-    // (two exact sign changes; bit-identical to the single conditional flip).
-    if (inverse) {
+    // The reference flips when !(inverse ^ all_zero) (parallel_phase_operations.cpp:63-66):
+    // inverse = true flips the all-zero entries; inverse = false flips every other
+    // entry, done here as "flip all, then flip the all-zero ones back" (two exact
+    // sign changes, bit-identical to one conditional flip).
+    if (!inverse) {
         const Layout& L = state.sync();
         sparq_pw_op_t op{};
         op.opcode = SPARQ_PW_NEG_AMP;
@@ -607,6 +609,11 @@ GlobalPhase::GlobalPhase(complex_t cc) {   // parallel_phase_operations.cpp:72-1
     d[0] = cc.real(); d[1] = cc.imag();
     complex_t cj = std::conj(cc);
     dag_d[0] = cj.real(); dag_d[1] = cj.imag();
+}
+
+PlusOneAndOverflow::PlusOneAndOverflow(size_t main, size_t overflow) {   // block_encoding_tridiagonal.cpp:20-84
+    require_ids({main, overflow});
+    set_op(*this, "PlusOneAndOverflow", SPARQ_PW_INC_WRAP, {main, overflow}, SPARQ_PW_DEC_WRAP);
 }
 
 // ------------------------------------------------------------ branching operators
@@ -757,6 +764,62 @@ U2_Bool::U2_Bool(std::string_view r, size_t dg, double phi, double lambda) : Rot
 U2_Bool::U2_Bool(size_t r, size_t dg, double phi, double lambda) : Rot_Bool(r, dg, u2(phi, lambda)) {}
 U3_Bool::U3_Bool(std::string_view r, size_t dg, double t, double phi, double lambda) : Rot_Bool(r, dg, u3(t, phi, lambda)) {}
 U3_Bool::U3_Bool(size_t r, size_t dg, double t, double phi, double lambda) : Rot_Bool(r, dg, u3(t, phi, lambda)) {}
+
+Rot_GeneralUnitary::Rot_GeneralUnitary(size_t reg, std::vector<complex_t> m) : id(reg), mat(std::move(m)) {   // rot.h:40-60
+    require_ids({id});
+    size_t full = size_t(1) << System::size_of(id);
+    if (mat.size() != full * full) throw std::invalid_argument("Matrix size does not match the register's size.");
+}
+void Rot_GeneralUnitary::apply(SparseState& state, bool dagger) const {   // rot.cpp:340-467
+    const Layout& L = state.sync();
+    state.flush();
+    size_t full = size_t(1) << System::size_of(id);
+    std::vector<double> m(full * full * 2);
+    for (size_t i = 0; i < full; ++i)
+        for (size_t j = 0; j < full; ++j) {
+            complex_t v = dagger ? std::conj(mat[j * full + i]) : mat[i * full + j];
+            m[2 * (i * full + j)] = v.real();
+            m[2 * (i * full + j) + 1] = v.imag();
+        }
+    auto conds = conditions(L);
+    check(sparq_group_unitary(state.handle(), L.at(id), m.data(), conds.data(), (uint32_t)conds.size(), clearzero_default_eps()));
+    System::update_max_size(state.size());
+}
+std::vector<complex_t> stateprep_unitary(const std::vector<complex_t>& target) {   // rot.cpp:469-503
+    const size_t N = target.size();
+    if (N == 0) throw std::invalid_argument("Cannot prepare an empty vector.");
+    double scale = 0;
+    for (const auto& t : target) {
+        if (!std::isfinite(t.real()) || !std::isfinite(t.imag())) throw std::invalid_argument("State amplitudes must be finite.");
+        scale = std::max(scale, std::max(std::abs(t.real()), std::abs(t.imag())));
+    }
+    if (scale == 0) throw std::invalid_argument("Cannot prepare the zero vector.");
+    std::vector<complex_t> u(N);
+    double sq = 0;
+    for (size_t i = 0; i < N; ++i) {
+        u[i] = target[i] / scale;
+        sq += abs_sqr(u[i]);
+    }
+    const double nrm = std::sqrt(sq);
+    for (auto& x : u) x /= nrm;
+    const double lead = std::abs(u[0]);
+    const complex_t ph = lead == 0 ? complex_t(1) : u[0] / lead;
+    u[0] += ph;
+    double usq = 0;
+    for (const auto& x : u) usq += abs_sqr(x);
+    std::vector<complex_t> R(N * N);
+    for (size_t r = 0; r < N; ++r)
+        for (size_t c = 0; c < N; ++c) {
+            R[r * N + c] = complex_t(r == c ? 1 : 0) - (2.0 / usq) * u[r] * std::conj(u[c]);
+            if (c == 0) R[r * N + c] *= -ph;
+        }
+    return R;
+}
+Rot_GeneralStatePrep::Rot_GeneralStatePrep(size_t reg, const std::vector<complex_t>& vec)   // rot.cpp:511-538
+    : Rot_GeneralUnitary(reg, [&] {
+          if (vec.size() != (size_t(1) << System::size_of(reg))) throw std::invalid_argument("State size does not match the register's size.");
+          return stateprep_unitary(vec);
+      }()) {}
 
 // ------------------------------------------------------------ structural operators
 void ClearZero::operator()(SparseState& state) const {   // system_operations.cpp:371-382

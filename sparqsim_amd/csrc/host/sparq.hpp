@@ -312,6 +312,12 @@ struct Reflection_Bool : PointwiseOperator {
 struct GlobalPhase : PointwiseOperator {
     explicit GlobalPhase(complex_t c);
 };
+// main += 1 modulo 2^w, flipping `overflow` on wrap-around
+// (SparQ_Algorithm/include/BlockEncoding/block_encoding_tridiagonal.h:30-60)
+struct PlusOneAndOverflow : PointwiseOperator {
+    PlusOneAndOverflow(std::string_view main, std::string_view overflow) : PlusOneAndOverflow(System::get(main), System::get(overflow)) {}
+    PlusOneAndOverflow(size_t main, size_t overflow);
+};
 
 // Branching / structural operators
 struct Hadamard_Int : SelfAdjointOperator, Controllable {
@@ -371,6 +377,25 @@ struct U2_Bool : Rot_Bool { U2_Bool(std::string_view r, size_t digit, double phi
 struct U3_Bool : Rot_Bool { U3_Bool(std::string_view r, size_t digit, double theta, double phi, double lambda); U3_Bool(size_t r, size_t digit, double theta, double phi, double lambda); };
 u22_t make_func(uint64_t value, size_t n_digit);
 u22_t make_func_inv(uint64_t value, size_t n_digit);
+
+// Dense unitary on a whole register (SparQ/include/rot.h:16-120), row-major 2^w x 2^w.
+struct Rot_GeneralUnitary : BaseOperator, Controllable {
+    size_t id;
+    std::vector<complex_t> mat;
+    Rot_GeneralUnitary(std::string_view reg, std::vector<complex_t> m) : Rot_GeneralUnitary(System::get(reg), std::move(m)) {}
+    Rot_GeneralUnitary(size_t reg, std::vector<complex_t> m);
+    Controllable* controls() override { return this; }
+    void operator()(SparseState& state) const override { apply(state, false); }
+    void dag(SparseState& state) const override { apply(state, true); }
+    void apply(SparseState& state, bool dagger) const;
+};
+// Unitary whose first column is the normalised target (Householder reflection,
+// SparQ/src/rot.cpp:469-503).
+std::vector<complex_t> stateprep_unitary(const std::vector<complex_t>& target);
+struct Rot_GeneralStatePrep : Rot_GeneralUnitary {
+    Rot_GeneralStatePrep(std::string_view reg, const std::vector<complex_t>& vec) : Rot_GeneralStatePrep(System::get(reg), vec) {}
+    Rot_GeneralStatePrep(size_t reg, const std::vector<complex_t>& vec);
+};
 
 struct ClearZero : SelfAdjointOperator {
     double eps;

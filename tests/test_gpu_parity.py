@@ -126,26 +126,36 @@ def test_conditions_on_pointwise(ps, oracle):
     assert_same_state(eg, eo, exact_amps=True, what="conditions")
 
 
-def test_diagonal_ops(ps, oracle):
+DIAGONAL = [
+    ("Phase_Bool", lambda m: m.Phase_Bool("a", 2, 0.913), False),
+    ("Phase_Bool.dag", lambda m: m.Phase_Bool("b", 5, 1.7), True),
+    ("Z_Bool", lambda m: m.Z_Bool("c", 1), False),
+    ("S_Bool", lambda m: m.S_Bool("a", 0), False),
+    ("T_Bool", lambda m: m.T_Bool("b", 3), False),
+    ("RZ_Bool", lambda m: m.RZ_Bool("b", 1, 0.77), False),
+    ("RZ_Bool.dag", lambda m: m.RZ_Bool("b", 1, 0.77), True),
+    ("Y_Bool", lambda m: m.Y_Bool("c", 2), False),
+    ("GlobalPhase", lambda m: m.GlobalPhase(complex(0.6, 0.8)), False),
+    ("GlobalPhase.dag", lambda m: m.GlobalPhase(complex(0.6, 0.8)), True),
+    ("ZeroConditionalPhaseFlip", lambda m: m.ZeroConditionalPhaseFlip(["f", "g"]), False),
+    ("Reflection_Bool", lambda m: m.Reflection_Bool(["f"], False), False),
+    ("Reflection_Bool.inverse", lambda m: m.Reflection_Bool(["g", "f"], True), False),
+    ("Reflection_Bool.cond", lambda m: m.Reflection_Bool("c", True).conditioned_by_all_ones(["f", "g"]), False),
+    ("Reflection_Bool.cond2", lambda m: m.Reflection_Bool("c", False).conditioned_by_all_ones("f"), False),
+]
+
+
+@pytest.mark.parametrize("name,make,dagger", DIAGONAL, ids=[d[0] for d in DIAGONAL])
+def test_diagonal_ops(ps, oracle, name, make, dagger):
+    """Each diagonal operator on its own (compensating errors cannot cancel)."""
     def build(mod):
         st = loaded(mod, SPEC3, 4000, seed=3)
-        mod.Phase_Bool("a", 2, 0.913)(st)
-        mod.Phase_Bool("a", 2, 0.913).dag(st)
-        mod.Phase_Bool("b", 5, 1.7)(st)
-        mod.Z_Bool("c", 1)(st)
-        mod.S_Bool("a", 0)(st)
-        mod.T_Bool("b", 3)(st)
-        mod.RZ_Bool("b", 1, 0.77)(st)
-        mod.Y_Bool("c", 2)(st)
-        mod.GlobalPhase(complex(0.6, 0.8))(st)
-        mod.GlobalPhase(complex(0.6, 0.8)).dag(st)
-        mod.ZeroConditionalPhaseFlip(["f", "g"])(st)
-        mod.Reflection_Bool(["f"], False)(st)
-        mod.Reflection_Bool(["g", "f"], True)(st)
+        op = make(mod)
+        (op.dag if dagger else op)(st)
         return st
 
     eg, eo = run_both(ps, oracle, build)
-    assert_same_state(eg, eo, exact_amps=True, what="diagonal")
+    assert_same_state(eg, eo, exact_amps=True, what=name)
 
 
 def test_qram_load_conditioned(ps, oracle):

@@ -233,6 +233,21 @@ def test_sort_merge_invariants(oracle):
     assert list(amps) == [2, 5, 1]
 
 
+def test_reflection_bool_closed_form(oracle):
+    """Reflection_Bool(regs, inverse) negates when !(inverse XOR all-zero)
+    (parallel_phase_operations.cpp:46-70): inverse=False flips the non-zero
+    entries, inverse=True the all-zero ones; conditions gate the flip."""
+    st, _ = reg_state(oracle, [("x", "UnsignedInteger", 2), ("c", "Boolean", 1)])
+    vals = np.array([[x, c] for x in range(4) for c in range(2)], np.uint64)
+    st.load(vals, np.ones(8))
+    oracle.Reflection_Bool(["x"], False)(st)
+    m = amp_map(st)
+    assert all(m[(x, c)] == (1 if x == 0 else -1) for x in range(4) for c in range(2))
+    oracle.Reflection_Bool(["x"], True).conditioned_by_all_ones("c")(st)
+    m = amp_map(st)
+    assert all(m[(x, c)] == (1 if x == 0 else -1) * (-1 if (x == 0 and c == 1) else 1) for x in range(4) for c in range(2))
+
+
 def test_clear_zero_is_strict(oracle):
     st, _ = reg_state(oracle, [("a", "UnsignedInteger", 2)])
     st.load(np.array([[0], [1], [2]], np.uint64), np.array([0.5, 0.49999, 0.5j]))

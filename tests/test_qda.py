@@ -1,0 +1,101 @@
+"""QDA walk: the reference's golden fidelity lists (CPU oracle), and the HIP
+engine against the oracle step by step plus the full golden run (GPU)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from parity import assert_same_state, random_state
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qda_fidelity_lists.json")))
+TOL = GOLDEN["meta"]["tolerance"]      # 1e-5, CorrectnessTest_QDA_CompareList.inl:182
+
+
+def run_golden(mod, name, beta, max_points=None):
+    from sparqsim_amd import qda
+
+    ref = GOLDEN["lists"][name]
+    a, b, kappa, steps = qda.qda_schedule(3, 0.01, 2.0, beta)
+    assert steps == 2 * len(ref)
+    st = qda.setup_poiseuille(mod, 3)
+    worst, k = 0.0, 0
+    for n in range(steps):
+        w = qda.WalkSTridiagonal(mod, "main_reg", "anc_UA", "anc_1", "anc_2", "anc_3", "anc_4", n / steps, kappa, 1.3, 2.0, beta)
+        w(st)
+        mod.ClearZero()(st) if mod.__name__.startswith("sparqsim") else mod.ClearZero(None)(st)
+        if (n + 1) % 2 == 0:
+            vec, _ = qda.get_output(st, mod)
+            f = qda.fidelity(qda.mid_eigenstate(a, b, w.fs), vec)
+            worst = max(worst, abs(f - ref[k]))
+            k += 1
+            if max_points and k >= max_points:
+                break
+    return worst, k
+
+
+@pytest.mark.parametrize("name,beta", [("Neg_Tri", -1.0), ("Pos_Tri", 1.0)])
+def test_oracle_reproduces_golden_fidelities(oracle, name, beta):
+    worst, k = run_golden(oracle, name, beta)
+    assert k == 371
+    assert worst < TOL, worst
+
+
+def test_poiseuille_kappa_and_steps():
+    from sparqsim_amd import qda
+
+    a, b, kappa, steps = qda.qda_schedule(3, 0.01, 2.0, -1.0)
+    assert steps == 742 and abs(np.linalg.norm(a) - 1) < 1e-15
+    ev = 2 - 2 * np.cos(np.arange(1, 9) * np.pi / 9)       # closed-form spectrum of tridiag(-1, 2, -1)
+    assert abs(kappa - ev.max() / ev.min()) < 1e-9
+
+
+@pytest.mark.gpu
+def test_group_unitary_and_increment(ps, oracle):
+    from parity import run_both
+
+    rng = np.random.default_rng(7)
+    q, _ = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))
+    rows = [[complex(x) for x in r] for r in q]
+    prep = [0.5, 0.3 + 0.2j, 0.1, 0.7]
+
+    def build(mod):
+        st = mod.SparseState()
+        for name, w in (("m", 5), ("u", 2), ("o", 1), ("c", 1)):
+            mod.AddRegister(name, mod.UnsignedInteger, w)(st)
+        vals, amps = random_state(mod, 300, seed=70)
+        st.load(vals, amps)
+        mod.Rot_GeneralUnitary("u", rows)(st)
+        mod.Rot_GeneralStatePrep("u", prep).conditioned_by_all_ones("c")(st)
+        mod.Rot_GeneralStatePrep("u", prep).dag(st)
+        mod.PlusOneAndOverflow("m", "o").conditioned_by_value("u", 1)(st)
+        mod.PlusOneAndOverflow("m", "o").dag(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, what="Rot_GeneralUnitary / PlusOneAndOverflow")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("beta", [-1.0, 1.0])
+def test_walk_steps_match_oracle(ps, oracle, beta):
+    """Every entry of the state after each of the first 12 walk steps."""
+    from sparqsim_amd import qda
+
+    a, b, kappa, steps = qda.qda_schedule(3, 0.01, 2.0, beta)
+    sg = qda.setup_poiseuille(ps, 3)
+    so = qda.setup_poiseuille(oracle, 3)
+    for n in range(12):
+        for mod, st in ((ps, sg), (oracle, so)):
+            qda.WalkSTridiagonal(mod, "main_reg", "anc_UA", "anc_1", "anc_2", "anc_3", "anc_4", n / steps, kappa, 1.3, 2.0, beta)(st)
+        ps.ClearZero()(sg)
+        oracle.ClearZero(None)(so)
+        assert_same_state(sg.export(), so.export(), what=f"walk step {n}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,beta", [("Neg_Tri", -1.0), ("Pos_Tri", 1.0)])
+def test_engine_reproduces_golden_fidelities(ps, name, beta):
+    worst, k = run_golden(ps, name, beta)
+    assert k == 371
+    assert worst < TOL, worst
