@@ -63,6 +63,73 @@ def cpu_baseline(address_bits, data_bits, mem_words, budget_s=10.0):
     return circ.entry_ops * reps / total, total, reps
 
 
+def qda_walk_leg(ps, nqubit, nsteps, total_steps):
+    """The 2^nqubit Poiseuille QDA solve (A = tridiag(-1, 2.2666666666564934, -1),
+    b = ones/sqrt(n), kappa 16, p 1.3, ClearZero eps 1e-16) on the engine, timed over
+    its first `nsteps` walk steps of a `total_steps` schedule (one step = W(s) + ClearZero)."""
+    from sparqsim_amd import qda
+
+    alpha, beta, kappa, p = 2.2666666666564934, -1.0, 16.0, 1.3
+    old_eps = ps.get_clearzero_default_eps()
+    ps.set_clearzero_default_eps(1e-16)
+    st = qda.setup_poiseuille(ps, nqubit)
+    st.size()
+    sizes, times = [], []
+    try:
+        for n in range(nsteps):
+            t0 = time.perf_counter()
+            qda.WalkSTridiagonal(ps, "main_reg", "anc_UA", "anc_1", "anc_2", "anc_3", "anc_4",
+                                 n / total_steps, kappa, p, alpha, beta)(st)
+            ps.ClearZero()(st)
+            sizes.append(st.size())
+            times.append(time.perf_counter() - t0)
+    finally:
+        ps.set_clearzero_default_eps(old_eps)
+    wall = sum(times)
+    return {
+        "workload": f"2^{nqubit} Poiseuille QDA walk (tridiag(-1, 2.2666666666564934, -1), kappa 16, p 1.3, "
+                    f"czEps 1e-16), steps 0..{nsteps - 1} of a {total_steps}-step schedule",
+        "mode": "plain single-state",
+        "steps": nsteps, "wall_s": wall, "ms_per_step": wall / nsteps * 1e3,
+        "entries_first": sizes[0], "entries_last": sizes[-1], "entries_peak": max(sizes),
+        "entry_steps_per_s": sum(sizes) / wall,
+        "last_step_ms": times[-1] * 1e3,
+        "reference_published": "1.33 s/step at ~33.5M entries, 1x A800 (docs/verification/v1d-qda-linsolver-a800.md:52)",
+    }
+
+
+def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
+    """Batched noisy Grover: n_traj trajectories in one state keyed by `traj` vs the same
+    trajectories run one state at a time (timed on the first `plain_traj` of them and
+    scaled to n_traj; the trajectories are independent and of equal size)."""
+    from sparqsim_amd.grover import NoisyGrover
+
+    g = NoisyGrover(ps, n_traj, address_bits, address_bits, iterations, p)
+    n_events = sum(len(e) for e in g.events)
+    g.run_batched().size()                      # warm-up (allocator pool, code objects)
+    t0 = time.perf_counter()
+    st = g.run_batched()
+    n_final = st.size()
+    t_batched = time.perf_counter() - t0
+    del st
+    g.new_plain_registers()
+    g.run_plain(0).size()
+    t0 = time.perf_counter()
+    for t in range(plain_traj):
+        g.run_plain(t).size()
+    t_plain = (time.perf_counter() - t0) * n_traj / plain_traj
+    # each iteration applies 12 reference operators to every entry of every trajectory
+    entry_ops = 12 * iterations * n_final
+    return {
+        "workload": f"{n_traj} Grover trajectories, {address_bits}-bit addr/data/search, {iterations} iterations, "
+                    f"Pauli X/Z insertions p={p} per addr qubit per iteration ({n_events} events)",
+        "entries_final": n_final, "batched_s": t_batched, "batched_ms_per_iteration": t_batched / iterations * 1e3,
+        "batched_entries_per_s": entry_ops / t_batched,
+        "plain_s": t_plain, "plain_sample_trajectories": plain_traj,
+        "batched_speedup_vs_plain": t_plain / t_batched,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +140,13 @@ def main():
     ap.add_argument("--cpu-address-bits", type=int, default=20, help="size of the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--qda-qubits", type=int, default=20)
+    ap.add_argument("--qda-steps", type=int, default=40, help="walk steps timed in the QDA leg (0 skips it)")
+    ap.add_argument("--qda-total-steps", type=int, default=40568, help="length of the schedule s = n/total")
+    ap.add_argument("--grover-traj", type=int, default=1024, help="trajectories of the batched Grover leg (0 skips it)")
+    ap.add_argument("--grover-bits", type=int, default=14)
+    ap.add_argument("--grover-iterations", type=int, default=40)
+    ap.add_argument("--grover-plain-traj", type=int, default=16, help="trajectories timed one state at a time")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this engine has no multi-GPU path", file=sys.stderr)
@@ -136,6 +210,11 @@ def main():
                      "ms_per_step": r["ms"] / args.steps,
                      "GBs": (r["bytes"] / (r["ms"] * 1e-3) / 1e9) if r["ms"] > 0 else None} for r in prof[:12]],
     }
+    if args.qda_steps > 0:
+        result["qda_solve"] = qda_walk_leg(ps, args.qda_qubits, args.qda_steps, args.qda_total_steps)
+    if args.grover_traj > 0:
+        result["noisy_grover"] = grover_leg(ps, args.grover_traj, args.grover_bits, args.grover_iterations, 1e-3,
+                                            min(args.grover_plain_traj, args.grover_traj))
     if not args.no_cpu_baseline:
         ca = args.cpu_address_bits
         eps, secs, reps = cpu_baseline(ca, D, qram_memory(ca, D, config_index=1), args.cpu_seconds)

@@ -251,6 +251,14 @@ int sparq_prof_enable(int on);
 int sparq_prof_reset(void);
 int sparq_prof_report(sparq_prof_record_t* out, uint32_t capacity, uint32_t* count);
 
+/* Device memory held by the engine's caching allocator (no counterpart in the
+ * reference, whose CUDA path allocates through thrust): out[0] live bytes,
+ * out[1] cached free bytes, out[2] peak live bytes, out[3] cache hits,
+ * out[4] misses (runtime allocations), out[5] cached blocks.
+ * sparq_release_cached_memory synchronizes the device and frees the cache. */
+int sparq_memory_stats(uint64_t out[6]);
+int sparq_release_cached_memory(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,6 +2,7 @@
 // Every call converts exceptions into status codes and records a message.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -49,12 +50,60 @@ static void ensure_device() {
         throw Error(SPARQ_NO_DEVICE, "no HIP device available");
     }
     SPQ_HIP(hipSetDevice(0));
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, 0) == hipSuccess) {
-        uint64_t threshold = ~0ull;   // keep freed blocks in the pool
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
-    }
     g_device = 0;
+}
+
+// Per-state resources (stream, pinned readback words, timing events, zeroed device
+// counter) are recycled through a free list: creating and destroying streams and
+// pinned memory costs far more than a small state's operators.  A recycled stream is
+// drained first; its counter is zero, as every kernel that uses it leaves it.
+struct StateResources {
+    hipStream_t stream;
+    uint64_t* pinned;
+    hipEvent_t ev[8];
+    unsigned int* counter;
+};
+static std::mutex g_res_mu;
+static std::vector<StateResources> g_res_pool;
+
+static void acquire_resources(State& st) {
+    {
+        std::lock_guard<std::mutex> lock(g_res_mu);
+        if (!g_res_pool.empty()) {
+            StateResources r = g_res_pool.back();
+            g_res_pool.pop_back();
+            st.stream = r.stream;
+            st.pinned = r.pinned;
+            for (int i = 0; i < 8; ++i) st.ev[i] = r.ev[i];
+            st.counter = r.counter;
+            return;
+        }
+    }
+    SPQ_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+    SPQ_HIP(hipHostMalloc((void**)&st.pinned, 64 * sizeof(uint64_t), hipHostMallocDefault));
+    for (auto& e : st.ev) SPQ_HIP(hipEventCreate(&e));
+    SPQ_HIP(hipMalloc(&st.counter, 256));
+    SPQ_HIP(hipMemset(st.counter, 0, 256));
+}
+
+static void recycle_resources(State& st) {
+    if (!st.stream || !st.pinned || !st.counter) {   // partially built: return what exists
+        if (st.stream) (void)hipStreamSynchronize(st.stream);
+        for (auto& e : st.ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st.pinned) (void)hipHostFree(st.pinned);
+        if (st.counter) (void)hipFree(st.counter);
+        if (st.stream) {
+            forget_stream(st.stream);
+            (void)hipStreamDestroy(st.stream);
+        }
+        return;
+    }
+    (void)hipStreamSynchronize(st.stream);
+    StateResources r{st.stream, st.pinned, {}, st.counter};
+    for (int i = 0; i < 8; ++i) r.ev[i] = st.ev[i];
+    std::lock_guard<std::mutex> lock(g_res_mu);
+    g_res_pool.push_back(r);
 }
 
 static State& S(sparq_state_t h) {
@@ -153,15 +202,13 @@ int sparq_state_create(uint32_t words, uint64_t capacity, sparq_state_t* out) {
         State& st = h->st;
         st.W = words;
         try {
-            SPQ_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
-            SPQ_HIP(hipHostMalloc((void**)&st.pinned, 64 * sizeof(uint64_t), hipHostMallocDefault));
-            for (auto& e : st.ev) SPQ_HIP(hipEventCreate(&e));
-            SPQ_HIP(hipMalloc(&st.counter, 256));
-            SPQ_HIP(hipMemset(st.counter, 0, 256));
+            acquire_resources(st);
             st.cap = std::max<uint64_t>(capacity, 1);
             alloc_arrays(st, words, st.cap, &st.keys, &st.amps);
             st.n = 0;
         } catch (...) {
+            release(st);
+            recycle_resources(st);
             delete h;
             throw;
         }
@@ -174,12 +221,7 @@ int sparq_state_destroy(sparq_state_t h) {
         if (!h) return;
         State& st = h->st;
         release(st);
-        if (st.stream) (void)hipStreamSynchronize(st.stream);
-        for (auto& e : st.ev)
-            if (e) (void)hipEventDestroy(e);
-        if (st.pinned) (void)hipHostFree(st.pinned);
-        if (st.counter) (void)hipFree(st.counter);
-        if (st.stream) (void)hipStreamDestroy(st.stream);
+        recycle_resources(st);
         delete h;
     });
 }
@@ -276,6 +318,20 @@ int sparq_state_relayout(sparq_state_t h, uint32_t new_words, const sparq_field_
 
 int sparq_state_sync(sparq_state_t h) {
     return guarded([&] { SPQ_HIP(hipStreamSynchronize(S(h).stream)); });
+}
+
+int sparq_memory_stats(uint64_t out[6]) {
+    return guarded([&] {
+        SPQ_REQUIRE(out != nullptr, "null output");
+        memory_stats(out);
+    });
+}
+
+int sparq_release_cached_memory(void) {
+    return guarded([&] {
+        if (g_device < 0) return;
+        release_cached_memory();
+    });
 }
 
 int sparq_state_device_arrays(sparq_state_t h, uint64_t** keys, double** amps, uint64_t* capacity) {

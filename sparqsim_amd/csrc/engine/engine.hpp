@@ -55,9 +55,12 @@ struct State {
     int device = 0;
 };
 
-// ---- memory (stream-ordered pool allocations)
+// ---- memory (caching device allocator)
 void* dmalloc(size_t bytes, hipStream_t s);
 void dfree(void* p, hipStream_t s);
+void forget_stream(hipStream_t s);          // untag cached blocks of a drained stream
+void memory_stats(uint64_t out[6]);         // live, cached, peak bytes; hits, misses, cached blocks
+void release_cached_memory();               // synchronise the device and return cached blocks
 template <class T> T* dalloc(size_t count, hipStream_t s) { return static_cast<T*>(dmalloc(count * sizeof(T), s)); }
 
 // Allocate fresh key/amp arrays of capacity `cap` for `st` (content undefined).

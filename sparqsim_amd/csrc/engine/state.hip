@@ -1,6 +1,9 @@
 // State storage, device-wide scans, relayout and reductions.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 
 #include "blockscan.hpp"
 #include "engine.hpp"
@@ -9,20 +12,117 @@
 namespace sparq {
 
 // ------------------------------------------------------------------ memory
+// Caching device allocator.  Requests are rounded up to a size class (sixteen steps
+// per power of two, so at most 12.5 % slack) and a released block goes onto a
+// free list instead of back to the runtime.  Every block remembers the stream that
+// released it: that stream may reuse it at once (its queue orders the old and new
+// uses); any other stream first waits on an event recorded on the releasing stream.
+// A stream that is being destroyed is drained and its blocks are untagged.  The
+// runtime is called only on a miss (hipMalloc) and when a miss runs out of memory
+// (the cache is then synchronised, returned and the miss retried).
+namespace {
+struct CachedBlock {
+    void* p;
+    hipStream_t stream;   // last releasing stream; nullptr = no pending use
+};
+std::mutex g_mem_mu;
+std::unordered_map<void*, size_t> g_live;              // pointer -> class bytes
+std::multimap<size_t, CachedBlock> g_free_blocks;      // class bytes -> free blocks
+size_t g_cached_bytes = 0, g_live_bytes = 0, g_peak_bytes = 0;
+uint64_t g_misses = 0, g_hits = 0;
+
+size_t size_class(size_t b) {
+    if (b <= 4096) return 4096;
+    int e = 63 - __builtin_clzll((unsigned long long)(b - 1));   // 2^e < b <= 2^(e+1)
+    size_t step = size_t(1) << (e - 3 > 12 ? e - 3 : 12);        // eighth of 2^e
+    return (b + step - 1) / step * step;
+}
+
+void release_cache_locked() {
+    (void)hipDeviceSynchronize();
+    for (auto& kv : g_free_blocks) (void)hipFree(kv.second.p);
+    g_free_blocks.clear();
+    g_cached_bytes = 0;
+}
+}  // namespace
+
 void* dmalloc(size_t bytes, hipStream_t s) {
-    if (bytes == 0) bytes = 256;
-    void* p = nullptr;
-    hipError_t e = hipMallocAsync(&p, bytes, s);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error(e == hipErrorOutOfMemory ? SPARQ_OOM : SPARQ_RUNTIME,
-                    std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    const size_t cls = size_class(bytes == 0 ? 1 : bytes);
+    std::lock_guard<std::mutex> lock(g_mem_mu);
+    auto range = g_free_blocks.equal_range(cls);
+    auto pick = range.second;
+    for (auto it = range.first; it != range.second; ++it) {
+        if (it->second.stream == s || it->second.stream == nullptr) {
+            pick = it;
+            break;
+        }
+        if (pick == range.second) pick = it;
     }
+    void* p = nullptr;
+    if (pick != range.second) {
+        CachedBlock blk = pick->second;
+        g_free_blocks.erase(pick);
+        g_cached_bytes -= cls;
+        if (blk.stream != s && blk.stream != nullptr) {
+            hipEvent_t ev;
+            SPQ_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            SPQ_HIP(hipEventRecord(ev, blk.stream));
+            SPQ_HIP(hipStreamWaitEvent(s, ev, 0));
+            (void)hipEventDestroy(ev);
+        }
+        p = blk.p;
+        ++g_hits;
+    } else {
+        hipError_t e = hipMalloc(&p, cls);
+        if (e == hipErrorOutOfMemory && !g_free_blocks.empty()) {
+            (void)hipGetLastError();
+            release_cache_locked();
+            e = hipMalloc(&p, cls);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(e == hipErrorOutOfMemory ? SPARQ_OOM : SPARQ_RUNTIME,
+                        std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+        }
+        ++g_misses;
+    }
+    g_live[p] = cls;
+    g_live_bytes += cls;
+    g_peak_bytes = std::max(g_peak_bytes, g_live_bytes);
     return p;
 }
 
 void dfree(void* p, hipStream_t s) {
-    if (p) (void)hipFreeAsync(p, s);
+    if (!p) return;
+    std::lock_guard<std::mutex> lock(g_mem_mu);
+    auto it = g_live.find(p);
+    if (it == g_live.end()) return;   // not ours: never happens for engine buffers
+    const size_t cls = it->second;
+    g_live.erase(it);
+    g_live_bytes -= cls;
+    g_free_blocks.emplace(cls, CachedBlock{p, s});
+    g_cached_bytes += cls;
+}
+
+void forget_stream(hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g_mem_mu);
+    for (auto& kv : g_free_blocks)
+        if (kv.second.stream == s) kv.second.stream = nullptr;
+}
+
+void memory_stats(uint64_t out[6]) {
+    std::lock_guard<std::mutex> lock(g_mem_mu);
+    out[0] = g_live_bytes;
+    out[1] = g_cached_bytes;
+    out[2] = g_peak_bytes;
+    out[3] = g_hits;
+    out[4] = g_misses;
+    out[5] = g_free_blocks.size();
+}
+
+void release_cached_memory() {
+    std::lock_guard<std::mutex> lock(g_mem_mu);
+    release_cache_locked();
 }
 
 void alloc_arrays(State& st, uint32_t W, uint64_t cap, uint64_t** keys, double2** amps) {

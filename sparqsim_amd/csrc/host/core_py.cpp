@@ -297,4 +297,17 @@ PYBIND11_MODULE(_core, m) {
         check(sparq_device_name(buf, sizeof(buf)));
         return std::string(buf);
     });
+    m.def("memory_stats", []() {
+        uint64_t v[6] = {0};
+        check(sparq_memory_stats(v));
+        py::dict d;
+        d["live_bytes"] = v[0];
+        d["cached_bytes"] = v[1];
+        d["peak_bytes"] = v[2];
+        d["hits"] = v[3];
+        d["misses"] = v[4];
+        d["cached_blocks"] = v[5];
+        return d;
+    });
+    m.def("release_cached_memory", []() { check(sparq_release_cached_memory()); });
 }

@@ -1,0 +1,126 @@
+"""Grover composites and the batched noisy-trajectory workload.
+
+CPU: the oracle's Grover iteration is pinned by the closed form sin((2k+1) theta)
+for the marked amplitude (and the uniform cos((2k+1) theta)/sqrt(N-1) elsewhere),
+and the oracle's batched run (one state keyed by `traj`) equals its per-trajectory
+runs entry for entry.  GPU: the engine's batched and plain runs against the
+oracle's per-trajectory runs (keys bit-exact, amplitudes to 1e-12 relative).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from parity import assert_same_state, canonical
+from sparqsim_amd.grover import (NoisyGrover, grover_memory, grover_success_amplitude,
+                                 noise_events)
+
+
+def trajectory_of(export, t):
+    """Entries of trajectory t of a batched export, with the traj column dropped."""
+    ids, vals, amps = export
+    sel = vals[:, 0] == t
+    return [i - 1 for i in ids[1:]], vals[sel][:, 1:], amps[sel]
+
+
+# ---------------------------------------------------------------- inputs
+def test_memory_has_one_marked_word():
+    mem, target, marked = grover_memory(10, 10)
+    mem = np.array(mem)
+    assert len(mem) == 1024 and mem.max() < 1024
+    assert np.flatnonzero(mem == target).tolist() == [marked]
+
+
+def test_noise_schedule_deterministic_and_ordered():
+    a = noise_events(64, 10, 14, 1e-2)
+    b = noise_events(64, 10, 14, 1e-2)
+    assert a == b
+    n = sum(len(e) for e in a)
+    expected = 64 * 10 * 14 * 2 * 1e-2
+    assert abs(n - expected) < 5 * math.sqrt(expected)
+    for ev in a:
+        assert ev == sorted(ev)
+        assert all(0 <= t < 64 and 0 <= i < 14 and k in "XZ" for t, i, k in ev)
+    assert noise_events(8, 5, 6, 0.0) == [[] for _ in range(5)]
+
+
+# ---------------------------------------------------------------- oracle pinning
+@pytest.mark.parametrize("A,K", [(4, 1), (6, 3), (8, 5), (10, 7)])
+def test_oracle_grover_closed_form(oracle, A, K):
+    g = NoisyGrover(oracle, 1, A, A, K, 0.0)
+    g.new_plain_registers()
+    ids, vals, amps = g.run_plain(0).export()
+    n = 1 << A
+    assert len(amps) == n and sorted(vals[:, 0].tolist()) == list(range(n))
+    assert np.all(vals[:, 1] == 0) and np.all(vals[:, 2] == g.target)
+    s = grover_success_amplitude(A, K)
+    c = math.sqrt(max(0.0, 1 - s * s) / (n - 1))
+    theta = math.asin(2.0 ** (-A / 2))
+    c_signed = math.cos((2 * K + 1) * theta) / math.sqrt(n - 1)
+    assert abs(abs(c_signed) - c) < 1e-12
+    marked = vals[:, 0] == g.marked
+    assert np.allclose(amps[marked], s, atol=1e-12, rtol=0)
+    assert np.allclose(amps[~marked], c_signed, atol=1e-12, rtol=0)
+
+
+def test_oracle_batched_equals_plain(oracle):
+    g = NoisyGrover(oracle, 6, 6, 6, 4, 0.1)
+    assert sum(len(e) for e in g.events) > 0
+    eb = g.run_batched().export()
+    assert sorted(set(eb[1][:, 0].tolist())) == list(range(6))
+    g.new_plain_registers()
+    for t in range(6):
+        assert_same_state(trajectory_of(eb, t), g.run_plain(t).export(), exact_amps=True, what=f"traj {t}")
+
+
+def test_oracle_noise_changes_the_state(oracle):
+    """A plant: dropping the noise changes exactly the trajectories that had events,
+    so the batched-vs-plain comparison does see the noise operators."""
+    g = NoisyGrover(oracle, 6, 6, 6, 4, 0.02)
+    touched = {t for ev in g.events for (t, _, _) in ev}
+    assert touched and touched != set(range(6))
+    g.new_plain_registers()
+    noisy = [canonical(g.run_plain(t).export()) for t in range(6)]
+    g.events = [[] for _ in g.events]
+    clean = [canonical(g.run_plain(t).export()) for t in range(6)]
+    for t in range(6):
+        same = np.array_equal(noisy[t][1], clean[t][1]) and np.array_equal(noisy[t][2], clean[t][2])
+        assert same == (t not in touched), t
+
+
+# ---------------------------------------------------------------- engine parity
+@pytest.mark.gpu
+@pytest.mark.parametrize("T,A,D,K,p", [(6, 6, 6, 4, 0.1), (16, 8, 9, 3, 0.05), (3, 10, 10, 2, 0.02)])
+def test_engine_batched_vs_oracle(ps, oracle, T, A, D, K, p):
+    ge = NoisyGrover(ps, T, A, D, K, p)
+    eb = ge.run_batched().export()
+    go = NoisyGrover(oracle, T, A, D, K, p)
+    go.new_plain_registers()
+    for t in range(T):
+        assert_same_state(trajectory_of(eb, t), go.run_plain(t).export(), what=f"traj {t}")
+
+
+@pytest.mark.gpu
+def test_engine_plain_vs_oracle(ps, oracle):
+    T, A, D, K, p = 4, 7, 7, 3, 0.08
+    ge = NoisyGrover(ps, T, A, D, K, p)
+    go = NoisyGrover(oracle, T, A, D, K, p)
+    ge.new_plain_registers()
+    go.new_plain_registers()
+    for t in range(T):
+        assert_same_state(ge.run_plain(t).export(), go.run_plain(t).export(), what=f"traj {t}")
+
+
+@pytest.mark.gpu
+def test_engine_grover_closed_form(ps):
+    A, K = 12, 6
+    g = NoisyGrover(ps, 1, A, A, K, 0.0)
+    g.new_plain_registers()
+    ids, vals, amps = g.run_plain(0).export()
+    assert len(amps) == 1 << A
+    s = grover_success_amplitude(A, K)
+    theta = math.asin(2.0 ** (-A / 2))
+    c = math.cos((2 * K + 1) * theta) / math.sqrt((1 << A) - 1)
+    marked = vals[:, 0] == g.marked
+    assert np.allclose(amps[marked], s, atol=1e-12, rtol=0)
+    assert np.allclose(amps[~marked], c, atol=1e-12, rtol=0)
