@@ -1,10 +1,13 @@
 """In-tree build of the native parts of sparqsim_amd.
 
-* ``libsparq_hip.so`` -- the HIP engine (csrc/engine/*.hip) for gfx950, the
+* ``libsparq_hip.so``  -- the HIP engine (csrc/engine/*.hip) for gfx950, the
   C ABI declared in include/sparq_hip.h.
-* ``_core.<ext>.so``  -- the pybind11 host layer (csrc/host/*.cpp), linked
-  against the engine with an $ORIGIN rpath.
+* ``libsparq_host.so`` -- the host C++ layer (csrc/host/sparq.cpp): register
+  table, SparseState, the operator classes.  Shared so that the pybind module
+  and runtime-compiled operator plugins see one register table.
+* ``_core.<ext>.so``   -- the pybind11 bindings (csrc/host/core_py.cpp).
 
+Every library finds its neighbours through an $ORIGIN rpath.
 Run ``python -m sparqsim_amd.build`` (or ``__graft_entry__.build()``).
 """
 from __future__ import annotations
@@ -46,8 +49,19 @@ def engine_lib() -> Path:
     return PKG / "libsparq_hip.so"
 
 
+def host_lib() -> Path:
+    return PKG / "libsparq_host.so"
+
+
 def core_ext() -> Path:
     return PKG / ("_core" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def _link(objs, out: Path, libs: list[str]) -> Path:
+    tmp = out.with_suffix(".so.tmp")
+    _run(["g++", "-shared", *map(str, objs), "-o", str(tmp), f"-L{PKG}", *libs, "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
 
 
 def build_engine(jobs: int = 8) -> Path:
@@ -65,29 +79,28 @@ def build_engine(jobs: int = 8) -> Path:
     return out
 
 
-def build_core(jobs: int = 8) -> Path:
+def build_host(jobs: int = 8) -> tuple[Path, Path]:
     import pybind11
 
     BUILD.mkdir(exist_ok=True)
     inc = [f"-I{sysconfig.get_paths()['include']}", f"-I{pybind11.get_include()}"]
-    srcs = [HOST_SRC / "sparq.cpp", HOST_SRC / "core_py.cpp"]
-    objs = [BUILD / (s.stem + ".host.o") for s in srcs]
+    host_obj = BUILD / "sparq.host.o"
+    core_obj = BUILD / "core_py.host.o"
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(_run, ["g++", *HOST_FLAGS, *inc, "-c", str(s), "-o", str(o)]) for s, o in zip(srcs, objs)]
+        futs = [ex.submit(_run, ["g++", *HOST_FLAGS, "-c", str(HOST_SRC / "sparq.cpp"), "-o", str(host_obj)]),
+                ex.submit(_run, ["g++", *HOST_FLAGS, *inc, "-c", str(HOST_SRC / "core_py.cpp"), "-o", str(core_obj)])]
         for f in futs:
             f.result()
-    out = core_ext()
-    tmp = out.with_suffix(".so.tmp")
-    _run(["g++", "-shared", *map(str, objs), "-o", str(tmp), f"-L{PKG}", "-lsparq_hip", "-Wl,-rpath,$ORIGIN"])
-    os.replace(tmp, out)
-    return out
+    host = _link([host_obj], host_lib(), ["-lsparq_hip"])
+    core = _link([core_obj], core_ext(), ["-lsparq_host", "-lsparq_hip"])
+    return host, core
 
 
 def build_all() -> None:
     build_engine()
-    build_core()
+    build_host()
 
 
 if __name__ == "__main__":
     build_all()
-    print("built", engine_lib(), core_ext())
+    print("built", engine_lib(), host_lib(), core_ext())

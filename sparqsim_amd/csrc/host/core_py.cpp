@@ -310,4 +310,5 @@ PYBIND11_MODULE(_core, m) {
         return d;
     });
     m.def("release_cached_memory", []() { check(sparq_release_cached_memory()); });
+    m.def("_take_plugin_error", []() { return take_plugin_error(); });
 }

@@ -315,6 +315,120 @@ void SparseState::import_entries(const std::vector<uint64_t>& values, const std:
     check(sparq_state_upload(h_, keys.data(), a.data(), n));
 }
 
+void SparseState::to_host(std::vector<System>& entries) {
+    std::vector<size_t> ids;
+    std::vector<uint64_t> vals;
+    std::vector<complex_t> amps;
+    export_entries(ids, vals, amps);
+    entries.assign(amps.size(), System());
+    for (size_t i = 0; i < amps.size(); ++i) {
+        entries[i].amplitude = amps[i];
+        for (size_t j = 0; j < ids.size(); ++j) entries[i].registers[ids[j]].value = vals[i * ids.size() + j];
+    }
+}
+
+void SparseState::from_host(const std::vector<System>& entries) {
+    sync();
+    std::vector<size_t> ids;
+    for (size_t id = 0; id < layout_.field.size(); ++id) if (layout_.field[id].width) ids.push_back(id);
+    std::vector<uint64_t> vals(entries.size() * ids.size());
+    std::vector<complex_t> amps(entries.size());
+    for (size_t i = 0; i < entries.size(); ++i) {
+        amps[i] = entries[i].amplitude;
+        for (size_t j = 0; j < ids.size(); ++j)
+            vals[i * ids.size() + j] = ids[j] < entries[i].registers.size() ? entries[i].registers[ids[j]].value : 0;
+    }
+    import_entries(vals, amps);
+}
+
+// ------------------------------------------------------------ host entries
+bool System::operator<(const System& rhs) const {
+    for (size_t id = 0; id < name_register_map.size(); ++id) {
+        if (!status_of(id)) continue;
+        const uint64_t a = id < registers.size() ? registers[id].value : 0;
+        const uint64_t b = id < rhs.registers.size() ? rhs.registers[id].value : 0;
+        if (a != b) return a < b;
+    }
+    return false;
+}
+
+bool System::operator==(const System& rhs) const {
+    for (size_t id = 0; id < name_register_map.size(); ++id) {
+        if (!status_of(id)) continue;
+        const uint64_t a = id < registers.size() ? registers[id].value : 0;
+        const uint64_t b = id < rhs.registers.size() ? rhs.registers[id].value : 0;
+        if (a != b) return false;
+    }
+    return true;
+}
+
+std::string System::to_string() const {
+    std::ostringstream os;
+    os << amplitude.real() << (amplitude.imag() < 0 ? " - " : " + ") << std::abs(amplitude.imag()) << "i";
+    for (size_t id = 0; id < name_register_map.size(); ++id)
+        if (status_of(id)) os << " |" << name_of(id) << "=" << (id < registers.size() ? registers[id].value : 0) << ">";
+    return os.str();
+}
+
+// ------------------------------------------------------------ the two operator forms
+namespace {
+// An operator that overrides neither form would bounce between the two derived
+// forms; the guard turns that into an error.
+thread_local const BaseOperator* t_bridging = nullptr;
+struct Bridge {
+    const BaseOperator* prev;
+    Bridge(const BaseOperator* op, const char* what) : prev(t_bridging) {
+        if (t_bridging == op) throw std::runtime_error(what);
+        t_bridging = op;
+    }
+    ~Bridge() { t_bridging = prev; }
+};
+}  // namespace
+
+namespace {
+thread_local std::string t_plugin_error;
+}
+
+void record_plugin_error(const char* message) { t_plugin_error = message ? message : "unknown error"; }
+
+std::string take_plugin_error() {
+    std::string m;
+    m.swap(t_plugin_error);
+    return m;
+}
+
+void BaseOperator::operator()(SparseState& state) const {
+    Bridge b(this, "operator() is not implemented.");
+    std::vector<System> host;
+    state.to_host(host);
+    (*this)(host);
+    state.from_host(host);
+}
+
+void BaseOperator::operator()(std::vector<System>& state) const {
+    Bridge b(this, "operator() is not implemented.");
+    SparseState dev;
+    dev.from_host(state);
+    (*this)(dev);
+    dev.to_host(state);
+}
+
+void BaseOperator::dag(SparseState& state) const {
+    Bridge b(this, "Dagger is not implemented.");
+    std::vector<System> host;
+    state.to_host(host);
+    dag(host);
+    state.from_host(host);
+}
+
+void BaseOperator::dag(std::vector<System>& state) const {
+    Bridge b(this, "Dagger is not implemented.");
+    SparseState dev;
+    dev.from_host(state);
+    dag(dev);
+    dev.to_host(state);
+}
+
 std::string SparseState::to_string(int precision) {
     std::vector<size_t> ids;
     std::vector<uint64_t> vals;

@@ -19,6 +19,7 @@
 #include <string>
 #include <string_view>
 #include <tuple>
+#include <type_traits>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -44,9 +45,57 @@ void check(int status);
 // ------------------------------------------------------------ register table
 using StateInfoType = std::tuple<std::string, StateStorageType, size_t, bool>;
 
+// One register value of a host-side entry, with the reference's typed views
+// (SparQ/include/basic_components.h:90-131): Rational = value / 2^w, SignedInteger
+// = two's complement of the w-bit value, Boolean = value != 0.
+struct StateStorage {
+    uint64_t value = 0;
+    template <typename Ty> Ty as(size_t size) const {
+        const uint64_t v = value & width_mask(size);
+        if constexpr (std::is_floating_point_v<Ty>) {
+            return size >= 64 ? Ty(v * 1.0 / 2 / 9223372036854775808.0) : Ty(1.0 * v / double(uint64_t(1) << size));
+        } else if constexpr (std::is_same_v<Ty, bool>) {
+            return v != 0;
+        } else if constexpr (std::is_signed_v<Ty>) {
+            if (size == 0 || size >= 64) return Ty(int64_t(v));
+            return Ty((v >> (size - 1)) & 1 ? int64_t(v | ~width_mask(size)) : int64_t(v));
+        } else {
+            return Ty(v);
+        }
+    }
+    void flip(size_t digit) { value ^= uint64_t(1) << digit; }
+    bool operator==(const StateStorage& o) const { return value == o.value; }
+    bool operator!=(const StateStorage& o) const { return value != o.value; }
+    bool operator<(const StateStorage& o) const { return value < o.value; }
+    bool operator>(const StateStorage& o) const { return value > o.value; }
+};
+
 // Process-global register table with the reference's id-reuse rules
-// (SparQ/src/basic_components.cpp:341-459).
+// (SparQ/src/basic_components.cpp:341-459).  An instance is one host-side
+// entry (amplitude + a value per register id), the element of the
+// std::vector<System> view that host-side operators receive.
 struct System {
+    complex_t amplitude = 1.0;
+    mutable std::vector<StateStorage> registers;
+    System() : registers(name_register_map.size()) {}
+    StateStorage& get(size_t id) {
+        if (id >= name_register_map.size()) throw std::runtime_error("Register not found.");
+        if (registers.size() < name_register_map.size()) registers.resize(name_register_map.size());
+        return registers[id];
+    }
+    const StateStorage& get(size_t id) const {
+        if (id >= name_register_map.size()) throw std::runtime_error("Register not found.");
+        if (registers.size() < name_register_map.size()) registers.resize(name_register_map.size());
+        return registers[id];
+    }
+    StateStorage& last_register() { return get(get_last_activated_register()); }
+    const StateStorage& last_register() const { return get(get_last_activated_register()); }
+    // Canonical order: active registers in ascending id (basic_components.cpp:461-490).
+    bool operator<(const System& rhs) const;
+    bool operator==(const System& rhs) const;
+    bool operator!=(const System& rhs) const { return !(*this == rhs); }
+    std::string to_string() const;
+
     static std::vector<StateInfoType> name_register_map;
     static std::unordered_map<std::string, size_t> name_to_index;
     static std::vector<size_t> temporal_registers;
@@ -113,6 +162,9 @@ public:
     // Host copies: values[i][k] for the active register ids (ascending), amplitudes.
     void export_entries(std::vector<size_t>& ids, std::vector<uint64_t>& values, std::vector<complex_t>& amps);
     void import_entries(const std::vector<uint64_t>& values, const std::vector<complex_t>& amps);
+    // Host view as the reference's std::vector<System> (download / upload, entry order kept).
+    void to_host(std::vector<System>& entries);
+    void from_host(const std::vector<System>& entries);
     std::string to_string(int precision = 6);
 
 private:
@@ -148,15 +200,33 @@ struct Controllable {
 };
 
 // ------------------------------------------------------------ operators
+// An operator acts on the device state (operator()(SparseState&)) or on the
+// reference's host view (operator()(std::vector<System>&),
+// basic_components.h:748-830); a class overrides either.  The other form is
+// derived: a host-side operator applied to a SparseState downloads the entries,
+// runs, and uploads them; an engine operator applied to a std::vector<System>
+// uploads it into a temporary state, runs on the device, and downloads.  An
+// operator overriding neither form (or no dag) throws.
 struct BaseOperator {
     virtual ~BaseOperator() = default;
-    virtual void operator()(SparseState& state) const = 0;
-    virtual void dag(SparseState& state) const { throw std::runtime_error("Dagger is not implemented."); }
+    virtual void operator()(SparseState& state) const;
+    virtual void operator()(std::vector<System>& state) const;
+    virtual void dag(SparseState& state) const;
+    virtual void dag(std::vector<System>& state) const;
     virtual Controllable* controls() { return nullptr; }
 };
 struct SelfAdjointOperator : BaseOperator {
+    using BaseOperator::operator();
     void dag(SparseState& state) const override { (*this)(state); }
+    void dag(std::vector<System>& state) const override { (*this)(state); }
 };
+// Re-exposes both forms in a class that overrides one of them.
+#define SPQ_BOTH_VIEWS using BaseOperator::operator(); using BaseOperator::dag;
+
+// Plugin entry points catch exceptions (they must not cross the C boundary) and
+// park the message here; the Python proxy takes it and raises.
+void record_plugin_error(const char* message);
+std::string take_plugin_error();
 
 double& clearzero_default_eps();
 
@@ -175,6 +245,7 @@ struct PointwiseOperator : BaseOperator, Controllable {
     std::vector<std::pair<size_t, uint64_t>> extra_zero_regs;   // implicit "== 0" predicates
     bool has_dag = true;
     Controllable* controls() override { return this; }
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override { emit(state, false); }
     void dag(SparseState& state) const override;
     void emit(SparseState& state, bool dagger) const;
@@ -306,6 +377,7 @@ struct Reflection_Bool : PointwiseOperator {
     Reflection_Bool(const std::vector<size_t>& ids, bool inverse = false);
     Reflection_Bool(const std::vector<std::string>& names, bool inverse = false);
     Reflection_Bool(std::string_view name, bool inverse = false) : Reflection_Bool(std::vector<size_t>{System::get(name)}, inverse) {}
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
     void dag(SparseState& state) const override { (*this)(state); }
 };
@@ -325,6 +397,7 @@ struct Hadamard_Int : SelfAdjointOperator, Controllable {
     Hadamard_Int(std::string_view reg, size_t n) : Hadamard_Int(System::get(reg), n) {}
     Hadamard_Int(size_t reg, size_t n) : id(reg), n_digits(n) {}
     Controllable* controls() override { return this; }
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
 struct Hadamard_Int_Full : SelfAdjointOperator, Controllable {
@@ -332,6 +405,7 @@ struct Hadamard_Int_Full : SelfAdjointOperator, Controllable {
     explicit Hadamard_Int_Full(std::string_view reg) : Hadamard_Int_Full(System::get(reg)) {}
     explicit Hadamard_Int_Full(size_t reg) : id(reg) {}
     Controllable* controls() override { return this; }
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
 struct Hadamard_Bool : SelfAdjointOperator, Controllable {
@@ -339,12 +413,14 @@ struct Hadamard_Bool : SelfAdjointOperator, Controllable {
     explicit Hadamard_Bool(std::string_view reg) : Hadamard_Bool(System::get(reg)) {}
     explicit Hadamard_Bool(size_t reg);
     Controllable* controls() override { return this; }
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
 struct CondRot_Rational_Bool : BaseOperator {
     size_t register_in, register_out;
     CondRot_Rational_Bool(std::string_view in, std::string_view out) : CondRot_Rational_Bool(System::get(in), System::get(out)) {}
     CondRot_Rational_Bool(size_t in, size_t out);
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override { apply(state, 1.0); }
     void dag(SparseState& state) const override { apply(state, -1.0); }
     void apply(SparseState& state, double sign) const;
@@ -358,6 +434,7 @@ struct CondRot_General_Bool : BaseOperator {
     size_t in_id, out_id;
     std::function<u22_t(uint64_t)> func;
     CondRot_General_Bool(size_t in, size_t out, std::function<u22_t(uint64_t)> f);
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
 struct Rot_Bool : BaseOperator, Controllable {
@@ -366,6 +443,7 @@ struct Rot_Bool : BaseOperator, Controllable {
     Rot_Bool(std::string_view reg, size_t digit, u22_t m) : Rot_Bool(System::get(reg), digit, m) {}
     Rot_Bool(size_t reg, size_t digit, u22_t m);
     Controllable* controls() override { return this; }
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
     void dag(SparseState& state) const override;
 };
@@ -385,6 +463,7 @@ struct Rot_GeneralUnitary : BaseOperator, Controllable {
     Rot_GeneralUnitary(std::string_view reg, std::vector<complex_t> m) : Rot_GeneralUnitary(System::get(reg), std::move(m)) {}
     Rot_GeneralUnitary(size_t reg, std::vector<complex_t> m);
     Controllable* controls() override { return this; }
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override { apply(state, false); }
     void dag(SparseState& state) const override { apply(state, true); }
     void apply(SparseState& state, bool dagger) const;
@@ -401,21 +480,26 @@ struct ClearZero : SelfAdjointOperator {
     double eps;
     ClearZero() : eps(clearzero_default_eps()) {}
     explicit ClearZero(double e) : eps(e) {}
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
 struct Normalize : SelfAdjointOperator {
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
 struct SortUnconditional : SelfAdjointOperator {
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
 struct CheckDuplicateKey : SelfAdjointOperator {
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
 struct TestRemovable : SelfAdjointOperator {
     size_t register_id;
     explicit TestRemovable(std::string_view n) : register_id(System::get(n)) {}
     explicit TestRemovable(size_t id) : register_id(id) {}
+    SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
 
