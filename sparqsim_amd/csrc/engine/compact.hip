@@ -55,54 +55,72 @@ __device__ __forceinline__ bool keep_entry(const uint64_t* __restrict__ keys, co
     }
 }
 
+// Tiles are striped: item j of thread t is entry tile + j*kBlock + t, so every
+// load and store instruction of a wave touches 64 consecutive entries, and the
+// kept entries of one row land in one contiguous output run.
 template <int W, int MODE>
 __global__ void __launch_bounds__(kBlock) compact_count_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
                                                           uint64_t n, double eps, const CondList cl, uint32_t* __restrict__ counts) {
     __shared__ uint32_t lds[kBlock / 64];
-    uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+    const uint64_t tile = (uint64_t)blockIdx.x * kTile;
     uint32_t c = 0;
+#pragma unroll
     for (int j = 0; j < kItems; ++j) {
-        uint64_t i = base + j;
+        uint64_t i = tile + (uint64_t)j * kBlock + threadIdx.x;
         double2 s;
         if (i < n && keep_entry<W, MODE>(keys, amps, cap, n, i, eps, cl, s)) ++c;
     }
-    uint32_t tot;
-    (void)block_excl_scan<uint32_t>(c, lds, tot);
-    if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += lds[w];
+        counts[blockIdx.x] = tot;
+    }
 }
 
-// Kept entries go to kout/aout at offs[tile] + rank; for kPartition the others
-// go after all kept ones (at n_kept + tile_start - offs[tile] + rank).
+// Kept entries go to kout/aout at offs[tile] + rank in entry order; for
+// kPartition the others go after all kept ones (at n_kept + rejects before them).
 template <int W, int MODE>
 __global__ void __launch_bounds__(kBlock) compact_scatter_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps,
                                                             uint64_t cap, uint64_t n, double eps, const CondList cl,
                                                             const uint64_t* __restrict__ offs, const uint64_t* __restrict__ total,
                                                             uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out) {
-    __shared__ uint32_t lds[kBlock / 64];
+    constexpr int kWaves = kBlock / 64;
+    __shared__ uint32_t row_wave[kItems * kWaves];   // kept count per (row, wave), then its exclusive prefix
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t tile = (uint64_t)blockIdx.x * kTile;
-    uint64_t base = tile + (uint64_t)threadIdx.x * kItems;
-    uint32_t flags = 0, c = 0;
+    uint32_t flags = 0;
+    uint32_t lane_pre[kItems];
     double2 sums[kItems];
+#pragma unroll
     for (int j = 0; j < kItems; ++j) {
-        uint64_t i = base + j;
+        const uint64_t i = tile + (uint64_t)j * kBlock + threadIdx.x;
         sums[j] = make_double2(0.0, 0.0);
-        if (i < n && keep_entry<W, MODE>(keys, amps, cap, n, i, eps, cl, sums[j])) {
-            flags |= 1u << j;
-            ++c;
-        }
+        const bool k = i < n && keep_entry<W, MODE>(keys, amps, cap, n, i, eps, cl, sums[j]);
+        const uint64_t m = __ballot(k);
+        flags |= uint32_t(k) << j;
+        lane_pre[j] = (uint32_t)__popcll(m & lanemask_lt());
+        if (lane == 0) row_wave[j * kWaves + wave] = (uint32_t)__popcll(m);
     }
-    uint32_t tot;
-    uint32_t pre = block_excl_scan<uint32_t>(c, lds, tot);
-    uint64_t pos = offs[blockIdx.x] + pre;
-    uint64_t rej = 0;
-    if (MODE == kPartition) rej = total[0] + (base - tile) - pre + (tile - offs[blockIdx.x]);
+    __syncthreads();
+    if (threadIdx.x < 64) {   // exclusive scan of the 32 (row, wave) counts in entry order
+        uint32_t v = threadIdx.x < kItems * kWaves ? row_wave[threadIdx.x] : 0;
+        uint32_t inc = wave_incl_scan(v);
+        if (threadIdx.x < kItems * kWaves) row_wave[threadIdx.x] = inc - v;
+    }
+    __syncthreads();
+    const uint64_t kept_before_tile = offs[blockIdx.x];
+    const uint64_t rej_base = (MODE == kPartition) ? total[0] + (tile - kept_before_tile) : 0;
+#pragma unroll
     for (int j = 0; j < kItems; ++j) {
-        uint64_t i = base + j;
+        const uint64_t i = tile + (uint64_t)j * kBlock + threadIdx.x;
         if (i >= n) break;
-        bool kept = (flags >> j) & 1;
+        const uint32_t kept_rank = row_wave[j * kWaves + wave] + lane_pre[j];   // kept entries before i in the tile
         uint64_t dst;
-        if (kept) dst = pos++;
-        else if (MODE == kPartition) dst = rej++;
+        if ((flags >> j) & 1) dst = kept_before_tile + kept_rank;
+        else if (MODE == kPartition) dst = rej_base + ((uint64_t)j * kBlock + threadIdx.x - kept_rank);
         else continue;
 #pragma unroll
         for (int w = 0; w < W; ++w) kout[(uint64_t)w * cap_out + dst] = keys[(uint64_t)w * cap + i];
