@@ -259,6 +259,32 @@ int sparq_prof_report(sparq_prof_record_t* out, uint32_t capacity, uint32_t* cou
 int sparq_memory_stats(uint64_t out[6]);
 int sparq_release_cached_memory(void);
 
+/* Learned output sizer (no counterpart in the reference, which always counts
+ * before it allocates).  A 4-layer FP32 MLP, 16 features -> 64 -> 64 -> 64 -> 1
+ * with ReLU, predicts log2(G)/32 for the group count G of Hadamard_Int,
+ * Hadamard_Int_Full and dense group unitaries from the key-statistics pass.
+ * Parameter layout (floats): W1[64][16], b1[64], W2[64][64], b2[64],
+ * W3[64][64], b3[64], W4[64], b4.  When enabled, those operators size their
+ * output with 2^(32*y + margin) groups and skip the readback of the exact group
+ * count; the exact count, computed on the device, is read with the final size
+ * and an under-prediction reruns the exact path (keys and amplitudes are
+ * identical either way).  Training runs on the GPU: sparq_mlp_train performs
+ * `steps` Adam steps on minibatches of a seeded permutation, loss
+ * mean(w * (y_hat - y)^2) with w = alpha where y_hat < y, else 1. */
+#define SPARQ_SIZER_FEATURES 16
+#define SPARQ_SIZER_PARAMS 9473
+int sparq_mlp_forward(const float* params, const float* x, uint32_t n, float* y);
+int sparq_mlp_grad(const float* params, const float* x, const float* y, uint32_t n, float alpha, float* grad, float* loss);
+int sparq_mlp_train(float* params, const float* x, const float* y, uint32_t n, uint32_t steps, uint32_t batch, float lr,
+                    float alpha, uint64_t seed, float* loss_history);
+int sparq_sizer_load(const float* params, uint32_t count, float margin);
+int sparq_sizer_enable(int on);
+/* Logging of (features, log2(G)/32) for every sized operator run on the exact path. */
+int sparq_sizer_log(int on);
+int sparq_sizer_log_fetch(float* x, float* y, uint32_t capacity, uint32_t* count, int clear);
+/* out[0] operators sized by the exact count, out[1] by the network, out[2] network under-predictions (reruns). */
+int sparq_sizer_stats(uint64_t out[3]);
+
 #ifdef __cplusplus
 }
 #endif

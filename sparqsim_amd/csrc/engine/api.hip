@@ -22,27 +22,14 @@ struct sparq_table_s {
 static thread_local std::string g_last_error;
 static int g_device = -1;
 
+void sparq::set_last_error(const std::string& msg) { g_last_error = msg; }
+
 template <class F>
 static int guarded(F&& f) {
-    try {
-        f();
-        return SPARQ_OK;
-    } catch (const Error& e) {
-        g_last_error = e.msg;
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return SPARQ_OOM;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return SPARQ_RUNTIME;
-    } catch (...) {
-        g_last_error = "unknown error";
-        return SPARQ_RUNTIME;
-    }
+    return guarded_call(std::forward<F>(f));
 }
 
-static void ensure_device() {
+void sparq::ensure_device() {
     if (g_device >= 0) return;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {

@@ -66,11 +66,17 @@ __global__ void __launch_bounds__(kBlock) classify_kernel(const uint64_t* __rest
 
 // Output entry o = g*F + j: key of the group head with the low target bits = j.
 // DIRECT and SPARSE_SUM amplitudes are produced here; WHT blocks start at 0.
+// The group count G is read from the device; the output holds G_cap groups
+// and nothing is written when G > G_cap (a too small predicted capacity).
 template <int W>
 __global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
                                                         const uint64_t* __restrict__ start, const uint8_t* __restrict__ cls, uint32_t nd,
-                                                        sparq_field_t low, double extra, uint64_t M, uint64_t* __restrict__ kout,
-                                                        double2* __restrict__ aout, uint64_t cap_out, const double2* __restrict__ dense) {
+                                                        sparq_field_t low, double extra, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
+                                                        uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out,
+                                                        const double2* __restrict__ dense) {
+    const uint64_t G = G_dev[0];
+    if (G > G_cap) return;
+    const uint64_t M = G << nd;
     uint64_t o = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (o >= M) return;
     const uint64_t g = o >> nd, j = o & ((uint64_t(1) << nd) - 1);
@@ -114,9 +120,9 @@ template <int W>
 __global__ void __launch_bounds__(kBlock) expand_scatter_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
                                                            uint64_t n, const uint32_t* __restrict__ head, const uint64_t* __restrict__ pos,
                                                            const uint8_t* __restrict__ cls, uint32_t nd, sparq_field_t low,
-                                                           double2* __restrict__ aout) {
+                                                           const uint64_t* __restrict__ G_dev, uint64_t G_cap, double2* __restrict__ aout) {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || G_dev[0] > G_cap) return;
     // group id of entry i = inclusive count of heads up to i, minus one
     uint64_t g = head ? pos[i] + head[i] - 1 : 0;
     if (cls[g] != kWht) return;
@@ -175,10 +181,16 @@ __device__ __forceinline__ void lds_wht_rounds(double2* buf, uint32_t T, uint32_
 
 // First pass: stages [0, min(nd, 11)) on contiguous tiles of 2^11 elements of the
 // output array (a tile may hold many small groups; non-WHT groups are skipped).
-__global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restrict__ a, uint64_t M, uint32_t nd, uint32_t cnt,
-                                                           const uint8_t* __restrict__ cls) {
+// G_dev = {group count, WHT group count}; tiles past G << nd do nothing.
+__global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restrict__ a, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
+                                                           uint32_t nd, uint32_t cnt, const uint8_t* __restrict__ cls) {
     __shared__ double2 buf[1 << kWhtTileLog];
+    const uint64_t G = G_dev[0];
+    if (G > G_cap || G_dev[1] == 0) return;
+    const uint64_t M = G << nd;
     const uint64_t base = (uint64_t)blockIdx.x << kWhtTileLog;
+    if (base >= M) return;
+    if (nd >= (uint32_t)kWhtTileLog && cls[base >> nd] != kWht) return;   // the tile lies in one non-WHT group
     const uint32_t T = (uint32_t)min<uint64_t>(uint64_t(1) << kWhtTileLog, M - base);
     for (uint32_t e = threadIdx.x; e < T; e += kBlock) buf[e] = a[base + e];
     __syncthreads();
@@ -187,16 +199,19 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
 }
 
 // Later passes: stages [lo, lo+cnt) of one group, on tiles of 2^cnt x 2^c2
-// elements (c2 contiguous low index bits, c2 <= lo).
-__global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ a, uint64_t M, uint32_t nd, uint32_t lo, uint32_t cnt,
-                                                     uint32_t c2, const uint8_t* __restrict__ cls) {
-    __shared__ double2 buf[1 << kWhtTileLog];
+// elements (c2 contiguous low index bits, c2 <= lo) of up to 2^kWhtPassTileLog
+// elements (64 KiB of LDS).
+constexpr int kWhtPassTileLog = 12;
+__global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ a, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
+                                                     uint32_t nd, uint32_t lo, uint32_t cnt, uint32_t c2, const uint8_t* __restrict__ cls) {
+    __shared__ double2 buf[1 << kWhtPassTileLog];
     const uint32_t T = 1u << (cnt + c2);
     const uint64_t tile = blockIdx.x;
     const uint64_t per_group = (uint64_t(1) << nd) >> (cnt + c2);
     const uint64_t g = tile / per_group;
     const uint64_t r = tile % per_group;
-    if (cls[g] != kWht) return;
+    const uint64_t G = G_dev[0];
+    if (G > G_cap || g >= G || cls[g] != kWht) return;
     // r enumerates (high bits above lo+cnt) x (low chunk index below lo, in units of 2^c2)
     const uint64_t nlowchunks = (uint64_t(1) << lo) >> c2;
     const uint64_t lowchunk = r % nlowchunks, high = r / nlowchunks;
@@ -214,22 +229,35 @@ __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ 
     }
 }
 
-__global__ void single_group_kernel(uint64_t* __restrict__ start, uint8_t* __restrict__ cls, uint64_t n, uint8_t c) {
+__global__ void single_group_kernel(uint64_t* __restrict__ start, uint8_t* __restrict__ cls, uint64_t n, uint8_t c,
+                                    uint64_t* __restrict__ counts) {
     start[0] = 0;
     start[1] = n;
     cls[0] = c;
+    counts[0] = 1;
+    counts[1] = (c == kWht) ? 1 : 0;
+}
+
+// counts = {G, WHT groups, valid entries, -}: valid entries = G << nd, or 0
+// when G exceeds the capacity the output was sized for.
+__global__ void sized_total_kernel(uint64_t* __restrict__ counts, uint64_t G_cap, uint32_t nd) {
+    const uint64_t G = counts[0];
+    counts[2] = (G > G_cap) ? 0 : (G << nd);
 }
 
 template <int W>
-static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, const double2* dense) {
+static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, const double2* dense,
+                          bool allow_learned = true) {
     const uint64_t n = st.n;
     const uint64_t F = uint64_t(1) << nd;
     sparq_field_t low{target.word, target.shift, (uint8_t)nd, 0};
     // Sort by (everything else, low target bits) unless already in that order;
     // when no bit outside the low target bits varies there is a single group.
     bool single = (n == 1);
+    KeyStats ks;
     if (n > 1) {
-        KeyStats ks = key_stats(st, &low);
+        const SizerQuery q{n, st.W, nd, dense ? 2u : (full ? 1u : 0u), low.word, low.shift};
+        ks = key_stats(st, &low, allow_learned ? &q : nullptr);
         if (!ks.sorted) sort_by_plan(st, make_plan(ks, st.W, &low, 8));
         single = true;
         for (uint32_t w = 0; w < st.W; ++w) {
@@ -242,17 +270,20 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     uint64_t* pos = nullptr;
     uint64_t* start = nullptr;
     uint8_t* cls = nullptr;
-    uint64_t G, n_wht;
+    // counts = {G, WHT groups, valid output entries, kept entries} on the device
+    uint64_t* counts = dalloc<uint64_t>(4, st.stream);
+    uint64_t G = 0, n_wht = 0, G_cap = 0;
+    bool learned = false;
     if (single) {
         uint8_t c;
         if (dense) c = kDense;
         else if (!full) c = (n == 1) ? kDirect : kWht;
         else c = (n == F) ? kWht : (n + 1 <= nd ? kSparseSum : kWht);
-        G = 1;
+        G = G_cap = 1;
         n_wht = (c == kWht) ? 1 : 0;
         start = dalloc<uint64_t>(2, st.stream);
         cls = dalloc<uint8_t>(1, st.stream);
-        single_group_kernel<<<1, 1, 0, st.stream>>>(start, cls, n, c);
+        single_group_kernel<<<1, 1, 0, st.stream>>>(start, cls, n, c, counts);
         SPQ_HIP(hipGetLastError());
     } else {
         head = dalloc<uint32_t>(n, st.stream);
@@ -277,56 +308,112 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
                                                                           (unsigned long long*)(pos + n + 1));
         }
         SPQ_HIP(hipGetLastError());
-        uint64_t h2[2];
-        read_back(st, pos + n, sizeof(h2), h2);
-        G = h2[0];
-        n_wht = h2[1];
+        SPQ_HIP(hipMemcpyAsync(counts, pos + n, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, st.stream));
+        // With the sizer active the output is sized by the predicted group count and
+        // the exact count stays on the device until the final readback.
+        if (ks.have_prediction) {
+            // The prediction refines the exact bound G <= min(n, 2^(bits varying
+            // outside the low target bits)) and never exceeds it.
+            int vo = 0;
+            for (uint32_t w = 0; w < st.W; ++w) {
+                uint64_t v = ks.vary[w];
+                if (w == low.word) v &= ~(fmask_host(nd) << low.shift);
+                vo += __builtin_popcountll(v);
+            }
+            const uint64_t bound = vo >= 63 ? n : std::min<uint64_t>(n, uint64_t(1) << vo);
+            const double lg = std::min(40.0, (double)ks.prediction * 32.0 + sizer_margin());
+            const double g = std::ceil(std::exp2(std::max(0.0, lg)));
+            G_cap = (uint64_t)std::min<double>((double)bound, g);
+            learned = G_cap >= 1 && G_cap <= (~0ull >> nd) && (G_cap << nd) <= (uint64_t(1) << 33);
+        }
+        if (!learned) {
+            uint64_t h2[2];
+            read_back(st, counts, sizeof(h2), h2);
+            G = G_cap = h2[0];
+            n_wht = h2[1];
+            if (ks.have_features) sizer_record(ks.features, (float)(std::log2((double)G) / 32.0));
+            sizer_count(0);
+        }
     }
-    SPQ_REQUIRE(G <= (~0ull >> nd), "Hadamard output size overflows");
-    const uint64_t M = G << nd;
+    SPQ_REQUIRE(G_cap <= (~0ull >> nd), "Hadamard output size overflows");
+    const uint64_t M_cap = G_cap << nd;
     uint64_t* kout;
     double2* aout;
-    alloc_arrays(st, st.W, M, &kout, &aout);
+    alloc_arrays(st, st.W, M_cap, &kout, &aout);
     const double extra = 1.0 / std::sqrt((double)F);
     {
-        SPQ_PROF(st.stream, "expand_init_kernel", M * (8.0 * W + 16));
-        expand_init_kernel<W><<<grid_for(M, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, start, cls, nd, low, extra, M, kout,
-                                                                         aout, M, dense);
+        SPQ_PROF(st.stream, "expand_init_kernel", M_cap * (8.0 * W + 16));
+        expand_init_kernel<W><<<grid_for(M_cap, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, start, cls, nd, low, extra, counts,
+                                                                             G_cap, kout, aout, M_cap, dense);
     }
     SPQ_HIP(hipGetLastError());
-    if (n_wht) {
+    if (n_wht || learned) {
         {
             SPQ_PROF(st.stream, "expand_scatter_kernel", n * (8.0 * W + 16));
             expand_scatter_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, head, pos, cls, nd, low,
-                                                                                     aout);
+                                                                                     counts, G_cap, aout);
         }
         SPQ_HIP(hipGetLastError());
-        // Butterfly passes: the first covers min(nd, 11) stages on contiguous tiles,
-        // later ones up to 7 stages on 16-wide column tiles of one group.
+        // Butterfly passes: the first covers min(nd, 11) stages on contiguous tiles;
+        // the rest are split evenly into passes of <= 9 stages on column tiles of
+        // one group (2^cnt rows of 2^c2 contiguous elements, up to 2^12 elements).
+        const double wht_bytes = (double)(learned ? G_cap : n_wht) * F * 32.0;
         uint32_t lo = std::min<uint32_t>(nd, kWhtTileLog);
         {
-            SPQ_PROF(st.stream, "wht_first_pass_kernel", (double)n_wht * F * 32.0);
-            wht_first_pass_kernel<<<grid_for(M, 1u << kWhtTileLog), kBlock, 0, st.stream>>>(aout, M, nd, lo, cls);
+            SPQ_PROF(st.stream, "wht_first_pass_kernel", wht_bytes);
+            wht_first_pass_kernel<<<grid_for(M_cap, 1u << kWhtTileLog), kBlock, 0, st.stream>>>(aout, counts, G_cap, nd, lo, cls);
         }
         SPQ_HIP(hipGetLastError());
         while (lo < nd) {
-            uint32_t c2 = std::min<uint32_t>(4, lo);
-            uint32_t cnt = std::min<uint32_t>(nd - lo, kWhtTileLog - c2);
-            uint64_t tiles = M >> (cnt + c2);
+            const uint32_t rem = nd - lo;
+            const uint32_t passes = (rem + 8) / 9;
+            const uint32_t cnt = (rem + passes - 1) / passes;
+            const uint32_t c2 = std::min<uint32_t>(lo, kWhtPassTileLog - cnt);
+            const uint64_t tiles = M_cap >> (cnt + c2);
             {
-                SPQ_PROF(st.stream, "wht_pass_kernel", (double)n_wht * F * 32.0);
-                wht_pass_kernel<<<(unsigned)tiles, kBlock, 0, st.stream>>>(aout, M, nd, lo, cnt, c2, cls);
+                SPQ_PROF(st.stream, "wht_pass_kernel", wht_bytes);
+                wht_pass_kernel<<<(unsigned)tiles, kBlock, 0, st.stream>>>(aout, counts, G_cap, nd, lo, cnt, c2, cls);
             }
             SPQ_HIP(hipGetLastError());
             lo += cnt;
         }
     }
-    install_arrays(st, st.W, kout, aout, M, M);
     dfree(head, st.stream);
     dfree(pos, st.stream);
     dfree(start, st.stream);
     dfree(cls, st.stream);
-    clear_zero(st, eps);
+    if (!learned) {
+        dfree(counts, st.stream);
+        install_arrays(st, st.W, kout, aout, G << nd, M_cap);
+        clear_zero(st, eps);
+        return;
+    }
+    // One readback for the exact group count, the overflow check and ClearZero.
+    sized_total_kernel<<<1, 1, 0, st.stream>>>(counts, G_cap, nd);
+    SPQ_HIP(hipGetLastError());
+    uint64_t info[4];
+    uint64_t* kz = nullptr;
+    double2* az = nullptr;
+    clear_zero_sized(st, kout, aout, M_cap, counts, eps, info, &kz, &az);
+    G = info[0];
+    if (ks.have_features) sizer_record(ks.features, (float)(std::log2((double)G) / 32.0));
+    if (G > G_cap) {   // under-predicted: nothing was written; rerun with the exact count
+        sizer_count(2);
+        dfree(kout, st.stream);
+        dfree(aout, st.stream);
+        dfree(counts, st.stream);
+        hadamard_impl<W>(st, target, nd, full, eps, dense, false);
+        return;
+    }
+    sizer_count(1);
+    dfree(counts, st.stream);
+    if (kz == kout) {
+        install_arrays(st, st.W, kout, aout, info[3], M_cap);
+    } else {
+        dfree(kout, st.stream);
+        dfree(aout, st.stream);
+        install_arrays(st, st.W, kz, az, info[3], std::max<uint64_t>(info[3], 1));
+    }
 }
 
 void hadamard_int(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, uint64_t /*n_active*/) {

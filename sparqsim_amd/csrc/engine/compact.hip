@@ -60,8 +60,10 @@ __device__ __forceinline__ bool keep_entry(const uint64_t* __restrict__ keys, co
 // kept entries of one row land in one contiguous output run.
 template <int W, int MODE>
 __global__ void __launch_bounds__(kBlock) compact_count_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
-                                                          uint64_t n, double eps, const CondList cl, uint32_t* __restrict__ counts) {
+                                                          uint64_t n, const uint64_t* __restrict__ n_dev, double eps, const CondList cl,
+                                                          uint32_t* __restrict__ counts) {
     __shared__ uint32_t lds[kBlock / 64];
+    if (n_dev) n = min(n, n_dev[0]);   // entry count known only on the device
     const uint64_t tile = (uint64_t)blockIdx.x * kTile;
     uint32_t c = 0;
 #pragma unroll
@@ -84,13 +86,15 @@ __global__ void __launch_bounds__(kBlock) compact_count_kernel(const uint64_t* _
 // kPartition the others go after all kept ones (at n_kept + rejects before them).
 template <int W, int MODE>
 __global__ void __launch_bounds__(kBlock) compact_scatter_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps,
-                                                            uint64_t cap, uint64_t n, double eps, const CondList cl,
+                                                            uint64_t cap, uint64_t n, const uint64_t* __restrict__ n_dev, double eps,
+                                                            const CondList cl,
                                                             const uint64_t* __restrict__ offs, const uint64_t* __restrict__ total,
                                                             uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out) {
     constexpr int kWaves = kBlock / 64;
     __shared__ uint32_t row_wave[kItems * kWaves];   // kept count per (row, wave), then its exclusive prefix
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t tile = (uint64_t)blockIdx.x * kTile;
+    if (n_dev) n = min(n, n_dev[0]);
     uint32_t flags = 0;
     uint32_t lane_pre[kItems];
     double2 sums[kItems];
@@ -136,7 +140,7 @@ static uint64_t run_compact(State& st, double eps, const CondList& cl) {
     uint64_t* offs = dalloc<uint64_t>(nb + 1, st.stream);
     {
         SPQ_PROF(st.stream, "compact_count_kernel", n * (MODE == kClearZero ? 16.0 : MODE == kPartition ? 8.0 * W : 8.0 * W + 16));
-        compact_count_kernel<W, MODE><<<nb, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, eps, cl, counts);
+        compact_count_kernel<W, MODE><<<nb, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, nullptr, eps, cl, counts);
     }
     SPQ_HIP(hipGetLastError());
     exclusive_scan_u32(st, counts, offs, nb, offs + nb);
@@ -153,7 +157,7 @@ static uint64_t run_compact(State& st, double eps, const CondList& cl) {
     alloc_arrays(st, st.W, cap_out, &kout, &aout);
     {
         SPQ_PROF(st.stream, "compact_scatter_kernel", n * (MODE == kClearZero ? 16.0 : MODE == kPartition ? 8.0 * W : 8.0 * W + 16) + (double)out_n * (8.0 * W + 16));
-        compact_scatter_kernel<W, MODE><<<nb, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, eps, cl, offs, offs + nb, kout,
+        compact_scatter_kernel<W, MODE><<<nb, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, nullptr, eps, cl, offs, offs + nb, kout,
                                                                    aout, cap_out);
     }
     SPQ_HIP(hipGetLastError());
@@ -180,6 +184,49 @@ void clear_zero(State& st, double eps) {
     if (st.n == 0) return;
     uint64_t kept;
     SPQ_DISPATCH_W(st.W, clear_zero_w, st, eps, &kept);
+}
+
+// ClearZero of entries [0, counts[2]) of (keys, amps, cap), with the entry count
+// known only on the device.  counts[3] receives the kept count and counts[0..4)
+// is read back into info in the same transfer.  The result is (keys, amps)
+// itself when nothing is pruned, otherwise freshly allocated arrays.
+template <int W>
+static void clear_zero_sized_w(State& st, uint64_t* keys, double2* amps, uint64_t cap, uint64_t* counts, double eps, uint64_t* info,
+                               uint64_t** kout, double2** aout) {
+    const unsigned nb = grid_for(cap, kTile);
+    uint32_t* tile_counts = dalloc<uint32_t>(nb, st.stream);
+    uint64_t* offs = dalloc<uint64_t>(nb + 1, st.stream);
+    {
+        SPQ_PROF(st.stream, "compact_count_kernel", cap * 16.0);
+        compact_count_kernel<W, kClearZero><<<nb, kBlock, 0, st.stream>>>(keys, amps, cap, cap, counts + 2, eps, CondList{}, tile_counts);
+    }
+    SPQ_HIP(hipGetLastError());
+    exclusive_scan_u32(st, tile_counts, offs, nb, offs + nb);
+    SPQ_HIP(hipMemcpyAsync(counts + 3, offs + nb, sizeof(uint64_t), hipMemcpyDeviceToDevice, st.stream));
+    read_back(st, counts, 4 * sizeof(uint64_t), info);
+    const uint64_t valid = info[2], kept = info[3];
+    *kout = keys;
+    *aout = amps;
+    if (kept != valid) {
+        uint64_t* kk;
+        double2* aa;
+        alloc_arrays(st, st.W, std::max<uint64_t>(kept, 1), &kk, &aa);
+        {
+            SPQ_PROF(st.stream, "compact_scatter_kernel", valid * 16.0 + (double)kept * (8.0 * W + 16));
+            compact_scatter_kernel<W, kClearZero><<<nb, kBlock, 0, st.stream>>>(keys, amps, cap, cap, counts + 2, eps, CondList{}, offs,
+                                                                              offs + nb, kk, aa, std::max<uint64_t>(kept, 1));
+        }
+        SPQ_HIP(hipGetLastError());
+        *kout = kk;
+        *aout = aa;
+    }
+    dfree(tile_counts, st.stream);
+    dfree(offs, st.stream);
+}
+
+void clear_zero_sized(State& st, uint64_t* keys, double2* amps, uint64_t cap, uint64_t* counts, double eps, uint64_t info[4],
+                      uint64_t** kout, double2** aout) {
+    SPQ_DISPATCH_W(st.W, clear_zero_sized_w, st, keys, amps, cap, counts, eps, info, kout, aout);
 }
 
 void merge_sorted_duplicates(State& st, double eps) {

@@ -40,6 +40,31 @@ struct Error : std::exception {
         if (!(cond)) throw ::sparq::Error(SPARQ_INVALID, msg);  \
     } while (0)
 
+// C-ABI wrapper: runs f, maps exceptions to status codes and records the
+// message for sparq_last_error (api.hip).
+void set_last_error(const std::string& msg);
+template <class F>
+int guarded_call(F&& f) {
+    try {
+        f();
+        return SPARQ_OK;
+    } catch (const Error& e) {
+        set_last_error(e.msg);
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return SPARQ_OOM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return SPARQ_RUNTIME;
+    } catch (...) {
+        set_last_error("unknown error");
+        return SPARQ_RUNTIME;
+    }
+}
+// Selects device 0 on first use; throws SPARQ_NO_DEVICE without a GPU.
+void ensure_device();
+
 constexpr int kBlock = 256;           // threads per workgroup (4 waves of 64)
 constexpr int kWave = 64;
 
@@ -92,11 +117,30 @@ struct Digit { uint8_t word, shift, bits, pad; };
 void sort_by_plan(State& st, const std::vector<Digit>& plan);
 // One pass over the keys: bits that vary across the state, and whether the
 // state is already ordered with the optional field `last` least significant.
+// With a SizerQuery the same pass also histograms popcount(value of `last`)
+// into kStatsHistBins bins and the sizer's feature kernel runs on the result
+// (and its network when active) before the single readback.
+constexpr int kStatsHistBins = 6;
+struct SizerQuery {
+    uint64_t n;
+    uint32_t words, nd, kind;          // kind: 0 Hadamard_Int, 1 Hadamard_Int_Full, 2 dense group unitary
+    uint32_t target_word, target_shift;
+};
 struct KeyStats {
     std::vector<uint64_t> vary;
     bool sorted = true;
+    bool have_features = false, have_prediction = false;
+    float features[SPARQ_SIZER_FEATURES] = {};
+    float prediction = 0.f;            // network output: log2(G) / 32
 };
-KeyStats key_stats(State& st, const sparq_field_t* last);
+KeyStats key_stats(State& st, const sparq_field_t* last, const SizerQuery* q = nullptr);
+
+// ---- learned sizer (sizer.hip)
+bool sizer_wanted(bool& predict);     // logging or prediction requested; predict = network active
+void launch_sizer_features(State& st, const uint64_t* stats, const SizerQuery& q, bool predict, float* out);
+float sizer_margin();
+void sizer_record(const float* features, float target);
+void sizer_count(int what);           // 0 exact path, 1 predicted, 2 overflow fallback
 // LSD plan over the varying bits with `last` moved to the least significant place.
 std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t* last, int digit_bits);
 // Sort unless already ordered; returns whether a sort ran.
@@ -104,6 +148,10 @@ bool sort_state(State& st, const sparq_field_t* last);
 
 // ---- compaction (compact.hip)
 void clear_zero(State& st, double eps);
+// ClearZero of (keys, amps, cap) holding counts[2] entries (a device word); reads
+// counts[0..4) back into info with the kept count in info[3].
+void clear_zero_sized(State& st, uint64_t* keys, double2* amps, uint64_t cap, uint64_t* counts, double eps, uint64_t info[4],
+                      uint64_t** kout, double2** aout);
 void merge_sorted_duplicates(State& st, double eps);
 bool has_sorted_duplicates(State& st);
 

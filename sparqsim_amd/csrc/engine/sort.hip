@@ -51,52 +51,55 @@ __device__ __forceinline__ bool out_of_order(const uint64_t (&p)[W], const uint6
     return false;
 }
 
+// Record of R = 2W + 1 + kStatsHistBins words: [W] OR, [W] AND, [1] unsorted
+// flag, [kStatsHistBins] popcount histogram of `hist` (zero without one).
+template <int W>
+__device__ __forceinline__ uint64_t stats_combine(int j, uint64_t x, uint64_t y) {
+    if (j < W) return x | y;
+    if (j < 2 * W) return x & y;
+    if (j == 2 * W) return x | y;
+    return x + y;
+}
+
 template <int W>
 __global__ void __launch_bounds__(kBlock) key_stats_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, StatsOrder so,
-                                                      uint64_t* __restrict__ partial, unsigned int* __restrict__ counter,
-                                                      uint64_t* __restrict__ out /* [W] or, [W] and, [1] unsorted */) {
-    __shared__ uint64_t lds[4][2 * W + 1];
+                                                      sparq_field_t hist, uint64_t* __restrict__ partial, unsigned int* __restrict__ counter,
+                                                      uint64_t* __restrict__ out) {
+    constexpr int R = 2 * W + 1 + kStatsHistBins;
+    __shared__ uint64_t lds[kBlock / 64][R];
     __shared__ bool is_last;
-    uint64_t o[W], a[W], bad = 0;
+    uint64_t r[R];
 #pragma unroll
-    for (int w = 0; w < W; ++w) { o[w] = 0; a[w] = ~0ull; }
+    for (int j = 0; j < R; ++j) r[j] = (j >= W && j < 2 * W) ? ~0ull : 0;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         uint64_t k[W];
         load_key<W>(keys, cap, i, k);
 #pragma unroll
-        for (int w = 0; w < W; ++w) { o[w] |= k[w]; a[w] &= k[w]; }
-        if (i > 0 && !bad) {
+        for (int w = 0; w < W; ++w) { r[w] |= k[w]; r[W + w] &= k[w]; }
+        if (i > 0 && !r[2 * W]) {
             uint64_t p[W];
             load_key<W>(keys, cap, i - 1, p);
-            bad = out_of_order<W>(p, k, so) ? 1 : 0;
+            r[2 * W] = out_of_order<W>(p, k, so) ? 1 : 0;
+        }
+        if (hist.width) {
+            const int b = min(__popcll(fget<W>(k, hist)), kStatsHistBins - 1);
+#pragma unroll
+            for (int h = 0; h < kStatsHistBins; ++h) r[2 * W + 1 + h] += (h == b);
         }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            o[w] |= __shfl_xor(o[w], off, 64);
-            a[w] &= __shfl_xor(a[w], off, 64);
-        }
-        bad |= __shfl_xor(bad, off, 64);
-    }
+        for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], __shfl_xor(r[j], off, 64));
     const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == 0)
 #pragma unroll
-        for (int w = 0; w < W; ++w) { lds[wave][w] = o[w]; lds[wave][W + w] = a[w]; }
-        lds[wave][2 * W] = bad;
-    }
+        for (int j = 0; j < R; ++j) lds[wave][j] = r[j];
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t r[2 * W + 1];
-#pragma unroll
-        for (int j = 0; j < W; ++j) { r[j] = 0; r[W + j] = ~0ull; }
-        r[2 * W] = 0;
-        for (int v = 0; v < 4; ++v) {
-            for (int j = 0; j < W; ++j) { r[j] |= lds[v][j]; r[W + j] &= lds[v][W + j]; }
-            r[2 * W] |= lds[v][2 * W];
-        }
-        for (int j = 0; j < 2 * W + 1; ++j) partial[(uint64_t)blockIdx.x * (2 * W + 1) + j] = r[j];
+        for (int v = 1; v < kBlock / 64; ++v)
+            for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], lds[v][j]);
+        for (int j = 0; j < R; ++j) partial[(uint64_t)blockIdx.x * R + j] = r[j];
         __threadfence();
         unsigned prev = atomicAdd(counter, 1u);
         is_last = (prev == gridDim.x - 1);
@@ -105,58 +108,41 @@ __global__ void __launch_bounds__(kBlock) key_stats_kernel(const uint64_t* __res
     if (is_last) {
         // the last block combines all partials, one block-strided slice per thread
         __threadfence();
-        uint64_t r[2 * W + 1];
 #pragma unroll
-        for (int j = 0; j < W; ++j) { r[j] = 0; r[W + j] = ~0ull; }
-        r[2 * W] = 0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        for (int j = 0; j < R; ++j) r[j] = (j >= W && j < 2 * W) ? ~0ull : 0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
 #pragma unroll
-            for (int j = 0; j < 2 * W + 1; ++j) {
-                uint64_t v = __hip_atomic_load(&partial[(uint64_t)b * (2 * W + 1) + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (j < W) r[j] |= v;
-                else if (j < 2 * W) r[j] &= v;
-                else r[j] |= v;
-            }
-        }
+            for (int j = 0; j < R; ++j)
+                r[j] = stats_combine<W>(j, r[j], __hip_atomic_load(&partial[(uint64_t)b * R + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
+        for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-            for (int j = 0; j < W; ++j) {
-                r[j] |= __shfl_xor(r[j], off, 64);
-                r[W + j] &= __shfl_xor(r[W + j], off, 64);
-            }
-            r[2 * W] |= __shfl_xor(r[2 * W], off, 64);
-        }
+            for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], __shfl_xor(r[j], off, 64));
         __syncthreads();
-        if ((threadIdx.x & 63) == 0) {
+        if ((threadIdx.x & 63) == 0)
 #pragma unroll
-            for (int j = 0; j < 2 * W + 1; ++j) lds[wave][j] = r[j];
-        }
+            for (int j = 0; j < R; ++j) lds[wave][j] = r[j];
         __syncthreads();
         if (threadIdx.x == 0) {
-            for (int v = 1; v < 4; ++v)
-                for (int j = 0; j < 2 * W + 1; ++j) {
-                    if (j < W) r[j] |= lds[v][j];
-                    else if (j < 2 * W) r[j] &= lds[v][j];
-                    else r[j] |= lds[v][j];
-                }
-            for (int j = 0; j < 2 * W + 1; ++j) out[j] = r[j];
+            for (int v = 1; v < kBlock / 64; ++v)
+                for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], lds[v][j]);
+            for (int j = 0; j < R; ++j) out[j] = r[j];
             *counter = 0;   // ready for the next launch on this stream
         }
     }
 }
 
 template <int W>
-static void launch_key_stats(State& st, const StatsOrder& so, uint64_t* partial, uint64_t* out) {
+static void launch_key_stats(State& st, const StatsOrder& so, sparq_field_t hist, uint64_t* partial, uint64_t* out) {
     unsigned g = std::min<unsigned>(grid_for(st.n, kBlock * 8), 512);
     {
         SPQ_PROF(st.stream, "key_stats_kernel", st.n * 8.0 * W);
-        key_stats_kernel<W><<<g, kBlock, 0, st.stream>>>(st.keys, st.cap, st.n, so, partial, st.counter, out);
+        key_stats_kernel<W><<<g, kBlock, 0, st.stream>>>(st.keys, st.cap, st.n, so, hist, partial, st.counter, out);
     }
     SPQ_HIP(hipGetLastError());
 }
 
-KeyStats key_stats(State& st, const sparq_field_t* last) {
+KeyStats key_stats(State& st, const sparq_field_t* last, const SizerQuery* q) {
     KeyStats ks;
     ks.vary.assign(st.W, 0);
     ks.sorted = true;
@@ -167,15 +153,29 @@ KeyStats key_stats(State& st, const sparq_field_t* last) {
         so.last = *last;
         so.others[last->word] &= ~(fmask_host(last->width) << last->shift);
     }
-    uint64_t* partial = dalloc<uint64_t>((size_t)512 * (2 * st.W + 1), st.stream);
-    uint64_t* out = dalloc<uint64_t>(2 * st.W + 1, st.stream);
-    SPQ_DISPATCH_W(st.W, launch_key_stats, st, so, partial, out);
-    std::vector<uint64_t> h(2 * st.W + 1);
+    bool predict = false;
+    const bool sized = q && last && sizer_wanted(predict);
+    sparq_field_t hist{0, 0, 0, 0};
+    if (sized) hist = *last;
+    const size_t R = 2 * st.W + 1 + kStatsHistBins;
+    const size_t fwords = (SPARQ_SIZER_FEATURES + 2) / 2;   // features + prediction, as floats
+    uint64_t* partial = dalloc<uint64_t>((size_t)512 * R, st.stream);
+    uint64_t* out = dalloc<uint64_t>(R + fwords, st.stream);
+    SPQ_DISPATCH_W(st.W, launch_key_stats, st, so, hist, partial, out);
+    if (sized) launch_sizer_features(st, out, *q, predict, reinterpret_cast<float*>(out + R));
+    std::vector<uint64_t> h(R + (sized ? fwords : 0));
     read_back(st, out, h.size() * 8, h.data());
     dfree(partial, st.stream);
     dfree(out, st.stream);
     for (uint32_t w = 0; w < st.W; ++w) ks.vary[w] = h[w] ^ h[st.W + w];
     ks.sorted = h[2 * st.W] == 0;
+    if (sized) {
+        const float* f = reinterpret_cast<const float*>(h.data() + R);
+        std::copy(f, f + SPARQ_SIZER_FEATURES, ks.features);
+        ks.have_features = true;
+        ks.prediction = f[SPARQ_SIZER_FEATURES];
+        ks.have_prediction = predict;
+    }
     return ks;
 }
 
