@@ -63,10 +63,8 @@ def cpu_baseline(address_bits, data_bits, mem_words, budget_s=10.0):
     return circ.entry_ops * reps / total, total, reps
 
 
-def qda_walk_leg(ps, nqubit, nsteps, total_steps):
-    """The 2^nqubit Poiseuille QDA solve (A = tridiag(-1, 2.2666666666564934, -1),
-    b = ones/sqrt(n), kappa 16, p 1.3, ClearZero eps 1e-16) on the engine, timed over
-    its first `nsteps` walk steps of a `total_steps` schedule (one step = W(s) + ClearZero)."""
+def qda_walk_run(ps, nqubit, nsteps, total_steps):
+    """Steps 0..nsteps-1 of the 2^nqubit Poiseuille walk; returns (sizes, seconds per step)."""
     from sparqsim_amd import qda
 
     alpha, beta, kappa, p = 2.2666666666564934, -1.0, 16.0, 1.3
@@ -85,6 +83,28 @@ def qda_walk_leg(ps, nqubit, nsteps, total_steps):
             times.append(time.perf_counter() - t0)
     finally:
         ps.set_clearzero_default_eps(old_eps)
+    return sizes, times
+
+
+def qda_walk_leg(ps, nqubit, nsteps, total_steps):
+    """The 2^nqubit Poiseuille QDA solve (A = tridiag(-1, 2.2666666666564934, -1),
+    b = ones/sqrt(n), kappa 16, p 1.3, ClearZero eps 1e-16) on the engine, timed over
+    its first `nsteps` walk steps of a `total_steps` schedule (one step = W(s) +
+    ClearZero), once on the plain path and once with the learned sizer."""
+    from sparqsim_amd import sizer
+
+    sizer.enable(False)
+    sizes, times = qda_walk_run(ps, nqubit, nsteps, total_steps)
+    learned = None
+    if sizer.activate_default():
+        s0 = sizer.stats()
+        lsizes, ltimes = qda_walk_run(ps, nqubit, nsteps, total_steps)
+        s1 = sizer.stats()
+        sizer.enable(False)
+        assert lsizes == sizes, "the learned sizer changed the walk"
+        learned = {"wall_s": sum(ltimes), "ms_per_step": sum(ltimes) / nsteps * 1e3,
+                   "sized_ops_predicted": s1["predicted"] - s0["predicted"], "reruns": s1["reruns"] - s0["reruns"],
+                   "speedup_vs_plain": sum(times) / sum(ltimes)}
     wall = sum(times)
     return {
         "workload": f"2^{nqubit} Poiseuille QDA walk (tridiag(-1, 2.2666666666564934, -1), kappa 16, p 1.3, "
@@ -94,6 +114,7 @@ def qda_walk_leg(ps, nqubit, nsteps, total_steps):
         "entries_first": sizes[0], "entries_last": sizes[-1], "entries_peak": max(sizes),
         "entry_steps_per_s": sum(sizes) / wall,
         "last_step_ms": times[-1] * 1e3,
+        "learned_sizer": learned,
         "reference_published": "1.33 s/step at ~33.5M entries, 1x A800 (docs/verification/v1d-qda-linsolver-a800.md:52)",
     }
 
@@ -118,6 +139,14 @@ def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
     for t in range(plain_traj):
         g.run_plain(t).size()
     t_plain = (time.perf_counter() - t0) * n_traj / plain_traj
+    t_learned = None
+    from sparqsim_amd import sizer
+
+    if sizer.activate_default():
+        t0 = time.perf_counter()
+        g.run_batched().size()
+        t_learned = time.perf_counter() - t0
+        sizer.enable(False)
     # each iteration applies 12 reference operators to every entry of every trajectory
     entry_ops = 12 * iterations * n_final
     return {
@@ -125,8 +154,10 @@ def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
                     f"Pauli X/Z insertions p={p} per addr qubit per iteration ({n_events} events)",
         "entries_final": n_final, "batched_s": t_batched, "batched_ms_per_iteration": t_batched / iterations * 1e3,
         "batched_entries_per_s": entry_ops / t_batched,
+        "batched_learned_s": t_learned,
         "plain_s": t_plain, "plain_sample_trajectories": plain_traj,
         "batched_speedup_vs_plain": t_plain / t_batched,
+        "batched_learned_speedup_vs_plain": (t_plain / t_learned) if t_learned else None,
     }
 
 

@@ -64,6 +64,8 @@ __global__ void __launch_bounds__(kBlock) classify_kernel(const uint64_t* __rest
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(n_wht, (unsigned long long)__popcll(b));
 }
 
+constexpr int kExpandItems = 4;   // outputs per thread (strided by the block size)
+
 // Output entry o = g*F + j: key of the group head with the low target bits = j.
 // DIRECT and SPARSE_SUM amplitudes are produced here; WHT blocks start at 0.
 // The group count G is read from the device; the output holds G_cap groups
@@ -77,7 +79,9 @@ __global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __r
     const uint64_t G = G_dev[0];
     if (G > G_cap) return;
     const uint64_t M = G << nd;
-    uint64_t o = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+#pragma unroll
+    for (int it = 0; it < kExpandItems; ++it) {
+    const uint64_t o = ((uint64_t)blockIdx.x * kExpandItems + it) * kBlock + threadIdx.x;
     if (o >= M) return;
     const uint64_t g = o >> nd, j = o & ((uint64_t(1) << nd) - 1);
     const uint64_t h = start[g];
@@ -113,6 +117,32 @@ __global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __r
         }
     }
     aout[o] = r;
+    }
+}
+
+// The DIRECT case of a state with one entry (H on a basis state): pure
+// streaming writes of F = 2^nd outputs, 8 per thread, no per-output loads.
+template <int W>
+__global__ void __launch_bounds__(kBlock) expand_single_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps,
+                                                          uint64_t cap, sparq_field_t low, uint32_t nd, double extra,
+                                                          uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out) {
+    uint64_t k[W];
+    load_key<W>(keys, cap, 0, k);
+    const uint64_t vlow = fget<W>(k, low);
+    const double2 t = cscale(amps[0], extra);                   // hadamard.cpp:30-39
+    const double2 tn = make_double2(t.x * -1.0, t.y * -1.0);
+    const uint64_t F = uint64_t(1) << nd;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const uint64_t o = ((uint64_t)blockIdx.x * 8 + it) * kBlock + threadIdx.x;
+        if (o >= F) return;
+        uint64_t ko[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) ko[w] = k[w];
+        fput<W>(ko, low, o);
+        store_key<W>(kout, cap_out, o, ko);
+        aout[o] = (__popcll(o & vlow) & 1) ? tn : t;
+    }
 }
 
 // Place the members of WHT groups at their slot of the zeroed block.
@@ -140,6 +170,11 @@ __global__ void __launch_bounds__(kBlock) expand_scatter_kernel(const uint64_t* 
 // applies up to 3 consecutive stages to 8 elements in registers per round.
 constexpr int kWhtTileLog = 11;
 
+// LDS index of tile element e: one pad slot per 16 elements, so that the
+// 8-element register groups of neighbouring lanes (stride 8 or 64 elements)
+// fall in different banks.
+__host__ __device__ constexpr uint32_t lpad(uint32_t e) { return e + (e >> 4); }
+
 // Stages [s0, s0+cnt) of tile bits, applied in rounds of <= 3 stages.  Groups
 // of 2^k elements differing only in the round's stage bits are handled by one
 // thread; `base_index` is the tile's first index in the output array (for the
@@ -157,7 +192,7 @@ __device__ __forceinline__ void lds_wht_rounds(double2* buf, uint32_t T, uint32_
             double2 v[8];
 #pragma unroll
             for (uint32_t j = 0; j < 8; ++j)
-                if (j < (1u << k)) v[j] = buf[first | (j << sb)];
+                if (j < (1u << k)) v[j] = buf[lpad(first | (j << sb))];
 #pragma unroll
             for (uint32_t r = 0; r < 3; ++r) {
                 if (r >= k) break;
@@ -172,7 +207,7 @@ __device__ __forceinline__ void lds_wht_rounds(double2* buf, uint32_t T, uint32_
             }
 #pragma unroll
             for (uint32_t j = 0; j < 8; ++j)
-                if (j < (1u << k)) buf[first | (j << sb)] = v[j];
+                if (j < (1u << k)) buf[lpad(first | (j << sb))] = v[j];
         }
         __syncthreads();
         done += k;
@@ -184,7 +219,7 @@ __device__ __forceinline__ void lds_wht_rounds(double2* buf, uint32_t T, uint32_
 // G_dev = {group count, WHT group count}; tiles past G << nd do nothing.
 __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restrict__ a, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
                                                            uint32_t nd, uint32_t cnt, const uint8_t* __restrict__ cls) {
-    __shared__ double2 buf[1 << kWhtTileLog];
+    __shared__ double2 buf[lpad(1 << kWhtTileLog)];
     const uint64_t G = G_dev[0];
     if (G > G_cap || G_dev[1] == 0) return;
     const uint64_t M = G << nd;
@@ -192,10 +227,26 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
     if (base >= M) return;
     if (nd >= (uint32_t)kWhtTileLog && cls[base >> nd] != kWht) return;   // the tile lies in one non-WHT group
     const uint32_t T = (uint32_t)min<uint64_t>(uint64_t(1) << kWhtTileLog, M - base);
-    for (uint32_t e = threadIdx.x; e < T; e += kBlock) buf[e] = a[base + e];
+    // all loads of the tile are issued before the first LDS store
+    constexpr int kPer = (1 << kWhtTileLog) / kBlock;
+    double2 v[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t e = r * kBlock + threadIdx.x;
+        if (e < T) v[r] = a[base + e];
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t e = r * kBlock + threadIdx.x;
+        if (e < T) buf[lpad(e)] = v[r];
+    }
     __syncthreads();
     lds_wht_rounds(buf, T, 0, cnt, true, base, nd, cls);
-    for (uint32_t e = threadIdx.x; e < T; e += kBlock) a[base + e] = buf[e];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t e = r * kBlock + threadIdx.x;
+        if (e < T) a[base + e] = buf[lpad(e)];
+    }
 }
 
 // Later passes: stages [lo, lo+cnt) of one group, on tiles of 2^cnt x 2^c2
@@ -204,7 +255,7 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
 constexpr int kWhtPassTileLog = 12;
 __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ a, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
                                                      uint32_t nd, uint32_t lo, uint32_t cnt, uint32_t c2, const uint8_t* __restrict__ cls) {
-    __shared__ double2 buf[1 << kWhtPassTileLog];
+    __shared__ double2 buf[lpad(1 << kWhtPassTileLog)];
     const uint32_t T = 1u << (cnt + c2);
     const uint64_t tile = blockIdx.x;
     const uint64_t per_group = (uint64_t(1) << nd) >> (cnt + c2);
@@ -216,16 +267,27 @@ __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ 
     const uint64_t nlowchunks = (uint64_t(1) << lo) >> c2;
     const uint64_t lowchunk = r % nlowchunks, high = r / nlowchunks;
     const uint64_t base = (g << nd) + (high << (lo + cnt)) + (lowchunk << c2);
-    // element e of the tile: sub = e >> c2 (stage index bits), l = e & (2^c2-1)
-    for (uint32_t e = threadIdx.x; e < T; e += kBlock) {
-        uint64_t sub = e >> c2, l = e & ((1u << c2) - 1);
-        buf[e] = a[base + (sub << lo) + l];
+    // element e of the tile: sub = e >> c2 (stage index bits), l = e & (2^c2-1);
+    // loads in groups of 8 per thread, all issued before their LDS stores
+    constexpr int kPer = 8;
+    for (uint32_t e0 = 0; e0 < T; e0 += kPer * kBlock) {
+        double2 v[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t e = e0 + r * kBlock + threadIdx.x;
+            if (e < T) v[r] = a[base + ((uint64_t)(e >> c2) << lo) + (e & ((1u << c2) - 1))];
+        }
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t e = e0 + r * kBlock + threadIdx.x;
+            if (e < T) buf[lpad(e)] = v[r];
+        }
     }
     __syncthreads();
     lds_wht_rounds(buf, T, c2, cnt, false, 0, nd, cls);
     for (uint32_t e = threadIdx.x; e < T; e += kBlock) {
         uint64_t sub = e >> c2, l = e & ((1u << c2) - 1);
-        a[base + (sub << lo) + l] = buf[e];
+        a[base + (sub << lo) + l] = buf[lpad(e)];
     }
 }
 
@@ -243,6 +305,34 @@ __global__ void single_group_kernel(uint64_t* __restrict__ start, uint8_t* __res
 __global__ void sized_total_kernel(uint64_t* __restrict__ counts, uint64_t G_cap, uint32_t nd) {
     const uint64_t G = counts[0];
     counts[2] = (G > G_cap) ? 0 : (G << nd);
+}
+
+// Butterfly passes over G_cap blocks of 2^nd amplitudes (the WHT groups among
+// the first counts[0] blocks): the first pass covers min(nd, 11) stages on
+// contiguous tiles; the rest are split evenly into passes of <= 9 stages on
+// column tiles of one group (2^cnt rows of 2^c2 contiguous elements, up to
+// 2^12 elements).
+static void launch_wht(State& st, double2* a, const uint64_t* counts, uint64_t G_cap, uint32_t nd, const uint8_t* cls, double bytes) {
+    const uint64_t M_cap = G_cap << nd;
+    uint32_t lo = std::min<uint32_t>(nd, kWhtTileLog);
+    {
+        SPQ_PROF(st.stream, "wht_first_pass_kernel", bytes);
+        wht_first_pass_kernel<<<grid_for(M_cap, 1u << kWhtTileLog), kBlock, 0, st.stream>>>(a, counts, G_cap, nd, lo, cls);
+    }
+    SPQ_HIP(hipGetLastError());
+    while (lo < nd) {
+        const uint32_t rem = nd - lo;
+        const uint32_t passes = (rem + 8) / 9;
+        const uint32_t cnt = (rem + passes - 1) / passes;
+        const uint32_t c2 = std::min<uint32_t>(lo, kWhtPassTileLog - cnt);
+        const uint64_t tiles = M_cap >> (cnt + c2);
+        {
+            SPQ_PROF(st.stream, "wht_pass_kernel", bytes);
+            wht_pass_kernel<<<(unsigned)tiles, kBlock, 0, st.stream>>>(a, counts, G_cap, nd, lo, cnt, c2, cls);
+        }
+        SPQ_HIP(hipGetLastError());
+        lo += cnt;
+    }
 }
 
 template <int W>
@@ -285,6 +375,33 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
         cls = dalloc<uint8_t>(1, st.stream);
         single_group_kernel<<<1, 1, 0, st.stream>>>(start, cls, n, c, counts);
         SPQ_HIP(hipGetLastError());
+        if (c == kDirect) {
+            uint64_t* kout;
+            double2* aout;
+            alloc_arrays(st, st.W, F, &kout, &aout);
+            {
+                SPQ_PROF(st.stream, "expand_single_kernel", F * (8.0 * W + 16));
+                expand_single_kernel<W><<<grid_for(F, kBlock * 8), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, low, nd,
+                                                                                         1.0 / std::sqrt((double)F), kout, aout, F);
+            }
+            SPQ_HIP(hipGetLastError());
+            dfree(start, st.stream);
+            dfree(cls, st.stream);
+            dfree(counts, st.stream);
+            install_arrays(st, st.W, kout, aout, F, F);
+            clear_zero(st, eps);
+            return;
+        }
+        if (c == kWht && n == F) {
+            // One full group in target order: the output keys are the input keys,
+            // so the butterflies run in place on the amplitudes.
+            launch_wht(st, st.amps, counts, 1, nd, cls, (double)F * 32.0);
+            dfree(start, st.stream);
+            dfree(cls, st.stream);
+            dfree(counts, st.stream);
+            clear_zero(st, eps);
+            return;
+        }
     } else {
         head = dalloc<uint32_t>(n, st.stream);
         pos = dalloc<uint64_t>(n + 2, st.stream);   // [n] = group count, [n+1] = WHT group count
@@ -343,7 +460,7 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     const double extra = 1.0 / std::sqrt((double)F);
     {
         SPQ_PROF(st.stream, "expand_init_kernel", M_cap * (8.0 * W + 16));
-        expand_init_kernel<W><<<grid_for(M_cap, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, start, cls, nd, low, extra, counts,
+        expand_init_kernel<W><<<grid_for(M_cap, kBlock * kExpandItems), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, start, cls, nd, low, extra, counts,
                                                                              G_cap, kout, aout, M_cap, dense);
     }
     SPQ_HIP(hipGetLastError());
@@ -354,29 +471,7 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
                                                                                      counts, G_cap, aout);
         }
         SPQ_HIP(hipGetLastError());
-        // Butterfly passes: the first covers min(nd, 11) stages on contiguous tiles;
-        // the rest are split evenly into passes of <= 9 stages on column tiles of
-        // one group (2^cnt rows of 2^c2 contiguous elements, up to 2^12 elements).
-        const double wht_bytes = (double)(learned ? G_cap : n_wht) * F * 32.0;
-        uint32_t lo = std::min<uint32_t>(nd, kWhtTileLog);
-        {
-            SPQ_PROF(st.stream, "wht_first_pass_kernel", wht_bytes);
-            wht_first_pass_kernel<<<grid_for(M_cap, 1u << kWhtTileLog), kBlock, 0, st.stream>>>(aout, counts, G_cap, nd, lo, cls);
-        }
-        SPQ_HIP(hipGetLastError());
-        while (lo < nd) {
-            const uint32_t rem = nd - lo;
-            const uint32_t passes = (rem + 8) / 9;
-            const uint32_t cnt = (rem + passes - 1) / passes;
-            const uint32_t c2 = std::min<uint32_t>(lo, kWhtPassTileLog - cnt);
-            const uint64_t tiles = M_cap >> (cnt + c2);
-            {
-                SPQ_PROF(st.stream, "wht_pass_kernel", wht_bytes);
-                wht_pass_kernel<<<(unsigned)tiles, kBlock, 0, st.stream>>>(aout, counts, G_cap, nd, lo, cnt, c2, cls);
-            }
-            SPQ_HIP(hipGetLastError());
-            lo += cnt;
-        }
+        launch_wht(st, aout, counts, G_cap, nd, cls, (double)(learned ? G_cap : n_wht) * F * 32.0);
     }
     dfree(head, st.stream);
     dfree(pos, st.stream);

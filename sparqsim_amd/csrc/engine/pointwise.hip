@@ -244,23 +244,46 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
     }
 }
 
+template <int W>
+constexpr int kPwItems() { return W <= 2 ? 4 : (W <= 4 ? 2 : 1); }
+
 template <int W, bool AMP>
 __global__ void __launch_bounds__(kBlock) pointwise_kernel(uint64_t* __restrict__ keys, double2* __restrict__ amps, uint64_t cap,
                                                       uint64_t n, const PwProgram prog) {
-    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint64_t k[W];
-    load_key<W>(keys, cap, i, k);
-    double2 a = make_double2(0.0, 0.0);
-    if (AMP) a = amps[i];
+    // kPwItems(W) entries per thread (striped by the block size), so that the
+    // table gathers of several entries are in flight together.
+    constexpr int kI = kPwItems<W>();
+    const uint64_t base = (uint64_t)blockIdx.x * kBlock * kI + threadIdx.x;
+    uint64_t k[kI][W];
+    double2 a[kI];
+#pragma unroll
+    for (int it = 0; it < kI; ++it) {
+        const uint64_t i = base + (uint64_t)it * kBlock;
+        a[it] = make_double2(0.0, 0.0);
+        if (i < n) {
+            load_key<W>(keys, cap, i, k[it]);
+            if (AMP) a[it] = amps[i];
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) k[it][w] = 0;
+        }
+    }
     for (uint32_t o = 0; o < prog.nops; ++o) {
         const sparq_pw_op_t& op = prog.ops[o];
-        bool ok = true;
-        for (uint32_t c = 0; c < op.ncond; ++c) ok = ok && cond_ok<W>(k, prog.conds[op.cond_first + c]);
-        if (ok) run_op<W>(op, k, a);
+#pragma unroll
+        for (int it = 0; it < kI; ++it) {
+            bool ok = true;
+            for (uint32_t c = 0; c < op.ncond; ++c) ok = ok && cond_ok<W>(k[it], prog.conds[op.cond_first + c]);
+            if (ok) run_op<W>(op, k[it], a[it]);
+        }
     }
-    store_key<W>(keys, cap, i, k);
-    if (AMP) amps[i] = a;
+#pragma unroll
+    for (int it = 0; it < kI; ++it) {
+        const uint64_t i = base + (uint64_t)it * kBlock;
+        if (i >= n) break;
+        store_key<W>(keys, cap, i, k[it]);
+        if (AMP) amps[i] = a[it];
+    }
 }
 
 static bool touches_amp(uint32_t opcode) {
@@ -275,7 +298,7 @@ static bool touches_amp(uint32_t opcode) {
 
 template <int W>
 static void launch_pointwise(State& st, const PwProgram& prog, bool amp) {
-    unsigned g = grid_for(st.n, kBlock);
+    unsigned g = grid_for(st.n, kBlock * kPwItems<W>());
     SPQ_PROF(st.stream, "pointwise_kernel", st.n * (16.0 * W + (amp ? 32.0 : 0.0)));
     if (amp) pointwise_kernel<W, true><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
     else pointwise_kernel<W, false><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);

@@ -33,8 +33,8 @@ __device__ __forceinline__ uint64_t match_any(uint32_t d, int nbits) {
 // ---------------------------------------------------------------- key statistics
 // One read pass over the keys: bitwise OR and AND per word (bits that vary
 // across the state) and whether the state is already ordered by
-// (all bits except `last`, then `last`).  Per-block partials are combined by
-// the last block to finish (no spin waits).
+// (all bits except `last`, then `last`).  Per-block partial records are
+// combined by a one-block second kernel.
 struct StatsOrder {
     uint64_t others[SPARQ_MAX_WORDS];   // per-word mask of the primary comparison bits
     sparq_field_t last;                 // width 0: none
@@ -63,28 +63,38 @@ __device__ __forceinline__ uint64_t stats_combine(int j, uint64_t x, uint64_t y)
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) key_stats_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, StatsOrder so,
-                                                      sparq_field_t hist, uint64_t* __restrict__ partial, unsigned int* __restrict__ counter,
-                                                      uint64_t* __restrict__ out) {
+                                                      sparq_field_t hist, uint64_t* __restrict__ partial) {
     constexpr int R = 2 * W + 1 + kStatsHistBins;
+    constexpr int kI = W <= 2 ? 8 : (W <= 4 ? 4 : 2);   // entries per thread per tile, loads issued together
     __shared__ uint64_t lds[kBlock / 64][R];
-    __shared__ bool is_last;
     uint64_t r[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) r[j] = (j >= W && j < 2 * W) ? ~0ull : 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
-        uint64_t k[W];
-        load_key<W>(keys, cap, i, k);
+    const uint64_t ntiles = (n + (uint64_t)kBlock * kI - 1) / ((uint64_t)kBlock * kI);
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint64_t k[kI][W], p[kI][W];
 #pragma unroll
-        for (int w = 0; w < W; ++w) { r[w] |= k[w]; r[W + w] &= k[w]; }
-        if (i > 0 && !r[2 * W]) {
-            uint64_t p[W];
-            load_key<W>(keys, cap, i - 1, p);
-            r[2 * W] = out_of_order<W>(p, k, so) ? 1 : 0;
+        for (int j = 0; j < kI; ++j) {
+            const uint64_t i = t * kBlock * kI + (uint64_t)j * kBlock + threadIdx.x;
+            const uint64_t ic = i < n ? i : n - 1;            // clamped: duplicates do not change OR/AND/order
+            const uint64_t ip = ic > 0 ? ic - 1 : 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                k[j][w] = keys[(uint64_t)w * cap + ic];
+                p[j][w] = keys[(uint64_t)w * cap + ip];
+            }
         }
-        if (hist.width) {
-            const int b = min(__popcll(fget<W>(k, hist)), kStatsHistBins - 1);
 #pragma unroll
-            for (int h = 0; h < kStatsHistBins; ++h) r[2 * W + 1 + h] += (h == b);
+        for (int j = 0; j < kI; ++j) {
+            const uint64_t i = t * kBlock * kI + (uint64_t)j * kBlock + threadIdx.x;
+#pragma unroll
+            for (int w = 0; w < W; ++w) { r[w] |= k[j][w]; r[W + w] &= k[j][w]; }
+            r[2 * W] |= out_of_order<W>(p[j], k[j], so) ? 1 : 0;
+            if (hist.width && i < n) {
+                const int b = min(__popcll(fget<W>(k[j], hist)), kStatsHistBins - 1);
+#pragma unroll
+                for (int h = 0; h < kStatsHistBins; ++h) r[2 * W + 1 + h] += (h == b);
+            }
         }
     }
 #pragma unroll
@@ -100,44 +110,52 @@ __global__ void __launch_bounds__(kBlock) key_stats_kernel(const uint64_t* __res
         for (int v = 1; v < kBlock / 64; ++v)
             for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], lds[v][j]);
         for (int j = 0; j < R; ++j) partial[(uint64_t)blockIdx.x * R + j] = r[j];
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) {
-        // the last block combines all partials, one block-strided slice per thread
-        __threadfence();
-#pragma unroll
-        for (int j = 0; j < R; ++j) r[j] = (j >= W && j < 2 * W) ? ~0ull : 0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-#pragma unroll
-            for (int j = 0; j < R; ++j)
-                r[j] = stats_combine<W>(j, r[j], __hip_atomic_load(&partial[(uint64_t)b * R + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-            for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], __shfl_xor(r[j], off, 64));
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0)
-#pragma unroll
-            for (int j = 0; j < R; ++j) lds[wave][j] = r[j];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int v = 1; v < kBlock / 64; ++v)
-                for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], lds[v][j]);
-            for (int j = 0; j < R; ++j) out[j] = r[j];
-            *counter = 0;   // ready for the next launch on this stream
-        }
     }
 }
 
+// Combines the per-block records (the kernel boundary makes them visible; no
+// device-scope fences, which cost an L2 write-back per block on this part).
+template <int W>
+__global__ void __launch_bounds__(kBlock) key_stats_final_kernel(const uint64_t* __restrict__ partial, uint32_t blocks,
+                                                            uint64_t* __restrict__ out) {
+    constexpr int R = 2 * W + 1 + kStatsHistBins;
+    __shared__ uint64_t lds[kBlock / 64][R];
+    uint64_t r[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) r[j] = (j >= W && j < 2 * W) ? ~0ull : 0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += kBlock)
+#pragma unroll
+        for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], partial[(uint64_t)b * R + j]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], __shfl_xor(r[j], off, 64));
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int j = 0; j < R; ++j) lds[wave][j] = r[j];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 1; v < kBlock / 64; ++v)
+            for (int j = 0; j < R; ++j) r[j] = stats_combine<W>(j, r[j], lds[v][j]);
+        for (int j = 0; j < R; ++j) out[j] = r[j];
+    }
+}
+
+constexpr unsigned kStatsMaxGrid = 1024;
+
 template <int W>
 static void launch_key_stats(State& st, const StatsOrder& so, sparq_field_t hist, uint64_t* partial, uint64_t* out) {
-    unsigned g = std::min<unsigned>(grid_for(st.n, kBlock * 8), 512);
+    constexpr int kI = W <= 2 ? 8 : (W <= 4 ? 4 : 2);
+    unsigned g = std::min<unsigned>(grid_for(st.n, kBlock * kI), kStatsMaxGrid);
     {
         SPQ_PROF(st.stream, "key_stats_kernel", st.n * 8.0 * W);
-        key_stats_kernel<W><<<g, kBlock, 0, st.stream>>>(st.keys, st.cap, st.n, so, hist, partial, st.counter, out);
+        key_stats_kernel<W><<<g, kBlock, 0, st.stream>>>(st.keys, st.cap, st.n, so, hist, partial);
+    }
+    SPQ_HIP(hipGetLastError());
+    {
+        SPQ_PROF(st.stream, "key_stats_final_kernel", (double)g * (2 * W + 1 + kStatsHistBins) * 8.0);
+        key_stats_final_kernel<W><<<1, kBlock, 0, st.stream>>>(partial, g, out);
     }
     SPQ_HIP(hipGetLastError());
 }
@@ -159,7 +177,7 @@ KeyStats key_stats(State& st, const sparq_field_t* last, const SizerQuery* q) {
     if (sized) hist = *last;
     const size_t R = 2 * st.W + 1 + kStatsHistBins;
     const size_t fwords = (SPARQ_SIZER_FEATURES + 2) / 2;   // features + prediction, as floats
-    uint64_t* partial = dalloc<uint64_t>((size_t)512 * R, st.stream);
+    uint64_t* partial = dalloc<uint64_t>((size_t)kStatsMaxGrid * R, st.stream);
     uint64_t* out = dalloc<uint64_t>(R + fwords, st.stream);
     SPQ_DISPATCH_W(st.W, launch_key_stats, st, so, hist, partial, out);
     if (sized) launch_sizer_features(st, out, *q, predict, reinterpret_cast<float*>(out + R));
