@@ -259,6 +259,12 @@ int sparq_prof_report(sparq_prof_record_t* out, uint32_t capacity, uint32_t* cou
 int sparq_memory_stats(uint64_t out[6]);
 int sparq_release_cached_memory(void);
 
+/* Radix-sort digit width (no counterpart in the reference, which sorts with
+ * comparison sorts): 8 or 11 forces that width for every later sort, 0 (the
+ * default) lets the engine's autotuner pick it per size class by timing the
+ * first sorts of each class with both widths. */
+int sparq_set_radix_bits(int bits);
+
 /* Learned output sizer (no counterpart in the reference, which always counts
  * before it allocates).  A 4-layer FP32 MLP, 16 features -> 64 -> 64 -> 64 -> 1
  * with ReLU, predicts log2(G)/32 for the group count G of Hadamard_Int,

@@ -314,6 +314,13 @@ int sparq_memory_stats(uint64_t out[6]) {
     });
 }
 
+int sparq_set_radix_bits(int bits) {
+    return guarded([&] {
+        SPQ_REQUIRE(bits == 0 || bits == 8 || bits == 11, "digit width must be 0 (auto), 8 or 11");
+        set_radix_bits(bits);
+    });
+}
+
 int sparq_release_cached_memory(void) {
     return guarded([&] {
         if (g_device < 0) return;
@@ -481,9 +488,28 @@ int sparq_pair_op(sparq_state_t h, sparq_field_t target, uint32_t bit, const spa
                   uint32_t ncond, double eps) {
     return guarded([&] {
         SPQ_REQUIRE(params != nullptr, "null params");
+        SPQ_REQUIRE(ncond <= SPARQ_MAX_PROGRAM_CONDS, "too many conditions");
         State& st = S(h);
+        // Conditions that do not read the target bit are evaluated inside the
+        // pair kernels (partners agree on them; failing entries pass through);
+        // otherwise the state is partitioned first.
+        bool inline_ok = true;
+        CondList cl{};
+        cl.n = ncond;
+        for (uint32_t c = 0; c < ncond; ++c) {
+            SPQ_REQUIRE(conds[c].field.word < st.W, "condition field out of range");
+            cl.c[c] = conds[c];
+            const sparq_field_t f = conds[c].field;
+            const uint32_t tb = (uint32_t)target.shift + bit;
+            if (f.word == target.word && tb >= f.shift && tb < (uint32_t)f.shift + f.width) inline_ok = false;
+        }
+        if (inline_ok) {
+            pair_op(st, target, bit, *params, cl);
+            clear_zero(st, eps);
+            return;
+        }
         with_conditions(st, conds, ncond, eps, [&](State& s) {
-            pair_op(s, target, bit, *params, s.n);
+            pair_op(s, target, bit, *params, CondList{});
             clear_zero(s, eps);
         });
     });

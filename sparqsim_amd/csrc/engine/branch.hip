@@ -348,7 +348,7 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     if (n > 1) {
         const SizerQuery q{n, st.W, nd, dense ? 2u : (full ? 1u : 0u), low.word, low.shift};
         ks = key_stats(st, &low, allow_learned ? &q : nullptr);
-        if (!ks.sorted) sort_by_plan(st, make_plan(ks, st.W, &low, 8));
+        if (!ks.sorted) sort_with_stats(st, ks, &low);
         single = true;
         for (uint32_t w = 0; w < st.W; ++w) {
             uint64_t v = ks.vary[w];
@@ -571,11 +571,17 @@ void replicate(State& st, sparq_field_t f) {
 // ---------------------------------------------------------------- pair ops
 template <int W>
 __global__ void __launch_bounds__(kBlock) pair_count_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, uint32_t word,
-                                                       uint64_t bm, uint32_t* __restrict__ cnt) {
+                                                       uint64_t bm, const CondList cl, uint32_t* __restrict__ cnt) {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     uint64_t k[W];
     load_key<W>(keys, cap, i, k);
+    bool ok = true;
+    for (uint32_t c = 0; c < cl.n; ++c) ok = ok && cond_ok<W>(k, cl.c[c]);
+    if (!ok) {            // passes through unchanged
+        cnt[i] = 1;
+        return;
+    }
     const bool one = kword<W>(k, word) & bm;
     bool paired = false;
     uint64_t j = one ? i - 1 : i + 1;
@@ -592,13 +598,21 @@ __global__ void __launch_bounds__(kBlock) pair_count_kernel(const uint64_t* __re
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) pair_apply_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
-                                                       uint64_t n, uint32_t word, uint64_t bm, const uint32_t* __restrict__ cnt,
-                                                       const uint64_t* __restrict__ off, const sparq_pair_params_t p,
-                                                       uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out) {
+                                                       uint64_t n, uint32_t word, uint64_t bm, const CondList cl,
+                                                       const uint32_t* __restrict__ cnt, const uint64_t* __restrict__ off,
+                                                       const sparq_pair_params_t p, uint64_t* __restrict__ kout,
+                                                       double2* __restrict__ aout, uint64_t cap_out) {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     uint64_t k[W];
     load_key<W>(keys, cap, i, k);
+    bool ok = true;
+    for (uint32_t c = 0; c < cl.n; ++c) ok = ok && cond_ok<W>(k, cl.c[c]);
+    if (!ok) {            // passes through unchanged
+        store_key<W>(kout, cap_out, off[i], k);
+        aout[off[i]] = amps[i];
+        return;
+    }
     const bool one = kword<W>(k, word) & bm;
     const bool paired = cnt[i] == 1;
     if (paired && one) return;               // handled by the zero partner
@@ -667,7 +681,7 @@ __global__ void __launch_bounds__(kBlock) pair_apply_kernel(const uint64_t* __re
 }
 
 template <int W>
-static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p) {
+static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl) {
     const uint64_t n = st.n;
     sparq_field_t b{target.word, (uint8_t)(target.shift + bit), 1, 0};
     sort_state(st, &b);
@@ -676,7 +690,7 @@ static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq
     uint64_t* off = dalloc<uint64_t>(n + 1, st.stream);
     {
         SPQ_PROF(st.stream, "pair_count_kernel", n * (8.0 * W + 4));
-        pair_count_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, n, b.word, bm, cnt);
+        pair_count_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, n, b.word, bm, cl, cnt);
     }
     SPQ_HIP(hipGetLastError());
     exclusive_scan_u32(st, cnt, off, n, off + n);
@@ -686,7 +700,8 @@ static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq
     alloc_arrays(st, st.W, M, &kout, &aout);
     {
         SPQ_PROF(st.stream, "pair_apply_kernel", n * (8.0 * W + 28) + M * (8.0 * W + 16));
-        pair_apply_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, b.word, bm, cnt, off, p, kout, aout, M);
+        pair_apply_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, b.word, bm, cl, cnt, off, p, kout,
+                                                                         aout, M);
     }
     SPQ_HIP(hipGetLastError());
     install_arrays(st, st.W, kout, aout, M, M);
@@ -694,14 +709,14 @@ static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq
     dfree(off, st.stream);
 }
 
-void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, uint64_t /*n_active*/) {
+void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl) {
     SPQ_REQUIRE(target.word < st.W && bit < target.width, "bad target bit");
     SPQ_REQUIRE(p.kind <= SPARQ_PAIR_ROT_GENERAL, "unknown pair kind");
     if (p.kind == SPARQ_PAIR_CONDROT_RATIONAL || p.kind == SPARQ_PAIR_CONDROT_TABLE)
         SPQ_REQUIRE(p.in.word < st.W && p.in.width >= 1, "bad input register");
     if (p.kind == SPARQ_PAIR_CONDROT_TABLE) SPQ_REQUIRE(p.table != nullptr && p.in.width <= 24, "table-driven rotation needs a table");
     if (st.n == 0) return;
-    SPQ_DISPATCH_W(st.W, pair_impl, st, target, bit, p);
+    SPQ_DISPATCH_W(st.W, pair_impl, st, target, bit, p, cl);
 }
 
 }  // namespace sparq

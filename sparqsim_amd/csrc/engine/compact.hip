@@ -13,11 +13,6 @@ namespace sparq {
 constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;   // 2048 entries per workgroup, blocked per thread
 
-struct CondList {
-    uint32_t n;
-    sparq_cond_t c[SPARQ_MAX_PROGRAM_CONDS];
-};
-
 enum Mode { kClearZero = 0, kPartition = 1, kMerge = 2 };
 
 // Predicate of entry i for the given mode; for kMerge also returns the run sum.

@@ -110,17 +110,27 @@ void exclusive_scan_u32(State& st, const uint32_t* in, uint64_t* out, uint64_t n
 void exclusive_scan_u64(State& st, const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev);
 
 // ---- sort (sort.hip)
-// A digit of the LSD plan: bits [shift, shift+bits) of key word `word`.
-struct Digit { uint8_t word, shift, bits, pad; };
+// A digit of the LSD plan: up to three bit ranges of key word `word`,
+// concatenated with range 0 least significant; `bits` is their total width.
+struct Digit {
+    uint8_t word, nseg, bits, pad;
+    uint8_t shift[3], width[3];
+};
 // Stable LSD radix sort of the state by the given digit plan (least significant
-// digit first).  Reorders keys and amps.
-void sort_by_plan(State& st, const std::vector<Digit>& plan);
+// digit first, every digit at most radix_bits wide).  Reorders keys and amps.
+void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits = 8);
+// Digit width forced for every sort (8 or 11), or 0 for the autotuner.
+void set_radix_bits(int bits);
 // One pass over the keys: bits that vary across the state, and whether the
 // state is already ordered with the optional field `last` least significant.
 // With a SizerQuery the same pass also histograms popcount(value of `last`)
 // into kStatsHistBins bins and the sizer's feature kernel runs on the result
 // (and its network when active) before the single readback.
 constexpr int kStatsHistBins = 6;
+// Key-statistics record: [0, W) OR of the keys, [W, 2W) AND, [2W] order
+// violations of (others, last), [2W+1] violations of `last` alone being
+// non-decreasing, [2W+2, +kStatsHistBins) popcount histogram of `last`.
+constexpr int stats_words(int W) { return 2 * W + 2 + kStatsHistBins; }
 struct SizerQuery {
     uint64_t n;
     uint32_t words, nd, kind;          // kind: 0 Hadamard_Int, 1 Hadamard_Int_Full, 2 dense group unitary
@@ -129,6 +139,7 @@ struct SizerQuery {
 struct KeyStats {
     std::vector<uint64_t> vary;
     bool sorted = true;
+    bool last_sorted = true;           // the `last` field alone is non-decreasing
     bool have_features = false, have_prediction = false;
     float features[SPARQ_SIZER_FEATURES] = {};
     float prediction = 0.f;            // network output: log2(G) / 32
@@ -141,8 +152,11 @@ void launch_sizer_features(State& st, const uint64_t* stats, const SizerQuery& q
 float sizer_margin();
 void sizer_record(const float* features, float target);
 void sizer_count(int what);           // 0 exact path, 1 predicted, 2 overflow fallback
-// LSD plan over the varying bits with `last` moved to the least significant place.
+// LSD plan over the varying bits with `last` moved to the least significant
+// place; the digits of `last` are left out when it is already non-decreasing.
 std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t* last, int digit_bits);
+// Sort by (others, last) given the statistics (digit width by the autotuner).
+void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last);
 // Sort unless already ordered; returns whether a sort ran.
 bool sort_state(State& st, const sparq_field_t* last);
 
@@ -164,7 +178,14 @@ void hadamard_int(State& st, sparq_field_t target, uint32_t n_digits, bool full_
                   uint64_t n_active);
 void replicate(State& st, sparq_field_t f);
 void group_unitary(State& st, sparq_field_t target, const double* mat, double eps);
-void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, uint64_t n_active);
+// Conditions as a kernel argument (AND of all).
+struct CondList {
+    uint32_t n;
+    sparq_cond_t c[SPARQ_MAX_PROGRAM_CONDS];
+};
+// With conditions, entries that fail them pass through unchanged; the
+// conditions must not read the target bit (the caller checks).
+void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl);
 
 // ---- pointwise (pointwise.hip)
 void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_cond_t* conds, uint32_t nconds);

@@ -6,6 +6,9 @@
 // moves W key words and the 16-byte amplitude.  Only bits that vary across
 // the state are sorted (an OR/AND reduction finds them first).
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
 
 #include "blockscan.hpp"
 #include "engine.hpp"
@@ -15,18 +18,18 @@ namespace sparq {
 
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kBlock * kSortItems;   // 4096 entries per workgroup
-constexpr int kRadix = 256;                      // digits of up to 8 bits
-constexpr uint32_t kInvalidDigit = 0x100;
 
-// Lanes of the wave holding the same 9-bit value as the calling lane.
+// Lanes of the wave holding the same value as the calling lane, for values of
+// `nbits` digit bits plus the invalid marker at bit RB.
+template <int RB>
 __device__ __forceinline__ uint64_t match_any(uint32_t d, int nbits) {
     uint64_t m = ~0ull;
     for (int b = 0; b < nbits; ++b) {
         uint64_t bal = __ballot((d >> b) & 1);
         m &= ((d >> b) & 1) ? bal : ~bal;
     }
-    uint64_t bal = __ballot((d >> 8) & 1);
-    m &= ((d >> 8) & 1) ? bal : ~bal;
+    uint64_t bal = __ballot((d >> RB) & 1);
+    m &= ((d >> RB) & 1) ? bal : ~bal;
     return m;
 }
 
@@ -51,20 +54,20 @@ __device__ __forceinline__ bool out_of_order(const uint64_t (&p)[W], const uint6
     return false;
 }
 
-// Record of R = 2W + 1 + kStatsHistBins words: [W] OR, [W] AND, [1] unsorted
-// flag, [kStatsHistBins] popcount histogram of `hist` (zero without one).
+// Record layout: stats_words(W) (engine.hpp); the histogram of `hist` is zero
+// without one.
 template <int W>
 __device__ __forceinline__ uint64_t stats_combine(int j, uint64_t x, uint64_t y) {
     if (j < W) return x | y;
     if (j < 2 * W) return x & y;
-    if (j == 2 * W) return x | y;
+    if (j <= 2 * W + 1) return x | y;
     return x + y;
 }
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) key_stats_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, StatsOrder so,
                                                       sparq_field_t hist, uint64_t* __restrict__ partial) {
-    constexpr int R = 2 * W + 1 + kStatsHistBins;
+    constexpr int R = stats_words(W);
     constexpr int kI = W <= 2 ? 8 : (W <= 4 ? 4 : 2);   // entries per thread per tile, loads issued together
     __shared__ uint64_t lds[kBlock / 64][R];
     uint64_t r[R];
@@ -90,10 +93,11 @@ __global__ void __launch_bounds__(kBlock) key_stats_kernel(const uint64_t* __res
 #pragma unroll
             for (int w = 0; w < W; ++w) { r[w] |= k[j][w]; r[W + w] &= k[j][w]; }
             r[2 * W] |= out_of_order<W>(p[j], k[j], so) ? 1 : 0;
+            if (so.last.width) r[2 * W + 1] |= (fget<W>(p[j], so.last) > fget<W>(k[j], so.last)) ? 1 : 0;
             if (hist.width && i < n) {
                 const int b = min(__popcll(fget<W>(k[j], hist)), kStatsHistBins - 1);
 #pragma unroll
-                for (int h = 0; h < kStatsHistBins; ++h) r[2 * W + 1 + h] += (h == b);
+                for (int h = 0; h < kStatsHistBins; ++h) r[2 * W + 2 + h] += (h == b);
             }
         }
     }
@@ -118,7 +122,7 @@ __global__ void __launch_bounds__(kBlock) key_stats_kernel(const uint64_t* __res
 template <int W>
 __global__ void __launch_bounds__(kBlock) key_stats_final_kernel(const uint64_t* __restrict__ partial, uint32_t blocks,
                                                             uint64_t* __restrict__ out) {
-    constexpr int R = 2 * W + 1 + kStatsHistBins;
+    constexpr int R = stats_words(W);
     __shared__ uint64_t lds[kBlock / 64][R];
     uint64_t r[R];
 #pragma unroll
@@ -154,7 +158,7 @@ static void launch_key_stats(State& st, const StatsOrder& so, sparq_field_t hist
     }
     SPQ_HIP(hipGetLastError());
     {
-        SPQ_PROF(st.stream, "key_stats_final_kernel", (double)g * (2 * W + 1 + kStatsHistBins) * 8.0);
+        SPQ_PROF(st.stream, "key_stats_final_kernel", (double)g * stats_words(W) * 8.0);
         key_stats_final_kernel<W><<<1, kBlock, 0, st.stream>>>(partial, g, out);
     }
     SPQ_HIP(hipGetLastError());
@@ -175,7 +179,7 @@ KeyStats key_stats(State& st, const sparq_field_t* last, const SizerQuery* q) {
     const bool sized = q && last && sizer_wanted(predict);
     sparq_field_t hist{0, 0, 0, 0};
     if (sized) hist = *last;
-    const size_t R = 2 * st.W + 1 + kStatsHistBins;
+    const size_t R = stats_words((int)st.W);
     const size_t fwords = (SPARQ_SIZER_FEATURES + 2) / 2;   // features + prediction, as floats
     uint64_t* partial = dalloc<uint64_t>((size_t)kStatsMaxGrid * R, st.stream);
     uint64_t* out = dalloc<uint64_t>(R + fwords, st.stream);
@@ -187,6 +191,7 @@ KeyStats key_stats(State& st, const sparq_field_t* last, const SizerQuery* q) {
     dfree(out, st.stream);
     for (uint32_t w = 0; w < st.W; ++w) ks.vary[w] = h[w] ^ h[st.W + w];
     ks.sorted = h[2 * st.W] == 0;
+    ks.last_sorted = last != nullptr && h[2 * st.W + 1] == 0;
     if (sized) {
         const float* f = reinterpret_cast<const float*>(h.data() + R);
         std::copy(f, f + SPARQ_SIZER_FEATURES, ks.features);
@@ -199,78 +204,165 @@ KeyStats key_stats(State& st, const sparq_field_t* last, const SizerQuery* q) {
 
 std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t* last, int digit_bits) {
     const std::vector<uint64_t>& vary = ks.vary;
-    // Segments of eligible bits in LSD order: (word, lo, hi).
-    struct Seg { int word, lo, hi; };
-    std::vector<Seg> segs;
-    if (last) segs.push_back({last->word, last->shift, last->shift + last->width});
-    for (int w = (int)W - 1; w >= 0; --w) {
-        if (last && last->word == w) {
-            if (last->shift > 0) segs.push_back({w, 0, last->shift});
-            if (last->shift + last->width < 64) segs.push_back({w, last->shift + last->width, 64});
-        } else {
-            segs.push_back({w, 0, 64});
+    // Varying bit positions in LSD order of the key (others, last): the bits of
+    // `last` (unless that field is already non-decreasing), then every other
+    // bit from the least significant word's lowest bit upward.
+    std::vector<std::pair<int, int>> bits;   // (word, position)
+    const bool skip_last = last && ks.last_sorted;
+    if (last && !skip_last)
+        for (int p = last->shift; p < last->shift + last->width; ++p)
+            if ((vary[last->word] >> p) & 1) bits.emplace_back(last->word, p);
+    for (int w = (int)W - 1; w >= 0; --w)
+        for (int p = 0; p < 64; ++p) {
+            if (last && last->word == w && p >= last->shift && p < last->shift + last->width) continue;
+            if ((vary[w] >> p) & 1) bits.emplace_back(w, p);
         }
-    }
+    // Consecutive positions of one word share a digit of up to digit_bits bits
+    // made of at most three contiguous ranges.
     std::vector<Digit> plan;
-    for (const Seg& s : segs) {
-        int p = s.lo;
-        while (p < s.hi) {
-            if (!((vary[s.word] >> p) & 1)) { ++p; continue; }
-            int b = std::min(digit_bits, s.hi - p);
-            plan.push_back(Digit{(uint8_t)s.word, (uint8_t)p, (uint8_t)b, 0});
-            p += b;
+    for (size_t i = 0; i < bits.size();) {
+        Digit d{};
+        d.word = (uint8_t)bits[i].first;
+        while (i < bits.size() && bits[i].first == d.word && d.bits < digit_bits) {
+            const int p = bits[i].second;
+            const bool extends = d.nseg > 0 && p == d.shift[d.nseg - 1] + d.width[d.nseg - 1];
+            if (!extends) {
+                if (d.nseg == 3) break;
+                d.shift[d.nseg] = (uint8_t)p;
+                d.width[d.nseg] = 0;
+                ++d.nseg;
+            }
+            ++d.width[d.nseg - 1];
+            ++d.bits;
+            ++i;
         }
+        plan.push_back(d);
     }
     return plan;
+}
+
+// ---------------------------------------------------------------- digit-width autotuner
+// The plan is built for 8- and 11-bit digits; when the wider digits save
+// passes, the first sorts of each size class (log2 N) run once with each width
+// under HIP events, and later sorts of that class pick the width with the
+// smaller passes x measured milliseconds per pass.
+namespace {
+std::mutex g_tune_mu;
+float g_ms_per_pass[2][64];        // [8-bit, 11-bit][log2 N]; < 0: not measured
+bool g_tune_init = false;
+int g_forced_bits = 0;
+hipEvent_t g_tune_ev[2] = {nullptr, nullptr};
+}  // namespace
+
+void set_radix_bits(int bits) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_forced_bits = bits;
+}
+
+void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
+    const std::vector<Digit> p8 = make_plan(ks, st.W, last, 8);
+    if (p8.empty()) return;
+    const std::vector<Digit> p11 = make_plan(ks, st.W, last, 11);
+    int lg = 0;
+    while ((uint64_t(1) << (lg + 1)) <= st.n && lg < 63) ++lg;
+    int use = 0, measure = -1;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        if (!g_tune_init) {
+            for (auto& row : g_ms_per_pass)
+                for (float& v : row) v = -1.f;
+            g_tune_init = true;
+        }
+        if (g_forced_bits == 8 || g_forced_bits == 11) {
+            use = g_forced_bits == 11 ? 1 : 0;
+        } else if (p11.size() < p8.size() && st.n >= (1u << 16)) {
+            if (g_ms_per_pass[0][lg] < 0) measure = use = 0;
+            else if (g_ms_per_pass[1][lg] < 0) measure = use = 1;
+            else use = (p11.size() * g_ms_per_pass[1][lg] < p8.size() * g_ms_per_pass[0][lg]) ? 1 : 0;
+        }
+    }
+    const std::vector<Digit>& plan = use ? p11 : p8;
+    if (measure < 0) {
+        sort_by_plan(st, plan, use ? 11 : 8);
+        return;
+    }
+    if (!g_tune_ev[0]) {
+        SPQ_HIP(hipEventCreate(&g_tune_ev[0]));
+        SPQ_HIP(hipEventCreate(&g_tune_ev[1]));
+    }
+    SPQ_HIP(hipEventRecord(g_tune_ev[0], st.stream));
+    sort_by_plan(st, plan, use ? 11 : 8);
+    SPQ_HIP(hipEventRecord(g_tune_ev[1], st.stream));
+    SPQ_HIP(hipEventSynchronize(g_tune_ev[1]));
+    float ms = 0.f;
+    SPQ_HIP(hipEventElapsedTime(&ms, g_tune_ev[0], g_tune_ev[1]));
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_ms_per_pass[measure][lg] = ms / (float)plan.size();
 }
 
 bool sort_state(State& st, const sparq_field_t* last) {
     if (st.n <= 1) return false;
     KeyStats ks = key_stats(st, last);
     if (ks.sorted) return false;
-    sort_by_plan(st, make_plan(ks, st.W, last, 8));
+    sort_with_stats(st, ks, last);
     return true;
 }
 
 // ---------------------------------------------------------------- passes
-__device__ __forceinline__ uint32_t digit_of(uint64_t k, Digit dg) {
-    return (uint32_t)((k >> dg.shift) & ((1ull << dg.bits) - 1));
+__device__ __forceinline__ uint32_t digit_of(uint64_t k, const Digit& dg) {
+    uint32_t v = 0, at = 0;
+    for (int s = 0; s < dg.nseg; ++s) {
+        v |= (uint32_t)((k >> dg.shift[s]) & ((1ull << dg.width[s]) - 1)) << at;
+        at += dg.width[s];
+    }
+    return v;
 }
 
-// Block histogram: block_hist[d * nb + b].
+// Block histogram: block_hist[d * nb + b] for digits of up to RB bits.
+template <int RB>
 __global__ void __launch_bounds__(kBlock) radix_hist_kernel(const uint64_t* __restrict__ kw, uint64_t n, Digit dg,
                                                        uint32_t* __restrict__ block_hist, uint32_t nb) {
-    __shared__ uint32_t hist[kRadix];
-    hist[threadIdx.x] = 0;
+    constexpr int kR = 1 << RB;
+    constexpr uint32_t kInvalid = 1u << RB;
+    __shared__ uint32_t hist[kR];
+    for (int d = threadIdx.x; d < kR; d += kBlock) hist[d] = 0;
     __syncthreads();
     const uint64_t tile = (uint64_t)blockIdx.x * kSortTile;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int it = 0; it < kSortItems; ++it) {
         uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
-        uint32_t d = e < n ? digit_of(kw[e], dg) : kInvalidDigit;
-        uint64_t peers = match_any(d, dg.bits);
+        uint32_t d = e < n ? digit_of(kw[e], dg) : kInvalid;
+        uint64_t peers = match_any<RB>(d, dg.bits);
         int leader = __ffsll((unsigned long long)peers) - 1;
-        if (lane == leader && d != kInvalidDigit) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
+        if (lane == leader && d != kInvalid) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    block_hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = hist[threadIdx.x];
+    for (int d = threadIdx.x; d < kR; d += kBlock) block_hist[(uint64_t)d * nb + blockIdx.x] = hist[d];
 }
 
 // Stable scatter of one digit pass.  idx_in == nullptr means identity order.
+// Each thread owns kR / kBlock consecutive digits for the digit scan.
+template <int RB>
 __global__ void __launch_bounds__(kBlock) radix_scatter_kernel(const uint64_t* __restrict__ kw_in, const uint32_t* __restrict__ idx_in,
                                                           uint64_t n, Digit dg, const uint64_t* __restrict__ offs, uint32_t nb,
                                                           uint64_t* __restrict__ kw_out, uint32_t* __restrict__ idx_out,
                                                           int write_keys) {
-    __shared__ uint32_t wcnt[4][kRadix];       // per-wave running digit counts
-    __shared__ uint32_t dstart[kRadix];        // first tile position of each digit
-    __shared__ uint64_t goff[kRadix];          // global position of tile position 0 of each digit
+    constexpr int kR = 1 << RB;
+    constexpr uint32_t kInvalid = 1u << RB;
+    constexpr int kPerThread = kR / kBlock;
+    static_assert(kR % kBlock == 0, "radix must be a multiple of the block size");
+    __shared__ uint32_t wcnt[4][kR];           // per-wave running digit counts
+    __shared__ uint32_t dstart[kR];            // first tile position of each digit
+    __shared__ uint64_t goff[kR];              // global position of tile position 0 of each digit
     __shared__ uint64_t skey[kSortTile];
     __shared__ uint32_t sidx[kSortTile];
     __shared__ uint32_t scan_lds[kBlock / 64];
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int d = threadIdx.x; d < kR; d += kBlock) {
 #pragma unroll
-    for (int w = 0; w < 4; ++w) wcnt[w][threadIdx.x] = 0;
+        for (int w = 0; w < 4; ++w) wcnt[w][d] = 0;
+    }
     __syncthreads();
 
     const uint64_t tile = (uint64_t)blockIdx.x * kSortTile;
@@ -284,30 +376,44 @@ __global__ void __launch_bounds__(kBlock) radix_scatter_kernel(const uint64_t* _
         bool valid = e < n;
         key[it] = valid ? kw_in[e] : 0;
         idx[it] = valid ? (idx_in ? idx_in[e] : (uint32_t)e) : 0;
-        uint32_t d = valid ? digit_of(key[it], dg) : kInvalidDigit;
+        uint32_t d = valid ? digit_of(key[it], dg) : kInvalid;
         dig[it] = d;
-        uint64_t peers = match_any(d, dg.bits);
+        uint64_t peers = match_any<RB>(d, dg.bits);
         uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
-        uint32_t base = valid ? wcnt[wave][d & (kRadix - 1)] : 0;
+        uint32_t base = valid ? wcnt[wave][d & (kR - 1)] : 0;
         rank[it] = base + before;
-        if (valid && before == 0) wcnt[wave][d & (kRadix - 1)] = base + (uint32_t)__popcll(peers);
+        if (valid && before == 0) wcnt[wave][d & (kR - 1)] = base + (uint32_t)__popcll(peers);
     }
     __syncthreads();
     // Digit totals and their exclusive scan over digits; per-wave offsets.
-    uint32_t d = threadIdx.x;
-    uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
+    uint32_t c[kPerThread][4];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int d = threadIdx.x * kPerThread + q;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            c[q][w] = wcnt[w][d];
+            sum += c[q][w];
+        }
+    }
     uint32_t tot;
-    uint32_t start = block_excl_scan<uint32_t>(c0 + c1 + c2 + c3, scan_lds, tot);
-    dstart[d] = start;
-    wcnt[0][d] = start;
-    wcnt[1][d] = start + c0;
-    wcnt[2][d] = start + c0 + c1;
-    wcnt[3][d] = start + c0 + c1 + c2;
-    goff[d] = offs[(uint64_t)d * nb + blockIdx.x];
+    uint32_t start = block_excl_scan<uint32_t>(sum, scan_lds, tot);
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int d = threadIdx.x * kPerThread + q;
+        dstart[d] = start;
+        wcnt[0][d] = start;
+        wcnt[1][d] = start + c[q][0];
+        wcnt[2][d] = start + c[q][0] + c[q][1];
+        wcnt[3][d] = start + c[q][0] + c[q][1] + c[q][2];
+        goff[d] = offs[(uint64_t)d * nb + blockIdx.x];
+        start += c[q][0] + c[q][1] + c[q][2] + c[q][3];
+    }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kSortItems; ++it) {
-        if (dig[it] != kInvalidDigit) {
+        if (dig[it] != kInvalid) {
             uint32_t pos = wcnt[wave][dig[it]] + rank[it];
             skey[pos] = key[it];
             sidx[pos] = idx[it];
@@ -351,17 +457,30 @@ static void launch_gather_state(State& st, const uint32_t* idx, uint64_t* kout, 
     SPQ_HIP(hipGetLastError());
 }
 
-void sort_by_plan(State& st, const std::vector<Digit>& plan) {
+void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits) {
     const uint64_t n = st.n;
     if (plan.empty() || n <= 1) return;
+    SPQ_REQUIRE(radix_bits == 8 || radix_bits == 11, "digit width must be 8 or 11");
+    static const bool trace = std::getenv("SPARQ_TRACE_SORT") != nullptr;   // diagnostic: one line per sort
+    if (trace) {
+        std::fprintf(stderr, "[sort] n=%llu radix=%d passes=%zu:", (unsigned long long)n, radix_bits, plan.size());
+        for (const Digit& d : plan) {
+            std::fprintf(stderr, " w%u[", d.word);
+            for (int s = 0; s < d.nseg; ++s) std::fprintf(stderr, "%s%u+%u", s ? "," : "", d.shift[s], d.width[s]);
+            std::fprintf(stderr, "]");
+        }
+        std::fprintf(stderr, "\n");
+    }
+    for (const Digit& d : plan) SPQ_REQUIRE(d.bits >= 1 && d.bits <= radix_bits && d.nseg >= 1 && d.nseg <= 3, "bad digit");
     SPQ_REQUIRE(n < (1ull << 32), "sort supports fewer than 2^32 entries");
     const uint32_t nb = grid_for(n, kSortTile);
+    const uint64_t radix = uint64_t(1) << radix_bits;
     uint64_t* kwA = dalloc<uint64_t>(n, st.stream);
     uint64_t* kwB = dalloc<uint64_t>(n, st.stream);
     uint32_t* idxA = dalloc<uint32_t>(n, st.stream);
     uint32_t* idxB = dalloc<uint32_t>(n, st.stream);
-    uint32_t* bh = dalloc<uint32_t>((size_t)kRadix * nb, st.stream);
-    uint64_t* offs = dalloc<uint64_t>((size_t)kRadix * nb, st.stream);
+    uint32_t* bh = dalloc<uint32_t>((size_t)radix * nb, st.stream);
+    uint64_t* offs = dalloc<uint64_t>((size_t)radix * nb, st.stream);
 
     bool identity = true;
     int cur_word = -1;
@@ -383,16 +502,21 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan) {
         }
         {
             SPQ_PROF(st.stream, "radix_hist_kernel", n * 8.0);
-            radix_hist_kernel<<<nb, kBlock, 0, st.stream>>>(src, n, dg, bh, nb);
+            if (radix_bits == 11) radix_hist_kernel<11><<<nb, kBlock, 0, st.stream>>>(src, n, dg, bh, nb);
+            else radix_hist_kernel<8><<<nb, kBlock, 0, st.stream>>>(src, n, dg, bh, nb);
         }
         SPQ_HIP(hipGetLastError());
-        exclusive_scan_u32(st, bh, offs, (uint64_t)kRadix * nb, nullptr);
+        exclusive_scan_u32(st, bh, offs, radix * nb, nullptr);
         // The key word of the last pass of a word is not needed afterwards.
         bool need_keys = (p + 1 < plan.size()) && plan[p + 1].word == dg.word;
         {
             SPQ_PROF(st.stream, "radix_scatter_kernel", n * (8.0 + (identity ? 0.0 : 4.0) + 4.0 + (need_keys ? 8.0 : 0.0)));
-            radix_scatter_kernel<<<nb, kBlock, 0, st.stream>>>(src, identity ? nullptr : idxA, n, dg, offs, nb, kwB, idxB,
-                                                            need_keys ? 1 : 0);
+            if (radix_bits == 11)
+                radix_scatter_kernel<11><<<nb, kBlock, 0, st.stream>>>(src, identity ? nullptr : idxA, n, dg, offs, nb, kwB, idxB,
+                                                                    need_keys ? 1 : 0);
+            else
+                radix_scatter_kernel<8><<<nb, kBlock, 0, st.stream>>>(src, identity ? nullptr : idxA, n, dg, offs, nb, kwB, idxB,
+                                                                   need_keys ? 1 : 0);
         }
         SPQ_HIP(hipGetLastError());
         std::swap(kwA, kwB);
