@@ -174,6 +174,7 @@ def main():
     ap.add_argument("--qda-qubits", type=int, default=20)
     ap.add_argument("--qda-steps", type=int, default=40, help="walk steps timed in the QDA leg (0 skips it)")
     ap.add_argument("--qda-total-steps", type=int, default=40568, help="length of the schedule s = n/total")
+    ap.add_argument("--qda-solve-steps", type=int, default=40, help="complete solve on a schedule of this length (0 skips)")
     ap.add_argument("--grover-traj", type=int, default=1024, help="trajectories of the batched Grover leg (0 skips it)")
     ap.add_argument("--grover-bits", type=int, default=14)
     ap.add_argument("--grover-iterations", type=int, default=40)
@@ -243,6 +244,19 @@ def main():
     }
     if args.qda_steps > 0:
         result["qda_solve"] = qda_walk_leg(ps, args.qda_qubits, args.qda_steps, args.qda_total_steps)
+    if args.qda_solve_steps > 0:
+        from sparqsim_amd import qda_solve
+
+        t0 = time.perf_counter()
+        r = qda_solve.solve(ps, nqubit=args.qda_qubits, max_steps=args.qda_solve_steps, cz_eps=1e-16, log=lambda s: None)
+        result["qda_compressed_solve"] = {
+            "workload": f"complete 2^{args.qda_qubits} Poiseuille QDA solve on a {r['steps_run']}-step schedule "
+                        "(walk + ClearZero(1e-16) + Normalize per step, as PoiseuilleQDABench)",
+            "wall_s": time.perf_counter() - t0, "ms_per_step": r["ms_per_step"], "final_res_dir": r["final_res_dir"],
+            "final_fidelity": r["final_fidelity"], "final_entries": r["peak_logged_entries"],
+            "reference_published": "2^20, 40-step, czEps 1e-16: res_dir 5.178906e-3, fid 0.999999 "
+                                   "(docs/verification/v1d-qda-linsolver-a800.md:123)",
+        }
     if args.grover_traj > 0:
         result["noisy_grover"] = grover_leg(ps, args.grover_traj, args.grover_bits, args.grover_iterations, 1e-3,
                                             min(args.grover_plain_traj, args.grover_traj))

@@ -259,6 +259,17 @@ int sparq_prof_report(sparq_prof_record_t* out, uint32_t capacity, uint32_t* cou
 int sparq_memory_stats(uint64_t out[6]);
 int sparq_release_cached_memory(void);
 
+/* Post-selection into a dense vector, the device side of GetOutput
+ * (SparQ_Algorithm/src/DiscreteAdiabatic/qda_fundamental.cpp:123-169): every
+ * entry whose `zero` fields are all 0 is written to out[index], index = the
+ * `index` fields concatenated with field 0 least significant; out holds
+ * len = 2^(sum of index widths) complex values (2*len doubles, re/im
+ * interleaved), zero elsewhere; *weight = sum of |a|^2 of the selected entries
+ * (the caller renormalises, as GetOutput does).  Keys are unique, so no index is
+ * written twice. */
+int sparq_post_select_dense(sparq_state_t st, const sparq_field_t* zero, uint32_t nzero, const sparq_field_t* index,
+                            uint32_t nindex, double* out, uint64_t len, double* weight);
+
 /* Radix-sort digit width (no counterpart in the reference, which sorts with
  * comparison sorts): 8 or 11 forces that width for every later sort, 0 (the
  * default) lets the engine's autotuner pick it per size class by timing the

@@ -168,6 +168,7 @@ PYBIND11_MODULE(_oracle, m) {
         return fnop([e](State& st) { ClearZero(st, e); }, "ClearZero");
     }, py::arg("eps") = py::none());
     m.def("set_clearzero_default_eps", [](double e) { clearzero_default_eps() = e; });
+    m.def("get_clearzero_default_eps", []() { return clearzero_default_eps(); });
     m.def("Normalize", []() { return fnop([](State& st) { Normalize(st); }, "Normalize"); });
     m.def("SortUnconditional", []() { return fnop([](State& st) { SortUnconditional(st); }, "SortUnconditional"); });
     m.def("Init_Unsafe", [](py::object r, uint64_t v) {

@@ -314,6 +314,15 @@ int sparq_memory_stats(uint64_t out[6]) {
     });
 }
 
+int sparq_post_select_dense(sparq_state_t h, const sparq_field_t* zero, uint32_t nzero, const sparq_field_t* index, uint32_t nindex,
+                            double* out, uint64_t len, double* weight) {
+    return guarded([&] {
+        SPQ_REQUIRE(out != nullptr && (zero != nullptr || nzero == 0) && index != nullptr, "null argument");
+        const double w = post_select_dense(S(h), zero, nzero, index, nindex, out, len);
+        if (weight) *weight = w;
+    });
+}
+
 int sparq_set_radix_bits(int bits) {
     return guarded([&] {
         SPQ_REQUIRE(bits == 0 || bits == 8 || bits == 11, "digit width must be 0 (auto), 8 or 11");

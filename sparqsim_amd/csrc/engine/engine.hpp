@@ -162,6 +162,9 @@ bool sort_state(State& st, const sparq_field_t* last);
 
 // ---- compaction (compact.hip)
 void clear_zero(State& st, double eps);
+// Dense post-selection into out_host[2*len] (state.hip); returns the selected weight.
+double post_select_dense(State& st, const sparq_field_t* zero, uint32_t nzero, const sparq_field_t* index, uint32_t nindex,
+                         double* out_host, uint64_t len);
 // ClearZero of (keys, amps, cap) holding counts[2] entries (a device word); reads
 // counts[0..4) back into info with the kept count in info[3].
 void clear_zero_sized(State& st, uint64_t* keys, double2* amps, uint64_t cap, uint64_t* counts, double eps, uint64_t info[4],

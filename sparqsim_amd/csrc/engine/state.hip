@@ -340,6 +340,90 @@ double sum_abs2(State& st) {
     return s;
 }
 
+// ------------------------------------------------------------------ dense post-selection
+// Entries whose `zero` fields are all 0 are written to out[index], index =
+// concatenation of the `index` fields (field 0 least significant); per-block
+// partial sums of |a|^2 of the selected entries.
+struct FieldList {
+    uint32_t n;
+    sparq_field_t f[8];
+};
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) post_select_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                                        uint64_t n, const FieldList zero, const FieldList index, double2* __restrict__ out,
+                                                        uint64_t len, double* __restrict__ partial) {
+    __shared__ double lds[kBlock / 64];
+    double s = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        uint64_t k[W];
+        load_key<W>(keys, cap, i, k);
+        bool sel = true;
+        for (uint32_t z = 0; z < zero.n; ++z) sel = sel && fget<W>(k, zero.f[z]) == 0;
+        if (!sel) continue;
+        uint64_t idx = 0;
+        uint32_t at = 0;
+        for (uint32_t f = 0; f < index.n; ++f) {
+            idx |= fget<W>(k, index.f[f]) << at;
+            at += index.f[f].width;
+        }
+        const double2 a = amps[i];
+        if (idx < len) out[idx] = a;
+        s += cabs2(a);
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+template <int W>
+static void launch_post_select(State& st, const FieldList& zero, const FieldList& index, double2* out, uint64_t len, double* partial,
+                               unsigned g) {
+    {
+        SPQ_PROF(st.stream, "post_select_kernel", st.n * (8.0 * W + 16));
+        post_select_kernel<W><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, zero, index, out, len, partial);
+    }
+    SPQ_HIP(hipGetLastError());
+}
+
+double post_select_dense(State& st, const sparq_field_t* zero, uint32_t nzero, const sparq_field_t* index, uint32_t nindex,
+                         double* out_host, uint64_t len) {
+    SPQ_REQUIRE(nzero <= 8 && nindex >= 1 && nindex <= 8, "at most 8 zero fields and 1..8 index fields");
+    FieldList zl{}, il{};
+    uint32_t bits = 0;
+    zl.n = nzero;
+    il.n = nindex;
+    for (uint32_t i = 0; i < nzero; ++i) {
+        SPQ_REQUIRE(zero[i].word < st.W, "field out of range");
+        zl.f[i] = zero[i];
+    }
+    for (uint32_t i = 0; i < nindex; ++i) {
+        SPQ_REQUIRE(index[i].word < st.W, "field out of range");
+        il.f[i] = index[i];
+        bits += index[i].width;
+    }
+    SPQ_REQUIRE(bits <= 40 && len == (uint64_t(1) << bits), "output length must be 2^(total index bits) (at most 2^40)");
+    double2* out = dalloc<double2>(len, st.stream);
+    SPQ_HIP(hipMemsetAsync(out, 0, len * sizeof(double2), st.stream));
+    const unsigned g = st.n ? std::min<unsigned>(grid_for(st.n, kBlock), 1024) : 1;
+    double* partial = dalloc<double>(g, st.stream);
+    SPQ_HIP(hipMemsetAsync(partial, 0, g * sizeof(double), st.stream));
+    if (st.n) SPQ_DISPATCH_W(st.W, launch_post_select, st, zl, il, out, len, partial, g);
+    std::vector<double> h(g);
+    SPQ_HIP(hipMemcpyAsync(out_host, out, len * sizeof(double2), hipMemcpyDeviceToHost, st.stream));
+    read_back(st, partial, g * sizeof(double), h.data());
+    dfree(out, st.stream);
+    dfree(partial, st.stream);
+    double s = 0;
+    for (double v : h) s += v;
+    return s;
+}
+
 __global__ void __launch_bounds__(kBlock) scale_kernel(double2* __restrict__ amps, uint64_t n, double s) {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) amps[i] = cscale(amps[i], s);

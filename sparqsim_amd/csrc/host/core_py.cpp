@@ -143,6 +143,20 @@ PYBIND11_MODULE(_core, m) {
             py::gil_scoped_release r;
             s.import_entries(v, a);
         })
+        .def("post_select_dense", [](SparseState& s, py::object zero, py::object index) {
+            std::vector<size_t> z, ix;
+            for (auto it : zero) z.push_back(resolve(it));
+            for (auto it : index) ix.push_back(resolve(it));
+            std::vector<complex_t> out;
+            double w;
+            {
+                py::gil_scoped_release r;
+                w = s.post_select_dense(z, ix, out);
+            }
+            py::array_t<std::complex<double>> a(out.size());
+            if (!out.empty()) std::memcpy(a.mutable_data(), out.data(), out.size() * 16);
+            return py::make_tuple(a, w);
+        }, py::arg("zero_regs"), py::arg("index_regs"))
         .def("to_string", [](SparseState& s, int precision) { return s.to_string(precision); }, py::arg("precision") = 6)
         .def("__str__", [](SparseState& s) { return s.to_string(6); })
         .def("__repr__", [](SparseState& s) { return "<SparseState entries=" + std::to_string(s.size()) + ">"; });

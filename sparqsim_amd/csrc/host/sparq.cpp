@@ -341,6 +341,26 @@ void SparseState::from_host(const std::vector<System>& entries) {
     import_entries(vals, amps);
 }
 
+double SparseState::post_select_dense(const std::vector<size_t>& zero_ids, const std::vector<size_t>& index_ids,
+                                      std::vector<complex_t>& out) {
+    const Layout& L = sync();
+    flush();
+    std::vector<sparq_field_t> z, ix;
+    size_t bits = 0;
+    for (size_t id : zero_ids) z.push_back(L.at(id));
+    for (size_t id : index_ids) {
+        ix.push_back(L.at(id));
+        bits += ix.back().width;
+    }
+    if (bits > 40) throw std::invalid_argument("post-selected index wider than 40 bits");
+    const uint64_t len = uint64_t(1) << bits;
+    out.assign(len, complex_t(0, 0));
+    double w = 0;
+    check(sparq_post_select_dense(h_, z.data(), (uint32_t)z.size(), ix.data(), (uint32_t)ix.size(),
+                                  reinterpret_cast<double*>(out.data()), len, &w));
+    return w;
+}
+
 // ------------------------------------------------------------ host entries
 bool System::operator<(const System& rhs) const {
     for (size_t id = 0; id < name_register_map.size(); ++id) {

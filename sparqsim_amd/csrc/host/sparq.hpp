@@ -165,6 +165,11 @@ public:
     // Host view as the reference's std::vector<System> (download / upload, entry order kept).
     void to_host(std::vector<System>& entries);
     void from_host(const std::vector<System>& entries);
+    // Dense post-selection on the device (GetOutput without the renormalisation):
+    // out[index] for entries whose zero_ids registers are 0, index = the index_ids
+    // registers concatenated (first least significant); returns the selected weight.
+    double post_select_dense(const std::vector<size_t>& zero_ids, const std::vector<size_t>& index_ids,
+                             std::vector<complex_t>& out);
     std::string to_string(int precision = 6);
 
 private:

@@ -187,18 +187,22 @@ def mid_eigenstate(a: np.ndarray, b: np.ndarray, fs: float) -> np.ndarray:
 def get_output(state, mod, main_reg="main_reg", anc_UA="anc_UA", anc_1="anc_1", anc_2="anc_2", anc_3="anc_3", anc_4="anc_4"):
     """Post-selected output vector (qda_fundamental.cpp:123-169): entries with
     anc_UA = anc_3 = anc_2 = 0, indexed by main | anc_1 << n | anc_4 << (n+1),
-    renormalised.  Returns (vector, probability)."""
-    ids, vals, amps = state.export()
-    col = {i: k for k, i in enumerate(ids)}
+    renormalised.  Returns (vector, probability).  On the engine the selection
+    runs on the device (`post_select_dense`) and only the 4n vector is copied."""
     g = mod.System.get_id
     n = mod.System.size_of(main_reg)
-    sel = (vals[:, col[g(anc_UA)]] == 0) & (vals[:, col[g(anc_3)]] == 0) & (vals[:, col[g(anc_2)]] == 0)
-    idx = (vals[sel, col[g(main_reg)]].astype(np.int64)
-           | (vals[sel, col[g(anc_1)]].astype(np.int64) << n)
-           | (vals[sel, col[g(anc_4)]].astype(np.int64) << (n + 1)))
-    vec = np.zeros(1 << (n + 2), dtype=complex)
-    vec[idx] = amps[sel]
-    prob = float(np.sum(np.abs(amps[sel]) ** 2))
+    if hasattr(state, "post_select_dense"):
+        vec, prob = state.post_select_dense([anc_UA, anc_3, anc_2], [main_reg, anc_1, anc_4])
+    else:
+        ids, vals, amps = state.export()
+        col = {i: k for k, i in enumerate(ids)}
+        sel = (vals[:, col[g(anc_UA)]] == 0) & (vals[:, col[g(anc_3)]] == 0) & (vals[:, col[g(anc_2)]] == 0)
+        idx = (vals[sel, col[g(main_reg)]].astype(np.int64)
+               | (vals[sel, col[g(anc_1)]].astype(np.int64) << n)
+               | (vals[sel, col[g(anc_4)]].astype(np.int64) << (n + 1)))
+        vec = np.zeros(1 << (n + 2), dtype=complex)
+        vec[idx] = amps[sel]
+        prob = float(np.sum(np.abs(amps[sel]) ** 2))
     if abs(prob - 1.0) >= 1e-14:
         vec = vec * (1.0 / math.sqrt(prob) if prob != 0 else 1.0)
     return vec, prob
@@ -213,16 +217,64 @@ def fidelity(ideal: np.ndarray, out: np.ndarray) -> float:
 QDA_STEP_CONSTANT = 2305    # test/CPUTest/CommonTest/CorrectnessTest_QDA_CompareList.inl:132
 
 
-def setup_poiseuille(mod, nqubit: int):
-    """Registers and |b> = H|0> on main_reg (CorrectnessTest_QDA_CompareList.inl:118-131)."""
+def setup_poiseuille(mod, nqubit: int, ancilla_order=("anc_1", "anc_2", "anc_3", "anc_4")):
+    """Registers and |b> = H|0> on main_reg (CorrectnessTest_QDA_CompareList.inl:118-131;
+    the production bench, Experiments/QDA/QDA_Poiseuille/PoiseuilleQDABench.cu:149-160,
+    adds the Boolean ancillas in the order anc_4, anc_3, anc_2, anc_1)."""
     mod.System.clear()
     st = mod.SparseState()
     mod.AddRegister("main_reg", mod.UnsignedInteger, nqubit)(st)
     mod.AddRegister("anc_UA", mod.UnsignedInteger, 4)(st)
-    for name in ("anc_1", "anc_2", "anc_3", "anc_4"):
+    for name in ancilla_order:
         mod.AddRegister(name, mod.Boolean, 1)(st)
     mod.Hadamard_Int_Full("main_reg")(st)
     return st
+
+
+def analytic_kappa(alpha: float, beta: float, n: int) -> float:
+    """max/min |alpha + 2 beta cos(pi k/(n+1))| over k = 1..n (PoiseuilleQDABench.cu:57-65)."""
+    k = np.arange(1, n + 1, dtype=np.float64)
+    lam = np.abs(alpha + 2.0 * beta * np.cos(np.pi * k / (n + 1)))
+    return float(lam.max() / lam.min())
+
+
+def thomas_solution(alpha: float, beta: float, n: int) -> np.ndarray:
+    """Normalised solution of tridiag(beta, alpha, beta) x = ones/sqrt(n) by the Thomas
+    algorithm (PoiseuilleQDABench.cu:67-88)."""
+    from scipy.linalg import solve_banded
+
+    ab = np.zeros((3, n))
+    ab[0, 1:] = beta
+    ab[1, :] = alpha
+    ab[2, :-1] = beta
+    x = solve_banded((1, 1), ab, np.full(n, 1.0 / math.sqrt(n)))
+    return x / np.linalg.norm(x)
+
+
+def solution_metrics(out: np.ndarray, n: int, alpha: float, beta: float, xref: np.ndarray) -> dict:
+    """res_dir, res_opt, success probability and fidelity of the solution block out[n:2n)
+    of a GetOutput vector, as the production bench computes them
+    (PoiseuilleQDABench.cu:194-229): y = A x/|x|, res_dir = |y/|y| - b|, res_opt^2 =
+    |y|^2 - <y, b>^2, fid = |<xref, x/|x|>|^2."""
+    blk = out[n:2 * n]
+    succ = float(np.sum(np.abs(blk) ** 2))
+    nrm = math.sqrt(succ)
+    if nrm <= 1e-300:
+        return {"res_dir": float("nan"), "res_opt": float("nan"), "success_prob": succ, "fidelity": float("nan")}
+    x = blk / nrm
+    y = alpha * x
+    y[:-1] += beta * x[1:]
+    y[1:] += beta * x[:-1]
+    y_nrm2 = float(np.sum(np.abs(y) ** 2))
+    y_nrm = math.sqrt(y_nrm2)
+    binv = 1.0 / math.sqrt(n)
+    proj_re = binv * float(np.sum(y.real))
+    proj_im = binv * float(np.sum(y.imag))
+    res_dir = math.sqrt(max(2.0 - 2.0 * proj_re / y_nrm, 0.0))
+    res_opt = math.sqrt(max(y_nrm2 - (proj_re * proj_re + proj_im * proj_im), 0.0))
+    fr = float(np.dot(xref, x.real))
+    fi = float(np.dot(xref, x.imag))
+    return {"res_dir": res_dir, "res_opt": res_opt, "success_prob": succ, "fidelity": fr * fr + fi * fi}
 
 
 def qda_schedule(nqubit: int, step_rate: float, alpha: float, beta: float):

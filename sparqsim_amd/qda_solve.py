@@ -1,0 +1,108 @@
+"""Poiseuille QDA linear solve on the engine -- the counterpart of the reference's
+production benchmark (Experiments/QDA/QDA_Poiseuille/PoiseuilleQDABench.cu).
+
+    python -m sparqsim_amd.qda_solve --nqubit 20 --max-steps 40 --cz-eps 1e-16
+
+Solves tridiag(beta, alpha, beta) x = ones/sqrt(n) with the discrete-adiabatic
+walk: steps = ceil(steprate * 2305 * kappa) (kappa analytic unless given), capped
+by --max-steps and made even; each step is W(s = i/steps), ClearZero(cz_eps),
+Normalize; at every log point the post-selected output gives res_dir, res_opt,
+success probability and fidelity against the Thomas solution, exactly as the
+reference bench computes them.  Prints one line per log point and a JSON
+summary; --learned enables the learned output sizer.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import time
+
+from . import qda
+
+
+def solve(ps, nqubit=20, steprate=1.1, p=1.3, alpha=2.2666666666564934, beta=-1.0, kappa=0.0, log_every=250,
+          tol=1e-6, max_steps=10 ** 9, cz_eps=1e-14, ancilla_order=("anc_4", "anc_3", "anc_2", "anc_1"), log=print):
+    n = 1 << nqubit
+    kappa_analytic = qda.analytic_kappa(alpha, beta, n)
+    kappa = kappa if kappa > 0 else kappa_analytic
+    steps = min(int(math.ceil(steprate * qda.QDA_STEP_CONSTANT * kappa)), max_steps)
+    if steps % 2:
+        steps -= 1
+    steps = max(steps, 2)
+    if log_every % 2:
+        log_every += 1
+    xref = qda.thomas_solution(alpha, beta, n)
+    old_eps = ps.get_clearzero_default_eps()
+    ps.set_clearzero_default_eps(cz_eps)       # walk-internal ClearZero() calls follow the threshold
+    curve, peak, termination = [], 0, "fixed_step_budget"
+    t0 = time.perf_counter()
+    t_extract = 0.0
+    try:
+        st = qda.setup_poiseuille(ps, nqubit, ancilla_order)
+        clear = ps.ClearZero(cz_eps)
+        norm = ps.Normalize()
+        for i in range(steps):
+            qda.WalkSTridiagonal(ps, "main_reg", "anc_UA", "anc_1", "anc_2", "anc_3", "anc_4", i / steps, kappa, p,
+                                 alpha, beta)(st)
+            clear(st)
+            norm(st)
+            if (i + 1) % log_every == 0 or i == steps - 1:
+                size = st.size()
+                peak = max(peak, size)
+                te = time.perf_counter()
+                out, _ = qda.get_output(st, ps)
+                m = qda.solution_metrics(out, n, alpha, beta, xref)
+                t_extract += time.perf_counter() - te
+                m.update(step=i + 1, entries=size, wall_s=time.perf_counter() - t0)
+                curve.append(m)
+                log(f"step {i + 1:7d}/{steps} res_dir={m['res_dir']:.6e} res_opt={m['res_opt']:.6e} "
+                    f"succP={m['success_prob']:.4f} fid={m['fidelity']:.6f} entries={size} wall={m['wall_s']:.1f}s")
+                if not math.isnan(m["res_dir"]) and m["res_dir"] < tol:
+                    termination = f"residual_below_tol({tol:g})"
+                    break
+    finally:
+        ps.set_clearzero_default_eps(old_eps)
+    wall = time.perf_counter() - t0
+    last = curve[-1]
+    return {"nqubit": nqubit, "n": n, "kappa": kappa, "kappa_analytic": kappa_analytic, "steps_budget": steps,
+            "steps_run": last["step"], "cz_eps": cz_eps, "p": p, "alpha": alpha, "beta": beta,
+            "final_res_dir": last["res_dir"], "final_res_opt": last["res_opt"], "final_success_prob": last["success_prob"],
+            "final_fidelity": last["fidelity"], "wall_s": wall, "walk_s": wall - t_extract,
+            "ms_per_step": (wall - t_extract) / last["step"] * 1e3, "peak_logged_entries": peak,
+            "termination": termination, "curve": curve}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--nqubit", type=int, default=20)
+    ap.add_argument("--steprate", type=float, default=1.1)
+    ap.add_argument("--p", type=float, default=1.3)
+    ap.add_argument("--alpha", type=float, default=2.2666666666564934)
+    ap.add_argument("--beta", type=float, default=-1.0)
+    ap.add_argument("--kappa", type=float, default=0.0)
+    ap.add_argument("--log-every", type=int, default=250)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-steps", type=int, default=10 ** 9)
+    ap.add_argument("--cz-eps", type=float, default=1e-14)
+    ap.add_argument("--learned", action="store_true", help="size branching operators with the learned sizer")
+    ap.add_argument("--json", default=None, help="write the summary (with the curve) to this file")
+    args = ap.parse_args()
+    import sparqsim_amd as ps
+    from . import sizer
+
+    if args.learned and not sizer.activate_default():
+        raise SystemExit("no sizer weights found")
+    res = solve(ps, args.nqubit, args.steprate, args.p, args.alpha, args.beta, args.kappa, args.log_every, args.tol,
+                args.max_steps, args.cz_eps, log=lambda s: print(s, flush=True))
+    if args.learned:
+        res["sizer"] = sizer.stats()
+    summary = {k: v for k, v in res.items() if k != "curve"}
+    print(json.dumps(summary), flush=True)
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump(res, f)
+
+
+if __name__ == "__main__":
+    main()
