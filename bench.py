@@ -20,6 +20,14 @@ sys.path.insert(0, ROOT)
 METRIC = "sparse entries/s + HBM GB/s; 2^20 QDA solve time (batched+learned vs plain)"
 HBM_PEAK_GBS = 8000.0
 
+# Measured HBM bytes per launch at 2^20 entries / 32-bit data (rocprofv3 --pmc FETCH_SIZE x2.0 +
+# WRITE_SIZE x1.0, factors calibrated on a 512 MiB device copy); profiles/qram_interference_2p20_traffic.md.
+PMC_TRAFFIC_MIB_2P20 = {
+    "pointwise_kernel": 28.4 + 24.0, "compact_scatter_kernel": 24.1 + 24.0, "compact_count_kernel": 16.0,
+    "key_stats_kernel": 8.2, "expand_single_kernel": 28.0, "wht_first_pass_kernel": 32.1 + 32.0,
+    "wht_pass_kernel": 22.9 + 24.9,
+}
+
 
 class ProfRecord(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64),
@@ -234,7 +242,9 @@ def main():
                                f"2^{A} uint32 memory words), H->QRAMLoad->{D}xZ->QRAMLoad->H",
                    "entries": 1 << A, "entry_ops_per_step": ops, "final_entries": n_out, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (PMC_TRAFFIC_MIB_2P20[dom["name"]] * 2**20
+                                 if (A, D) == (20, 32) and dom["name"] in PMC_TRAFFIC_MIB_2P20 else None),
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
         "step_hbm_GBs": hbm_total / (ms_per_step * 1e-3) / 1e9,
         "kernel_ms_per_step": kernels_ms,
