@@ -187,10 +187,22 @@ def main():
     ap.add_argument("--grover-bits", type=int, default=14)
     ap.add_argument("--grover-iterations", type=int, default=40)
     ap.add_argument("--grover-plain-traj", type=int, default=16, help="trajectories timed one state at a time")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 512 MiB device copies (the reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this engine has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": a.numel() * 8, "copy_bytes_written_per_launch": a.numel() * 8}))
+        return
 
     import sparqsim_amd as ps
     from sparqsim_amd.workloads import QRAMInterference, qram_memory

@@ -57,7 +57,9 @@ enum sparq_cond_kind {
     SPARQ_COND_NONZERO = 0,  /* field != 0                         (conditioned_by_nonzeros) */
     SPARQ_COND_ALL_ONES = 1, /* field == 2^width - 1               (conditioned_by_all_ones) */
     SPARQ_COND_BIT = 2,      /* bit `value` of field is 1          (conditioned_by_bit)      */
-    SPARQ_COND_VALUE = 3     /* field == value                     (conditioned_by_value)    */
+    SPARQ_COND_VALUE = 3,    /* field == value                     (conditioned_by_value)    */
+    SPARQ_COND_GE = 4,       /* field >= value      (PartialTraceSelectRange lower bound, partial_trace.cpp:187-225) */
+    SPARQ_COND_LE = 5        /* field <= value      (PartialTraceSelectRange upper bound) */
 };
 typedef struct {
     uint32_t kind;
@@ -123,6 +125,22 @@ enum sparq_pw_opcode {
     SPARQ_PW_INC_WRAP,            /* f0 += 1 mod 2^w; on wrap f1 = (f1+1)%2       PlusOneAndOverflow
                                      (SparQ_Algorithm/src/BlockEncoding/block_encoding_tridiagonal.cpp:20-47) */
     SPARQ_PW_DEC_WRAP,            /* f0 -= 1 mod 2^w; on wrap f1 = (f1+1)%2       PlusOneAndOverflow::dag (:55-84) */
+    /* Sparse-matrix access of the CKS quantum walk (SparQ_Algorithm/src/hamiltonian_simulation.cpp) */
+    SPARQ_PW_XOR_AFFINE,          /* f3 ^= f0 + c0*f1 (+ f2 when f2.width)        GetRowAddr (:222-237), GetDataAddr (:239-255) */
+    SPARQ_PW_XOR_BSEARCH,         /* f2 ^= search of f1 in table[f0, f0+c0):      QuantumBinarySearch_Fast (:182-220)
+                                     c1 halvings, mid = (l+r)/2, hit -> mid,
+                                     miss -> 0; table index masked by c2 */
+    SPARQ_PW_XOR_QW_ANGLE,        /* f1 ^= rational(acos(sqrt(r))/2pi), r =       GetQWRotateAngle_Int_Int_Int (:257-290)
+                                     |f0| / Amax on c0 data bits; c1 = 1:
+                                     unsigned, Amax = 2^c0-1; c1 = 0: two's
+                                     complement, Amax = 2^(c0-1)-1 */
+    /* Direct oracles of the tridiagonal walk (SparQ_Algorithm/src/hamiltonian_tridiagonal.cpp); the
+       matrix is c0 = n_row | k_width << 48, c1 = alpha_enc, c2 = beta_enc; fields f0 offset, f1 row,
+       f2 slot / column, f3 output / search result.  M(i) is TridiagonalSpec::combined_mem
+       (SparQ_Algorithm/include/hamiltonian_simulation_tridiagonal.h:104-190). */
+    SPARQ_PW_TRIDIAG_LOAD,        /* f3 ^= M(f0 + 4*f1 + f2)                      TridiagOracle1 (:16-34) */
+    SPARQ_PW_TRIDIAG_COL2SLOT,    /* column -> slot                               TridiagOracle2 (:36-61) */
+    SPARQ_PW_TRIDIAG_SLOT2COL,    /* slot -> column                               TridiagOracle2::dag (:63-88) */
     SPARQ_PW_OPCODE_END
 };
 
@@ -227,6 +245,8 @@ int sparq_replicate(sparq_state_t st, sparq_field_t target);
 
 /* Normalize (dark_magic.cpp:11-33). *norm2 receives the sum of |amp|^2 before scaling. */
 int sparq_normalize(sparq_state_t st, double* norm2);
+/* Sum of |amp|^2 without scaling (CheckNormalization / CheckNan, debugger.cpp:13-70: NaN iff an amplitude is NaN). */
+int sparq_norm2(sparq_state_t st, double* norm2);
 /* TestRemovable helper (debugger.cpp:191-212): *uniform = 1 when every entry has the same value. */
 int sparq_field_uniform(sparq_state_t st, sparq_field_t f, int* uniform, uint64_t* value);
 
@@ -261,14 +281,22 @@ int sparq_release_cached_memory(void);
 
 /* Post-selection into a dense vector, the device side of GetOutput
  * (SparQ_Algorithm/src/DiscreteAdiabatic/qda_fundamental.cpp:123-169): every
- * entry whose `zero` fields are all 0 is written to out[index], index = the
- * `index` fields concatenated with field 0 least significant; out holds
- * len = 2^(sum of index widths) complex values (2*len doubles, re/im
- * interleaved), zero elsewhere; *weight = sum of |a|^2 of the selected entries
+ * entry whose `zero` fields are all 0 and whose index (the `index` fields
+ * concatenated with field 0 least significant) is below len is written to
+ * out[index]; out holds len complex values (2*len doubles, re/im interleaved),
+ * zero elsewhere, len <= 2^(sum of index widths) and <= 2^40 (GetOutput uses
+ * the full 2^(sum of widths)); *weight = sum of |a|^2 of the selected entries
  * (the caller renormalises, as GetOutput does).  Keys are unique, so no index is
  * written twice. */
 int sparq_post_select_dense(sparq_state_t st, const sparq_field_t* zero, uint32_t nzero, const sparq_field_t* index,
                             uint32_t nindex, double* out, uint64_t len, double* weight);
+
+/* Post-selection in place, PartialTraceSelect / PartialTraceSelectRange
+ * (SparQ/src/partial_trace.cpp:148-225): drop every entry that fails one of the
+ * conditions (kept entries stay in order), then scale the kept amplitudes by
+ * f = 1/sqrt(sum of their |a|^2).  *factor = f; +inf when nothing is kept
+ * (the state is then empty), 0 for an empty input state. */
+int sparq_select(sparq_state_t st, const sparq_cond_t* conds, uint32_t nconds, double* factor);
 
 /* Radix-sort digit width (no counterpart in the reference, which sorts with
  * comparison sorts): 8 or 11 forces that width for every later sort, 0 (the

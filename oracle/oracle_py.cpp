@@ -11,7 +11,7 @@
 
 #include <chrono>
 
-#include "sparq_oracle_ops.hpp"
+#include "sparq_oracle_cks.hpp"
 
 namespace py = pybind11;
 using namespace oracle;
@@ -247,6 +247,86 @@ PYBIND11_MODULE(_oracle, m) {
     m.def("ZeroConditionalPhaseFlip", [](py::object regs) { return ZeroConditionalPhaseFlip(rids(regs)); });
     m.def("Reflection_Bool", [](py::object regs, bool inv) { return Reflection_Bool(rids(regs), inv); }, py::arg("regs"), py::arg("inverse") = false);
     m.def("GlobalPhase", [](complex_t c) { return GlobalPhase(c); });
+
+    // ---- CKS quantum-walk primitives and post-selection
+    py::class_<SparseMatrix, std::shared_ptr<SparseMatrix>>(m, "SparseMatrix")
+        .def(py::init([](std::vector<double> vals, std::vector<size_t> sp, size_t ds, size_t nnz, size_t nrow, bool pos) {
+            if (vals.size() != nnz * nrow || sp.size() != nnz * nrow) throw std::invalid_argument("need nnz_col entries per row");
+            auto mat = std::make_shared<SparseMatrix>();
+            for (double x : vals) mat->elements.push_back(quantise_element(x, ds, pos));
+            mat->sparsity = sp; mat->data_size = ds; mat->nnz_col = nnz; mat->n_row = nrow; mat->positive_only = pos;
+            return mat;
+        }), py::arg("elements"), py::arg("sparsity"), py::arg("data_size"), py::arg("nnz_col"), py::arg("n_row"), py::arg("positive_only"))
+        .def_static("from_encoded", [](std::vector<uint64_t> enc, std::vector<size_t> sp, size_t ds, size_t nnz, size_t nrow, bool pos) {
+            if (enc.size() != nnz * nrow || sp.size() != nnz * nrow) throw std::invalid_argument("need nnz_col entries per row");
+            auto mat = std::make_shared<SparseMatrix>();
+            mat->elements = enc; mat->sparsity = sp; mat->data_size = ds; mat->nnz_col = nnz; mat->n_row = nrow; mat->positive_only = pos;
+            return mat;
+        }, py::arg("elements"), py::arg("sparsity"), py::arg("data_size"), py::arg("nnz_col"), py::arg("n_row"), py::arg("positive_only") = true)
+        .def("get_data", &SparseMatrix::get_data)
+        .def("get_sparsity_offset", &SparseMatrix::get_sparsity_offset)
+        .def_readonly("elements", &SparseMatrix::elements)
+        .def_readonly("sparsity", &SparseMatrix::sparsity)
+        .def_readonly("positive_only", &SparseMatrix::positive_only)
+        .def_readonly("data_size", &SparseMatrix::data_size)
+        .def_readonly("nnz_col", &SparseMatrix::nnz_col)
+        .def_readonly("n_row", &SparseMatrix::n_row);
+    py::class_<TridiagonalSpec>(m, "TridiagonalSpec")
+        .def_readonly("n_row", &TridiagonalSpec::n_row)
+        .def_readonly("data_size", &TridiagonalSpec::data_size)
+        .def_readonly("alpha_enc", &TridiagonalSpec::alpha_enc)
+        .def_readonly("beta_enc", &TridiagonalSpec::beta_enc)
+        .def_readonly("sparse_offset", &TridiagonalSpec::sparse_offset)
+        .def_readonly("addr_mask", &TridiagonalSpec::addr_mask)
+        .def_readonly("k_width", &TridiagonalSpec::k_width)
+        .def_property_readonly_static("slot_count", [](py::object) { return TridiagonalSpec::slot_count; })
+        .def("window_base", &TridiagonalSpec::window_base)
+        .def("col_of", &TridiagonalSpec::col_of)
+        .def("value_of", &TridiagonalSpec::value_of)
+        .def("combined_mem", &TridiagonalSpec::combined_mem);
+    m.def("make_tridiagonal_matrix_spec", [](double a, double b, size_t n, size_t ds) {
+        auto [mat, spec] = make_tridiagonal_matrix_spec(a, b, n, ds);
+        return py::make_tuple(std::make_shared<SparseMatrix>(std::move(mat)), spec);
+    });
+    m.def("GetRowAddr", [](py::object o, py::object r, size_t sz, py::object out) { return GetRowAddr(rid(o), rid(r), sz, rid(out)); });
+    m.def("GetDataAddr", [](py::object o, py::object r, py::object c, size_t sz, py::object out) {
+        return GetDataAddr(rid(o), rid(r), rid(c), sz, rid(out));
+    });
+    m.def("QuantumBinarySearch_Fast", [](std::shared_ptr<QRAMCircuit> q, py::object o, size_t len, py::object t, py::object r) {
+        return QuantumBinarySearch_Fast(q, rid(o), len, rid(t), rid(r));
+    });
+    m.def("GetQWRotateAngle_Int_Int_Int", [](py::object d, py::object r, py::object c, py::object o, const SparseMatrix& mat) {
+        return GetQWRotateAngle_Int_Int_Int(rid(d), rid(r), rid(c), rid(o), mat);
+    });
+    m.def("TridiagOracle1", [](py::object o, py::object r, py::object s, py::object out, const TridiagonalSpec& sp) {
+        return TridiagOracle1(rid(o), rid(r), rid(s), rid(out), sp);
+    });
+    m.def("TridiagOracle2", [](py::object o, py::object r, py::object k, py::object sr, const TridiagonalSpec& sp) {
+        return TridiagOracle2(rid(o), rid(r), rid(k), rid(sr), sp);
+    });
+    m.def("TridiagParityPhase", [](py::object r) { return TridiagParityPhase(rid(r)); });
+    m.def("CheckNan", []() { return fnop([](State& st) { CheckNan(st); }, "CheckNan"); });
+    m.def("CheckNormalization", [](double t) { return fnop([t](State& st) { CheckNormalization(st, t); }, "CheckNormalization"); },
+          py::arg("threshold") = 1e-5);
+    struct Select { std::vector<size_t> regs; std::vector<uint64_t> vals; };
+    py::class_<Select>(m, "PartialTraceSelect")
+        .def(py::init([](py::dict d) {
+            Select s;
+            for (auto item : d) { s.regs.push_back(rid(item.first)); s.vals.push_back(item.second.cast<uint64_t>()); }
+            return s;
+        }))
+        .def("__call__", [](const Select& s, PyState& p) { return PartialTraceSelect(p.s.basis_states, s.regs, s.vals); });
+    struct SelectRange { size_t reg; size_t lo, hi; };
+    py::class_<SelectRange>(m, "PartialTraceSelectRange")
+        .def(py::init([](py::object r, std::pair<size_t, size_t> rg) { return SelectRange{rid(r), rg.first, rg.second}; }))
+        .def("__call__", [](const SelectRange& s, PyState& p) { return PartialTraceSelectRange(p.s.basis_states, s.reg, s.lo, s.hi); });
+    py::class_<ChebyshevPolynomialCoefficient>(m, "ChebyshevPolynomialCoefficient")
+        .def(py::init<size_t>())
+        .def_readonly("b", &ChebyshevPolynomialCoefficient::b)
+        .def("C", &ChebyshevPolynomialCoefficient::C)
+        .def("coef", &ChebyshevPolynomialCoefficient::coef)
+        .def("sign", &ChebyshevPolynomialCoefficient::sign)
+        .def("step", &ChebyshevPolynomialCoefficient::step);
 
     // Timing helper for the CPU baseline: wall seconds of fn() measured in C++.
     m.def("wall_seconds", [](py::function fn) {

@@ -1,0 +1,567 @@
+"""CKS (Childs-Kothari-Somma) Chebyshev quantum-walk linear solver built from the engine's operators.
+
+The composites follow the reference's templates operator by operator
+(SparQ_Algorithm/include/hamiltonian_simulation.h:983-2065 for the QRAM-backed
+sparse-access walk, SparQ_Algorithm/include/hamiltonian_simulation_tridiagonal.h:
+398-855 for the tridiagonal walk with direct oracles) and run on any module that
+exposes the pysparq operator surface -- the HIP engine (`sparqsim_amd`) or the
+CPU oracle.  The per-entry primitives (GetRowAddr, GetDataAddr,
+QuantumBinarySearch_Fast, GetQWRotateAngle_Int_Int_Int, TridiagOracle1/2,
+TridiagParityPhase) and PartialTraceSelect(Range) are engine operators; this
+file holds the sequencing, the LCU bookkeeping and the classical pieces
+(Chebyshev coefficients, residual metrics).
+
+A walk over R independent right-hand sides runs as one state: a register
+`rhs` with the lowest id (the most significant key field) holds the
+right-hand-side index, no walk operator reads it, so every right-hand side
+evolves independently while each operator is one launch over all of them.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+# ---------------------------------------------------------------- Chebyshev coefficients
+class ChebyshevPolynomialCoefficient:
+    """Coefficients of the LCU sum over walk powers W^(2j+1)
+    (SparQ_Algorithm/src/hamiltonian_simulation.cpp:44-99)."""
+
+    def __init__(self, b: int):
+        self.b = int(b)
+
+    def C(self, Big: int, Small: int) -> float:
+        """binom(Big, Small) / 4^b, evaluated with one early division by 2^b to stay finite."""
+        ret = 1.0
+        flag = True
+        pow2_b = math.pow(2, self.b)
+        for i in range(Small):
+            ret /= float(Small - i)
+            ret *= float(Big - i)
+            if flag and ret > pow2_b:
+                flag = False
+                ret /= pow2_b
+        if flag:
+            ret /= pow2_b
+        ret /= pow2_b
+        return ret
+
+    def coef(self, j: int) -> float:
+        if self.b > 100:
+            return math.erfc((j + 0.5) / math.sqrt(self.b)) * 2
+        return 4.0 * sum(self.C(2 * self.b, self.b + i) for i in range(j + 1, self.b + 1))
+
+    def sign(self, j: int) -> bool:
+        return bool(j & 1)
+
+    def step(self, j: int) -> int:
+        return 2 * j + 1
+
+
+def chebyshev_inverse_coefficients(m: float, M: float, J: int, ngrid: int = 4096):
+    """Least-squares fit of 1/x ~ sum_j d_j T_(2j+1)(x) on a Chebyshev-Lobatto grid of
+    [m, M] (ChebyshevInverseCoefficients, hamiltonian_simulation_tridiagonal.h:54-100).
+    The reference solves with an SVD (Eigen BDCSVD, rank threshold eps*min(rows, cols));
+    numpy's SVD least squares is used with the same rank threshold.  Returns (d, sum|d|)."""
+    th = math.pi * np.arange(ngrid + 1, dtype=np.float64) / float(ngrid)
+    x = 0.5 * (M + m) + 0.5 * (M - m) * np.cos(th)
+    V = np.empty((ngrid + 1, J + 1))
+    t_prev, t_cur = np.ones_like(x), x.copy()
+    V[:, 0] = t_cur
+    for k in range(2, 2 * J + 2):
+        t_prev, t_cur = t_cur, 2.0 * x * t_cur - t_prev
+        if k % 2 == 1:
+            V[:, (k - 1) // 2] = t_cur
+    y = 1.0 / x
+    rcond = np.finfo(np.float64).eps * min(V.shape)
+    d = np.linalg.lstsq(V, y, rcond=rcond)[0]
+    return [float(v) for v in d], float(np.sum(np.abs(d)))
+
+
+def kappa_schedule(kappa: float, eps: float):
+    """(b, j0) of the LCU truncation (hamiltonian_simulation.h:1880-1890)."""
+    b = int(kappa * kappa * (math.log(kappa) - math.log(eps)))
+    j0 = int(math.sqrt(b * (math.log(4 * b) - math.log(eps))))
+    return b, j0
+
+
+# ---------------------------------------------------------------- sparse-access oracles (QRAM path)
+class SparseMatrixOracle1:
+    """|offset>|i>|s>|0> -> |offset>|i>|s>|a_is>: element load through the QRAM
+    (hamiltonian_simulation.h:983-1058).  Self-adjoint."""
+
+    def __init__(self, mod, qram, reg_offset, reg_row, reg_col_id, reg_output, row_size):
+        self.mod, self.qram = mod, qram
+        self.reg_offset, self.reg_row, self.reg_col_id, self.reg_output = reg_offset, reg_row, reg_col_id, reg_output
+        self.row_size = row_size
+
+    def __call__(self, st):
+        m = self.mod
+        m.AddRegister("data_addr", m.UnsignedInteger, self.qram.address_size)(st)
+        m.GetDataAddr(self.reg_offset, self.reg_row, self.reg_col_id, self.row_size, "data_addr")(st)
+        m.QRAMLoad(self.qram, "data_addr", self.reg_output)(st)
+        m.GetDataAddr(self.reg_offset, self.reg_row, self.reg_col_id, self.row_size, "data_addr")(st)
+        m.RemoveRegister("data_addr")(st)
+
+    dag = __call__
+
+
+class SparseMatrixOracle2:
+    """Forward: |offset>|i>|j>|0> -> |offset>|i>|s_j> (column -> slot by binary
+    search); dag: slot -> column (hamiltonian_simulation.h:1060-1172)."""
+
+    def __init__(self, mod, qram, reg_sparse_offset, reg_row, reg_col, reg_search_result, row_size):
+        self.mod, self.qram = mod, qram
+        self.reg_sparse_offset, self.reg_row, self.reg_col, self.reg_sr = reg_sparse_offset, reg_row, reg_col, reg_search_result
+        self.row_size = row_size
+
+    def __call__(self, st):
+        m, q = self.mod, self.qram
+        m.AddRegister("row_addr", m.UnsignedInteger, q.address_size)(st)
+        m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr")(st)
+        m.QuantumBinarySearch_Fast(q, "row_addr", self.row_size, self.reg_col, self.reg_sr)(st)
+        m.QRAMLoad(q, self.reg_sr, self.reg_col)(st)
+        m.Swap_General_General(self.reg_col, self.reg_sr)(st)
+        m.Add_AnyInt_AnyInt_InPlace(self.reg_col, "row_addr").dag(st)
+        m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr")(st)
+        m.RemoveRegister("row_addr")(st)
+
+    def dag(self, st):
+        m, q = self.mod, self.qram
+        m.AddRegister("row_addr", m.UnsignedInteger, q.address_size)(st)
+        m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr")(st)
+        m.Add_AnyInt_AnyInt_InPlace(self.reg_col, "row_addr")(st)
+        m.Swap_General_General(self.reg_col, self.reg_sr)(st)
+        m.QRAMLoad(q, self.reg_sr, self.reg_col)(st)
+        m.QuantumBinarySearch_Fast(q, "row_addr", self.row_size, self.reg_col, self.reg_sr)(st)
+        m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr")(st)
+        m.RemoveRegister("row_addr")(st)
+
+
+class T:
+    """State preparation of the walk: |j>|0> -> |j> sum_k sqrt(|A_jk|)/.. |k>|b2>
+    (hamiltonian_simulation.h:1276-1446).  With `spec` the direct oracles replace
+    the QRAM ones (T_Tridiagonal, hamiltonian_simulation_tridiagonal.h:398-534)."""
+
+    def __init__(self, mod, qram, reg_data_offset, reg_sparse_offset, reg_j, reg_b1, reg_k, reg_b2, reg_search_result,
+                 nnz_col, data_size, mat, spec=None):
+        self.mod, self.qram, self.mat, self.spec = mod, qram, mat, spec
+        self.reg_data_offset, self.reg_sparse_offset = reg_data_offset, reg_sparse_offset
+        self.reg_j, self.reg_b1, self.reg_k, self.reg_b2, self.reg_sr = reg_j, reg_b1, reg_k, reg_b2, reg_search_result
+        self.nnz_col, self.data_size = nnz_col, data_size
+        self.k_digits = int(math.log2(nnz_col))
+
+    # element load and slot <-> column conversion, QRAM or direct
+    def _o1(self, st):
+        if self.spec is not None:
+            self.mod.TridiagOracle1(self.reg_data_offset, self.reg_j, self.reg_k, "data", self.spec)(st)
+        else:
+            SparseMatrixOracle1(self.mod, self.qram, self.reg_data_offset, self.reg_j, self.reg_k, "data", self.nnz_col)(st)
+
+    def _o2(self, st, dagger: bool):
+        if self.spec is not None:
+            op = self.mod.TridiagOracle2(self.reg_sparse_offset, self.reg_j, self.reg_k, self.reg_sr, self.spec)
+        else:
+            op = SparseMatrixOracle2(self.mod, self.qram, self.reg_sparse_offset, self.reg_j, self.reg_k, self.reg_sr, self.nnz_col)
+        if dagger:
+            op.dag(st)
+        else:
+            op(st)
+
+    def _angle(self, st):
+        self.mod.GetQWRotateAngle_Int_Int_Int("data", self.reg_j, self.reg_k, "rotation", self.mat)(st)
+
+    def __call__(self, st):
+        m = self.mod
+        m.AddRegister("data", m.UnsignedInteger, self.data_size)(st)
+        m.Hadamard_Int(self.reg_k, self.k_digits)(st)
+        self._o1(st)
+        self._o2(st, True)
+        m.AddRegister("rotation", m.Rational, 32)(st)
+        self._angle(st)
+        m.CondRot_Fixed_Bool("rotation", self.reg_b2)(st)
+        self._angle(st)
+        m.RemoveRegister("rotation")(st)
+        self._o2(st, False)
+        self._o1(st)
+        m.RemoveRegister("data")(st)
+        self._o2(st, True)
+        m.CheckNan()(st)
+
+    def dag(self, st):
+        m = self.mod
+        m.AddRegister("data", m.UnsignedInteger, self.data_size)(st)
+        self._o2(st, False)
+        m.CheckNan()(st)
+        self._o1(st)
+        self._o2(st, False)
+        m.AddRegister("rotation", m.Rational, 32)(st)
+        self._angle(st)
+        m.CondRot_Fixed_Bool("rotation", self.reg_b2).dag(st)
+        self._angle(st)
+        m.RemoveRegister("rotation")(st)
+        m.ClearZero()(st)
+        self._o2(st, True)
+        m.CheckNormalization()(st)
+        self._o1(st)
+        m.Hadamard_Int(self.reg_k, self.k_digits)(st)
+        m.ClearZero()(st)
+        m.RemoveRegister("data")(st)
+
+
+# ---------------------------------------------------------------- the walk
+class QuantumWalkNSteps:
+    """Registers and step sequence of the Chebyshev walk W = T^dag P0 T Swap
+    (hamiltonian_simulation.h:1551-1773).  With `spec` it is
+    QuantumWalkNSteps_Tridiagonal (hamiltonian_simulation_tridiagonal.h:536-685):
+    direct oracles, no QRAM, register width max(log2(n)+1, data_size)."""
+
+    data_offset = "data_offset"
+    sparse_offset = "sparse_offset"
+    j = "row_id"
+    b1 = "reg_b1"
+    k = "col_id"
+    b2 = "reg_b2"
+    j_comp = "j_comp"
+    k_comp = "k_comp"
+
+    def __init__(self, mod, mat, spec=None, qram=None):
+        self.mod, self.mat, self.spec = mod, mat, spec
+        self.data_size = mat.data_size
+        self.n_row = mat.n_row
+        self.nnz_col = mat.nnz_col
+        self.offset = mat.get_sparsity_offset()
+        if spec is None:
+            data = mat.get_data()
+            self.addr_size = int(math.log2(len(data)))
+            self.default_register_size = max(self.addr_size, self.data_size)
+            self.qram = qram if qram is not None else mod.QRAMCircuit_qutrit(self.addr_size, self.data_size, list(data))
+        else:
+            self.addr_size = None
+            self.default_register_size = max(int(math.log2(self.n_row)) + 1, self.data_size)
+            self.qram = None
+
+    def GetVecInputReg(self) -> str:
+        return self.j
+
+    def get_init_size(self) -> int:
+        return int(math.log2(self.n_row))
+
+    def InitEnvironment(self):
+        S, m, w = self.mod.System, self.mod, self.default_register_size
+        S.add_register(self.data_offset, m.UnsignedInteger, w)
+        S.add_register(self.sparse_offset, m.UnsignedInteger, w)
+        S.add_register(self.j, m.UnsignedInteger, w)
+        S.add_register(self.b1, m.Boolean, 1)
+        S.add_register(self.k, m.UnsignedInteger, w)
+        S.add_register(self.b2, m.Boolean, 1)
+        S.add_register(self.j_comp, m.UnsignedInteger, w)
+        S.add_register(self.k_comp, m.UnsignedInteger, w)
+
+    def CreateSys(self):
+        st = self.mod.SparseState()
+        self.mod.Init_Unsafe(self.sparse_offset, self.offset)(st)
+        return st
+
+    def _T(self):
+        return T(self.mod, self.qram, self.data_offset, self.sparse_offset, self.j, self.b1, self.k, self.b2, self.k_comp,
+                 self.nnz_col, self.default_register_size, self.mat, self.spec)
+
+    def _swaps(self, st):
+        m = self.mod
+        m.Swap_General_General(self.j, self.k)(st)
+        m.Swap_General_General(self.b1, self.b2)(st)
+        m.Swap_General_General(self.j_comp, self.k_comp)(st)
+
+    def FirstStep(self, st):
+        t = self._T()
+        t(st)
+        self._swaps(st)
+        t.dag(st)
+
+    def StepImplOneStep(self, st):
+        m = self.mod
+        t = self._T()
+        m.ZeroConditionalPhaseFlip([self.j_comp, self.k_comp, self.b1, self.k, self.b2])(st)
+        t(st)
+        m.CheckNan()(st)
+        self._swaps(st)
+        t.dag(st)
+        m.CheckNan()(st)
+
+    def Step(self, st):
+        self.StepImplOneStep(st)
+        self.StepImplOneStep(st)
+
+    def MakeNStepState(self, n_steps: int):
+        st = self.CreateSys()
+        self.mod.Hadamard_Int(self.j, self.get_init_size())(st)
+        self.mod.ClearZero()(st)
+        if n_steps == 0:
+            return st
+        self.FirstStep(st)
+        for _ in range(n_steps - 1):
+            self.StepImplOneStep(st)
+        return st
+
+
+class LCU_Container_NoiseFree:
+    """Sum_j c_j W^(2j+1)|b>, accumulated term by term with add_systems
+    (LCU_Container_NoiseFree, hamiltonian_simulation.h:1858-2065; the tridiagonal
+    variant with optional fitted 1/x coefficients,
+    hamiltonian_simulation_tridiagonal.h:687-855)."""
+
+    def __init__(self, mod, mat, kappa: float, eps: float, spec=None):
+        self.mod = mod
+        self.current_state = mod.SparseState()
+        self.quantum_walk_obj = QuantumWalkNSteps(mod, mat, spec)
+        self.kappa, self.eps = kappa, eps
+        self.b, self.j0 = kappa_schedule(kappa, eps)
+        self.chebyshev_obj = ChebyshevPolynomialCoefficient(self.b)
+        self.inv_coef = None
+        self.j = 0
+        self.a = 0.0
+        self.quantum_walk_obj.InitEnvironment()
+        self.step_state = self.quantum_walk_obj.CreateSys()
+
+    def set_inverse_coefficients(self, coef):
+        """Use fitted coefficients d_0..d_J (chebyshev_inverse_coefficients) in place of
+        the binomial-tail family; j0 becomes J."""
+        self.inv_coef = list(coef) if coef is not None else None
+        self.j0 = len(self.inv_coef) - 1 if self.inv_coef is not None else kappa_schedule(self.kappa, self.eps)[1]
+
+    def GetInputVecReg(self) -> str:
+        return self.quantum_walk_obj.GetVecInputReg()
+
+    def ExternalInput_V2(self, op):
+        op(self.step_state)
+        self.mod.ClearZero()(self.step_state)
+        self.quantum_walk_obj.FirstStep(self.step_state)
+
+    def term(self, j: int):
+        if self.inv_coef is not None:
+            return abs(self.inv_coef[j]), self.inv_coef[j] < 0.0
+        return self.chebyshev_obj.coef(j), self.chebyshev_obj.sign(j)
+
+    def Step(self) -> bool:
+        if self.j > self.j0:
+            return False
+        if self.j != 0:
+            self.quantum_walk_obj.Step(self.step_state)
+        c, sgn = self.term(self.j)
+        self.a += c
+        self.Add(self.step_state, c, sgn)
+        self.j += 1
+        return True
+
+    def Add(self, new_state, coef: float, sign: bool):
+        self.mod.add_systems(self.current_state, new_state, -coef if sign else coef)
+
+    def _impl_partial_trace(self, st) -> float:
+        w, m = self.quantum_walk_obj, self.mod
+        ret = m.PartialTraceSelect({w.b1: 0, w.k: 0, w.b2: 0, w.j_comp: 0, w.k_comp: 0, w.sparse_offset: w.offset,
+                                    w.data_offset: 0})(st)
+        ret *= m.PartialTraceSelectRange(w.j, (0, w.n_row - 1))(st)
+        return 1.0 / ret / ret / self.a / self.a
+
+    def PartialTrace(self) -> float:
+        return self._impl_partial_trace(self.current_state)
+
+    def PartialTrace_Nondestructive(self):
+        st = self.current_state.clone()
+        p = self._impl_partial_trace(st)
+        return st, p
+
+
+# ---------------------------------------------------------------- Poiseuille case (tridiagonal)
+def spectrum_interval(spec, ds: int, n: int):
+    """Exact interval of the quantised walk matrix (CKSTridiagonalProbe.cpp:200-212):
+    lambda = (alpha_q + 2 beta_q cos(k pi/(n+1))) / (2^ds * 4), widened by 1e-12."""
+    ccos = math.cos(math.pi / float(n + 1))
+    denom = float(2 ** ds) * 4.0
+    lam1 = (float(spec.alpha_enc) + 2.0 * float(spec.beta_enc) * ccos) / denom
+    lamn = (float(spec.alpha_enc) - 2.0 * float(spec.beta_enc) * ccos) / denom
+    return min(lam1, lamn) * (1.0 - 1e-12), max(lam1, lamn) * (1.0 + 1e-12)
+
+
+def extract_solution(mod, state, input_reg, n: int, dphase: bool) -> np.ndarray:
+    """Amplitude vector m[j] of a post-selected state, with D = diag((-1)^j) undone
+    (CKSTridiagonalProbe.cpp:96-101)."""
+    ids, vals, amps = state.export()
+    col = list(ids).index(mod.System.get_id(input_reg))
+    m = np.zeros(n, dtype=np.complex128)
+    m[vals[:, col].astype(np.int64)] = amps
+    if dphase:
+        m[1::2] *= -1.0
+    return m
+
+
+def run_probe(mod, nqubit: int = 4, kappa: float = 0.0, eps: float = 0.01, ds: int = 32, path: str = "direct",
+              dphase: bool = True, coef: str = "stock", J: int = 640, max_terms: int | None = None,
+              alpha: float = 2.2666666666564934, beta: float = -1.0, log=None):
+    """The CKS LCU solve of tridiag(beta, alpha, beta) x = ones/sqrt(n) as
+    Experiment_CKS_TridiagonalProbe runs it (CKSTridiagonalProbe.cpp:74-135, 228-345):
+    input D|b> (Hadamard + TridiagParityPhase), FirstStep, then Step() for every
+    LCU term; the final post-selected state gives res_dir, res_opt, fidelity and
+    the success probability.  path = "qram" (QuantumWalkNSteps over the QRAM
+    tables) or "direct" (QuantumWalkNSteps_Tridiagonal)."""
+    from . import qda
+
+    n = 1 << nqubit
+    kappa_analytic = qda.analytic_kappa(alpha, beta, n)
+    kappa = kappa if kappa > 0 else kappa_analytic
+    mod.System.clear()
+    mat, spec = mod.make_tridiagonal_matrix_spec(alpha, beta, n, ds)
+    obj = LCU_Container_NoiseFree(mod, mat, kappa, eps, spec if path == "direct" else None)
+    if coef == "true":
+        m_spec, M_spec = spectrum_interval(spec, ds, n)
+        d, _ = chebyshev_inverse_coefficients(m_spec, M_spec, J)
+        obj.set_inverse_coefficients(d)
+    xref = qda.thomas_solution(alpha, beta, n)
+    inp = obj.GetInputVecReg()
+    mod.Hadamard_Int(inp, obj.quantum_walk_obj.get_init_size())(obj.step_state)
+    if dphase:
+        mod.TridiagParityPhase(inp)(obj.step_state)
+    mod.ClearZero()(obj.step_state)
+    obj.quantum_walk_obj.FirstStep(obj.step_state)
+    after_first = obj.step_state.size()
+    terms = 0
+    while (max_terms is None or terms < max_terms) and obj.Step():
+        terms += 1
+        if log is not None and terms % 10 == 0:
+            log(f"[{path}] j={terms}/{obj.j0} entries={obj.step_state.size()} max_sys={mod.System.max_system_size()}")
+    state, p_succ = obj.PartialTrace_Nondestructive()
+    m = extract_solution(mod, state, inp, n, dphase)
+    met = qda.block_metrics(m, alpha, beta, xref)
+    return {"n": n, "kappa": kappa, "b": obj.b, "j0": obj.j0, "terms": terms, "p_succ": p_succ, "res_dir": met["res_dir"],
+            "res_opt": met["res_opt"], "fidelity": met["fidelity"], "succP_blk": met["success_prob"],
+            "entries_after_first_step": after_first, "max_system_size": mod.System.max_system_size(),
+            "alpha_enc": spec.alpha_enc, "beta_enc": spec.beta_enc, "a": obj.a}
+
+
+# ---------------------------------------------------------------- many right-hand sides in one state
+def select_dense(mod, state, zero_regs, index_regs, length: int):
+    """out[index] of the entries whose zero_regs are 0 and whose index (index_regs
+    concatenated, first least significant) is below length; returns (out, weight).
+    The device post-selection when the module has it, else a host filter."""
+    if hasattr(state, "post_select_dense"):
+        return state.post_select_dense(list(zero_regs), list(index_regs), length)
+    ids, vals, amps = state.export()
+    ids = list(ids)
+    sel = np.ones(len(amps), dtype=bool)
+    for r in zero_regs:
+        sel &= vals[:, ids.index(mod.System.get_id(r))] == 0
+    idx = np.zeros(len(amps), dtype=np.uint64)
+    at = 0
+    for r in index_regs:
+        idx |= vals[:, ids.index(mod.System.get_id(r))] << np.uint64(at)
+        at += mod.System.size_of(r)
+    sel &= idx < np.uint64(length)
+    out = np.zeros(length, dtype=np.complex128)
+    out[idx[sel].astype(np.int64)] = amps[sel]
+    return out, float(np.sum(np.abs(amps[sel]) ** 2))
+
+
+def rhs_metrics(m: np.ndarray, b: np.ndarray, alpha: float, beta: float, xref: np.ndarray) -> dict:
+    """block metrics against a general normalised real right-hand side b:
+    y = A x/|x|, res_dir = sqrt(2 - 2 Re<b, y>/|y|), res_opt^2 = |y|^2 - |<b, y>|^2,
+    fid = |<xref, x/|x|>|^2 (the formulas of CKSPoiseuilleBench.cu:95-131 with b general)."""
+    w = float(np.sum(np.abs(m) ** 2))
+    if w <= 1e-300:
+        return {"res_dir": float("nan"), "res_opt": float("nan"), "weight": w, "fidelity": float("nan")}
+    x = m / math.sqrt(w)
+    y = alpha * x
+    y[:-1] += beta * x[1:]
+    y[1:] += beta * x[:-1]
+    y2 = float(np.sum(np.abs(y) ** 2))
+    proj = complex(np.dot(b, y))
+    res_dir = math.sqrt(max(2.0 - 2.0 * proj.real / math.sqrt(y2), 0.0))
+    res_opt = math.sqrt(max(y2 - abs(proj) ** 2, 0.0))
+    f = complex(np.dot(xref, x))
+    return {"res_dir": res_dir, "res_opt": res_opt, "weight": w, "fidelity": abs(f) ** 2}
+
+
+def thomas_solve(alpha: float, beta: float, b: np.ndarray) -> np.ndarray:
+    """Normalised solution of tridiag(beta, alpha, beta) x = b."""
+    from scipy.linalg import solve_banded
+
+    n = b.shape[0]
+    ab = np.zeros((3, n))
+    ab[0, 1:] = beta
+    ab[1, :] = alpha
+    ab[2, :-1] = beta
+    x = solve_banded((1, 1), ab, b)
+    return x / np.linalg.norm(x)
+
+
+class BatchedPoiseuilleCKS:
+    """The tridiagonal CKS walk over R right-hand sides in one state.  A register
+    `rhs` is created before the walk registers (lowest id: the most significant
+    key field); entries of different right-hand sides never share a key, so
+    sorting, merging and pruning never mix them and every right-hand side
+    follows exactly the single-RHS sequence of LCU_Container_NoiseFree_Tridiagonal
+    (hamiltonian_simulation_tridiagonal.h:687-855).  The accumulator starts
+    empty instead of |0...0> (which the single-RHS post-selection removes by its
+    sparse_offset test), so the dense post-selection can read it directly."""
+
+    def __init__(self, mod, nqubit: int, rhs: np.ndarray, kappa: float = 16.0, eps: float = 0.01, ds: int = 32,
+                 alpha: float = 2.2666666666564934, beta: float = -1.0, coef: str = "true", J: int = 640,
+                 dphase: bool = True):
+        self.mod, self.alpha, self.beta, self.dphase = mod, alpha, beta, dphase
+        self.n = 1 << nqubit
+        rhs = np.asarray(rhs, dtype=np.float64)
+        if rhs.ndim != 2 or rhs.shape[1] != self.n:
+            raise ValueError("rhs must be an R x 2^nqubit array")
+        self.R = rhs.shape[0]
+        self.rhs = rhs / np.linalg.norm(rhs, axis=1, keepdims=True)
+        self.rhs_bits = max(1, int(math.ceil(math.log2(self.R))))
+        mod.System.clear()
+        mod.System.add_register("rhs", mod.UnsignedInteger, self.rhs_bits)
+        self.mat, self.spec = mod.make_tridiagonal_matrix_spec(alpha, beta, self.n, ds)
+        self.lcu = LCU_Container_NoiseFree(mod, self.mat, kappa, eps, self.spec)
+        if coef == "true":
+            m_spec, M_spec = spectrum_interval(self.spec, ds, self.n)
+            self.lcu.set_inverse_coefficients(chebyshev_inverse_coefficients(m_spec, M_spec, J)[0])
+        w = self.lcu.quantum_walk_obj
+        mod.PartialTraceSelect({w.sparse_offset: w.offset})(self.lcu.current_state)   # empty accumulator
+        self.xref = np.stack([thomas_solve(alpha, beta, b) for b in self.rhs])
+
+    def prepare_input(self):
+        """Load sum_r |r>|b_r>/sqrt(R) (one normalised right-hand side per r), apply the
+        parity phase D, ClearZero and the first walk step."""
+        mod, w, st = self.mod, self.lcu.quantum_walk_obj, self.lcu.step_state
+        ids = [i for i in range(mod.System.register_count()) if mod.System.status_of(i)]
+        col = {i: c for c, i in enumerate(ids)}
+        N = self.R * self.n
+        vals = np.zeros((N, len(ids)), dtype=np.uint64)
+        vals[:, col[mod.System.get_id("rhs")]] = np.repeat(np.arange(self.R, dtype=np.uint64), self.n)
+        vals[:, col[mod.System.get_id(w.j)]] = np.tile(np.arange(self.n, dtype=np.uint64), self.R)
+        vals[:, col[mod.System.get_id(w.sparse_offset)]] = w.offset
+        # each right-hand side with weight 1/R, so the whole state stays normalised
+        st.load(vals, (self.rhs.reshape(-1) / math.sqrt(self.R)).astype(np.complex128))
+        del vals
+        if self.dphase:
+            mod.TridiagParityPhase(w.j)(st)
+        mod.ClearZero()(st)
+        w.FirstStep(st)
+
+    def Step(self) -> bool:
+        return self.lcu.Step()
+
+    def extract(self):
+        """Per right-hand side: the solution block m_r (D undone), its metrics and
+        p_succ = R |m_r|^2 / a^2 (the single-RHS partial-trace success probability)."""
+        w = self.lcu.quantum_walk_obj
+        Rp = 1 << self.rhs_bits
+        out, _ = select_dense(self.mod, self.lcu.current_state, [w.b1, w.k, w.b2, w.j_comp, w.k_comp, w.data_offset],
+                              ["rhs", w.j], Rp * self.n)
+        blocks = out.reshape(self.n, Rp).T[: self.R].copy()
+        if self.dphase:
+            blocks[:, 1::2] *= -1.0
+        res = []
+        for r in range(self.R):
+            met = rhs_metrics(blocks[r], self.rhs[r], self.alpha, self.beta, self.xref[r])
+            met["p_succ"] = met["weight"] * self.R / (self.lcu.a * self.lcu.a)
+            res.append(met)
+        return blocks, res

@@ -1,6 +1,8 @@
 // extern "C" entry points of the engine (declared in include/sparq_hip.h).
 // Every call converts exceptions into status codes and records a message.
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -323,6 +325,25 @@ int sparq_post_select_dense(sparq_state_t h, const sparq_field_t* zero, uint32_t
     });
 }
 
+int sparq_select(sparq_state_t h, const sparq_cond_t* conds, uint32_t nconds, double* factor) {
+    return guarded([&] {
+        SPQ_REQUIRE(conds != nullptr || nconds == 0, "null argument");
+        State& st = S(h);
+        double f = 0.0;
+        if (st.n != 0) {
+            select_by_conditions(st, conds, nconds);
+            const double s = sum_abs2(st);
+            if (s != 0) {
+                f = 1.0 / std::sqrt(s);
+                scale_amps(st, f);
+            } else {
+                f = std::numeric_limits<double>::infinity();
+            }
+        }
+        if (factor) *factor = f;
+    });
+}
+
 int sparq_set_radix_bits(int bits) {
     return guarded([&] {
         SPQ_REQUIRE(bits == 0 || bits == 8 || bits == 11, "digit width must be 0 (auto), 8 or 11");
@@ -536,6 +557,13 @@ int sparq_normalize(sparq_state_t h, double* norm2) {
         double f = 1.0 / std::sqrt(s);
         if (std::isnan(f)) throw Error(SPARQ_RUNTIME, "Normalize: bad result (NaN norm)");
         scale_amps(st, f);
+    });
+}
+
+int sparq_norm2(sparq_state_t h, double* norm2) {
+    return guarded([&] {
+        SPQ_REQUIRE(norm2 != nullptr, "null argument");
+        *norm2 = sum_abs2(S(h));
     });
 }
 

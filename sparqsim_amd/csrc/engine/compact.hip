@@ -13,7 +13,7 @@ namespace sparq {
 constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;   // 2048 entries per workgroup, blocked per thread
 
-enum Mode { kClearZero = 0, kPartition = 1, kMerge = 2 };
+enum Mode { kClearZero = 0, kPartition = 1, kMerge = 2, kSelect = 3 };
 
 // Predicate of entry i for the given mode; for kMerge also returns the run sum.
 template <int W, int MODE>
@@ -22,11 +22,12 @@ __device__ __forceinline__ bool keep_entry(const uint64_t* __restrict__ keys, co
     if (MODE == kClearZero) {
         sum = amps[i];
         return !(cabs2(sum) < eps);
-    } else if (MODE == kPartition) {
+    } else if (MODE == kPartition || MODE == kSelect) {
         uint64_t k[W];
         load_key<W>(keys, cap, i, k);
         bool ok = true;
         for (uint32_t c = 0; c < cl.n; ++c) ok = ok && cond_ok<W>(k, cl.c[c]);
+        if (MODE == kSelect && ok) sum = amps[i];
         return ok;
     } else {
         uint64_t k[W];
@@ -134,7 +135,7 @@ static uint64_t run_compact(State& st, double eps, const CondList& cl) {
     uint32_t* counts = dalloc<uint32_t>(nb, st.stream);
     uint64_t* offs = dalloc<uint64_t>(nb + 1, st.stream);
     {
-        SPQ_PROF(st.stream, "compact_count_kernel", n * (MODE == kClearZero ? 16.0 : MODE == kPartition ? 8.0 * W : 8.0 * W + 16));
+        SPQ_PROF(st.stream, "compact_count_kernel", n * (MODE == kClearZero ? 16.0 : MODE == kMerge ? 8.0 * W + 16 : 8.0 * W));
         compact_count_kernel<W, MODE><<<nb, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, nullptr, eps, cl, counts);
     }
     SPQ_HIP(hipGetLastError());
@@ -151,7 +152,7 @@ static uint64_t run_compact(State& st, double eps, const CondList& cl) {
     double2* aout;
     alloc_arrays(st, st.W, cap_out, &kout, &aout);
     {
-        SPQ_PROF(st.stream, "compact_scatter_kernel", n * (MODE == kClearZero ? 16.0 : MODE == kPartition ? 8.0 * W : 8.0 * W + 16) + (double)out_n * (8.0 * W + 16));
+        SPQ_PROF(st.stream, "compact_scatter_kernel", n * (MODE == kClearZero ? 16.0 : MODE == kMerge ? 8.0 * W + 16 : 8.0 * W) + (double)out_n * (8.0 * W + 16));
         compact_scatter_kernel<W, MODE><<<nb, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, nullptr, eps, cl, offs, offs + nb, kout,
                                                                    aout, cap_out);
     }
@@ -169,6 +170,10 @@ static void clear_zero_w(State& st, double eps, uint64_t* kept) {
 template <int W>
 static void partition_w(State& st, const CondList& cl, uint64_t* kept) {
     *kept = run_compact<W, kPartition>(st, 0.0, cl);
+}
+template <int W>
+static void select_w(State& st, const CondList& cl, uint64_t* kept) {
+    *kept = run_compact<W, kSelect>(st, 0.0, cl);
 }
 template <int W>
 static void merge_w(State& st, double eps, uint64_t* kept) {
@@ -242,6 +247,20 @@ uint64_t partition_by_conditions(State& st, const sparq_cond_t* conds, uint32_t 
     if (st.n == 0) return 0;
     uint64_t kept;
     SPQ_DISPATCH_W(st.W, partition_w, st, cl, &kept);
+    return kept;
+}
+
+uint64_t select_by_conditions(State& st, const sparq_cond_t* conds, uint32_t ncond) {
+    SPQ_REQUIRE(ncond <= SPARQ_MAX_PROGRAM_CONDS, "too many conditions");
+    CondList cl{};
+    cl.n = ncond;
+    for (uint32_t c = 0; c < ncond; ++c) {
+        SPQ_REQUIRE(conds[c].field.word < st.W, "condition field out of range");
+        cl.c[c] = conds[c];
+    }
+    if (st.n == 0 || ncond == 0) return st.n;
+    uint64_t kept;
+    SPQ_DISPATCH_W(st.W, select_w, st, cl, &kept);
     return kept;
 }
 

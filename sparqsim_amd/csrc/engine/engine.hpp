@@ -175,6 +175,8 @@ bool has_sorted_duplicates(State& st);
 // ---- condition split (compact.hip): stable partition so that entries that
 // satisfy `conds` come first; returns their count.
 uint64_t partition_by_conditions(State& st, const sparq_cond_t* conds, uint32_t ncond);
+// Keep only the entries satisfying every condition (in order); returns the kept count.
+uint64_t select_by_conditions(State& st, const sparq_cond_t* conds, uint32_t ncond);
 
 // ---- branching (branch.hip)
 void hadamard_int(State& st, sparq_field_t target, uint32_t n_digits, bool full_variant, double eps,

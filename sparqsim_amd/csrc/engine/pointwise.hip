@@ -41,6 +41,48 @@ __device__ __forceinline__ uint64_t table_word(const void* t, uint32_t bytes, ui
 
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
+// Binary search of `target` in table[l, l+len) with `steps` halvings; the hit
+// address, or 0 when no probed word equals the target
+// (QuantumBinarySearch_Fast::binary_search, SparQ_Algorithm/src/hamiltonian_simulation.cpp:182-199).
+__device__ __forceinline__ uint64_t table_search(const void* t, uint32_t bytes, uint64_t mask, uint64_t l, uint64_t len,
+                                                 uint64_t steps, uint64_t target) {
+    uint64_t r = l + len;
+    for (uint64_t s = 0; s < steps; ++s) {
+        const uint64_t mid = (l + r) / 2;
+        const uint64_t v = table_word(t, bytes, mid & mask);
+        if (v == target) return mid;
+        if (v < target) l = mid;
+        else r = mid;
+    }
+    return 0;
+}
+
+// The tridiagonal matrix of the direct walk oracles: four slots per row over
+// the column window [base(j), base(j)+4), and the combined memory
+// [values of all rows | columns of all rows] of 8n words
+// (TridiagonalSpec, SparQ_Algorithm/include/hamiltonian_simulation_tridiagonal.h:104-190).
+struct Tridiag {
+    uint64_t n, alpha, beta, kmask;
+    __device__ __forceinline__ explicit Tridiag(const sparq_pw_op_t& op)
+        : n(op.c[0] & ((uint64_t(1) << 48) - 1)), alpha(op.c[1]), beta(op.c[2]), kmask(fmask((uint32_t)(op.c[0] >> 48))) {}
+    __device__ __forceinline__ uint64_t base(uint64_t j) const {
+        const uint64_t b = j > 0 ? j - 1 : 0;
+        return b < n - 4 ? b : n - 4;
+    }
+    __device__ __forceinline__ uint64_t value(uint64_t j, uint64_t s) const {
+        const uint64_t c = base(j) + s;
+        if (c == j) return alpha;
+        if (c + 1 == j || c == j + 1) return beta;
+        return 0;
+    }
+    __device__ __forceinline__ uint64_t mem(uint64_t i) const {
+        i &= 8 * n - 1;
+        if (i < 4 * n) return value(i >> 2, i & 3);
+        const uint64_t t = i - 4 * n;
+        return base(t >> 2) + (t & 3);
+    }
+};
+
 template <int W>
 __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W], double2& amp) {
     const sparq_field_t f0 = op.f[0], f1 = op.f[1], f2 = op.f[2], f3 = op.f[3];
@@ -185,6 +227,60 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
             fput<W>(k, f1, fget<W>(k, f1) ^ table_word(op.table, op.table_word_bytes, a));
             break;
         }
+        case SPARQ_PW_XOR_AFFINE: {
+            uint64_t a = fget<W>(k, f0) + op.c[0] * fget<W>(k, f1);
+            if (f2.width) a += fget<W>(k, f2);
+            fput<W>(k, f3, fget<W>(k, f3) ^ a);
+            break;
+        }
+        case SPARQ_PW_XOR_BSEARCH: {
+            const uint64_t hit = table_search(op.table, op.table_word_bytes, op.c[2], fget<W>(k, f0), op.c[0], op.c[1], fget<W>(k, f1));
+            fput<W>(k, f2, fget<W>(k, f2) ^ hit);
+            break;
+        }
+        case SPARQ_PW_XOR_QW_ANGLE: {
+            const uint32_t ds = (uint32_t)op.c[0];
+            uint64_t v = fget<W>(k, f0);
+            double ratio;
+            if (op.c[1]) {
+                const uint64_t amax = fmask(ds);
+                ratio = (double)(v & amax) * 1.0 / (double)amax;
+            } else {
+                const int64_t vr = sext(v & fmask(ds), ds);
+                ratio = (double)(vr < 0 ? -vr : vr) * 1.0 / (double)fmask(ds - 1);
+            }
+            ratio = fmax(0.0, fmin(1.0, ratio));
+            const double o = acos(sqrt(ratio)) / kPi / 2;
+            fput<W>(k, f1, fget<W>(k, f1) ^ rational_bits(o, f1.width));
+            break;
+        }
+        case SPARQ_PW_TRIDIAG_LOAD: {
+            const Tridiag m(op);
+            fput<W>(k, f3, fget<W>(k, f3) ^ m.mem(fget<W>(k, f0) + 4 * fget<W>(k, f1) + fget<W>(k, f2)));
+            break;
+        }
+        case SPARQ_PW_TRIDIAG_COL2SLOT: {
+            const Tridiag m(op);
+            const uint64_t j = fget<W>(k, f1), col = fget<W>(k, f2), sr = fget<W>(k, f3);
+            const uint64_t base = m.base(j), row_addr = fget<W>(k, f0) + 4 * j;
+            const bool hit = col >= base && col < base + 4;
+            const uint64_t sr1 = sr ^ (hit ? row_addr + (col - base) : 0);
+            const uint64_t k1 = col ^ m.mem(sr1);
+            fput<W>(k, f2, (sr1 - row_addr) & m.kmask);
+            fput<W>(k, f3, k1 & m.kmask);
+            break;
+        }
+        case SPARQ_PW_TRIDIAG_SLOT2COL: {
+            const Tridiag m(op);
+            const uint64_t j = fget<W>(k, f1), slot = fget<W>(k, f2), sr = fget<W>(k, f3);
+            const uint64_t base = m.base(j), row_addr = fget<W>(k, f0) + 4 * j;
+            const uint64_t k1 = (slot + row_addr) & m.kmask;
+            const uint64_t k2 = sr ^ m.mem(k1);
+            const bool hit = k2 >= base && k2 < base + 4;
+            fput<W>(k, f2, k2);
+            fput<W>(k, f3, k1 ^ (hit ? row_addr + (k2 - base) : 0));
+            break;
+        }
         case SPARQ_PW_PHASE_BITS: {
             uint64_t bits = fget<W>(k, f0) & op.c[0];
             const double2 ph = make_double2(op.d[0], op.d[1]);
@@ -323,7 +419,13 @@ void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_c
         SPQ_REQUIRE((uint32_t)op.cond_first + op.ncond <= nconds, "condition range out of bounds");
         for (int f = 0; f < 4; ++f)
             if (op.f[f].width) check_field(st, op.f[f]);
-        if (op.opcode == SPARQ_PW_QRAM_LOAD) {
+        if (op.opcode == SPARQ_PW_TRIDIAG_LOAD || op.opcode == SPARQ_PW_TRIDIAG_COL2SLOT || op.opcode == SPARQ_PW_TRIDIAG_SLOT2COL) {
+            const uint64_t n = op.c[0] & ((uint64_t(1) << 48) - 1), kw = op.c[0] >> 48;
+            SPQ_REQUIRE(n >= 4 && (n & (n - 1)) == 0, "tridiagonal oracle needs a power-of-two dimension >= 4");
+            SPQ_REQUIRE(kw >= 1 && kw <= 64, "tridiagonal oracle: bad column width");
+        }
+        if (op.opcode == SPARQ_PW_XOR_QW_ANGLE) SPQ_REQUIRE(op.c[0] >= 2 && op.c[0] <= 63, "walk angle: data size out of range");
+        if (op.opcode == SPARQ_PW_QRAM_LOAD || op.opcode == SPARQ_PW_XOR_BSEARCH) {
             SPQ_REQUIRE(op.table != nullptr, "QRAM load without table");
             SPQ_REQUIRE(op.table_word_bytes == 1 || op.table_word_bytes == 2 || op.table_word_bytes == 4 ||
                             op.table_word_bytes == 8, "bad table word size");

@@ -341,9 +341,9 @@ double sum_abs2(State& st) {
 }
 
 // ------------------------------------------------------------------ dense post-selection
-// Entries whose `zero` fields are all 0 are written to out[index], index =
-// concatenation of the `index` fields (field 0 least significant); per-block
-// partial sums of |a|^2 of the selected entries.
+// Entries whose `zero` fields are all 0 and whose index (the concatenation of
+// the `index` fields, field 0 least significant) is below len are written to
+// out[index]; per-block partial sums of |a|^2 of the selected entries.
 struct FieldList {
     uint32_t n;
     sparq_field_t f[8];
@@ -367,8 +367,9 @@ __global__ void __launch_bounds__(kBlock) post_select_kernel(const uint64_t* __r
             idx |= fget<W>(k, index.f[f]) << at;
             at += index.f[f].width;
         }
+        if (idx >= len) continue;   // outside the requested range: not selected
         const double2 a = amps[i];
-        if (idx < len) out[idx] = a;
+        out[idx] = a;
         s += cabs2(a);
     }
     s = wave_sum(s);
@@ -407,7 +408,8 @@ double post_select_dense(State& st, const sparq_field_t* zero, uint32_t nzero, c
         il.f[i] = index[i];
         bits += index[i].width;
     }
-    SPQ_REQUIRE(bits <= 40 && len == (uint64_t(1) << bits), "output length must be 2^(total index bits) (at most 2^40)");
+    SPQ_REQUIRE(bits <= 64 && len >= 1 && len <= (uint64_t(1) << 40) && (bits == 64 || len <= (uint64_t(1) << bits)),
+                "output length must be in [1, min(2^(total index bits), 2^40)]");
     double2* out = dalloc<double2>(len, st.stream);
     SPQ_HIP(hipMemsetAsync(out, 0, len * sizeof(double2), st.stream));
     const unsigned g = st.n ? std::min<unsigned>(grid_for(st.n, kBlock), 1024) : 1;

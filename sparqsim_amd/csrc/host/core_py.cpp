@@ -143,7 +143,7 @@ PYBIND11_MODULE(_core, m) {
             py::gil_scoped_release r;
             s.import_entries(v, a);
         })
-        .def("post_select_dense", [](SparseState& s, py::object zero, py::object index) {
+        .def("post_select_dense", [](SparseState& s, py::object zero, py::object index, uint64_t length) {
             std::vector<size_t> z, ix;
             for (auto it : zero) z.push_back(resolve(it));
             for (auto it : index) ix.push_back(resolve(it));
@@ -151,12 +151,12 @@ PYBIND11_MODULE(_core, m) {
             double w;
             {
                 py::gil_scoped_release r;
-                w = s.post_select_dense(z, ix, out);
+                w = s.post_select_dense(z, ix, out, length);
             }
             py::array_t<std::complex<double>> a(out.size());
             if (!out.empty()) std::memcpy(a.mutable_data(), out.data(), out.size() * 16);
             return py::make_tuple(a, w);
-        }, py::arg("zero_regs"), py::arg("index_regs"))
+        }, py::arg("zero_regs"), py::arg("index_regs"), py::arg("length") = 0)
         .def("to_string", [](SparseState& s, int precision) { return s.to_string(precision); }, py::arg("precision") = 6)
         .def("__str__", [](SparseState& s) { return s.to_string(6); })
         .def("__repr__", [](SparseState& s) { return "<SparseState entries=" + std::to_string(s.size()) + ">"; });
@@ -305,6 +305,92 @@ PYBIND11_MODULE(_core, m) {
         .def("__call__", [](const CombineRegister& a, SparseState& s) { return a(s); }, Release());
     m.def("reset_systems", [](SparseState& s) { reset_systems(s); }, Release());
     m.def("add_systems", [](SparseState& cur, SparseState& add, double coef) { add_systems(cur, add, coef); }, Release());
+
+    // debug checks and post-selection (debugger.h:78-160, partial_trace.h:83-145, PySparQ/core.cpp:249-402)
+    py::class_<CheckNormalization, SelfAdjointOperator, std::shared_ptr<CheckNormalization>>(m, "CheckNormalization")
+        .def(py::init<>()).def(py::init<double>(), py::arg("threshold"));
+    py::class_<CheckNan, SelfAdjointOperator, std::shared_ptr<CheckNan>>(m, "CheckNan").def(py::init<>());
+    m.def("set_debug_checks", [](bool on) { debug_checks() = on; }, py::arg("on"),
+          "Enable CheckNormalization / CheckNan (off by default, as in the release build).");
+    m.def("get_debug_checks", []() { return debug_checks(); });
+    py::class_<PartialTraceSelect>(m, "PartialTraceSelect")
+        .def(py::init<const std::map<std::string, uint64_t>&>(), py::arg("name_value_map"))
+        .def(py::init<const std::map<size_t, uint64_t>&>(), py::arg("id_value_map"))
+        .def(py::init<const std::vector<size_t>&, const std::vector<uint64_t>&>(), py::arg("reg_ids"), py::arg("select_values"))
+        .def(py::init<const std::vector<std::string>&, const std::vector<uint64_t>&>(), py::arg("reg_names"), py::arg("select_values"))
+        .def("__call__", [](const PartialTraceSelect& p, SparseState& s) { return p(s); }, py::arg("state"), Release());
+    py::class_<PartialTraceSelectRange>(m, "PartialTraceSelectRange")
+        .def(py::init<std::string_view, std::pair<size_t, size_t>>(), py::arg("register_name"), py::arg("select_range"))
+        .def(py::init<size_t, std::pair<size_t, size_t>>(), py::arg("register_id"), py::arg("select_range"))
+        .def("__call__", [](const PartialTraceSelectRange& p, SparseState& s) { return p(s); }, py::arg("state"), Release());
+
+    // CKS quantum-walk primitives (PySparQ/core.cpp:53-65, 1472-1510)
+    py::class_<SparseMatrix, std::shared_ptr<SparseMatrix>>(m, "SparseMatrix")
+        .def(py::init<>())
+        .def(py::init<const std::vector<double>&, const std::vector<size_t>&, size_t, size_t, size_t, bool>(), py::arg("elements"),
+             py::arg("sparsity"), py::arg("data_size"), py::arg("nnz_col"), py::arg("n_row"), py::arg("positive_only"))
+        .def_static("from_encoded", [](const std::vector<uint64_t>& enc, const std::vector<size_t>& sp, size_t ds, size_t nnz,
+                                       size_t nrow, bool pos) { return std::make_shared<SparseMatrix>(enc, sp, ds, nnz, nrow, pos); },
+                    py::arg("elements"), py::arg("sparsity"), py::arg("data_size"), py::arg("nnz_col"), py::arg("n_row"),
+                    py::arg("positive_only") = true)
+        .def("get_data", &SparseMatrix::get_data)
+        .def("get_sparsity_offset", &SparseMatrix::get_sparsity_offset)
+        .def_readonly("elements", &SparseMatrix::elements)
+        .def_readonly("sparsity", &SparseMatrix::sparsity)
+        .def_readonly("positive_only", &SparseMatrix::positive_only)
+        .def_readonly("data_size", &SparseMatrix::data_size)
+        .def_readonly("nnz_col", &SparseMatrix::nnz_col)
+        .def_readonly("n_row", &SparseMatrix::n_row);
+    pw_class<GetQWRotateAngle_Int_Int_Int>(m, "GetQWRotateAngle_Int_Int_Int")
+        .def(py::init<std::string_view, std::string_view, std::string_view, std::string_view, const SparseMatrix*>(), py::arg("data"),
+             py::arg("row"), py::arg("col"), py::arg("out"), py::arg("mat"))
+        .def(py::init<size_t, size_t, size_t, size_t, const SparseMatrix*>(), py::arg("data"), py::arg("row"), py::arg("col"),
+             py::arg("out"), py::arg("mat"));
+    pw_class<QuantumBinarySearch_Fast>(m, "QuantumBinarySearch_Fast")
+        .def(py::init<const QRAMCircuit_qutrit*, std::string_view, size_t, std::string_view, std::string_view>(), py::arg("qram"),
+             py::arg("address_offset_register"), py::arg("total_length"), py::arg("target_register"), py::arg("result_register"),
+             py::keep_alive<1, 2>())
+        .def(py::init<const QRAMCircuit_qutrit*, size_t, size_t, size_t, size_t>(), py::arg("qram"),
+             py::arg("address_offset_register"), py::arg("total_length"), py::arg("target_register"), py::arg("result_register"),
+             py::keep_alive<1, 2>());
+    pw_class<GetRowAddr>(m, "GetRowAddr")
+        .def(py::init<std::string_view, std::string_view, size_t, std::string_view>(), py::arg("reg_offset"), py::arg("reg_row"),
+             py::arg("row_size"), py::arg("reg_row_offset"))
+        .def(py::init<size_t, size_t, size_t, size_t>(), py::arg("reg_offset"), py::arg("reg_row"), py::arg("row_size"),
+             py::arg("reg_row_offset"));
+    pw_class<GetDataAddr>(m, "GetDataAddr")
+        .def(py::init<std::string_view, std::string_view, std::string_view, size_t, std::string_view>(), py::arg("reg_offset"),
+             py::arg("reg_row"), py::arg("reg_col_sparse"), py::arg("row_size"), py::arg("reg_data_offset"))
+        .def(py::init<size_t, size_t, size_t, size_t, size_t>(), py::arg("reg_offset"), py::arg("reg_row"), py::arg("reg_col_sparse"),
+             py::arg("row_size"), py::arg("reg_data_offset"));
+    py::class_<TridiagonalSpec>(m, "TridiagonalSpec")
+        .def_readonly("n_row", &TridiagonalSpec::n_row)
+        .def_readonly("data_size", &TridiagonalSpec::data_size)
+        .def_readonly("alpha_enc", &TridiagonalSpec::alpha_enc)
+        .def_readonly("beta_enc", &TridiagonalSpec::beta_enc)
+        .def_readonly("sparse_offset", &TridiagonalSpec::sparse_offset)
+        .def_readonly("addr_mask", &TridiagonalSpec::addr_mask)
+        .def_readonly("k_width", &TridiagonalSpec::k_width)
+        .def_property_readonly_static("slot_count", [](py::object) { return TridiagonalSpec::slot_count; })
+        .def("window_base", &TridiagonalSpec::window_base)
+        .def("col_of", &TridiagonalSpec::col_of)
+        .def("value_of", &TridiagonalSpec::value_of)
+        .def("combined_mem", &TridiagonalSpec::combined_mem);
+    m.def("make_tridiagonal_matrix_spec", [](double alpha, double beta, size_t n, size_t ds) {
+        auto [mat, spec] = make_tridiagonal_matrix_spec(alpha, beta, n, ds);
+        return py::make_tuple(std::make_shared<SparseMatrix>(std::move(mat)), spec);
+    }, py::arg("alpha"), py::arg("beta"), py::arg("n"), py::arg("ds"));
+    pw_class<TridiagOracle1>(m, "TridiagOracle1")
+        .def(py::init<std::string_view, std::string_view, std::string_view, std::string_view, const TridiagonalSpec&>(),
+             py::arg("reg_offset"), py::arg("reg_row"), py::arg("reg_slot"), py::arg("reg_out"), py::arg("spec"))
+        .def(py::init<size_t, size_t, size_t, size_t, const TridiagonalSpec&>(), py::arg("reg_offset"), py::arg("reg_row"),
+             py::arg("reg_slot"), py::arg("reg_out"), py::arg("spec"));
+    pw_class<TridiagOracle2>(m, "TridiagOracle2")
+        .def(py::init<std::string_view, std::string_view, std::string_view, std::string_view, const TridiagonalSpec&>(),
+             py::arg("reg_offset"), py::arg("reg_row"), py::arg("reg_k"), py::arg("reg_sr"), py::arg("spec"))
+        .def(py::init<size_t, size_t, size_t, size_t, const TridiagonalSpec&>(), py::arg("reg_offset"), py::arg("reg_row"),
+             py::arg("reg_k"), py::arg("reg_sr"), py::arg("spec"));
+    pw_class<TridiagParityPhase>(m, "TridiagParityPhase").def(py::init<std::string_view>(), py::arg("reg")).def(py::init<size_t>());
 
     m.def("device_name", []() {
         char buf[256] = {0};

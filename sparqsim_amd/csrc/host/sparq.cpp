@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <sstream>
 
@@ -342,7 +343,7 @@ void SparseState::from_host(const std::vector<System>& entries) {
 }
 
 double SparseState::post_select_dense(const std::vector<size_t>& zero_ids, const std::vector<size_t>& index_ids,
-                                      std::vector<complex_t>& out) {
+                                      std::vector<complex_t>& out, uint64_t length) {
     const Layout& L = sync();
     flush();
     std::vector<sparq_field_t> z, ix;
@@ -352,8 +353,8 @@ double SparseState::post_select_dense(const std::vector<size_t>& zero_ids, const
         ix.push_back(L.at(id));
         bits += ix.back().width;
     }
-    if (bits > 40) throw std::invalid_argument("post-selected index wider than 40 bits");
-    const uint64_t len = uint64_t(1) << bits;
+    if (length == 0 && bits > 40) throw std::invalid_argument("post-selected index wider than 40 bits");
+    const uint64_t len = length ? length : uint64_t(1) << bits;
     out.assign(len, complex_t(0, 0));
     double w = 0;
     check(sparq_post_select_dense(h_, z.data(), (uint32_t)z.size(), ix.data(), (uint32_t)ix.size(),
@@ -482,7 +483,8 @@ void PointwiseOperator::emit(SparseState& state, bool dagger) const {
     const Layout& L = state.sync();
     sparq_pw_op_t op{};
     op.opcode = dagger && dag_opcode ? dag_opcode : opcode;
-    for (size_t i = 0; i < regs.size() && i < 4; ++i) op.f[i] = L.at(regs[i]);
+    for (size_t i = 0; i < regs.size() && i < 4; ++i)
+        if (regs[i] != kNoRegister) op.f[i] = L.at(regs[i]);   // absent operand: width-0 field
     const uint64_t* cc = dagger && dag_opcode ? dag_c : c;
     const double* dd = dagger && dag_opcode ? dag_d : d;
     for (int i = 0; i < 3; ++i) op.c[i] = cc[i];
@@ -491,6 +493,7 @@ void PointwiseOperator::emit(SparseState& state, bool dagger) const {
     op.table_word_bytes = table_word_bytes;
     std::vector<sparq_cond_t> conds = conditions(L);
     for (auto& z : extra_zero_regs) conds.push_back(sparq_cond_t{SPARQ_COND_VALUE, L.at(z.first), z.second});
+    for (auto& b : extra_bit_regs) conds.push_back(sparq_cond_t{SPARQ_COND_BIT, L.at(b.first), b.second});
     state.push_pointwise(op, conds);
 }
 void PointwiseOperator::dag(SparseState& state) const {
@@ -1051,6 +1054,253 @@ size_t CombineRegister::operator()(SparseState& state) const {   // system_opera
     moves.emplace_back(old.field[sp], sparq_field_t{nf.word, nf.shift, (uint8_t)w2, 0});
     state.relayout_with(T, moves);
     return fp;
+}
+
+// ------------------------------------------------------------ debug checks
+bool& debug_checks() {
+    static bool on = false;
+    return on;
+}
+void CheckNormalization::operator()(SparseState& state) const {   // debugger.cpp:13-28
+    if (!debug_checks()) return;
+    state.flush();
+    double f = 0;
+    check(sparq_norm2(state.handle(), &f));
+    if (!(std::fabs(f - 1.0) < threshold) || std::isnan(f))
+        throw std::runtime_error("State is not normalized (factor " + std::to_string(f) + ").");
+}
+void CheckNan::operator()(SparseState& state) const {   // debugger.cpp:57-70
+    if (!debug_checks()) return;
+    state.flush();
+    double f = 0;
+    check(sparq_norm2(state.handle(), &f));
+    if (std::isnan(f)) throw std::runtime_error("Bad result: NaN amplitude.");
+}
+
+// ------------------------------------------------------------ post-selection
+PartialTraceSelect::PartialTraceSelect(const std::map<std::string, uint64_t>& sel) {
+    for (auto& [name, v] : sel) {
+        partial_trace_registers.push_back(System::get(name));
+        select_values.push_back(v);
+    }
+}
+PartialTraceSelect::PartialTraceSelect(const std::map<size_t, uint64_t>& sel) {
+    for (auto& [id, v] : sel) {
+        partial_trace_registers.push_back(id);
+        select_values.push_back(v);
+    }
+}
+PartialTraceSelect::PartialTraceSelect(const std::vector<size_t>& regs, const std::vector<uint64_t>& values)
+    : partial_trace_registers(regs), select_values(values) {
+    if (regs.size() != values.size()) throw std::invalid_argument("PartialTraceSelect: one value per register");
+}
+PartialTraceSelect::PartialTraceSelect(const std::vector<std::string>& regs, const std::vector<uint64_t>& values)
+    : select_values(values) {
+    if (regs.size() != values.size()) throw std::invalid_argument("PartialTraceSelect: one value per register");
+    for (auto& n : regs) partial_trace_registers.push_back(System::get(n));
+}
+static double select_on_device(SparseState& state, const std::vector<sparq_cond_t>& conds) {
+    state.flush();
+    double f = 0;
+    check(sparq_select(state.handle(), conds.data(), (uint32_t)conds.size(), &f));
+    return f;
+}
+double PartialTraceSelect::operator()(SparseState& state) const {   // partial_trace.cpp:148-185
+    if (partial_trace_registers.empty()) return 0;
+    const Layout& L = state.sync();
+    std::vector<sparq_cond_t> conds;
+    for (size_t i = 0; i < partial_trace_registers.size(); ++i) {
+        const size_t id = partial_trace_registers[i];
+        if (!System::status_of(id)) throw std::runtime_error("Register not found.");
+        // A value wider than the register can match no entry.
+        if (select_values[i] & ~width_mask(System::size_of(id))) {
+            conds.push_back(sparq_cond_t{SPARQ_COND_VALUE, L.at(id), 0});
+            conds.push_back(sparq_cond_t{SPARQ_COND_NONZERO, L.at(id), 0});
+        } else {
+            conds.push_back(sparq_cond_t{SPARQ_COND_VALUE, L.at(id), select_values[i]});
+        }
+    }
+    return select_on_device(state, conds);
+}
+double PartialTraceSelect::operator()(std::vector<System>& state) const {
+    if (partial_trace_registers.empty() || state.empty()) return 0;
+    double sum = 0;
+    auto drop = [&](const System& s) {
+        for (size_t i = 0; i < partial_trace_registers.size(); ++i)
+            if (s.get(partial_trace_registers[i]).value != select_values[i]) return true;
+        sum += std::norm(s.amplitude);
+        return false;
+    };
+    state.erase(std::remove_if(state.begin(), state.end(), drop), state.end());
+    if (sum == 0) return std::numeric_limits<double>::infinity();
+    const double f = 1.0 / std::sqrt(sum);
+    for (auto& s : state) s.amplitude *= f;
+    return f;
+}
+double PartialTraceSelectRange::operator()(SparseState& state) const {   // partial_trace.cpp:187-225
+    const Layout& L = state.sync();
+    if (!System::status_of(partial_trace_register)) throw std::runtime_error("Register not found.");
+    const sparq_field_t f = L.at(partial_trace_register);
+    std::vector<sparq_cond_t> conds{sparq_cond_t{SPARQ_COND_GE, f, select_range.first},
+                                    sparq_cond_t{SPARQ_COND_LE, f, select_range.second}};
+    return select_on_device(state, conds);
+}
+double PartialTraceSelectRange::operator()(std::vector<System>& state) const {
+    if (state.empty()) return 0;
+    double sum = 0;
+    auto drop = [&](const System& s) {
+        const uint64_t v = s.get(partial_trace_register).value;
+        if (v >= select_range.first && v <= select_range.second) {
+            sum += std::norm(s.amplitude);
+            return false;
+        }
+        return true;
+    };
+    state.erase(std::remove_if(state.begin(), state.end(), drop), state.end());
+    if (sum == 0) return std::numeric_limits<double>::infinity();
+    const double f = 1.0 / std::sqrt(sum);
+    for (auto& s : state) s.amplitude *= f;
+    return f;
+}
+
+// ------------------------------------------------------------ CKS quantum walk primitives
+static uint64_t encode_element(double x, size_t ds, bool positive_only) {
+    if (positive_only) {
+        const double top = double(width_mask(ds));
+        double v = std::floor(x * std::ldexp(1.0, (int)ds));
+        v = std::min(std::max(v, 0.0), top);
+        return uint64_t(v);
+    }
+    const double top = double(width_mask(ds - 1));
+    double v = std::trunc(x * std::ldexp(1.0, (int)ds - 1));
+    v = std::min(std::max(v, -top), top);
+    return uint64_t(int64_t(v)) & width_mask(ds);
+}
+static void check_sparse_shape(size_t ne, size_t ns, size_t ds, size_t nnz, size_t nrow) {
+    if (ds < 2 || ds > 63) throw std::invalid_argument("SparseMatrix: data_size must be in [2, 63]");
+    if (nnz == 0 || nrow == 0) throw std::invalid_argument("SparseMatrix: empty matrix");
+    if (ne != nnz * nrow || ns != nnz * nrow) throw std::invalid_argument("SparseMatrix: need nnz_col entries per row");
+}
+SparseMatrix::SparseMatrix(const std::vector<double>& values, const std::vector<size_t>& sp, size_t ds, size_t nnz, size_t nrow,
+                           bool pos)
+    : sparsity(sp), data_size(ds), nnz_col(nnz), n_row(nrow), positive_only(pos) {
+    check_sparse_shape(values.size(), sp.size(), ds, nnz, nrow);
+    elements.reserve(values.size());
+    for (double x : values) elements.push_back(encode_element(x, ds, pos));
+}
+SparseMatrix::SparseMatrix(const std::vector<uint64_t>& enc, const std::vector<size_t>& sp, size_t ds, size_t nnz, size_t nrow, bool pos)
+    : elements(enc), sparsity(sp), data_size(ds), nnz_col(nnz), n_row(nrow), positive_only(pos) {
+    check_sparse_shape(enc.size(), sp.size(), ds, nnz, nrow);
+}
+std::vector<uint64_t> SparseMatrix::get_data() const {
+    size_t total = 1;
+    while (total < elements.size() + sparsity.size()) total <<= 1;
+    std::vector<uint64_t> mem(total, 0);
+    std::copy(elements.begin(), elements.end(), mem.begin());
+    for (size_t i = 0; i < sparsity.size(); ++i) mem[elements.size() + i] = sparsity[i];
+    return mem;
+}
+
+GetRowAddr::GetRowAddr(size_t offset, size_t row, size_t row_sz, size_t out) {
+    require_ids({offset, row, out});
+    set_op(*this, "GetRowAddr", SPARQ_PW_XOR_AFFINE, {offset, row, kNoRegister, out});
+    c[0] = row_sz;
+}
+GetDataAddr::GetDataAddr(size_t offset, size_t row, size_t col, size_t row_sz, size_t out) {
+    require_ids({offset, row, col, out});
+    set_op(*this, "GetDataAddr", SPARQ_PW_XOR_AFFINE, {offset, row, col, out});
+    c[0] = row_sz;
+}
+QuantumBinarySearch_Fast::QuantumBinarySearch_Fast(const QRAMCircuit_qutrit* q, size_t offset, size_t len, size_t target,
+                                                   size_t result)
+    : total_length(len), max_step(size_t(std::log2(double(len))) + 1) {
+    if (q == nullptr || len == 0) throw std::invalid_argument("QuantumBinarySearch_Fast: bad arguments");
+    require_ids({offset, target, result});
+    set_op(*this, "QuantumBinarySearch_Fast", SPARQ_PW_XOR_BSEARCH, {offset, target, result});
+    c[0] = len;
+    c[1] = max_step;
+    c[2] = width_mask(q->address_size);
+    const void* ptr = nullptr;
+    check(sparq_table_device_ptr(q->table.get(), &ptr));
+    table = ptr;
+    table_word_bytes = q->word_bytes;
+    table_owner = q->table;
+}
+GetQWRotateAngle_Int_Int_Int::GetQWRotateAngle_Int_Int_Int(size_t data, size_t row, size_t col, size_t out, const SparseMatrix* mat) {
+    if (mat == nullptr) throw std::invalid_argument("GetQWRotateAngle_Int_Int_Int: null matrix");
+    require_ids({data, row, col, out});
+    if (mat->data_size < 2 || mat->data_size > 63) throw std::invalid_argument("GetQWRotateAngle_Int_Int_Int: bad data size");
+    set_op(*this, "GetQWRotateAngle_Int_Int_Int", SPARQ_PW_XOR_QW_ANGLE, {data, out});
+    c[0] = mat->data_size;
+    c[1] = mat->positive_only ? 1 : 0;
+}
+
+size_t TridiagonalSpec::window_base(size_t j) const {   // hamiltonian_simulation_tridiagonal.h:130-140
+    const size_t b = j > 0 ? j - 1 : 0;
+    const size_t bmax = n_row - slot_count;
+    return b < bmax ? b : bmax;
+}
+uint64_t TridiagonalSpec::value_of(size_t j, size_t s) const {   // :152-160
+    const size_t c = col_of(j, s);
+    if (c == j) return alpha_enc;
+    if (c + 1 == j || c == j + 1) return beta_enc;
+    return 0;
+}
+uint64_t TridiagonalSpec::combined_mem(uint64_t idx) const {   // :169-178
+    idx &= addr_mask;
+    const uint64_t half = uint64_t(n_row) * slot_count;
+    if (idx < half) return value_of(size_t(idx >> 2), size_t(idx & 3));
+    const uint64_t t = idx - half;
+    return uint64_t(col_of(size_t(t >> 2), size_t(t & 3)));
+}
+std::pair<SparseMatrix, TridiagonalSpec> make_tridiagonal_matrix_spec(double alpha, double beta, size_t n, size_t ds) {   // :192-225
+    if (n < 4 || (n & (n - 1))) throw std::invalid_argument("tridiagonal walk needs a power-of-two dimension >= 4");
+    if (ds < 2 || ds > 63) throw std::invalid_argument("data size must be in [2, 63]");
+    const double scale = 1.0 / (std::fabs(alpha) + 2.0 * std::fabs(beta));
+    TridiagonalSpec spec;
+    spec.n_row = n;
+    spec.data_size = ds;
+    spec.alpha_enc = uint64_t(std::floor(std::fabs(alpha) * scale * std::ldexp(1.0, (int)ds)));
+    spec.beta_enc = uint64_t(std::floor(std::fabs(beta) * scale * std::ldexp(1.0, (int)ds)));
+    spec.sparse_offset = n * TridiagonalSpec::slot_count;
+    spec.addr_mask = 2 * spec.sparse_offset - 1;
+    size_t lg = 0;
+    while ((size_t(1) << lg) < n) ++lg;
+    spec.k_width = std::max(lg + 1, ds);
+    std::vector<uint64_t> elements;
+    std::vector<size_t> sparsity;
+    elements.reserve(n * TridiagonalSpec::slot_count);
+    sparsity.reserve(n * TridiagonalSpec::slot_count);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t s = 0; s < TridiagonalSpec::slot_count; ++s) {
+            elements.push_back(spec.value_of(i, s));
+            sparsity.push_back(spec.col_of(i, s));
+        }
+    return {SparseMatrix(elements, sparsity, ds, TridiagonalSpec::slot_count, n), spec};
+}
+static void set_tridiag(PointwiseOperator& p, const TridiagonalSpec& spec) {
+    if (spec.n_row < 4 || (spec.n_row & (spec.n_row - 1)) || spec.n_row >= (uint64_t(1) << 48))
+        throw std::invalid_argument("tridiagonal oracle: bad dimension");
+    if (spec.addr_mask != 8 * spec.n_row - 1) throw std::invalid_argument("tridiagonal oracle: address mask must be 8n-1");
+    p.c[0] = uint64_t(spec.n_row) | (uint64_t(spec.k_width) << 48);
+    p.c[1] = spec.alpha_enc;
+    p.c[2] = spec.beta_enc;
+}
+TridiagOracle1::TridiagOracle1(size_t offset, size_t row, size_t slot, size_t out, const TridiagonalSpec& spec) {
+    require_ids({offset, row, slot, out});
+    set_op(*this, "TridiagOracle1", SPARQ_PW_TRIDIAG_LOAD, {offset, row, slot, out});
+    set_tridiag(*this, spec);
+}
+TridiagOracle2::TridiagOracle2(size_t offset, size_t row, size_t k, size_t sr, const TridiagonalSpec& spec) {
+    require_ids({offset, row, k, sr});
+    set_op(*this, "TridiagOracle2", SPARQ_PW_TRIDIAG_COL2SLOT, {offset, row, k, sr}, SPARQ_PW_TRIDIAG_SLOT2COL);
+    set_tridiag(*this, spec);
+    std::copy(c, c + 3, dag_c);
+}
+TridiagParityPhase::TridiagParityPhase(size_t reg) {   // hamiltonian_tridiagonal.cpp:90-105
+    require_ids({reg});
+    set_op(*this, "TridiagParityPhase", SPARQ_PW_NEG_AMP, {});
+    extra_bit_regs.emplace_back(reg, 0);
 }
 
 void add_systems(SparseState& current, SparseState& new_state, double coef) {   // system_operations.h:157-181

@@ -168,8 +168,9 @@ public:
     // Dense post-selection on the device (GetOutput without the renormalisation):
     // out[index] for entries whose zero_ids registers are 0, index = the index_ids
     // registers concatenated (first least significant); returns the selected weight.
+    // A nonzero `length` keeps only indices below it (out has `length` values).
     double post_select_dense(const std::vector<size_t>& zero_ids, const std::vector<size_t>& index_ids,
-                             std::vector<complex_t>& out);
+                             std::vector<complex_t>& out, uint64_t length = 0);
     std::string to_string(int precision = 6);
 
 private:
@@ -236,6 +237,7 @@ std::string take_plugin_error();
 double& clearzero_default_eps();
 
 // Per-entry operator described by an engine opcode over register ids.
+constexpr size_t kNoRegister = ~size_t(0);   // an absent operand in PointwiseOperator::regs
 struct PointwiseOperator : BaseOperator, Controllable {
     std::string name;
     uint32_t opcode = 0, dag_opcode = 0;    // dag_opcode 0: self-adjoint
@@ -247,7 +249,8 @@ struct PointwiseOperator : BaseOperator, Controllable {
     std::shared_ptr<void> table_owner;      // keeps a QRAM table alive
     const void* table = nullptr;
     uint32_t table_word_bytes = 0;
-    std::vector<std::pair<size_t, uint64_t>> extra_zero_regs;   // implicit "== 0" predicates
+    std::vector<std::pair<size_t, uint64_t>> extra_zero_regs;   // implicit "== value" predicates
+    std::vector<std::pair<size_t, uint64_t>> extra_bit_regs;    // implicit "bit set" predicates
     bool has_dag = true;
     Controllable* controls() override { return this; }
     SPQ_BOTH_VIEWS
@@ -540,6 +543,143 @@ struct CombineRegister {
     CombineRegister(std::string_view f, std::string_view s) : first_name(f), second_name(s) {}
     CombineRegister(size_t f, size_t s) : first_name(System::name_of(f)), second_name(System::name_of(s)) {}
     size_t operator()(SparseState& state) const;
+};
+
+// ------------------------------------------------------------ debug checks
+// CheckNormalization / CheckNan (SparQ/src/debugger.cpp:13-70): one device
+// reduction; they throw on a bad state.  Like the reference's release build
+// (QRAM_Release, PySparQ/CMakeLists.txt:8) they are no-ops unless
+// debug_checks() is set.
+bool& debug_checks();
+struct CheckNormalization : SelfAdjointOperator {
+    double threshold = 1e-5;
+    CheckNormalization() = default;
+    explicit CheckNormalization(double t) : threshold(t) {}
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+struct CheckNan : SelfAdjointOperator {
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+
+// ------------------------------------------------------------ post-selection
+// PartialTraceSelect / PartialTraceSelectRange (SparQ/include/partial_trace.h:83-145,
+// SparQ/src/partial_trace.cpp:148-225): keep the entries whose registers hold
+// the selected values (or lie in [first, second]), renormalise, and return the
+// factor 1/sqrt(kept probability) (+inf if nothing is kept, 0 for an empty state
+// or an empty register list).
+struct PartialTraceSelect {
+    std::vector<size_t> partial_trace_registers;
+    std::vector<uint64_t> select_values;
+    explicit PartialTraceSelect(const std::map<std::string, uint64_t>& sel);
+    explicit PartialTraceSelect(const std::map<size_t, uint64_t>& sel);
+    PartialTraceSelect(const std::vector<size_t>& regs, const std::vector<uint64_t>& values);
+    PartialTraceSelect(const std::vector<std::string>& regs, const std::vector<uint64_t>& values);
+    double operator()(SparseState& state) const;
+    double operator()(std::vector<System>& state) const;
+};
+struct PartialTraceSelectRange {
+    size_t partial_trace_register;
+    std::pair<size_t, size_t> select_range;
+    PartialTraceSelectRange(std::string_view reg, std::pair<size_t, size_t> range)
+        : partial_trace_register(System::get(reg)), select_range(range) {}
+    PartialTraceSelectRange(size_t reg, std::pair<size_t, size_t> range) : partial_trace_register(reg), select_range(range) {}
+    double operator()(SparseState& state) const;
+    double operator()(std::vector<System>& state) const;
+};
+
+// ------------------------------------------------------------ CKS quantum walk primitives
+// (SparQ_Algorithm/include/hamiltonian_simulation.h, SparQ_Algorithm/src/hamiltonian_simulation.cpp)
+
+// Compact QRAM storage of a sparse matrix with nnz_col slots per row: slot s of
+// row i holds its quantised element at memory[i*nnz_col + s] and its column at
+// memory[get_sparsity_offset() + i*nnz_col + s]; columns ascend within a row so
+// the binary search of SparseMatrixOracle2 finds them.  get_data() is that
+// memory, zero-padded to a power of two (the QRAM address space).  The layout is
+// the one TridiagonalSpec::combined_mem reproduces
+// (SparQ_Algorithm/include/hamiltonian_simulation_tridiagonal.h:169-190); the
+// constructor from real values quantises x to floor(x*2^ds) (positive_only) or
+// the ds-bit two's complement of trunc(x*2^(ds-1)) (signed), clipped like
+// SparseMatrix.from_dense (PySparQ/pysparq/algorithms/cks_solver.py:288-349).
+struct SparseMatrix {
+    std::vector<uint64_t> elements;
+    std::vector<size_t> sparsity;
+    size_t data_size = 0, nnz_col = 0, n_row = 0;
+    bool positive_only = true;
+    SparseMatrix() = default;
+    SparseMatrix(const std::vector<double>& values, const std::vector<size_t>& sparsity, size_t data_size, size_t nnz_col,
+                 size_t n_row, bool positive_only);
+    SparseMatrix(const std::vector<uint64_t>& encoded, const std::vector<size_t>& sparsity, size_t data_size, size_t nnz_col,
+                 size_t n_row, bool positive_only = true);
+    std::vector<uint64_t> get_data() const;
+    size_t get_sparsity_offset() const { return elements.size(); }
+};
+
+// row_offset ^= offset + row_sz*row (hamiltonian_simulation.cpp:222-237)
+struct GetRowAddr : PointwiseOperator {
+    GetRowAddr(std::string_view offset, std::string_view row, size_t row_sz, std::string_view row_offset)
+        : GetRowAddr(System::get(offset), System::get(row), row_sz, System::get(row_offset)) {}
+    GetRowAddr(size_t offset, size_t row, size_t row_sz, size_t row_offset);
+};
+// data_offset ^= offset + row_sz*row + col_sparse (hamiltonian_simulation.cpp:239-255)
+struct GetDataAddr : PointwiseOperator {
+    GetDataAddr(std::string_view offset, std::string_view row, std::string_view col_sparse, size_t row_sz,
+                std::string_view data_offset)
+        : GetDataAddr(System::get(offset), System::get(row), System::get(col_sparse), row_sz, System::get(data_offset)) {}
+    GetDataAddr(size_t offset, size_t row, size_t col_sparse, size_t row_sz, size_t data_offset);
+};
+// result ^= address of target in memory[offset, offset+total_length), 0 when absent
+// (hamiltonian_simulation.cpp:182-220; max_step = floor(log2(total_length)) + 1)
+struct QuantumBinarySearch_Fast : PointwiseOperator {
+    size_t total_length, max_step;
+    QuantumBinarySearch_Fast(const QRAMCircuit_qutrit* q, std::string_view offset, size_t total_length, std::string_view target,
+                             std::string_view result)
+        : QuantumBinarySearch_Fast(q, System::get(offset), total_length, System::get(target), System::get(result)) {}
+    QuantumBinarySearch_Fast(const QRAMCircuit_qutrit* q, size_t offset, size_t total_length, size_t target, size_t result);
+};
+// out ^= rational(acos(sqrt(|a|/Amax)) / 2pi) of the quantised element in `data`
+// (hamiltonian_simulation.cpp:257-290); row and col are not read
+struct GetQWRotateAngle_Int_Int_Int : PointwiseOperator {
+    GetQWRotateAngle_Int_Int_Int(std::string_view data, std::string_view row, std::string_view col, std::string_view out,
+                                 const SparseMatrix* mat)
+        : GetQWRotateAngle_Int_Int_Int(System::get(data), System::get(row), System::get(col), System::get(out), mat) {}
+    GetQWRotateAngle_Int_Int_Int(size_t data, size_t row, size_t col, size_t out, const SparseMatrix* mat);
+};
+
+// The tridiagonal walk's direct oracles (hamiltonian_simulation_tridiagonal.h:104-396,
+// SparQ_Algorithm/src/hamiltonian_tridiagonal.cpp:16-105).
+struct TridiagonalSpec {
+    size_t n_row = 0, data_size = 0;
+    uint64_t alpha_enc = 0, beta_enc = 0;
+    size_t sparse_offset = 0;
+    uint64_t addr_mask = 0;
+    size_t k_width = 0;
+    static constexpr size_t slot_count = 4;
+    size_t window_base(size_t j) const;
+    size_t col_of(size_t j, size_t s) const { return window_base(j) + s; }
+    uint64_t value_of(size_t j, size_t s) const;
+    uint64_t combined_mem(uint64_t idx) const;
+};
+std::pair<SparseMatrix, TridiagonalSpec> make_tridiagonal_matrix_spec(double alpha, double beta, size_t n, size_t ds);
+// out ^= M(offset + 4*row + slot)
+struct TridiagOracle1 : PointwiseOperator {
+    TridiagOracle1(std::string_view offset, std::string_view row, std::string_view slot, std::string_view out,
+                   const TridiagonalSpec& spec)
+        : TridiagOracle1(System::get(offset), System::get(row), System::get(slot), System::get(out), spec) {}
+    TridiagOracle1(size_t offset, size_t row, size_t slot, size_t out, const TridiagonalSpec& spec);
+};
+// forward: column -> slot; dag: slot -> column (garbage branches tagged in sr)
+struct TridiagOracle2 : PointwiseOperator {
+    TridiagOracle2(std::string_view offset, std::string_view row, std::string_view k, std::string_view sr,
+                   const TridiagonalSpec& spec)
+        : TridiagOracle2(System::get(offset), System::get(row), System::get(k), System::get(sr), spec) {}
+    TridiagOracle2(size_t offset, size_t row, size_t k, size_t sr, const TridiagonalSpec& spec);
+};
+// amplitude negated where bit 0 of the register is 1
+struct TridiagParityPhase : PointwiseOperator {
+    explicit TridiagParityPhase(std::string_view reg) : TridiagParityPhase(System::get(reg)) {}
+    explicit TridiagParityPhase(size_t reg);
 };
 
 // add_systems (system_operations.h:157-181)

@@ -251,12 +251,13 @@ def thomas_solution(alpha: float, beta: float, n: int) -> np.ndarray:
     return x / np.linalg.norm(x)
 
 
-def solution_metrics(out: np.ndarray, n: int, alpha: float, beta: float, xref: np.ndarray) -> dict:
-    """res_dir, res_opt, success probability and fidelity of the solution block out[n:2n)
-    of a GetOutput vector, as the production bench computes them
-    (PoiseuilleQDABench.cu:194-229): y = A x/|x|, res_dir = |y/|y| - b|, res_opt^2 =
-    |y|^2 - <y, b>^2, fid = |<xref, x/|x|>|^2."""
-    blk = out[n:2 * n]
+def block_metrics(blk: np.ndarray, alpha: float, beta: float, xref: np.ndarray) -> dict:
+    """res_dir, res_opt, block weight and fidelity of an unnormalised solution vector
+    blk of tridiag(beta, alpha, beta) x = ones/sqrt(n), as the production benches
+    compute them (PoiseuilleQDABench.cu:194-229, CKSPoiseuilleBench.cu:95-131):
+    y = A x/|x|, res_dir = sqrt(2 - 2<y, b>/|y|), res_opt^2 = |y|^2 - |<y, b>|^2,
+    fid = |<xref, x/|x|>|^2."""
+    n = blk.shape[0]
     succ = float(np.sum(np.abs(blk) ** 2))
     nrm = math.sqrt(succ)
     if nrm <= 1e-300:
@@ -275,6 +276,11 @@ def solution_metrics(out: np.ndarray, n: int, alpha: float, beta: float, xref: n
     fr = float(np.dot(xref, x.real))
     fi = float(np.dot(xref, x.imag))
     return {"res_dir": res_dir, "res_opt": res_opt, "success_prob": succ, "fidelity": fr * fr + fi * fi}
+
+
+def solution_metrics(out: np.ndarray, n: int, alpha: float, beta: float, xref: np.ndarray) -> dict:
+    """block_metrics of the solution block out[n:2n) of a GetOutput vector."""
+    return block_metrics(out[n:2 * n], alpha, beta, xref)
 
 
 def qda_schedule(nqubit: int, step_rate: float, alpha: float, beta: float):

@@ -5,7 +5,8 @@ production benchmark (Experiments/QDA/QDA_Poiseuille/PoiseuilleQDABench.cu).
 
 Solves tridiag(beta, alpha, beta) x = ones/sqrt(n) with the discrete-adiabatic
 walk: steps = ceil(steprate * 2305 * kappa) (kappa analytic unless given), capped
-by --max-steps and made even; each step is W(s = i/steps), ClearZero(cz_eps),
+by --max-steps and made even (--stop-after N runs only its first N steps); each
+step is W(s = i/steps), ClearZero(cz_eps),
 Normalize; at every log point the post-selected output gives res_dir, res_opt,
 success probability and fidelity against the Thomas solution, exactly as the
 reference bench computes them.  Prints one line per log point and a JSON
@@ -22,7 +23,8 @@ from . import qda
 
 
 def solve(ps, nqubit=20, steprate=1.1, p=1.3, alpha=2.2666666666564934, beta=-1.0, kappa=0.0, log_every=250,
-          tol=1e-6, max_steps=10 ** 9, cz_eps=1e-14, ancilla_order=("anc_4", "anc_3", "anc_2", "anc_1"), log=print):
+          tol=1e-6, max_steps=10 ** 9, cz_eps=1e-14, ancilla_order=("anc_4", "anc_3", "anc_2", "anc_1"), log=print,
+          stop_after=None):
     n = 1 << nqubit
     kappa_analytic = qda.analytic_kappa(alpha, beta, n)
     kappa = kappa if kappa > 0 else kappa_analytic
@@ -42,12 +44,12 @@ def solve(ps, nqubit=20, steprate=1.1, p=1.3, alpha=2.2666666666564934, beta=-1.
         st = qda.setup_poiseuille(ps, nqubit, ancilla_order)
         clear = ps.ClearZero(cz_eps)
         norm = ps.Normalize()
-        for i in range(steps):
+        for i in range(steps if stop_after is None else min(steps, stop_after)):
             qda.WalkSTridiagonal(ps, "main_reg", "anc_UA", "anc_1", "anc_2", "anc_3", "anc_4", i / steps, kappa, p,
                                  alpha, beta)(st)
             clear(st)
             norm(st)
-            if (i + 1) % log_every == 0 or i == steps - 1:
+            if (i + 1) % log_every == 0 or i == steps - 1 or (stop_after is not None and i + 1 == stop_after):
                 size = st.size()
                 peak = max(peak, size)
                 te = time.perf_counter()
@@ -85,6 +87,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-steps", type=int, default=10 ** 9)
     ap.add_argument("--cz-eps", type=float, default=1e-14)
+    ap.add_argument("--stop-after", type=int, default=None, help="run only the first N steps of the schedule")
     ap.add_argument("--learned", action="store_true", help="size branching operators with the learned sizer")
     ap.add_argument("--json", default=None, help="write the summary (with the curve) to this file")
     args = ap.parse_args()
@@ -94,7 +97,7 @@ def main():
     if args.learned and not sizer.activate_default():
         raise SystemExit("no sizer weights found")
     res = solve(ps, args.nqubit, args.steprate, args.p, args.alpha, args.beta, args.kappa, args.log_every, args.tol,
-                args.max_steps, args.cz_eps, log=lambda s: print(s, flush=True))
+                args.max_steps, args.cz_eps, log=lambda s: print(s, flush=True), stop_after=args.stop_after)
     if args.learned:
         res["sizer"] = sizer.stats()
     summary = {k: v for k, v in res.items() if k != "curve"}
