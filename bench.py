@@ -169,6 +169,27 @@ def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
     }
 
 
+def cks_leg(ps, nqubit, terms, R):
+    """The 2^nqubit Poiseuille CKS walk (direct tridiagonal oracles, fitted
+    coefficients J = 640): LCU terms timed for one right-hand side and for R in
+    one state (entries from the walk itself; inputs built on the device)."""
+    from sparqsim_amd import cks_solve
+
+    out = {"workload": f"CKS Chebyshev walk, tridiag(-1, 2.2666666666564934, -1) at 2^{nqubit}, kappa 16, "
+                       f"ds 32, J 640, {terms} LCU terms (2 walk steps each) after FirstStep",
+           "reference_published": "2^20 on A800: ~4.2 s per LCU term, 20,971,512 entries, 2681.9 s to converge "
+                                  "(docs/verification/v1e-cks-direct-oracle-linsolver.md:136-145)"}
+    for r in (1, R):
+        res = cks_solve.solve(ps, nqubit=nqubit, R=r, terms=terms, log_every=10 ** 9, log=lambda s: None)
+        entries = res["entries_after_first_step"]
+        out[f"rhs{r}"] = {"s_per_term": res["s_per_term"], "s_per_rhs_term": res["s_per_term"] / r,
+                          "entries": entries, "walk_entry_steps_per_s": entries * 2 / res["s_per_term"]}
+        ps.System.clear()
+        ps.release_cached_memory()
+    out["batched_speedup_per_rhs"] = out["rhs1"]["s_per_term"] / out[f"rhs{R}"]["s_per_rhs_term"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +208,9 @@ def main():
     ap.add_argument("--grover-bits", type=int, default=14)
     ap.add_argument("--grover-iterations", type=int, default=40)
     ap.add_argument("--grover-plain-traj", type=int, default=16, help="trajectories timed one state at a time")
+    ap.add_argument("--cks-qubits", type=int, default=20)
+    ap.add_argument("--cks-terms", type=int, default=6, help="LCU terms timed in the CKS leg (0 skips it)")
+    ap.add_argument("--cks-rhs", type=int, default=16, help="right-hand sides of the batched CKS run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 512 MiB device copies (the reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -279,6 +303,8 @@ def main():
             "reference_published": "2^20, 40-step, czEps 1e-16: res_dir 5.178906e-3, fid 0.999999 "
                                    "(docs/verification/v1d-qda-linsolver-a800.md:123)",
         }
+    if args.cks_terms > 0:
+        result["cks"] = cks_leg(ps, args.cks_qubits, args.cks_terms, args.cks_rhs)
     if args.grover_traj > 0:
         result["noisy_grover"] = grover_leg(ps, args.grover_traj, args.grover_bits, args.grover_iterations, 1e-3,
                                             min(args.grover_plain_traj, args.grover_traj))

@@ -176,6 +176,11 @@ int sparq_state_download(sparq_state_t st, uint64_t* keys, double* amps, uint64_
  * Bits of the new key not covered by any destination field are zero. */
 int sparq_state_relayout(sparq_state_t st, uint32_t new_words,
                          const sparq_field_t* from, const sparq_field_t* to, uint32_t nfields);
+/* Change the key width without moving fields: words below min(old, new) keep
+ * their content, added words read as zero, removed words are dropped (the caller
+ * guarantees they hold no active register).  Growing back to a width the state
+ * held before reuses its allocation. */
+int sparq_state_resize_words(sparq_state_t st, uint32_t new_words);
 int sparq_state_sync(sparq_state_t st);
 /* Raw device pointers of the current arrays (valid until the next call that changes the size). */
 int sparq_state_device_arrays(sparq_state_t st, uint64_t** keys, double** amps, uint64_t* capacity);

@@ -305,6 +305,10 @@ int sparq_state_relayout(sparq_state_t h, uint32_t new_words, const sparq_field_
     });
 }
 
+int sparq_state_resize_words(sparq_state_t h, uint32_t new_words) {
+    return guarded([&] { resize_words(S(h), new_words); });
+}
+
 int sparq_state_sync(sparq_state_t h) {
     return guarded([&] { SPQ_HIP(hipStreamSynchronize(S(h).stream)); });
 }

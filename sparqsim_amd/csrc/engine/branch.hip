@@ -45,23 +45,31 @@ __global__ void __launch_bounds__(kBlock) group_start_kernel(const uint32_t* __r
 // Class of each group (grid covers an upper bound of the group count; the
 // actual count is read from device memory).  counts[1] receives the number of
 // WHT groups so the butterfly passes can be skipped when there are none.
+// Grid-stride over the groups; the WHT-group count is reduced per block and
+// added with one atomic per block.
 __global__ void __launch_bounds__(kBlock) classify_kernel(const uint64_t* __restrict__ start, const uint64_t* __restrict__ G_dev, uint64_t F,
                                                      uint32_t nd, int full, uint8_t* __restrict__ cls,
                                                      unsigned long long* __restrict__ n_wht) {
+    __shared__ uint32_t lds[kBlock / 64];
     const uint64_t G = G_dev[0];
-    uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool wht = false;
-    if (g < G) {
-        uint64_t cnt = start[g + 1] - start[g];
+    uint32_t wht = 0;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < G; g += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t cnt = start[g + 1] - start[g];
         uint8_t c;
         if (full == 2) c = kDense;                                         // rot.cpp:395-455
         else if (!full) c = (cnt == 1) ? kDirect : kWht;                   // hadamard.cpp:26-131
         else c = (cnt == F) ? kWht : (cnt + 1 <= nd ? kSparseSum : kWht);  // hadamard.cpp:220-256 (few_threshold = n-1)
         cls[g] = c;
-        wht = c == kWht;
+        wht += c == kWht;
     }
-    uint64_t b = __ballot(wht);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(n_wht, (unsigned long long)__popcll(b));
+    wht = wave_sum(wht);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = wht;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
+        if (t) atomicAdd(n_wht, (unsigned long long)t);
+    }
 }
 
 constexpr int kExpandItems = 4;   // outputs per thread (strided by the block size)
@@ -421,7 +429,8 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
         SPQ_HIP(hipMemsetAsync(pos + n + 1, 0, sizeof(uint64_t), st.stream));
         {
             SPQ_PROF(st.stream, "classify_kernel", n * 17.0);
-            classify_kernel<<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(start, pos + n, F, nd, dense ? 2 : (full ? 1 : 0), cls,
+            classify_kernel<<<std::min<unsigned>(grid_for(n, kBlock * 4), 2048), kBlock, 0, st.stream>>>(start, pos + n, F, nd,
+                                                                                                     dense ? 2 : (full ? 1 : 0), cls,
                                                                           (unsigned long long*)(pos + n + 1));
         }
         SPQ_HIP(hipGetLastError());

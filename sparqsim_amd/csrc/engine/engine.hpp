@@ -70,6 +70,7 @@ constexpr int kWave = 64;
 
 struct State {
     uint32_t W = 1;
+    uint32_t Wcap = 0;          // key words the allocation holds when more than W (0: exactly W)
     uint64_t n = 0, cap = 0;
     uint64_t* keys = nullptr;   // [W][cap]
     double2* amps = nullptr;    // [cap]
@@ -92,6 +93,9 @@ template <class T> T* dalloc(size_t count, hipStream_t s) { return static_cast<T
 void alloc_arrays(State& st, uint32_t W, uint64_t cap, uint64_t** keys, double2** amps);
 // Install new arrays as the state's content, freeing the old ones.
 void install_arrays(State& st, uint32_t W, uint64_t* keys, double2* amps, uint64_t n, uint64_t cap);
+// Change the key width in place: words below min(old, new) keep their content,
+// added words read as zero, removed words are dropped.
+void resize_words(State& st, uint32_t new_words);
 // Read `count` uint64 values from device memory (synchronizes the stream).
 void read_back(State& st, const void* dptr, size_t bytes, void* host);
 uint64_t read_u64(State& st, const uint64_t* dptr);

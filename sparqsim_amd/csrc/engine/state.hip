@@ -145,8 +145,28 @@ void install_arrays(State& st, uint32_t W, uint64_t* keys, double2* amps, uint64
     st.keys = keys;
     st.amps = amps;
     st.W = W;
+    st.Wcap = 0;
     st.n = n;
     st.cap = cap;
+}
+
+void resize_words(State& st, uint32_t new_words) {
+    SPQ_REQUIRE(new_words >= 1 && new_words <= SPARQ_MAX_WORDS, "key width out of range");
+    const uint32_t have = std::max(st.W, st.Wcap);
+    if (new_words <= have) {   // the allocation already holds the words: no data motion
+        if (new_words > st.W)
+            SPQ_HIP(hipMemsetAsync(st.keys + (uint64_t)st.W * st.cap, 0, (size_t)(new_words - st.W) * st.cap * sizeof(uint64_t), st.stream));
+        st.Wcap = have;
+        st.W = new_words;
+        return;
+    }
+    uint64_t* k = dalloc<uint64_t>((size_t)new_words * st.cap, st.stream);
+    SPQ_HIP(hipMemcpyAsync(k, st.keys, (size_t)st.W * st.cap * sizeof(uint64_t), hipMemcpyDeviceToDevice, st.stream));
+    SPQ_HIP(hipMemsetAsync(k + (uint64_t)st.W * st.cap, 0, (size_t)(new_words - st.W) * st.cap * sizeof(uint64_t), st.stream));
+    dfree(st.keys, st.stream);
+    st.keys = k;
+    st.W = new_words;
+    st.Wcap = 0;
 }
 
 void read_back(State& st, const void* dptr, size_t bytes, void* host) {
@@ -292,6 +312,7 @@ void relayout(State& st, uint32_t new_words, const sparq_field_t* from, const sp
     dfree(st.keys, st.stream);
     st.keys = kout;
     st.W = new_words;
+    st.Wcap = 0;
 }
 
 // ------------------------------------------------------------------ basis state
@@ -440,25 +461,37 @@ void scale_amps(State& st, double s) {
     SPQ_HIP(hipGetLastError());
 }
 
-template <int W>
-__global__ void __launch_bounds__(kBlock) field_minmax_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, sparq_field_t f,
+// Min and max of one field over all entries: reads only the field's key word
+// (8 bytes per entry), reduces in registers and across the waves of a block,
+// then one atomic pair per block.
+__global__ void __launch_bounds__(kBlock) field_minmax_kernel(const uint64_t* __restrict__ word, uint64_t n, sparq_field_t f,
                                                          unsigned long long* mm) {
-    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint64_t k[W];
-    load_key<W>(keys, cap, i, k);
-    unsigned long long v = fget<W>(k, f);
-    atomicMin(&mm[0], v);
-    atomicMax(&mm[1], v);
-}
-
-template <int W>
-static void launch_minmax(State& st, sparq_field_t f, unsigned long long* mm) {
-    {
-        SPQ_PROF(st.stream, "field_minmax_kernel", st.n * 8.0 * W);
-        field_minmax_kernel<W><<<grid_for(st.n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, st.n, f, mm);
+    __shared__ unsigned long long lmin[kBlock / 64], lmax[kBlock / 64];
+    const unsigned long long mask = fmask(f.width);
+    unsigned long long vmin = ~0ull, vmax = 0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        const unsigned long long v = (word[i] >> f.shift) & mask;
+        vmin = v < vmin ? v : vmin;
+        vmax = v > vmax ? v : vmax;
     }
-    SPQ_HIP(hipGetLastError());
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long a = __shfl_xor(vmin, off, 64), b = __shfl_xor(vmax, off, 64);
+        vmin = a < vmin ? a : vmin;
+        vmax = b > vmax ? b : vmax;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        lmin[threadIdx.x >> 6] = vmin;
+        lmax[threadIdx.x >> 6] = vmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) {
+            vmin = lmin[w] < vmin ? lmin[w] : vmin;
+            vmax = lmax[w] > vmax ? lmax[w] : vmax;
+        }
+        atomicMin(&mm[0], vmin);
+        atomicMax(&mm[1], vmax);
+    }
 }
 
 bool field_uniform(State& st, sparq_field_t f, uint64_t* value) {
@@ -469,7 +502,12 @@ bool field_uniform(State& st, sparq_field_t f, uint64_t* value) {
     unsigned long long* mm = dalloc<unsigned long long>(2, st.stream);
     unsigned long long init[2] = {~0ull, 0ull};
     SPQ_HIP(hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, st.stream));
-    SPQ_DISPATCH_W(st.W, launch_minmax, st, f, mm);
+    {
+        SPQ_PROF(st.stream, "field_minmax_kernel", st.n * 8.0);
+        const unsigned g = std::min<unsigned>(grid_for(st.n, kBlock * 4), 2048);
+        field_minmax_kernel<<<g, kBlock, 0, st.stream>>>(st.keys + (uint64_t)f.word * st.cap, st.n, f, mm);
+    }
+    SPQ_HIP(hipGetLastError());
     unsigned long long h[2];
     read_back(st, mm, sizeof(h), h);
     dfree(mm, st.stream);

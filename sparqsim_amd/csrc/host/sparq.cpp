@@ -174,6 +174,8 @@ SparseState& SparseState::operator=(const SparseState& other) {
     std::swap(layout_, tmp.layout_);
     pend_ops_.clear();
     pend_conds_.clear();
+    zeroed_.clear();
+    layout_version_ = ~0ull;
     return *this;
 }
 SparseState::SparseState(SparseState&& other) noexcept
@@ -185,6 +187,8 @@ SparseState& SparseState::operator=(SparseState&& other) noexcept {
     std::swap(layout_, other.layout_);
     std::swap(pend_ops_, other.pend_ops_);
     std::swap(pend_conds_, other.pend_conds_);
+    std::swap(zeroed_, other.zeroed_);
+    std::swap(layout_version_, other.layout_version_);
     return *this;
 }
 SparseState::~SparseState() {
@@ -210,10 +214,13 @@ const Layout& SparseState::sync() {
     if (layout_version_ == System::layout_version) return layout_;
     Layout T = Layout::from_registry();
     layout_version_ = System::layout_version;
-    if (T == layout_) return layout_;
+    if (T == layout_) {
+        zeroed_.clear();
+        return layout_;
+    }
     flush();
-    std::vector<sparq_field_t> from, to;
-    bool identity = T.words == layout_.words;
+    std::vector<sparq_field_t> from, to, clear;
+    bool in_place = true;   // every surviving field keeps its word and shift
     for (size_t id = 0; id < layout_.field.size(); ++id) {
         sparq_field_t a = layout_.field[id];
         if (a.width == 0) continue;
@@ -221,12 +228,28 @@ const Layout& SparseState::sync() {
             sparq_field_t b = T.field[id];
             from.push_back(a);
             to.push_back(b);
-            if (a.word != b.word || a.shift != b.shift) identity = false;
-        } else {
-            identity = false;   // a dropped field must be cleared
+            if (a.word != b.word || a.shift != b.shift) in_place = false;
+        } else if (a.word < T.words && std::find(zeroed_.begin(), zeroed_.end(), id) == zeroed_.end()) {
+            clear.push_back(a);   // a dropped field in a surviving word must read as zero
         }
     }
-    if (!identity) check(sparq_state_relayout(h_, T.words, from.data(), to.data(), (uint32_t)from.size()));
+    zeroed_.clear();
+    if (in_place) {
+        // Fields stay put: clear dropped non-zero fields in place (one fused
+        // pointwise pass over their words) and change the word count without
+        // moving keys.  New fields land on zero bits (gaps, cleared fields or
+        // added words).
+        for (size_t i = 0; i < clear.size(); ++i) {
+            sparq_pw_op_t op{};
+            op.opcode = SPARQ_PW_SET_FIELD;
+            op.f[0] = clear[i];
+            push_pointwise(op, {});
+        }
+        flush();
+        if (T.words != layout_.words) check(sparq_state_resize_words(h_, T.words));
+    } else {
+        check(sparq_state_relayout(h_, T.words, from.data(), to.data(), (uint32_t)from.size()));
+    }
     layout_ = T;
     return layout_;
 }
@@ -1006,8 +1029,14 @@ size_t AddRegisterWithHadamard::operator()(SparseState& state) const {   // syst
     return id;
 }
 void RemoveRegister::operator()(SparseState& state) const {   // system_operations.cpp:303-322
-    TestRemovable{register_id}(state);
+    const Layout& L = state.sync();
     if (register_id >= System::name_register_map.size()) throw std::runtime_error("Register not found.");
+    state.flush();
+    int uni = 1;
+    uint64_t v = 0;
+    check(sparq_field_uniform(state.handle(), L.at(register_id), &uni, &v));
+    if (!uni) throw std::runtime_error("Removing failed: register " + System::name_of(register_id) + " is not uniform");
+    if (v == 0) state.mark_zero(register_id);   // its bits need no clearing when the field is dropped
     System::remove_register(register_id);
     state.sync();
 }

@@ -155,6 +155,10 @@ public:
     // Replace the layout after an explicit field move (Split/Combine).
     void relayout_with(const Layout& target, const std::vector<std::pair<sparq_field_t, sparq_field_t>>& moves);
 
+    // Record that register `id` is zero in every entry (checked by the caller);
+    // the next layout change then drops its field without clearing it.
+    void mark_zero(size_t id) { zeroed_.push_back(id); }
+
     // Queue a per-entry op; consecutive ops run fused in one kernel.
     void push_pointwise(const sparq_pw_op_t& op, const std::vector<sparq_cond_t>& conds);
     void flush();
@@ -179,6 +183,7 @@ private:
     uint64_t layout_version_ = ~0ull;   // System::layout_version the layout was checked against
     std::vector<sparq_pw_op_t> pend_ops_;
     std::vector<sparq_cond_t> pend_conds_;
+    std::vector<size_t> zeroed_;   // registers known to be zero in every entry (see mark_zero)
 };
 
 // ------------------------------------------------------------ conditions
