@@ -169,7 +169,7 @@ def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
     }
 
 
-def cks_leg(ps, nqubit, terms, R):
+def cks_leg(ps, nqubit, terms, R, band_terms=0):
     """The 2^nqubit Poiseuille CKS walk (direct tridiagonal oracles, fitted
     coefficients J = 640): LCU terms timed for one right-hand side and for R in
     one state (entries from the walk itself; inputs built on the device)."""
@@ -187,6 +187,20 @@ def cks_leg(ps, nqubit, terms, R):
         ps.System.clear()
         ps.release_cached_memory()
     out["batched_speedup_per_rhs"] = out["rhs1"]["s_per_term"] / out[f"rhs{R}"]["s_per_rhs_term"]
+    if band_terms > 0:
+        t0 = time.perf_counter()
+        res = cks_solve.solve(ps, nqubit=nqubit, R=R, kappa=50.0, J=1024, terms=band_terms, log_every=10 ** 9,
+                              log=lambda s: None, matrix="band-random")
+        entries = res["entries_after_first_step"]
+        out[f"band_random_rhs{R}"] = {
+            "workload": f"seeded band-plus-random symmetric positive matrix at 2^{nqubit} (diagonal, +-1 band, one "
+                        f"random partner per row; kappa 50), QRAM-backed sparse-access oracles (binary search), "
+                        f"{R} right-hand sides in one state, {band_terms} LCU terms",
+            "s_per_term": res["s_per_term"], "s_per_rhs_term": res["s_per_term"] / R, "entries": entries,
+            "walk_entry_steps_per_s": entries * 2 / res["s_per_term"], "setup_s": res["setup_s"],
+            "wall_s": time.perf_counter() - t0}
+        ps.System.clear()
+        ps.release_cached_memory()
     return out
 
 
@@ -211,6 +225,8 @@ def main():
     ap.add_argument("--cks-qubits", type=int, default=20)
     ap.add_argument("--cks-terms", type=int, default=6, help="LCU terms timed in the CKS leg (0 skips it)")
     ap.add_argument("--cks-rhs", type=int, default=16, help="right-hand sides of the batched CKS run")
+    ap.add_argument("--cks-band-terms", type=int, default=3,
+                    help="LCU terms timed on the band-plus-random matrix with the batched right-hand sides (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 512 MiB device copies (the reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -304,7 +320,7 @@ def main():
                                    "(docs/verification/v1d-qda-linsolver-a800.md:123)",
         }
     if args.cks_terms > 0:
-        result["cks"] = cks_leg(ps, args.cks_qubits, args.cks_terms, args.cks_rhs)
+        result["cks"] = cks_leg(ps, args.cks_qubits, args.cks_terms, args.cks_rhs, args.cks_band_terms)
     if args.grover_traj > 0:
         result["noisy_grover"] = grover_leg(ps, args.grover_traj, args.grover_bits, args.grover_iterations, 1e-3,
                                             min(args.grover_plain_traj, args.grover_traj))

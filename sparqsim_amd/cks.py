@@ -463,17 +463,21 @@ def select_dense(mod, state, zero_regs, index_regs, length: int):
     return out, float(np.sum(np.abs(amps[sel]) ** 2))
 
 
-def rhs_metrics(m: np.ndarray, b: np.ndarray, alpha: float, beta: float, xref: np.ndarray) -> dict:
+def rhs_metrics(m: np.ndarray, b: np.ndarray, alpha: float, beta: float, xref: np.ndarray, matvec=None) -> dict:
     """block metrics against a general normalised real right-hand side b:
     y = A x/|x|, res_dir = sqrt(2 - 2 Re<b, y>/|y|), res_opt^2 = |y|^2 - |<b, y>|^2,
-    fid = |<xref, x/|x|>|^2 (the formulas of CKSPoiseuilleBench.cu:95-131 with b general)."""
+    fid = |<xref, x/|x|>|^2 (the formulas of CKSPoiseuilleBench.cu:95-131 with b general).
+    A is tridiag(beta, alpha, beta) unless a matvec is given."""
     w = float(np.sum(np.abs(m) ** 2))
     if w <= 1e-300:
         return {"res_dir": float("nan"), "res_opt": float("nan"), "weight": w, "fidelity": float("nan")}
     x = m / math.sqrt(w)
-    y = alpha * x
-    y[:-1] += beta * x[1:]
-    y[1:] += beta * x[:-1]
+    if matvec is not None:
+        y = matvec(x)
+    else:
+        y = alpha * x
+        y[:-1] += beta * x[1:]
+        y[1:] += beta * x[:-1]
     y2 = float(np.sum(np.abs(y) ** 2))
     proj = complex(np.dot(b, y))
     res_dir = math.sqrt(max(2.0 - 2.0 * proj.real / math.sqrt(y2), 0.0))
@@ -562,6 +566,142 @@ class BatchedPoiseuilleCKS:
         res = []
         for r in range(self.R):
             met = rhs_metrics(blocks[r], self.rhs[r], self.alpha, self.beta, self.xref[r])
+            met["p_succ"] = met["weight"] * self.R / (self.lcu.a * self.lcu.a)
+            res.append(met)
+        return blocks, res
+
+
+# ---------------------------------------------------------------- banded-plus-random sparse matrices (QRAM path)
+def band_random_matrix(n: int, kappa: float = 50.0, seed: int = 0x5EED2026 + 4, nnz_col: int = 4):
+    """Symmetric positive matrix with 4 slots per row: the diagonal, the +-1 band
+    and one random symmetric partner (a seeded perfect matching avoiding the band),
+    off-diagonal values uniform in [0.5, 1), the diagonal shifted so that
+    lambda_max / lambda_min = kappa, scaled to a largest entry of 1.  Rows 0 and
+    n-1 lack one band neighbour and carry a zero-valued slot on an unused column.
+    Entries are positive because the walk's rotation is sign-blind (it encodes
+    |A|).  Returns (values[n*4], sparsity[n*4], scipy CSR of the matrix)."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import eigsh
+
+    if n < 8 or n & (n - 1) or nnz_col != 4:
+        raise ValueError("need a power-of-two dimension >= 8 and 4 slots per row")
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    a, b = perm[0::2].copy(), perm[1::2].copy()
+    for _ in range(1000):   # re-pair the few partners that fall on the band or the diagonal
+        bad = np.nonzero(np.abs(a - b) <= 1)[0]
+        if bad.size == 0:
+            break
+        for i in bad:
+            j = int(rng.integers(0, a.size))
+            b[i], b[j] = b[j], b[i]
+    else:
+        raise RuntimeError("could not draw a band-avoiding matching")
+    partner = np.empty(n, dtype=np.int64)
+    partner[a], partner[b] = b, a
+    band = rng.uniform(0.5, 1.0, n - 1)            # (i, i+1)
+    pair = rng.uniform(0.5, 1.0, n // 2)           # (a_k, b_k)
+    pval = np.empty(n)
+    pval[a], pval[b] = pair, pair
+    rows = np.concatenate([np.arange(n - 1), np.arange(1, n), np.arange(n)])
+    cols = np.concatenate([np.arange(1, n), np.arange(n - 1), partner])
+    vals = np.concatenate([band, band, pval])
+    B = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    v0 = np.random.default_rng(seed + 1).standard_normal(n)   # fixed start: the same matrix on every call
+    lmax = float(eigsh(B, k=1, which="LA", return_eigenvectors=False, tol=1e-7, v0=v0)[0])
+    lmin = float(eigsh(B, k=1, which="SA", return_eigenvectors=False, tol=1e-7, v0=v0)[0])
+    mu = (lmax - kappa * lmin) / (kappa - 1.0)
+    A = (B + mu * sp.identity(n, format="csr")).tocsr()
+    scale = max(mu, float(vals.max()))
+    A = A / scale
+    A.eig_bounds = ((lmin + mu) / scale, (lmax + mu) / scale)   # Lanczos estimates (from inside)
+    A.sort_indices()
+    counts = np.diff(A.indptr)
+    cols = np.zeros((n, 4), dtype=np.int64)
+    vals4 = np.zeros((n, 4))
+    full = np.nonzero(counts == 4)[0]
+    take = A.indptr[full][:, None] + np.arange(4)
+    cols[full], vals4[full] = A.indices[take], A.data[take]
+    for i in np.nonzero(counts != 4)[0]:            # the two boundary rows
+        cs = list(A.indices[A.indptr[i]:A.indptr[i + 1]])
+        vs = list(A.data[A.indptr[i]:A.indptr[i + 1]])
+        c = 0
+        while len(cs) < 4:                         # zero-valued padding on an unused column
+            if c not in cs:
+                cs.append(c)
+                vs.append(0.0)
+            c += 1
+        order = np.argsort(cs)
+        cols[i], vals4[i] = np.asarray(cs)[order], np.asarray(vs)[order]
+    return vals4.reshape(-1).tolist(), cols.reshape(-1).tolist(), A
+
+
+def walk_spectrum(A, ds: int, nnz_col: int = 4, widen: float = 1e-5):
+    """Enclosing interval of the walk block's eigenvalues, v_jk / (nnz_col (2^ds - 1))
+    with v = floor(A_jk 2^ds) the quantised elements: the extreme eigenvalues of A
+    (Lanczos estimates, which lie inside the spectrum) scaled by 2^ds / (nnz_col
+    (2^ds - 1)), widened by `widen` relative and by the quantisation bound
+    2^-ds absolute."""
+    lo, hi = A.eig_bounds
+    c = float(2 ** ds) / (nnz_col * float(2 ** ds - 1))
+    q = 2.0 ** -ds
+    return lo * c * (1 - widen) - q, hi * c * (1 + widen) + q
+
+
+def _spd_solve(A, b):
+    """Classical solution of the symmetric positive definite system A x = b (CG to 1e-13)."""
+    from scipy.sparse.linalg import cg
+
+    x, info = cg(A, b, rtol=1e-13, atol=0.0, maxiter=20000)
+    if info != 0:
+        raise RuntimeError("CG did not converge")
+    return x
+
+
+class BatchedSparseCKS(BatchedPoiseuilleCKS):
+    """The QRAM-backed CKS walk (QuantumWalkNSteps with SparseMatrixOracle1/2,
+    hamiltonian_simulation.h:983-1773) for a band_random_matrix over R right-hand
+    sides in one state; fitted 1/x coefficients on the walk spectrum."""
+
+    def __init__(self, mod, nqubit: int, rhs: np.ndarray, kappa: float = 50.0, ds: int = 32, J: int = 1024,
+                 seed: int = 0x5EED2026 + 4, eps: float = 0.01):
+        self.mod, self.dphase = mod, False
+        self.alpha = self.beta = None
+        self.n = 1 << nqubit
+        rhs = np.asarray(rhs, dtype=np.float64)
+        if rhs.ndim != 2 or rhs.shape[1] != self.n:
+            raise ValueError("rhs must be an R x 2^nqubit array")
+        self.R = rhs.shape[0]
+        self.rhs = rhs / np.linalg.norm(rhs, axis=1, keepdims=True)
+        self.rhs_bits = max(1, int(math.ceil(math.log2(self.R))))
+        values, sparsity, self.A = band_random_matrix(self.n, kappa, seed)
+        mod.System.clear()
+        mod.System.add_register("rhs", mod.UnsignedInteger, self.rhs_bits)
+        self.mat = mod.SparseMatrix(values, sparsity, ds, 4, self.n, True)
+        self.spec = None
+        self.lcu = LCU_Container_NoiseFree(mod, self.mat, kappa, eps, None)
+        self.m_spec, self.M_spec = walk_spectrum(self.A, ds)
+        self.lcu.set_inverse_coefficients(chebyshev_inverse_coefficients(self.m_spec, self.M_spec, J)[0])
+        w = self.lcu.quantum_walk_obj
+        mod.PartialTraceSelect({w.sparse_offset: w.offset})(self.lcu.current_state)   # empty accumulator
+        self._xref = None
+
+    @property
+    def xref(self):
+        """Classical solutions (computed on first use)."""
+        if self._xref is None:
+            self._xref = np.stack([x / np.linalg.norm(x) for x in (_spd_solve(self.A, b) for b in self.rhs)])
+        return self._xref
+
+    def extract(self):
+        w = self.lcu.quantum_walk_obj
+        Rp = 1 << self.rhs_bits
+        out, _ = select_dense(self.mod, self.lcu.current_state, [w.b1, w.k, w.b2, w.j_comp, w.k_comp, w.data_offset],
+                              ["rhs", w.j], Rp * self.n)
+        blocks = out.reshape(self.n, Rp).T[: self.R].copy()
+        res = []
+        for r in range(self.R):
+            met = rhs_metrics(blocks[r], self.rhs[r], 0.0, 0.0, self.xref[r], matvec=lambda x: self.A @ x)
             met["p_succ"] = met["weight"] * self.R / (self.lcu.a * self.lcu.a)
             res.append(met)
         return blocks, res

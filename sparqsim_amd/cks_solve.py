@@ -6,7 +6,9 @@ right-hand sides in one state.
 
 Solves tridiag(beta, alpha, beta) x_r = b_r for R right-hand sides (r = 0 is
 ones/sqrt(n), the others seeded N(0,1) vectors) with the Chebyshev quantum walk
-and direct tridiagonal oracles: input D|b_r> (D = diag((-1)^j)), FirstStep, then
+and direct tridiagonal oracles (--matrix band-random: a seeded symmetric
+positive matrix with the diagonal, the +-1 band and one random partner per row,
+condition number --kappa, through the QRAM-backed sparse-access oracles): input D|b_r> (D = diag((-1)^j)), FirstStep, then
 one LCU term per Step (two walk steps each); at every log point the dense
 post-selection gives, per right-hand side, res_dir, res_opt, fidelity against the
 classical tridiagonal solve and p_succ.  Coefficients: the fitted 1/x family on
@@ -31,11 +33,16 @@ def make_rhs(n: int, R: int, seed: int = 0x5EED2026) -> np.ndarray:
 
 
 def solve(ps, nqubit=20, R=1, kappa=16.0, eps=0.01, ds=32, coef="true", J=640, terms=None, log_every=64, tol=1e-6,
-          alpha=2.2666666666564934, beta=-1.0, log=print):
+          alpha=2.2666666666564934, beta=-1.0, log=print, matrix="poiseuille"):
     n = 1 << nqubit
     t_setup = time.perf_counter()
-    batch = cks.BatchedPoiseuilleCKS(ps, nqubit, make_rhs(n, R), kappa=kappa, eps=eps, ds=ds, alpha=alpha, beta=beta,
-                                     coef=coef, J=J)
+    if matrix == "poiseuille":
+        batch = cks.BatchedPoiseuilleCKS(ps, nqubit, make_rhs(n, R), kappa=kappa, eps=eps, ds=ds, alpha=alpha, beta=beta,
+                                         coef=coef, J=J)
+    elif matrix == "band-random":
+        batch = cks.BatchedSparseCKS(ps, nqubit, make_rhs(n, R), kappa=kappa, ds=ds, J=J, eps=eps)
+    else:
+        raise ValueError("matrix must be 'poiseuille' or 'band-random'")
     batch.prepare_input()
     after_first = batch.lcu.step_state.size()
     t_setup = time.perf_counter() - t_setup
@@ -63,7 +70,7 @@ def solve(ps, nqubit=20, R=1, kappa=16.0, eps=0.01, ds=32, coef="true", J=640, t
                 break
     wall = time.perf_counter() - t0
     last = curve[-1]
-    return {"n": n, "nqubit": nqubit, "rhs": R, "kappa": kappa, "eps": eps, "ds": ds, "coef": coef,
+    return {"n": n, "nqubit": nqubit, "rhs": R, "matrix": matrix, "kappa": kappa, "eps": eps, "ds": ds, "coef": coef,
             "J": J if coef == "true" else None, "lcu_terms_budget": batch.lcu.j0 + 1, "terms_run": j,
             "entries_after_first_step": after_first, "termination": termination, "wall_s": wall, "walk_s": wall - t_extract,
             "s_per_term": (wall - t_extract) / max(j, 1), "setup_s": t_setup,
@@ -84,11 +91,14 @@ def main():
     ap.add_argument("--log-every", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--json", default=None, help="write the summary (with the curve) to this file")
+    ap.add_argument("--matrix", choices=["poiseuille", "band-random"], default="poiseuille",
+                    help="tridiagonal Poiseuille case (direct oracles) or a seeded band-plus-random matrix "
+                         "(QRAM-backed sparse-access oracles, fitted coefficients; --kappa sets its condition number)")
     args = ap.parse_args()
     import sparqsim_amd as ps
 
     res = solve(ps, args.nqubit, args.rhs, args.kappa, args.eps, args.ds, args.coef, args.J, args.terms, args.log_every,
-                args.tol, log=lambda s: print(s, flush=True))
+                args.tol, log=lambda s: print(s, flush=True), matrix=args.matrix)
     summary = {k: v for k, v in res.items() if k not in ("curve", "final")}
     summary["final_worst_res_dir"] = max(r["res_dir"] for r in res["final"])
     summary["final_rhs0"] = res["final"][0]

@@ -388,3 +388,64 @@ def test_engine_2p20_state_after_first_step(ps):
     ps.ClearZero()(st)
     w.FirstStep(st)
     assert st.size() == 20971512
+
+
+# ---------------------------------------------------------------- band-plus-random matrix, QRAM-backed walk
+def test_band_random_matrix_and_walk_spectrum():
+    v, s, A = cks.band_random_matrix(64, 50.0)
+    D = A.toarray()
+    ev = np.linalg.eigvalsh(D)
+    assert np.allclose(D, D.T) and D.min() >= 0 and D.max() == pytest.approx(1.0)
+    assert ev[-1] / ev[0] == pytest.approx(50.0, rel=1e-6)
+    cols = np.array(s).reshape(64, 4)
+    assert all(len(set(r)) == 4 and list(r) == sorted(r) for r in cols)          # ascending, distinct slots
+    assert all(np.count_nonzero(D[i]) == (3 if i in (0, 63) else 4) for i in range(64))
+    # the walk block v_jk / (4 (2^32 - 1)) of the quantised matrix lies inside the fitted interval
+    q = np.minimum(np.floor(np.array(v) * 2.0 ** 32), 2.0 ** 32 - 1).reshape(64, 4)
+    W = np.zeros((64, 64))
+    for i in range(64):
+        W[i, cols[i]] = q[i] / (4 * (2.0 ** 32 - 1))
+    wev = np.linalg.eigvalsh(W)
+    lo, hi = cks.walk_spectrum(A, 32)
+    assert lo <= wev[0] and wev[-1] <= hi and hi / lo < 50.01
+
+
+def test_oracle_band_random_solve_converges(oracle):
+    rng = np.random.default_rng(8)
+    b = cks.BatchedSparseCKS(oracle, 5, np.stack([np.ones(32), rng.standard_normal(32)]), kappa=50.0, J=1024)
+    b.prepare_input()
+    while b.Step():
+        pass
+    _, res = b.extract()
+    for r in res:
+        assert r["fidelity"] >= 1 - 1e-9 and r["res_dir"] <= 1e-4
+
+
+@pytest.mark.gpu
+def test_engine_band_random_walk_matches_oracle(ps, oracle):
+    rng = np.random.default_rng(9)
+    rhs = np.stack([np.ones(32)] + [rng.standard_normal(32) for _ in range(2)])
+    out = []
+    for mod in (ps, oracle):
+        b = cks.BatchedSparseCKS(mod, 5, rhs, kappa=50.0, J=1024)
+        b.prepare_input()
+        first = b.lcu.step_state.export()
+        for _ in range(20):
+            b.Step()
+        out.append((first, b.extract(), b.lcu.current_state.export()))
+    assert_same_state(out[0][0], out[1][0], what="after FirstStep")
+    assert_same_state(out[0][2], out[1][2], what="accumulated state after 20 terms")
+    for r in range(3):
+        assert np.max(np.abs(out[0][1][0][r] - out[1][1][0][r])) <= 1e-12 * np.max(np.abs(out[1][1][0][r]))
+
+
+@pytest.mark.gpu
+def test_engine_band_random_solve_converges(ps):
+    rng = np.random.default_rng(10)
+    b = cks.BatchedSparseCKS(ps, 6, np.stack([np.ones(64), rng.standard_normal(64)]), kappa=50.0, J=1024)
+    b.prepare_input()
+    while b.Step():
+        pass
+    _, res = b.extract()
+    for r in res:
+        assert r["fidelity"] >= 1 - 1e-9 and r["res_dir"] <= 1e-4
