@@ -604,8 +604,11 @@ struct Hadamard_Bool {
 };
 
 // CondRot_Rational_Bool (condrot.cpp:12-151).  Not controllable.
+// Conditions (an extension: the reference class is not controllable) restrict the
+// rotation to the entries that satisfy them, as for Hadamard_Bool.
 struct CondRot_Rational_Bool {
     size_t in_id, out_id;
+    Conditions cond;
     CondRot_Rational_Bool(size_t in, size_t out) : in_id(in), out_id(out) {}
     void apply(State& st, double sign) const {
         if (st.empty()) return;
@@ -614,7 +617,7 @@ struct CondRot_Rational_Bool {
             st[z].amplitude = a0 * c - a1 * s;
             st[o].amplitude = a0 * s + a1 * c;
         };
-        pair_scan(st, out_id, nullptr,
+        pair_scan(st, out_id, &cond,
             [&](size_t z, size_t o) {
                 double rot = st[z].as_double(in_id) * (sign * 2 * pi);
                 mapping(z, o, std::cos(rot), std::sin(rot));

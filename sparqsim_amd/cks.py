@@ -23,6 +23,15 @@ import math
 import numpy as np
 
 
+def _ctl(op, cond):
+    """AND the conditions of a controlled composite (qda.Cond or None) into one operator."""
+    if cond is None:
+        return op
+    from .qda import ctl
+
+    return ctl(op, cond)
+
+
 # ---------------------------------------------------------------- Chebyshev coefficients
 class ChebyshevPolynomialCoefficient:
     """Coefficients of the LCU sum over walk powers W^(2j+1)
@@ -96,12 +105,12 @@ class SparseMatrixOracle1:
         self.reg_offset, self.reg_row, self.reg_col_id, self.reg_output = reg_offset, reg_row, reg_col_id, reg_output
         self.row_size = row_size
 
-    def __call__(self, st):
-        m = self.mod
+    def __call__(self, st, cond=None):
+        m, c = self.mod, (lambda op: _ctl(op, cond))
         m.AddRegister("data_addr", m.UnsignedInteger, self.qram.address_size)(st)
-        m.GetDataAddr(self.reg_offset, self.reg_row, self.reg_col_id, self.row_size, "data_addr")(st)
-        m.QRAMLoad(self.qram, "data_addr", self.reg_output)(st)
-        m.GetDataAddr(self.reg_offset, self.reg_row, self.reg_col_id, self.row_size, "data_addr")(st)
+        c(m.GetDataAddr(self.reg_offset, self.reg_row, self.reg_col_id, self.row_size, "data_addr"))(st)
+        c(m.QRAMLoad(self.qram, "data_addr", self.reg_output))(st)
+        c(m.GetDataAddr(self.reg_offset, self.reg_row, self.reg_col_id, self.row_size, "data_addr"))(st)
         m.RemoveRegister("data_addr")(st)
 
     dag = __call__
@@ -116,26 +125,26 @@ class SparseMatrixOracle2:
         self.reg_sparse_offset, self.reg_row, self.reg_col, self.reg_sr = reg_sparse_offset, reg_row, reg_col, reg_search_result
         self.row_size = row_size
 
-    def __call__(self, st):
-        m, q = self.mod, self.qram
+    def __call__(self, st, cond=None):
+        m, q, c = self.mod, self.qram, (lambda op: _ctl(op, cond))
         m.AddRegister("row_addr", m.UnsignedInteger, q.address_size)(st)
-        m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr")(st)
-        m.QuantumBinarySearch_Fast(q, "row_addr", self.row_size, self.reg_col, self.reg_sr)(st)
-        m.QRAMLoad(q, self.reg_sr, self.reg_col)(st)
-        m.Swap_General_General(self.reg_col, self.reg_sr)(st)
-        m.Add_AnyInt_AnyInt_InPlace(self.reg_col, "row_addr").dag(st)
-        m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr")(st)
+        c(m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr"))(st)
+        c(m.QuantumBinarySearch_Fast(q, "row_addr", self.row_size, self.reg_col, self.reg_sr))(st)
+        c(m.QRAMLoad(q, self.reg_sr, self.reg_col))(st)
+        c(m.Swap_General_General(self.reg_col, self.reg_sr))(st)
+        c(m.Add_AnyInt_AnyInt_InPlace(self.reg_col, "row_addr")).dag(st)
+        c(m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr"))(st)
         m.RemoveRegister("row_addr")(st)
 
-    def dag(self, st):
-        m, q = self.mod, self.qram
+    def dag(self, st, cond=None):
+        m, q, c = self.mod, self.qram, (lambda op: _ctl(op, cond))
         m.AddRegister("row_addr", m.UnsignedInteger, q.address_size)(st)
-        m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr")(st)
-        m.Add_AnyInt_AnyInt_InPlace(self.reg_col, "row_addr")(st)
-        m.Swap_General_General(self.reg_col, self.reg_sr)(st)
-        m.QRAMLoad(q, self.reg_sr, self.reg_col)(st)
-        m.QuantumBinarySearch_Fast(q, "row_addr", self.row_size, self.reg_col, self.reg_sr)(st)
-        m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr")(st)
+        c(m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr"))(st)
+        c(m.Add_AnyInt_AnyInt_InPlace(self.reg_col, "row_addr"))(st)
+        c(m.Swap_General_General(self.reg_col, self.reg_sr))(st)
+        c(m.QRAMLoad(q, self.reg_sr, self.reg_col))(st)
+        c(m.QuantumBinarySearch_Fast(q, "row_addr", self.row_size, self.reg_col, self.reg_sr))(st)
+        c(m.GetRowAddr(self.reg_sparse_offset, self.reg_row, self.row_size, "row_addr"))(st)
         m.RemoveRegister("row_addr")(st)
 
 
@@ -152,60 +161,65 @@ class T:
         self.nnz_col, self.data_size = nnz_col, data_size
         self.k_digits = int(math.log2(nnz_col))
 
-    # element load and slot <-> column conversion, QRAM or direct
-    def _o1(self, st):
+    # element load and slot <-> column conversion, QRAM or direct; `cond` (used when T
+    # runs inside a controlled block encoding) is AND-ed into every operator
+    def _o1(self, st, cond):
         if self.spec is not None:
-            self.mod.TridiagOracle1(self.reg_data_offset, self.reg_j, self.reg_k, "data", self.spec)(st)
+            _ctl(self.mod.TridiagOracle1(self.reg_data_offset, self.reg_j, self.reg_k, "data", self.spec), cond)(st)
         else:
-            SparseMatrixOracle1(self.mod, self.qram, self.reg_data_offset, self.reg_j, self.reg_k, "data", self.nnz_col)(st)
+            SparseMatrixOracle1(self.mod, self.qram, self.reg_data_offset, self.reg_j, self.reg_k, "data", self.nnz_col)(st, cond)
 
-    def _o2(self, st, dagger: bool):
+    def _o2(self, st, dagger: bool, cond):
         if self.spec is not None:
-            op = self.mod.TridiagOracle2(self.reg_sparse_offset, self.reg_j, self.reg_k, self.reg_sr, self.spec)
-        else:
-            op = SparseMatrixOracle2(self.mod, self.qram, self.reg_sparse_offset, self.reg_j, self.reg_k, self.reg_sr, self.nnz_col)
+            op = _ctl(self.mod.TridiagOracle2(self.reg_sparse_offset, self.reg_j, self.reg_k, self.reg_sr, self.spec), cond)
+            if dagger:
+                op.dag(st)
+            else:
+                op(st)
+            return
+        op = SparseMatrixOracle2(self.mod, self.qram, self.reg_sparse_offset, self.reg_j, self.reg_k, self.reg_sr, self.nnz_col)
         if dagger:
-            op.dag(st)
+            op.dag(st, cond)
         else:
-            op(st)
+            op(st, cond)
 
-    def _angle(self, st):
-        self.mod.GetQWRotateAngle_Int_Int_Int("data", self.reg_j, self.reg_k, "rotation", self.mat)(st)
+    def _angle(self, st, cond):
+        _ctl(self.mod.GetQWRotateAngle_Int_Int_Int("data", self.reg_j, self.reg_k, "rotation", self.mat), cond)(st)
 
-    def __call__(self, st):
+    def __call__(self, st, cond=None):
         m = self.mod
         m.AddRegister("data", m.UnsignedInteger, self.data_size)(st)
-        m.Hadamard_Int(self.reg_k, self.k_digits)(st)
-        self._o1(st)
-        self._o2(st, True)
+        _ctl(m.Hadamard_Int(self.reg_k, self.k_digits), cond)(st)
+        self._o1(st, cond)
+        self._o2(st, True, cond)
         m.AddRegister("rotation", m.Rational, 32)(st)
-        self._angle(st)
-        m.CondRot_Fixed_Bool("rotation", self.reg_b2)(st)
-        self._angle(st)
+        self._angle(st, cond)
+        _ctl(m.CondRot_Fixed_Bool("rotation", self.reg_b2), cond)(st)
+        self._angle(st, cond)
         m.RemoveRegister("rotation")(st)
-        self._o2(st, False)
-        self._o1(st)
+        self._o2(st, False, cond)
+        self._o1(st, cond)
         m.RemoveRegister("data")(st)
-        self._o2(st, True)
+        self._o2(st, True, cond)
         m.CheckNan()(st)
 
-    def dag(self, st):
+    def dag(self, st, cond=None):
         m = self.mod
         m.AddRegister("data", m.UnsignedInteger, self.data_size)(st)
-        self._o2(st, False)
+        self._o2(st, False, cond)
         m.CheckNan()(st)
-        self._o1(st)
-        self._o2(st, False)
+        self._o1(st, cond)
+        self._o2(st, False, cond)
         m.AddRegister("rotation", m.Rational, 32)(st)
-        self._angle(st)
-        m.CondRot_Fixed_Bool("rotation", self.reg_b2).dag(st)
-        self._angle(st)
+        self._angle(st, cond)
+        _ctl(m.CondRot_Fixed_Bool("rotation", self.reg_b2), cond).dag(st)
+        self._angle(st, cond)
         m.RemoveRegister("rotation")(st)
         m.ClearZero()(st)
-        self._o2(st, True)
+        self._o2(st, True, cond)
         m.CheckNormalization()(st)
-        self._o1(st)
-        m.Hadamard_Int(self.reg_k, self.k_digits)(st)
+        self._o1(st, cond)
+        _ctl(m.Hadamard_Int(self.reg_k, self.k_digits), cond)(st)
         m.ClearZero()(st)
         m.RemoveRegister("data")(st)
 
@@ -573,39 +587,53 @@ class BatchedPoiseuilleCKS:
 
 # ---------------------------------------------------------------- banded-plus-random sparse matrices (QRAM path)
 def band_random_matrix(n: int, kappa: float = 50.0, seed: int = 0x5EED2026 + 4, nnz_col: int = 4):
-    """Symmetric positive matrix with 4 slots per row: the diagonal, the +-1 band
-    and one random symmetric partner (a seeded perfect matching avoiding the band),
-    off-diagonal values uniform in [0.5, 1), the diagonal shifted so that
-    lambda_max / lambda_min = kappa, scaled to a largest entry of 1.  Rows 0 and
-    n-1 lack one band neighbour and carry a zero-valued slot on an unused column.
-    Entries are positive because the walk's rotation is sign-blind (it encodes
-    |A|).  Returns (values[n*4], sparsity[n*4], scipy CSR of the matrix)."""
+    """Symmetric positive matrix with nnz_col slots per row (4 or 8): the diagonal,
+    the +-1 band and nnz_col - 3 random symmetric partners (seeded perfect
+    matchings avoiding the band and each other), off-diagonal values uniform in
+    [0.5, 1), the diagonal shifted so that lambda_max / lambda_min = kappa, scaled
+    to a largest entry of 1.  Rows 0 and n-1 lack one band neighbour and carry a
+    zero-valued slot on an unused column.  Entries are positive because the walk's
+    rotation is sign-blind (it encodes |A|).  Returns (values[n*nnz_col],
+    sparsity[n*nnz_col], scipy CSR of the matrix with .eig_bounds)."""
     import scipy.sparse as sp
     from scipy.sparse.linalg import eigsh
 
-    if n < 8 or n & (n - 1) or nnz_col != 4:
-        raise ValueError("need a power-of-two dimension >= 8 and 4 slots per row")
+    if n < 4 * nnz_col or n & (n - 1) or nnz_col not in (4, 8):
+        raise ValueError("need a power-of-two dimension >= 4*nnz_col and 4 or 8 slots per row")
     rng = np.random.default_rng(seed)
-    perm = rng.permutation(n)
-    a, b = perm[0::2].copy(), perm[1::2].copy()
-    for _ in range(1000):   # re-pair the few partners that fall on the band or the diagonal
-        bad = np.nonzero(np.abs(a - b) <= 1)[0]
-        if bad.size == 0:
-            break
-        for i in bad:
-            j = int(rng.integers(0, a.size))
-            b[i], b[j] = b[j], b[i]
-    else:
-        raise RuntimeError("could not draw a band-avoiding matching")
-    partner = np.empty(n, dtype=np.int64)
-    partner[a], partner[b] = b, a
+    taken = set()
+    rows_l, cols_l, vals_l = [], [], []
     band = rng.uniform(0.5, 1.0, n - 1)            # (i, i+1)
-    pair = rng.uniform(0.5, 1.0, n // 2)           # (a_k, b_k)
-    pval = np.empty(n)
-    pval[a], pval[b] = pair, pair
-    rows = np.concatenate([np.arange(n - 1), np.arange(1, n), np.arange(n)])
-    cols = np.concatenate([np.arange(1, n), np.arange(n - 1), partner])
-    vals = np.concatenate([band, band, pval])
+    rows_l += [np.arange(n - 1), np.arange(1, n)]
+    cols_l += [np.arange(1, n), np.arange(n - 1)]
+    vals_l += [band, band]
+    for _ in range(nnz_col - 3):
+        for _attempt in range(100):
+            perm = rng.permutation(n)
+            a, b = perm[0::2].copy(), perm[1::2].copy()
+            for _ in range(1000):   # re-pair partners on the band, the diagonal or an earlier matching
+                lo, hi = np.minimum(a, b), np.maximum(a, b)
+                bad = np.nonzero((hi - lo <= 1) | np.isin(lo * n + hi, list(taken)))[0] if taken else \
+                    np.nonzero(hi - lo <= 1)[0]
+                if bad.size == 0:
+                    break
+                for i in bad:
+                    j = int(rng.integers(0, a.size))
+                    b[i], b[j] = b[j], b[i]
+            else:
+                continue
+            break
+        else:
+            raise RuntimeError("could not draw a band-avoiding matching")
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        taken.update((lo * n + hi).tolist())
+        pair = rng.uniform(0.5, 1.0, n // 2)
+        rows_l += [a, b]
+        cols_l += [b, a]
+        vals_l += [pair, pair]
+    rows = np.concatenate(rows_l)
+    cols = np.concatenate(cols_l)
+    vals = np.concatenate(vals_l)
     B = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
     v0 = np.random.default_rng(seed + 1).standard_normal(n)   # fixed start: the same matrix on every call
     lmax = float(eigsh(B, k=1, which="LA", return_eigenvectors=False, tol=1e-7, v0=v0)[0])
@@ -617,23 +645,23 @@ def band_random_matrix(n: int, kappa: float = 50.0, seed: int = 0x5EED2026 + 4, 
     A.eig_bounds = ((lmin + mu) / scale, (lmax + mu) / scale)   # Lanczos estimates (from inside)
     A.sort_indices()
     counts = np.diff(A.indptr)
-    cols = np.zeros((n, 4), dtype=np.int64)
-    vals4 = np.zeros((n, 4))
-    full = np.nonzero(counts == 4)[0]
-    take = A.indptr[full][:, None] + np.arange(4)
-    cols[full], vals4[full] = A.indices[take], A.data[take]
-    for i in np.nonzero(counts != 4)[0]:            # the two boundary rows
+    cols = np.zeros((n, nnz_col), dtype=np.int64)
+    valsq = np.zeros((n, nnz_col))
+    full = np.nonzero(counts == nnz_col)[0]
+    take = A.indptr[full][:, None] + np.arange(nnz_col)
+    cols[full], valsq[full] = A.indices[take], A.data[take]
+    for i in np.nonzero(counts != nnz_col)[0]:      # the two boundary rows
         cs = list(A.indices[A.indptr[i]:A.indptr[i + 1]])
         vs = list(A.data[A.indptr[i]:A.indptr[i + 1]])
         c = 0
-        while len(cs) < 4:                         # zero-valued padding on an unused column
+        while len(cs) < nnz_col:                   # zero-valued padding on an unused column
             if c not in cs:
                 cs.append(c)
                 vs.append(0.0)
             c += 1
         order = np.argsort(cs)
-        cols[i], vals4[i] = np.asarray(cs)[order], np.asarray(vs)[order]
-    return vals4.reshape(-1).tolist(), cols.reshape(-1).tolist(), A
+        cols[i], valsq[i] = np.asarray(cs)[order], np.asarray(vs)[order]
+    return valsq.reshape(-1).tolist(), cols.reshape(-1).tolist(), A
 
 
 def walk_spectrum(A, ds: int, nnz_col: int = 4, widen: float = 1e-5):

@@ -251,8 +251,9 @@ PYBIND11_MODULE(_core, m) {
     auto hb = py::class_<Hadamard_Bool, SelfAdjointOperator, std::shared_ptr<Hadamard_Bool>>(m, "Hadamard_Bool")
         .def(py::init<std::string_view>(), py::arg("reg_in")).def(py::init<size_t>(), py::arg("reg_in"));
     bind_controllable(hb);
-    py::class_<CondRot_Rational_Bool, BaseOperator, std::shared_ptr<CondRot_Rational_Bool>>(m, "CondRot_Rational_Bool")
+    auto crb = py::class_<CondRot_Rational_Bool, BaseOperator, std::shared_ptr<CondRot_Rational_Bool>>(m, "CondRot_Rational_Bool")
         .def(py::init<std::string_view, std::string_view>()).def(py::init<size_t, size_t>());
+    bind_controllable(crb);
     py::class_<CondRot_Fixed_Bool, CondRot_Rational_Bool, std::shared_ptr<CondRot_Fixed_Bool>>(m, "CondRot_Fixed_Bool")
         .def(py::init<std::string_view, std::string_view>()).def(py::init<size_t, size_t>());
     py::class_<CondRot_General_Bool, BaseOperator, std::shared_ptr<CondRot_General_Bool>>(m, "CondRot_General_Bool")

@@ -813,7 +813,10 @@ void CondRot_Rational_Bool::apply(SparseState& state, double sign) const {   // 
     p.kind = SPARQ_PAIR_CONDROT_RATIONAL;
     p.in = L.at(register_in);
     p.mat[0] = sign;
-    check(sparq_pair_op(state.handle(), L.at(register_out), 0, &p, nullptr, 0, clearzero_default_eps()));
+    // Conditions are an extension (the reference class is not controllable): the
+    // rotation acts on the entries that satisfy them, the others pass through.
+    std::vector<sparq_cond_t> conds = conditions(L);
+    check(sparq_pair_op(state.handle(), L.at(register_out), 0, &p, conds.data(), (uint32_t)conds.size(), clearzero_default_eps()));
 }
 
 u22_t make_func(uint64_t value, size_t n) {   // condrot.h:19-30

@@ -429,10 +429,11 @@ struct Hadamard_Bool : SelfAdjointOperator, Controllable {
     SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
-struct CondRot_Rational_Bool : BaseOperator {
+struct CondRot_Rational_Bool : BaseOperator, Controllable {
     size_t register_in, register_out;
     CondRot_Rational_Bool(std::string_view in, std::string_view out) : CondRot_Rational_Bool(System::get(in), System::get(out)) {}
     CondRot_Rational_Bool(size_t in, size_t out);
+    Controllable* controls() override { return this; }
     SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override { apply(state, 1.0); }
     void dag(SparseState& state) const override { apply(state, -1.0); }

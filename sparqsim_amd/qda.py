@@ -87,8 +87,8 @@ class BlockEncodingTridiagonal:
 class BlockEncodingHs:
     """Block encoding of H(s) = (1-f)H0 + f H1 (qda_fundamental.h:53-162)."""
 
-    def __init__(self, mod, enc_A, main_reg, anc_UA, anc_1, anc_2, anc_3, anc_4, fs):
-        self.mod, self.enc_A = mod, enc_A
+    def __init__(self, mod, enc_A, main_reg, anc_UA, anc_1, anc_2, anc_3, anc_4, fs, enc_b=None):
+        self.mod, self.enc_A, self.enc_b = mod, enc_A, enc_b
         self.main_reg, self.anc_UA = main_reg, anc_UA
         self.a1, self.a2, self.a3, self.a4 = anc_1, anc_2, anc_3, anc_4
         sqrt_n = 1.0 / math.sqrt((1 - fs) * (1 - fs) + fs * fs)
@@ -97,7 +97,8 @@ class BlockEncodingHs:
     def apply(self, st):
         m = self.mod
         a1, a2, a3, a4, main = self.a1, self.a2, self.a3, self.a4, self.main_reg
-        enc_b = m.Hadamard_Int_Full(main)      # self-adjoint state preparation of b = ones/sqrt(n)
+        # self-adjoint state preparation of b = ones/sqrt(n)
+        enc_b = self.enc_b if self.enc_b is not None else m.Hadamard_Int_Full(main)
         m.Hadamard_Bool(a3)(st)
         enc_b(st)                              # enc_b.dag
         m.X_Bool(a1, 0)(st)
