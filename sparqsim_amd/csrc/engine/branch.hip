@@ -74,6 +74,16 @@ __global__ void __launch_bounds__(kBlock) classify_kernel(const uint64_t* __rest
 
 constexpr int kExpandItems = 4;   // outputs per thread (strided by the block size)
 
+// Largest output (bytes of keys + amplitudes) a learned capacity may allocate.
+static double learned_budget_bytes() {
+    static double budget = [] {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || total_b == 0) return 4.0e9;
+        return (double)total_b / 8.0;
+    }();
+    return budget;
+}
+
 // Output entry o = g*F + j: key of the group head with the low target bits = j.
 // DIRECT and SPARSE_SUM amplitudes are produced here; WHT blocks start at 0.
 // The group count G is read from the device; the output holds G_cap groups
@@ -450,7 +460,11 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
             const double lg = std::min(40.0, (double)ks.prediction * 32.0 + sizer_margin());
             const double g = std::ceil(std::exp2(std::max(0.0, lg)));
             G_cap = (uint64_t)std::min<double>((double)bound, g);
-            learned = G_cap >= 1 && G_cap <= (~0ull >> nd) && (G_cap << nd) <= (uint64_t(1) << 33);
+            // A predicted output beyond the learned-allocation budget (1/8 of the
+            // device memory) is not trusted: a misprediction there would fail the
+            // allocation instead of costing a rerun, so count exactly.
+            learned = G_cap >= 1 && G_cap <= (~0ull >> nd) && (G_cap << nd) <= (uint64_t(1) << 33) &&
+                      (double)(G_cap << nd) * (8.0 * st.W + 16.0) <= learned_budget_bytes();
         }
         if (!learned) {
             uint64_t h2[2];

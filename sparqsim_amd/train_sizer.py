@@ -5,7 +5,8 @@
 Runs a mix of workloads with sizer logging on -- QDA walk steps on the
 Poiseuille problem (Hadamard_Int_Full on the main register, dense state
 preparation on the ancillas), batched noisy Grover trajectories
-(Hadamard_Int_Full per trajectory segment) and seeded random states under
+(Hadamard_Int_Full per trajectory segment), the QDA over the sparse block
+encoding and the CKS walks, and seeded random states under
 Hadamard_Int / Hadamard_Int_Full / Rot_GeneralUnitary -- then fits the network
 with the engine's GPU kernels, picks the safety margin on the training split,
 reports the held-out split, and saves parameters + margin (npz).  Needs a GPU.
@@ -78,6 +79,31 @@ def qda_steps(ps, sizes, steps):
         ps.set_clearzero_default_eps(old)
 
 
+def sparse_qda_steps(ps, sizes, steps, starts=(0.0, 0.5, 0.9)):
+    """Steps of the QDA over the sparse block encoding (qda_sparse) from fresh
+    states at several points of the schedule."""
+    from . import qda_sparse
+
+    for nq in sizes:
+        for s0 in starts:
+            st, walk, enc, A, kappa = qda_sparse.setup(ps, nq)
+            for i in range(steps):
+                qda_sparse.WalkSSparse(ps, enc, "anc_1", "anc_2", "anc_3", "anc_4", s0 + i / 2000, kappa, 1.3)(st)
+                ps.ClearZero()(st)
+                ps.Normalize()(st)
+            st.size()
+
+
+def cks_steps(ps, sizes, terms):
+    """LCU terms of the CKS walk: the tridiagonal walk and the QRAM-backed walk."""
+    from . import cks_solve
+
+    for nq in sizes:
+        cks_solve.solve(ps, nqubit=nq, R=2, terms=terms, log_every=10 ** 9, log=lambda s: None)
+        cks_solve.solve(ps, nqubit=max(nq, 5), R=2, kappa=50.0, J=64, terms=terms, log_every=10 ** 9,
+                        log=lambda s: None, matrix="band-random")
+
+
 def grover_runs(ps, configs):
     from .grover import NoisyGrover
 
@@ -85,12 +111,17 @@ def grover_runs(ps, configs):
         NoisyGrover(ps, traj, bits, bits, 2, 0.05).run_batched().size()
 
 
-def collect(ps, seed: int = 0, trials: int = 600, qda_sizes=(4, 6, 8, 10, 12, 14), qda_step_count: int = 6):
+def collect(ps, seed: int = 0, trials: int = 600, qda_sizes=(4, 6, 8, 10, 12, 14), qda_step_count: int = 6,
+            sparse_qda_sizes=(), cks_sizes=()):
     rng = np.random.default_rng(BASE_SEED + seed)
     with sizer.logging():
         qda_steps(ps, qda_sizes, qda_step_count)
         grover_runs(ps, ((1, 6), (4, 8), (16, 8), (8, 10), (32, 6), (64, 9)))
         random_structured_states(ps, rng, trials)
+        if sparse_qda_sizes:
+            sparse_qda_steps(ps, sparse_qda_sizes, 4)
+        if cks_sizes:
+            cks_steps(ps, cks_sizes, 3)
     return sizer.fetch_logs()
 
 
@@ -135,7 +166,8 @@ def main():
     import sparqsim_amd as ps
 
     t0 = time.time()
-    x, y = collect(ps, seed=0, trials=args.trials, qda_sizes=(4, 6, 8, 10, 12, 14, 16, 18, 20), qda_step_count=8)
+    x, y = collect(ps, seed=0, trials=args.trials, qda_sizes=(4, 6, 8, 10, 12, 14, 16, 18, 20), qda_step_count=8,
+                   sparse_qda_sizes=(5, 6, 8, 10, 12), cks_sizes=(4, 6, 8, 10, 12))
     t_collect = time.time() - t0
     # train / validation (picks the margin) / test (reported) = 70 / 15 / 15
     order = np.random.default_rng(7).permutation(len(y))
