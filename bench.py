@@ -169,6 +169,34 @@ def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
     }
 
 
+def qda_sparse_leg(ps, nqubit, steps):
+    """QDA over the sparse block encoding (CKS oracles, 8 slots per row, encoded
+    kappa ~105): the first `steps` steps of the 2000-step schedule, plain and with
+    the learned sizer."""
+    from sparqsim_amd import qda_sparse, sizer
+
+    out = {"workload": f"QDA over a sparse block encoding (T^dag S T of the CKS walk) of a seeded 8-slot "
+                       f"band-plus-random matrix at 2^{nqubit}, steps 0..{steps - 1} of a 2000-step schedule"}
+    sizer.enable(False)
+    for mode in ("plain", "learned"):
+        if mode == "learned" and not sizer.activate_default():
+            break
+        s0 = sizer.stats()
+        met, st, *_ = qda_sparse.solve(ps, nqubit, 2000, stop_after=steps)
+        out[mode] = {"ms_per_step": met["ms_per_step"], "entries": met["entries"], "fidelity": met["fidelity"]}
+        if mode == "learned":
+            s1 = sizer.stats()
+            out[mode]["sized_ops_predicted"] = s1["predicted"] - s0["predicted"]
+            out[mode]["reruns"] = s1["reruns"] - s0["reruns"]
+            sizer.enable(False)
+        del st
+        ps.System.clear()
+        ps.release_cached_memory()
+    if "learned" in out:
+        out["learned_speedup_vs_plain"] = out["plain"]["ms_per_step"] / out["learned"]["ms_per_step"]
+    return out
+
+
 def cks_leg(ps, nqubit, terms, R, band_terms=0):
     """The 2^nqubit Poiseuille CKS walk (direct tridiagonal oracles, fitted
     coefficients J = 640): LCU terms timed for one right-hand side and for R in
@@ -222,6 +250,8 @@ def main():
     ap.add_argument("--grover-bits", type=int, default=14)
     ap.add_argument("--grover-iterations", type=int, default=40)
     ap.add_argument("--grover-plain-traj", type=int, default=16, help="trajectories timed one state at a time")
+    ap.add_argument("--qda-sparse-qubits", type=int, default=16)
+    ap.add_argument("--qda-sparse-steps", type=int, default=40, help="steps timed on the sparse-encoding QDA (0 skips)")
     ap.add_argument("--cks-qubits", type=int, default=20)
     ap.add_argument("--cks-terms", type=int, default=6, help="LCU terms timed in the CKS leg (0 skips it)")
     ap.add_argument("--cks-rhs", type=int, default=16, help="right-hand sides of the batched CKS run")
@@ -319,6 +349,8 @@ def main():
             "reference_published": "2^20, 40-step, czEps 1e-16: res_dir 5.178906e-3, fid 0.999999 "
                                    "(docs/verification/v1d-qda-linsolver-a800.md:123)",
         }
+    if args.qda_sparse_steps > 0:
+        result["qda_sparse"] = qda_sparse_leg(ps, args.qda_sparse_qubits, args.qda_sparse_steps)
     if args.cks_terms > 0:
         result["cks"] = cks_leg(ps, args.cks_qubits, args.cks_terms, args.cks_rhs, args.cks_band_terms)
     if args.grover_traj > 0:

@@ -243,12 +243,14 @@ std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t
 
 // ---------------------------------------------------------------- digit-width autotuner
 // The plan is built for 8- and 11-bit digits; when the wider digits save
-// passes, the first sorts of each size class (log2 N) run once with each width
-// under HIP events, and later sorts of that class pick the width with the
-// smaller passes x measured milliseconds per pass.
+// passes, the first sorts of each size class (log2 N) try each width twice --
+// an unmeasured warm-up (first allocations of the class, cold caches) and then
+// a run under HIP events -- and later sorts of that class take 11-bit digits
+// only when passes x measured milliseconds per pass is at least 10 % lower.
 namespace {
 std::mutex g_tune_mu;
 float g_ms_per_pass[2][64];        // [8-bit, 11-bit][log2 N]; < 0: not measured
+int g_tune_phase[64];              // per class: 0, 1 (8-bit warm-up, measured), 2, 3 (11-bit), 4 done
 bool g_tune_init = false;
 int g_forced_bits = 0;
 hipEvent_t g_tune_ev[2] = {nullptr, nullptr};
@@ -271,14 +273,20 @@ void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
         if (!g_tune_init) {
             for (auto& row : g_ms_per_pass)
                 for (float& v : row) v = -1.f;
+            for (int& ph : g_tune_phase) ph = 0;
             g_tune_init = true;
         }
         if (g_forced_bits == 8 || g_forced_bits == 11) {
             use = g_forced_bits == 11 ? 1 : 0;
         } else if (p11.size() < p8.size() && st.n >= (1u << 16)) {
-            if (g_ms_per_pass[0][lg] < 0) measure = use = 0;
-            else if (g_ms_per_pass[1][lg] < 0) measure = use = 1;
-            else use = (p11.size() * g_ms_per_pass[1][lg] < p8.size() * g_ms_per_pass[0][lg]) ? 1 : 0;
+            const int ph = g_tune_phase[lg];
+            if (ph < 4) {
+                use = ph >= 2 ? 1 : 0;
+                if (ph & 1) measure = use;
+                g_tune_phase[lg] = ph + 1;
+            } else {
+                use = (p11.size() * g_ms_per_pass[1][lg] < 0.9f * p8.size() * g_ms_per_pass[0][lg]) ? 1 : 0;
+            }
         }
     }
     const std::vector<Digit>& plan = use ? p11 : p8;
