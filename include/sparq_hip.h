@@ -252,6 +252,14 @@ int sparq_replicate(sparq_state_t st, sparq_field_t target);
 int sparq_normalize(sparq_state_t st, double* norm2);
 /* Sum of |amp|^2 without scaling (CheckNormalization / CheckNan, debugger.cpp:13-70: NaN iff an amplitude is NaN). */
 int sparq_norm2(sparq_state_t st, double* norm2);
+/* |amp|^2 summed over each tile of `tile` consecutive entries in entry order;
+ * ntiles must be ceil(n / tile).  With sparq_state_read_range it samples an
+ * entry with probability |amp|^2 (MeasureZ, SparQ/src/measurement.cpp:127-182). */
+int sparq_tile_norm2(sparq_state_t st, uint64_t tile, double* out, uint64_t ntiles);
+/* |amp|^2 summed over the entries satisfying every condition (Probability, measurement.cpp:281-300). */
+int sparq_cond_norm2(sparq_state_t st, const sparq_cond_t* conds, uint32_t nconds, double* out);
+/* Copy entries [first, first+count) to the host: keys entry-major (count*W words), amps re/im. */
+int sparq_state_read_range(sparq_state_t st, uint64_t first, uint64_t count, uint64_t* keys, double* amps);
 /* TestRemovable helper (debugger.cpp:191-212): *uniform = 1 when every entry has the same value. */
 int sparq_field_uniform(sparq_state_t st, sparq_field_t f, int* uniform, uint64_t* value);
 

@@ -11,7 +11,7 @@
 
 #include <chrono>
 
-#include "sparq_oracle_cks.hpp"
+#include "sparq_oracle_meas.hpp"
 
 namespace py = pybind11;
 using namespace oracle;
@@ -327,6 +327,35 @@ PYBIND11_MODULE(_oracle, m) {
         .def("coef", &ChebyshevPolynomialCoefficient::coef)
         .def("sign", &ChebyshevPolynomialCoefficient::sign)
         .def("step", &ChebyshevPolynomialCoefficient::step);
+
+    // ---- measurement
+    m.def("set_seed", [](long long s) { RandomEngine::get().set_seed((uint64_t)s); });
+    m.def("get_seed", []() { return (long long)RandomEngine::get().seed; });
+    struct Meas { std::vector<size_t> regs; };
+    py::class_<Meas>(m, "MeasureZ")
+        .def(py::init([](py::object r) { return Meas{rids(r)}; }))
+        .def("__call__", [](const Meas& mz, PyState& p) { return MeasureZ(p.s.basis_states, mz.regs); });
+    struct Rst { std::vector<size_t> regs; std::vector<uint64_t> targets; };
+    py::class_<Rst>(m, "Reset")
+        .def(py::init([](py::object r, py::object t) {
+            Rst x{rids(r), {}};
+            if (t.is_none()) x.targets.assign(x.regs.size(), 0);
+            else if (py::isinstance<py::int_>(t)) x.targets.assign(1, t.cast<uint64_t>());
+            else x.targets = t.cast<std::vector<uint64_t>>();
+            if (x.targets.size() != x.regs.size()) throw std::invalid_argument("one target per register");
+            return x;
+        }), py::arg("registers"), py::arg("targets") = py::none())
+        .def("__call__", [](const Rst& r, PyState& p) { return Reset(p.s.basis_states, r.regs, r.targets); });
+    struct Prob { std::vector<size_t> regs; std::vector<uint64_t> vals; };
+    py::class_<Prob>(m, "Probability")
+        .def(py::init([](py::dict d) {
+            Prob x;
+            for (auto item : d) { x.regs.push_back(rid(item.first)); x.vals.push_back(item.second.cast<uint64_t>()); }
+            return x;
+        }))
+        .def(py::init([](py::object r, uint64_t v) { return Prob{{rid(r)}, {v}}; }))
+        .def("__call__", [](const Prob& x, PyState& p) { return Probability(p.s.basis_states, x.regs, x.vals); })
+        .def_static("distribution", [](PyState& p, py::object r) { return Distribution(p.s.basis_states, rid(r)); });
 
     // Timing helper for the CPU baseline: wall seconds of fn() measured in C++.
     m.def("wall_seconds", [](py::function fn) {

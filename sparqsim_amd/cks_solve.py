@@ -78,6 +78,30 @@ def solve(ps, nqubit=20, R=1, kappa=16.0, eps=0.01, ds=32, coef="true", J=640, t
             "curve": curve}
 
 
+def write_artifacts(outdir, res, args):
+    """residual_curve.csv (walk step = 2 term + 1) + qda_summary_cpp.json with the keys of
+    CKSPoiseuilleBench.cu:305-420, for right-hand side 0."""
+    import os
+
+    from .state_io import write_residual_curve, write_summary_json
+
+    os.makedirs(outdir, exist_ok=True)
+    rows = [(2 * c["term"] + 1, c["per_rhs"][0]["res_dir"], c["per_rhs"][0]["res_opt"], c["per_rhs"][0]["p_succ"],
+             c["per_rhs"][0]["fidelity"]) for c in res["curve"]]
+    write_residual_curve(os.path.join(outdir, "residual_curve.csv"), rows)
+    last = res["final"][0]
+    write_summary_json(os.path.join(outdir, "qda_summary_cpp.json"), {
+        "engine": "cks (sparqsim_amd Chebyshev quantum walk + LCU)", "backend": "HIP (MI355X engine)",
+        "dimension": res["n"], "nqubit": res["nqubit"], "kappa": res["kappa"], "eps": res["eps"], "ds": res["ds"],
+        "coef_mode": res["coef"], "J": res["J"], "matrix": res["matrix"], "rhs": res["rhs"],
+        "lcu_terms": res["lcu_terms_budget"], "walk_steps": 2 * res["lcu_terms_budget"],
+        "termination": res["termination"], "tol": args.tol, "final_res_dir": last["res_dir"],
+        "final_res_opt": last["res_opt"], "final_success_prob": last["p_succ"],
+        "final_fidelity_vs_classical": last["fidelity"], "wall_time_s": res["wall_s"],
+        "time_per_term_s": res["s_per_term"], "extract_time_s": res["wall_s"] - res["walk_s"],
+        "max_system_size": res["max_system_size"], "note": "sparqsim_amd engine run"})
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--nqubit", type=int, default=20)
@@ -91,6 +115,9 @@ def main():
     ap.add_argument("--log-every", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--json", default=None, help="write the summary (with the curve) to this file")
+    ap.add_argument("--outdir", default=None,
+                    help="write residual_curve.csv and qda_summary_cpp.json here (the reference bench's artifacts, "
+                         "right-hand side 0)")
     ap.add_argument("--matrix", choices=["poiseuille", "band-random"], default="poiseuille",
                     help="tridiagonal Poiseuille case (direct oracles) or a seeded band-plus-random matrix "
                          "(QRAM-backed sparse-access oracles, fitted coefficients; --kappa sets its condition number)")
@@ -106,6 +133,8 @@ def main():
     if args.json:
         with open(args.json, "w") as f:
             json.dump(res, f)
+    if args.outdir:
+        write_artifacts(args.outdir, res, args)
 
 
 if __name__ == "__main__":

@@ -564,6 +564,36 @@ int sparq_normalize(sparq_state_t h, double* norm2) {
     });
 }
 
+int sparq_tile_norm2(sparq_state_t h, uint64_t tile, double* out, uint64_t ntiles) {
+    return guarded([&] {
+        SPQ_REQUIRE(out != nullptr || ntiles == 0, "null argument");
+        tile_abs2(S(h), tile, out, ntiles);
+    });
+}
+
+int sparq_cond_norm2(sparq_state_t h, const sparq_cond_t* conds, uint32_t nconds, double* out) {
+    return guarded([&] {
+        SPQ_REQUIRE(out != nullptr && (conds != nullptr || nconds == 0), "null argument");
+        *out = cond_abs2(S(h), conds, nconds);
+    });
+}
+
+int sparq_state_read_range(sparq_state_t h, uint64_t first, uint64_t count, uint64_t* keys, double* amps) {
+    return guarded([&] {
+        State& st = S(h);
+        SPQ_REQUIRE(first <= st.n && count <= st.n - first, "range outside the state");
+        if (count == 0) return;
+        if (keys) {
+            std::vector<uint64_t> word(count);
+            for (uint32_t w = 0; w < st.W; ++w) {
+                read_back(st, st.keys + (uint64_t)w * st.cap + first, count * sizeof(uint64_t), word.data());
+                for (uint64_t i = 0; i < count; ++i) keys[i * st.W + w] = word[i];
+            }
+        }
+        if (amps) read_back(st, st.amps + first, count * sizeof(double2), amps);
+    });
+}
+
 int sparq_norm2(sparq_state_t h, double* norm2) {
     return guarded([&] {
         SPQ_REQUIRE(norm2 != nullptr, "null argument");

@@ -122,7 +122,9 @@ struct Digit {
 };
 // Stable LSD radix sort of the state by the given digit plan (least significant
 // digit first, every digit at most radix_bits wide).  Reorders keys and amps.
-void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits = 8);
+// pass_begin / pass_end (optional) are recorded around the digit passes alone.
+void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits = 8, hipEvent_t pass_begin = nullptr,
+                  hipEvent_t pass_end = nullptr);
 // Digit width forced for every sort (8 or 11), or 0 for the autotuner.
 void set_radix_bits(int bits);
 // One pass over the keys: bits that vary across the state, and whether the
@@ -203,6 +205,10 @@ struct BasisKey { uint64_t w[SPARQ_MAX_WORDS]; };
 // Entry 0 := |key>, amplitude 1 (the state's arrays must hold at least one entry).
 void write_basis(State& st, const BasisKey& k);
 double sum_abs2(State& st);
+// |a|^2 summed per tile of `tile` entries, read back into host_out[ntiles].
+void tile_abs2(State& st, uint64_t tile, double* host_out, uint64_t ntiles);
+// |a|^2 summed over the entries satisfying every condition.
+double cond_abs2(State& st, const sparq_cond_t* conds, uint32_t ncond);
 void scale_amps(State& st, double s);
 bool field_uniform(State& st, sparq_field_t f, uint64_t* value);
 

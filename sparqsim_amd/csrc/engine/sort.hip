@@ -245,11 +245,13 @@ std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t
 // The plan is built for 8- and 11-bit digits; when the wider digits save
 // passes, the first sorts of each size class (log2 N) try each width twice --
 // an unmeasured warm-up (first allocations of the class, cold caches) and then
-// a run under HIP events -- and later sorts of that class take 11-bit digits
-// only when passes x measured milliseconds per pass is at least 10 % lower.
+// a run whose digit passes alone are timed with HIP events, normalised per
+// entry and pass (the two timed sorts have different sizes and pass counts).
+// Later sorts of that class take 11-bit digits only when passes x measured
+// cost per entry-pass is at least 10 % lower.
 namespace {
 std::mutex g_tune_mu;
-float g_ms_per_pass[2][64];        // [8-bit, 11-bit][log2 N]; < 0: not measured
+float g_ms_per_pass[2][64];        // [8-bit, 11-bit][log2 N]: ms per pass per 2^20 entries; < 0: not measured
 int g_tune_phase[64];              // per class: 0, 1 (8-bit warm-up, measured), 2, 3 (11-bit), 4 done
 bool g_tune_init = false;
 int g_forced_bits = 0;
@@ -298,14 +300,13 @@ void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
         SPQ_HIP(hipEventCreate(&g_tune_ev[0]));
         SPQ_HIP(hipEventCreate(&g_tune_ev[1]));
     }
-    SPQ_HIP(hipEventRecord(g_tune_ev[0], st.stream));
-    sort_by_plan(st, plan, use ? 11 : 8);
-    SPQ_HIP(hipEventRecord(g_tune_ev[1], st.stream));
+    const uint64_t n = st.n;
+    sort_by_plan(st, plan, use ? 11 : 8, g_tune_ev[0], g_tune_ev[1]);
     SPQ_HIP(hipEventSynchronize(g_tune_ev[1]));
     float ms = 0.f;
     SPQ_HIP(hipEventElapsedTime(&ms, g_tune_ev[0], g_tune_ev[1]));
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_ms_per_pass[measure][lg] = ms / (float)plan.size();
+    g_ms_per_pass[measure][lg] = ms / (float)plan.size() / (float)((double)n / 1048576.0);
 }
 
 bool sort_state(State& st, const sparq_field_t* last) {
@@ -465,7 +466,7 @@ static void launch_gather_state(State& st, const uint32_t* idx, uint64_t* kout, 
     SPQ_HIP(hipGetLastError());
 }
 
-void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits) {
+void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hipEvent_t pass_begin, hipEvent_t pass_end) {
     const uint64_t n = st.n;
     if (plan.empty() || n <= 1) return;
     SPQ_REQUIRE(radix_bits == 8 || radix_bits == 11, "digit width must be 8 or 11");
@@ -493,6 +494,7 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits) {
     bool identity = true;
     int cur_word = -1;
     const uint64_t* src = nullptr;
+    if (pass_begin) SPQ_HIP(hipEventRecord(pass_begin, st.stream));   // the digit passes only
     for (size_t p = 0; p < plan.size(); ++p) {
         const Digit dg = plan[p];
         if (dg.word != cur_word) {
@@ -532,6 +534,7 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits) {
         identity = false;
         src = kwA;
     }
+    if (pass_end) SPQ_HIP(hipEventRecord(pass_end, st.stream));
     uint64_t* kout;
     double2* aout;
     alloc_arrays(st, st.W, st.cap, &kout, &aout);

@@ -44,22 +44,50 @@ void release_cache_locked() {
     g_free_blocks.clear();
     g_cached_bytes = 0;
 }
+
+// Live + cached bytes above which a miss first returns cached blocks (largest
+// first) to the runtime: 85 % of the device memory.
+size_t cache_budget() {
+    static size_t budget = [] {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || total_b == 0) return ~size_t(0);
+        return total_b / 20 * 17;
+    }();
+    return budget;
+}
+
+void trim_cache_locked(size_t need) {
+    const size_t budget = cache_budget();
+    if (g_live_bytes + g_cached_bytes + need <= budget || g_free_blocks.empty()) return;
+    (void)hipDeviceSynchronize();
+    while (!g_free_blocks.empty() && g_live_bytes + g_cached_bytes + need > budget) {
+        auto it = std::prev(g_free_blocks.end());
+        (void)hipFree(it->second.p);
+        g_cached_bytes -= it->first;
+        g_free_blocks.erase(it);
+    }
+}
 }  // namespace
 
 void* dmalloc(size_t bytes, hipStream_t s) {
-    const size_t cls = size_class(bytes == 0 ? 1 : bytes);
+    const size_t cls0 = size_class(bytes == 0 ? 1 : bytes);
     std::lock_guard<std::mutex> lock(g_mem_mu);
-    auto range = g_free_blocks.equal_range(cls);
-    auto pick = range.second;
-    for (auto it = range.first; it != range.second; ++it) {
+    // A cached block of the request's class, else the smallest one of up to twice
+    // its size (sizes drift from operator to operator); a block last used on the
+    // same stream (or on none) is preferred.
+    auto pick = g_free_blocks.end();
+    for (auto it = g_free_blocks.lower_bound(cls0); it != g_free_blocks.end() && it->first <= 2 * cls0; ++it) {
         if (it->second.stream == s || it->second.stream == nullptr) {
             pick = it;
             break;
         }
-        if (pick == range.second) pick = it;
+        if (pick == g_free_blocks.end()) pick = it;
+        if (it->first != pick->first) break;
     }
     void* p = nullptr;
-    if (pick != range.second) {
+    size_t cls = cls0;
+    if (pick != g_free_blocks.end()) {
+        cls = pick->first;
         CachedBlock blk = pick->second;
         g_free_blocks.erase(pick);
         g_cached_bytes -= cls;
@@ -73,6 +101,7 @@ void* dmalloc(size_t bytes, hipStream_t s) {
         p = blk.p;
         ++g_hits;
     } else {
+        trim_cache_locked(cls);
         hipError_t e = hipMalloc(&p, cls);
         if (e == hipErrorOutOfMemory && !g_free_blocks.empty()) {
             (void)hipGetLastError();
@@ -353,6 +382,90 @@ double sum_abs2(State& st) {
         sum_abs2_kernel<<<g, kBlock, 0, st.stream>>>(st.amps, st.n, partial);
     }
     SPQ_HIP(hipGetLastError());
+    std::vector<double> h(g);
+    read_back(st, partial, g * sizeof(double), h.data());
+    dfree(partial, st.stream);
+    double s = 0;
+    for (double v : h) s += v;
+    return s;
+}
+
+// Sum of |a|^2 over each tile of `tile` consecutive entries (in entry order):
+// the coarse level of sampling an entry with probability |a|^2 (MeasureZ).
+__global__ void __launch_bounds__(kBlock) tile_abs2_kernel(const double2* __restrict__ amps, uint64_t n, uint64_t tile,
+                                                     double* __restrict__ out) {
+    __shared__ double lds[kBlock / 64];
+    const uint64_t b = (uint64_t)blockIdx.x * tile, e = min(n, b + tile);
+    double s = 0;
+    for (uint64_t i = b + threadIdx.x; i < e; i += kBlock) s += cabs2(amps[i]);
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
+        out[blockIdx.x] = t;
+    }
+}
+
+void tile_abs2(State& st, uint64_t tile, double* host_out, uint64_t ntiles) {
+    SPQ_REQUIRE(tile >= 1 && ntiles == (st.n + tile - 1) / tile, "tile count must be ceil(n / tile)");
+    if (ntiles == 0) return;
+    SPQ_REQUIRE(ntiles <= (1u << 31), "too many tiles");
+    double* out = dalloc<double>(ntiles, st.stream);
+    {
+        SPQ_PROF(st.stream, "tile_abs2_kernel", st.n * 16.0);
+        tile_abs2_kernel<<<(unsigned)ntiles, kBlock, 0, st.stream>>>(st.amps, st.n, tile, out);
+    }
+    SPQ_HIP(hipGetLastError());
+    read_back(st, out, ntiles * sizeof(double), host_out);
+    dfree(out, st.stream);
+}
+
+// Sum of |a|^2 over the entries satisfying every condition (Probability).
+template <int W>
+__global__ void __launch_bounds__(kBlock) cond_abs2_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                                      uint64_t n, const CondList cl, double* __restrict__ partial) {
+    __shared__ double lds[kBlock / 64];
+    double s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        uint64_t k[W];
+        load_key<W>(keys, cap, i, k);
+        bool ok = true;
+        for (uint32_t c = 0; c < cl.n; ++c) ok = ok && cond_ok<W>(k, cl.c[c]);
+        if (ok) s += cabs2(amps[i]);
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+template <int W>
+static void launch_cond_abs2(State& st, const CondList& cl, double* partial, unsigned g) {
+    {
+        SPQ_PROF(st.stream, "cond_abs2_kernel", st.n * (8.0 * W + 16));
+        cond_abs2_kernel<W><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, cl, partial);
+    }
+    SPQ_HIP(hipGetLastError());
+}
+
+double cond_abs2(State& st, const sparq_cond_t* conds, uint32_t ncond) {
+    SPQ_REQUIRE(ncond <= SPARQ_MAX_PROGRAM_CONDS, "too many conditions");
+    CondList cl{};
+    cl.n = ncond;
+    for (uint32_t c = 0; c < ncond; ++c) {
+        SPQ_REQUIRE(conds[c].field.word < st.W, "condition field out of range");
+        cl.c[c] = conds[c];
+    }
+    if (st.n == 0) return 0.0;
+    const unsigned g = std::min<unsigned>(grid_for(st.n, kBlock), 1024);
+    double* partial = dalloc<double>(g, st.stream);
+    SPQ_DISPATCH_W(st.W, launch_cond_abs2, st, cl, partial, g);
     std::vector<double> h(g);
     read_back(st, partial, g * sizeof(double), h.data());
     dfree(partial, st.stream);

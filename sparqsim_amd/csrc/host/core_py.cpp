@@ -325,6 +325,42 @@ PYBIND11_MODULE(_core, m) {
         .def(py::init<size_t, std::pair<size_t, size_t>>(), py::arg("register_id"), py::arg("select_range"))
         .def("__call__", [](const PartialTraceSelectRange& p, SparseState& s) { return p(s); }, py::arg("state"), Release());
 
+    // measurement with the seedable engine (PySparQ/core.cpp:404-505)
+    m.def("set_seed", [](long long seed) { random_engine::set_seed((uint64_t)seed); }, py::arg("seed"));
+    m.def("get_seed", []() { return (long long)random_engine::get_seed(); });
+    py::class_<MeasureZ>(m, "MeasureZ")
+        .def(py::init<const std::vector<std::string>&>(), py::arg("register_names"))
+        .def(py::init<const std::vector<size_t>&>(), py::arg("register_ids"))
+        .def(py::init<std::string_view>(), py::arg("register_name"))
+        .def(py::init<size_t>(), py::arg("register_id"))
+        .def_readonly("registers", &MeasureZ::registers)
+        .def("__call__", [](const MeasureZ& mz, SparseState& s) { return mz(s); }, py::arg("state"), Release());
+    py::class_<Reset>(m, "Reset")
+        .def(py::init<const std::vector<std::string>&>(), py::arg("register_names"))
+        .def(py::init<const std::vector<std::string>&, const std::vector<uint64_t>&>(), py::arg("register_names"), py::arg("targets"))
+        .def(py::init<const std::vector<size_t>&>(), py::arg("register_ids"))
+        .def(py::init<const std::vector<size_t>&, const std::vector<uint64_t>&>(), py::arg("register_ids"), py::arg("targets"))
+        .def(py::init<std::string_view, uint64_t>(), py::arg("register_name"), py::arg("target") = 0)
+        .def(py::init<size_t, uint64_t>(), py::arg("register_id"), py::arg("target") = 0)
+        .def_readonly("registers", &Reset::registers)
+        .def_readonly("target_values", &Reset::target_values)
+        .def("__call__", [](const Reset& r, SparseState& s) { return r(s); }, py::arg("state"), Release());
+    py::class_<Probability>(m, "Probability")
+        .def(py::init<const std::map<std::string, uint64_t>&>(), py::arg("name_value_map"))
+        .def(py::init<const std::map<size_t, uint64_t>&>(), py::arg("id_value_map"))
+        .def(py::init<const std::vector<std::string>&, const std::vector<uint64_t>&>(), py::arg("register_names"),
+             py::arg("target_values"))
+        .def(py::init<const std::vector<size_t>&, const std::vector<uint64_t>&>(), py::arg("register_ids"), py::arg("target_values"))
+        .def(py::init<std::string_view, uint64_t>(), py::arg("register_name"), py::arg("value"))
+        .def(py::init<size_t, uint64_t>(), py::arg("register_id"), py::arg("value"))
+        .def_readonly("registers", &Probability::registers)
+        .def_readonly("values", &Probability::values)
+        .def("__call__", [](const Probability& p, SparseState& s) { return p(s); }, py::arg("state"), Release())
+        .def_static("distribution", [](SparseState& s, py::object r) {
+            return py::isinstance<py::str>(r) ? Probability::distribution(s, r.cast<std::string>())
+                                              : Probability::distribution(s, r.cast<size_t>());
+        }, py::arg("state"), py::arg("register"));
+
     // CKS quantum-walk primitives (PySparQ/core.cpp:53-65, 1472-1510)
     py::class_<SparseMatrix, std::shared_ptr<SparseMatrix>>(m, "SparseMatrix")
         .def(py::init<>())

@@ -1195,6 +1195,194 @@ double PartialTraceSelectRange::operator()(std::vector<System>& state) const {
     return f;
 }
 
+// ------------------------------------------------------------ measurement
+namespace random_engine {
+static uint64_t g_seed = 0x5EED2026ull, g_state = 0x5EED2026ull;
+void set_seed(uint64_t seed) { g_seed = g_state = seed; }
+uint64_t get_seed() { return g_seed; }
+double uniform01() {
+    uint64_t z = (g_state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return double(z >> 11) * (1.0 / 9007199254740992.0);
+}
+}  // namespace random_engine
+
+namespace {
+// measurement.cpp:18-103: ids exist, are active, are distinct; values fit the widths
+void validate_ids(const std::vector<size_t>& ids, const char* ctx) {
+    for (size_t id : ids)
+        if (id >= System::name_register_map.size() || !System::status_of(id))
+            throw std::invalid_argument(std::string(ctx) + ": register id " + std::to_string(id) + " is not an active register.");
+    std::vector<size_t> sorted(ids);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        throw std::invalid_argument(std::string(ctx) + ": duplicate register id in the same register list.");
+}
+std::vector<size_t> resolve_names(const std::vector<std::string>& names, const char* ctx) {
+    std::vector<size_t> ids;
+    for (auto& n : names) {
+        size_t id = System::get(n);
+        if (id == SIZE_MAX) throw std::invalid_argument(std::string(ctx) + ": register '" + n + "' not found.");
+        ids.push_back(id);
+    }
+    validate_ids(ids, ctx);
+    return ids;
+}
+void validate_fit(const std::vector<size_t>& ids, const std::vector<uint64_t>& v, const char* ctx) {
+    if (ids.size() != v.size()) throw std::invalid_argument(std::string(ctx) + ": one value per register.");
+    for (size_t i = 0; i < ids.size(); ++i)
+        if (v[i] & ~width_mask(System::size_of(ids[i])))
+            throw std::invalid_argument(std::string(ctx) + ": value " + std::to_string(v[i]) + " does not fit in register '" +
+                                        System::name_of(ids[i]) + "'.");
+}
+std::vector<sparq_cond_t> value_conds(const Layout& L, const std::vector<size_t>& ids, const std::vector<uint64_t>& v) {
+    std::vector<sparq_cond_t> c;
+    for (size_t i = 0; i < ids.size(); ++i) c.push_back(sparq_cond_t{SPARQ_COND_VALUE, L.at(ids[i]), v[i]});
+    return c;
+}
+}  // namespace
+
+MeasureZ::MeasureZ(const std::vector<std::string>& names) : registers(resolve_names(names, "MeasureZ")) {}
+MeasureZ::MeasureZ(const std::vector<size_t>& ids) : registers(ids) { validate_ids(ids, "MeasureZ"); }
+MeasureZ::MeasureZ(std::string_view name) : registers(resolve_names({std::string(name)}, "MeasureZ")) {}
+MeasureZ::MeasureZ(size_t id) : registers{id} { validate_ids(registers, "MeasureZ"); }
+std::pair<std::vector<uint64_t>, double> MeasureZ::operator()(SparseState& state) const {   // measurement.cpp:127-182
+    const size_t n = state.size();
+    if (n == 0) throw std::invalid_argument("MeasureZ: cannot measure an empty state.");
+    constexpr uint64_t kTile = 4096;
+    SortUnconditional{}(state);   // canonical order: the walk below does not depend on the engine's entry order
+    const Layout& L = state.sync();
+    const uint64_t ntiles = (n + kTile - 1) / kTile;
+    std::vector<double> tiles(ntiles);
+    check(sparq_tile_norm2(state.handle(), kTile, tiles.data(), ntiles));
+    double total = 0;
+    for (double t : tiles) total += t;
+    if (!std::isfinite(total) || total <= 0.0 || !(std::fabs(total - 1.0) < 1e-5))
+        throw std::runtime_error("MeasureZ: input state is not normalized (total probability = " + std::to_string(total) +
+                                 "); renormalize (e.g. via Normalize()) before measuring.");
+    if (registers.empty()) return {{}, 1.0};
+    double r = random_engine::uniform01();
+    uint64_t pick = n - 1;   // reference fallback: the last entry
+    for (uint64_t t = 0; t < ntiles; ++t) {
+        if (!(r < tiles[t])) {
+            r -= tiles[t];
+            continue;
+        }
+        const uint64_t first = t * kTile, count = std::min<uint64_t>(kTile, n - first);
+        std::vector<double> a(2 * count);
+        check(sparq_state_read_range(state.handle(), first, count, nullptr, a.data()));
+        pick = first + count - 1;
+        for (uint64_t i = 0; i < count; ++i) {
+            const double p = a[2 * i] * a[2 * i] + a[2 * i + 1] * a[2 * i + 1];
+            if (r < p) {
+                pick = first + i;
+                break;
+            }
+            r -= p;
+        }
+        break;
+    }
+    std::vector<uint64_t> key(L.words);
+    check(sparq_state_read_range(state.handle(), pick, 1, key.data(), nullptr));
+    std::vector<uint64_t> outcome;
+    for (size_t id : registers) outcome.push_back(field_value(key, L.at(id)));
+    const std::vector<sparq_cond_t> conds = value_conds(L, registers, outcome);
+    double sum_prob = 0, f = 0;
+    check(sparq_cond_norm2(state.handle(), conds.data(), (uint32_t)conds.size(), &sum_prob));
+    if (!(sum_prob > 0)) throw std::runtime_error("MeasureZ: sampled outcome has zero support.");
+    check(sparq_select(state.handle(), conds.data(), (uint32_t)conds.size(), &f));
+    return {outcome, sum_prob};
+}
+
+Reset::Reset(const std::vector<std::string>& names) : Reset(names, std::vector<uint64_t>(names.size(), 0)) {}
+Reset::Reset(const std::vector<std::string>& names, const std::vector<uint64_t>& targets)
+    : registers(resolve_names(names, "Reset")), target_values(targets) {
+    validate_fit(registers, target_values, "Reset");
+}
+Reset::Reset(const std::vector<size_t>& ids) : Reset(ids, std::vector<uint64_t>(ids.size(), 0)) {}
+Reset::Reset(const std::vector<size_t>& ids, const std::vector<uint64_t>& targets) : registers(ids), target_values(targets) {
+    validate_ids(ids, "Reset");
+    validate_fit(registers, target_values, "Reset");
+}
+Reset::Reset(std::string_view name, uint64_t target) : registers(resolve_names({std::string(name)}, "Reset")), target_values{target} {
+    validate_fit(registers, target_values, "Reset");
+}
+Reset::Reset(size_t id, uint64_t target) : registers{id}, target_values{target} {
+    validate_ids(registers, "Reset");
+    validate_fit(registers, target_values, "Reset");
+}
+std::vector<uint64_t> Reset::operator()(SparseState& state) const {   // measurement.cpp:213-228
+    if (registers.empty()) return {};
+    auto res = MeasureZ(registers)(state);
+    const Layout& L = state.sync();
+    for (size_t i = 0; i < registers.size(); ++i) {
+        sparq_pw_op_t op{};
+        op.opcode = SPARQ_PW_SET_FIELD;
+        op.f[0] = L.at(registers[i]);
+        op.c[0] = target_values[i];
+        state.push_pointwise(op, {});
+    }
+    state.flush();
+    return res.first;
+}
+
+Probability::Probability(const std::map<std::string, uint64_t>& a) {
+    for (auto& [k, v] : a) {
+        registers.push_back(resolve_names({k}, "Probability")[0]);
+        values.push_back(v);
+    }
+    validate_fit(registers, values, "Probability");
+}
+Probability::Probability(const std::map<size_t, uint64_t>& a) {
+    for (auto& [k, v] : a) {
+        registers.push_back(k);
+        values.push_back(v);
+    }
+    validate_ids(registers, "Probability");
+    validate_fit(registers, values, "Probability");
+}
+Probability::Probability(const std::vector<std::string>& names, const std::vector<uint64_t>& v)
+    : registers(resolve_names(names, "Probability")), values(v) {
+    validate_fit(registers, values, "Probability");
+}
+Probability::Probability(const std::vector<size_t>& ids, const std::vector<uint64_t>& v) : registers(ids), values(v) {
+    validate_ids(registers, "Probability");
+    validate_fit(registers, values, "Probability");
+}
+Probability::Probability(std::string_view name, uint64_t value)
+    : registers(resolve_names({std::string(name)}, "Probability")), values{value} {
+    validate_fit(registers, values, "Probability");
+}
+Probability::Probability(size_t id, uint64_t value) : registers{id}, values{value} {
+    validate_ids(registers, "Probability");
+    validate_fit(registers, values, "Probability");
+}
+double Probability::operator()(SparseState& state) const {   // measurement.cpp:281-300
+    if (registers.empty()) return 1.0;
+    const Layout& L = state.sync();
+    state.flush();
+    const std::vector<sparq_cond_t> conds = value_conds(L, registers, values);
+    double p = 0;
+    check(sparq_cond_norm2(state.handle(), conds.data(), (uint32_t)conds.size(), &p));
+    return p;
+}
+std::map<uint64_t, double> Probability::distribution(SparseState& state, size_t id) {   // measurement.cpp:302-318
+    validate_ids({id}, "Probability::distribution");
+    std::vector<size_t> ids;
+    std::vector<uint64_t> vals;
+    std::vector<complex_t> amps;
+    state.export_entries(ids, vals, amps);
+    const size_t col = std::find(ids.begin(), ids.end(), id) - ids.begin();
+    std::map<uint64_t, double> dist;
+    for (size_t i = 0; i < amps.size(); ++i) dist[vals[i * ids.size() + col]] += std::norm(amps[i]);
+    return dist;
+}
+std::map<uint64_t, double> Probability::distribution(SparseState& state, std::string_view name) {
+    return distribution(state, resolve_names({std::string(name)}, "Probability::distribution")[0]);
+}
+
 // ------------------------------------------------------------ CKS quantum walk primitives
 static uint64_t encode_element(double x, size_t ds, bool positive_only) {
     if (positive_only) {

@@ -595,6 +595,55 @@ struct PartialTraceSelectRange {
     double operator()(std::vector<System>& state) const;
 };
 
+// ------------------------------------------------------------ measurement
+// Seedable global random engine (the role of random_engine.h in the reference,
+// PySparQ/core.cpp:404-424): splitmix64, uniform01 = top 53 bits / 2^53.  The
+// reference's engine lives in the absent submodule, so streams differ from it.
+namespace random_engine {
+void set_seed(uint64_t seed);
+uint64_t get_seed();
+double uniform01();
+}  // namespace random_engine
+
+// MeasureZ / Reset / Probability (SparQ/include/measurement.h:60-140,
+// SparQ/src/measurement.cpp).  MeasureZ sorts the state into canonical order
+// and walks the entries in that order with r = uniform01(), so the outcome for
+// a given seed does not depend on the engine's internal entry order (the
+// reference walks its own entry order); it then keeps the entries with the
+// sampled register values and renormalises.  Returns (outcome, probability).
+struct MeasureZ {
+    std::vector<size_t> registers;
+    explicit MeasureZ(const std::vector<std::string>& names);
+    explicit MeasureZ(const std::vector<size_t>& ids);
+    explicit MeasureZ(std::string_view name);
+    explicit MeasureZ(size_t id);
+    std::pair<std::vector<uint64_t>, double> operator()(SparseState& state) const;
+};
+struct Reset {
+    std::vector<size_t> registers;
+    std::vector<uint64_t> target_values;
+    explicit Reset(const std::vector<std::string>& names);
+    Reset(const std::vector<std::string>& names, const std::vector<uint64_t>& targets);
+    explicit Reset(const std::vector<size_t>& ids);
+    Reset(const std::vector<size_t>& ids, const std::vector<uint64_t>& targets);
+    explicit Reset(std::string_view name, uint64_t target = 0);
+    explicit Reset(size_t id, uint64_t target = 0);
+    std::vector<uint64_t> operator()(SparseState& state) const;
+};
+struct Probability {
+    std::vector<size_t> registers;
+    std::vector<uint64_t> values;
+    explicit Probability(const std::map<std::string, uint64_t>& assignments);
+    explicit Probability(const std::map<size_t, uint64_t>& assignments);
+    Probability(const std::vector<std::string>& names, const std::vector<uint64_t>& values);
+    Probability(const std::vector<size_t>& ids, const std::vector<uint64_t>& values);
+    Probability(std::string_view name, uint64_t value);
+    Probability(size_t id, uint64_t value);
+    double operator()(SparseState& state) const;
+    static std::map<uint64_t, double> distribution(SparseState& state, size_t id);
+    static std::map<uint64_t, double> distribution(SparseState& state, std::string_view name);
+};
+
 // ------------------------------------------------------------ CKS quantum walk primitives
 // (SparQ_Algorithm/include/hamiltonian_simulation.h, SparQ_Algorithm/src/hamiltonian_simulation.cpp)
 

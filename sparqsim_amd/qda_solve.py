@@ -75,6 +75,29 @@ def solve(ps, nqubit=20, steprate=1.1, p=1.3, alpha=2.2666666666564934, beta=-1.
             "termination": termination, "curve": curve}
 
 
+def write_artifacts(outdir, res, args, ps):
+    """residual_curve.csv + qda_summary_cpp.json with the keys of PoiseuilleQDABench.cu:170-293."""
+    import os
+
+    from .state_io import write_residual_curve, write_summary_json
+
+    os.makedirs(outdir, exist_ok=True)
+    write_residual_curve(os.path.join(outdir, "residual_curve.csv"),
+                         [(m["step"], m["res_dir"], m["res_opt"], m["success_prob"], m["fidelity"]) for m in res["curve"]])
+    write_summary_json(os.path.join(outdir, "qda_summary_cpp.json"), {
+        "engine": "qda (sparqsim_amd WalkSTridiagonal + device GetOutput post-selection)",
+        "backend": "HIP (MI355X engine)", "dimension": res["n"], "nqubit": res["nqubit"], "kappa": res["kappa"],
+        "kappa_analytic": res["kappa_analytic"], "eps": 0.01, "alpha": res["alpha"], "beta": res["beta"], "p": res["p"],
+        "clearzero_eps": res["cz_eps"], "problem": "1D Poisson tridiagonal, b=ones/sqrt(n)",
+        "step_budget": res["steps_budget"], "steps_executed": res["steps_run"], "step_rate": args.steprate,
+        "step_constant": qda.QDA_STEP_CONSTANT, "termination": res["termination"], "tol": args.tol,
+        "final_res_dir": res["final_res_dir"], "final_res_opt": res["final_res_opt"],
+        "final_success_prob": res["final_success_prob"], "final_fidelity_vs_classical": res["final_fidelity"],
+        "wall_time_s": res["wall_s"], "time_per_step_s": res["wall_s"] / max(res["steps_run"], 1),
+        "extract_time_s": res["wall_s"] - res["walk_s"], "max_system_size": ps.System.max_system_size(),
+        "note": "sparqsim_amd engine run"})
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--nqubit", type=int, default=20)
@@ -90,6 +113,8 @@ def main():
     ap.add_argument("--stop-after", type=int, default=None, help="run only the first N steps of the schedule")
     ap.add_argument("--learned", action="store_true", help="size branching operators with the learned sizer")
     ap.add_argument("--json", default=None, help="write the summary (with the curve) to this file")
+    ap.add_argument("--outdir", default=None,
+                    help="write residual_curve.csv and qda_summary_cpp.json here (the reference bench's artifacts)")
     args = ap.parse_args()
     import sparqsim_amd as ps
     from . import sizer
@@ -102,6 +127,8 @@ def main():
         res["sizer"] = sizer.stats()
     summary = {k: v for k, v in res.items() if k != "curve"}
     print(json.dumps(summary), flush=True)
+    if args.outdir:
+        write_artifacts(args.outdir, res, args, ps)
     if args.json:
         with open(args.json, "w") as f:
             json.dump(res, f)
