@@ -143,13 +143,16 @@ __global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __r
 template <int W>
 __global__ void __launch_bounds__(kBlock) expand_single_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps,
                                                           uint64_t cap, sparq_field_t low, uint32_t nd, double extra,
-                                                          uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out) {
+                                                          uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out,
+                                                          double eps, uint64_t* __restrict__ pruned) {
     uint64_t k[W];
     load_key<W>(keys, cap, 0, k);
     const uint64_t vlow = fget<W>(k, low);
     const double2 t = cscale(amps[0], extra);                   // hadamard.cpp:30-39
     const double2 tn = make_double2(t.x * -1.0, t.y * -1.0);
     const uint64_t F = uint64_t(1) << nd;
+    // every output is t or -t: ClearZero keeps all of them or none
+    if (blockIdx.x == 0 && threadIdx.x == 0) pruned[0] = (cabs2(t) < eps) ? 1 : 0;
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
         const uint64_t o = ((uint64_t)blockIdx.x * 8 + it) * kBlock + threadIdx.x;
@@ -159,7 +162,8 @@ __global__ void __launch_bounds__(kBlock) expand_single_kernel(const uint64_t* _
         for (int w = 0; w < W; ++w) ko[w] = k[w];
         fput<W>(ko, low, o);
         store_key<W>(kout, cap_out, o, ko);
-        aout[o] = (__popcll(o & vlow) & 1) ? tn : t;
+        const bool neg = __popcll(o & vlow) & 1;
+        aout[o] = make_double2(neg ? tn.x : t.x, neg ? tn.y : t.y);   // per component: no indexed pair
     }
 }
 
@@ -245,13 +249,15 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
     if (base >= M) return;
     if (nd >= (uint32_t)kWhtTileLog && cls[base >> nd] != kWht) return;   // the tile lies in one non-WHT group
     const uint32_t T = (uint32_t)min<uint64_t>(uint64_t(1) << kWhtTileLog, M - base);
-    // all loads of the tile are issued before the first LDS store
+    // all loads of the tile are issued before the first LDS store; a partial
+    // tile loads its element 0 in place of the missing ones (unconditional
+    // loads keep v in registers)
     constexpr int kPer = (1 << kWhtTileLog) / kBlock;
     double2 v[kPer];
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
         const uint32_t e = r * kBlock + threadIdx.x;
-        if (e < T) v[r] = a[base + e];
+        v[r] = a[base + (e < T ? e : 0)];
     }
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
@@ -259,7 +265,8 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
         if (e < T) buf[lpad(e)] = v[r];
     }
     __syncthreads();
-    lds_wht_rounds(buf, T, 0, cnt, true, base, nd, cls);
+    // a tile inside one group was checked above; smaller groups are checked per round
+    lds_wht_rounds(buf, T, 0, cnt, nd < (uint32_t)kWhtTileLog, base, nd, cls);
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
         const uint32_t e = r * kBlock + threadIdx.x;
@@ -270,7 +277,7 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
 // Later passes: stages [lo, lo+cnt) of one group, on tiles of 2^cnt x 2^c2
 // elements (c2 contiguous low index bits, c2 <= lo) of up to 2^kWhtPassTileLog
 // elements (64 KiB of LDS).
-constexpr int kWhtPassTileLog = 12;
+constexpr int kWhtPassTileLog = 11;
 __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ a, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
                                                      uint32_t nd, uint32_t lo, uint32_t cnt, uint32_t c2, const uint8_t* __restrict__ cls) {
     __shared__ double2 buf[lpad(1 << kWhtPassTileLog)];
@@ -286,26 +293,29 @@ __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ 
     const uint64_t lowchunk = r % nlowchunks, high = r / nlowchunks;
     const uint64_t base = (g << nd) + (high << (lo + cnt)) + (lowchunk << c2);
     // element e of the tile: sub = e >> c2 (stage index bits), l = e & (2^c2-1);
-    // loads in groups of 8 per thread, all issued before their LDS stores
-    constexpr int kPer = 8;
-    for (uint32_t e0 = 0; e0 < T; e0 += kPer * kBlock) {
+    // loads in groups of 16 per thread (a whole 2^12 tile in one group), all
+    // issued before their LDS stores
+    constexpr int kPer = (1 << kWhtPassTileLog) / kBlock;   // T <= kPer * kBlock
+    {
         double2 v[kPer];
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
-            const uint32_t e = e0 + r * kBlock + threadIdx.x;
-            if (e < T) v[r] = a[base + ((uint64_t)(e >> c2) << lo) + (e & ((1u << c2) - 1))];
+            const uint32_t e0 = r * kBlock + threadIdx.x;
+            const uint32_t e = e0 < T ? e0 : 0;   // unconditional loads keep v in registers
+            v[r] = a[base + ((uint64_t)(e >> c2) << lo) + (e & ((1u << c2) - 1))];
         }
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
-            const uint32_t e = e0 + r * kBlock + threadIdx.x;
+            const uint32_t e = r * kBlock + threadIdx.x;
             if (e < T) buf[lpad(e)] = v[r];
         }
     }
     __syncthreads();
     lds_wht_rounds(buf, T, c2, cnt, false, 0, nd, cls);
-    for (uint32_t e = threadIdx.x; e < T; e += kBlock) {
-        uint64_t sub = e >> c2, l = e & ((1u << c2) - 1);
-        a[base + (sub << lo) + l] = buf[lpad(e)];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t e = r * kBlock + threadIdx.x;
+        if (e < T) a[base + ((uint64_t)(e >> c2) << lo) + (e & ((1u << c2) - 1))] = buf[lpad(e)];
     }
 }
 
@@ -387,12 +397,6 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
         if (dense) c = kDense;
         else if (!full) c = (n == 1) ? kDirect : kWht;
         else c = (n == F) ? kWht : (n + 1 <= nd ? kSparseSum : kWht);
-        G = G_cap = 1;
-        n_wht = (c == kWht) ? 1 : 0;
-        start = dalloc<uint64_t>(2, st.stream);
-        cls = dalloc<uint8_t>(1, st.stream);
-        single_group_kernel<<<1, 1, 0, st.stream>>>(start, cls, n, c, counts);
-        SPQ_HIP(hipGetLastError());
         if (c == kDirect) {
             uint64_t* kout;
             double2* aout;
@@ -400,16 +404,21 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
             {
                 SPQ_PROF(st.stream, "expand_single_kernel", F * (8.0 * W + 16));
                 expand_single_kernel<W><<<grid_for(F, kBlock * 8), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, low, nd,
-                                                                                         1.0 / std::sqrt((double)F), kout, aout, F);
+                                                                                         1.0 / std::sqrt((double)F), kout, aout, F,
+                                                                                         eps, counts);
             }
             SPQ_HIP(hipGetLastError());
-            dfree(start, st.stream);
-            dfree(cls, st.stream);
+            const uint64_t pruned = read_u64(st, counts);
             dfree(counts, st.stream);
-            install_arrays(st, st.W, kout, aout, F, F);
-            clear_zero(st, eps);
+            install_arrays(st, st.W, kout, aout, pruned ? 0 : F, F);
             return;
         }
+        G = G_cap = 1;
+        n_wht = (c == kWht) ? 1 : 0;
+        start = dalloc<uint64_t>(2, st.stream);
+        cls = dalloc<uint8_t>(1, st.stream);
+        single_group_kernel<<<1, 1, 0, st.stream>>>(start, cls, n, c, counts);
+        SPQ_HIP(hipGetLastError());
         if (c == kWht && n == F) {
             // One full group in target order: the output keys are the input keys,
             // so the butterflies run in place on the amplitudes.

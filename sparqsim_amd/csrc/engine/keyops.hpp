@@ -17,17 +17,29 @@ __device__ __forceinline__ uint64_t fmask(uint32_t width) {
 }
 
 // Select word w of a register-resident key without dynamic register indexing.
+// Word w of a register-resident key.  Masks rather than selects: a select chain
+// over the words can be folded back into an indexed array, which lives in
+// scratch memory.
+__device__ __forceinline__ uint64_t kword_mask(int w, int i) { return uint64_t(0) - (uint64_t)(w == i); }
 template <int W>
 __device__ __forceinline__ uint64_t kword(const uint64_t (&k)[W], int w) {
-    uint64_t r = k[0];
+    if (W == 1) return k[0];   // a field of a one-word key is in word 0
+    uint64_t r = 0;
 #pragma unroll
-    for (int i = 1; i < W; ++i) r = (w == i) ? k[i] : r;
+    for (int i = 0; i < W; ++i) r |= k[i] & kword_mask(w, i);
     return r;
 }
 template <int W>
 __device__ __forceinline__ void kset_word(uint64_t (&k)[W], int w, uint64_t v) {
+    if (W == 1) {
+        k[0] = v;
+        return;
+    }
 #pragma unroll
-    for (int i = 0; i < W; ++i) k[i] = (w == i) ? v : k[i];
+    for (int i = 0; i < W; ++i) {
+        const uint64_t m = kword_mask(w, i);
+        k[i] = (k[i] & ~m) | (v & m);
+    }
 }
 template <int W>
 __device__ __forceinline__ uint64_t fget(const uint64_t (&k)[W], sparq_field_t f) {

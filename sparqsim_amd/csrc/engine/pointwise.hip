@@ -83,7 +83,9 @@ struct Tridiag {
     }
 };
 
-template <int W>
+// LIGHT: the heavy opcodes (is_heavy_opcode) compile to nothing, so a program
+// of light opcodes runs with far fewer registers per lane.
+template <int W, bool LIGHT>
 __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W], double2& amp) {
     const sparq_field_t f0 = op.f[0], f1 = op.f[1], f2 = op.f[2], f3 = op.f[3];
     switch (op.opcode) {
@@ -122,7 +124,11 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
         }
         case SPARQ_PW_XOR_MUL_CONST: fput<W>(k, f1, fget<W>(k, f1) ^ (fget<W>(k, f0) * op.c[0])); break;
         case SPARQ_PW_ADD_MUL_CONST: fput<W>(k, f1, fget<W>(k, f1) + op.c[0] * fget<W>(k, f0)); break;
-        case SPARQ_PW_MODMUL_CONST: fput<W>(k, f0, (fget<W>(k, f0) * op.c[0]) % op.c[1]); break;
+        case SPARQ_PW_MODMUL_CONST:
+            if constexpr (!LIGHT) {
+                fput<W>(k, f0, (fget<W>(k, f0) * op.c[0]) % op.c[1]);
+            }
+            break;
         case SPARQ_PW_XOR_ADD: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) + fget<W>(k, f1))); break;
         case SPARQ_PW_ADD_FIELD: fput<W>(k, f1, fget<W>(k, f1) + fget<W>(k, f0)); break;
         case SPARQ_PW_SUB_FIELD: fput<W>(k, f1, fget<W>(k, f1) - fget<W>(k, f0)); break;
@@ -137,12 +143,17 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
             break;
         }
         case SPARQ_PW_XOR_MUL: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) * fget<W>(k, f1))); break;
-        case SPARQ_PW_XOR_DIV: {
-            uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
-            fput<W>(k, f2, fget<W>(k, f2) ^ (b == 0 ? 0 : a / b));
+        case SPARQ_PW_XOR_DIV:
+            if constexpr (!LIGHT) {
+                uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
+                fput<W>(k, f2, fget<W>(k, f2) ^ (b == 0 ? 0 : a / b));
+            }
             break;
-        }
-        case SPARQ_PW_XOR_ISQRT: fput<W>(k, f1, fget<W>(k, f1) ^ isqrt_u64(fget<W>(k, f0))); break;
+        case SPARQ_PW_XOR_ISQRT:
+            if constexpr (!LIGHT) {
+                fput<W>(k, f1, fget<W>(k, f1) ^ isqrt_u64(fget<W>(k, f0)));
+            }
+            break;
         case SPARQ_PW_XOR_SELECT: {
             uint64_t sel = (fget<W>(k, f0) & 1) ? fget<W>(k, f1) : fget<W>(k, f2);
             fput<W>(k, f3, fget<W>(k, f3) ^ sel);
@@ -173,14 +184,15 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
             fput<W>(k, f3, fget<W>(k, f3) ^ (((sa == sb) && (ss != sa)) ? 1 : 0));
             break;
         }
-        case SPARQ_PW_FLAG_MULOVERFLOW: {
-            uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
-            uint64_t lo = a * b, hi = __umul64hi(a, b);
-            uint32_t w = f2.width;
-            bool p = hi != 0 || (w < 64 && lo >= (uint64_t(1) << w));
-            fput<W>(k, f3, fget<W>(k, f3) ^ (p ? 1 : 0));
+        case SPARQ_PW_FLAG_MULOVERFLOW:
+            if constexpr (!LIGHT) {
+                uint64_t a = fget<W>(k, f0), b = fget<W>(k, f1);
+                uint64_t lo = a * b, hi = __umul64hi(a, b);
+                uint32_t w = f2.width;
+                bool p = hi != 0 || (w < 64 && lo >= (uint64_t(1) << w));
+                fput<W>(k, f3, fget<W>(k, f3) ^ (p ? 1 : 0));
+            }
             break;
-        }
         case SPARQ_PW_FLAG_ISZERO: fput<W>(k, f1, fget<W>(k, f1) ^ (fget<W>(k, f0) == 0 ? 1 : 0)); break;
         case SPARQ_PW_FLAG_NEGATIVE: fput<W>(k, f1, fget<W>(k, f1) ^ (sext(fget<W>(k, f0), f0.width) < 0 ? 1 : 0)); break;
         case SPARQ_PW_ADD_ANY:
@@ -200,28 +212,31 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
         }
         case SPARQ_PW_FLAG_LESS_U: fput<W>(k, f2, fget<W>(k, f2) ^ (fget<W>(k, f0) < fget<W>(k, f1) ? 1 : 0)); break;
         case SPARQ_PW_XOR_MID: fput<W>(k, f2, fget<W>(k, f2) ^ ((fget<W>(k, f0) + fget<W>(k, f1)) / 2)); break;
-        case SPARQ_PW_XOR_DIVSQRT_ACOS: {
-            double o = acos(sqrt(1.0 * fget<W>(k, f0) / fget<W>(k, f1))) / kPi / 2;
-            fput<W>(k, f2, fget<W>(k, f2) ^ rational_bits(o, f2.width));
-            break;
-        }
-        case SPARQ_PW_XOR_SQRTDIV_ACOS: {
-            double o = acos(sext(fget<W>(k, f0), f0.width) / sqrt((double)fget<W>(k, f1))) / kPi / 2;
-            fput<W>(k, f2, fget<W>(k, f2) ^ rational_bits(o, f2.width));
-            break;
-        }
-        case SPARQ_PW_XOR_ROTATE_ANGLE: {
-            int64_t l = sext(fget<W>(k, f0), f0.width), r = sext(fget<W>(k, f1), f1.width);
-            double o;
-            if (l == 0 && r >= 0) o = 0.25;
-            else if (l == 0 && r < 0) o = 0.75;
-            else {
-                o = atan2((double)r, (double)l) / kPi / 2;
-                if (o < 0) o += 1;
+        case SPARQ_PW_XOR_DIVSQRT_ACOS:
+            if constexpr (!LIGHT) {
+                double o = acos(sqrt(1.0 * fget<W>(k, f0) / fget<W>(k, f1))) / kPi / 2;
+                fput<W>(k, f2, fget<W>(k, f2) ^ rational_bits(o, f2.width));
             }
-            fput<W>(k, f2, fget<W>(k, f2) ^ rational_bits(o, f2.width));
             break;
-        }
+        case SPARQ_PW_XOR_SQRTDIV_ACOS:
+            if constexpr (!LIGHT) {
+                double o = acos(sext(fget<W>(k, f0), f0.width) / sqrt((double)fget<W>(k, f1))) / kPi / 2;
+                fput<W>(k, f2, fget<W>(k, f2) ^ rational_bits(o, f2.width));
+            }
+            break;
+        case SPARQ_PW_XOR_ROTATE_ANGLE:
+            if constexpr (!LIGHT) {
+                int64_t l = sext(fget<W>(k, f0), f0.width), r = sext(fget<W>(k, f1), f1.width);
+                double o;
+                if (l == 0 && r >= 0) o = 0.25;
+                else if (l == 0 && r < 0) o = 0.75;
+                else {
+                    o = atan2((double)r, (double)l) / kPi / 2;
+                    if (o < 0) o += 1;
+                }
+                fput<W>(k, f2, fget<W>(k, f2) ^ rational_bits(o, f2.width));
+            }
+            break;
         case SPARQ_PW_QRAM_LOAD: {
             uint64_t a = fget<W>(k, f0) & fmask((uint32_t)op.c[0]);
             fput<W>(k, f1, fget<W>(k, f1) ^ table_word(op.table, op.table_word_bytes, a));
@@ -233,54 +248,59 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
             fput<W>(k, f3, fget<W>(k, f3) ^ a);
             break;
         }
-        case SPARQ_PW_XOR_BSEARCH: {
-            const uint64_t hit = table_search(op.table, op.table_word_bytes, op.c[2], fget<W>(k, f0), op.c[0], op.c[1], fget<W>(k, f1));
-            fput<W>(k, f2, fget<W>(k, f2) ^ hit);
-            break;
-        }
-        case SPARQ_PW_XOR_QW_ANGLE: {
-            const uint32_t ds = (uint32_t)op.c[0];
-            uint64_t v = fget<W>(k, f0);
-            double ratio;
-            if (op.c[1]) {
-                const uint64_t amax = fmask(ds);
-                ratio = (double)(v & amax) * 1.0 / (double)amax;
-            } else {
-                const int64_t vr = sext(v & fmask(ds), ds);
-                ratio = (double)(vr < 0 ? -vr : vr) * 1.0 / (double)fmask(ds - 1);
+        case SPARQ_PW_XOR_BSEARCH:
+            if constexpr (!LIGHT) {
+                const uint64_t hit = table_search(op.table, op.table_word_bytes, op.c[2], fget<W>(k, f0), op.c[0], op.c[1], fget<W>(k, f1));
+                fput<W>(k, f2, fget<W>(k, f2) ^ hit);
             }
-            ratio = fmax(0.0, fmin(1.0, ratio));
-            const double o = acos(sqrt(ratio)) / kPi / 2;
-            fput<W>(k, f1, fget<W>(k, f1) ^ rational_bits(o, f1.width));
             break;
-        }
-        case SPARQ_PW_TRIDIAG_LOAD: {
-            const Tridiag m(op);
-            fput<W>(k, f3, fget<W>(k, f3) ^ m.mem(fget<W>(k, f0) + 4 * fget<W>(k, f1) + fget<W>(k, f2)));
+        case SPARQ_PW_XOR_QW_ANGLE:
+            if constexpr (!LIGHT) {
+                const uint32_t ds = (uint32_t)op.c[0];
+                uint64_t v = fget<W>(k, f0);
+                double ratio;
+                if (op.c[1]) {
+                    const uint64_t amax = fmask(ds);
+                    ratio = (double)(v & amax) * 1.0 / (double)amax;
+                } else {
+                    const int64_t vr = sext(v & fmask(ds), ds);
+                    ratio = (double)(vr < 0 ? -vr : vr) * 1.0 / (double)fmask(ds - 1);
+                }
+                ratio = fmax(0.0, fmin(1.0, ratio));
+                const double o = acos(sqrt(ratio)) / kPi / 2;
+                fput<W>(k, f1, fget<W>(k, f1) ^ rational_bits(o, f1.width));
+            }
             break;
-        }
-        case SPARQ_PW_TRIDIAG_COL2SLOT: {
-            const Tridiag m(op);
-            const uint64_t j = fget<W>(k, f1), col = fget<W>(k, f2), sr = fget<W>(k, f3);
-            const uint64_t base = m.base(j), row_addr = fget<W>(k, f0) + 4 * j;
-            const bool hit = col >= base && col < base + 4;
-            const uint64_t sr1 = sr ^ (hit ? row_addr + (col - base) : 0);
-            const uint64_t k1 = col ^ m.mem(sr1);
-            fput<W>(k, f2, (sr1 - row_addr) & m.kmask);
-            fput<W>(k, f3, k1 & m.kmask);
+        case SPARQ_PW_TRIDIAG_LOAD:
+            if constexpr (!LIGHT) {
+                const Tridiag m(op);
+                fput<W>(k, f3, fget<W>(k, f3) ^ m.mem(fget<W>(k, f0) + 4 * fget<W>(k, f1) + fget<W>(k, f2)));
+            }
             break;
-        }
-        case SPARQ_PW_TRIDIAG_SLOT2COL: {
-            const Tridiag m(op);
-            const uint64_t j = fget<W>(k, f1), slot = fget<W>(k, f2), sr = fget<W>(k, f3);
-            const uint64_t base = m.base(j), row_addr = fget<W>(k, f0) + 4 * j;
-            const uint64_t k1 = (slot + row_addr) & m.kmask;
-            const uint64_t k2 = sr ^ m.mem(k1);
-            const bool hit = k2 >= base && k2 < base + 4;
-            fput<W>(k, f2, k2);
-            fput<W>(k, f3, k1 ^ (hit ? row_addr + (k2 - base) : 0));
+        case SPARQ_PW_TRIDIAG_COL2SLOT:
+            if constexpr (!LIGHT) {
+                const Tridiag m(op);
+                const uint64_t j = fget<W>(k, f1), col = fget<W>(k, f2), sr = fget<W>(k, f3);
+                const uint64_t base = m.base(j), row_addr = fget<W>(k, f0) + 4 * j;
+                const bool hit = col >= base && col < base + 4;
+                const uint64_t sr1 = sr ^ (hit ? row_addr + (col - base) : 0);
+                const uint64_t k1 = col ^ m.mem(sr1);
+                fput<W>(k, f2, (sr1 - row_addr) & m.kmask);
+                fput<W>(k, f3, k1 & m.kmask);
+            }
             break;
-        }
+        case SPARQ_PW_TRIDIAG_SLOT2COL:
+            if constexpr (!LIGHT) {
+                const Tridiag m(op);
+                const uint64_t j = fget<W>(k, f1), slot = fget<W>(k, f2), sr = fget<W>(k, f3);
+                const uint64_t base = m.base(j), row_addr = fget<W>(k, f0) + 4 * j;
+                const uint64_t k1 = (slot + row_addr) & m.kmask;
+                const uint64_t k2 = sr ^ m.mem(k1);
+                const bool hit = k2 >= base && k2 < base + 4;
+                fput<W>(k, f2, k2);
+                fput<W>(k, f3, k1 ^ (hit ? row_addr + (k2 - base) : 0));
+            }
+            break;
         case SPARQ_PW_PHASE_BITS: {
             uint64_t bits = fget<W>(k, f0) & op.c[0];
             const double2 ph = make_double2(op.d[0], op.d[1]);
@@ -341,13 +361,15 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
 }
 
 template <int W>
-constexpr int kPwItems() { return W <= 2 ? 4 : (W <= 4 ? 2 : 1); }
+constexpr int kPwItems() { return (W == 3 || W == 4) ? 2 : 1; }
 
-template <int W, bool AMP>
+template <int W, bool AMP, bool LIGHT>
 __global__ void __launch_bounds__(kBlock) pointwise_kernel(uint64_t* __restrict__ keys, double2* __restrict__ amps, uint64_t cap,
                                                       uint64_t n, const PwProgram prog) {
-    // kPwItems(W) entries per thread (striped by the block size), so that the
-    // table gathers of several entries are in flight together.
+    // kPwItems(W) entries per thread (striped by the block size).  One entry
+    // per thread for one- and two-word keys: the interpreter's registers for
+    // several entries cut the waves per SIMD more than the extra gathers in
+    // flight gain (23 vs 27 us for the 2^20 QRAM-load + 32-phase program).
     constexpr int kI = kPwItems<W>();
     const uint64_t base = (uint64_t)blockIdx.x * kBlock * kI + threadIdx.x;
     uint64_t k[kI][W];
@@ -370,7 +392,7 @@ __global__ void __launch_bounds__(kBlock) pointwise_kernel(uint64_t* __restrict_
         for (int it = 0; it < kI; ++it) {
             bool ok = true;
             for (uint32_t c = 0; c < op.ncond; ++c) ok = ok && cond_ok<W>(k[it], prog.conds[op.cond_first + c]);
-            if (ok) run_op<W>(op, k[it], a[it]);
+            if (ok) run_op<W, LIGHT>(op, k[it], a[it]);
         }
     }
 #pragma unroll
@@ -379,6 +401,20 @@ __global__ void __launch_bounds__(kBlock) pointwise_kernel(uint64_t* __restrict_
         if (i >= n) break;
         store_key<W>(keys, cap, i, k[it]);
         if (AMP) amps[i] = a[it];
+    }
+}
+
+// Opcodes with 64-bit division, square roots, arc cosines or table searches:
+// their registers set the occupancy of every program that may contain them.
+static bool is_heavy_opcode(uint32_t opcode) {
+    switch (opcode) {
+        case SPARQ_PW_MODMUL_CONST: case SPARQ_PW_XOR_DIV: case SPARQ_PW_XOR_ISQRT: case SPARQ_PW_FLAG_MULOVERFLOW:
+        case SPARQ_PW_XOR_DIVSQRT_ACOS: case SPARQ_PW_XOR_SQRTDIV_ACOS: case SPARQ_PW_XOR_ROTATE_ANGLE:
+        case SPARQ_PW_XOR_BSEARCH: case SPARQ_PW_XOR_QW_ANGLE: case SPARQ_PW_TRIDIAG_LOAD: case SPARQ_PW_TRIDIAG_COL2SLOT:
+        case SPARQ_PW_TRIDIAG_SLOT2COL:
+            return true;
+        default:
+            return false;
     }
 }
 
@@ -393,11 +429,16 @@ static bool touches_amp(uint32_t opcode) {
 }
 
 template <int W>
-static void launch_pointwise(State& st, const PwProgram& prog, bool amp) {
+static void launch_pointwise(State& st, const PwProgram& prog, bool amp, bool light) {
     unsigned g = grid_for(st.n, kBlock * kPwItems<W>());
     SPQ_PROF(st.stream, "pointwise_kernel", st.n * (16.0 * W + (amp ? 32.0 : 0.0)));
-    if (amp) pointwise_kernel<W, true><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
-    else pointwise_kernel<W, false><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
+    if (light) {
+        if (amp) pointwise_kernel<W, true, true><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
+        else pointwise_kernel<W, false, true><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
+    } else {
+        if (amp) pointwise_kernel<W, true, false><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
+        else pointwise_kernel<W, false, false><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
+    }
     SPQ_HIP(hipGetLastError());
 }
 
@@ -412,7 +453,7 @@ void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_c
     PwProgram prog{};
     prog.nops = nops;
     prog.nconds = nconds;
-    bool amp = false;
+    bool amp = false, light = true;
     for (uint32_t o = 0; o < nops; ++o) {
         const sparq_pw_op_t& op = ops[o];
         SPQ_REQUIRE(op.opcode >= SPARQ_PW_FLIP_BIT && op.opcode < SPARQ_PW_OPCODE_END, "unknown pointwise opcode");
@@ -431,6 +472,7 @@ void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_c
                             op.table_word_bytes == 8, "bad table word size");
         }
         amp = amp || touches_amp(op.opcode);
+        light = light && !is_heavy_opcode(op.opcode);
         prog.ops[o] = op;
     }
     for (uint32_t c = 0; c < nconds; ++c) {
@@ -438,7 +480,7 @@ void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_c
         prog.conds[c] = conds[c];
     }
     if (st.n == 0 || nops == 0) return;
-    SPQ_DISPATCH_W(st.W, launch_pointwise, st, prog, amp);
+    SPQ_DISPATCH_W(st.W, launch_pointwise, st, prog, amp, light);
 }
 
 }  // namespace sparq
