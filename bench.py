@@ -23,9 +23,9 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per launch at 2^20 entries / 32-bit data (rocprofv3 --pmc FETCH_SIZE x2.0 +
 # WRITE_SIZE x1.0, factors calibrated on a 512 MiB device copy); profiles/qram_interference_2p20_traffic.md.
 PMC_TRAFFIC_MIB_2P20 = {
-    "pointwise_kernel": 28.4 + 24.0, "compact_scatter_kernel": 24.1 + 24.0, "compact_count_kernel": 16.0,
-    "key_stats_kernel": 8.2, "expand_single_kernel": 28.0, "wht_first_pass_kernel": 32.1 + 32.0,
-    "wht_pass_kernel": 22.9 + 24.9,
+    "pointwise_kernel": 28.1 + 24.0, "compact_scatter_kernel": 24.1 + 24.0, "compact_count_kernel": 16.0,
+    "key_stats_kernel": 8.2, "expand_single_kernel": 0.0 + 24.0, "wht_first_pass_kernel": 16.1 + 16.0,
+    "wht_pass_kernel": 16.1 + 16.0,
 }
 
 

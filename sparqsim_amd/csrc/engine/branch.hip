@@ -238,17 +238,16 @@ __device__ __forceinline__ void lds_wht_rounds(double2* buf, uint32_t T, uint32_
 
 // First pass: stages [0, min(nd, 11)) on contiguous tiles of 2^11 elements of the
 // output array (a tile may hold many small groups; non-WHT groups are skipped).
-// G_dev = {group count, WHT group count}; tiles past G << nd do nothing.
+// G_dev = {group count, WHT group count} on the device (tiles past G << nd do
+// nothing), or null when all G_cap groups are WHT groups.  The tile's loads are
+// issued before the device-side counts are checked: every tile lies inside the
+// G_cap << nd allocated elements, and a tile that fails the check discards them.
 __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restrict__ a, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
                                                            uint32_t nd, uint32_t cnt, const uint8_t* __restrict__ cls) {
     __shared__ double2 buf[lpad(1 << kWhtTileLog)];
-    const uint64_t G = G_dev[0];
-    if (G > G_cap || G_dev[1] == 0) return;
-    const uint64_t M = G << nd;
     const uint64_t base = (uint64_t)blockIdx.x << kWhtTileLog;
-    if (base >= M) return;
-    if (nd >= (uint32_t)kWhtTileLog && cls[base >> nd] != kWht) return;   // the tile lies in one non-WHT group
-    const uint32_t T = (uint32_t)min<uint64_t>(uint64_t(1) << kWhtTileLog, M - base);
+    const uint64_t M_cap = G_cap << nd;
+    const uint32_t T_cap = (uint32_t)min<uint64_t>(uint64_t(1) << kWhtTileLog, M_cap - base);
     // all loads of the tile are issued before the first LDS store; a partial
     // tile loads its element 0 in place of the missing ones (unconditional
     // loads keep v in registers)
@@ -257,7 +256,16 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
         const uint32_t e = r * kBlock + threadIdx.x;
-        v[r] = a[base + (e < T ? e : 0)];
+        v[r] = a[base + (e < T_cap ? e : 0)];
+    }
+    uint32_t T = T_cap;
+    if (G_dev) {
+        const uint64_t G = G_dev[0];
+        if (G > G_cap || G_dev[1] == 0) return;
+        const uint64_t M = G << nd;
+        if (base >= M) return;
+        if (nd >= (uint32_t)kWhtTileLog && cls[base >> nd] != kWht) return;   // the tile lies in one non-WHT group
+        T = (uint32_t)min<uint64_t>(uint64_t(1) << kWhtTileLog, M - base);
     }
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
@@ -266,7 +274,7 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
     }
     __syncthreads();
     // a tile inside one group was checked above; smaller groups are checked per round
-    lds_wht_rounds(buf, T, 0, cnt, nd < (uint32_t)kWhtTileLog, base, nd, cls);
+    lds_wht_rounds(buf, T, 0, cnt, G_dev != nullptr && nd < (uint32_t)kWhtTileLog, base, nd, cls);
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
         const uint32_t e = r * kBlock + threadIdx.x;
@@ -278,23 +286,30 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
 // elements (c2 contiguous low index bits, c2 <= lo) of up to 2^kWhtPassTileLog
 // elements (64 KiB of LDS).
 constexpr int kWhtPassTileLog = 11;
+constexpr uint32_t kXcds = 8;
 __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ a, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
                                                      uint32_t nd, uint32_t lo, uint32_t cnt, uint32_t c2, const uint8_t* __restrict__ cls) {
     __shared__ double2 buf[lpad(1 << kWhtPassTileLog)];
     const uint32_t T = 1u << (cnt + c2);
-    const uint64_t tile = blockIdx.x;
+    // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+    // L2.  With rows of 2^c2 < 8 elements (shorter than a 128-byte line)
+    // neighbouring tiles share the lines of their rows, so consecutive tiles go
+    // to the workgroups of one XCD (blockIdx.x, blockIdx.x + 8, ...): the 2^20
+    // pass then reads its 16 MiB once instead of twice.  Longer rows keep the
+    // plain order, which spreads the XCDs' accesses over the array.
+    const uint32_t nb = gridDim.x;
+    const uint64_t tile = (c2 < 3 && nb % kXcds == 0) ? (uint64_t)(blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds
+                                                      : blockIdx.x;
     const uint64_t per_group = (uint64_t(1) << nd) >> (cnt + c2);
     const uint64_t g = tile / per_group;
     const uint64_t r = tile % per_group;
-    const uint64_t G = G_dev[0];
-    if (G > G_cap || g >= G || cls[g] != kWht) return;
     // r enumerates (high bits above lo+cnt) x (low chunk index below lo, in units of 2^c2)
     const uint64_t nlowchunks = (uint64_t(1) << lo) >> c2;
     const uint64_t lowchunk = r % nlowchunks, high = r / nlowchunks;
     const uint64_t base = (g << nd) + (high << (lo + cnt)) + (lowchunk << c2);
     // element e of the tile: sub = e >> c2 (stage index bits), l = e & (2^c2-1);
-    // loads in groups of 16 per thread (a whole 2^12 tile in one group), all
-    // issued before their LDS stores
+    // the whole tile is loaded before its LDS stores, and before the group is
+    // checked (g < G_cap: the tile lies inside the allocated elements)
     constexpr int kPer = (1 << kWhtPassTileLog) / kBlock;   // T <= kPer * kBlock
     {
         double2 v[kPer];
@@ -303,6 +318,10 @@ __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ 
             const uint32_t e0 = r * kBlock + threadIdx.x;
             const uint32_t e = e0 < T ? e0 : 0;   // unconditional loads keep v in registers
             v[r] = a[base + ((uint64_t)(e >> c2) << lo) + (e & ((1u << c2) - 1))];
+        }
+        if (G_dev) {
+            const uint64_t G = G_dev[0];
+            if (G > G_cap || g >= G || cls[g] != kWht) return;
         }
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
@@ -336,7 +355,7 @@ __global__ void sized_total_kernel(uint64_t* __restrict__ counts, uint64_t G_cap
 }
 
 // Butterfly passes over G_cap blocks of 2^nd amplitudes (the WHT groups among
-// the first counts[0] blocks): the first pass covers min(nd, 11) stages on
+// the first counts[0] blocks; counts null: all G_cap blocks): the first pass covers min(nd, 11) stages on
 // contiguous tiles; the rest are split evenly into passes of <= 9 stages on
 // column tiles of one group (2^cnt rows of 2^c2 contiguous elements, up to
 // 2^12 elements).
@@ -413,22 +432,21 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
             install_arrays(st, st.W, kout, aout, pruned ? 0 : F, F);
             return;
         }
+        if (c == kWht && n == F) {
+            // One full group in target order: the output keys are the input keys,
+            // so the butterflies run in place on the amplitudes (one WHT group,
+            // known on the host).
+            dfree(counts, st.stream);
+            launch_wht(st, st.amps, nullptr, 1, nd, nullptr, (double)F * 32.0);
+            clear_zero(st, eps);
+            return;
+        }
         G = G_cap = 1;
         n_wht = (c == kWht) ? 1 : 0;
         start = dalloc<uint64_t>(2, st.stream);
         cls = dalloc<uint8_t>(1, st.stream);
         single_group_kernel<<<1, 1, 0, st.stream>>>(start, cls, n, c, counts);
         SPQ_HIP(hipGetLastError());
-        if (c == kWht && n == F) {
-            // One full group in target order: the output keys are the input keys,
-            // so the butterflies run in place on the amplitudes.
-            launch_wht(st, st.amps, counts, 1, nd, cls, (double)F * 32.0);
-            dfree(start, st.stream);
-            dfree(cls, st.stream);
-            dfree(counts, st.stream);
-            clear_zero(st, eps);
-            return;
-        }
     } else {
         head = dalloc<uint32_t>(n, st.stream);
         pos = dalloc<uint64_t>(n + 2, st.stream);   // [n] = group count, [n+1] = WHT group count

@@ -338,11 +338,19 @@ __global__ void __launch_bounds__(kBlock) radix_hist_kernel(const uint64_t* __re
     __syncthreads();
     const uint64_t tile = (uint64_t)blockIdx.x * kSortTile;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // all loads of the tile first (clamped in-bounds indices: no per-item wait)
+    uint64_t key[kSortItems];
+#pragma unroll
     for (int it = 0; it < kSortItems; ++it) {
-        uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
-        uint32_t d = e < n ? digit_of(kw[e], dg) : kInvalid;
-        uint64_t peers = match_any<RB>(d, dg.bits);
-        int leader = __ffsll((unsigned long long)peers) - 1;
+        const uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
+        key[it] = kw[e < n ? e : n - 1];
+    }
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
+        const uint32_t d = e < n ? digit_of(key[it], dg) : kInvalid;
+        const uint64_t peers = match_any<RB>(d, dg.bits);
+        const int leader = __ffsll((unsigned long long)peers) - 1;
         if (lane == leader && d != kInvalid) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
@@ -379,20 +387,34 @@ __global__ void __launch_bounds__(kBlock) radix_scatter_kernel(const uint64_t* _
     uint32_t idx[kSortItems];
     uint32_t rank[kSortItems];
     uint32_t dig[kSortItems];
+    // all loads of the tile first (clamped in-bounds indices: no per-item wait)
 #pragma unroll
     for (int it = 0; it < kSortItems; ++it) {
-        uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
-        bool valid = e < n;
-        key[it] = valid ? kw_in[e] : 0;
-        idx[it] = valid ? (idx_in ? idx_in[e] : (uint32_t)e) : 0;
-        uint32_t d = valid ? digit_of(key[it], dg) : kInvalid;
-        dig[it] = d;
-        uint64_t peers = match_any<RB>(d, dg.bits);
-        uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
-        uint32_t base = valid ? wcnt[wave][d & (kR - 1)] : 0;
-        rank[it] = base + before;
-        if (valid && before == 0) wcnt[wave][d & (kR - 1)] = base + (uint32_t)__popcll(peers);
+        const uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
+        const uint64_t ec = e < n ? e : n - 1;
+        key[it] = kw_in[ec];
+        idx[it] = idx_in ? idx_in[ec] : (uint32_t)ec;
     }
+    // Stable per-wave ranking: items in (it, lane) order, which is entry order.
+    // The leader of each group of equal digits adds the group's size to the
+    // wave's digit counter with one LDS atomic that returns the old count; a
+    // wave's LDS operations complete in issue order, so item it+1 sees item
+    // it's additions and the 16 atomics need no wait between them.
+    uint32_t before[kSortItems], got[kSortItems];
+    int leader[kSortItems];
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
+        const bool valid = e < n;
+        const uint32_t d = valid ? digit_of(key[it], dg) : kInvalid;
+        dig[it] = d;
+        const uint64_t peers = match_any<RB>(d, dg.bits);
+        before[it] = (uint32_t)__popcll(peers & lanemask_lt());
+        leader[it] = __ffsll((unsigned long long)peers) - 1;
+        got[it] = (valid && before[it] == 0) ? atomicAdd(&wcnt[wave][d & (kR - 1)], (uint32_t)__popcll(peers)) : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) rank[it] = (uint32_t)__shfl((int)got[it], leader[it], 64) + before[it];
     __syncthreads();
     // Digit totals and their exclusive scan over digits; per-wave offsets.
     uint32_t c[kPerThread][4];
@@ -445,23 +467,45 @@ __global__ void __launch_bounds__(kBlock) gather_word_kernel(const uint64_t* __r
     if (i < n) dst[i] = src[idx[i]];
 }
 
+constexpr int kGatherItems = 4;
+
 template <int W>
 __global__ void __launch_bounds__(kBlock) gather_state_kernel(const uint64_t* __restrict__ kin, const double2* __restrict__ ain, uint64_t cap_in,
                                                          const uint32_t* __restrict__ idx, uint64_t n, uint64_t* __restrict__ kout,
                                                          double2* __restrict__ aout, uint64_t cap_out) {
-    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s = idx[i];
+    // kGatherItems entries per thread (striped by the block size): their index
+    // loads, then their key and amplitude loads, are in flight together.
+    const uint64_t base = (uint64_t)blockIdx.x * kBlock * kGatherItems + threadIdx.x;
+    uint32_t s[kGatherItems];
 #pragma unroll
-    for (int w = 0; w < W; ++w) kout[(uint64_t)w * cap_out + i] = kin[(uint64_t)w * cap_in + s];
-    aout[i] = ain[s];
+    for (int it = 0; it < kGatherItems; ++it) {
+        const uint64_t i = base + (uint64_t)it * kBlock;
+        s[it] = idx[i < n ? i : n - 1];
+    }
+    uint64_t k[kGatherItems][W];
+    double2 a[kGatherItems];
+#pragma unroll
+    for (int it = 0; it < kGatherItems; ++it) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) k[it][w] = kin[(uint64_t)w * cap_in + s[it]];
+        a[it] = ain[s[it]];
+    }
+#pragma unroll
+    for (int it = 0; it < kGatherItems; ++it) {
+        const uint64_t i = base + (uint64_t)it * kBlock;
+        if (i >= n) break;
+#pragma unroll
+        for (int w = 0; w < W; ++w) kout[(uint64_t)w * cap_out + i] = k[it][w];
+        aout[i] = a[it];
+    }
 }
 
 template <int W>
 static void launch_gather_state(State& st, const uint32_t* idx, uint64_t* kout, double2* aout, uint64_t cap_out) {
     {
         SPQ_PROF(st.stream, "gather_state_kernel", st.n * (4.0 + 2 * (8.0 * W + 16)));
-        gather_state_kernel<W><<<grid_for(st.n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, idx, st.n, kout, aout, cap_out);
+        gather_state_kernel<W><<<grid_for(st.n, kBlock * kGatherItems), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, idx, st.n, kout,
+                                                                                            aout, cap_out);
     }
     SPQ_HIP(hipGetLastError());
 }
