@@ -286,7 +286,6 @@ __global__ void __launch_bounds__(kBlock) wht_first_pass_kernel(double2* __restr
 // elements (c2 contiguous low index bits, c2 <= lo) of up to 2^kWhtPassTileLog
 // elements (64 KiB of LDS).
 constexpr int kWhtPassTileLog = 11;
-constexpr uint32_t kXcds = 8;
 __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ a, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
                                                      uint32_t nd, uint32_t lo, uint32_t cnt, uint32_t c2, const uint8_t* __restrict__ cls) {
     __shared__ double2 buf[lpad(1 << kWhtPassTileLog)];
@@ -297,9 +296,7 @@ __global__ void __launch_bounds__(kBlock) wht_pass_kernel(double2* __restrict__ 
     // to the workgroups of one XCD (blockIdx.x, blockIdx.x + 8, ...): the 2^20
     // pass then reads its 16 MiB once instead of twice.  Longer rows keep the
     // plain order, which spreads the XCDs' accesses over the array.
-    const uint32_t nb = gridDim.x;
-    const uint64_t tile = (c2 < 3 && nb % kXcds == 0) ? (uint64_t)(blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds
-                                                      : blockIdx.x;
+    const uint64_t tile = c2 < 3 ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint64_t per_group = (uint64_t(1) << nd) >> (cnt + c2);
     const uint64_t g = tile / per_group;
     const uint64_t r = tile % per_group;

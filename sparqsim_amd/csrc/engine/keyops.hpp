@@ -17,6 +17,17 @@ __device__ __forceinline__ uint64_t fmask(uint32_t width) {
 }
 
 // Select word w of a register-resident key without dynamic register indexing.
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx.x % 8), each
+// with its own L2.  xcd_tile maps block b to a tile index so that each XCD gets
+// a contiguous range of tiles (XCD x: tiles [x*q + min(x, r), ...) with
+// q = nb / 8, r = nb % 8; a bijection on [0, nb)).  Tiles whose outputs or
+// inputs share cache lines with their neighbours then meet in one L2.
+constexpr uint32_t kXcds = 8;
+__host__ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+    const uint32_t x = b % kXcds, j = b / kXcds, q = nb / kXcds, r = nb % kXcds;
+    return x * q + min(x, r) + j;
+}
+
 // Word w of a register-resident key.  Masks rather than selects: a select chain
 // over the words can be folded back into an indexed array, which lives in
 // scratch memory.

@@ -16,8 +16,11 @@
 
 namespace sparq {
 
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kBlock * kSortItems;   // 4096 entries per workgroup
+constexpr int kSortTile = 4096;   // entries per workgroup of a digit pass
+// Digit-pass workgroup sizes: 8 waves of 8 entries per lane for 8-bit digits;
+// 4 waves of 16 for 11-bit digits (whose per-wave digit counters fill the LDS).
+constexpr int kSortBlock8 = 512;
+constexpr int kSortBlock11 = 256;
 
 // Lanes of the wave holding the same value as the calling lane, for values of
 // `nbits` digit bits plus the invalid marker at bit RB.
@@ -327,70 +330,99 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, const Digit& dg) {
     return v;
 }
 
-// Block histogram: block_hist[d * nb + b] for digits of up to RB bits.
-template <int RB>
-__global__ void __launch_bounds__(kBlock) radix_hist_kernel(const uint64_t* __restrict__ kw, uint64_t n, Digit dg,
-                                                       uint32_t* __restrict__ block_hist, uint32_t nb) {
+// Block histogram, block-major: block_hist[b * 2^RB + d] for digits of up to RB
+// bits (one contiguous row per block; a digit-major layout would spread each
+// block's counts over 2^RB lines shared with other blocks on other XCDs).
+template <int RB, int BS>
+__global__ void __launch_bounds__(BS) radix_hist_kernel(const uint64_t* __restrict__ kw, uint64_t n, Digit dg,
+                                                   uint32_t* __restrict__ block_hist, uint32_t nb) {
     constexpr int kR = 1 << RB;
     constexpr uint32_t kInvalid = 1u << RB;
+    constexpr int NW = BS / 64, NI = kSortTile / BS;
     __shared__ uint32_t hist[kR];
-    for (int d = threadIdx.x; d < kR; d += kBlock) hist[d] = 0;
+    for (int d = threadIdx.x; d < kR; d += BS) hist[d] = 0;
     __syncthreads();
     const uint64_t tile = (uint64_t)blockIdx.x * kSortTile;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // all loads of the tile first (clamped in-bounds indices: no per-item wait)
-    uint64_t key[kSortItems];
+    uint64_t key[NI];
 #pragma unroll
-    for (int it = 0; it < kSortItems; ++it) {
-        const uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
+    for (int it = 0; it < NI; ++it) {
+        const uint64_t e = tile + (uint64_t)wave * (kSortTile / NW) + it * 64 + lane;
         key[it] = kw[e < n ? e : n - 1];
     }
 #pragma unroll
-    for (int it = 0; it < kSortItems; ++it) {
-        const uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
+    for (int it = 0; it < NI; ++it) {
+        const uint64_t e = tile + (uint64_t)wave * (kSortTile / NW) + it * 64 + lane;
         const uint32_t d = e < n ? digit_of(key[it], dg) : kInvalid;
         const uint64_t peers = match_any<RB>(d, dg.bits);
         const int leader = __ffsll((unsigned long long)peers) - 1;
         if (lane == leader && d != kInvalid) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < kR; d += kBlock) block_hist[(uint64_t)d * nb + blockIdx.x] = hist[d];
+    for (int d = threadIdx.x; d < kR; d += BS) block_hist[(uint64_t)blockIdx.x * kR + d] = hist[d];
+}
+
+// out[c * rows + r] = in[r * cols + c] through 32 x 32 LDS tiles (both sides
+// coalesced); turns the block-major histograms into the digit-major order of
+// the offset scan.
+__global__ void __launch_bounds__(kBlock) transpose_u32_kernel(const uint32_t* __restrict__ in, uint32_t rows, uint32_t cols,
+                                                          uint32_t* __restrict__ out) {
+    __shared__ uint32_t t[32][33];
+    const uint32_t c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+#pragma unroll
+    for (uint32_t k = 0; k < 32; k += 8) {
+        const uint32_t r = r0 + ty + k, c = c0 + tx;
+        if (r < rows && c < cols) t[ty + k][tx] = in[(uint64_t)r * cols + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < 32; k += 8) {
+        const uint32_t c = c0 + ty + k, r = r0 + tx;
+        if (r < rows && c < cols) out[(uint64_t)c * rows + r] = t[tx][ty + k];
+    }
 }
 
 // Stable scatter of one digit pass.  idx_in == nullptr means identity order.
-// Each thread owns kR / kBlock consecutive digits for the digit scan.
-template <int RB>
-__global__ void __launch_bounds__(kBlock) radix_scatter_kernel(const uint64_t* __restrict__ kw_in, const uint32_t* __restrict__ idx_in,
-                                                          uint64_t n, Digit dg, const uint64_t* __restrict__ offs, uint32_t nb,
-                                                          uint64_t* __restrict__ kw_out, uint32_t* __restrict__ idx_out,
-                                                          int write_keys) {
+// BS threads (BS / 64 waves) rank a tile of kSortTile entries; the digit scan
+// gives each of the first kR threads (or each thread kR / BS digits) its digits.
+template <int RB, int BS>
+__global__ void __launch_bounds__(BS) radix_scatter_kernel(const uint64_t* __restrict__ kw_in, const uint32_t* __restrict__ idx_in,
+                                                      uint64_t n, Digit dg, const uint64_t* __restrict__ offs, uint32_t nb,
+                                                      uint64_t* __restrict__ kw_out, uint32_t* __restrict__ idx_out,
+                                                      int write_keys) {
     constexpr int kR = 1 << RB;
     constexpr uint32_t kInvalid = 1u << RB;
-    constexpr int kPerThread = kR / kBlock;
-    static_assert(kR % kBlock == 0, "radix must be a multiple of the block size");
-    __shared__ uint32_t wcnt[4][kR];           // per-wave running digit counts
+    constexpr int NW = BS / 64, NI = kSortTile / BS;
+    constexpr int kPerThread = kR >= BS ? kR / BS : 1;
+    static_assert(kR % BS == 0 || BS % kR == 0, "radix and block size must divide one another");
+    __shared__ uint32_t wcnt[NW][kR];          // per-wave running digit counts
     __shared__ uint32_t dstart[kR];            // first tile position of each digit
     __shared__ uint64_t goff[kR];              // global position of tile position 0 of each digit
     __shared__ uint64_t skey[kSortTile];
     __shared__ uint32_t sidx[kSortTile];
-    __shared__ uint32_t scan_lds[kBlock / 64];
+    __shared__ uint32_t scan_lds[NW];
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int d = threadIdx.x; d < kR; d += kBlock) {
+    for (int d = threadIdx.x; d < kR; d += BS) {
 #pragma unroll
-        for (int w = 0; w < 4; ++w) wcnt[w][d] = 0;
+        for (int w = 0; w < NW; ++w) wcnt[w][d] = 0;
     }
     __syncthreads();
 
-    const uint64_t tile = (uint64_t)blockIdx.x * kSortTile;
-    uint64_t key[kSortItems];
-    uint32_t idx[kSortItems];
-    uint32_t rank[kSortItems];
-    uint32_t dig[kSortItems];
+    // Each digit's output region holds the runs of tile 0, 1, 2, ... in order;
+    // neighbouring tiles' runs share lines, so they are written from one XCD.
+    const uint32_t tile_id = xcd_tile(blockIdx.x, nb);
+    const uint64_t tile = (uint64_t)tile_id * kSortTile;
+    uint64_t key[NI];
+    uint32_t idx[NI];
+    uint32_t rank[NI];
+    uint32_t dig[NI];
     // all loads of the tile first (clamped in-bounds indices: no per-item wait)
 #pragma unroll
-    for (int it = 0; it < kSortItems; ++it) {
-        const uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
+    for (int it = 0; it < NI; ++it) {
+        const uint64_t e = tile + (uint64_t)wave * (kSortTile / NW) + it * 64 + lane;
         const uint64_t ec = e < n ? e : n - 1;
         key[it] = kw_in[ec];
         idx[it] = idx_in ? idx_in[ec] : (uint32_t)ec;
@@ -400,11 +432,11 @@ __global__ void __launch_bounds__(kBlock) radix_scatter_kernel(const uint64_t* _
     // wave's digit counter with one LDS atomic that returns the old count; a
     // wave's LDS operations complete in issue order, so item it+1 sees item
     // it's additions and the 16 atomics need no wait between them.
-    uint32_t before[kSortItems], got[kSortItems];
-    int leader[kSortItems];
+    uint32_t before[NI], got[NI];
+    int leader[NI];
 #pragma unroll
-    for (int it = 0; it < kSortItems; ++it) {
-        const uint64_t e = tile + (uint64_t)wave * (kSortTile / 4) + it * 64 + lane;
+    for (int it = 0; it < NI; ++it) {
+        const uint64_t e = tile + (uint64_t)wave * (kSortTile / NW) + it * 64 + lane;
         const bool valid = e < n;
         const uint32_t d = valid ? digit_of(key[it], dg) : kInvalid;
         dig[it] = d;
@@ -414,36 +446,39 @@ __global__ void __launch_bounds__(kBlock) radix_scatter_kernel(const uint64_t* _
         got[it] = (valid && before[it] == 0) ? atomicAdd(&wcnt[wave][d & (kR - 1)], (uint32_t)__popcll(peers)) : 0u;
     }
 #pragma unroll
-    for (int it = 0; it < kSortItems; ++it) rank[it] = (uint32_t)__shfl((int)got[it], leader[it], 64) + before[it];
+    for (int it = 0; it < NI; ++it) rank[it] = (uint32_t)__shfl((int)got[it], leader[it], 64) + before[it];
     __syncthreads();
     // Digit totals and their exclusive scan over digits; per-wave offsets.
-    uint32_t c[kPerThread][4];
+    const bool owner = kR >= BS || (int)threadIdx.x < kR;
+    uint32_t c[kPerThread][NW];
     uint32_t sum = 0;
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
-        const int d = threadIdx.x * kPerThread + q;
+        const int d = owner ? threadIdx.x * kPerThread + q : 0;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            c[q][w] = wcnt[w][d];
+        for (int w = 0; w < NW; ++w) {
+            c[q][w] = owner ? wcnt[w][d] : 0;
             sum += c[q][w];
         }
     }
     uint32_t tot;
     uint32_t start = block_excl_scan<uint32_t>(sum, scan_lds, tot);
+    if (owner) {
 #pragma unroll
-    for (int q = 0; q < kPerThread; ++q) {
-        const int d = threadIdx.x * kPerThread + q;
-        dstart[d] = start;
-        wcnt[0][d] = start;
-        wcnt[1][d] = start + c[q][0];
-        wcnt[2][d] = start + c[q][0] + c[q][1];
-        wcnt[3][d] = start + c[q][0] + c[q][1] + c[q][2];
-        goff[d] = offs[(uint64_t)d * nb + blockIdx.x];
-        start += c[q][0] + c[q][1] + c[q][2] + c[q][3];
+        for (int q = 0; q < kPerThread; ++q) {
+            const int d = threadIdx.x * kPerThread + q;
+            dstart[d] = start;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                wcnt[w][d] = start;
+                start += c[q][w];
+            }
+            goff[d] = offs[(uint64_t)d * nb + tile_id];
+        }
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < kSortItems; ++it) {
+    for (int it = 0; it < NI; ++it) {
         if (dig[it] != kInvalid) {
             uint32_t pos = wcnt[wave][dig[it]] + rank[it];
             skey[pos] = key[it];
@@ -452,7 +487,7 @@ __global__ void __launch_bounds__(kBlock) radix_scatter_kernel(const uint64_t* _
     }
     __syncthreads();
     const uint32_t count = (uint32_t)min<uint64_t>((uint64_t)kSortTile, n - tile);
-    for (uint32_t p = threadIdx.x; p < count; p += kBlock) {
+    for (uint32_t p = threadIdx.x; p < count; p += BS) {
         uint64_t k = skey[p];
         uint32_t dd = digit_of(k, dg);
         uint64_t dst = goff[dd] + (p - dstart[dd]);
@@ -532,7 +567,8 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hip
     uint64_t* kwB = dalloc<uint64_t>(n, st.stream);
     uint32_t* idxA = dalloc<uint32_t>(n, st.stream);
     uint32_t* idxB = dalloc<uint32_t>(n, st.stream);
-    uint32_t* bh = dalloc<uint32_t>((size_t)radix * nb, st.stream);
+    uint32_t* bh = dalloc<uint32_t>((size_t)radix * nb, st.stream);      // digit-major
+    uint32_t* bh_rows = dalloc<uint32_t>((size_t)radix * nb, st.stream); // block-major
     uint64_t* offs = dalloc<uint64_t>((size_t)radix * nb, st.stream);
 
     bool identity = true;
@@ -556,8 +592,14 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hip
         }
         {
             SPQ_PROF(st.stream, "radix_hist_kernel", n * 8.0);
-            if (radix_bits == 11) radix_hist_kernel<11><<<nb, kBlock, 0, st.stream>>>(src, n, dg, bh, nb);
-            else radix_hist_kernel<8><<<nb, kBlock, 0, st.stream>>>(src, n, dg, bh, nb);
+            if (radix_bits == 11) radix_hist_kernel<11, kSortBlock11><<<nb, kSortBlock11, 0, st.stream>>>(src, n, dg, bh_rows, nb);
+            else radix_hist_kernel<8, kSortBlock8><<<nb, kSortBlock8, 0, st.stream>>>(src, n, dg, bh_rows, nb);
+        }
+        SPQ_HIP(hipGetLastError());
+        {
+            SPQ_PROF(st.stream, "transpose_u32_kernel", (double)radix * nb * 8.0);
+            const dim3 tg((unsigned)((radix + 31) / 32), (nb + 31) / 32);
+            transpose_u32_kernel<<<tg, kBlock, 0, st.stream>>>(bh_rows, nb, (uint32_t)radix, bh);
         }
         SPQ_HIP(hipGetLastError());
         exclusive_scan_u32(st, bh, offs, radix * nb, nullptr);
@@ -566,11 +608,11 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hip
         {
             SPQ_PROF(st.stream, "radix_scatter_kernel", n * (8.0 + (identity ? 0.0 : 4.0) + 4.0 + (need_keys ? 8.0 : 0.0)));
             if (radix_bits == 11)
-                radix_scatter_kernel<11><<<nb, kBlock, 0, st.stream>>>(src, identity ? nullptr : idxA, n, dg, offs, nb, kwB, idxB,
-                                                                    need_keys ? 1 : 0);
+                radix_scatter_kernel<11, kSortBlock11><<<nb, kSortBlock11, 0, st.stream>>>(src, identity ? nullptr : idxA, n, dg, offs, nb,
+                                                                                        kwB, idxB, need_keys ? 1 : 0);
             else
-                radix_scatter_kernel<8><<<nb, kBlock, 0, st.stream>>>(src, identity ? nullptr : idxA, n, dg, offs, nb, kwB, idxB,
-                                                                   need_keys ? 1 : 0);
+                radix_scatter_kernel<8, kSortBlock8><<<nb, kSortBlock8, 0, st.stream>>>(src, identity ? nullptr : idxA, n, dg, offs, nb,
+                                                                                      kwB, idxB, need_keys ? 1 : 0);
         }
         SPQ_HIP(hipGetLastError());
         std::swap(kwA, kwB);
@@ -589,6 +631,7 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hip
     dfree(idxA, st.stream);
     dfree(idxB, st.stream);
     dfree(bh, st.stream);
+    dfree(bh_rows, st.stream);
     dfree(offs, st.stream);
 }
 
