@@ -197,6 +197,55 @@ def qda_sparse_leg(ps, nqubit, steps):
     return out
 
 
+def sort_leg(ps, lib, address_bits, reps):
+    """SortUnconditional of 2^address_bits entries whose 48-bit keys (b: 24 bits,
+    most significant, then a: address_bits) are scrambled before each sort by a
+    QRAM load b ^= M[a] with M uniform from splitmix64: every sort moves all
+    entries through the digit passes of both fields and the permutation gather.
+    Reports each sort kernel's algorithmic bytes / time (HIP events on the launch
+    stream) against the HBM peak."""
+    from sparqsim_amd.workloads import qram_memory
+
+    ps.System.clear()
+    st = ps.SparseState()
+    ps.AddRegister("b", ps.UnsignedInteger, 24)(st)
+    ps.AddRegister("a", ps.UnsignedInteger, address_bits)(st)
+    q = ps.QRAMCircuit_qutrit(address_bits, 24, qram_memory(address_bits, 24, config_index=7))
+    ps.Hadamard_Int("a", address_bits)(st)
+    load, sort = ps.QRAMLoad(q, "a", "b"), ps.SortUnconditional()
+
+    def once():
+        load(st)
+        sort(st)
+        st.size()
+
+    for _ in range(3):
+        once()
+    lib.sparq_prof_reset()
+    lib.sparq_prof_enable(1)
+    for _ in range(reps):
+        once()
+    lib.sparq_prof_enable(0)
+    prof = kernel_profile(lib)
+    sort_kernels = ("radix_hist_kernel", "radix_scatter_kernel", "transpose_u32_kernel", "scan_reduce_kernel",
+                    "scan_apply_kernel", "gather_state_kernel", "gather_word_kernel", "key_stats_kernel",
+                    "key_stats_final_kernel")
+    rows = [r for r in prof if r["name"] in sort_kernels and r["ms"] > 0]
+    ms = sum(r["ms"] for r in rows)
+    byt = sum(r["bytes"] for r in rows)
+    n = st.size()
+    ps.System.clear()
+    return {
+        "workload": f"SortUnconditional of 2^{address_bits} entries, 48-bit keys scrambled by a QRAM load before each "
+                    f"sort, {reps} sorts",
+        "entries": n, "ms_per_sort": ms / reps, "entries_per_s": n * reps / (ms * 1e-3),
+        "GBs": byt / (ms * 1e-3) / 1e9, "frac_of_peak": byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "kernels": [{"name": r["name"], "launches_per_sort": r["launches"] / reps, "us_per_launch": r["ms"] / r["launches"] * 1e3,
+                     "GBs": r["bytes"] / (r["ms"] * 1e-3) / 1e9,
+                     "frac_of_peak": r["bytes"] / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS} for r in rows],
+    }
+
+
 def cks_leg(ps, nqubit, terms, R, band_terms=0):
     """The 2^nqubit Poiseuille CKS walk (direct tridiagonal oracles, fitted
     coefficients J = 640): LCU terms timed for one right-hand side and for R in
@@ -257,6 +306,8 @@ def main():
     ap.add_argument("--cks-rhs", type=int, default=16, help="right-hand sides of the batched CKS run")
     ap.add_argument("--cks-band-terms", type=int, default=3,
                     help="LCU terms timed on the band-plus-random matrix with the batched right-hand sides (0 skips)")
+    ap.add_argument("--sort-bits", type=int, default=24, help="entries (log2) of the sort leg (0 skips it)")
+    ap.add_argument("--sort-reps", type=int, default=6)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 512 MiB device copies (the reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -353,6 +404,8 @@ def main():
         result["qda_sparse"] = qda_sparse_leg(ps, args.qda_sparse_qubits, args.qda_sparse_steps)
     if args.cks_terms > 0:
         result["cks"] = cks_leg(ps, args.cks_qubits, args.cks_terms, args.cks_rhs, args.cks_band_terms)
+    if args.sort_bits > 0:
+        result["sort"] = sort_leg(ps, lib, args.sort_bits, args.sort_reps)
     if args.grover_traj > 0:
         result["noisy_grover"] = grover_leg(ps, args.grover_traj, args.grover_bits, args.grover_iterations, 1e-3,
                                             min(args.grover_plain_traj, args.grover_traj))

@@ -214,36 +214,76 @@ uint64_t read_u64(State& st, const uint64_t* dptr) {
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 
-template <class TIn>
+// The kScanItems consecutive inputs of a lane.  VEC (input 16-byte aligned):
+// a lane whose items all lie below n reads them with 16-byte loads (its items
+// start at a multiple of 32 bytes); the others read item by item.
+template <class TIn, bool VEC>
+__device__ __forceinline__ void scan_load(const TIn* __restrict__ in, uint64_t n, uint64_t base, uint64_t (&v)[kScanItems]) {
+    static_assert(kScanItems * sizeof(TIn) % 16 == 0, "items must fill whole 16-byte words");
+    if (VEC && base + kScanItems <= n) {
+        constexpr int kWords = kScanItems * (int)sizeof(TIn) / 16;
+        constexpr int kPerWord = 16 / (int)sizeof(TIn);
+        const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (int q = 0; q < kWords; ++q) {
+            const uint4 w = p[q];
+            const uint32_t u[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int j = 0; j < kPerWord; ++j) {
+                if constexpr (sizeof(TIn) == 4) v[q * kPerWord + j] = u[j];
+                else v[q * kPerWord + j] = (uint64_t)u[2 * j] | ((uint64_t)u[2 * j + 1] << 32);   // little-endian
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) v[j] = (base + j < n) ? (uint64_t)in[base + j] : 0;
+}
+
+template <class TIn, bool VEC>
 __global__ void __launch_bounds__(kBlock) scan_reduce_kernel(const TIn* __restrict__ in, uint64_t n, uint64_t* __restrict__ sums) {
     __shared__ uint64_t lds[kBlock / 64];
     uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t v[kScanItems];
+    scan_load<TIn, VEC>(in, n, base, v);
     uint64_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j) s += (base + j < n) ? (uint64_t)in[base + j] : 0;
+    for (int j = 0; j < kScanItems; ++j) s += v[j];
     uint64_t tot;
     (void)block_excl_scan<uint64_t>(s, lds, tot);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-template <class TIn>
+// VEC: input and output 16-byte aligned; a lane whose items all lie below n
+// writes its 8 prefix values with 16-byte stores.
+template <class TIn, bool VEC>
 __global__ void __launch_bounds__(kBlock) scan_apply_kernel(const TIn* __restrict__ in, uint64_t n, const uint64_t* __restrict__ offs,
                                                        uint64_t* __restrict__ out, uint64_t* __restrict__ total) {
     __shared__ uint64_t lds[kBlock / 64];
     uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     uint64_t v[kScanItems];
+    scan_load<TIn, VEC>(in, n, base, v);
     uint64_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        v[j] = (base + j < n) ? (uint64_t)in[base + j] : 0;
-        s += v[j];
-    }
+    for (int j = 0; j < kScanItems; ++j) s += v[j];
     uint64_t tot;
     uint64_t pre = block_excl_scan<uint64_t>(s, lds, tot) + (offs ? offs[blockIdx.x] : 0);
+    if (VEC && base + kScanItems <= n) {
+        uint64_t o[kScanItems];
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        if (base + j < n) out[base + j] = pre;
-        pre += v[j];
+        for (int j = 0; j < kScanItems; ++j) {
+            o[j] = pre;
+            pre += v[j];
+        }
+        ulonglong2* q = reinterpret_cast<ulonglong2*>(out + base);
+#pragma unroll
+        for (int j = 0; j < kScanItems / 2; ++j) q[j] = make_ulonglong2(o[2 * j], o[2 * j + 1]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            if (base + j < n) out[base + j] = pre;
+            pre += v[j];
+        }
     }
     if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) total[0] = pre;
 }
@@ -255,10 +295,12 @@ static void excl_scan_impl(State& st, const TIn* in, uint64_t* out, uint64_t n, 
         return;
     }
     unsigned nb = grid_for(n, kScanTile);
+    const bool vec = (((uintptr_t)in | (uintptr_t)out) & 15) == 0;
     if (nb == 1) {
         {
             SPQ_PROF(st.stream, "scan_apply_kernel", n * ((double)sizeof(TIn) + 8));
-            scan_apply_kernel<TIn><<<1, kBlock, 0, st.stream>>>(in, n, nullptr, out, total_dev);
+            if (vec) scan_apply_kernel<TIn, true><<<1, kBlock, 0, st.stream>>>(in, n, nullptr, out, total_dev);
+            else scan_apply_kernel<TIn, false><<<1, kBlock, 0, st.stream>>>(in, n, nullptr, out, total_dev);
         }
         SPQ_HIP(hipGetLastError());
         return;
@@ -267,13 +309,15 @@ static void excl_scan_impl(State& st, const TIn* in, uint64_t* out, uint64_t n, 
     uint64_t* offs = dalloc<uint64_t>(nb, st.stream);
     {
         SPQ_PROF(st.stream, "scan_reduce_kernel", n * (double)sizeof(TIn));
-        scan_reduce_kernel<TIn><<<nb, kBlock, 0, st.stream>>>(in, n, sums);
+        if (vec) scan_reduce_kernel<TIn, true><<<nb, kBlock, 0, st.stream>>>(in, n, sums);
+        else scan_reduce_kernel<TIn, false><<<nb, kBlock, 0, st.stream>>>(in, n, sums);
     }
     SPQ_HIP(hipGetLastError());
     excl_scan_impl<uint64_t>(st, sums, offs, nb, nullptr);
     {
         SPQ_PROF(st.stream, "scan_apply_kernel", n * ((double)sizeof(TIn) + 8));
-        scan_apply_kernel<TIn><<<nb, kBlock, 0, st.stream>>>(in, n, offs, out, total_dev);
+        if (vec) scan_apply_kernel<TIn, true><<<nb, kBlock, 0, st.stream>>>(in, n, offs, out, total_dev);
+        else scan_apply_kernel<TIn, false><<<nb, kBlock, 0, st.stream>>>(in, n, offs, out, total_dev);
     }
     SPQ_HIP(hipGetLastError());
     dfree(sums, st.stream);
