@@ -95,7 +95,13 @@ static void recycle_resources(State& st) {
     g_res_pool.push_back(r);
 }
 
+// Every access through S may change the state: it clears basis_amp_one.
 static State& S(sparq_state_t h) {
+    if (!h) throw Error(SPARQ_INVALID, "null state handle");
+    h->st.basis_amp_one = false;
+    return h->st;
+}
+static State& S_keep_basis(sparq_state_t h) {
     if (!h) throw Error(SPARQ_INVALID, "null state handle");
     return h->st;
 }
@@ -256,6 +262,7 @@ int sparq_state_set_basis(sparq_state_t h, const uint64_t* key) {
         }
         write_basis(st, k);   // stream-ordered; no host synchronization
         st.n = 1;
+        st.basis_amp_one = !st.arrays_escaped;
     });
 }
 
@@ -365,6 +372,7 @@ int sparq_release_cached_memory(void) {
 int sparq_state_device_arrays(sparq_state_t h, uint64_t** keys, double** amps, uint64_t* capacity) {
     return guarded([&] {
         State& st = S(h);
+        st.arrays_escaped = true;
         if (keys) *keys = st.keys;
         if (amps) *amps = reinterpret_cast<double*>(st.amps);
         if (capacity) *capacity = st.cap;
@@ -490,8 +498,10 @@ int sparq_check_duplicates(sparq_state_t h, int* has_dup) {
 int sparq_hadamard_int(sparq_state_t h, sparq_field_t target, uint32_t n_digits, const sparq_cond_t* conds, uint32_t ncond,
                        double eps) {
     return guarded([&] {
-        State& st = S(h);
-        with_conditions(st, conds, ncond, eps, [&](State& s) { hadamard_int(s, target, n_digits, false, eps, s.n); });
+        State& st = S_keep_basis(h);
+        const bool unit = st.basis_amp_one && ncond == 0;
+        st.basis_amp_one = false;
+        with_conditions(st, conds, ncond, eps, [&](State& s) { hadamard_int(s, target, n_digits, false, eps, unit); });
     });
 }
 
@@ -505,7 +515,7 @@ int sparq_hadamard_int_full(sparq_state_t h, sparq_field_t target, const sparq_c
             sort_state(st, &target);
             if (has_sorted_duplicates(st)) throw Error(SPARQ_RUNTIME, "Hadamard_Int_Full: duplicate key in state");
         }
-        with_conditions(st, conds, ncond, eps, [&](State& s) { hadamard_int(s, target, target.width, true, eps, s.n); });
+        with_conditions(st, conds, ncond, eps, [&](State& s) { hadamard_int(s, target, target.width, true, eps); });
     });
 }
 

@@ -381,7 +381,7 @@ static void launch_wht(State& st, double2* a, const uint64_t* counts, uint64_t G
 
 template <int W>
 static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, const double2* dense,
-                          bool allow_learned = true) {
+                          bool allow_learned = true, bool unit_basis = false) {
     const uint64_t n = st.n;
     const uint64_t F = uint64_t(1) << nd;
     sparq_field_t low{target.word, target.shift, (uint8_t)nd, 0};
@@ -424,7 +424,10 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
                                                                                          eps, counts);
             }
             SPQ_HIP(hipGetLastError());
-            const uint64_t pruned = read_u64(st, counts);
+            // The kernel's comparison cabs2(a * extra) < eps, done here when
+            // a = (1, 0) is known: no readback.
+            const double extra = 1.0 / std::sqrt((double)F);
+            const uint64_t pruned = unit_basis ? ((extra * extra + 0.0 * 0.0 < eps) ? 1 : 0) : read_u64(st, counts);
             dfree(counts, st.stream);
             install_arrays(st, st.W, kout, aout, pruned ? 0 : F, F);
             return;
@@ -558,12 +561,12 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     }
 }
 
-void hadamard_int(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, uint64_t /*n_active*/) {
+void hadamard_int(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, bool unit_basis) {
     SPQ_REQUIRE(target.word < st.W && target.width >= 1 && (uint32_t)target.shift + target.width <= 64, "bad target field");
     SPQ_REQUIRE(nd >= 1 && nd <= target.width, "n_digits must be in [1, width]");
     SPQ_REQUIRE(nd <= 40, "n_digits too large for a sparse expansion");
     if (st.n == 0) return;
-    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, nd, full, eps, (const double2*)nullptr);
+    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, nd, full, eps, (const double2*)nullptr, true, unit_basis && st.n == 1);
 }
 
 void group_unitary(State& st, sparq_field_t target, const double* mat, double eps) {

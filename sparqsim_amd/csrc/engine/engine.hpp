@@ -79,6 +79,11 @@ struct State {
     uint64_t* pinned = nullptr; // 64 words of pinned host memory for small readbacks
     unsigned int* counter = nullptr;   // device word, zero between kernels (last-block reductions)
     int device = 0;
+    // One entry of amplitude exactly (1, 0), written by set_basis and untouched
+    // since (every API access but Hadamard_Int's clears it).  Never set once the
+    // device arrays have been handed to a caller, who may write them directly.
+    bool basis_amp_one = false;
+    bool arrays_escaped = false;
 };
 
 // ---- memory (caching device allocator)
@@ -185,8 +190,10 @@ uint64_t partition_by_conditions(State& st, const sparq_cond_t* conds, uint32_t 
 uint64_t select_by_conditions(State& st, const sparq_cond_t* conds, uint32_t ncond);
 
 // ---- branching (branch.hip)
+// unit_basis: st is one entry of amplitude exactly (1, 0) (lets the one-entry
+// expansion decide its ClearZero on the host).
 void hadamard_int(State& st, sparq_field_t target, uint32_t n_digits, bool full_variant, double eps,
-                  uint64_t n_active);
+                  bool unit_basis = false);
 void replicate(State& st, sparq_field_t f);
 void group_unitary(State& st, sparq_field_t target, const double* mat, double eps);
 // Conditions as a kernel argument (AND of all).

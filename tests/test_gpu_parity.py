@@ -326,6 +326,33 @@ def test_clear_zero_threshold(ps, oracle):
     assert_same_state(eg, eo, exact_amps=True, what="ClearZero")
 
 
+@pytest.mark.parametrize("touched", [False, True], ids=["fresh_basis", "after_phase"])
+def test_single_entry_expansion_clear_zero(ps, oracle, touched):
+    """H(a, 20) on one entry: every output is +-a/2^10, so ClearZero keeps all of
+    them or none.  A fresh basis state (amplitude exactly 1) is decided on the
+    host; after another operator the decision is read from the device.  Both
+    sides of |a|^2 / 2^20 = 2^-20 against the oracle."""
+    old = ps.get_clearzero_default_eps()
+    try:
+        for eps, expect in ((2.0 ** -20 * 1.0001, 0), (2.0 ** -20 * 0.9999, 1 << 20)):
+            sizes = []
+            for mod in (ps, oracle):
+                mod.set_clearzero_default_eps(eps)
+                mod.System.clear()
+                st = mod.SparseState()
+                mod.AddRegister("a", mod.UnsignedInteger, 20)(st)
+                mod.reset_systems(st)
+                if touched:
+                    mod.GlobalPhase(complex(1.0, 0.0))(st)
+                mod.Hadamard_Int("a", 20)(st)
+                sizes.append(st.size())
+                mod.set_clearzero_default_eps(old)
+            assert sizes == [expect, expect], (eps, sizes)
+    finally:
+        ps.set_clearzero_default_eps(old)
+        oracle.set_clearzero_default_eps(old)
+
+
 def test_empty_state_ops(ps):
     st = ps.SparseState()
     ps.AddRegister("a", ps.UnsignedInteger, 4)(st)
