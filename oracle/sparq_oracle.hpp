@@ -11,6 +11,16 @@
 // (extern/qram-simulator) is not vendored, so its helpers are re-derived from
 // their call sites; see the L0 block below.
 //
+// Parity status.  tests/test_oracle.py, test_qda.py, test_grover.py, test_cks.py
+// and test_measurement.py pin the functions here against the reference's known
+// answers and golden lists, closed forms, dense models and exhaustive truth
+// tables.  Parity unpinned (no independent check; the engine is compared with
+// this restatement only): the exact rounding of get_rational (only its 2^-w
+// bound is pinned), the summation order of >= 3 duplicates in
+// sort_merge_unique_erase, the float-to-fixed-point rule of SparseMatrix
+// (sparq_oracle_cks.hpp), and condition evaluation (ConditionSatisfied), which
+// the tests exercise only through the operators that carry conditions.
+//
 // Build knob: ORACLE_OMP enables the reference's own `#pragma omp parallel for`
 // loops on the per-entry operators (the reference ships with SINGLE_THREAD,
 // SparQ/include/global_macros.h:10); sorts stay sequential as in the

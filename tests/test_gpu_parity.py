@@ -353,6 +353,32 @@ def test_single_entry_expansion_clear_zero(ps, oracle, touched):
         oracle.set_clearzero_default_eps(old)
 
 
+@pytest.mark.parametrize("op", ["Div_Sqrt_Arccos_UInt_UInt", "Sqrt_Div_Arccos_Int_UInt", "GetRotateAngle_Int_Int"])
+def test_rational_angle_ops(ps, oracle, op):
+    """Arc-cosine / arc-tangent ops into a 12-bit Rational register over every
+    input pair of small registers (acos arguments kept in [0, 1]), as one
+    superposition: keys bit-exact against the oracle."""
+    if op == "Div_Sqrt_Arccos_UInt_UInt":
+        spec = [("x", "UnsignedInteger", 5), ("y", "UnsignedInteger", 5)]
+        pairs = [(a, b) for b in range(1, 32) for a in range(b + 1)]
+    elif op == "Sqrt_Div_Arccos_Int_UInt":
+        spec = [("x", "SignedInteger", 5), ("y", "UnsignedInteger", 6)]
+        pairs = [(l, r) for r in range(1, 64) for l in range(int(np.sqrt(r)) + 1)]
+    else:
+        spec = [("x", "SignedInteger", 4), ("y", "SignedInteger", 4)]
+        pairs = [(l, r) for l in range(16) for r in range(16)]
+
+    def build(mod):
+        st = regs(mod, spec + [("o", "Rational", 12)])
+        vals = np.array([(a, b, 0) for a, b in pairs], dtype=np.uint64)
+        st.load(vals, np.full(len(pairs), 1.0 / np.sqrt(len(pairs)), dtype=np.complex128))
+        getattr(mod, op)("x", "y", "o")(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, exact_amps=True, what=op)
+
+
 def test_empty_state_ops(ps):
     st = ps.SparseState()
     ps.AddRegister("a", ps.UnsignedInteger, 4)(st)

@@ -275,3 +275,138 @@ def test_split_combine_roundtrip(oracle):
         assert list(m) == [(v >> 3, v & 7)]
         oracle.CombineRegister("a", "lo")(st)
         assert list(amp_map(st)) == [(v,)]
+
+
+# ---- flags, select, abs: exhaustive truth tables against Python integer arithmetic
+# (definitions of SparQ/src/quantum_arithmetic.cpp:532-554, 623-644, 731-816)
+def _sx(v, w):
+    return v - (1 << w) if v >= 1 << (w - 1) else v
+
+
+def test_flag_truth_tables(oracle):
+    for w in (2, 3, 4):
+        spec = [("a", "UnsignedInteger", 4), ("b", "UnsignedInteger", 4), ("r", "UnsignedInteger", w), ("f", "Boolean", 1)]
+        for a, b in itertools.product(range(16), repeat=2):
+            for f0 in (0, 1):
+                got = run_classical(oracle, spec, {"a": a, "b": b, "f": f0}, lambda: oracle.Carry_UInt_UInt("a", "b", "r", "f"))
+                assert got["f"] == f0 ^ int(a + b >= 1 << w), ("carry", w, a, b)   # carry out of a w-bit sum
+                got = run_classical(oracle, spec, {"a": a, "b": b, "f": f0},
+                                    lambda: oracle.MulOverflow_UInt_UInt("a", "b", "r", "f"))
+                assert got["f"] == f0 ^ int(a * b >= 1 << w), ("muloverflow", w, a, b)
+        spec = [("a", "SignedInteger", 4), ("b", "SignedInteger", 4), ("r", "UnsignedInteger", w), ("f", "Boolean", 1)]
+        for a, b in itertools.product(range(16), repeat=2):
+            # both operands reduced to w-bit two's complement; overflow = the true sum leaves the w-bit range
+            sa, sb = _sx(_sx(a, 4) % (1 << w), w), _sx(_sx(b, 4) % (1 << w), w)
+            expect = int(not (-(1 << (w - 1)) <= sa + sb < 1 << (w - 1)))
+            got = run_classical(oracle, spec, {"a": a, "b": b}, lambda: oracle.Overflow_SInt_SInt("a", "b", "r", "f"))
+            assert got["f"] == expect, ("overflow", w, a, b)
+
+
+def test_select_and_abs_truth_tables(oracle):
+    spec = [("c", "UnsignedInteger", 2), ("x", "UnsignedInteger", 3), ("y", "UnsignedInteger", 3), ("r", "UnsignedInteger", 2)]
+    for c, x, y, r0 in itertools.product(range(4), range(8), range(8), (0, 2)):
+        got = run_classical(oracle, spec, {"c": c, "x": x, "y": y, "r": r0},
+                            lambda: oracle.Select_Bool_UInt_UInt("c", "x", "y", "r"))
+        assert got["r"] == (r0 ^ (x if c & 1 else y)) & 3, (c, x, y, r0)   # bit 0 selects, result truncated
+    spec = [("v", "SignedInteger", 4), ("r", "UnsignedInteger", 3)]
+    for v, r0 in itertools.product(range(16), (0, 5)):
+        got = run_classical(oracle, spec, {"v": v, "r": r0}, lambda: oracle.Abs_SInt("v", "r"))
+        assert got["r"] == (r0 ^ abs(_sx(v, 4))) & 7, (v, r0)
+
+
+# ---- rational-output ops: the angle formula to the resolution of the output register
+# (quantum_arithmetic.cpp:413-456, 458-490; get_rational is pinned only to 2^-w)
+def test_rational_angle_ops(oracle):
+    import math
+
+    w = 12
+
+    def decode(bits):
+        return bits / float(1 << w)
+
+    def close(d, x):
+        e = abs(d - x)
+        return min(e, 1 - e) <= 2.0 ** -w
+
+    spec = [("a", "UnsignedInteger", 5), ("b", "UnsignedInteger", 5), ("o", "Rational", w)]
+    for b in range(1, 32):
+        for a in range(0, b + 1):
+            got = run_classical(oracle, spec, {"a": a, "b": b}, lambda: oracle.Div_Sqrt_Arccos_UInt_UInt("a", "b", "o"))
+            assert close(decode(got["o"]), math.acos(math.sqrt(a / b)) / (2 * math.pi)), (a, b)
+    spec = [("l", "SignedInteger", 5), ("r", "UnsignedInteger", 6), ("o", "Rational", w)]
+    for r in range(1, 64):
+        for l in range(0, int(math.isqrt(r)) + 1):
+            got = run_classical(oracle, spec, {"l": l, "r": r}, lambda: oracle.Sqrt_Div_Arccos_Int_UInt("l", "r", "o"))
+            assert close(decode(got["o"]), math.acos(l / math.sqrt(r)) / (2 * math.pi)), (l, r)
+    spec = [("l", "SignedInteger", 4), ("r", "SignedInteger", 4), ("o", "Rational", w)]
+    for l, r in itertools.product(range(16), repeat=2):
+        sl, sr = _sx(l, 4), _sx(r, 4)
+        x = 0.25 if sl == 0 and sr == 0 else (math.atan2(sr, sl) / (2 * math.pi)) % 1.0
+        got = run_classical(oracle, spec, {"l": l, "r": r}, lambda: oracle.GetRotateAngle_Int_Int("l", "r", "o"))
+        assert close(decode(got["o"]), x), (sl, sr)
+
+
+# ---- one-bit gates against a dense numpy model on random superpositions
+# (basic_gates.cpp:11-55 Phase, 418-441 Y, 623-653 RZ; parallel_phase_operations.cpp:11-31)
+def _dense_one_bit(amps, a_vals, digit, U):
+    out = {}
+    for (a, b), amp in zip(a_vals, amps):
+        bit = (a >> digit) & 1
+        for nb in (0, 1):
+            na = (a & ~(1 << digit)) | (nb << digit)
+            if U[nb][bit] != 0:
+                out[(na, b)] = out.get((na, b), 0) + U[nb][bit] * amp
+    return out
+
+
+def test_one_bit_gates_dense(oracle):
+    rng = np.random.default_rng(11)
+    lam, theta = 0.7, 1.9
+    gates = {
+        "Phase_Bool": (lambda: oracle.Phase_Bool("a", 1, lam), [[1, 0], [0, np.exp(1j * lam)]]),
+        "RZ_Bool": (lambda: oracle.RZ_Bool("a", 1, theta), [[np.exp(-0.5j * theta), 0], [0, np.exp(0.5j * theta)]]),
+        "Y_Bool": (lambda: oracle.Y_Bool("a", 1), [[0, -1j], [1j, 0]]),
+    }
+    keys = [(a, b) for a in range(8) for b in range(4) if rng.random() < 0.6]
+    amps = rng.standard_normal(len(keys)) + 1j * rng.standard_normal(len(keys))
+    for name, (make, U) in gates.items():
+        oracle.System.clear()
+        st, _ = reg_state(oracle, [("a", "UnsignedInteger", 3), ("b", "UnsignedInteger", 2)])
+        st.load(np.array(keys, dtype=np.uint64), amps)
+        make()(st)
+        got = amp_map(st)
+        want = _dense_one_bit(amps, keys, 1, U)
+        assert set(got) == {k for k, v in want.items() if v != 0}, name
+        for k, v in want.items():
+            assert abs(got[k] - v) <= 1e-14 * abs(v), (name, k)
+    oracle.System.clear()
+    st, _ = reg_state(oracle, [("a", "UnsignedInteger", 3), ("b", "UnsignedInteger", 2)])
+    st.load(np.array(keys, dtype=np.uint64), amps)
+    oracle.ZeroConditionalPhaseFlip(["a", "b"])(st)
+    got = amp_map(st)
+    for (a, b), amp in zip(keys, amps):
+        assert got[(a, b)] == (-amp if a == 0 and b == 0 else amp)
+
+
+# ---- one-line key rewrites (quantum_arithmetic.cpp:511-530, 819-857, 859-888, 909-930)
+def test_simple_key_rewrite_truth_tables(oracle):
+    spec = [("a", "UnsignedInteger", 4), ("r", "UnsignedInteger", 3), ("f", "Boolean", 1)]
+    for a, r0, f0 in itertools.product(range(16), (0, 6), (0, 1)):
+        got = run_classical(oracle, spec, {"a": a, "r": r0}, lambda: oracle.Neg_UInt("a", "r"))
+        assert got["r"] == (r0 ^ (-a & 7)), ("neg", a, r0)                  # two's-complement negation, truncated
+        got = run_classical(oracle, spec, {"a": a, "r": r0}, lambda: oracle.Assign("a", "r"))
+        assert got["r"] == (r0 ^ a) & 7 and got["a"] == a, ("assign", a, r0)  # XOR-copy, truncated
+        got = run_classical(oracle, spec, {"a": a, "f": f0}, lambda: oracle.IsZero_UInt("a", "f"))
+        assert got["f"] == f0 ^ int(a == 0), ("iszero", a)
+    spec = [("s", "SignedInteger", 4), ("f", "Boolean", 1)]
+    for s, f0 in itertools.product(range(16), (0, 1)):
+        got = run_classical(oracle, spec, {"s": s, "f": f0}, lambda: oracle.Negative_SInt("s", "f"))
+        assert got["f"] == f0 ^ int(_sx(s, 4) < 0), ("negative", s)
+    for rtype in ("UnsignedInteger", "SignedInteger"):
+        spec = [("l", "UnsignedInteger", 4), ("x", rtype, 3)]
+        for l, x in itertools.product(range(16), range(8)):
+            ext = _sx(x, 3) if rtype == "SignedInteger" else x                 # rhs read by its declared type
+            got = run_classical(oracle, spec, {"l": l, "x": x}, lambda: oracle.Add_AnyInt_AnyInt_InPlace("l", "x"))
+            assert got["l"] == (l + ext) % 16 and got["x"] == x, ("add_any", rtype, l, x)
+            got = run_classical(oracle, spec, {"l": l, "x": x}, lambda: oracle.Add_AnyInt_AnyInt_InPlace("l", "x"), dagger=True)
+            assert got["l"] == (l - ext) % 16, ("add_any dag", rtype, l, x)
