@@ -210,7 +210,7 @@ def sort_leg(ps, lib, address_bits, reps):
     st = ps.SparseState()
     ps.AddRegister("b", ps.UnsignedInteger, 24)(st)
     ps.AddRegister("a", ps.UnsignedInteger, address_bits)(st)
-    q = ps.QRAMCircuit_qutrit(address_bits, 24, qram_memory(address_bits, 24, config_index=7))
+    q = ps.QRAMCircuit_qutrit(address_bits, 24, qram_memory(address_bits, 24, seed_offset=7))
     ps.Hadamard_Int("a", address_bits)(st)
     load, sort = ps.QRAMLoad(q, "a", "b"), ps.SortUnconditional()
 
@@ -330,7 +330,7 @@ def main():
 
     A, D = args.address_bits, args.data_bits
     lib = ctypes.CDLL(ps.ENGINE_LIBRARY)
-    mem = qram_memory(A, D, config_index=1)
+    mem = qram_memory(A, D, seed_offset=1)
     ps.System.clear()
     st = ps.SparseState()
     ps.AddRegister("addr", ps.UnsignedInteger, A)(st)
@@ -411,7 +411,7 @@ def main():
                                             min(args.grover_plain_traj, args.grover_traj))
     if not args.no_cpu_baseline:
         ca = args.cpu_address_bits
-        eps, secs, reps = cpu_baseline(ca, D, qram_memory(ca, D, config_index=1), args.cpu_seconds)
+        eps, secs, reps = cpu_baseline(ca, D, qram_memory(ca, D, seed_offset=1), args.cpu_seconds)
         result["cpu_baseline"] = {"value": eps, "unit": "entries/s", "cores": 1, "kind": "port",
                                   "sample": f"same circuit at 2^{ca} entries, {reps} passes in {secs:.1f} s, "
                                             "oracle (restated reference CPU path), one thread"}

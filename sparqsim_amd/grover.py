@@ -22,7 +22,7 @@ import numpy as np
 
 from .workloads import BASE_SEED, splitmix64
 
-GROVER_CONFIG_INDEX = 3
+GROVER_SEED_OFFSET = 3
 
 
 # ---------------------------------------------------------------- composites
@@ -75,12 +75,12 @@ class GroverOperator:
 
 
 # ---------------------------------------------------------------- workload inputs
-def grover_memory(address_bits: int, data_bits: int, config_index: int = GROVER_CONFIG_INDEX):
+def grover_memory(address_bits: int, data_bits: int, seed_offset: int = GROVER_SEED_OFFSET):
     """2^address_bits words with exactly one equal to the target; every other word is
     uniform in [0, 2^data_bits) and differs from the target.  Returns (memory, target, marked)."""
     n = 1 << address_bits
     mask = np.uint64((1 << data_bits) - 1)
-    r = splitmix64(BASE_SEED + config_index, n + 2)
+    r = splitmix64(BASE_SEED + seed_offset, n + 2)
     target = int(r[0] & mask)
     marked = int(r[1] % np.uint64(n))
     words = r[2:] & mask
@@ -90,16 +90,16 @@ def grover_memory(address_bits: int, data_bits: int, config_index: int = GROVER_
 
 
 def noise_events(n_traj: int, iterations: int, address_bits: int, p: float,
-                 config_index: int = GROVER_CONFIG_INDEX):
+                 seed_offset: int = GROVER_SEED_OFFSET):
     """Seeded Pauli insertions: after iteration k of trajectory t, for each address qubit
     i in ascending order, X(addr, i) with probability p and then Z(addr, i) with
-    probability p.  Trajectory t draws from splitmix64 seeded (BASE_SEED + config_index)
+    probability p.  Trajectory t draws from splitmix64 seeded (BASE_SEED + seed_offset)
     ^ ((t + 1) << 32), two uniforms per (k, i).  Returns events[k] = [(t, i, 'X'|'Z'), ...]
     ordered by t, then i, X before Z."""
     per = iterations * address_bits * 2
     events = [[] for _ in range(iterations)]
     for t in range(n_traj):
-        u = (splitmix64((BASE_SEED + config_index) ^ ((t + 1) << 32), per) >> np.uint64(11)).astype(np.float64)
+        u = (splitmix64((BASE_SEED + seed_offset) ^ ((t + 1) << 32), per) >> np.uint64(11)).astype(np.float64)
         hit = (u * (1.0 / (1 << 53))) < p
         for j in np.flatnonzero(hit):
             j = int(j)
@@ -129,11 +129,11 @@ class NoisyGrover:
     `iterations` x (GroverOperator, noise)."""
 
     def __init__(self, mod, n_traj: int, address_bits: int, data_bits: int, iterations: int, p: float,
-                 config_index: int = GROVER_CONFIG_INDEX):
+                 seed_offset: int = GROVER_SEED_OFFSET):
         self.mod, self.n_traj, self.A, self.D = mod, n_traj, address_bits, data_bits
         self.iterations, self.p = iterations, p
-        self.memory, self.target, self.marked = grover_memory(address_bits, data_bits, config_index)
-        self.events = noise_events(n_traj, iterations, address_bits, p, config_index)
+        self.memory, self.target, self.marked = grover_memory(address_bits, data_bits, seed_offset)
+        self.events = noise_events(n_traj, iterations, address_bits, p, seed_offset)
         self.traj_bits = max(1, (n_traj - 1).bit_length())
         self.qram = None
 

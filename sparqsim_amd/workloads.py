@@ -1,7 +1,7 @@
 """Seeded synthetic workloads shared by the benchmark, the tests and the CPU baseline.
 
 All randomness comes from splitmix64 over a counter (base seed 0x5EED2026, plus
-the configuration index), so the CPU oracle and the GPU engine see identical
+a per-workload seed offset), so the CPU oracle and the GPU engine see identical
 inputs without depending on any library RNG.
 """
 from __future__ import annotations
@@ -24,9 +24,9 @@ def splitmix64(seed: int, n: int) -> np.ndarray:
         return z ^ (z >> np.uint64(31))
 
 
-def qram_memory(address_bits: int, data_bits: int, config_index: int) -> list[int]:
+def qram_memory(address_bits: int, data_bits: int, seed_offset: int) -> list[int]:
     """2^address_bits words, uniform in [0, 2^data_bits)."""
-    words = splitmix64(BASE_SEED + config_index, 1 << address_bits)
+    words = splitmix64(BASE_SEED + seed_offset, 1 << address_bits)
     if data_bits < 64:
         words = words & np.uint64((1 << data_bits) - 1)
     return words.tolist()
