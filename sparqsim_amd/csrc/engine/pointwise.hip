@@ -302,11 +302,24 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
             }
             break;
         case SPARQ_PW_PHASE_BITS: {
+            // One multiplication by the phase per set bit, as one Phase_Bool per
+            // digit does (basic_gates.cpp:11-31).  For a phase (-1, s), Z's
+            // (cos pi, sin pi), a*(-1, s) = (-a.x - a.y*s, a.x*s - a.y) is the
+            // same IEEE result as the general complex product (x*(-1) is exact
+            // and x + (-y) == x - y) with two fewer multiplications.
             uint64_t bits = fget<W>(k, f0) & op.c[0];
             const double2 ph = make_double2(op.d[0], op.d[1]);
-            while (bits) {
-                amp = cmul(amp, ph);
-                bits &= bits - 1;
+            if (ph.x == -1.0) {
+                const double sn = ph.y;
+                while (bits) {
+                    amp = make_double2(-amp.x - amp.y * sn, amp.x * sn - amp.y);
+                    bits &= bits - 1;
+                }
+            } else {
+                while (bits) {
+                    amp = cmul(amp, ph);
+                    bits &= bits - 1;
+                }
             }
             break;
         }
