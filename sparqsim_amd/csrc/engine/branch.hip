@@ -174,13 +174,20 @@ __global__ void __launch_bounds__(kBlock) expand_scatter_kernel(const uint64_t* 
                                                            const uint8_t* __restrict__ cls, uint32_t nd, sparq_field_t low,
                                                            const uint64_t* __restrict__ G_dev, uint64_t G_cap, double2* __restrict__ aout) {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n || G_dev[0] > G_cap) return;
-    // group id of entry i = inclusive count of heads up to i, minus one
-    uint64_t g = head ? pos[i] + head[i] - 1 : 0;
-    if (cls[g] != kWht) return;
+    if (i >= n) return;
+    // the loads that do not depend on the group go out together (all in bounds
+    // for i < n); only the group's class waits for the group id
+    const uint64_t G = G_dev[0];
+    const uint64_t p = head ? pos[i] : 0;
+    const uint32_t h = head ? head[i] : 1;
     uint64_t k[W];
     load_key<W>(keys, cap, i, k);
-    aout[(g << nd) + fget<W>(k, low)] = amps[i];
+    const double2 a = amps[i];
+    if (G > G_cap) return;
+    // group id of entry i = inclusive count of heads up to i, minus one
+    const uint64_t g = p + h - 1;
+    if (cls[g] != kWht) return;
+    aout[(g << nd) + fget<W>(k, low)] = a;
 }
 
 // Walsh-Hadamard butterflies.  Stage q pairs indices i, i + 2^q of a group's

@@ -626,10 +626,23 @@ __global__ void __launch_bounds__(kBlock) field_minmax_kernel(const uint64_t* __
     __shared__ unsigned long long lmin[kBlock / 64], lmax[kBlock / 64];
     const unsigned long long mask = fmask(f.width);
     unsigned long long vmin = ~0ull, vmax = 0ull;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
-        const unsigned long long v = (word[i] >> f.shift) & mask;
-        vmin = v < vmin ? v : vmin;
-        vmax = v > vmax ? v : vmax;
+    // kMinmaxItems striped loads per lane per tile, issued together; indices
+    // past n read entry n-1 again, which changes neither minimum nor maximum.
+    constexpr int kMinmaxItems = 8;
+    const uint64_t tile = (uint64_t)kBlock * kMinmaxItems;
+    for (uint64_t t = (uint64_t)blockIdx.x * tile; t < n; t += (uint64_t)gridDim.x * tile) {
+        unsigned long long v[kMinmaxItems];
+#pragma unroll
+        for (int j = 0; j < kMinmaxItems; ++j) {
+            const uint64_t i = t + (uint64_t)j * kBlock + threadIdx.x;
+            v[j] = word[i < n ? i : n - 1];
+        }
+#pragma unroll
+        for (int j = 0; j < kMinmaxItems; ++j) {
+            const unsigned long long x = (v[j] >> f.shift) & mask;
+            vmin = x < vmin ? x : vmin;
+            vmax = x > vmax ? x : vmax;
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long a = __shfl_xor(vmin, off, 64), b = __shfl_xor(vmax, off, 64);
@@ -661,7 +674,7 @@ bool field_uniform(State& st, sparq_field_t f, uint64_t* value) {
     SPQ_HIP(hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, st.stream));
     {
         SPQ_PROF(st.stream, "field_minmax_kernel", st.n * 8.0);
-        const unsigned g = std::min<unsigned>(grid_for(st.n, kBlock * 4), 2048);
+        const unsigned g = std::min<unsigned>(grid_for(st.n, kBlock * 8), 2048);
         field_minmax_kernel<<<g, kBlock, 0, st.stream>>>(st.keys + (uint64_t)f.word * st.cap, st.n, f, mm);
     }
     SPQ_HIP(hipGetLastError());
