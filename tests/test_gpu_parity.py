@@ -73,6 +73,59 @@ def test_config2_qram_interference(ps, oracle, A):
     assert_same_state(eg, eo, what="config2")
 
 
+def _fwht(v):
+    """Unnormalised Walsh-Hadamard transform, in place over the last axis (numpy)."""
+    v = v.copy()
+    h = 1
+    n = v.shape[-1]
+    while h < n:
+        v = v.reshape(-1, 2, h)
+        a, b = v[:, 0, :].copy(), v[:, 1, :].copy()
+        v[:, 0, :], v[:, 1, :] = a + b, a - b
+        v = v.reshape(n)
+        h *= 2
+    return v
+
+
+def test_config2_full_size_closed_form(ps):
+    """The benchmarked QRAM-interference pass at 2^20 entries (20-bit addr, 32-bit
+    data, the benchmark's seeded memory): H U H with U = (-1)^parity(M[addr]) on
+    |0, 0> leaves amplitude WHT(s)_y / 2^20 on |y, 0>, s_x = (-1)^popcount(M[x]).
+    Checked against a numpy transform: the kept entries (those clear of the
+    ClearZero threshold by a factor 2 either way) and every amplitude to 1e-12
+    of the largest (Phase_Bool(pi) multiplies by (-1, 1.2e-16) per set bit)."""
+    from sparqsim_amd.workloads import qram_memory
+
+    A, D = 20, 32
+    mem = qram_memory(A, D, seed_offset=1)
+    st = regs(ps, [("addr", "UnsignedInteger", A), ("data", "UnsignedInteger", D)])
+    q = ps.QRAMCircuit_qutrit(A, D, mem)
+    ps.Hadamard_Int("addr", A)(st)
+    ps.QRAMLoad(q, "addr", "data")(st)
+    for i in range(D):
+        ps.Phase_Bool("data", i, np.pi)(st)
+    ps.QRAMLoad(q, "addr", "data")(st)
+    ps.Hadamard_Int("addr", A)(st)
+    ids, vals, amps = st.export()
+    assert np.all(vals[:, 1] == 0), "data register not uncomputed"
+    words = np.array(mem, dtype=np.uint64)
+    parity = np.zeros(words.shape, dtype=np.uint64)
+    for b in range(D):
+        parity ^= (words >> np.uint64(b)) & np.uint64(1)
+    expect = _fwht(1.0 - 2.0 * parity.astype(np.float64)) / float(1 << A)
+    eps = ps.get_clearzero_default_eps()
+    p2 = expect ** 2
+    got = dict(zip((int(y) for y in vals[:, 0]), amps))
+    must = np.nonzero(p2 >= 2 * eps)[0]
+    never = np.nonzero(p2 < eps / 2)[0]
+    assert all(int(y) in got for y in must), "entries above the ClearZero threshold missing"
+    assert not any(int(y) in got for y in never), "entries below the ClearZero threshold kept"
+    scale = np.max(np.abs(expect))
+    err = max(abs(got[y] - expect[y]) for y in got)
+    assert err <= 1e-12 * scale, err
+    assert abs(float(np.sum(np.abs(amps) ** 2)) - float(np.sum(p2[list(got)]))) <= 1e-12
+
+
 # ---------------------------------------------------------------- per-entry ops
 SPEC3 = [("a", "UnsignedInteger", 7), ("b", "UnsignedInteger", 9), ("c", "UnsignedInteger", 6),
          ("s", "SignedInteger", 8), ("f", "Boolean", 1), ("g", "Boolean", 1)]

@@ -124,3 +124,32 @@ def test_engine_grover_closed_form(ps):
     marked = vals[:, 0] == g.marked
     assert np.allclose(amps[marked], s, atol=1e-12, rtol=0)
     assert np.allclose(amps[~marked], c, atol=1e-12, rtol=0)
+
+
+@pytest.mark.gpu
+def test_engine_batched_full_size_properties(ps):
+    """The benchmarked batch (1 024 trajectories, 14-bit registers, 40 iterations,
+    p = 1e-3): every trajectory keeps norm 1 (the circuit is unitary), and every
+    trajectory that drew no noise event holds the closed form -- 2^14 entries,
+    sin((2K+1) theta) on the marked address and cos((2K+1) theta)/sqrt(N-1)
+    elsewhere -- to 1e-12."""
+    T, A, K, p = 1024, 14, 40, 1e-3
+    g = NoisyGrover(ps, T, A, A, K, p)
+    ids, vals, amps = g.run_batched().export()
+    traj = vals[:, 0].astype(np.int64)
+    norms = np.bincount(traj, weights=np.abs(amps) ** 2, minlength=T)
+    assert np.all(np.abs(norms - 1.0) <= 1e-10), float(np.max(np.abs(norms - 1.0)))
+    touched = {t for it in g.events for (t, _, _) in it}
+    clean = [t for t in range(T) if t not in touched]
+    assert len(clean) > 200                            # (1 - p)^(2 A K) of 1 024: about a third
+    s = grover_success_amplitude(A, K)
+    theta = math.asin(2.0 ** (-A / 2))
+    c = math.cos((2 * K + 1) * theta) / math.sqrt((1 << A) - 1)
+    counts = np.bincount(traj, minlength=T)
+    for t in clean[:64]:
+        _, v, a = trajectory_of((ids, vals, amps), t)
+        assert counts[t] == 1 << A, t
+        assert np.all(v[:, 1] == 0) and np.all(v[:, 2] == g.target), t   # data uncomputed, search holds the target
+        marked = v[:, 0] == g.marked
+        assert np.allclose(a[marked], s, atol=1e-12, rtol=0), t
+        assert np.allclose(a[~marked], c, atol=1e-12, rtol=0), t
