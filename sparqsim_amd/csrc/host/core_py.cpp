@@ -165,6 +165,9 @@ PYBIND11_MODULE(_core, m) {
         .def("__call__", [](const BaseOperator& o, SparseState& s) { o(s); }, py::arg("state"), Release())
         .def("dag", [](const BaseOperator& o, SparseState& s) { o.dag(s); }, py::arg("state"), Release());
     py::class_<SelfAdjointOperator, BaseOperator, std::shared_ptr<SelfAdjointOperator>>(m, "SelfAdjointOperator");
+    py::class_<OperatorSequence, BaseOperator, std::shared_ptr<OperatorSequence>>(m, "OperatorSequence")
+        .def(py::init<std::vector<std::shared_ptr<BaseOperator>>>(), py::arg("ops"))
+        .def("__len__", [](const OperatorSequence& q) { return q.ops.size(); });
 
     auto pwo = py::class_<PointwiseOperator, BaseOperator, std::shared_ptr<PointwiseOperator>>(m, "PointwiseOperator");
     pwo.def_readonly("name", &PointwiseOperator::name);

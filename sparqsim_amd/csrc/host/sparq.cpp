@@ -449,6 +449,17 @@ void BaseOperator::operator()(SparseState& state) const {
     state.from_host(host);
 }
 
+OperatorSequence::OperatorSequence(std::vector<std::shared_ptr<BaseOperator>> o) : ops(std::move(o)) {
+    for (const auto& op : ops)
+        if (!op) throw std::invalid_argument("OperatorSequence: null operator");
+}
+void OperatorSequence::operator()(SparseState& state) const {
+    for (const auto& op : ops) (*op)(state);
+}
+void OperatorSequence::dag(SparseState& state) const {
+    for (auto it = ops.rbegin(); it != ops.rend(); ++it) (*it)->dag(state);
+}
+
 void BaseOperator::operator()(std::vector<System>& state) const {
     Bridge b(this, "operator() is not implemented.");
     SparseState dev;

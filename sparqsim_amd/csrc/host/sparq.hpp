@@ -234,6 +234,18 @@ struct SelfAdjointOperator : BaseOperator {
 // Re-exposes both forms in a class that overrides one of them.
 #define SPQ_BOTH_VIEWS using BaseOperator::operator(); using BaseOperator::dag;
 
+// A fixed list of operators applied in order (dag: their adjoints in reverse
+// order), so that a circuit built once is run by one native call.  No
+// counterpart in the reference, whose C++ callers loop over operators
+// themselves.
+struct OperatorSequence : BaseOperator {
+    SPQ_BOTH_VIEWS
+    std::vector<std::shared_ptr<BaseOperator>> ops;
+    explicit OperatorSequence(std::vector<std::shared_ptr<BaseOperator>> o);
+    void operator()(SparseState& state) const override;
+    void dag(SparseState& state) const override;
+};
+
 // Plugin entry points catch exceptions (they must not cross the C boundary) and
 // park the message here; the Python proxy takes it and raises.
 void record_plugin_error(const char* message);

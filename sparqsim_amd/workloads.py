@@ -46,8 +46,14 @@ class QRAMInterference:
         self.ops += [mod.Phase_Bool("data", i, np.pi) for i in range(data_bits)]
         self.ops += [mod.QRAMLoad(qram, "addr", "data"), mod.Hadamard_Int("addr", address_bits)]
         self.entry_ops = (1 << address_bits) * (4 + data_bits)
+        # one native call per pass where the module offers it (the engine)
+        seq = getattr(mod, "OperatorSequence", None)
+        self._seq = seq(self.ops) if seq is not None else None
 
     def __call__(self, state):
+        if self._seq is not None:
+            self._seq(state)
+            return state
         for op in self.ops:
             op(state)
         return state

@@ -432,6 +432,34 @@ def test_rational_angle_ops(ps, oracle, op):
     assert_same_state(eg, eo, exact_amps=True, what=op)
 
 
+def test_operator_sequence(ps, oracle):
+    """OperatorSequence applies its operators in order (one native call) and its
+    dag applies their adjoints in reverse: the same state as the operators one
+    by one, and the identity after op; op.dag."""
+    spec = [("a", "UnsignedInteger", 6), ("b", "UnsignedInteger", 6), ("f", "Boolean", 1)]
+
+    def ops(mod):
+        return [mod.Hadamard_Int("a", 6), mod.Add_UInt_UInt_InPlace("a", "b"), mod.Phase_Bool("b", 2, 0.3),
+                mod.RZ_Bool("f", 0, 1.1), mod.Hadamard_Bool("f"), mod.X_Bool("b", 5)]
+
+    def build(mod):
+        st = regs(mod, spec)
+        for op in ops(mod):
+            op(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    ps.System.clear()
+    st = regs(ps, spec)
+    seq = ps.OperatorSequence(ops(ps))
+    assert len(seq) == 6
+    seq(st)
+    assert_same_state(st.export(), eo, what="sequence vs oracle")
+    seq.dag(st)
+    ids, vals, amps = st.export()
+    assert vals.shape[0] == 1 and np.all(vals == 0) and abs(amps[0] - 1) < 1e-12
+
+
 def test_empty_state_ops(ps):
     st = ps.SparseState()
     ps.AddRegister("a", ps.UnsignedInteger, 4)(st)
