@@ -496,13 +496,28 @@ __global__ void __launch_bounds__(BS) radix_scatter_kernel(const uint64_t* __res
     }
 }
 
+constexpr int kGatherItems = 4;
+
+// dst[i] = src[idx[i]], kGatherItems entries per lane (striped by the block
+// size): the index loads, then the word loads, are in flight together.
 __global__ void __launch_bounds__(kBlock) gather_word_kernel(const uint64_t* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
                                                         uint64_t* __restrict__ dst) {
-    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) dst[i] = src[idx[i]];
+    const uint64_t base = (uint64_t)blockIdx.x * kBlock * kGatherItems + threadIdx.x;
+    uint32_t s[kGatherItems];
+#pragma unroll
+    for (int it = 0; it < kGatherItems; ++it) {
+        const uint64_t i = base + (uint64_t)it * kBlock;
+        s[it] = idx[i < n ? i : n - 1];
+    }
+    uint64_t v[kGatherItems];
+#pragma unroll
+    for (int it = 0; it < kGatherItems; ++it) v[it] = src[s[it]];
+#pragma unroll
+    for (int it = 0; it < kGatherItems; ++it) {
+        const uint64_t i = base + (uint64_t)it * kBlock;
+        if (i < n) dst[i] = v[it];
+    }
 }
-
-constexpr int kGatherItems = 4;
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) gather_state_kernel(const uint64_t* __restrict__ kin, const double2* __restrict__ ain, uint64_t cap_in,
@@ -583,7 +598,8 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hip
             } else {
                 {
                     SPQ_PROF(st.stream, "gather_word_kernel", n * 20.0);
-                    gather_word_kernel<<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys + (uint64_t)dg.word * st.cap, idxA, n, kwA);
+                    gather_word_kernel<<<grid_for(n, kBlock * kGatherItems), kBlock, 0, st.stream>>>(st.keys + (uint64_t)dg.word * st.cap,
+                                                                                                   idxA, n, kwA);
                 }
                 SPQ_HIP(hipGetLastError());
                 src = kwA;
