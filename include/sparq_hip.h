@@ -182,7 +182,8 @@ int sparq_state_relayout(sparq_state_t st, uint32_t new_words,
  * held before reuses its allocation. */
 int sparq_state_resize_words(sparq_state_t st, uint32_t new_words);
 int sparq_state_sync(sparq_state_t st);
-/* Raw device pointers of the current arrays (valid until the next call that changes the size). */
+/* Raw device pointers of the current arrays (valid until the next call that changes the size).
+   Once called, set_basis no longer marks the state's amplitude as known (the caller may write it). */
 int sparq_state_device_arrays(sparq_state_t st, uint64_t** keys, double** amps, uint64_t* capacity);
 /* The HIP stream (hipStream_t) that all work of this state is ordered on. */
 int sparq_state_stream(sparq_state_t st, void** stream);
