@@ -17,9 +17,10 @@
 // tables.  Parity unpinned (no independent check; the engine is compared with
 // this restatement only): the exact rounding of get_rational (only its 2^-w
 // bound is pinned), the summation order of >= 3 duplicates in
-// sort_merge_unique_erase, the float-to-fixed-point rule of SparseMatrix
-// (sparq_oracle_cks.hpp), and condition evaluation (ConditionSatisfied), which
-// the tests exercise only through the operators that carry conditions.
+// sort_merge_unique_erase, and the float-to-fixed-point rule of SparseMatrix
+// (sparq_oracle_cks.hpp).  Condition evaluation (Conditions::satisfied) is
+// pinned by an exhaustive truth table of every predicate kind and their
+// combinations (test_oracle.py::test_condition_truth_tables).
 //
 // Build knob: ORACLE_OMP enables the reference's own `#pragma omp parallel for`
 // loops on the per-entry operators (the reference ships with SINGLE_THREAD,

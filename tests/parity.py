@@ -4,6 +4,8 @@ test/GPUTest/Benchmark/CudaBenchmark.cu:40-74): register values bit-exact,
 amplitudes within a stated tolerance."""
 from __future__ import annotations
 
+import itertools
+
 import numpy as np
 
 # Amplitude tolerance of the parity tests: |a_gpu - a_ref| <= RTOL * max(|a_ref|, scale)
@@ -70,3 +72,49 @@ def run_both(ps, oracle, build, seed=0):
     so = build(oracle)
     eo = so.export()
     return eg, eo
+
+
+# ---- condition evaluation (global_macros.h:13-17, 82-86): an operator carrying
+# conditions acts only on entries where every listed predicate holds -- register
+# nonzero, all bits set, a given bit set, equal to a given value -- and leaves the
+# others untouched.  Brute force over every value of three small registers, with
+# a flip of a fourth as the conditioned operator.
+def check_condition_truth_tables(mod):
+    spec = [("c", "UnsignedInteger", 3), ("d", "UnsignedInteger", 2), ("s", "SignedInteger", 2), ("t", "Boolean", 1)]
+    combos = list(itertools.product(range(8), range(4), range(4)))
+    cases = [
+        (lambda op: op.conditioned_by_value("c", 5), lambda c, d, s: c == 5),
+        (lambda op: op.conditioned_by_value([("c", 3), ("d", 2)]), lambda c, d, s: c == 3 and d == 2),
+        (lambda op: op.conditioned_by_nonzeros("d"), lambda c, d, s: d != 0),
+        (lambda op: op.conditioned_by_nonzeros(["s", "c"]), lambda c, d, s: s != 0 and c != 0),
+        (lambda op: op.conditioned_by_all_ones("c"), lambda c, d, s: c == 7),
+        (lambda op: op.conditioned_by_all_ones(["d", "s"]), lambda c, d, s: d == 3 and s == 3),
+        (lambda op: op.conditioned_by_bit("c", 1), lambda c, d, s: (c >> 1) & 1 == 1),
+        (lambda op: op.conditioned_by_bit([("c", 0), ("d", 1)]), lambda c, d, s: c & 1 and d & 2),
+        (lambda op: op.conditioned_by_bit("c", 2).conditioned_by_nonzeros("s"), lambda c, d, s: c >= 4 and s != 0),
+        (lambda op: op.conditioned_by_value("d", 1).conditioned_by_all_ones("s").conditioned_by_bit("c", 0),
+         lambda c, d, s: d == 1 and s == 3 and c & 1),
+        (lambda op: op.conditioned_by_value("c", 6).conditioned_by_value("c", 2), lambda c, d, s: c == 2),   # replaces
+    ]
+    for k, (cond, pred) in enumerate(cases):
+        mod.System.clear()
+        st, _ = _regs(mod, spec)
+        vals = np.array([[c, d, s, 0] for c, d, s in combos], dtype=np.uint64)
+        amps = np.arange(1, len(combos) + 1, dtype=complex) * (1 + 0.5j)
+        st.load(vals, amps)
+        cond(mod.X_Bool("t", 0))(st)
+        got = _amp_map(st)
+        want = {(c, d, s, int(bool(pred(c, d, s)))): a for (c, d, s), a in zip(combos, amps)}
+        assert got.keys() == want.keys(), ("condition case", k)
+        assert all(got[key] == want[key] for key in want), ("condition case", k)
+
+
+def _regs(mod, spec):
+    st = mod.SparseState()
+    ids = [mod.AddRegister(n, getattr(mod, t), w)(st) for n, t, w in spec]
+    return st, ids
+
+
+def _amp_map(st):
+    ids, vals, amps = st.export()
+    return {tuple(int(x) for x in v): a for v, a in zip(vals, amps)}

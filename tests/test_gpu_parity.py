@@ -8,7 +8,7 @@ bit-identical.
 import numpy as np
 import pytest
 
-from parity import assert_same_state, random_state, run_both
+from parity import assert_same_state, check_condition_truth_tables, random_state, run_both
 
 pytestmark = pytest.mark.gpu
 
@@ -470,3 +470,10 @@ def test_empty_state_ops(ps):
     ps.X_Bool("a", 1)(st)
     ps.Hadamard_Bool  # noqa: B018
     assert st.size() == 0
+
+
+def test_condition_truth_tables(ps):
+    """Condition evaluation on the engine against the predicates of
+    global_macros.h:13-17, 82-86, brute force over three small registers (the
+    same table pins the oracle); moved keys keep their amplitudes bit for bit."""
+    check_condition_truth_tables(ps)

@@ -9,6 +9,8 @@ import itertools
 import numpy as np
 import pytest
 
+from parity import check_condition_truth_tables
+
 
 def reg_state(o, spec):
     st = o.SparseState()
@@ -410,3 +412,10 @@ def test_simple_key_rewrite_truth_tables(oracle):
             assert got["l"] == (l + ext) % 16 and got["x"] == x, ("add_any", rtype, l, x)
             got = run_classical(oracle, spec, {"l": l, "x": x}, lambda: oracle.Add_AnyInt_AnyInt_InPlace("l", "x"), dagger=True)
             assert got["l"] == (l - ext) % 16, ("add_any dag", rtype, l, x)
+
+
+
+# ---- condition evaluation (global_macros.h:13-17, 82-86): brute force over every
+# value of three small registers against Python predicates (tests/parity.py).
+def test_condition_truth_tables(oracle):
+    check_condition_truth_tables(oracle)
