@@ -18,7 +18,7 @@ try:
 except ImportError as exc:  # pragma: no cover - exercised only when unbuilt
     raise ImportError(
         "sparqsim_amd native extension is not built or failed to load "
-        f"({exc}); run `python -m sparqsim_amd.build`"
+        f"({exc}); run `python sparqsim_amd/build.py`"
     ) from exc
 
 from ._core import *  # noqa: F401,F403,E402

@@ -8,7 +8,7 @@
 * ``_core.<ext>.so``   -- the pybind11 bindings (csrc/host/core_py.cpp).
 
 Every library finds its neighbours through an $ORIGIN rpath.
-Run ``python -m sparqsim_amd.build`` (or ``__graft_entry__.build()``).
+Run ``python sparqsim_amd/build.py`` (or ``__graft_entry__.build()``).
 """
 from __future__ import annotations
 

@@ -460,7 +460,8 @@ int sparq_add_systems(sparq_state_t hc, sparq_state_t ha, double coef, double ep
         if (add.n == 0) return;
         SPQ_HIP(hipEventRecord(add.ev[7], add.stream));
         SPQ_HIP(hipStreamWaitEvent(cur.stream, add.ev[7], 0));
-        const uint64_t orig = cur.n, total = cur.n + add.n;
+        // `add` may be `cur` itself: its size is read before the arrays are replaced
+        const uint64_t orig = cur.n, add_n = add.n, total = orig + add_n;
         uint64_t* k;
         double2* a;
         alloc_arrays(cur, cur.W, total, &k, &a);
@@ -468,13 +469,13 @@ int sparq_add_systems(sparq_state_t hc, sparq_state_t ha, double coef, double ep
         out.keys = k;
         out.amps = a;
         out.cap = total;
-        copy_entries(out, 0, cur, 0, cur.n);
-        copy_entries(out, cur.n, add, 0, add.n);
+        copy_entries(out, 0, cur, 0, orig);
+        copy_entries(out, orig, add, 0, add_n);
         install_arrays(cur, cur.W, k, a, total, total);
         if (std::abs(coef - 1.0) > 1e-14) {
             State tail = cur;   // view of the appended entries
             tail.amps = cur.amps + orig;
-            tail.n = add.n;
+            tail.n = add_n;
             scale_amps(tail, coef);
         }
         if (orig != 0) {
@@ -495,9 +496,24 @@ int sparq_check_duplicates(sparq_state_t h, int* has_dup) {
     });
 }
 
+// The group transforms (Hadamard_Int, Hadamard_Int_Full, Rot_GeneralUnitary)
+// evaluate their conditions per entry; the reference evaluates them on a group's
+// first entry (hadamard.cpp:28,43,80).  The two agree unless a condition reads
+// the bits the transform changes, which is rejected.
+static void require_conditions_off_target(sparq_field_t target, uint32_t n_digits, const sparq_cond_t* conds, uint32_t ncond) {
+    SPQ_REQUIRE(conds != nullptr || ncond == 0, "null conditions");
+    const uint32_t lo = target.shift, hi = (uint32_t)target.shift + n_digits;
+    for (uint32_t c = 0; c < ncond; ++c) {
+        const sparq_field_t f = conds[c].field;
+        const bool overlap = f.word == target.word && f.shift < hi && (uint32_t)f.shift + f.width > lo;
+        SPQ_REQUIRE(!overlap, "a condition of a group transform reads the transformed bits of its target register");
+    }
+}
+
 int sparq_hadamard_int(sparq_state_t h, sparq_field_t target, uint32_t n_digits, const sparq_cond_t* conds, uint32_t ncond,
                        double eps) {
     return guarded([&] {
+        require_conditions_off_target(target, n_digits, conds, ncond);
         State& st = S_keep_basis(h);
         const bool unit = st.basis_amp_one && ncond == 0;
         st.basis_amp_one = false;
@@ -507,6 +523,7 @@ int sparq_hadamard_int(sparq_state_t h, sparq_field_t target, uint32_t n_digits,
 
 int sparq_hadamard_int_full(sparq_state_t h, sparq_field_t target, const sparq_cond_t* conds, uint32_t ncond, double eps) {
     return guarded([&] {
+        require_conditions_off_target(target, target.width, conds, ncond);
         State& st = S(h);
         // CheckDuplicateKey precedes the transform (hadamard.cpp:183)
         if (st.n > 1) {
@@ -523,6 +540,7 @@ int sparq_group_unitary(sparq_state_t h, sparq_field_t target, const double* mat
                         double eps) {
     return guarded([&] {
         SPQ_REQUIRE(mat != nullptr, "null matrix");
+        require_conditions_off_target(target, target.width, conds, ncond);
         State& st = S(h);
         with_conditions(st, conds, ncond, eps, [&](State& s) { group_unitary(s, target, mat, eps); });
     });

@@ -183,11 +183,19 @@ __global__ void __launch_bounds__(kBlock) expand_scatter_kernel(const uint64_t* 
     uint64_t k[W];
     load_key<W>(keys, cap, i, k);
     const double2 a = amps[i];
+    // The grouping sort is stable, so equal keys (a state uploaded with
+    // duplicates) are neighbours: only the last of them is placed, so the
+    // slot's value does not depend on the order of the writes.
+    const uint64_t inext = i + 1 < n ? i + 1 : i;
+    const uint64_t wnext = keys[(uint64_t)low.word * cap + inext];
+    const uint32_t hnext = head ? head[inext] : 0u;
     if (G > G_cap) return;
+    const uint64_t vlow = fget<W>(k, low);
+    if (inext != i && hnext == 0 && ((wnext >> low.shift) & fmask(low.width)) == vlow) return;
     // group id of entry i = inclusive count of heads up to i, minus one
     const uint64_t g = p + h - 1;
     if (cls[g] != kWht) return;
-    aout[(g << nd) + fget<W>(k, low)] = a;
+    aout[(g << nd) + vlow] = a;
 }
 
 // Walsh-Hadamard butterflies.  Stage q pairs indices i, i + 2^q of a group's

@@ -258,7 +258,6 @@ float g_ms_per_pass[2][64];        // [8-bit, 11-bit][log2 N]: ms per pass per 2
 int g_tune_phase[64];              // per class: 0, 1 (8-bit warm-up, measured), 2, 3 (11-bit), 4 done
 bool g_tune_init = false;
 int g_forced_bits = 0;
-hipEvent_t g_tune_ev[2] = {nullptr, nullptr};
 }  // namespace
 
 void set_radix_bits(int bits) {
@@ -290,7 +289,9 @@ void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
                 if (ph & 1) measure = use;
                 g_tune_phase[lg] = ph + 1;
             } else {
-                use = (p11.size() * g_ms_per_pass[1][lg] < 0.9f * p8.size() * g_ms_per_pass[0][lg]) ? 1 : 0;
+                // a measurement still running on another thread reads as unmeasured
+                const float c8 = g_ms_per_pass[0][lg], c11 = g_ms_per_pass[1][lg];
+                use = (c8 > 0.f && c11 > 0.f && p11.size() * c11 < 0.9f * p8.size() * c8) ? 1 : 0;
             }
         }
     }
@@ -299,17 +300,19 @@ void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
         sort_by_plan(st, plan, use ? 11 : 8);
         return;
     }
-    if (!g_tune_ev[0]) {
-        SPQ_HIP(hipEventCreate(&g_tune_ev[0]));
-        SPQ_HIP(hipEventCreate(&g_tune_ev[1]));
-    }
+    // events of this call only: sorts of other states may be timed concurrently
+    hipEvent_t ev[2];
+    SPQ_HIP(hipEventCreate(&ev[0]));
+    SPQ_HIP(hipEventCreate(&ev[1]));
     const uint64_t n = st.n;
-    sort_by_plan(st, plan, use ? 11 : 8, g_tune_ev[0], g_tune_ev[1]);
-    SPQ_HIP(hipEventSynchronize(g_tune_ev[1]));
+    sort_by_plan(st, plan, use ? 11 : 8, ev[0], ev[1]);
+    SPQ_HIP(hipEventSynchronize(ev[1]));
     float ms = 0.f;
-    SPQ_HIP(hipEventElapsedTime(&ms, g_tune_ev[0], g_tune_ev[1]));
+    SPQ_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    SPQ_HIP(hipEventDestroy(ev[0]));
+    SPQ_HIP(hipEventDestroy(ev[1]));
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_ms_per_pass[measure][lg] = ms / (float)plan.size() / (float)((double)n / 1048576.0);
+    g_ms_per_pass[measure][lg] = std::max(ms, 1e-6f) / (float)plan.size() / (float)((double)n / 1048576.0);
 }
 
 bool sort_state(State& st, const sparq_field_t* last) {

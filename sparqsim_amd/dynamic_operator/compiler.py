@@ -176,7 +176,7 @@ def compile_cpp_code(cpp_code: str, class_name: str, cache_dir: Optional[str] = 
     if find_project_root() is None:
         raise RuntimeError("sparqsim_amd host headers not found")
     if not (PKG / "libsparq_host.so").exists():
-        raise RuntimeError("libsparq_host.so is not built (run python -m sparqsim_amd.build)")
+        raise RuntimeError("libsparq_host.so is not built (run python sparqsim_amd/build.py)")
     code_hash = compute_code_hash(cpp_code, class_name, cfg)
     cache_dir = cache_dir or DEFAULT_CACHE
     os.makedirs(cache_dir, exist_ok=True)

@@ -253,6 +253,35 @@ def test_hadamard_int_conditioned(ps, oracle):
     assert_same_state(eg, eo, what="Hadamard_Int conditioned")
 
 
+def test_group_transform_condition_on_target_rejected(ps):
+    """A condition that reads the transformed bits of a group transform's own
+    target is refused (the reference decides such a group by its first entry,
+    hadamard.cpp:28,43,80, the engine per entry); a condition on the target's
+    untouched upper bits is accepted.  The state is left unchanged."""
+    st = loaded(ps, [("x", "UnsignedInteger", 5), ("y", "UnsignedInteger", 4)], 200, seed=22)
+    before = st.export()
+    with pytest.raises(ValueError):
+        ps.Hadamard_Int("x", 5).conditioned_by_bit("x", 2)(st)
+    with pytest.raises(ValueError):
+        ps.Hadamard_Int_Full("x").conditioned_by_nonzeros("x")(st)
+    assert_same_state(st.export(), before, exact_amps=True, what="state after a refused operator")
+
+
+def test_hadamard_int_conditioned_on_upper_target_bits(ps, oracle):
+    """Hadamard_Int on the low 3 bits of x, conditioned on bit 4 of x: every
+    member of a group shares the upper bits, so per-entry and per-group
+    evaluation agree and the engine matches the oracle."""
+    spec = [("x", "UnsignedInteger", 5), ("y", "UnsignedInteger", 4)]
+
+    def build(mod):
+        st = loaded(mod, spec, 300, seed=23)
+        mod.Hadamard_Int("x", 3).conditioned_by_bit("x", 4)(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, what="Hadamard_Int conditioned on its upper bits")
+
+
 @pytest.mark.parametrize("n", [1, 3, 40, 256])
 def test_hadamard_int_full(ps, oracle, n):
     spec = [("x", "UnsignedInteger", 8), ("y", "UnsignedInteger", 3)]
@@ -346,6 +375,42 @@ def test_add_systems_and_normalize(ps, oracle):
 
     eg, eo = run_both(ps, oracle, build)
     assert_same_state(eg, eo, what="add_systems")
+
+
+def test_hadamard_int_duplicate_key_keeps_last(ps):
+    """A state uploaded with a duplicated key (the reference's general-group
+    Hadamard writes every member into its slot, hadamard.cpp:109-116, so one
+    copy overwrites the other): the engine's grouping sort is stable and it
+    places the later copy, every run the same.  Group b=1 holds a=3 (twice,
+    amplitudes x then z) and a=5 (y); group b=2 holds only duplicates."""
+    st = regs(ps, [("a", "UnsignedInteger", 4), ("b", "UnsignedInteger", 3)])
+    x, y, z, u, v = 0.3 + 0.1j, -0.2 + 0.4j, 0.5 - 0.25j, 0.125j, -0.75
+    vals = np.array([[3, 1], [5, 1], [3, 1], [6, 2], [6, 2]], dtype=np.uint64)
+    st.load(vals, np.array([x, y, z, u, v]))
+    ps.Hadamard_Int("a", 4)(st)
+    ids, got_v, got_a = st.export()
+    got = {(int(a), int(b)): amp for (a, b), amp in zip(got_v, got_a)}
+    sign = lambda j, m: -1.0 if bin(j & m).count("1") & 1 else 1.0  # noqa: E731
+    want = {}
+    for j in range(16):
+        want[(j, 1)] = (z * sign(j, 3) + y * sign(j, 5)) / 4
+        want[(j, 2)] = v * sign(j, 6) / 4
+    want = {k: a for k, a in want.items() if abs(a) ** 2 >= 1e-14}
+    assert got.keys() == want.keys()
+    assert max(abs(got[k] - want[k]) for k in want) <= 1e-15
+
+
+def test_add_systems_to_itself(ps):
+    """add_systems(s, s, c) appends s scaled by c and merges every key with its
+    copy: each amplitude becomes a*(1+c) (one sum of two terms, so bit-exact
+    against numpy); nothing falls under the merge threshold."""
+    spec = [("a", "UnsignedInteger", 9), ("b", "UnsignedInteger", 4)]
+    st = loaded(ps, spec, 900, seed=52)
+    ids, vals, amps = st.export()
+    ps.add_systems(st, st, 0.5)
+    ps.add_systems(st, st, 1.0)
+    want = (amps + amps * 0.5) * 2.0
+    assert_same_state(st.export(), (ids, vals, want), exact_amps=True, what="add_systems(s, s)")
 
 
 def test_sort_unconditional_order(ps, oracle):
