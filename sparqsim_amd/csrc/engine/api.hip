@@ -505,7 +505,10 @@ static void require_conditions_off_target(sparq_field_t target, uint32_t n_digit
     const uint32_t lo = target.shift, hi = (uint32_t)target.shift + n_digits;
     for (uint32_t c = 0; c < ncond; ++c) {
         const sparq_field_t f = conds[c].field;
-        const bool overlap = f.word == target.word && f.shift < hi && (uint32_t)f.shift + f.width > lo;
+        // a bit condition reads one bit of its register, the others the whole field
+        const uint32_t flo = conds[c].kind == SPARQ_COND_BIT ? f.shift + (uint32_t)conds[c].value : f.shift;
+        const uint32_t fhi = conds[c].kind == SPARQ_COND_BIT ? flo + 1 : (uint32_t)f.shift + f.width;
+        const bool overlap = f.word == target.word && flo < hi && fhi > lo;
         SPQ_REQUIRE(!overlap, "a condition of a group transform reads the transformed bits of its target register");
     }
 }

@@ -163,6 +163,55 @@ def test_caching(tmp_path):
     assert get_cache_info(d)["file_count"] == 0
 
 
+def test_constructor_parameter_types():
+    """C++ parameter spellings map to the ctypes type of the same width and
+    signedness (LP64); unknown spellings are refused at load time."""
+    from sparqsim_amd.dynamic_operator.operator_wrapper import ctype_of
+
+    want = {"size_t": ctypes.c_size_t, "const size_t": ctypes.c_size_t, "std::size_t": ctypes.c_size_t,
+            "int": ctypes.c_int32, "unsigned int": ctypes.c_uint32, "unsigned": ctypes.c_uint32,
+            "long": ctypes.c_int64, "unsigned long long": ctypes.c_uint64, "short": ctypes.c_int16,
+            "uint8_t": ctypes.c_uint8, "std::int64_t": ctypes.c_int64, "double": ctypes.c_double,
+            "float": ctypes.c_float, "bool": ctypes.c_bool, "const char*": ctypes.c_char_p,
+            "char *": ctypes.c_char_p, "double*": ctypes.c_void_p, "unsigned char": ctypes.c_ubyte,
+            "signed char": ctypes.c_byte}
+    for spelling, t in want.items():
+        assert ctype_of(spelling) is t, spelling
+    from sparqsim_amd.dynamic_operator import DynamicOperatorLoadError
+
+    with pytest.raises(DynamicOperatorLoadError):
+        ctype_of("std::vector<int>")
+
+
+def test_proxy_lifetime_and_arguments(cache):
+    """Positional and keyword construction, the errors of a bad call, close()
+    and cleanup_all_instances destroy the C++ object once; a missing library
+    is a load error."""
+    from sparqsim_amd.dynamic_operator import (DynamicOperatorLoadError, cleanup_all_instances,
+                                               create_operator_class)
+
+    cls = compile_operator("TestPhaseOp", PHASE, "BaseOperator",
+                           constructor_args=[("size_t", "reg_id"), ("double", "phase")], cache_dir=cache)
+    a = cls(1, phase=0.5)
+    assert repr(a) == "TestPhaseOp(reg_id=1, phase=0.5)"
+    with pytest.raises(TypeError):
+        cls(1, 0.5, 3)
+    with pytest.raises(TypeError):
+        cls(1, reg_id=2, phase=0.1)
+    with pytest.raises(TypeError):
+        cls(reg_id=1, angle=0.1)
+    a.close()
+    a.close()
+    with pytest.raises(RuntimeError):
+        a._ptr()
+    b = cls(reg_id=0, phase=0.25)
+    cleanup_all_instances()
+    with pytest.raises(RuntimeError):
+        b._ptr()
+    with pytest.raises(DynamicOperatorLoadError):
+        create_operator_class("Nothing", "/nonexistent/libnothing.so")
+
+
 # ---------------------------------------------------------------- GPU
 def _setup(mod, seed):
     mod.System.clear()
