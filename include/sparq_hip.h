@@ -206,6 +206,27 @@ int sparq_clear_zero(sparq_state_t st, double eps);
  * to the least significant position (SortExceptKey, sort_state.cpp:10-36). */
 int sparq_sort(sparq_state_t st, const sparq_field_t* last);
 
+/* Stable sort by an ordering of fields: the `major` fields in the given order
+ * (most significant first), then every key bit not in a listed field in
+ * canonical order, then the `minor` fields in the given order.  Fields must not
+ * overlap.  SortByKey (sort_state.cpp:46-53: major = {key}), SortByKey2
+ * (:101-131: major = {key1, key2}), SortExceptKey (:10-36: minor = {key}),
+ * SortExceptBit (:55-82: minor = {key without the bit, the bit}),
+ * SortExceptKeyHadamard (:138-171: minor = the listed qubits of key, highest
+ * first).  Entries equal under the ordering keep their order (the reference's
+ * std::sort leaves it unspecified). */
+int sparq_sort_by(sparq_state_t st, const sparq_field_t* major, uint32_t nmajor, const sparq_field_t* minor,
+                  uint32_t nminor);
+/* SortByAmplitude (sort_state.cpp:90-99): |amp|^2 descending, stable. */
+int sparq_sort_by_amplitude(sparq_state_t st);
+/* split_systems (system_operations.cpp:53-106): the entries of `st` failing
+ * any condition move, in order, to `rest` (whose content is replaced); those
+ * satisfying all stay in `st`, in order.  Both states have the same key width. */
+int sparq_partition(sparq_state_t st, const sparq_cond_t* conds, uint32_t ncond, sparq_state_t rest);
+/* combine_systems (system_operations.cpp:53-106): append the entries of `from`
+ * (unchanged) after those of `to`. */
+int sparq_append(sparq_state_t to, sparq_state_t from);
+
 /* sort_merge_unique_erase: sort, sum amplitudes of equal keys, drop |sum|^2 < eps
  * (system_operations.h:157-181, basic_components.cpp:517-526). */
 int sparq_merge_duplicates(sparq_state_t st, double eps);

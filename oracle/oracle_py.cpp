@@ -169,6 +169,8 @@ PYBIND11_MODULE(_oracle, m) {
     }, py::arg("eps") = py::none());
     m.def("set_clearzero_default_eps", [](double e) { clearzero_default_eps() = e; });
     m.def("get_clearzero_default_eps", []() { return clearzero_default_eps(); });
+    // fixed-point encoding of a fraction into a w-bit Rational (checked by tests/test_oracle.py)
+    m.def("get_rational", [](double x, size_t w) { return get_rational(x, w); });
     m.def("Normalize", []() { return fnop([](State& st) { Normalize(st); }, "Normalize"); });
     m.def("SortUnconditional", []() { return fnop([](State& st) { SortUnconditional(st); }, "SortUnconditional"); });
     m.def("Init_Unsafe", [](py::object r, uint64_t v) {
@@ -194,7 +196,9 @@ PYBIND11_MODULE(_oracle, m) {
         std::string b = py::isinstance<py::str>(s) ? s.cast<std::string>() : Registry::name_of(s.cast<size_t>());
         return py::cpp_function([a, b](PyState& p) { return CombineRegister(p.s.basis_states, a, b); });
     });
-    m.def("add_systems", [](PyState& cur, const PyState& add, double coef) { add_systems(cur.s.basis_states, add.s.basis_states, coef); });
+    // eps: merge threshold on |sum|^2 (the reference always uses the library epsilon, 1e-14)
+    m.def("add_systems", [](PyState& cur, const PyState& add, double coef, double eps) { add_systems(cur.s.basis_states, add.s.basis_states, coef, eps); },
+          py::arg("cur"), py::arg("add"), py::arg("coef"), py::arg("eps") = epsilon);
     m.def("reset_systems", [](PyState& p) { p.s.basis_states.clear(); p.s.basis_states.emplace_back(); });   // system_operations.cpp:108-117
     m.def("CheckDuplicateKey", [](PyState& p) { CheckDuplicateKey(p.s.basis_states); });
 

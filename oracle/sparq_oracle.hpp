@@ -14,11 +14,15 @@
 // Parity status.  tests/test_oracle.py, test_qda.py, test_grover.py, test_cks.py
 // and test_measurement.py pin the functions here against the reference's known
 // answers and golden lists, closed forms, dense models and exhaustive truth
-// tables.  Parity unpinned (no independent check; the engine is compared with
-// this restatement only): the exact rounding of get_rational (only its 2^-w
-// bound is pinned), the summation order of >= 3 duplicates in
-// sort_merge_unique_erase, and the float-to-fixed-point rule of SparseMatrix
-// (sparq_oracle_cks.hpp).  Condition evaluation (Conditions::satisfied) is
+// tables.  get_rational is pinned at every width 3..64 by the reference's
+// 2^-w bound and exact dyadics (test_oracle.py::
+// test_get_rational_reference_bound_every_width); the SparseMatrix encoder by
+// the reference front end's clip bounds (largest codes), exact dyadics, one
+// code step and the sign (test_sparse_matrix_quantisation_against_float_input).
+// Within those bounds the rounding mode (truncation) is this build's choice.
+// Parity unpinned (no independent check; the engine is compared with this
+// restatement only): the summation order of >= 3 duplicates in
+// sort_merge_unique_erase.  Condition evaluation (Conditions::satisfied) is
 // pinned by an exhaustive truth table of every predicate kind and their
 // combinations (test_oracle.py::test_condition_truth_tables).
 //
@@ -74,8 +78,9 @@ inline int bitcount(uint64_t v) { return __builtin_popcountll(v); }
 inline bool bit_parity(uint64_t v) { return __builtin_popcountll(v) & 1; }
 // Fixed-point encoding of a fraction x in [0,1) into w bits.  The reference
 // only pins |decode(encode(x)) - x| <= 2^-w
-// (test/CPUTest/CommonTest/CorrectnessTest_Common.inl:171-195); truncation
-// satisfies that bound.  Exact rounding: parity unpinned.
+// (test/CPUTest/CommonTest/CorrectnessTest_Common.inl:171-195), checked at
+// every width 3..64 together with exact dyadics; the rule inside that bound is
+// truncation, floor(x 2^w).
 inline uint64_t get_rational(double x, size_t w) {
     double scaled = (w >= 64) ? x * 2.0 * double(pow2(63)) : x * double(pow2(w));
     uint64_t v = scaled <= 0 ? 0 : (uint64_t)scaled;
@@ -308,7 +313,7 @@ inline void ClearZero(State& st) { ClearZero(st, clearzero_default_eps()); }
 // remove_system (|a|^2 < epsilon, basic_components.cpp:523-526) holds.
 // Duplicates are folded in sorted order (the reference's std::sort is unstable,
 // so its summation order for >= 3 duplicates is unspecified: parity unpinned).
-inline void sort_merge_unique_erase(State& st) {
+inline void sort_merge_unique_erase(State& st, double eps = epsilon) {
     std::stable_sort(st.begin(), st.end());
     State out;
     out.reserve(st.size());
@@ -317,19 +322,19 @@ inline void sort_merge_unique_erase(State& st) {
         System head = st[i];
         size_t j = i + 1;
         while (j < st.size() && st[j] == head) { head.amplitude += st[j].amplitude; ++j; }
-        if (!(abs_sqr(head.amplitude) < epsilon)) out.push_back(std::move(head));
+        if (!(abs_sqr(head.amplitude) < eps)) out.push_back(std::move(head));
         i = j;
     }
     st.swap(out);
 }
 // add_systems (system_operations.h:157-181)
-inline void add_systems(State& cur, const State& add, double coef) {
+inline void add_systems(State& cur, const State& add, double coef, double eps = epsilon) {
     if (add.empty()) return;
     size_t orig = cur.size();
     cur.insert(cur.end(), add.begin(), add.end());
     if (std::abs(coef - 1.0) > epsilon)
         for (size_t i = orig; i < cur.size(); ++i) cur[i].amplitude *= coef;
-    if (orig != 0) sort_merge_unique_erase(cur);
+    if (orig != 0) sort_merge_unique_erase(cur, eps);
 }
 
 // split_systems / combine_systems (system_operations.cpp:53-106):

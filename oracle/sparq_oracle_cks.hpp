@@ -13,9 +13,12 @@ namespace oracle {
 // ---- sparse matrix in compact QRAM storage.  The memory is [elements | sparsity],
 // zero-padded to a power of two; the tridiagonal case reproduces
 // TridiagonalSpec::combined_mem (hamiltonian_simulation_tridiagonal.h:169-178).
-// The float quantisation lives in the absent submodule: floor(x*2^ds) for
-// positive-only matrices, the two's complement of trunc(x*2^(ds-1)) otherwise
-// (parity unpinned for that constructor).
+// The float quantisation lives in the absent submodule; the reference's Python
+// front end fixes its scale and range by clipping every value to
+// (2^ds-1)/2^ds (positive only) or +-(2^(ds-1)-1)/2^(ds-1)
+// (PySparQ/pysparq/algorithms/cks_solver.py:325-342).  Here: floor(x*2^ds) for
+// positive-only matrices, the two's complement of trunc(x*2^(ds-1)) otherwise;
+// both within one code step and exact on dyadics (tests/test_oracle.py).
 struct SparseMatrix {
     std::vector<uint64_t> elements;
     std::vector<size_t> sparsity;

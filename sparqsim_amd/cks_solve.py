@@ -33,16 +33,33 @@ def make_rhs(n: int, R: int, seed: int = 0x5EED2026) -> np.ndarray:
 
 
 def solve(ps, nqubit=20, R=1, kappa=16.0, eps=0.01, ds=32, coef="true", J=640, terms=None, log_every=64, tol=1e-6,
-          alpha=2.2666666666564934, beta=-1.0, log=print, matrix="poiseuille"):
+          alpha=2.2666666666564934, beta=-1.0, log=print, matrix="poiseuille", fit_top=None, clearzero_eps=None,
+          merge_eps=None):
+    """clearzero_eps: the ClearZero threshold of every walk operator (the process default is restored
+    afterwards); merge_eps: the accumulator's add_systems threshold.  None keeps 1e-14 for both."""
+    n = 1 << nqubit
+    saved_eps = ps.get_clearzero_default_eps()
+    if clearzero_eps is not None:
+        ps.set_clearzero_default_eps(clearzero_eps)
+    try:
+        return _solve(ps, nqubit, R, kappa, eps, ds, coef, J, terms, log_every, tol, alpha, beta, log, matrix, fit_top,
+                      merge_eps)
+    finally:
+        ps.set_clearzero_default_eps(saved_eps)
+
+
+def _solve(ps, nqubit, R, kappa, eps, ds, coef, J, terms, log_every, tol, alpha, beta, log, matrix, fit_top, merge_eps):
     n = 1 << nqubit
     t_setup = time.perf_counter()
     if matrix == "poiseuille":
         batch = cks.BatchedPoiseuilleCKS(ps, nqubit, make_rhs(n, R), kappa=kappa, eps=eps, ds=ds, alpha=alpha, beta=beta,
-                                         coef=coef, J=J)
+                                         coef=coef, J=J, fit_top=fit_top or "spectrum")
     elif matrix == "band-random":
-        batch = cks.BatchedSparseCKS(ps, nqubit, make_rhs(n, R), kappa=kappa, ds=ds, J=J, eps=eps)
+        batch = cks.BatchedSparseCKS(ps, nqubit, make_rhs(n, R), kappa=kappa, ds=ds, J=J, eps=eps,
+                                     fit_top=fit_top or "unit")
     else:
         raise ValueError("matrix must be 'poiseuille' or 'band-random'")
+    batch.lcu.merge_eps = merge_eps
     batch.prepare_input()
     after_first = batch.lcu.step_state.size()
     t_setup = time.perf_counter() - t_setup
@@ -71,6 +88,9 @@ def solve(ps, nqubit=20, R=1, kappa=16.0, eps=0.01, ds=32, coef="true", J=640, t
     wall = time.perf_counter() - t0
     last = curve[-1]
     return {"n": n, "nqubit": nqubit, "rhs": R, "matrix": matrix, "kappa": kappa, "eps": eps, "ds": ds, "coef": coef,
+            "fit_interval": getattr(batch, "fit_interval", None), "clearzero_eps": ps.get_clearzero_default_eps(),
+            "merge_eps": merge_eps if merge_eps is not None else 1e-14,
+            "sum_abs_coef": float(np.sum(np.abs(batch.lcu.inv_coef))) if batch.lcu.inv_coef is not None else None,
             "J": J if coef == "true" else None, "lcu_terms_budget": batch.lcu.j0 + 1, "terms_run": j,
             "entries_after_first_step": after_first, "termination": termination, "wall_s": wall, "walk_s": wall - t_extract,
             "s_per_term": (wall - t_extract) / max(j, 1), "setup_s": t_setup,
@@ -118,6 +138,13 @@ def main():
     ap.add_argument("--outdir", default=None,
                     help="write residual_curve.csv and qda_summary_cpp.json here (the reference bench's artifacts, "
                          "right-hand side 0)")
+    ap.add_argument("--fit-top", choices=["unit", "spectrum"], default=None,
+                    help="upper end of the 1/x fit: 1 (default for band-random) or the largest walk eigenvalue "
+                         "(default for the Poiseuille case, as the reference fits)")
+    ap.add_argument("--clearzero-eps", type=float, default=None,
+                    help="ClearZero threshold of the walk operators on |a|^2 (default 1e-14)")
+    ap.add_argument("--merge-eps", type=float, default=None,
+                    help="threshold of the accumulator's add_systems merge on |a|^2 (default 1e-14)")
     ap.add_argument("--matrix", choices=["poiseuille", "band-random"], default="poiseuille",
                     help="tridiagonal Poiseuille case (direct oracles) or a seeded band-plus-random matrix "
                          "(QRAM-backed sparse-access oracles, fitted coefficients; --kappa sets its condition number)")
@@ -125,7 +152,8 @@ def main():
     import sparqsim_amd as ps
 
     res = solve(ps, args.nqubit, args.rhs, args.kappa, args.eps, args.ds, args.coef, args.J, args.terms, args.log_every,
-                args.tol, log=lambda s: print(s, flush=True), matrix=args.matrix)
+                args.tol, log=lambda s: print(s, flush=True), matrix=args.matrix, fit_top=args.fit_top,
+                clearzero_eps=args.clearzero_eps, merge_eps=args.merge_eps)
     summary = {k: v for k, v in res.items() if k not in ("curve", "final")}
     summary["final_worst_res_dir"] = max(r["res_dir"] for r in res["final"])
     summary["final_rhs0"] = res["final"][0]

@@ -444,6 +444,74 @@ int sparq_sort(sparq_state_t h, const sparq_field_t* last) {
     });
 }
 
+int sparq_sort_by(sparq_state_t h, const sparq_field_t* major, uint32_t nmajor, const sparq_field_t* minor, uint32_t nminor) {
+    return guarded([&] {
+        SPQ_REQUIRE((major != nullptr || nmajor == 0) && (minor != nullptr || nminor == 0), "null field list");
+        State& st = S(h);
+        sort_state_by(st, std::vector<sparq_field_t>(major, major + nmajor), std::vector<sparq_field_t>(minor, minor + nminor));
+    });
+}
+
+int sparq_sort_by_amplitude(sparq_state_t h) {
+    return guarded([&] { sort_by_amplitude(S(h)); });
+}
+
+// Make `dst`'s stream wait for the work queued so far on `src`'s stream.
+static void stream_after(State& dst, State& src) {
+    if (dst.stream == src.stream) return;
+    SPQ_HIP(hipEventRecord(src.ev[7], src.stream));
+    SPQ_HIP(hipStreamWaitEvent(dst.stream, src.ev[7], 0));
+}
+
+int sparq_partition(sparq_state_t h, const sparq_cond_t* conds, uint32_t ncond, sparq_state_t hrest) {
+    return guarded([&] {
+        SPQ_REQUIRE(conds != nullptr || ncond == 0, "null conditions");
+        SPQ_REQUIRE(h != hrest, "the remainder must be another state");
+        State& st = S(h);
+        State& rest = S(hrest);
+        SPQ_REQUIRE(rest.W == st.W, "states have different key widths");
+        for (uint32_t c = 0; c < ncond; ++c) SPQ_REQUIRE(conds[c].field.word < st.W, "condition field out of range");
+        const uint64_t kept = ncond ? partition_by_conditions(st, conds, ncond) : st.n;
+        const uint64_t moved = st.n - kept;
+        // the remainder's new arrays are written on st's stream after rest's pending work
+        stream_after(st, rest);
+        uint64_t* k;
+        double2* a;
+        alloc_arrays(rest, rest.W, std::max<uint64_t>(moved, 1), &k, &a);
+        State out = rest;
+        out.keys = k;
+        out.amps = a;
+        out.cap = std::max<uint64_t>(moved, 1);
+        out.stream = st.stream;
+        copy_entries(out, 0, st, kept, moved);
+        stream_after(rest, st);
+        install_arrays(rest, rest.W, k, a, moved, out.cap);
+        st.n = kept;
+    });
+}
+
+int sparq_append(sparq_state_t hto, sparq_state_t hfrom) {
+    return guarded([&] {
+        State& to = S(hto);
+        State& from = S(hfrom);
+        SPQ_REQUIRE(to.W == from.W, "states have different key widths");
+        if (from.n == 0) return;
+        stream_after(to, from);
+        const uint64_t orig = to.n, add_n = from.n, total = orig + add_n;
+        uint64_t* k;
+        double2* a;
+        alloc_arrays(to, to.W, total, &k, &a);
+        State out = to;
+        out.keys = k;
+        out.amps = a;
+        out.cap = total;
+        copy_entries(out, 0, to, 0, orig);
+        copy_entries(out, orig, from, 0, add_n);
+        install_arrays(to, to.W, k, a, total, total);
+        stream_after(from, to);   // `from` stays unchanged until the copy has run
+    });
+}
+
 int sparq_merge_duplicates(sparq_state_t h, double eps) {
     return guarded([&] {
         State& st = S(h);

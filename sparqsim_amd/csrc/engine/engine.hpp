@@ -128,8 +128,10 @@ struct Digit {
 // Stable LSD radix sort of the state by the given digit plan (least significant
 // digit first, every digit at most radix_bits wide).  Reorders keys and amps.
 // pass_begin / pass_end (optional) are recorded around the digit passes alone.
+// word_src (optional, [words][word_cap]) replaces the state's keys as the source
+// of the digits; the state's keys and amplitudes are permuted either way.
 void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits = 8, hipEvent_t pass_begin = nullptr,
-                  hipEvent_t pass_end = nullptr);
+                  hipEvent_t pass_end = nullptr, const uint64_t* word_src = nullptr, uint64_t word_cap = 0);
 // Digit width forced for every sort (8 or 11), or 0 for the autotuner.
 void set_radix_bits(int bits);
 // One pass over the keys: bits that vary across the state, and whether the
@@ -170,6 +172,11 @@ std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t
 void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last);
 // Sort unless already ordered; returns whether a sort ran.
 bool sort_state(State& st, const sparq_field_t* last);
+// Stable sort by (major fields in order, the other key bits canonically, minor
+// fields in order); the fields must not overlap.
+void sort_state_by(State& st, const std::vector<sparq_field_t>& major, const std::vector<sparq_field_t>& minor);
+// Stable sort by |amplitude|^2, largest first.
+void sort_by_amplitude(State& st);
 
 // ---- compaction (compact.hip)
 void clear_zero(State& st, double eps);

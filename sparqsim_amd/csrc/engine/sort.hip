@@ -205,23 +205,9 @@ KeyStats key_stats(State& st, const sparq_field_t* last, const SizerQuery* q) {
     return ks;
 }
 
-std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t* last, int digit_bits) {
-    const std::vector<uint64_t>& vary = ks.vary;
-    // Varying bit positions in LSD order of the key (others, last): the bits of
-    // `last` (unless that field is already non-decreasing), then every other
-    // bit from the least significant word's lowest bit upward.
-    std::vector<std::pair<int, int>> bits;   // (word, position)
-    const bool skip_last = last && ks.last_sorted;
-    if (last && !skip_last)
-        for (int p = last->shift; p < last->shift + last->width; ++p)
-            if ((vary[last->word] >> p) & 1) bits.emplace_back(last->word, p);
-    for (int w = (int)W - 1; w >= 0; --w)
-        for (int p = 0; p < 64; ++p) {
-            if (last && last->word == w && p >= last->shift && p < last->shift + last->width) continue;
-            if ((vary[w] >> p) & 1) bits.emplace_back(w, p);
-        }
-    // Consecutive positions of one word share a digit of up to digit_bits bits
-    // made of at most three contiguous ranges.
+// Consecutive positions of one word share a digit of up to digit_bits bits
+// made of at most three contiguous ranges.  `bits` is in LSD order.
+static std::vector<Digit> digits_of(const std::vector<std::pair<int, int>>& bits, int digit_bits) {
     std::vector<Digit> plan;
     for (size_t i = 0; i < bits.size();) {
         Digit d{};
@@ -242,6 +228,38 @@ std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t
         plan.push_back(d);
     }
     return plan;
+}
+
+// Varying bit positions, in LSD order, of the ordering
+// (major fields in the given order, the remaining key bits in canonical order,
+// minor fields in the given order): the minor fields' bits first (the last
+// minor field least significant), each field from its lowest bit up, then the
+// unlisted bits from the least significant word's lowest bit upward, then the
+// major fields.  `skip_minor` leaves the minor fields out (already ordered).
+static std::vector<std::pair<int, int>> order_bits(const std::vector<uint64_t>& vary, uint32_t W, const std::vector<sparq_field_t>& major,
+                                                    const std::vector<sparq_field_t>& minor, bool skip_minor) {
+    std::vector<uint64_t> listed(W, 0);
+    for (const auto* fs : {&major, &minor})
+        for (const sparq_field_t& f : *fs) listed[f.word] |= fmask_host(f.width) << f.shift;
+    std::vector<std::pair<int, int>> bits;
+    auto push_field = [&](const sparq_field_t& f) {
+        for (int p = f.shift; p < f.shift + f.width; ++p)
+            if ((vary[f.word] >> p) & 1) bits.emplace_back(f.word, p);
+    };
+    if (!skip_minor)
+        for (size_t i = minor.size(); i-- > 0;) push_field(minor[i]);
+    for (int w = (int)W - 1; w >= 0; --w)
+        for (int p = 0; p < 64; ++p)
+            if (((vary[w] >> p) & 1) && !((listed[w] >> p) & 1)) bits.emplace_back(w, p);
+    for (size_t i = major.size(); i-- > 0;) push_field(major[i]);
+    return bits;
+}
+
+std::vector<Digit> make_plan(const KeyStats& ks, uint32_t W, const sparq_field_t* last, int digit_bits) {
+    // the digits of `last` are left out when that field is already non-decreasing
+    std::vector<sparq_field_t> minor;
+    if (last) minor.push_back(*last);
+    return digits_of(order_bits(ks.vary, W, {}, minor, last && ks.last_sorted), digit_bits);
 }
 
 // ---------------------------------------------------------------- digit-width autotuner
@@ -265,10 +283,10 @@ void set_radix_bits(int bits) {
     g_forced_bits = bits;
 }
 
-void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
-    const std::vector<Digit> p8 = make_plan(ks, st.W, last, 8);
+// Sort by one of two plans of the same ordering (8- and 11-bit digits), the
+// digit width picked by the autotuner.
+static void sort_with_plans(State& st, const std::vector<Digit>& p8, const std::vector<Digit>& p11) {
     if (p8.empty()) return;
-    const std::vector<Digit> p11 = make_plan(ks, st.W, last, 11);
     int lg = 0;
     while ((uint64_t(1) << (lg + 1)) <= st.n && lg < 63) ++lg;
     int use = 0, measure = -1;
@@ -313,6 +331,63 @@ void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
     SPQ_HIP(hipEventDestroy(ev[1]));
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_ms_per_pass[measure][lg] = std::max(ms, 1e-6f) / (float)plan.size() / (float)((double)n / 1048576.0);
+}
+
+void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
+    sort_with_plans(st, make_plan(ks, st.W, last, 8), make_plan(ks, st.W, last, 11));
+}
+
+void sort_state_by(State& st, const std::vector<sparq_field_t>& major, const std::vector<sparq_field_t>& minor) {
+    for (const auto* fs : {&major, &minor})
+        for (const sparq_field_t& f : *fs)
+            SPQ_REQUIRE(f.word < st.W && f.width >= 1 && (uint32_t)f.shift + f.width <= 64, "sort field out of range");
+    std::vector<uint64_t> listed(st.W, 0);
+    for (const auto* fs : {&major, &minor})
+        for (const sparq_field_t& f : *fs) {
+            const uint64_t m = fmask_host(f.width) << f.shift;
+            SPQ_REQUIRE((listed[f.word] & m) == 0, "sort fields overlap");
+            listed[f.word] |= m;
+        }
+    if (major.empty() && minor.size() <= 1) {
+        sort_state(st, minor.empty() ? nullptr : &minor[0]);
+        return;
+    }
+    if (st.n <= 1) return;
+    const KeyStats ks = key_stats(st, nullptr);
+    sort_with_plans(st, digits_of(order_bits(ks.vary, st.W, major, minor, false), 8),
+                    digits_of(order_bits(ks.vary, st.W, major, minor, false), 11));
+}
+
+// Sort key of SortByAmplitude: |a|^2 descending.  A non-negative double's bit
+// pattern orders like its value, so the complement orders descending.
+__global__ void __launch_bounds__(kBlock) amp_order_key_kernel(const double2* __restrict__ amps, uint64_t n, uint64_t* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double2 a = amps[i];
+    const double p = a.x * a.x + a.y * a.y;
+    out[i] = ~(uint64_t)__double_as_longlong(p);
+}
+
+void sort_by_amplitude(State& st) {
+    if (st.n <= 1) return;
+    uint64_t* aux = dalloc<uint64_t>(st.n, st.stream);
+    {
+        SPQ_PROF(st.stream, "amp_order_key_kernel", st.n * 24.0);
+        amp_order_key_kernel<<<grid_for(st.n, kBlock), kBlock, 0, st.stream>>>(st.amps, st.n, aux);
+    }
+    SPQ_HIP(hipGetLastError());
+    State view = st;   // one-word key view of the order keys, for their statistics
+    view.W = 1;
+    view.keys = aux;
+    view.cap = st.n;
+    const KeyStats ks = key_stats(view, nullptr);
+    if (!ks.sorted) {
+        std::vector<std::pair<int, int>> bits;
+        for (int p = 0; p < 64; ++p)
+            if ((ks.vary[0] >> p) & 1) bits.emplace_back(0, p);
+        sort_by_plan(st, digits_of(bits, 8), 8, nullptr, nullptr, aux, st.n);
+    }
+    dfree(aux, st.stream);
 }
 
 bool sort_state(State& st, const sparq_field_t* last) {
@@ -563,8 +638,12 @@ static void launch_gather_state(State& st, const uint32_t* idx, uint64_t* kout, 
     SPQ_HIP(hipGetLastError());
 }
 
-void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hipEvent_t pass_begin, hipEvent_t pass_end) {
+void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hipEvent_t pass_begin, hipEvent_t pass_end,
+                  const uint64_t* word_src, uint64_t word_cap) {
     const uint64_t n = st.n;
+    // the digits read key words from the state, or from word_src when given
+    const uint64_t* wbase = word_src ? word_src : st.keys;
+    const uint64_t wcap = word_src ? word_cap : st.cap;
     if (plan.empty() || n <= 1) return;
     SPQ_REQUIRE(radix_bits == 8 || radix_bits == 11, "digit width must be 8 or 11");
     static const bool trace = std::getenv("SPARQ_TRACE_SORT") != nullptr;   // diagnostic: one line per sort
@@ -597,11 +676,11 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hip
         const Digit dg = plan[p];
         if (dg.word != cur_word) {
             if (identity) {
-                src = st.keys + (uint64_t)dg.word * st.cap;
+                src = wbase + (uint64_t)dg.word * wcap;
             } else {
                 {
                     SPQ_PROF(st.stream, "gather_word_kernel", n * 20.0);
-                    gather_word_kernel<<<grid_for(n, kBlock * kGatherItems), kBlock, 0, st.stream>>>(st.keys + (uint64_t)dg.word * st.cap,
+                    gather_word_kernel<<<grid_for(n, kBlock * kGatherItems), kBlock, 0, st.stream>>>(wbase + (uint64_t)dg.word * wcap,
                                                                                                    idxA, n, kwA);
                 }
                 SPQ_HIP(hipGetLastError());

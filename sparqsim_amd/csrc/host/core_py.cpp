@@ -91,7 +91,28 @@ PYBIND11_MODULE(_core, m) {
         .value("SignedInteger", SignedInteger).value("Boolean", Boolean).value("Rational", Rational)
         .export_values();
 
+    // StateStorage / System entries (PySparQ/core.cpp:75-77, 104-170): one host-side
+    // entry of a state, as SparseState.basis_states returns them
+    py::class_<StateStorage>(m, "StateStorage")
+        .def(py::init<>())
+        .def_readonly("value", &StateStorage::value)
+        .def("__repr__", [](const StateStorage& s) { return "StateStorage(" + std::to_string(s.value) + ")"; });
+
     py::class_<System>(m, "System")
+        .def(py::init<>())
+        .def_readwrite("amplitude", &System::amplitude)
+        .def("get", [](System& s, py::object r) -> StateStorage& { return s.get(resolve(r)); }, py::arg("reg"),
+             py::return_value_policy::reference_internal)
+        .def_property_readonly("registers", [](System& s) {
+            s.registers.resize(std::max(s.registers.size(), System::name_register_map.size()));
+            return s.registers;
+        })
+        .def("__less__", &System::operator<)
+        .def("__lt__", &System::operator<)
+        .def("__eq__", &System::operator==)
+        .def("__ne__", &System::operator!=)
+        .def("__str__", [](const System& s) { return s.to_string(); })
+        .def("to_string", [](const System& s) { return s.to_string(); })
         .def_static("clear", &System::clear)
         .def_static("add_register", [](std::string_view n, StateStorageType t, size_t s) { return System::add_register(n, t, s); })
         .def_static("get_id", [](std::string_view n) { return System::get(n); })
@@ -117,6 +138,16 @@ PYBIND11_MODULE(_core, m) {
         .def("size", [](SparseState& s) { return s.size(); }, Release())
         .def("empty", [](SparseState& s) { return s.empty(); }, Release())
         .def("key_words", [](SparseState& s) { return s.sync().words; })
+        // basis_states (PySparQ/core.cpp:182): the entries as System objects,
+        // downloaded from the device in the engine's entry order (read-only copy)
+        .def_property_readonly("basis_states", [](SparseState& s) {
+            std::vector<System> entries;
+            {
+                py::gil_scoped_release r;
+                s.to_host(entries);
+            }
+            return entries;
+        })
         .def("_handle", [](SparseState& s) { return reinterpret_cast<std::uintptr_t>(s.handle()); })
         .def("_cpp_ptr", [](SparseState& s) { return reinterpret_cast<std::uintptr_t>(&s); })
         .def("flush", [](SparseState& s) { s.flush(); }, Release())
@@ -290,6 +321,55 @@ PYBIND11_MODULE(_core, m) {
     py::class_<Normalize, SelfAdjointOperator, std::shared_ptr<Normalize>>(m, "Normalize").def(py::init<>());
     py::class_<SortUnconditional, SelfAdjointOperator, std::shared_ptr<SortUnconditional>>(m, "SortUnconditional").def(py::init<>());
     py::class_<CheckDuplicateKey, SelfAdjointOperator, std::shared_ptr<CheckDuplicateKey>>(m, "CheckDuplicateKey").def(py::init<>());
+    // grouping sorts (PySparQ/core.cpp:1249-1273)
+    py::class_<SortExceptKey, SelfAdjointOperator, std::shared_ptr<SortExceptKey>>(m, "SortExceptKey")
+        .def(py::init<std::string_view>(), py::arg("key")).def(py::init<size_t>(), py::arg("key_id"));
+    py::class_<SortByKey, SelfAdjointOperator, std::shared_ptr<SortByKey>>(m, "SortByKey")
+        .def(py::init<std::string_view>(), py::arg("key")).def(py::init<size_t>(), py::arg("key_id"));
+    py::class_<SortExceptBit, SelfAdjointOperator, std::shared_ptr<SortExceptBit>>(m, "SortExceptBit")
+        .def(py::init<std::string_view, size_t>(), py::arg("key"), py::arg("digit"))
+        .def(py::init<size_t, size_t>(), py::arg("key_id"), py::arg("digit"));
+    py::class_<SortExceptKeyHadamard, SelfAdjointOperator, std::shared_ptr<SortExceptKeyHadamard>>(m, "SortExceptKeyHadamard")
+        .def(py::init([](py::object key, py::iterable q) {
+            std::vector<size_t> v;
+            for (auto it : q) v.push_back(it.cast<size_t>());
+            return std::make_shared<SortExceptKeyHadamard>(resolve(key), v);
+        }), py::arg("key"), py::arg("qubit_ids"));
+    py::class_<SortByAmplitude, SelfAdjointOperator, std::shared_ptr<SortByAmplitude>>(m, "SortByAmplitude").def(py::init<>());
+    py::class_<SortByKey2, SelfAdjointOperator, std::shared_ptr<SortByKey2>>(m, "SortByKey2")
+        .def(py::init<std::string_view, std::string_view>(), py::arg("key1"), py::arg("key2"))
+        .def(py::init<size_t, size_t>(), py::arg("key1"), py::arg("key2"));
+    auto hp = py::class_<Hadamard_Partial, SelfAdjointOperator, std::shared_ptr<Hadamard_Partial>>(m, "Hadamard_Partial")
+        .def(py::init([](py::object reg, py::iterable q) {
+            std::vector<size_t> v;
+            for (auto it : q) v.push_back(it.cast<size_t>());
+            return std::make_shared<Hadamard_Partial>(resolve(reg), v);
+        }), py::arg("reg_in"), py::arg("qubit_positions"));
+    bind_controllable(hp);
+    py::class_<MoveBackRegister>(m, "MoveBackRegister")
+        .def(py::init<std::string_view>(), py::arg("reg")).def(py::init<size_t>(), py::arg("reg_id"))
+        .def("__call__", [](const MoveBackRegister& a, SparseState& s) { a(s); }, Release());
+    py::class_<Push, BaseOperator, std::shared_ptr<Push>>(m, "Push")
+        .def(py::init<std::string_view, std::string_view>(), py::arg("reg"), py::arg("garbage"))
+        .def(py::init<size_t, std::string_view>(), py::arg("reg_id"), py::arg("garbage"));
+    py::class_<Pop, BaseOperator, std::shared_ptr<Pop>>(m, "Pop")
+        .def(py::init<std::string_view>(), py::arg("reg")).def(py::init<size_t>(), py::arg("reg_id"));
+    m.def("split_systems", [](SparseState& s, py::iterable nz, py::iterable ones, py::iterable by_bit, py::iterable by_val) {
+        std::vector<size_t> a, b;
+        std::vector<std::pair<size_t, size_t>> c;
+        std::vector<std::pair<size_t, uint64_t>> d;
+        for (auto it : nz) a.push_back(resolve(it));
+        for (auto it : ones) b.push_back(resolve(it));
+        for (auto it : by_bit) { auto t = it.cast<py::tuple>(); c.emplace_back(resolve(t[0]), t[1].cast<size_t>()); }
+        for (auto it : by_val) { auto t = it.cast<py::tuple>(); d.emplace_back(resolve(t[0]), t[1].cast<uint64_t>()); }
+        py::gil_scoped_release r;
+        return split_systems(s, a, b, c, d);
+    }, py::arg("state"), py::arg("condition_variable_nonzeros"), py::arg("condition_variable_all_ones"),
+       py::arg("condition_variable_by_bit"), py::arg("condition_variable_by_value"));
+    m.def("combine_systems", [](SparseState& to, SparseState& from) { combine_systems(to, from); }, py::arg("to"),
+          py::arg("from_"), Release());
+    m.def("merge_system", &merge_system);
+    m.def("remove_system", &remove_system);
     py::class_<TestRemovable, SelfAdjointOperator, std::shared_ptr<TestRemovable>>(m, "TestRemovable")
         .def(py::init<std::string_view>()).def(py::init<size_t>());
     py::class_<AddRegister>(m, "AddRegister")
@@ -308,7 +388,9 @@ PYBIND11_MODULE(_core, m) {
         .def(py::init<std::string_view, std::string_view>()).def(py::init<size_t, size_t>())
         .def("__call__", [](const CombineRegister& a, SparseState& s) { return a(s); }, Release());
     m.def("reset_systems", [](SparseState& s) { reset_systems(s); }, Release());
-    m.def("add_systems", [](SparseState& cur, SparseState& add, double coef) { add_systems(cur, add, coef); }, Release());
+    // eps: merge threshold on |sum|^2 (the reference always uses the library epsilon, 1e-14)
+    m.def("add_systems", [](SparseState& cur, SparseState& add, double coef, double eps) { add_systems(cur, add, coef, eps); },
+          py::arg("cur"), py::arg("add"), py::arg("coef"), py::arg("eps") = epsilon, Release());
 
     // debug checks and post-selection (debugger.h:78-160, partial_trace.h:83-145, PySparQ/core.cpp:249-402)
     py::class_<CheckNormalization, SelfAdjointOperator, std::shared_ptr<CheckNormalization>>(m, "CheckNormalization")

@@ -1010,6 +1010,83 @@ void SortUnconditional::operator()(SparseState& state) const {   // sort_state.c
     state.flush();
     check(sparq_sort(state.handle(), nullptr));
 }
+// ---- grouping sorts (sort_state.cpp:10-171)
+static void sort_fields(SparseState& state, const std::vector<sparq_field_t>& major, const std::vector<sparq_field_t>& minor) {
+    state.flush();
+    check(sparq_sort_by(state.handle(), major.data(), (uint32_t)major.size(), minor.data(), (uint32_t)minor.size()));
+}
+static sparq_field_t bits_of(sparq_field_t f, uint32_t lo, uint32_t count) {
+    return sparq_field_t{f.word, (uint8_t)(f.shift + lo), (uint8_t)count, 0};
+}
+void SortExceptKey::operator()(SparseState& state) const {
+    const Layout& L = state.sync();
+    sort_fields(state, {}, {L.at(id)});
+}
+void SortByKey::operator()(SparseState& state) const {
+    const Layout& L = state.sync();
+    sort_fields(state, {L.at(register_key)}, {});
+}
+SortExceptBit::SortExceptBit(size_t key, size_t d) : id(key), digit(d) {
+    if (digit >= System::size_of(id)) throw std::invalid_argument("SortExceptBit: digit outside the register");
+}
+void SortExceptBit::operator()(SparseState& state) const {
+    // others in id order, then the register with the bit masked (its bits above,
+    // then below, the digit), then the bit itself
+    const Layout& L = state.sync();
+    const sparq_field_t f = L.at(id);
+    std::vector<sparq_field_t> minor;
+    if (digit + 1 < f.width) minor.push_back(bits_of(f, (uint32_t)digit + 1, f.width - (uint32_t)digit - 1));
+    if (digit > 0) minor.push_back(bits_of(f, 0, (uint32_t)digit));
+    minor.push_back(bits_of(f, (uint32_t)digit, 1));
+    sort_fields(state, {}, minor);
+}
+static std::vector<size_t> checked_positions(size_t id, std::vector<size_t> pos, const char* who) {
+    std::sort(pos.begin(), pos.end());
+    if (std::adjacent_find(pos.begin(), pos.end()) != pos.end()) throw std::invalid_argument(std::string(who) + ": repeated qubit");
+    for (size_t q : pos)
+        if (q >= System::size_of(id)) throw std::invalid_argument(std::string(who) + ": qubit outside the register");
+    return pos;
+}
+SortExceptKeyHadamard::SortExceptKeyHadamard(size_t key, const std::vector<size_t>& q)
+    : id(key), qubit_ids(checked_positions(key, q, "SortExceptKeyHadamard")) {}
+void SortExceptKeyHadamard::operator()(SparseState& state) const {
+    // the key with the listed qubits masked compares in its id position; the
+    // full key decides last, i.e. the listed qubits, most significant first
+    const Layout& L = state.sync();
+    const sparq_field_t f = L.at(id);
+    std::vector<sparq_field_t> minor;
+    for (size_t i = qubit_ids.size(); i-- > 0;) minor.push_back(bits_of(f, (uint32_t)qubit_ids[i], 1));
+    sort_fields(state, {}, minor);
+}
+void SortByAmplitude::operator()(SparseState& state) const {
+    state.sync();
+    state.flush();
+    check(sparq_sort_by_amplitude(state.handle()));
+}
+void SortByKey2::operator()(SparseState& state) const {
+    if (id1 == id2) throw std::invalid_argument("SortByKey2: the two keys must differ");
+    const Layout& L = state.sync();
+    sort_fields(state, {L.at(id1), L.at(id2)}, {});
+}
+
+// ---- Hadamard_Partial (hadamard.cpp:470-712)
+Hadamard_Partial::Hadamard_Partial(size_t reg, const std::vector<size_t>& pos)
+    : id(reg), qubit_positions(checked_positions(reg, pos, "Hadamard_Partial")) {}
+void Hadamard_Partial::operator()(SparseState& state) const {
+    const Layout& L = state.sync();
+    state.flush();
+    const auto conds = conds_of(this, L);
+    sparq_pair_params_t p{};
+    p.kind = SPARQ_PAIR_HADAMARD_BOOL;
+    // H on each listed bit in ascending order: the butterfly stages of the
+    // transform over those bits; only the last prunes
+    for (size_t i = 0; i < qubit_positions.size(); ++i) {
+        const double eps = i + 1 == qubit_positions.size() ? clearzero_default_eps() : 0.0;
+        check(sparq_pair_op(state.handle(), L.at(id), (uint32_t)qubit_positions[i], &p, conds.data(), (uint32_t)conds.size(), eps));
+    }
+    System::update_max_size(state.size());
+}
+
 void CheckDuplicateKey::operator()(SparseState& state) const {   // debugger.cpp:214-222
     state.sync();
     state.flush();
@@ -1054,6 +1131,47 @@ void RemoveRegister::operator()(SparseState& state) const {   // system_operatio
     System::remove_register(register_id);
     state.sync();
 }
+void MoveBackRegister::operator()(SparseState& state) const {   // system_operations.cpp:223-249
+    auto& map = System::name_register_map;
+    if (register_id >= map.size()) throw std::runtime_error("Register not found.");
+    const size_t last = map.size() - 1;
+    if (register_id == last) return;   // already at the back
+    if (!System::status_of(register_id) || !System::status_of(last))
+        throw std::invalid_argument("MoveBackRegister: the register and the last table entry must be active");
+    const Layout old = state.sync();
+    state.flush();
+    std::swap(map[register_id], map[last]);
+    System::name_to_index[std::get<0>(map[register_id])] = register_id;
+    System::name_to_index[std::get<0>(map[last])] = last;
+    ++System::layout_version;
+    const Layout T = Layout::from_registry();
+    std::vector<std::pair<sparq_field_t, sparq_field_t>> moves;
+    for (size_t id = 0; id < T.field.size(); ++id) {
+        if (T.field[id].width == 0) continue;
+        const size_t from = id == register_id ? last : id == last ? register_id : id;
+        moves.emplace_back(old.field[from], T.field[id]);
+    }
+    state.relayout_with(T, moves);
+}
+
+void Push::operator()(SparseState& state) const {   // system_operations.cpp:338-354
+    const size_t garbage = AddRegister(garbage_name, System::type_of(reg_id), System::size_of(reg_id))(state);
+    Swap_General_General(reg_id, garbage)(state);
+    System::temporal_registers.push_back(garbage);
+}
+void Push::dag(SparseState& state) const {
+    const Pop pop(reg_id);
+    pop(state);
+}
+void Pop::operator()(SparseState& state) const {   // system_operations.cpp:356-369
+    if (System::temporal_registers.empty()) throw std::runtime_error("Pop: no pushed register");
+    const size_t garbage = System::temporal_registers.back();
+    Swap_General_General(reg_id, garbage)(state);
+    const RemoveRegister remove(garbage);
+    remove(state);
+    System::temporal_registers.pop_back();
+}
+
 size_t SplitRegister::operator()(SparseState& state) const {   // system_operations.cpp:134-166
     const Layout old = state.sync();
     state.flush();
@@ -1534,13 +1652,42 @@ TridiagParityPhase::TridiagParityPhase(size_t reg) {   // hamiltonian_tridiagona
     extra_bit_regs.emplace_back(reg, 0);
 }
 
-void add_systems(SparseState& current, SparseState& new_state, double coef) {   // system_operations.h:157-181
+void add_systems(SparseState& current, SparseState& new_state, double coef, double eps) {   // system_operations.h:157-181
     current.sync();
     new_state.sync();
     current.flush();
     new_state.flush();
-    check(sparq_add_systems(current.handle(), new_state.handle(), coef, epsilon));
+    check(sparq_add_systems(current.handle(), new_state.handle(), coef, eps));
 }
+
+SparseState split_systems(SparseState& state, const std::vector<size_t>& nonzeros, const std::vector<size_t>& all_ones,
+                          const std::vector<std::pair<size_t, size_t>>& by_bit,
+                          const std::vector<std::pair<size_t, uint64_t>>& by_value) {   // system_operations.cpp:53-97
+    Controllable c;
+    c.condition_variable_nonzeros = nonzeros;
+    c.condition_variable_all_ones = all_ones;
+    c.condition_variable_by_bit = by_bit;
+    c.condition_variable_by_value = by_value;
+    const Layout& L = state.sync();
+    state.flush();
+    const auto conds = c.conditions(L);
+    SparseState rest;
+    rest.sync();
+    check(sparq_partition(state.handle(), conds.data(), (uint32_t)conds.size(), rest.handle()));
+    return rest;
+}
+void combine_systems(SparseState& to, SparseState& from) {   // system_operations.cpp:99-106
+    to.sync();
+    from.sync();
+    to.flush();
+    from.flush();
+    check(sparq_append(to.handle(), from.handle()));
+}
+void merge_system(System& a, System& b) {   // basic_components.cpp:517-521
+    a.amplitude += b.amplitude;
+    b.amplitude = 0;
+}
+bool remove_system(const System& s) { return std::norm(s.amplitude) < epsilon; }   // :523-526
 
 void reset_systems(SparseState& state) {
     state.sync();

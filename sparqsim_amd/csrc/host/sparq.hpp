@@ -517,6 +517,66 @@ struct SortUnconditional : SelfAdjointOperator {
     SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
 };
+// Grouping sorts (SparQ/src/sort_state.cpp:10-171), stable on the device; the
+// reference's std::sort leaves the order of entries equal under the ordering
+// unspecified.
+struct SortExceptKey : SelfAdjointOperator {       // others in id order, then key
+    size_t id;
+    explicit SortExceptKey(std::string_view key) : SortExceptKey(System::get(key)) {}
+    explicit SortExceptKey(size_t key) : id(key) {}
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+struct SortByKey : SelfAdjointOperator {           // key first (the rest in canonical order)
+    size_t register_key;
+    explicit SortByKey(std::string_view key) : SortByKey(System::get(key)) {}
+    explicit SortByKey(size_t key) : register_key(key) {}
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+struct SortExceptBit : SelfAdjointOperator {       // others, then key with bit `digit` masked, then the bit
+    size_t id, digit;
+    SortExceptBit(std::string_view key, size_t d) : SortExceptBit(System::get(key), d) {}
+    SortExceptBit(size_t key, size_t d);
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+struct SortExceptKeyHadamard : SelfAdjointOperator {   // key's listed qubits moved last
+    size_t id;
+    std::vector<size_t> qubit_ids;                 // ascending, distinct
+    SortExceptKeyHadamard(std::string_view key, const std::vector<size_t>& q) : SortExceptKeyHadamard(System::get(key), q) {}
+    SortExceptKeyHadamard(size_t key, const std::vector<size_t>& q);
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+struct SortByAmplitude : SelfAdjointOperator {     // |a|^2 descending
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+struct SortByKey2 : SelfAdjointOperator {          // key1, key2, then the others
+    size_t id1, id2;
+    SortByKey2(std::string_view k1, std::string_view k2) : SortByKey2(System::get(k1), System::get(k2)) {}
+    SortByKey2(size_t k1, size_t k2) : id1(k1), id2(k2) {}
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+
+// Hadamard on the listed bits of a register (SparQ/include/hadamard.h:255-330,
+// SparQ/src/hadamard.cpp:470-712): H on every listed qubit, i.e. the
+// Walsh-Hadamard transform over those bits of each group of entries equal
+// elsewhere (the reference's sort-based path, hadamard.cpp:573-597).  Its
+// hash-based path pairs entries as for a single qubit; the two agree for one
+// position.  ClearZero once at the end, as the reference does.
+struct Hadamard_Partial : SelfAdjointOperator, Controllable {
+    size_t id;
+    std::vector<size_t> qubit_positions;           // ascending, distinct
+    Hadamard_Partial(std::string_view reg, const std::vector<size_t>& pos) : Hadamard_Partial(System::get(reg), pos) {}
+    Hadamard_Partial(size_t reg, const std::vector<size_t>& pos);
+    Controllable* controls() override { return this; }
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+
 struct CheckDuplicateKey : SelfAdjointOperator {
     SPQ_BOTH_VIEWS
     void operator()(SparseState& state) const override;
@@ -549,6 +609,37 @@ struct RemoveRegister {
     explicit RemoveRegister(size_t id) : register_id(id) {}
     void operator()(SparseState& state) const;
 };
+// MoveBackRegister (SparQ/src/system_operations.cpp:223-249): the register
+// takes the last id of the table and the register holding that id takes its
+// place; the values move with them in `state` only (like the reference, other
+// states are not updated: it must not run while other states are in use).
+struct MoveBackRegister {
+    size_t register_id;
+    explicit MoveBackRegister(std::string_view reg) : register_id(System::get(reg)) {}
+    explicit MoveBackRegister(size_t reg) : register_id(reg) {}
+    void operator()(SparseState& state) const;
+};
+// Push / Pop (SparQ/src/system_operations.cpp:338-369): Push adds a register
+// `garbage` of reg's type and size, swaps it with reg (reg becomes 0) and stacks
+// its id; Pop swaps reg with the last stacked register and removes that one
+// (it must then be zero).  Push's dag is Pop(reg); Pop has no dag.
+struct Push : BaseOperator {
+    size_t reg_id;
+    std::string garbage_name;
+    Push(std::string_view reg, std::string_view garbage) : Push(System::get(reg), garbage) {}
+    Push(size_t reg, std::string_view garbage) : reg_id(reg), garbage_name(garbage) {}
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+    void dag(SparseState& state) const override;
+};
+struct Pop : BaseOperator {
+    size_t reg_id;
+    explicit Pop(std::string_view reg) : Pop(System::get(reg)) {}
+    explicit Pop(size_t reg) : reg_id(reg) {}
+    SPQ_BOTH_VIEWS
+    void operator()(SparseState& state) const override;
+};
+
 struct SplitRegister {
     std::string first_name, second_name;
     size_t second_size;
@@ -749,8 +840,21 @@ struct TridiagParityPhase : PointwiseOperator {
     explicit TridiagParityPhase(size_t reg);
 };
 
+// split_systems / combine_systems (SparQ/src/system_operations.cpp:53-106):
+// the entries failing the conditions move to the returned state; combine
+// appends `from`'s entries to `to`.
+SparseState split_systems(SparseState& state, const std::vector<size_t>& nonzeros, const std::vector<size_t>& all_ones,
+                          const std::vector<std::pair<size_t, size_t>>& by_bit,
+                          const std::vector<std::pair<size_t, uint64_t>>& by_value);
+void combine_systems(SparseState& to, SparseState& from);
+// merge_system / remove_system (SparQ/src/basic_components.cpp:517-526): a takes
+// the amplitude sum of two equal entries and b's becomes 0; the merge's prune
+// test is |a|^2 < epsilon.
+void merge_system(System& a, System& b);
+bool remove_system(const System& s);
+
 // add_systems (system_operations.h:157-181)
-void add_systems(SparseState& current, SparseState& new_state, double coef);
+void add_systems(SparseState& current, SparseState& new_state, double coef, double eps = epsilon);
 // reset_systems (system_operations.cpp:108-117): back to the single entry |0...0>, amplitude 1.
 void reset_systems(SparseState& state);
 

@@ -410,6 +410,57 @@ def test_band_random_matrix_and_walk_spectrum():
     assert lo <= wev[0] and wev[-1] <= hi and hi / lo < 50.01
 
 
+def test_walk_fit_interval_bounds_coefficients():
+    """The 1/x fit for the sparse walk: the interval [lo, 1] encloses the walk
+    block's exact spectrum (dense eigenvalues at dim 64); on it the fit matches
+    1/x to 1e-7 at every eigenvalue with sum|d| < 1e3, while the fit on the
+    estimated spectrum [m, M] (M about 0.49) needs sum|d| > 1e4 for the same
+    accuracy -- the factor by which every per-term error of the walk state is
+    amplified in the LCU output."""
+    v, s, A = cks.band_random_matrix(64, 50.0)
+    cols = np.array(s).reshape(64, 4)
+    q = np.minimum(np.floor(np.array(v) * 2.0 ** 32), 2.0 ** 32 - 1).reshape(64, 4)
+    W = np.zeros((64, 64))
+    for i in range(64):
+        W[i, cols[i]] = q[i] / (4 * (2.0 ** 32 - 1))
+    wev = np.linalg.eigvalsh(W)
+    lo, hi = cks.walk_fit_interval(A, 32)
+    assert lo <= wev[0] and hi == 1.0
+    m, M = cks.walk_fit_interval(A, 32, top="spectrum")
+    assert m <= wev[0] and wev[-1] <= M < 0.6
+
+    def p(d, x):   # sum_j d_j T_(2j+1)(x), T_k(x) = cos(k arccos x)
+        return sum(dj * np.cos((2 * j + 1) * np.arccos(x)) for j, dj in enumerate(d))
+
+    d_unit, s_unit = cks.chebyshev_inverse_coefficients(lo, hi, 1024)
+    d_spec, s_spec = cks.chebyshev_inverse_coefficients(m, M, 1024)
+    assert np.max(np.abs(wev * p(d_unit, wev) - 1)) < 1e-7
+    assert np.max(np.abs(wev * p(d_spec, wev) - 1)) < 1e-7
+    assert s_unit < 1e3 < 1e4 < s_spec
+
+
+def test_oracle_band_random_pruning_amplification(oracle):
+    """The cause of the 2^20 band-plus-random non-convergence, reproduced at dim
+    32: ClearZero / merge thresholds scaled to the mean |a|^2 of the 2^20 x 16
+    right-hand-side state (1e-14 x 1.3e5) leave res_dir ~1e-2 with coefficients
+    fitted on [m, M] (sum|d| ~ 3e4) and ~1e-4 with the fit on [lo, 1]."""
+    rng = np.random.default_rng(7)
+    rhs = np.stack([np.ones(32), rng.standard_normal(32)])
+    out = {}
+    try:
+        for top in ("spectrum", "unit"):
+            oracle.set_clearzero_default_eps(1.3e-9)
+            b = cks.BatchedSparseCKS(oracle, 5, rhs, kappa=50.0, J=1024, fit_top=top)
+            b.lcu.merge_eps = 1.3e-9
+            b.prepare_input()
+            while b.Step():
+                pass
+            out[top] = max(r["res_dir"] for r in b.extract()[1])
+    finally:
+        oracle.set_clearzero_default_eps(1e-14)
+    assert out["spectrum"] > 3e-3 and out["unit"] < 1e-3
+
+
 def test_oracle_band_random_solve_converges(oracle):
     rng = np.random.default_rng(8)
     b = cks.BatchedSparseCKS(oracle, 5, np.stack([np.ones(32), rng.standard_normal(32)]), kappa=50.0, J=1024)

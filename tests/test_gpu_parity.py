@@ -497,6 +497,29 @@ def test_rational_angle_ops(ps, oracle, op):
     assert_same_state(eg, eo, exact_amps=True, what=op)
 
 
+@pytest.mark.parametrize("w", [3, 7, 31, 32, 33, 52, 53, 63, 64])
+def test_rational_encoding_every_width(ps, oracle, w):
+    """The device's fraction -> w-bit Rational encoding equals the oracle's
+    (pinned against the reference's 2^-w bound and exact dyadics in
+    test_oracle.py) at widths up to 64: arc-tangent and arc-cosine outputs of
+    every input pair, bit-exact."""
+    spec = [("x", "SignedInteger", 4), ("y", "SignedInteger", 4)]
+    pairs = [(l, r) for l in range(16) for r in range(16)]
+
+    for op in ("GetRotateAngle_Int_Int", "Div_Sqrt_Arccos_UInt_UInt"):
+        def build(mod):
+            sp = spec if op == "GetRotateAngle_Int_Int" else [("x", "UnsignedInteger", 4), ("y", "UnsignedInteger", 4)]
+            st = regs(mod, sp + [("o", "Rational", w)])
+            use = pairs if op == "GetRotateAngle_Int_Int" else [(a, b) for b in range(1, 16) for a in range(b + 1)]
+            vals = np.array([(a, b, 0) for a, b in use], dtype=np.uint64)
+            st.load(vals, np.full(len(use), 1.0 / np.sqrt(len(use)), dtype=np.complex128))
+            getattr(mod, op)("x", "y", "o")(st)
+            return st
+
+        eg, eo = run_both(ps, oracle, build)
+        assert_same_state(eg, eo, exact_amps=True, what=f"{op} into {w} bits")
+
+
 def test_operator_sequence(ps, oracle):
     """OperatorSequence applies its operators in order (one native call) and its
     dag applies their adjoints in reverse: the same state as the operators one

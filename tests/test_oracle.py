@@ -419,3 +419,74 @@ def test_simple_key_rewrite_truth_tables(oracle):
 # value of three small registers against Python predicates (tests/parity.py).
 def test_condition_truth_tables(oracle):
     check_condition_truth_tables(oracle)
+
+
+# ---- fixed-point encodings: the reference's own bounds
+# get_rational (test/CPUTest/CommonTest/CorrectnessTest_Common.inl:171-195):
+# |decode(encode(x)) - x| <= 2^-w for w = 3..32 at x = 0.1231231.  Checked here
+# at every width 3..64 on that value, 400 seeded uniform values and the exact
+# dyadics k / 2^w (which must encode to k), with exact rational arithmetic.  The
+# rule is truncation, floor(x 2^w): within the reference's bound, the rounding
+# mode itself is this build's (stated in the oracle's header).
+def test_get_rational_reference_bound_every_width(oracle):
+    from fractions import Fraction
+
+    rng = np.random.default_rng(0x5EED2026)
+    xs = [0.1231231, 0.0, 0.5, 0.999999999] + [float(x) for x in rng.uniform(0.0, 1.0, 400)]
+    for w in range(3, 65):
+        one = 1 << w
+        for x in xs:
+            v = oracle.get_rational(x, w)
+            assert 0 <= v < one
+            err = abs(Fraction(v, one) - Fraction(x))
+            assert err <= Fraction(1, one), (w, x, v)
+            assert v == (Fraction(x) * one).__floor__(), (w, x, v)
+        # exact dyadics: every k/2^w that a double holds exactly
+        ks = [0, 1, 2, 3, one // 2, one - 1] + [int(k) for k in rng.integers(0, min(one, 2 ** 53), 50)]
+        for k in ks:
+            x = k / one
+            if Fraction(x) != Fraction(k, one):
+                continue
+            assert oracle.get_rational(x, w) == k, (w, k)
+
+
+# SparseMatrix quantisation (the reference's Python front end clips every value
+# to [0, (2^ds - 1)/2^ds] (positive_only) or +-(2^(ds-1) - 1)/2^(ds-1) and hands
+# it to the C++ SparseMatrix, PySparQ/pysparq/algorithms/cks_solver.py:325-342;
+# the encoder itself lives in the absent submodule).  What those bounds fix:
+# the clip ends encode to the largest code (2^ds - 1, and 2^(ds-1) - 1 / its
+# two's complement negative), exact dyadics encode exactly, every value
+# decodes to within one code step of its input with its sign kept, and the
+# oracle and the engine's host layer encode identically.
+@pytest.mark.parametrize("positive_only", [True, False])
+def test_sparse_matrix_quantisation_against_float_input(oracle, positive_only):
+    from fractions import Fraction
+
+    import sparqsim_amd as ps
+
+    rng = np.random.default_rng(11)
+    for ds in (4, 8, 16, 31, 32, 40):
+        step = Fraction(1, 1 << ds) if positive_only else Fraction(1, 1 << (ds - 1))
+        top = 1 - step                                   # max_encoded of the reference front end
+        lo = Fraction(0) if positive_only else -top
+        xs = [float(top), float(lo), 0.0, 0.5, float(step), float(lo + step)]
+        xs += [float(x) for x in rng.uniform(float(lo), float(top), 200)]
+        dy = [int(k) for k in rng.integers(0, (1 << ds) if positive_only else (1 << (ds - 1)), 40)]
+        xs += [float(k * step) for k in dy] + ([] if positive_only else [float(-k * step) for k in dy])
+        xs = [x for x in xs if lo <= Fraction(x) <= top]
+        n = len(xs)
+        enc_o = list(oracle.SparseMatrix(xs, list(range(n)), ds, n, 1, positive_only).elements)
+        enc_e = list(ps.SparseMatrix(xs, list(range(n)), ds, n, 1, positive_only).elements)
+        assert enc_o == enc_e, ds
+        mask = (1 << ds) - 1
+        for x, v in zip(xs, enc_o):
+            assert 0 <= v <= mask
+            s = v if positive_only or v < (1 << (ds - 1)) else v - (1 << ds)
+            dec = s * step
+            assert abs(dec - Fraction(x)) < step, (ds, x, v)
+            if Fraction(x) % step == 0:
+                assert dec == Fraction(x), (ds, x, v)           # exact dyadics
+            assert (s < 0) <= (x < 0) and (s > 0) <= (x > 0)    # sign kept (or zero)
+        assert enc_o[0] == (mask if positive_only else (1 << (ds - 1)) - 1)
+        if not positive_only:
+            assert enc_o[1] == (1 << (ds - 1)) + 1             # -(2^(ds-1) - 1) in two's complement

@@ -15,7 +15,12 @@ TOL = GOLDEN["meta"]["tolerance"]      # 1e-5, CorrectnessTest_QDA_CompareList.i
 def run_golden(mod, name, beta, max_points=None):
     from sparqsim_amd import qda
 
-    ref = GOLDEN["lists"][name]
+    # The reference checks the same walk over two block encodings of A (tridiagonal
+    # direct oracles and Block_Encoding_via_QRAM, CorrectnessTest_QDA_CompareList.inl:
+    # 165-202); their lists agree to 3e-15 (test_golden_lists_of_both_encodings_agree),
+    # so the tridiagonal walk is held to both.  The QRAM encoding itself is not built.
+    refs = [GOLDEN["lists"][name], GOLDEN["lists"][name.replace("_Tri", "_QRAM")]]
+    ref = refs[0]
     a, b, kappa, steps = qda.qda_schedule(3, 0.01, 2.0, beta)
     assert steps == 2 * len(ref)
     st = qda.setup_poiseuille(mod, 3)
@@ -27,7 +32,7 @@ def run_golden(mod, name, beta, max_points=None):
         if (n + 1) % 2 == 0:
             vec, _ = qda.get_output(st, mod)
             f = qda.fidelity(qda.mid_eigenstate(a, b, w.fs), vec)
-            worst = max(worst, abs(f - ref[k]))
+            worst = max(worst, max(abs(f - r[k]) for r in refs))
             k += 1
             if max_points and k >= max_points:
                 break
@@ -39,6 +44,17 @@ def test_oracle_reproduces_golden_fidelities(oracle, name, beta):
     worst, k = run_golden(oracle, name, beta)
     assert k == 371
     assert worst < TOL, worst
+
+
+def test_golden_lists_of_both_encodings_agree():
+    """Neg/Pos_QRAM (Block_Encoding_via_QRAM) and Neg/Pos_Tri (direct tridiagonal
+    oracles): 371 fidelities each, equal to 3e-15 -- the two encodings give the
+    same walk, so one walk is checked against both lists."""
+    L = GOLDEN["lists"]
+    for sign in ("Neg", "Pos"):
+        tri, qram = np.array(L[f"{sign}_Tri"]), np.array(L[f"{sign}_QRAM"])
+        assert tri.shape == qram.shape == (371,)
+        assert np.max(np.abs(tri - qram)) < 1e-14
 
 
 def test_poiseuille_kappa_and_steps():
