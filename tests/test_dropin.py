@@ -240,8 +240,12 @@ def test_push_pop_round_trip(ps):
     after = rows(st)
     assert after[0] == before[0] and same_entries(before, after)
     assert "b_saved" not in [ps.System.name_of(i) for i in after[0]]     # the stacked register is gone
-    ps.Push("b", "tmp").dag(st)                  # Push.dag is Pop
+    push = ps.Push("b", "tmp")
+    push(st)
+    push.dag(st)                                 # Push.dag is Pop
     assert same_entries(before, rows(st))
+    with pytest.raises(RuntimeError):
+        ps.Pop("b")(st)                          # nothing stacked
 
 
 @pytest.mark.gpu

@@ -497,12 +497,15 @@ def test_rational_angle_ops(ps, oracle, op):
     assert_same_state(eg, eo, exact_amps=True, what=op)
 
 
-@pytest.mark.parametrize("w", [3, 7, 31, 32, 33, 52, 53, 63, 64])
+@pytest.mark.parametrize("w", [3, 7, 31, 32, 33, 48, 52, 53, 63, 64])
 def test_rational_encoding_every_width(ps, oracle, w):
     """The device's fraction -> w-bit Rational encoding equals the oracle's
     (pinned against the reference's 2^-w bound and exact dyadics in
-    test_oracle.py) at widths up to 64: arc-tangent and arc-cosine outputs of
-    every input pair, bit-exact."""
+    test_oracle.py): arc-tangent and arc-cosine outputs of every input pair,
+    bit-exact up to 48 bits.  Wider outputs resolve the double result of atan2 /
+    acos itself, where the device math library and the host's may differ by an
+    ulp (neither is correctly rounded): there a difference of up to
+    2 ulp -> 2 * 2^(w-53) + 1 codes is allowed (ulp of a value in [0, 1) <= 2^-53)."""
     spec = [("x", "SignedInteger", 4), ("y", "SignedInteger", 4)]
     pairs = [(l, r) for l in range(16) for r in range(16)]
 
@@ -517,7 +520,16 @@ def test_rational_encoding_every_width(ps, oracle, w):
             return st
 
         eg, eo = run_both(ps, oracle, build)
-        assert_same_state(eg, eo, exact_amps=True, what=f"{op} into {w} bits")
+        if w <= 48:
+            assert_same_state(eg, eo, exact_amps=True, what=f"{op} into {w} bits")
+            continue
+        from parity import canonical
+
+        (_, vg, ag), (_, vr, ar) = canonical(eg), canonical(eo)   # inputs sort first, so rows pair up
+        assert np.array_equal(vg[:, :2], vr[:, :2]) and np.array_equal(ag, ar)
+        tol = 2 * 2 ** (w - 53) + 1
+        diff = np.abs(vg[:, 2].astype(object) - vr[:, 2].astype(object))
+        assert max(diff) <= tol, (op, w, max(diff))
 
 
 def test_operator_sequence(ps, oracle):
