@@ -201,6 +201,22 @@ int sparq_pointwise(sparq_state_t st, const sparq_pw_op_t* ops, uint32_t nops,
 /* ClearZero: stable removal of |amp|^2 < eps (system_operations.cpp:371-382). */
 int sparq_clear_zero(sparq_state_t st, double eps);
 
+/* ClearZero into caller-owned device arrays, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same result as sparq_clear_zero: the entries of keys_in /
+ * amps_in ([words][cap_in] key words, word-major; n complex amplitudes as
+ * re/im pairs, 16-byte aligned) with |amp|^2 >= eps, in order, go to keys_out /
+ * amps_out ([words][cap_out]; cap_in, cap_out >= n), and their count to
+ * *kept_dev (device memory).  Work is queued on `stream` (a hipStream_t; NULL
+ * for the default stream) with no synchronization or readback.  Limit: n <=
+ * 2^28 entries.  Per-tile counts and offsets live in a fixed ~1.5 MiB device
+ * array of the library, one per GPU, shared by every call on that GPU: calls
+ * that overlap on one GPU (different streams, or other threads) race. */
+int sparq_clear_zero_noalloc(const uint64_t* keys_in, const double* amps_in, uint32_t words, uint64_t n, uint64_t cap_in,
+                             double eps, uint64_t* keys_out, double* amps_out, uint64_t cap_out, uint64_t* kept_dev,
+                             void* stream);
+
 /* Sort by the full key, unsigned lexicographic over the words (SortUnconditional,
  * sort_state.cpp:84-88); when `last` is non-NULL the bits of that field are moved
  * to the least significant position (SortExceptKey, sort_state.cpp:10-36). */

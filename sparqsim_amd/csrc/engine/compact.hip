@@ -3,6 +3,7 @@
 // ClearZero bytes per entry: (8W+16) read + (8W+16) per kept entry written
 // (+16 for the amplitude re-read of the counting pass).
 #include <algorithm>
+#include <mutex>
 
 #include "blockscan.hpp"
 #include "engine.hpp"
@@ -296,3 +297,89 @@ bool has_sorted_duplicates(State& st) {
 }
 
 }  // namespace sparq
+
+// ---------------------------------------------------------------- ClearZero without allocation
+// sparq_clear_zero_noalloc: the ClearZero compaction of sparq_clear_zero over
+// caller-owned arrays, for callers that cannot allocate.  Its per-tile counts,
+// their offsets and the scan's partial sums live in the file-scope array
+// `scratch` below (one copy per GPU, held until the process ends); host code
+// reaches it only through the address hipGetSymbolAddress returns on each
+// device.  Nothing here allocates, synchronizes or reads back.
+namespace sparq {
+namespace {
+constexpr uint64_t kNoAllocMaxEntries = uint64_t(1) << 28;
+constexpr uint64_t kNoAllocTiles = kNoAllocMaxEntries / kTile;                      // 131072
+constexpr uint64_t kNoAllocCountBytes = kNoAllocTiles * sizeof(uint32_t);           // 512 KiB
+constexpr uint64_t kNoAllocOffsWords = kNoAllocTiles + 1;
+constexpr uint64_t kNoAllocScanWords = 2 * ((kNoAllocTiles + 2047) / 2048) + 64;   // partial sums above level 0
+constexpr uint64_t kNoAllocScratchBytes = kNoAllocCountBytes + 8 * (kNoAllocOffsWords + kNoAllocScanWords);
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+}  // namespace sparq
+
+static __device__ __align__(256) unsigned char scratch[sparq::kNoAllocScratchBytes];
+
+namespace sparq {
+namespace {
+// Device address of `scratch` on the current device, resolved once per device.
+uint8_t* scratch_on_current_device() {
+    int dev = 0;
+    SPQ_HIP(hipGetDevice(&dev));
+    SPQ_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SPQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        SPQ_REQUIRE(p != nullptr, "no device address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<uint8_t*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+__global__ void store_zero_kernel(uint64_t* __restrict__ out) { out[0] = 0; }
+
+template <int W>
+void clear_zero_noalloc_w(hipStream_t s, const uint64_t* keys, const double2* amps, uint64_t n, uint64_t cap_in, double eps,
+                          uint64_t* kout, double2* aout, uint64_t cap_out, uint64_t* kept_dev, uint8_t* base) {
+    const unsigned nb = grid_for(n, kTile);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base);
+    uint64_t* offs = reinterpret_cast<uint64_t*>(base + kNoAllocCountBytes);
+    uint64_t* scan_scratch = offs + kNoAllocOffsWords;
+    compact_count_kernel<W, kClearZero><<<nb, kBlock, 0, s>>>(keys, amps, cap_in, n, nullptr, eps, CondList{}, counts);
+    SPQ_HIP(hipGetLastError());
+    // the scan writes every offset and the kept total: nothing in scratch needs clearing
+    exclusive_scan_u32_scratch(s, counts, offs, nb, kept_dev, scan_scratch, kNoAllocScanWords);
+    compact_scatter_kernel<W, kClearZero><<<nb, kBlock, 0, s>>>(keys, amps, cap_in, n, nullptr, eps, CondList{}, offs, kept_dev, kout,
+                                                             aout, cap_out);
+    SPQ_HIP(hipGetLastError());
+}
+}  // namespace
+}  // namespace sparq
+
+extern "C" int sparq_clear_zero_noalloc(const uint64_t* keys_in, const double* amps_in, uint32_t words, uint64_t n, uint64_t cap_in,
+                                        double eps, uint64_t* keys_out, double* amps_out, uint64_t cap_out, uint64_t* kept_dev,
+                                        void* stream) {
+    using namespace sparq;
+    return guarded_call([&] {
+        SPQ_REQUIRE(words >= 1 && words <= SPARQ_MAX_WORDS, "key width out of range");
+        SPQ_REQUIRE(n <= kNoAllocMaxEntries, "sparq_clear_zero_noalloc: more than 2^28 entries (the scratch limit)");
+        SPQ_REQUIRE(kept_dev != nullptr, "null kept-count output");
+        SPQ_REQUIRE(n == 0 || (keys_in && amps_in && keys_out && amps_out), "null array");
+        SPQ_REQUIRE(cap_in >= n && cap_out >= n, "array capacities must hold n entries");
+        SPQ_REQUIRE((reinterpret_cast<uintptr_t>(amps_in) | reinterpret_cast<uintptr_t>(amps_out)) % 16 == 0,
+                    "amplitude arrays must be 16-byte aligned");
+        ensure_device();
+        uint8_t* base = scratch_on_current_device();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (n == 0) {
+            store_zero_kernel<<<1, 1, 0, s>>>(kept_dev);
+            SPQ_HIP(hipGetLastError());
+            return;
+        }
+        const double2* a = reinterpret_cast<const double2*>(amps_in);
+        double2* ao = reinterpret_cast<double2*>(amps_out);
+        SPQ_DISPATCH_W(words, clear_zero_noalloc_w, s, keys_in, a, n, cap_in, eps, keys_out, ao, cap_out, kept_dev, base);
+    });
+}

@@ -117,6 +117,10 @@ inline unsigned grid_for(uint64_t n, unsigned per_block) {
 // Exclusive scan of `n` uint32 values into uint64 offsets; returns the total (syncs).
 void exclusive_scan_u32(State& st, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev);
 void exclusive_scan_u64(State& st, const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev);
+// The u32 scan on stream `s` with its partial sums in `scratch` (2 words per
+// 2048 inputs per level above the first); allocates nothing.  n >= 1.
+void exclusive_scan_u32_scratch(hipStream_t s, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev, uint64_t* scratch,
+                                uint64_t scratch_words);
 
 // ---- sort (sort.hip)
 // A digit of the LSD plan: up to three bit ranges of key word `word`,

@@ -324,6 +324,38 @@ static void excl_scan_impl(State& st, const TIn* in, uint64_t* out, uint64_t n, 
     dfree(offs, st.stream);
 }
 
+// The same scan with its per-level partial sums in caller-provided device
+// memory (2 words per kScanTile inputs at every level above the first) and no
+// profiling events: nothing is allocated.  n >= 1.
+template <class TIn>
+static void excl_scan_scratch_impl(hipStream_t s, const TIn* in, uint64_t* out, uint64_t n, uint64_t* total_dev, uint64_t* scratch,
+                                   uint64_t scratch_words) {
+    const unsigned nb = grid_for(n, kScanTile);
+    const bool vec = (((uintptr_t)in | (uintptr_t)out) & 15) == 0;
+    if (nb == 1) {
+        if (vec) scan_apply_kernel<TIn, true><<<1, kBlock, 0, s>>>(in, n, nullptr, out, total_dev);
+        else scan_apply_kernel<TIn, false><<<1, kBlock, 0, s>>>(in, n, nullptr, out, total_dev);
+        SPQ_HIP(hipGetLastError());
+        return;
+    }
+    SPQ_REQUIRE(2 * (uint64_t)nb <= scratch_words, "scan scratch too small");
+    uint64_t* sums = scratch;
+    uint64_t* offs = scratch + nb;
+    if (vec) scan_reduce_kernel<TIn, true><<<nb, kBlock, 0, s>>>(in, n, sums);
+    else scan_reduce_kernel<TIn, false><<<nb, kBlock, 0, s>>>(in, n, sums);
+    SPQ_HIP(hipGetLastError());
+    excl_scan_scratch_impl<uint64_t>(s, sums, offs, nb, nullptr, scratch + 2 * (uint64_t)nb, scratch_words - 2 * (uint64_t)nb);
+    if (vec) scan_apply_kernel<TIn, true><<<nb, kBlock, 0, s>>>(in, n, offs, out, total_dev);
+    else scan_apply_kernel<TIn, false><<<nb, kBlock, 0, s>>>(in, n, offs, out, total_dev);
+    SPQ_HIP(hipGetLastError());
+}
+
+void exclusive_scan_u32_scratch(hipStream_t s, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev, uint64_t* scratch,
+                                uint64_t scratch_words) {
+    SPQ_REQUIRE(n >= 1, "empty scan");
+    excl_scan_scratch_impl<uint32_t>(s, in, out, n, total_dev, scratch, scratch_words);
+}
+
 void exclusive_scan_u32(State& st, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total_dev) {
     excl_scan_impl<uint32_t>(st, in, out, n, total_dev);
 }
