@@ -47,11 +47,14 @@ def kernel_profile(lib):
     return sorted(out, key=lambda r: -r["ms"])
 
 
-def cpu_baseline(address_bits, data_bits, mem_words, budget_s=10.0):
-    """The oracle (single-thread restatement of the reference CPU path) on a bounded
-    sample: the same circuit, repeated until `budget_s` seconds have elapsed."""
+def cpu_baseline(address_bits, data_bits, mem_words, budget_s=10.0, module="_oracle"):
+    """The oracle (restatement of the reference CPU path; `_oracle_omp` has the
+    reference's own OpenMP loops enabled) on a bounded sample: the same circuit,
+    repeated until `budget_s` seconds have elapsed."""
+    import importlib
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import _oracle as o
+    o = importlib.import_module(module)
     from sparqsim_amd.workloads import QRAMInterference
 
     o.System.clear()
@@ -69,6 +72,29 @@ def cpu_baseline(address_bits, data_bits, mem_words, budget_s=10.0):
         reps += 1
     o.System.clear()
     return circ.entry_ops * reps / total, total, reps
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_process(module, threads, address_bits, seconds):
+    """cpu_baseline in a child process (the two oracle builds cannot share one
+    interpreter); OMP_NUM_THREADS = threads.  Returns (entries/s, seconds, passes)."""
+    import subprocess
+
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", module,
+                          "--cpu-address-bits", str(address_bits), "--cpu-seconds", str(seconds)],
+                         env=env, capture_output=True, text=True, timeout=600, check=True)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return r["entries_per_s"], r["seconds"], r["passes"]
 
 
 def qda_walk_run(ps, nqubit, nsteps, total_steps):
@@ -197,13 +223,19 @@ def qda_sparse_leg(ps, nqubit, steps):
     return out
 
 
-def sort_leg(ps, lib, address_bits, reps):
+SORT_KERNELS = ("radix_hist_kernel", "radix_scatter_kernel", "transpose_u32_kernel", "scan_reduce_kernel",
+                "scan_apply_kernel", "gather_state_kernel", "gather_word_kernel", "key_stats_kernel",
+                "key_stats_final_kernel", "payload_scatter_kernel", "local_sort_kernel")
+
+
+def sort_leg(ps, lib, address_bits, reps, mode=0):
     """SortUnconditional of 2^address_bits entries whose 48-bit keys (b: 24 bits,
     most significant, then a: address_bits) are scrambled before each sort by a
-    QRAM load b ^= M[a] with M uniform from splitmix64: every sort moves all
-    entries through the digit passes of both fields and the permutation gather.
-    Reports each sort kernel's algorithmic bytes / time (HIP events on the launch
-    stream) against the HBM peak."""
+    QRAM load b ^= M[a] with M uniform from splitmix64.  mode 0: whole entries
+    through prefix passes, then the LDS chunk sort; mode 1: LSD passes over an
+    index payload and the permutation gather.  Reports each sort kernel's
+    algorithmic bytes / time (HIP events on the launch stream) against the HBM peak."""
+    ps.set_sort_mode(mode)
     from sparqsim_amd.workloads import qram_memory
 
     ps.System.clear()
@@ -227,17 +259,15 @@ def sort_leg(ps, lib, address_bits, reps):
         once()
     lib.sparq_prof_enable(0)
     prof = kernel_profile(lib)
-    sort_kernels = ("radix_hist_kernel", "radix_scatter_kernel", "transpose_u32_kernel", "scan_reduce_kernel",
-                    "scan_apply_kernel", "gather_state_kernel", "gather_word_kernel", "key_stats_kernel",
-                    "key_stats_final_kernel")
-    rows = [r for r in prof if r["name"] in sort_kernels and r["ms"] > 0]
+    ps.set_sort_mode(0)
+    rows = [r for r in prof if r["name"] in SORT_KERNELS and r["ms"] > 0]
     ms = sum(r["ms"] for r in rows)
     byt = sum(r["bytes"] for r in rows)
     n = st.size()
     ps.System.clear()
     return {
         "workload": f"SortUnconditional of 2^{address_bits} entries, 48-bit keys scrambled by a QRAM load before each "
-                    f"sort, {reps} sorts",
+                    f"sort, {reps} sorts, " + ("prefix passes + LDS chunk sort" if mode == 0 else "LSD passes + gather"),
         "entries": n, "ms_per_sort": ms / reps, "entries_per_s": n * reps / (ms * 1e-3),
         "GBs": byt / (ms * 1e-3) / 1e9, "frac_of_peak": byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "kernels": [{"name": r["name"], "launches_per_sort": r["launches"] / reps, "us_per_launch": r["ms"] / r["launches"] * 1e3,
@@ -291,6 +321,7 @@ def main():
     ap.add_argument("--cpu-address-bits", type=int, default=20, help="size of the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--qda-qubits", type=int, default=20)
     ap.add_argument("--qda-steps", type=int, default=40, help="walk steps timed in the QDA leg (0 skips it)")
     ap.add_argument("--qda-total-steps", type=int, default=40568, help="length of the schedule s = n/total")
@@ -311,6 +342,14 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 512 MiB device copies (the reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.cpu_baseline_child:   # child of cpu_baseline_process: CPU only
+        from sparqsim_amd.workloads import qram_memory
+
+        eps, secs, reps = cpu_baseline(args.cpu_address_bits, args.data_bits,
+                                       qram_memory(args.cpu_address_bits, args.data_bits, seed_offset=1),
+                                       args.cpu_seconds, args.cpu_baseline_child)
+        print(json.dumps({"entries_per_s": eps, "seconds": secs, "passes": reps}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this engine has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
@@ -405,16 +444,28 @@ def main():
     if args.cks_terms > 0:
         result["cks"] = cks_leg(ps, args.cks_qubits, args.cks_terms, args.cks_rhs, args.cks_band_terms)
     if args.sort_bits > 0:
-        result["sort"] = sort_leg(ps, lib, args.sort_bits, args.sort_reps)
+        result["sort"] = sort_leg(ps, lib, args.sort_bits, args.sort_reps, mode=0)
+        result["sort_lsd"] = sort_leg(ps, lib, args.sort_bits, args.sort_reps, mode=1)
     if args.grover_traj > 0:
         result["noisy_grover"] = grover_leg(ps, args.grover_traj, args.grover_bits, args.grover_iterations, 1e-3,
                                             min(args.grover_plain_traj, args.grover_traj))
     if not args.no_cpu_baseline:
+        # BASELINE mode (b): the reference's OpenMP loops on the host's cores (sorts stay
+        # sequential, as in the reference); mode (a), one thread as shipped, beside it
         ca = args.cpu_address_bits
-        eps, secs, reps = cpu_baseline(ca, D, qram_memory(ca, D, seed_offset=1), args.cpu_seconds)
-        result["cpu_baseline"] = {"value": eps, "unit": "entries/s", "cores": 1, "kind": "port",
-                                  "sample": f"same circuit at 2^{ca} entries, {reps} passes in {secs:.1f} s, "
-                                            "oracle (restated reference CPU path), one thread"}
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        half = args.cpu_seconds / 2
+        eps_omp, secs_omp, reps_omp = cpu_baseline_process("_oracle_omp", threads, ca, half)
+        eps_1, secs_1, reps_1 = cpu_baseline_process("_oracle", 1, ca, half)
+        model = cpu_model()
+        result["cpu_baseline"] = {"value": eps_omp, "unit": "entries/s", "cores": threads, "kind": "port",
+                                  "sample": f"same circuit at 2^{ca} entries, {reps_omp} passes in {secs_omp:.1f} s, "
+                                            f"oracle (restated reference CPU path) with the reference's OpenMP loops, "
+                                            f"{threads} threads on {model}"}
+        result["cpu_baseline_single_thread"] = {
+            "value": eps_1, "unit": "entries/s", "cores": 1, "kind": "port",
+            "sample": f"same circuit at 2^{ca} entries, {reps_1} passes in {secs_1:.1f} s, one thread "
+                      f"(the reference as shipped, SINGLE_THREAD) on {model}"}
     print(json.dumps(result), flush=True)
 
 

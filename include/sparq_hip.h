@@ -354,6 +354,13 @@ int sparq_select(sparq_state_t st, const sparq_cond_t* conds, uint32_t nconds, d
  * default) lets the engine's autotuner pick it per size class by timing the
  * first sorts of each class with both widths. */
 int sparq_set_radix_bits(int bits);
+/* Sort strategy (no counterpart in the reference): 0 (the default) moves whole
+ * entries by the top 8-24 ordering bits in a few LSD passes and then sorts
+ * chunks of <= 4096 entries in LDS (keys of <= 5 words, <= 128 ordering bits
+ * below the prefix; anything else, or a bucket too large for one chunk, takes
+ * the LSD path); 1 always takes the LSD radix passes over an index payload and
+ * a final gather.  Both give the same stable order. */
+int sparq_set_sort_mode(int mode);
 
 /* Learned output sizer (no counterpart in the reference, which always counts
  * before it allocates).  A 4-layer FP32 MLP, 16 features -> 64 -> 64 -> 64 -> 1

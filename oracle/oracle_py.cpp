@@ -46,7 +46,12 @@ struct FnOp {
 };
 static OpPtr fnop(std::function<void(State&)> f, const char* n) { return wrap(FnOp{std::move(f)}, n); }
 
+// The OpenMP build (-DORACLE_OMP) is the same module under the name _oracle_omp.
+#ifdef ORACLE_OMP
+PYBIND11_MODULE(_oracle_omp, m) {
+#else
 PYBIND11_MODULE(_oracle, m) {
+#endif
     m.doc() = "CPU oracle of the SparQ sparse-state operator-apply path (test infrastructure)";
 
     py::enum_<StateStorageType>(m, "StateStorageType")

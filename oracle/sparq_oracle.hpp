@@ -449,7 +449,10 @@ struct Hadamard_Int {
             st[l].amplitude = 0;
         } else if (n == full) {                         // hadamard.cpp:42-61
             for (size_t qn = 0; qn < n_digits; ++qn)
-                for (size_t i = 0; i < full; ++i)
+#ifdef ORACLE_OMP
+#pragma omp parallel for
+#endif
+                for (int64_t i = 0; i < (int64_t)full; ++i)
                     if ((i >> qn) & 1) {
                         complex_t a0 = st[i + l - pow2(qn)].amplitude, a1 = st[i + l].amplitude;
                         st[i + l - pow2(qn)].amplitude = (a0 + a1) * sqrt2inv;
@@ -464,7 +467,10 @@ struct Hadamard_Int {
                 st[i].amplitude = 0;
             }
             for (size_t qn = 0; qn < n_digits; ++qn)
-                for (size_t i = 0; i < full; ++i)
+#ifdef ORACLE_OMP
+#pragma omp parallel for
+#endif
+                for (int64_t i = 0; i < (int64_t)full; ++i)
                     if ((i >> qn) & 1) {
                         complex_t a0 = st[i + orig - pow2(qn)].amplitude, a1 = st[i + orig].amplitude;
                         st[i + orig - pow2(qn)].amplitude = (a0 + a1) * sqrt2inv;

@@ -362,6 +362,13 @@ int sparq_set_radix_bits(int bits) {
     });
 }
 
+int sparq_set_sort_mode(int mode) {
+    return guarded([&] {
+        SPQ_REQUIRE(mode == 0 || mode == 1, "sort mode must be 0 (prefix + local) or 1 (LSD passes only)");
+        set_sort_mode(mode);
+    });
+}
+
 int sparq_release_cached_memory(void) {
     return guarded([&] {
         if (g_device < 0) return;

@@ -138,6 +138,8 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits = 8,
                   hipEvent_t pass_end = nullptr, const uint64_t* word_src = nullptr, uint64_t word_cap = 0);
 // Digit width forced for every sort (8 or 11), or 0 for the autotuner.
 void set_radix_bits(int bits);
+// 0: prefix passes + LDS-local chunk sort where it applies, 1: LSD passes only.
+void set_sort_mode(int mode);
 // One pass over the keys: bits that vary across the state, and whether the
 // state is already ordered with the optional field `last` least significant.
 // With a SizerQuery the same pass also histograms popcount(value of `last`)

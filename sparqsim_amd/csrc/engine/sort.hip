@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
+#include <unordered_map>
 
 #include "blockscan.hpp"
 #include "engine.hpp"
@@ -333,8 +334,24 @@ static void sort_with_plans(State& st, const std::vector<Digit>& p8, const std::
     g_ms_per_pass[measure][lg] = std::max(ms, 1e-6f) / (float)plan.size() / (float)((double)n / 1048576.0);
 }
 
+// Sort by an ordering given as its varying bit positions in LSD order: the
+// prefix + local path when it applies (sort_local), else the LSD radix passes
+// with the autotuned digit width.
+static bool sort_local(State& st, const std::vector<std::pair<int, int>>& bits);
+static int g_sort_mode = 0;   // 0: prefix + local sort where it applies, 1: LSD passes only
+
+void set_sort_mode(int mode) { g_sort_mode = mode; }
+
+static void sort_ordering(State& st, const std::vector<std::pair<int, int>>& bits) {
+    if (bits.empty() || st.n <= 1) return;
+    if (g_sort_mode == 0 && sort_local(st, bits)) return;
+    sort_with_plans(st, digits_of(bits, 8), digits_of(bits, 11));
+}
+
 void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
-    sort_with_plans(st, make_plan(ks, st.W, last, 8), make_plan(ks, st.W, last, 11));
+    std::vector<sparq_field_t> minor;
+    if (last) minor.push_back(*last);
+    sort_ordering(st, order_bits(ks.vary, st.W, {}, minor, last && ks.last_sorted));
 }
 
 void sort_state_by(State& st, const std::vector<sparq_field_t>& major, const std::vector<sparq_field_t>& minor) {
@@ -354,8 +371,7 @@ void sort_state_by(State& st, const std::vector<sparq_field_t>& major, const std
     }
     if (st.n <= 1) return;
     const KeyStats ks = key_stats(st, nullptr);
-    sort_with_plans(st, digits_of(order_bits(ks.vary, st.W, major, minor, false), 8),
-                    digits_of(order_bits(ks.vary, st.W, major, minor, false), 11));
+    sort_ordering(st, order_bits(ks.vary, st.W, major, minor, false));
 }
 
 // Sort key of SortByAmplitude: |a|^2 descending.  A non-negative double's bit
@@ -731,6 +747,541 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits, hip
     dfree(bh, st.stream);
     dfree(bh_rows, st.stream);
     dfree(offs, st.stream);
+}
+
+
+// ================================================================ prefix + local sort
+// For states that fit (key words W <= kLocalMaxW, at most 128 ordering bits
+// below the prefix): a few LSD passes over the top B ordering bits move whole
+// entries (key words and amplitude, 8-bit digits, tile-staged writes), which
+// leaves buckets of equal prefix contiguous; then one workgroup per chunk of
+// consecutive buckets (<= kLocalCap entries) sorts the chunk in LDS by the
+// remaining bits (compacted into <= 2 words) and its bucket ordinal, and
+// writes the entries in order.  No index permutation and no random gather over
+// the whole state: the chunk's own entries are re-read from L2.  A chunk
+// larger than kLocalCap (a skewed key distribution) sets a device flag; the
+// host then discards the output and runs the LSD path on the untouched input.
+constexpr int kLocalCap = 2048;       // entries per chunk
+constexpr int kLocalBlock = 256;      // 4 waves of 8 items (three workgroups per CU)
+constexpr int kLocalMaxW = 5;
+constexpr int kMaxSegs = 40;
+constexpr int kPayloadTile = 4096;    // entries per workgroup of a payload pass
+constexpr int kPayloadBlock = 512;
+
+// MSB-first bit ranges of the key whose concatenation is a compact key.
+struct BitSegs {
+    uint32_t n;
+    uint8_t word[kMaxSegs], shift[kMaxSegs], width[kMaxSegs];
+};
+struct LocalArgs {
+    BitSegs pre;        // the top B ordering bits (bucket prefix)
+    BitSegs rem;        // the remaining R ordering bits
+    uint32_t rem_bits;  // R
+    uint32_t h;         // chunk spacing: chunk j starts at the first bucket start >= j*h
+    uint64_t n;
+};
+
+template <int W>
+__device__ __forceinline__ uint32_t prefix_of(const uint64_t (&k)[W], const BitSegs& sg) {
+    uint32_t v = 0;
+    for (uint32_t i = 0; i < sg.n; ++i) {
+        const uint64_t bits = (kword<W>(k, sg.word[i]) >> sg.shift[i]) & fmask(sg.width[i]);
+        v = (v << sg.width[i]) | (uint32_t)bits;
+    }
+    return v;
+}
+
+// Compact key of the remaining bits: Q words, word 0 least significant.
+template <int W, int Q>
+__device__ __forceinline__ void compact_of(const uint64_t (&k)[W], const BitSegs& sg, uint64_t (&c)[Q]) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) c[q] = 0;
+    for (uint32_t i = 0; i < sg.n; ++i) {
+        const uint32_t w = sg.width[i];
+        const uint64_t bits = (kword<W>(k, sg.word[i]) >> sg.shift[i]) & fmask(w);
+        if (Q == 2) c[Q - 1] = w >= 64 ? c[0] : ((c[Q - 1] << w) | (w ? (c[0] >> (64 - w)) : 0));
+        c[0] = (w >= 64 ? 0 : (c[0] << w)) | bits;
+    }
+}
+
+// First bucket start at or after q0 (bucket = run of equal prefix), or n.
+// Called by the whole workgroup; the result is uniform.
+template <int W>
+__device__ uint64_t chunk_start(uint64_t q0, const uint64_t* __restrict__ keys, uint64_t cap, const LocalArgs& la, uint32_t* lds_min) {
+    if (q0 == 0) return 0;
+    for (uint64_t base = q0; base < la.n; base += kLocalBlock) {
+        const uint64_t q = base + threadIdx.x;
+        bool head = false;
+        if (q < la.n) {   // q >= 1: q0 >= 1
+            uint64_t a[W], b[W];
+            load_key<W>(keys, cap, q, a);
+            load_key<W>(keys, cap, q - 1, b);
+            head = prefix_of<W>(a, la.pre) != prefix_of<W>(b, la.pre);
+        }
+        if (threadIdx.x == 0) *lds_min = ~0u;
+        __syncthreads();
+        if (head) atomicMin(lds_min, (uint32_t)threadIdx.x);
+        __syncthreads();
+        const uint32_t m = *lds_min;
+        __syncthreads();
+        if (m != ~0u) return base + m;
+    }
+    return la.n;
+}
+
+template <int W, int Q>
+__global__ void __launch_bounds__(kLocalBlock) local_sort_kernel(const uint64_t* __restrict__ kin, const double2* __restrict__ ain,
+                                                            uint64_t cap_in, LocalArgs la, uint64_t* __restrict__ kout,
+                                                            double2* __restrict__ aout, uint64_t cap_out,
+                                                            unsigned int* __restrict__ overflow) {
+    constexpr int NW = kLocalBlock / 64, IT = kLocalCap / kLocalBlock, R = 256;
+    constexpr uint32_t kInvalid = 1u << 8;
+    __shared__ uint64_t sck[Q][kLocalCap];
+    __shared__ uint32_t soi[kLocalCap];       // bucket ordinal << 16 | chunk position of the entry
+    __shared__ uint32_t wcnt[NW][R];
+    uint64_t* spre = &sck[0][0];              // prefixes, before the passes use sck
+    __shared__ uint32_t rowpre[NW * IT];
+    __shared__ uint32_t scan_lds[NW];
+    __shared__ uint64_t red[2 * Q][NW];
+    __shared__ uint32_t misc[2];
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t s = la.h ? chunk_start<W>((uint64_t)blockIdx.x * la.h, kin, cap_in, la, &misc[0]) : 0;
+    const uint64_t e = la.h ? chunk_start<W>((uint64_t)(blockIdx.x + 1) * la.h, kin, cap_in, la, &misc[0]) : la.n;
+    const uint64_t len = e - s;
+    if (len > (uint64_t)kLocalCap) {
+        if (threadIdx.x == 0) atomicOr(overflow, 1u);
+        return;
+    }
+    if (len == 0) return;
+
+    // ---- load: item it of (wave, lane) is chunk position p = wave*(cap/NW) + it*64 + lane
+    uint64_t ck[IT][Q];
+    uint32_t oi[IT];   // bucket ordinal << 16 | chunk position (both < kLocalCap)
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const uint32_t p = wave * (kLocalCap / NW) + it * 64 + lane;
+        const uint64_t src = s + (p < len ? p : len - 1);
+        uint64_t a[W];
+        load_key<W>(kin, cap_in, src, a);
+        compact_of<W, Q>(a, la.rem, ck[it]);
+        if (p < len) spre[p] = prefix_of<W>(a, la.pre);
+        oi[it] = p;
+    }
+    __syncthreads();
+    // ---- bucket ordinal: heads where the prefix changes (position 0 starts the chunk's first bucket)
+    uint32_t mine[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const uint32_t p = wave * (kLocalCap / NW) + it * 64 + lane;
+        const bool head = p < len && (p == 0 || spre[p] != spre[p - 1]);
+        const uint64_t m = __ballot(head);
+        mine[it] = (uint32_t)__popcll(m & (lanemask_lt() | (1ull << lane)));   // heads at or before p in this row
+        if (lane == 0) rowpre[wave * IT + it] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {   // exclusive scan of the NW*IT row counts, rows in position order
+        const uint32_t v = threadIdx.x < NW * IT ? rowpre[threadIdx.x] : 0;
+        const uint32_t inc = wave_incl_scan(v);
+        if (threadIdx.x < NW * IT) rowpre[threadIdx.x] = inc - v;
+        if (threadIdx.x == 63) misc[1] = inc;     // bucket count
+    }
+    __syncthreads();
+    const uint32_t nbuckets = misc[1];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) oi[it] |= (rowpre[wave * IT + it] + mine[it] - 1) << 16;
+    // ---- bits of the compact key that vary inside the chunk
+    uint64_t vo[Q], va[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        vo[q] = 0;
+        va[q] = ~0ull;
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const uint32_t p = wave * (kLocalCap / NW) + it * 64 + lane;
+        if (p < len) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                vo[q] |= ck[it][q];
+                va[q] &= ck[it][q];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            vo[q] |= __shfl_xor(vo[q], off, 64);
+            va[q] &= __shfl_xor(va[q], off, 64);
+        }
+        if (lane == 0) {
+            red[2 * q][wave] = vo[q];
+            red[2 * q + 1][wave] = va[q];
+        }
+    }
+    __syncthreads();
+    uint64_t vary[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        uint64_t o = 0, an = ~0ull;
+        for (int w = 0; w < NW; ++w) {
+            o |= red[2 * q][w];
+            an &= red[2 * q + 1][w];
+        }
+        vary[q] = o ^ an;
+    }
+    const uint32_t ord_bits = nbuckets <= 1 ? 0 : 32 - __clz(nbuckets - 1);
+
+    // ---- LSD passes over 8-bit windows: compact key windows that vary, then the ordinal
+    const uint32_t key_windows = (la.rem_bits + 7) / 8;
+    const uint32_t ord_windows = (ord_bits + 7) / 8;
+    for (uint32_t win = 0; win < key_windows + ord_windows; ++win) {
+        const bool of_ord = win >= key_windows;
+        const uint32_t wq = of_ord ? 0 : (win * 8) / 64, wsh = of_ord ? (win - key_windows) * 8 : (win * 8) % 64;
+        if (!of_ord) {
+            uint64_t vw = 0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) vw |= (q == (int)wq) ? vary[q] : 0;
+            if (((vw >> wsh) & 0xff) == 0) continue;   // uniform: the window does not vary
+        }
+        for (int d = threadIdx.x; d < R; d += kLocalBlock)
+#pragma unroll
+            for (int w = 0; w < NW; ++w) wcnt[w][d] = 0;
+        __syncthreads();
+        uint32_t dig[IT], before[IT], got[IT];
+        int leader[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const uint32_t p = wave * (kLocalCap / NW) + it * 64 + lane;
+            uint64_t src = 0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) src |= (q == (int)wq) ? ck[it][q] : 0;
+            const uint32_t d = p < len ? (uint32_t)(((of_ord ? (uint64_t)(oi[it] >> 16) : src) >> wsh) & 0xff) : kInvalid;
+            dig[it] = d;
+            const uint64_t peers = match_any<8>(d, 8);
+            before[it] = (uint32_t)__popcll(peers & lanemask_lt());
+            leader[it] = __ffsll((unsigned long long)peers) - 1;
+            got[it] = (p < len && before[it] == 0) ? atomicAdd(&wcnt[wave][d & (R - 1)], (uint32_t)__popcll(peers)) : 0u;
+        }
+        uint32_t rank[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) rank[it] = (uint32_t)__shfl((int)got[it], leader[it], 64) + before[it];
+        __syncthreads();
+        // digit totals -> exclusive scan over digits -> per-wave starts
+        uint32_t c[NW], sum = 0;
+        const bool owner = (int)threadIdx.x < R;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            c[w] = owner ? wcnt[w][threadIdx.x] : 0;
+            sum += c[w];
+        }
+        uint32_t tot;
+        uint32_t start = block_excl_scan<uint32_t>(sum, scan_lds, tot);
+        if (owner) {
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                wcnt[w][threadIdx.x] = start;
+                start += c[w];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            if (dig[it] != kInvalid) {
+                const uint32_t pos = wcnt[wave][dig[it]] + rank[it];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) sck[q][pos] = ck[it][q];
+                soi[pos] = oi[it];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const uint32_t p = wave * (kLocalCap / NW) + it * 64 + lane;
+            if (p < len) {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) ck[it][q] = sck[q][p];
+                oi[it] = soi[p];
+            }
+        }
+        __syncthreads();
+    }
+    // ---- write out: position p takes the entry idx of the chunk (re-read from L2)
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const uint32_t p = wave * (kLocalCap / NW) + it * 64 + lane;
+        if (p < len) {
+            const uint64_t src = s + (oi[it] & 0xffffu);
+            uint64_t a[W];
+            load_key<W>(kin, cap_in, src, a);
+            const double2 v = ain[src];
+            store_key<W>(kout, cap_out, s + p, a);
+            aout[s + p] = v;
+        }
+    }
+}
+
+// One stable LSD pass over the whole entries (W key words and the amplitude)
+// by an 8-bit digit: per-wave ballot ranking, then every plane (each key word,
+// then the two amplitude halves) is staged through LDS in digit order and
+// written out in runs.
+template <int W>
+__global__ void __launch_bounds__(kPayloadBlock) payload_scatter_kernel(const uint64_t* __restrict__ kin, const double2* __restrict__ ain,
+                                                                   uint64_t cap_in, uint64_t n, Digit dg, const uint64_t* __restrict__ offs,
+                                                                   uint32_t nb, uint64_t* __restrict__ kout, double2* __restrict__ aout,
+                                                                   uint64_t cap_out) {
+    constexpr int R = 256, NW = kPayloadBlock / 64, NI = kPayloadTile / kPayloadBlock;
+    constexpr uint32_t kInvalid = 1u << 8;
+    __shared__ uint32_t wcnt[NW][R];
+    __shared__ uint32_t dstart[R];
+    __shared__ uint64_t goff[R];
+    __shared__ uint64_t stage[kPayloadTile];
+    __shared__ uint16_t sdig[kPayloadTile];
+    __shared__ uint32_t scan_lds[NW];
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int d = threadIdx.x; d < R; d += kPayloadBlock)
+#pragma unroll
+        for (int w = 0; w < NW; ++w) wcnt[w][d] = 0;
+    __syncthreads();
+    const uint32_t tile_id = xcd_tile(blockIdx.x, nb);
+    const uint64_t tile = (uint64_t)tile_id * kPayloadTile;
+    uint32_t dig[NI], rank[NI];
+    {
+        uint32_t before[NI], got[NI];
+        int leader[NI];
+#pragma unroll
+        for (int it = 0; it < NI; ++it) {
+            const uint64_t e = tile + (uint64_t)wave * (kPayloadTile / NW) + it * 64 + lane;
+            const uint64_t word = kin[(uint64_t)dg.word * cap_in + (e < n ? e : n - 1)];
+            const uint32_t d = e < n ? digit_of(word, dg) : kInvalid;
+            dig[it] = d;
+            const uint64_t peers = match_any<8>(d, dg.bits);
+            before[it] = (uint32_t)__popcll(peers & lanemask_lt());
+            leader[it] = __ffsll((unsigned long long)peers) - 1;
+            got[it] = (e < n && before[it] == 0) ? atomicAdd(&wcnt[wave][d & (R - 1)], (uint32_t)__popcll(peers)) : 0u;
+        }
+#pragma unroll
+        for (int it = 0; it < NI; ++it) rank[it] = (uint32_t)__shfl((int)got[it], leader[it], 64) + before[it];
+    }
+    __syncthreads();
+    {
+        uint32_t c[NW], sum = 0;
+        const bool owner = (int)threadIdx.x < R;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            c[w] = owner ? wcnt[w][threadIdx.x] : 0;
+            sum += c[w];
+        }
+        uint32_t tot;
+        uint32_t start = block_excl_scan<uint32_t>(sum, scan_lds, tot);
+        if (owner) {
+            dstart[threadIdx.x] = start;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                wcnt[w][threadIdx.x] = start;
+                start += c[w];
+            }
+            goff[threadIdx.x] = offs[(uint64_t)threadIdx.x * nb + tile_id];
+        }
+    }
+    __syncthreads();
+    uint32_t pos[NI];
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+        pos[it] = dig[it] != kInvalid ? wcnt[wave][dig[it]] + rank[it] : 0u;
+        if (dig[it] != kInvalid) sdig[pos[it]] = (uint16_t)dig[it];
+    }
+    const uint32_t count = (uint32_t)min<uint64_t>((uint64_t)kPayloadTile, n - tile);
+    // planes: key words 0..W-1, then the amplitude's real and imaginary halves
+    double im[NI];
+    for (int plane = 0; plane < W + 2; ++plane) {
+#pragma unroll
+        for (int it = 0; it < NI; ++it) {
+            const uint64_t e = tile + (uint64_t)wave * (kPayloadTile / NW) + it * 64 + lane;
+            if (dig[it] != kInvalid) {
+                uint64_t v;
+                if (plane < W) {
+                    v = kin[(uint64_t)plane * cap_in + e];
+                } else if (plane == W) {
+                    const double2 a = ain[e];   // one load; the imaginary half waits in a register
+                    im[it] = a.y;
+                    v = (uint64_t)__double_as_longlong(a.x);
+                } else {
+                    v = (uint64_t)__double_as_longlong(im[it]);
+                }
+                stage[pos[it]] = v;
+            }
+        }
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < count; p += kPayloadBlock) {
+            const uint32_t d = sdig[p];
+            const uint64_t dst = goff[d] + (p - dstart[d]);
+            const uint64_t v = stage[p];
+            if (plane < W) kout[(uint64_t)plane * cap_out + dst] = v;
+            else reinterpret_cast<double*>(aout + dst)[plane - W] = __longlong_as_double((long long)v);
+        }
+        __syncthreads();
+    }
+}
+
+template <int W>
+static void launch_payload_pass(State& st, const uint64_t* kin, const double2* ain, uint64_t cap_in, const Digit& dg, const uint64_t* offs,
+                                uint32_t nb, uint64_t* kout, double2* aout, uint64_t cap_out) {
+    {
+        SPQ_PROF(st.stream, "payload_scatter_kernel", st.n * 2.0 * (8.0 * W + 16));
+        payload_scatter_kernel<W><<<nb, kPayloadBlock, 0, st.stream>>>(kin, ain, cap_in, st.n, dg, offs, nb, kout, aout, cap_out);
+    }
+    SPQ_HIP(hipGetLastError());
+}
+
+template <int W>
+static void launch_local(State& st, const uint64_t* kin, const double2* ain, uint64_t cap_in, const LocalArgs& la, unsigned grid, int Q,
+                         uint64_t* kout, double2* aout, uint64_t cap_out, unsigned int* overflow) {
+    SPQ_PROF(st.stream, "local_sort_kernel", st.n * (8.0 * W + 2.0 * (8.0 * W + 16)));
+    if (Q == 1)
+        local_sort_kernel<W, 1><<<grid, kLocalBlock, 0, st.stream>>>(kin, ain, cap_in, la, kout, aout, cap_out, overflow);
+    else
+        local_sort_kernel<W, 2><<<grid, kLocalBlock, 0, st.stream>>>(kin, ain, cap_in, la, kout, aout, cap_out, overflow);
+    SPQ_HIP(hipGetLastError());
+}
+
+// MSB-first segments of a run of bit positions given in LSD order.
+static bool segments_of(const std::vector<std::pair<int, int>>& lsd, BitSegs& sg) {
+    sg = BitSegs{};
+    for (size_t i = lsd.size(); i-- > 0;) {
+        const int w = lsd[i].first, p = lsd[i].second;
+        if (sg.n > 0 && sg.word[sg.n - 1] == w && sg.shift[sg.n - 1] == p + 1) {
+            --sg.shift[sg.n - 1];
+            ++sg.width[sg.n - 1];
+            continue;
+        }
+        if (sg.n == (uint32_t)kMaxSegs) return false;
+        sg.word[sg.n] = (uint8_t)w;
+        sg.shift[sg.n] = (uint8_t)p;
+        sg.width[sg.n] = 1;
+        ++sg.n;
+    }
+    return true;
+}
+
+// Orderings whose buckets overflowed a chunk at a size class: a walk repeats the
+// same sorts every step, so later sorts of the ordering go straight to the LSD
+// passes instead of running the prefix passes again only to fall back.
+static std::mutex g_local_mu;
+static std::unordered_map<uint64_t, int> g_local_overflowed;
+
+static uint64_t ordering_signature(const std::vector<std::pair<int, int>>& bits, uint32_t W, uint64_t n) {
+    uint64_t h = 0x9E3779B97F4A7C15ull ^ W;
+    int lg = 0;
+    while ((uint64_t(1) << (lg + 1)) <= n && lg < 63) ++lg;
+    auto mix = [&h](uint64_t v) { h = (h ^ v) * 0x100000001B3ull; h ^= h >> 29; };
+    mix((uint64_t)lg);
+    for (const auto& b : bits) mix((uint64_t)b.first << 8 | (uint64_t)b.second);
+    return h;
+}
+
+static bool sort_local(State& st, const std::vector<std::pair<int, int>>& bits) {
+    const uint64_t n = st.n;
+    const uint32_t T = (uint32_t)bits.size();
+    if (st.W > (uint32_t)kLocalMaxW || n >= (1ull << 32)) return false;
+    const uint64_t sig = ordering_signature(bits, st.W, n);
+    {
+        std::lock_guard<std::mutex> lock(g_local_mu);
+        if (g_local_overflowed.count(sig)) return false;
+    }
+    // prefix bits B (multiple of 8): buckets of about n / 2^B <= 512 entries expected
+    uint32_t B = 0;
+    if (n > (uint64_t)kLocalCap) {
+        uint32_t need = 0;
+        while ((n >> need) > (uint64_t)kLocalCap / 8) ++need;
+        B = std::min<uint32_t>(T, (need + 7) / 8 * 8);
+        if (B > 24) return false;
+    }
+    if (B > 0 && (n >> B) > (uint64_t)kLocalCap / 8) return false;   // too few ordering bits for small buckets
+    const uint32_t R = T - B;
+    if (R > 128) return false;
+    const int Q = R > 64 ? 2 : 1;
+    const std::vector<std::pair<int, int>> low(bits.begin(), bits.begin() + R), top(bits.begin() + R, bits.end());
+    LocalArgs la{};
+    if (!segments_of(low, la.rem) || !segments_of(top, la.pre)) return false;
+    la.rem_bits = R;
+    la.n = n;
+    la.h = B ? (uint32_t)(kLocalCap * 3 / 4) : 0;   // chunk = h entries + the bucket straddling its end
+    const unsigned grid = B ? grid_for(n, la.h) : 1;
+    if (!B && n > (uint64_t)kLocalCap) return false;
+
+    // prefix passes: whole entries, ping-pong between two scratch sets
+    const std::vector<Digit> plan = digits_of(top, 8);
+    const uint64_t* src_k = st.keys;
+    const double2* src_a = st.amps;
+    uint64_t src_cap = st.cap;
+    uint64_t* setk[2] = {nullptr, nullptr};
+    double2* seta[2] = {nullptr, nullptr};
+    int cur = -1;   // set holding the current data (-1: the state's arrays)
+    const uint32_t nb = grid_for(n, kPayloadTile);
+    uint32_t* bh = nullptr;
+    uint32_t* bh_rows = nullptr;
+    uint64_t* offs = nullptr;
+    if (!plan.empty()) {
+        bh = dalloc<uint32_t>((size_t)256 * nb, st.stream);
+        bh_rows = dalloc<uint32_t>((size_t)256 * nb, st.stream);
+        offs = dalloc<uint64_t>((size_t)256 * nb, st.stream);
+    }
+    for (const Digit& dg : plan) {
+        const int dst = cur == 0 ? 1 : 0;
+        if (!setk[dst]) alloc_arrays(st, st.W, n, &setk[dst], &seta[dst]);
+        {
+            SPQ_PROF(st.stream, "radix_hist_kernel", n * 8.0);
+            radix_hist_kernel<8, kPayloadBlock><<<nb, kPayloadBlock, 0, st.stream>>>(src_k + (uint64_t)dg.word * src_cap, n, dg, bh_rows, nb);
+        }
+        SPQ_HIP(hipGetLastError());
+        {
+            SPQ_PROF(st.stream, "transpose_u32_kernel", 256.0 * nb * 8.0);
+            const dim3 tg(256 / 32, (nb + 31) / 32);
+            transpose_u32_kernel<<<tg, kBlock, 0, st.stream>>>(bh_rows, nb, 256, bh);
+        }
+        SPQ_HIP(hipGetLastError());
+        exclusive_scan_u32(st, bh, offs, (uint64_t)256 * nb, nullptr);
+        SPQ_DISPATCH_W(st.W, launch_payload_pass, st, src_k, src_a, src_cap, dg, offs, nb, setk[dst], seta[dst], n);
+        cur = dst;
+        src_k = setk[dst];
+        src_a = seta[dst];
+        src_cap = n;
+    }
+    // local sort into the other set (or a fresh one)
+    const int out = cur == 0 ? 1 : 0;
+    if (!setk[out]) alloc_arrays(st, st.W, n, &setk[out], &seta[out]);
+    unsigned int* flag = B ? st.counter + 8 : nullptr;   // zero between kernels; cleared again below
+    SPQ_DISPATCH_W(st.W, launch_local, st, src_k, src_a, src_cap, la, grid, Q, setk[out], seta[out], n, flag);
+    bool ok = true;
+    if (B) {
+        unsigned int f = 0;
+        read_back(st, flag, sizeof(f), &f);
+        if (f) {
+            SPQ_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), st.stream));
+            ok = false;
+            std::lock_guard<std::mutex> lock(g_local_mu);
+            g_local_overflowed[sig] = 1;
+        }
+    }
+    if (bh) {
+        dfree(bh, st.stream);
+        dfree(bh_rows, st.stream);
+        dfree(offs, st.stream);
+    }
+    if (ok) {
+        install_arrays(st, st.W, setk[out], seta[out], n, n);
+        setk[out] = nullptr;
+        seta[out] = nullptr;
+    }
+    for (int i = 0; i < 2; ++i)
+        if (setk[i]) {
+            dfree(setk[i], st.stream);
+            dfree(seta[i], st.stream);
+        }
+    return ok;
 }
 
 }  // namespace sparq

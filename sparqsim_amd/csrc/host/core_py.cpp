@@ -533,6 +533,8 @@ PYBIND11_MODULE(_core, m) {
     });
     m.def("release_cached_memory", []() { check(sparq_release_cached_memory()); });
     m.def("_take_plugin_error", []() { return take_plugin_error(); });
+    m.def("set_sort_mode", [](int mode) { check(sparq_set_sort_mode(mode)); }, py::arg("mode"),
+          "Sort strategy: 0 prefix passes + LDS chunk sort where it applies (default), 1 LSD passes only.");
     m.def("set_radix_bits", [](int bits) { check(sparq_set_radix_bits(bits)); }, py::arg("bits"),
           "Digit width of the radix sort: 8 or 11 (forced) or 0 (autotuned, the default).");
 }
