@@ -196,19 +196,21 @@ def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
 
 
 def qda_sparse_leg(ps, nqubit, steps):
-    """QDA over the sparse block encoding (CKS oracles, 8 slots per row, encoded
-    kappa ~105): the first `steps` steps of the 2000-step schedule, plain and with
-    the learned sizer."""
+    """QDA over the sparse block encoding of the configuration's complex
+    Hermitian matrix (8 N(0,1) + i N(0,1) entries per row at random columns,
+    phase oracle, encoded kappa 100): the first `steps` steps of the 2000-step
+    schedule, plain and with the learned sizer."""
     from sparqsim_amd import qda_sparse, sizer
 
-    out = {"workload": f"QDA over a sparse block encoding (T^dag S T of the CKS walk) of a seeded 8-slot "
-                       f"band-plus-random matrix at 2^{nqubit}, steps 0..{steps - 1} of a 2000-step schedule"}
+    out = {"workload": f"QDA over a sparse block encoding (T^dag S P T of the CKS walk with the phase oracle) of a "
+                       f"seeded complex Hermitian matrix with 8 random-column slots per row at 2^{nqubit}, encoded "
+                       f"kappa 100, steps 0..{steps - 1} of a 2000-step schedule"}
     sizer.enable(False)
     for mode in ("plain", "learned"):
         if mode == "learned" and not sizer.activate_default():
             break
         s0 = sizer.stats()
-        met, st, *_ = qda_sparse.solve(ps, nqubit, 2000, stop_after=steps)
+        met, st, *_ = qda_sparse.solve(ps, nqubit, 2000, stop_after=steps, hermitian=True, kappa_encoded=100.0)
         out[mode] = {"ms_per_step": met["ms_per_step"], "entries": met["entries"], "fidelity": met["fidelity"]}
         if mode == "learned":
             s1 = sizer.stats()

@@ -498,10 +498,10 @@ def rhs_metrics(m: np.ndarray, b: np.ndarray, alpha: float, beta: float, xref: n
         y[:-1] += beta * x[1:]
         y[1:] += beta * x[:-1]
     y2 = float(np.sum(np.abs(y) ** 2))
-    proj = complex(np.dot(b, y))
+    proj = complex(np.vdot(b, y))
     res_dir = math.sqrt(max(2.0 - 2.0 * proj.real / math.sqrt(y2), 0.0))
     res_opt = math.sqrt(max(y2 - abs(proj) ** 2, 0.0))
-    f = complex(np.dot(xref, x))
+    f = complex(np.vdot(xref, x))
     return {"res_dir": res_dir, "res_opt": res_opt, "weight": w, "fidelity": abs(f) ** 2}
 
 
@@ -673,6 +673,129 @@ def band_random_matrix(n: int, kappa: float = 50.0, seed: int = 0x5EED2026 + 4, 
         order = np.argsort(cs)
         cols[i], valsq[i] = np.asarray(cs)[order], np.asarray(vs)[order]
     return valsq.reshape(-1).tolist(), cols.reshape(-1).tolist(), A
+
+
+def _symmetric_matchings(n: int, count: int, rng, avoid_band: bool):
+    """`count` perfect matchings of [0, n) drawn from `rng` that share no pair
+    with each other (nor with the +-1 band when avoid_band): each adds exactly
+    one symmetric partner per row.  Returns a list of (a, b) index arrays."""
+    taken = set()
+    out = []
+    for _ in range(count):
+        for _attempt in range(100):
+            perm = rng.permutation(n)
+            a, b = perm[0::2].copy(), perm[1::2].copy()
+            for _ in range(1000):   # re-pair partners that collide
+                lo, hi = np.minimum(a, b), np.maximum(a, b)
+                bad_mask = (hi - lo <= 1) if avoid_band else (hi == lo)
+                if taken:
+                    bad_mask |= np.isin(lo * n + hi, list(taken))
+                bad = np.nonzero(bad_mask)[0]
+                if bad.size == 0:
+                    break
+                for i in bad:
+                    j = int(rng.integers(0, a.size))
+                    b[i], b[j] = b[j], b[i]
+            else:
+                continue
+            break
+        else:
+            raise RuntimeError("could not draw a disjoint matching")
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        taken.update((lo * n + hi).tolist())
+        out.append((a, b))
+    return out
+
+
+def hermitian_random_matrix(n: int, kappa: float = 100.0, seed: int = 0x5EED2026 + 3, nnz_col: int = 8):
+    """Complex Hermitian positive definite matrix with nnz_col slots per row at
+    uniform random columns: the diagonal plus nnz_col - 1 seeded perfect
+    matchings (pattern symmetric by construction), off-diagonal values
+    N(0,1) + i N(0,1) with A_ji = conj(A_ij), a real N(0,1) diagonal shifted so
+    that lambda_max / lambda_min = kappa (Lanczos extremes), scaled to a largest
+    |entry| of 1.  Returns (|values|[n*nnz], sparsity[n*nnz] ascending per row,
+    phases[n*nnz] = arg(value), scipy CSR with .eig_bounds)."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import eigsh
+
+    if n < 4 * nnz_col or n & (n - 1):
+        raise ValueError("need a power-of-two dimension >= 4*nnz_col")
+    rng = np.random.default_rng(seed)
+    rows_l, cols_l, vals_l = [np.arange(n)], [np.arange(n)], [rng.standard_normal(n).astype(np.complex128)]
+    for a, b in _symmetric_matchings(n, nnz_col - 1, rng, avoid_band=False):
+        z = rng.standard_normal(a.size) + 1j * rng.standard_normal(a.size)
+        rows_l += [a, b]
+        cols_l += [b, a]
+        vals_l += [z, np.conj(z)]
+    B = sp.csr_matrix((np.concatenate(vals_l), (np.concatenate(rows_l), np.concatenate(cols_l))), shape=(n, n))
+    v0 = np.random.default_rng(seed + 1).standard_normal(n)
+    lmax = float(eigsh(B, k=1, which="LA", return_eigenvectors=False, tol=1e-7, v0=v0)[0])
+    lmin = float(eigsh(B, k=1, which="SA", return_eigenvectors=False, tol=1e-7, v0=v0)[0])
+    mu = (lmax - kappa * lmin) / (kappa - 1.0)
+    A = (B + mu * sp.identity(n, format="csr", dtype=np.complex128)).tocsr()
+    scale = float(np.max(np.abs(A.data)))
+    A = A / scale
+    A.eig_bounds = ((lmin + mu) / scale, (lmax + mu) / scale)   # Lanczos estimates (from inside)
+    A.sort_indices()
+    counts = np.diff(A.indptr)
+    if np.any(counts != nnz_col):
+        raise RuntimeError("pattern lost a slot (a zero entry)")
+    take = A.indptr[:-1, None] + np.arange(nnz_col)
+    cols = A.indices[take]
+    vals = A.data[take]
+    return np.abs(vals).reshape(-1).tolist(), cols.reshape(-1).tolist(), np.angle(vals).reshape(-1), A
+
+
+class PhaseOracle:
+    """|j>|k> -> exp(-i arg A_jk) |j>|k> on the walk's row / column registers, the
+    factor that turns the sign-blind walk product T^dag S T (which encodes |A_ij|)
+    into a block encoding of a complex Hermitian A: between T and S it puts
+    exp(i arg A_ij) on the <i|..|j> element.  No counterpart in the reference,
+    whose walk encodes |A| only (SparQ_Algorithm/include/hamiltonian_simulation.h:
+    1357-1365).  The phase of each stored slot is a 32-bit fraction of a turn in
+    a second QRAM table addressed like the element table (row * nnz + slot);
+    the column is converted to its slot and back with SparseMatrixOracle2, the
+    phase word is loaded, applied as 32 Phase_Bool factors exp(2 pi i 2^b / 2^32)
+    on its set bits, and unloaded."""
+
+    def __init__(self, mod, walk: "QuantumWalkNSteps", phases: np.ndarray):
+        self.mod, self.walk = mod, walk
+        n, nnz = walk.n_row, walk.nnz_col
+        size = n * nnz
+        self.addr_bits = max(1, int(math.ceil(math.log2(size))))
+        turns = np.mod(-np.asarray(phases, dtype=np.float64) / (2 * math.pi), 1.0)
+        words = np.floor(turns * 2.0 ** 32).astype(np.uint64) & np.uint64(0xFFFFFFFF)
+        mem = np.zeros(1 << self.addr_bits, dtype=np.uint64)
+        mem[:size] = words
+        self.words = words
+        self.qram = mod.QRAMCircuit_qutrit(self.addr_bits, 32, [int(x) for x in mem])
+
+    def _slot_op(self, st, to_slot: bool, cond):
+        w = self.walk
+        op = SparseMatrixOracle2(self.mod, w.qram, w.sparse_offset, w.j, w.k, w.k_comp, w.nnz_col)
+        if to_slot:
+            op(st, cond)
+        else:
+            op.dag(st, cond)
+
+    def apply(self, st, cond=None, sign: float = 1.0):
+        m, w = self.mod, self.walk
+        c = lambda op: _ctl(op, cond)   # noqa: E731
+        self._slot_op(st, True, cond)
+        m.AddRegister("phase_addr", m.UnsignedInteger, self.addr_bits)(st)
+        c(m.GetDataAddr(w.data_offset, w.j, w.k, w.nnz_col, "phase_addr"))(st)
+        m.AddRegister("phase_turn", m.UnsignedInteger, 32)(st)
+        c(m.QRAMLoad(self.qram, "phase_addr", "phase_turn"))(st)
+        for b in range(32):
+            c(m.Phase_Bool("phase_turn", b, sign * 2.0 * math.pi * 2.0 ** (b - 32)))(st)
+        c(m.QRAMLoad(self.qram, "phase_addr", "phase_turn"))(st)
+        m.RemoveRegister("phase_turn")(st)
+        c(m.GetDataAddr(w.data_offset, w.j, w.k, w.nnz_col, "phase_addr"))(st)
+        m.RemoveRegister("phase_addr")(st)
+        self._slot_op(st, False, cond)
+
+    def apply_dag(self, st, cond=None):
+        self.apply(st, cond, sign=-1.0)
 
 
 def walk_spectrum(A, ds: int, nnz_col: int = 4, widen: float = 1e-5):

@@ -26,21 +26,34 @@ from . import cks, qda
 
 
 class BlockEncodingSparse:
-    """U_A = T^dag S T over the CKS walk registers (self-adjoint: S is a swap)."""
+    """U_A = T^dag S T over the CKS walk registers (self-adjoint: S is a swap),
+    or U_A = T^dag S P T with the phase oracle P of a complex Hermitian matrix
+    (then U_A^dag = T^dag P^dag S T is applied for the adjoint)."""
 
-    def __init__(self, mod, walk: cks.QuantumWalkNSteps):
-        self.mod, self.walk = mod, walk
+    def __init__(self, mod, walk: cks.QuantumWalkNSteps, phase: cks.PhaseOracle | None = None):
+        self.mod, self.walk, self.phase = mod, walk, phase
         self.ancillas = [walk.b1, walk.k, walk.b2, walk.j_comp, walk.k_comp, walk.data_offset]
 
-    def apply(self, st, cond: qda.Cond | None = None):
+    def _swaps(self, st, cond):
         w, m = self.walk, self.mod
-        t = w._T()
-        t(st, cond)
         for a, b in ((w.j, w.k), (w.b1, w.b2), (w.j_comp, w.k_comp)):
             qda.ctl(m.Swap_General_General(a, b), cond)(st)
+
+    def apply(self, st, cond: qda.Cond | None = None):
+        t = self.walk._T()
+        t(st, cond)
+        if self.phase is not None:
+            self.phase.apply(st, cond)
+        self._swaps(st, cond)
         t.dag(st, cond)
 
-    apply_dag = apply
+    def apply_dag(self, st, cond: qda.Cond | None = None):
+        t = self.walk._T()
+        t(st, cond)
+        self._swaps(st, cond)
+        if self.phase is not None:
+            self.phase.apply_dag(st, cond)
+        t.dag(st, cond)
 
 
 class WalkSSparse:
@@ -62,12 +75,39 @@ class WalkSSparse:
         m.GlobalPhase(complex(0, 1))(st)
 
 
-def setup(mod, nqubit: int, kappa_A: float = 30.0, nnz_col: int = 8, ds: int = 32, seed: int = 0x5EED2026 + 3):
+def hermitian_for_encoded_kappa(n: int, kappa_encoded: float, nnz_col: int = 8, ds: int = 32, seed: int = 0x5EED2026 + 3):
+    """cks.hermitian_random_matrix with kappa(A) chosen so that the matrix the walk
+    block-encodes, A / (nnz_col Amax), has 1 / lambda_min = kappa_encoded (the
+    condition number QDA's schedule runs on); a few secant-free rescalings of
+    kappa(A), since kappa_encoded is nearly proportional to it."""
+    kappa_A = kappa_encoded / 4.0
+    for _ in range(6):
+        values, sparsity, phases, A = cks.hermitian_random_matrix(n, kappa_A, seed, nnz_col)
+        ke = 1.0 / cks.walk_spectrum(A, ds, nnz_col)[0]
+        if abs(ke / kappa_encoded - 1.0) < 0.01:
+            break
+        kappa_A *= kappa_encoded / ke
+    A.kappa_A = kappa_A
+    return values, sparsity, phases, A
+
+
+def setup(mod, nqubit: int, kappa_A: float = 30.0, nnz_col: int = 8, ds: int = 32, seed: int = 0x5EED2026 + 3,
+          hermitian: bool = False, kappa_encoded: float | None = None):
     """Registers, the matrix and |b> for the sparse QDA: returns (state, walk, enc_A,
-    scipy matrix, kappa of the block-encoded matrix)."""
+    scipy matrix, kappa of the block-encoded matrix).  hermitian: the complex
+    Hermitian matrix of cks.hermitian_random_matrix (|A| through the walk, the
+    phases through cks.PhaseOracle), with kappa(A) = kappa_A or, when
+    kappa_encoded is given, chosen for that encoded condition number; otherwise
+    the real band-plus-random one."""
     mod.System.clear()
     n = 1 << nqubit
-    values, sparsity, A = cks.band_random_matrix(n, kappa_A, seed, nnz_col)
+    phases = None
+    if hermitian and kappa_encoded:
+        values, sparsity, phases, A = hermitian_for_encoded_kappa(n, kappa_encoded, nnz_col, ds, seed)
+    elif hermitian:
+        values, sparsity, phases, A = cks.hermitian_random_matrix(n, kappa_A, seed, nnz_col)
+    else:
+        values, sparsity, A = cks.band_random_matrix(n, kappa_A, seed, nnz_col)
     mat = mod.SparseMatrix(values, sparsity, ds, nnz_col, n, True)
     walk = cks.QuantumWalkNSteps(mod, mat, None)
     walk.InitEnvironment()
@@ -77,7 +117,7 @@ def setup(mod, nqubit: int, kappa_A: float = 30.0, nnz_col: int = 8, ds: int = 3
     mod.Hadamard_Int(walk.j, nqubit)(st)
     lo, hi = cks.walk_spectrum(A, ds, nnz_col)
     kappa_enc = 1.0 / lo          # eigenvalues of A/(nnz Amax) lie in [lo, hi] with hi < 1
-    enc = BlockEncodingSparse(mod, walk)
+    enc = BlockEncodingSparse(mod, walk, cks.PhaseOracle(mod, walk, phases) if hermitian else None)
     enc.matrix = mat
     return st, walk, enc, A, kappa_enc
 
@@ -100,12 +140,13 @@ def solution_metrics(blk: np.ndarray, A, xref: np.ndarray) -> dict:
 
 
 def solve(mod, nqubit: int = 16, steps: int = 2000, p: float = 1.3, kappa_A: float = 30.0, nnz_col: int = 8,
-          cz_eps: float = 1e-14, log_every: int = 0, log=print, stop_after: int | None = None):
+          cz_eps: float = 1e-14, log_every: int = 0, log=print, stop_after: int | None = None, hermitian: bool = False,
+          kappa_encoded: float | None = None):
     """The QDA schedule W(s = i/steps), ClearZero(cz_eps), Normalize per step
     (as PoiseuilleQDABench.cu) with the sparse block encoding; returns metrics."""
     import time
 
-    st, walk, enc, A, kappa = setup(mod, nqubit, kappa_A, nnz_col)
+    st, walk, enc, A, kappa = setup(mod, nqubit, kappa_A, nnz_col, hermitian=hermitian, kappa_encoded=kappa_encoded)
     n = 1 << nqubit
     xref = cks._spd_solve(A, np.full(n, 1.0 / math.sqrt(n)))
     xref /= np.linalg.norm(xref)
@@ -134,7 +175,8 @@ def solve(mod, nqubit: int = 16, steps: int = 2000, p: float = 1.3, kappa_A: flo
         mod.set_clearzero_default_eps(old)
     met = solution_metrics(blk, A, xref)
     met.update(steps=steps, steps_run=run, wall_s=wall, ms_per_step=wall / max(run, 1) * 1e3, entries=size,
-               peak_logged_entries=max(peak, size), kappa_encoded=kappa, n=n, block_probability=psel)
+               peak_logged_entries=max(peak, size), kappa_encoded=kappa, n=n, block_probability=psel,
+               kappa_A=float(getattr(A, "kappa_A", kappa_A)), hermitian=hermitian)
     return met, st, walk, blk
 
 
@@ -151,6 +193,10 @@ def main():
     ap.add_argument("--cz-eps", type=float, default=1e-14)
     ap.add_argument("--log-every", type=int, default=0)
     ap.add_argument("--learned", action="store_true", help="size branching operators with the learned sizer")
+    ap.add_argument("--hermitian", action="store_true",
+                    help="complex Hermitian matrix (N(0,1) + i N(0,1) entries at random columns) through the phase oracle")
+    ap.add_argument("--kappa-encoded", type=float, default=None,
+                    help="with --hermitian: pick kappa(A) so that the block-encoded matrix has this condition number")
     args = ap.parse_args()
     import sparqsim_amd as ps
     from . import sizer
@@ -158,7 +204,8 @@ def main():
     if args.learned and not sizer.activate_default():
         raise SystemExit("no sizer weights found")
     met, *_ = solve(ps, args.nqubit, args.steps, args.p, args.kappa, 8, args.cz_eps, args.log_every,
-                    log=lambda s: print(s, flush=True), stop_after=args.stop_after)
+                    log=lambda s: print(s, flush=True), stop_after=args.stop_after, hermitian=args.hermitian,
+                    kappa_encoded=args.kappa_encoded)
     if args.learned:
         met["sizer"] = sizer.stats()
     print(json.dumps(met), flush=True)
