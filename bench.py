@@ -29,22 +29,11 @@ PMC_TRAFFIC_MIB_2P20 = {
 }
 
 
-class ProfRecord(ctypes.Structure):
-    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64),
-                ("total_ms", ctypes.c_double), ("total_bytes", ctypes.c_double)]
-
-
 def kernel_profile(lib):
-    cap = 128
-    arr = (ProfRecord * cap)()
-    cnt = ctypes.c_uint32(0)
-    if lib.sparq_prof_report(arr, cap, ctypes.byref(cnt)) != 0:
-        raise RuntimeError("sparq_prof_report failed")
-    out = []
-    for i in range(min(cnt.value, cap)):
-        r = arr[i]
-        out.append({"name": r.name.decode(), "launches": int(r.launches), "ms": r.total_ms, "bytes": r.total_bytes})
-    return sorted(out, key=lambda r: -r["ms"])
+    """Per-kernel records of the engine's profiler (launches, ms, algorithmic bytes)."""
+    from sparqsim_amd import profiling
+
+    return profiling.report()
 
 
 def cpu_baseline(address_bits, data_bits, mem_words, budget_s=10.0, module="_oracle"):
@@ -256,11 +245,13 @@ def sort_leg(ps, lib, address_bits, reps, mode=0):
     for _ in range(3):
         once()
     lib.sparq_prof_reset()
+    ps.sort_stats(reset=True)
     lib.sparq_prof_enable(1)
     for _ in range(reps):
         once()
     lib.sparq_prof_enable(0)
     prof = kernel_profile(lib)
+    paths = ps.sort_stats(reset=True)
     ps.set_sort_mode(0)
     rows = [r for r in prof if r["name"] in SORT_KERNELS and r["ms"] > 0]
     ms = sum(r["ms"] for r in rows)
@@ -270,7 +261,7 @@ def sort_leg(ps, lib, address_bits, reps, mode=0):
     return {
         "workload": f"SortUnconditional of 2^{address_bits} entries, 48-bit keys scrambled by a QRAM load before each "
                     f"sort, {reps} sorts, " + ("prefix passes + LDS chunk sort" if mode == 0 else "LSD passes + gather"),
-        "entries": n, "ms_per_sort": ms / reps, "entries_per_s": n * reps / (ms * 1e-3),
+        "entries": n, "sort_paths": paths, "ms_per_sort": ms / reps, "entries_per_s": n * reps / (ms * 1e-3),
         "GBs": byt / (ms * 1e-3) / 1e9, "frac_of_peak": byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "kernels": [{"name": r["name"], "launches_per_sort": r["launches"] / reps, "us_per_launch": r["ms"] / r["launches"] * 1e3,
                      "GBs": r["bytes"] / (r["ms"] * 1e-3) / 1e9,

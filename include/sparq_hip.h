@@ -355,12 +355,21 @@ int sparq_select(sparq_state_t st, const sparq_cond_t* conds, uint32_t nconds, d
  * first sorts of each class with both widths. */
 int sparq_set_radix_bits(int bits);
 /* Sort strategy (no counterpart in the reference): 0 (the default) moves whole
- * entries by the top 8-24 ordering bits in a few LSD passes and then sorts
- * chunks of <= 4096 entries in LDS (keys of <= 5 words, <= 128 ordering bits
- * below the prefix; anything else, or a bucket too large for one chunk, takes
- * the LSD path); 1 always takes the LSD radix passes over an index payload and
- * a final gather.  Both give the same stable order. */
+ * entries by the top 8-32 ordering bits in a few LSD passes and then sorts
+ * chunks of <= 2048 entries in LDS (keys of <= 5 words, <= 128 ordering bits
+ * below the prefix).  A bucket too large for one chunk makes the engine retry
+ * with 8 more prefix bits (remembered per ordering and size class); anything
+ * that still does not fit takes the LSD path.  1 always takes the LSD radix
+ * passes over an index payload and a final gather.  Both give the same stable
+ * order. */
 int sparq_set_sort_mode(int mode);
+/* Sort-path counters since the last reset (no counterpart in the reference):
+ * out[0] sorts by the prefix + local path, out[1] their entries, out[2] local
+ * attempts that overflowed a chunk, out[3] sorts by the LSD passes, out[4]
+ * their entries, out[5] local sorts that needed no prefix pass (the state was
+ * already sorted by a long enough key prefix).  count >= 6; reset != 0 clears
+ * them after reading. */
+int sparq_sort_stats(uint64_t* out, uint32_t count, int reset);
 
 /* Learned output sizer (no counterpart in the reference, which always counts
  * before it allocates).  A 4-layer FP32 MLP, 16 features -> 64 -> 64 -> 64 -> 1

@@ -18,6 +18,7 @@ the quantised walk spectrum (--coef true, J terms) or the binomial-tail family
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import time
 
@@ -135,6 +136,9 @@ def main():
     ap.add_argument("--log-every", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--json", default=None, help="write the summary (with the curve) to this file")
+    ap.add_argument("--kernel-profile", default=None,
+                    help="write the engine's per-kernel profile (launches, time, algorithmic bytes) and the sort-path "
+                         "counters of the whole run to this JSON file")
     ap.add_argument("--outdir", default=None,
                     help="write residual_curve.csv and qda_summary_cpp.json here (the reference bench's artifacts, "
                          "right-hand side 0)")
@@ -151,9 +155,15 @@ def main():
     args = ap.parse_args()
     import sparqsim_amd as ps
 
-    res = solve(ps, args.nqubit, args.rhs, args.kappa, args.eps, args.ds, args.coef, args.J, args.terms, args.log_every,
-                args.tol, log=lambda s: print(s, flush=True), matrix=args.matrix, fit_top=args.fit_top,
-                clearzero_eps=args.clearzero_eps, merge_eps=args.merge_eps)
+    from .profiling import KernelProfile
+
+    with KernelProfile(ps) if args.kernel_profile else contextlib.nullcontext() as prof:
+        res = solve(ps, args.nqubit, args.rhs, args.kappa, args.eps, args.ds, args.coef, args.J, args.terms, args.log_every,
+                    args.tol, log=lambda s: print(s, flush=True), matrix=args.matrix, fit_top=args.fit_top,
+                    clearzero_eps=args.clearzero_eps, merge_eps=args.merge_eps)
+    if args.kernel_profile:
+        prof.write(args.kernel_profile, terms=res["curve"][-1]["term"] if res["curve"] else None,
+                   entries_after_first_step=res.get("entries_after_first_step"))
     summary = {k: v for k, v in res.items() if k not in ("curve", "final")}
     summary["final_worst_res_dir"] = max(r["res_dir"] for r in res["final"])
     summary["final_rhs0"] = res["final"][0]

@@ -39,6 +39,10 @@ void sparq::ensure_device() {
         throw Error(SPARQ_NO_DEVICE, "no HIP device available");
     }
     SPQ_HIP(hipSetDevice(0));
+    // Every branching operator waits for a few words of its result; spinning
+    // wakes the host sooner than a blocking wait.  A process that created the
+    // device's context already keeps its own setting.
+    if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
     g_device = 0;
 }
 
@@ -366,6 +370,13 @@ int sparq_set_sort_mode(int mode) {
     return guarded([&] {
         SPQ_REQUIRE(mode == 0 || mode == 1, "sort mode must be 0 (prefix + local) or 1 (LSD passes only)");
         set_sort_mode(mode);
+    });
+}
+
+int sparq_sort_stats(uint64_t* out, uint32_t count, int reset) {
+    return guarded([&] {
+        SPQ_REQUIRE(out && count >= 6, "sparq_sort_stats needs room for 6 counters");
+        sort_path_stats(out, reset);
     });
 }
 

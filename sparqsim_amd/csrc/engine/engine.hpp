@@ -140,6 +140,11 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits = 8,
 void set_radix_bits(int bits);
 // 0: prefix passes + LDS-local chunk sort where it applies, 1: LSD passes only.
 void set_sort_mode(int mode);
+// out[6]: sorts by the prefix + local path, their entries, local attempts that
+// overflowed a chunk, sorts by the LSD passes, their entries, and the local
+// sorts that ran no prefix pass (the state was already sorted by the prefix);
+// reset clears them.
+void sort_path_stats(uint64_t* out, int reset);
 // One pass over the keys: bits that vary across the state, and whether the
 // state is already ordered with the optional field `last` least significant.
 // With a SizerQuery the same pass also histograms popcount(value of `last`)
@@ -148,8 +153,9 @@ void set_sort_mode(int mode);
 constexpr int kStatsHistBins = 6;
 // Key-statistics record: [0, W) OR of the keys, [W, 2W) AND, [2W] order
 // violations of (others, last), [2W+1] violations of `last` alone being
-// non-decreasing, [2W+2, +kStatsHistBins) popcount histogram of `last`.
-constexpr int stats_words(int W) { return 2 * W + 2 + kStatsHistBins; }
+// non-decreasing, [2W+2, 3W+2) per word the first differing bits of the pairs
+// out of order, [3W+2, +kStatsHistBins) popcount histogram of `last`.
+constexpr int stats_words(int W) { return 3 * W + 2 + kStatsHistBins; }
 struct SizerQuery {
     uint64_t n;
     uint32_t words, nd, kind;          // kind: 0 Hadamard_Int, 1 Hadamard_Int_Full, 2 dense group unitary
@@ -159,6 +165,9 @@ struct KeyStats {
     std::vector<uint64_t> vary;
     bool sorted = true;
     bool last_sorted = true;           // the `last` field alone is non-decreasing
+    // per word, the first differing bits of adjacent pairs out of (others, last)
+    // order: the state is sorted by the bits above the most significant one
+    std::vector<uint64_t> disorder;
     bool have_features = false, have_prediction = false;
     float features[SPARQ_SIZER_FEATURES] = {};
     float prediction = 0.f;            // network output: log2(G) / 32
@@ -236,6 +245,9 @@ bool field_uniform(State& st, sparq_field_t f, uint64_t* value);
 bool prof_enabled();
 void prof_begin(hipStream_t s, const char* name, double bytes);
 void prof_end(hipStream_t s);
+// A host-side wait (a readback's copy + stream synchronisation), recorded under
+// a name starting with "host:" with its count and blocked milliseconds.
+void prof_host_wait(const char* name, double ms);
 struct ProfScope {
     bool on;
     hipStream_t s;

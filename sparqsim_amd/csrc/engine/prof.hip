@@ -62,6 +62,13 @@ void prof_begin(hipStream_t s, const char* name, double bytes) {
     g_pending.push_back(p);
 }
 
+void prof_host_wait(const char* name, double ms) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    Acc& a = g_acc[name];
+    a.launches += 1;
+    a.ms += ms;
+}
+
 void prof_end(hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_pending.empty()) SPQ_HIP(hipEventRecord(g_pending.back().ev1, s));

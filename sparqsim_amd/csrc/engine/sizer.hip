@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(64) sizer_feature_kernel(const uint64_t* __res
         x[7] = q.kind == 0 ? 1.f : 0.f;
         x[8] = q.kind == 1 ? 1.f : 0.f;
         x[9] = q.kind == 2 ? 1.f : 0.f;
-        for (int b = 0; b < kStatsHistBins; ++b) x[10 + b] = (float)stats[2 * W + 2 + b] / (float)q.n;
+        for (int b = 0; b < kStatsHistBins; ++b) x[10 + b] = (float)stats[3 * W + 2 + b] / (float)q.n;
     }
     __syncthreads();
     if (j < kF) out[j] = x[j];

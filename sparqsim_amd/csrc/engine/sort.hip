@@ -6,6 +6,7 @@
 // moves W key words and the 16-byte amplitude.  Only bits that vary across
 // the state are sorted (an OR/AND reduction finds them first).
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -39,9 +40,12 @@ __device__ __forceinline__ uint64_t match_any(uint32_t d, int nbits) {
 
 // ---------------------------------------------------------------- key statistics
 // One read pass over the keys: bitwise OR and AND per word (bits that vary
-// across the state) and whether the state is already ordered by
-// (all bits except `last`, then `last`).  Per-block partial records are
-// combined by a one-block second kernel.
+// across the state), whether the state is already ordered by
+// (all bits except `last`, then `last`), and for every adjacent pair out of
+// that order the most significant bit where the pair differs (one mask per
+// key word): the bits above the most significant such bit are a prefix the
+// state is already sorted by.  Per-block partial records are combined by a
+// one-block second kernel.
 struct StatsOrder {
     uint64_t others[SPARQ_MAX_WORDS];   // per-word mask of the primary comparison bits
     sparq_field_t last;                 // width 0: none
@@ -58,13 +62,28 @@ __device__ __forceinline__ bool out_of_order(const uint64_t (&p)[W], const uint6
     return false;
 }
 
+// The first bit (in the order of StatsOrder) where two keys differ, as a
+// one-bit mask in its word's slot of m: the most significant differing bit of
+// the first word that differs outside `last`; nothing when they differ only in
+// `last`.
+template <int W>
+__device__ __forceinline__ void first_difference(const uint64_t (&p)[W], const uint64_t (&c)[W], const StatsOrder& so, uint64_t (&m)[W]) {
+    bool found = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const uint64_t x = (p[w] ^ c[w]) & so.others[w];
+        if (!found && x) m[w] |= 1ull << (63 - __clzll(x));
+        found = found || x != 0;
+    }
+}
+
 // Record layout: stats_words(W) (engine.hpp); the histogram of `hist` is zero
 // without one.
 template <int W>
 __device__ __forceinline__ uint64_t stats_combine(int j, uint64_t x, uint64_t y) {
     if (j < W) return x | y;
     if (j < 2 * W) return x & y;
-    if (j <= 2 * W + 1) return x | y;
+    if (j < 3 * W + 2) return x | y;
     return x + y;
 }
 
@@ -96,12 +115,21 @@ __global__ void __launch_bounds__(kBlock) key_stats_kernel(const uint64_t* __res
             const uint64_t i = t * kBlock * kI + (uint64_t)j * kBlock + threadIdx.x;
 #pragma unroll
             for (int w = 0; w < W; ++w) { r[w] |= k[j][w]; r[W + w] &= k[j][w]; }
-            r[2 * W] |= out_of_order<W>(p[j], k[j], so) ? 1 : 0;
+            const bool bad = out_of_order<W>(p[j], k[j], so);
+            r[2 * W] |= bad ? 1 : 0;
             if (so.last.width) r[2 * W + 1] |= (fget<W>(p[j], so.last) > fget<W>(k[j], so.last)) ? 1 : 0;
+            if (bad) {
+                uint64_t m[W];
+#pragma unroll
+                for (int w = 0; w < W; ++w) m[w] = 0;
+                first_difference<W>(p[j], k[j], so, m);
+#pragma unroll
+                for (int w = 0; w < W; ++w) r[2 * W + 2 + w] |= m[w];
+            }
             if (hist.width && i < n) {
                 const int b = min(__popcll(fget<W>(k[j], hist)), kStatsHistBins - 1);
 #pragma unroll
-                for (int h = 0; h < kStatsHistBins; ++h) r[2 * W + 2 + h] += (h == b);
+                for (int h = 0; h < kStatsHistBins; ++h) r[3 * W + 2 + h] += (h == b);
             }
         }
     }
@@ -196,6 +224,7 @@ KeyStats key_stats(State& st, const sparq_field_t* last, const SizerQuery* q) {
     for (uint32_t w = 0; w < st.W; ++w) ks.vary[w] = h[w] ^ h[st.W + w];
     ks.sorted = h[2 * st.W] == 0;
     ks.last_sorted = last != nullptr && h[2 * st.W + 1] == 0;
+    ks.disorder.assign(h.begin() + 2 * st.W + 2, h.begin() + 3 * st.W + 2);
     if (sized) {
         const float* f = reinterpret_cast<const float*>(h.data() + R);
         std::copy(f, f + SPARQ_SIZER_FEATURES, ks.features);
@@ -337,21 +366,50 @@ static void sort_with_plans(State& st, const std::vector<Digit>& p8, const std::
 // Sort by an ordering given as its varying bit positions in LSD order: the
 // prefix + local path when it applies (sort_local), else the LSD radix passes
 // with the autotuned digit width.
-static bool sort_local(State& st, const std::vector<std::pair<int, int>>& bits);
+static bool sort_local(State& st, const std::vector<std::pair<int, int>>& bits, uint32_t presorted);
 static int g_sort_mode = 0;   // 0: prefix + local sort where it applies, 1: LSD passes only
+// sorts by path and the entries each path sorted (sort_path_stats)
+enum { kStatLocal, kStatLocalEntries, kStatOverflow, kStatLsd, kStatLsdEntries, kStatPresorted, kStatCount };
+static std::atomic<uint64_t> g_sort_stats[kStatCount];
 
 void set_sort_mode(int mode) { g_sort_mode = mode; }
 
-static void sort_ordering(State& st, const std::vector<std::pair<int, int>>& bits) {
+// `presorted`: the top that many bits of the ordering are a prefix the state is
+// already sorted by (0: unknown).
+static void sort_ordering(State& st, const std::vector<std::pair<int, int>>& bits, uint32_t presorted = 0) {
     if (bits.empty() || st.n <= 1) return;
-    if (g_sort_mode == 0 && sort_local(st, bits)) return;
+    if (g_sort_mode == 0 && sort_local(st, bits, presorted)) return;
+    g_sort_stats[kStatLsd] += 1;
+    g_sort_stats[kStatLsdEntries] += st.n;
     sort_with_plans(st, digits_of(bits, 8), digits_of(bits, 11));
+}
+
+// How many of the top ordering bits (LSD list `bits` of the ordering
+// (others, last)) lie above the most significant first-difference of the
+// pairs out of order: the state is already sorted by them.
+static uint32_t presorted_top(const std::vector<std::pair<int, int>>& bits, const KeyStats& ks, const sparq_field_t* last) {
+    int wv = -1, pv = -1;
+    for (size_t w = 0; w < ks.disorder.size(); ++w)
+        if (ks.disorder[w]) {
+            wv = (int)w;
+            pv = 63 - __builtin_clzll(ks.disorder[w]);
+            break;
+        }
+    uint32_t count = 0;
+    for (size_t i = bits.size(); i-- > 0;) {
+        const int w = bits[i].first, p = bits[i].second;
+        if (last && w == (int)last->word && p >= (int)last->shift && p < (int)(last->shift + last->width)) break;
+        if (wv >= 0 && !(w < wv || (w == wv && p > pv))) break;
+        ++count;
+    }
+    return count;
 }
 
 void sort_with_stats(State& st, const KeyStats& ks, const sparq_field_t* last) {
     std::vector<sparq_field_t> minor;
     if (last) minor.push_back(*last);
-    sort_ordering(st, order_bits(ks.vary, st.W, {}, minor, last && ks.last_sorted));
+    const std::vector<std::pair<int, int>> bits = order_bits(ks.vary, st.W, {}, minor, last && ks.last_sorted);
+    sort_ordering(st, bits, presorted_top(bits, ks, last));
 }
 
 void sort_state_by(State& st, const std::vector<sparq_field_t>& major, const std::vector<sparq_field_t>& minor) {
@@ -782,11 +840,12 @@ struct LocalArgs {
 };
 
 template <int W>
-__device__ __forceinline__ uint32_t prefix_of(const uint64_t (&k)[W], const BitSegs& sg) {
-    uint32_t v = 0;
+__device__ __forceinline__ uint64_t prefix_of(const uint64_t (&k)[W], const BitSegs& sg) {
+    uint64_t v = 0;
     for (uint32_t i = 0; i < sg.n; ++i) {
-        const uint64_t bits = (kword<W>(k, sg.word[i]) >> sg.shift[i]) & fmask(sg.width[i]);
-        v = (v << sg.width[i]) | (uint32_t)bits;
+        const uint32_t w = sg.width[i];
+        const uint64_t bits = (kword<W>(k, sg.word[i]) >> sg.shift[i]) & fmask(w);
+        v = (w >= 64 ? 0 : (v << w)) | bits;
     }
     return v;
 }
@@ -805,14 +864,19 @@ __device__ __forceinline__ void compact_of(const uint64_t (&k)[W], const BitSegs
 }
 
 // First bucket start at or after q0 (bucket = run of equal prefix), or n.
-// Called by the whole workgroup; the result is uniform.
+// Called by the whole workgroup; the result is uniform.  The scan stops after
+// kLocalCap + 1 positions: finding no head there means entries q0 - 1 ..
+// q0 + kLocalCap share one prefix, a bucket longer than a chunk, and the
+// result is kNoHead (the caller reports an overflow).
+constexpr uint64_t kNoHead = ~0ull;
 template <int W>
 __device__ uint64_t chunk_start(uint64_t q0, const uint64_t* __restrict__ keys, uint64_t cap, const LocalArgs& la, uint32_t* lds_min) {
     if (q0 == 0) return 0;
-    for (uint64_t base = q0; base < la.n; base += kLocalBlock) {
+    const uint64_t limit = min(la.n, q0 + (uint64_t)kLocalCap + 1);
+    for (uint64_t base = q0; base < limit; base += kLocalBlock) {
         const uint64_t q = base + threadIdx.x;
         bool head = false;
-        if (q < la.n) {   // q >= 1: q0 >= 1
+        if (q < limit) {   // q >= 1: q0 >= 1
             uint64_t a[W], b[W];
             load_key<W>(keys, cap, q, a);
             load_key<W>(keys, cap, q - 1, b);
@@ -826,7 +890,7 @@ __device__ uint64_t chunk_start(uint64_t q0, const uint64_t* __restrict__ keys, 
         __syncthreads();
         if (m != ~0u) return base + m;
     }
-    return la.n;
+    return limit == la.n ? la.n : kNoHead;
 }
 
 template <int W, int Q>
@@ -847,12 +911,12 @@ __global__ void __launch_bounds__(kLocalBlock) local_sort_kernel(const uint64_t*
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t s = la.h ? chunk_start<W>((uint64_t)blockIdx.x * la.h, kin, cap_in, la, &misc[0]) : 0;
-    const uint64_t e = la.h ? chunk_start<W>((uint64_t)(blockIdx.x + 1) * la.h, kin, cap_in, la, &misc[0]) : la.n;
-    const uint64_t len = e - s;
-    if (len > (uint64_t)kLocalCap) {
+    const uint64_t e = la.h && s != kNoHead ? chunk_start<W>((uint64_t)(blockIdx.x + 1) * la.h, kin, cap_in, la, &misc[0]) : la.n;
+    if (s == kNoHead || e == kNoHead || e - s > (uint64_t)kLocalCap) {
         if (threadIdx.x == 0) atomicOr(overflow, 1u);
         return;
     }
+    const uint64_t len = e - s;
     if (len == 0) return;
 
     // ---- load: item it of (wave, lane) is chunk position p = wave*(cap/NW) + it*64 + lane
@@ -1166,11 +1230,16 @@ static bool segments_of(const std::vector<std::pair<int, int>>& lsd, BitSegs& sg
     return true;
 }
 
-// Orderings whose buckets overflowed a chunk at a size class: a walk repeats the
-// same sorts every step, so later sorts of the ordering go straight to the LSD
-// passes instead of running the prefix passes again only to fall back.
+// Retry level per ordering and size class: a walk repeats the same sorts
+// every step, so an ordering whose buckets overflowed a chunk starts the next
+// sort at the level that worked (kGiveUp: straight to the LSD passes).
 static std::mutex g_local_mu;
-static std::unordered_map<uint64_t, int> g_local_overflowed;
+static std::unordered_map<uint64_t, int> g_local_level;
+constexpr int kGiveUp = -1;
+
+void sort_path_stats(uint64_t* out, int reset) {
+    for (int i = 0; i < kStatCount; ++i) out[i] = reset ? g_sort_stats[i].exchange(0) : g_sort_stats[i].load();
+}
 
 static uint64_t ordering_signature(const std::vector<std::pair<int, int>>& bits, uint32_t W, uint64_t n) {
     uint64_t h = 0x9E3779B97F4A7C15ull ^ W;
@@ -1182,38 +1251,25 @@ static uint64_t ordering_signature(const std::vector<std::pair<int, int>>& bits,
     return h;
 }
 
-static bool sort_local(State& st, const std::vector<std::pair<int, int>>& bits) {
+// One attempt with a B-bit prefix: returns true when the sorted entries are
+// installed, false when a bucket overflowed a chunk (the state is untouched).
+// B equal to the number of ordering bits: the prefix passes sort the whole
+// ordering and no chunk sort follows.  presorted: the state is already sorted
+// by the B-bit prefix, so no prefix pass runs.
+static bool local_attempt(State& st, const std::vector<std::pair<int, int>>& bits, uint32_t B, uint32_t h, bool presorted = false) {
     const uint64_t n = st.n;
-    const uint32_t T = (uint32_t)bits.size();
-    if (st.W > (uint32_t)kLocalMaxW || n >= (1ull << 32)) return false;
-    const uint64_t sig = ordering_signature(bits, st.W, n);
-    {
-        std::lock_guard<std::mutex> lock(g_local_mu);
-        if (g_local_overflowed.count(sig)) return false;
-    }
-    // prefix bits B (multiple of 8): buckets of about n / 2^B <= 512 entries expected
-    uint32_t B = 0;
-    if (n > (uint64_t)kLocalCap) {
-        uint32_t need = 0;
-        while ((n >> need) > (uint64_t)kLocalCap / 8) ++need;
-        B = std::min<uint32_t>(T, (need + 7) / 8 * 8);
-        if (B > 24) return false;
-    }
-    if (B > 0 && (n >> B) > (uint64_t)kLocalCap / 8) return false;   // too few ordering bits for small buckets
-    const uint32_t R = T - B;
-    if (R > 128) return false;
+    const uint32_t R = (uint32_t)bits.size() - B;
     const int Q = R > 64 ? 2 : 1;
     const std::vector<std::pair<int, int>> low(bits.begin(), bits.begin() + R), top(bits.begin() + R, bits.end());
     LocalArgs la{};
     if (!segments_of(low, la.rem) || !segments_of(top, la.pre)) return false;
     la.rem_bits = R;
     la.n = n;
-    la.h = B ? (uint32_t)(kLocalCap * 3 / 4) : 0;   // chunk = h entries + the bucket straddling its end
+    la.h = B ? h : 0;   // chunk = h entries + the bucket straddling its end
     const unsigned grid = B ? grid_for(n, la.h) : 1;
-    if (!B && n > (uint64_t)kLocalCap) return false;
 
     // prefix passes: whole entries, ping-pong between two scratch sets
-    const std::vector<Digit> plan = digits_of(top, 8);
+    const std::vector<Digit> plan = presorted ? std::vector<Digit>{} : digits_of(top, 8);
     const uint64_t* src_k = st.keys;
     const double2* src_a = st.amps;
     uint64_t src_cap = st.cap;
@@ -1251,19 +1307,19 @@ static bool sort_local(State& st, const std::vector<std::pair<int, int>>& bits) 
         src_cap = n;
     }
     // local sort into the other set (or a fresh one)
-    const int out = cur == 0 ? 1 : 0;
+    const bool passes_only = R == 0 && cur >= 0;
+    const int out = passes_only ? cur : (cur == 0 ? 1 : 0);
     if (!setk[out]) alloc_arrays(st, st.W, n, &setk[out], &seta[out]);
     unsigned int* flag = B ? st.counter + 8 : nullptr;   // zero between kernels; cleared again below
-    SPQ_DISPATCH_W(st.W, launch_local, st, src_k, src_a, src_cap, la, grid, Q, setk[out], seta[out], n, flag);
+    if (!passes_only)
+        SPQ_DISPATCH_W(st.W, launch_local, st, src_k, src_a, src_cap, la, grid, Q, setk[out], seta[out], n, flag);
     bool ok = true;
-    if (B) {
+    if (B && !passes_only) {
         unsigned int f = 0;
         read_back(st, flag, sizeof(f), &f);
         if (f) {
             SPQ_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), st.stream));
             ok = false;
-            std::lock_guard<std::mutex> lock(g_local_mu);
-            g_local_overflowed[sig] = 1;
         }
     }
     if (bh) {
@@ -1282,6 +1338,90 @@ static bool sort_local(State& st, const std::vector<std::pair<int, int>>& bits) 
             dfree(seta[i], st.stream);
         }
     return ok;
+}
+
+// The prefix + local path.  Level 0: B = the smallest multiple of 8 ordering
+// bits that leaves about n / 2^B <= kLocalCap / 8 entries per bucket, chunk
+// spacing h = 3/4 of a chunk (a chunk holds h entries plus the bucket that
+// straddles its end, so buckets of up to kLocalCap / 4 always fit).  Each level
+// after an overflow adds 8 prefix bits, up to 32 (the prefix is compared as one
+// 32-bit word) or all ordering bits; past that it lowers the spacing to 1/2,
+// then 1/4 of a chunk (buckets up to 1/2, then 3/4 of a chunk always fit).
+// An ordering of at most 32 bits that the prefix covers, or leaves at most 8
+// bits of, is sorted by the prefix passes alone: at most four whole-entry
+// passes cost less than index passes plus the random gather, and one pass
+// less than a chunk sort.
+static bool level_shape(uint32_t base, uint32_t max_b, int level, uint32_t& B, uint32_t& h) {
+    B = base;
+    h = kLocalCap * 3 / 4;
+    for (int l = 0; l < level; ++l) {
+        if (B < max_b)
+            B = std::min<uint32_t>(B + 8, max_b);
+        else if (h > (uint32_t)kLocalCap / 4)
+            h = h == (uint32_t)kLocalCap * 3 / 4 ? kLocalCap / 2 : kLocalCap / 4;
+        else
+            return false;
+    }
+    return true;
+}
+
+static bool sort_local(State& st, const std::vector<std::pair<int, int>>& bits, uint32_t presorted) {
+    const uint64_t n = st.n;
+    const uint32_t T = (uint32_t)bits.size();
+    if (st.W > (uint32_t)kLocalMaxW || n >= (1ull << 32)) return false;
+    if (n > (uint64_t)kLocalCap && presorted > 0) {
+        // Already sorted by a prefix at least as long as the passes would
+        // make (up to 64 bits of it serve as the bucket prefix): the chunk
+        // sort alone.  Buckets too large for it fall through to the passes.
+        uint32_t need = 0;
+        while ((n >> need) > (uint64_t)kLocalCap / 8) ++need;
+        const uint32_t B = std::min<uint32_t>(presorted, 64);
+        if (B >= need && B < T && T - B <= 128) {
+            if (local_attempt(st, bits, B, kLocalCap * 3 / 4, true)) {
+                g_sort_stats[kStatLocal] += 1;
+                g_sort_stats[kStatLocalEntries] += n;
+                g_sort_stats[kStatPresorted] += 1;
+                return true;
+            }
+            g_sort_stats[kStatOverflow] += 1;
+        }
+    }
+    const uint64_t sig = ordering_signature(bits, st.W, n);
+    int level = 0;
+    {
+        std::lock_guard<std::mutex> lock(g_local_mu);
+        auto it = g_local_level.find(sig);
+        if (it != g_local_level.end()) level = it->second;
+    }
+    if (level == kGiveUp) return false;
+    uint32_t base = 0;
+    if (n > (uint64_t)kLocalCap) {
+        uint32_t need = 0;
+        while ((n >> need) > (uint64_t)kLocalCap / 8) ++need;
+        base = (need + 7) / 8 * 8;
+    }
+    const uint32_t max_b = std::min<uint32_t>(T, 32);
+    base = std::min(base, max_b);
+    if (base > 0 && T <= 32 && T - base <= 8) base = T;   // one more whole-entry pass costs less than a chunk sort
+    for (;;) {
+        uint32_t B = 0, h = 0;
+        bool ok = level_shape(base, max_b, level, B, h) && T - B <= 128 && (B > 0) == (n > (uint64_t)kLocalCap);
+        if (ok) {
+            if (local_attempt(st, bits, B, h)) {
+                g_sort_stats[kStatLocal] += 1;
+                g_sort_stats[kStatLocalEntries] += n;
+                return true;
+            }
+            g_sort_stats[kStatOverflow] += 1;
+        }
+        // B == 0: the whole state is one chunk and never overflows; nothing to refine
+        level = ok && B > 0 ? level + 1 : kGiveUp;
+        {
+            std::lock_guard<std::mutex> lock(g_local_mu);
+            g_local_level[sig] = level;
+        }
+        if (level == kGiveUp) return false;
+    }
 }
 
 }  // namespace sparq

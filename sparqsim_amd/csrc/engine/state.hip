@@ -1,5 +1,6 @@
 // State storage, device-wide scans, relayout and reductions.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -198,15 +199,28 @@ void resize_words(State& st, uint32_t new_words) {
     st.Wcap = 0;
 }
 
+// Readbacks of up to kPinnedBytes go through the state's pinned words (a copy
+// into pageable memory is staged by the runtime and costs more per call).
+constexpr size_t kPinnedBytes = 64 * sizeof(uint64_t);
+
 void read_back(State& st, const void* dptr, size_t bytes, void* host) {
-    SPQ_HIP(hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, st.stream));
-    SPQ_HIP(hipStreamSynchronize(st.stream));
+    const bool on = prof_enabled();
+    const auto t0 = std::chrono::steady_clock::now();
+    if (bytes <= kPinnedBytes) {
+        SPQ_HIP(hipMemcpyAsync(st.pinned, dptr, bytes, hipMemcpyDeviceToHost, st.stream));
+        SPQ_HIP(hipStreamSynchronize(st.stream));
+        std::memcpy(host, st.pinned, bytes);
+    } else {
+        SPQ_HIP(hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, st.stream));
+        SPQ_HIP(hipStreamSynchronize(st.stream));
+    }
+    if (on) prof_host_wait("host:readback", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
 }
 
 uint64_t read_u64(State& st, const uint64_t* dptr) {
-    SPQ_HIP(hipMemcpyAsync(st.pinned, dptr, sizeof(uint64_t), hipMemcpyDeviceToHost, st.stream));
-    SPQ_HIP(hipStreamSynchronize(st.stream));
-    return st.pinned[0];
+    uint64_t v = 0;
+    read_back(st, dptr, sizeof(v), &v);
+    return v;
 }
 
 // ------------------------------------------------------------------ scans

@@ -533,6 +533,21 @@ PYBIND11_MODULE(_core, m) {
     });
     m.def("release_cached_memory", []() { check(sparq_release_cached_memory()); });
     m.def("_take_plugin_error", []() { return take_plugin_error(); });
+    m.def(
+        "sort_stats",
+        [](bool reset) {
+            uint64_t v[6] = {0, 0, 0, 0, 0, 0};
+            check(sparq_sort_stats(v, 6, reset ? 1 : 0));
+            py::dict d;
+            d["local_sorts"] = v[0];
+            d["local_entries"] = v[1];
+            d["local_overflows"] = v[2];
+            d["lsd_sorts"] = v[3];
+            d["lsd_entries"] = v[4];
+            d["presorted_sorts"] = v[5];
+            return d;
+        },
+        py::arg("reset") = false, "Sorts by path since the last reset: prefix + local chunk sort, overflowed local attempts, LSD passes, chunk sorts of states already sorted by their prefix.");
     m.def("set_sort_mode", [](int mode) { check(sparq_set_sort_mode(mode)); }, py::arg("mode"),
           "Sort strategy: 0 prefix passes + LDS chunk sort where it applies (default), 1 LSD passes only.");
     m.def("set_radix_bits", [](int bits) { check(sparq_set_radix_bits(bits)); }, py::arg("bits"),

@@ -15,6 +15,7 @@ summary; --learned enables the learned output sizer.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import math
 import time
@@ -24,7 +25,7 @@ from . import qda
 
 def solve(ps, nqubit=20, steprate=1.1, p=1.3, alpha=2.2666666666564934, beta=-1.0, kappa=0.0, log_every=250,
           tol=1e-6, max_steps=10 ** 9, cz_eps=1e-14, ancilla_order=("anc_4", "anc_3", "anc_2", "anc_1"), log=print,
-          stop_after=None):
+          stop_after=None, step_hook=None):
     n = 1 << nqubit
     kappa_analytic = qda.analytic_kappa(alpha, beta, n)
     kappa = kappa if kappa > 0 else kappa_analytic
@@ -45,6 +46,8 @@ def solve(ps, nqubit=20, steprate=1.1, p=1.3, alpha=2.2666666666564934, beta=-1.
         clear = ps.ClearZero(cz_eps)
         norm = ps.Normalize()
         for i in range(steps if stop_after is None else min(steps, stop_after)):
+            if step_hook is not None:
+                step_hook(i)
             qda.WalkSTridiagonal(ps, "main_reg", "anc_UA", "anc_1", "anc_2", "anc_3", "anc_4", i / steps, kappa, p,
                                  alpha, beta)(st)
             clear(st)
@@ -113,6 +116,10 @@ def main():
     ap.add_argument("--stop-after", type=int, default=None, help="run only the first N steps of the schedule")
     ap.add_argument("--learned", action="store_true", help="size branching operators with the learned sizer")
     ap.add_argument("--json", default=None, help="write the summary (with the curve) to this file")
+    ap.add_argument("--kernel-profile", default=None,
+                    help="write the engine's per-kernel profile (launches, time, algorithmic bytes) and the sort-path "
+                         "counters of the run (from --profile-from on) to this JSON file")
+    ap.add_argument("--profile-from", type=int, default=0, help="start the kernel profile at this step")
     ap.add_argument("--outdir", default=None,
                     help="write residual_curve.csv and qda_summary_cpp.json here (the reference bench's artifacts)")
     args = ap.parse_args()
@@ -121,8 +128,17 @@ def main():
 
     if args.learned and not sizer.activate_default():
         raise SystemExit("no sizer weights found")
-    res = solve(ps, args.nqubit, args.steprate, args.p, args.alpha, args.beta, args.kappa, args.log_every, args.tol,
-                args.max_steps, args.cz_eps, log=lambda s: print(s, flush=True), stop_after=args.stop_after)
+    from .profiling import KernelProfile
+
+    prof = KernelProfile(ps, start_now=args.profile_from == 0) if args.kernel_profile else None
+    hook = (lambda i: prof.start() if i == args.profile_from else None) if prof and args.profile_from else None
+    with prof or contextlib.nullcontext():
+        res = solve(ps, args.nqubit, args.steprate, args.p, args.alpha, args.beta, args.kappa, args.log_every, args.tol,
+                    args.max_steps, args.cz_eps, log=lambda s: print(s, flush=True), stop_after=args.stop_after,
+                    step_hook=hook)
+    if prof:
+        prof.write(args.kernel_profile, steps=res["steps_run"] - args.profile_from, first_step=args.profile_from,
+                   peak_logged_entries=res["peak_logged_entries"])
     if args.learned:
         res["sizer"] = sizer.stats()
     summary = {k: v for k, v in res.items() if k != "curve"}

@@ -182,6 +182,7 @@ def solve(mod, nqubit: int = 16, steps: int = 2000, p: float = 1.3, kappa_A: flo
 
 def main():
     import argparse
+    import contextlib
     import json
 
     ap = argparse.ArgumentParser(description="QDA solve with the sparse (CKS-oracle) block encoding")
@@ -197,15 +198,21 @@ def main():
                     help="complex Hermitian matrix (N(0,1) + i N(0,1) entries at random columns) through the phase oracle")
     ap.add_argument("--kappa-encoded", type=float, default=None,
                     help="with --hermitian: pick kappa(A) so that the block-encoded matrix has this condition number")
+    ap.add_argument("--kernel-profile", default=None,
+                    help="write the engine's per-kernel profile and the sort-path counters of the run to this JSON file")
     args = ap.parse_args()
     import sparqsim_amd as ps
     from . import sizer
+    from .profiling import KernelProfile
 
     if args.learned and not sizer.activate_default():
         raise SystemExit("no sizer weights found")
-    met, *_ = solve(ps, args.nqubit, args.steps, args.p, args.kappa, 8, args.cz_eps, args.log_every,
-                    log=lambda s: print(s, flush=True), stop_after=args.stop_after, hermitian=args.hermitian,
-                    kappa_encoded=args.kappa_encoded)
+    with KernelProfile(ps) if args.kernel_profile else contextlib.nullcontext() as prof:
+        met, *_ = solve(ps, args.nqubit, args.steps, args.p, args.kappa, 8, args.cz_eps, args.log_every,
+                        log=lambda s: print(s, flush=True), stop_after=args.stop_after, hermitian=args.hermitian,
+                        kappa_encoded=args.kappa_encoded)
+    if args.kernel_profile:
+        prof.write(args.kernel_profile, steps=met["steps_run"], entries=met["entries"])
     if args.learned:
         met["sizer"] = sizer.stats()
     print(json.dumps(met), flush=True)

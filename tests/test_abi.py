@@ -67,3 +67,19 @@ def test_registry_semantics_host_only():
     ps.System.clear()
     with pytest.raises(ValueError):
         ps.AddRegister("w", ps.UnsignedInteger, 65)
+
+
+def test_profiler_and_sort_counters_host_only():
+    """The engine profiler and the sort-path counters answer without a GPU:
+    nothing launched, nothing recorded."""
+    import sparqsim_amd as ps
+    from sparqsim_amd import profiling
+
+    with profiling.KernelProfile(ps) as prof:
+        pass
+    assert prof.records == []
+    assert prof.sort_stats == {"local_sorts": 0, "local_entries": 0, "local_overflows": 0, "lsd_sorts": 0,
+                               "lsd_entries": 0, "presorted_sorts": 0}
+    lib = profiling.engine_library()
+    out = (ctypes.c_uint64 * 5)()
+    assert lib.sparq_sort_stats(out, 5, 0) != 0          # too little room is an error, not an overrun
