@@ -14,6 +14,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -181,7 +183,40 @@ def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
         "plain_s": t_plain, "plain_sample_trajectories": plain_traj,
         "batched_speedup_vs_plain": t_plain / t_batched,
         "batched_learned_speedup_vs_plain": (t_plain / t_learned) if t_learned else None,
+        "uneven": grover_uneven_leg(ps, n_traj, address_bits, iterations, p),
     }
+
+
+def grover_uneven_leg(ps, n_traj, address_bits, iterations, p):
+    """The same batch with uneven nnz: trajectory t searches the first
+    2^(t mod (address_bits + 1)) addresses (1 to 2^address_bits entries), one
+    conditioned Hadamard per width in each diffusion; against every trajectory
+    run one state at a time (timed on one trajectory of each width and weighted
+    by the number of trajectories of that width)."""
+    from sparqsim_amd.grover import NoisyGrover, uneven_widths
+
+    widths = uneven_widths(n_traj, address_bits)
+    g = NoisyGrover(ps, n_traj, address_bits, address_bits, iterations, p, widths=widths)
+    g.run_batched().size()                      # warm-up
+    t0 = time.perf_counter()
+    st = g.run_batched()
+    n_final = st.size()
+    t_batched = time.perf_counter() - t0
+    counts = np.bincount(st.export()[1][:, 0].astype(np.int64), minlength=n_traj)
+    del st
+    g.new_plain_registers()
+    g.run_plain(0).size()
+    t_plain = 0.0
+    for c in sorted(set(widths)):
+        t = widths.index(c)
+        t0 = time.perf_counter()
+        g.run_plain(t).size()
+        t_plain += (time.perf_counter() - t0) * widths.count(c)
+    return {"workload": f"{n_traj} trajectories searching 2^0..2^{address_bits} addresses (t mod {address_bits + 1} bits)",
+            "entries_final": n_final, "entries_per_trajectory_min": int(counts.min()),
+            "entries_per_trajectory_max": int(counts.max()), "batched_s": t_batched,
+            "batched_entries_per_s": 12 * iterations * n_final / t_batched, "plain_s": t_plain,
+            "batched_speedup_vs_plain": t_plain / t_batched}
 
 
 def qda_sparse_leg(ps, nqubit, steps):

@@ -363,6 +363,13 @@ int sparq_set_radix_bits(int bits);
  * passes over an index payload and a final gather.  Both give the same stable
  * order. */
 int sparq_set_sort_mode(int mode);
+/* Dense-domain Hadamard (no counterpart in the reference): with on != 0 (the
+ * default) Hadamard_Int / Hadamard_Int_Full on a state whose low target bits
+ * and other varying key bits span at most 2^26 slots, filled to at least a
+ * quarter, run on a dense amplitude array (no grouping sort; the result is in
+ * canonical key order); 0 always takes the grouped path.  Same states either
+ * way (amplitudes to rounding). */
+int sparq_set_dense_transform(int on);
 /* Sort-path counters since the last reset (no counterpart in the reference):
  * out[0] sorts by the prefix + local path, out[1] their entries, out[2] local
  * attempts that overflowed a chunk, out[3] sorts by the LSD passes, out[4]

@@ -373,6 +373,10 @@ int sparq_set_sort_mode(int mode) {
     });
 }
 
+int sparq_set_dense_transform(int on) {
+    return guarded([&] { set_dense_transform(on != 0); });
+}
+
 int sparq_sort_stats(uint64_t* out, uint32_t count, int reset) {
     return guarded([&] {
         SPQ_REQUIRE(out && count >= 6, "sparq_sort_stats needs room for 6 counters");

@@ -18,6 +18,9 @@ namespace sparq {
 
 enum GroupClass : uint8_t { kDirect = 0, kSparseSum = 1, kWht = 2, kDense = 3 };
 
+static bool g_dense_transform = true;   // the dense-domain Hadamard path (below) where it applies
+void set_dense_transform(bool on) { g_dense_transform = on; }
+
 // ---------------------------------------------------------------- groups
 template <int W>
 __global__ void __launch_bounds__(kBlock) head_flag_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, sparq_field_t low,
@@ -394,6 +397,231 @@ static void launch_wht(State& st, double2* a, const uint64_t* counts, uint64_t G
     }
 }
 
+// ---------------------------------------------------------------- dense-domain transform
+// When the low target bits and the key bits varying outside them span a small
+// domain of 2^T slots that the state fills to at least an eighth, the
+// transform runs on a dense amplitude array instead of on groups found by a
+// sort: slot = (others index << nd) | low target value, the others index
+// packing the varying bits outside the low target bits in canonical order.
+// The array is group-major like the grouped path's output, so its butterfly
+// passes run unchanged, on the groups that hold an entry (absent groups stay
+// zero and are skipped).  The kept entries are written in slot order, keys
+// rebuilt from the slot and the bits every key shares.  The scatter marks
+// each slot it writes; a state with duplicate keys (fewer marks than
+// entries) keeps the grouped path.
+constexpr uint32_t kDenseMaxLog = 28;   // 2^28 slots: 4 GiB of amplitudes
+constexpr int kDenseTile = 4096;        // slots per workgroup of the count / emit kernels
+constexpr int kDenseItems = kDenseTile / kBlock;
+
+struct DenseMap {
+    uint32_t nseg;                      // runs of the others bits, least significant first
+    uint8_t word[kDenseMaxLog], shift[kDenseMaxLog], width[kDenseMaxLog];
+    uint32_t nd;
+    sparq_field_t low;
+    uint64_t base[SPARQ_MAX_WORDS];     // the bits every key shares (others and low target bits cleared)
+};
+
+template <int W>
+__device__ __forceinline__ uint64_t dense_others(const uint64_t (&k)[W], const DenseMap& dm) {
+    uint64_t v = 0;
+    uint32_t at = 0;
+    for (uint32_t s = 0; s < dm.nseg; ++s) {
+        v |= ((kword<W>(k, dm.word[s]) >> dm.shift[s]) & fmask(dm.width[s])) << at;
+        at += dm.width[s];
+    }
+    return v;
+}
+
+// Entry i to its slot; its slot's mark and its group's class (plain byte
+// stores: every writer of a word stores the same value).
+template <int W>
+__global__ void __launch_bounds__(kBlock) dense_scatter_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps,
+                                                          uint64_t cap, uint64_t n, DenseMap dm, double2* __restrict__ A,
+                                                          uint8_t* __restrict__ mark, uint8_t* __restrict__ cls) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k[W];
+    load_key<W>(keys, cap, i, k);
+    const double2 a = amps[i];
+    const uint64_t g = dense_others<W>(k, dm);
+    const uint64_t slot = (g << dm.nd) | fget<W>(k, dm.low);
+    A[slot] = a;
+    mark[slot] = 1;
+    cls[g] = kWht;
+}
+
+// Per tile of slots: kept slots (|a|^2 not below eps, as ClearZero), and the
+// marks added to marks_total (one atomic per workgroup); slots of absent
+// groups are skipped.
+__global__ void __launch_bounds__(kBlock) dense_count_kernel(const double2* __restrict__ A, const uint8_t* __restrict__ mark,
+                                                        const uint8_t* __restrict__ cls, uint32_t nd, double eps,
+                                                        uint32_t* __restrict__ kept_out, unsigned long long* __restrict__ marks_total) {
+    __shared__ uint32_t lds[2][kBlock / 64];
+    const uint64_t tile = (uint64_t)blockIdx.x * kDenseTile;
+    uint32_t c = 0, m = 0;
+#pragma unroll
+    for (int j = 0; j < kDenseItems; ++j) {
+        const uint64_t s = tile + (uint64_t)j * kBlock + threadIdx.x;
+        if (cls[s >> nd] != kWht) continue;
+        c += !(cabs2(A[s]) < eps);
+        m += mark[s];
+    }
+    c = wave_sum(c);
+    m = wave_sum(m);
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][threadIdx.x >> 6] = c;
+        lds[1][threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tc = 0, tm = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            tc += lds[0][w];
+            tm += lds[1][w];
+        }
+        kept_out[blockIdx.x] = tc;
+        if (tm) atomicAdd(marks_total, (unsigned long long)tm);
+    }
+}
+
+// Kept slots in slot order: key rebuilt from the slot, amplitude copied.
+template <int W>
+__global__ void __launch_bounds__(kBlock) dense_emit_kernel(const double2* __restrict__ A, const uint8_t* __restrict__ cls, DenseMap dm,
+                                                       double eps, const uint64_t* __restrict__ offs, uint64_t* __restrict__ kout,
+                                                       double2* __restrict__ aout, uint64_t cap_out) {
+    constexpr int kWaves = kBlock / 64;
+    __shared__ uint32_t row_wave[kDenseItems * kWaves];   // kept count per (item row, wave), then its exclusive prefix
+    const uint64_t tile = (uint64_t)blockIdx.x * kDenseTile;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double2 v[kDenseItems];
+    uint32_t flags = 0, lane_pre[kDenseItems];
+#pragma unroll
+    for (int j = 0; j < kDenseItems; ++j) {
+        const uint64_t s = tile + (uint64_t)j * kBlock + threadIdx.x;
+        const bool present = cls[s >> dm.nd] == kWht;
+        v[j] = present ? A[s] : make_double2(0.0, 0.0);
+        const bool keep = present && !(cabs2(v[j]) < eps);
+        const uint64_t m = __ballot(keep);
+        flags |= uint32_t(keep) << j;
+        lane_pre[j] = (uint32_t)__popcll(m & lanemask_lt());
+        if (lane == 0) row_wave[j * kWaves + wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {   // exclusive scan of the (row, wave) counts in slot order
+        const uint32_t x = threadIdx.x < kDenseItems * kWaves ? row_wave[threadIdx.x] : 0;
+        const uint32_t inc = wave_incl_scan(x);
+        if (threadIdx.x < kDenseItems * kWaves) row_wave[threadIdx.x] = inc - x;
+    }
+    __syncthreads();
+    const uint64_t before = offs[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kDenseItems; ++j) {
+        if (!((flags >> j) & 1)) continue;
+        const uint64_t s = tile + (uint64_t)j * kBlock + threadIdx.x;
+        const uint64_t others = s >> dm.nd;
+        uint64_t k[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) k[w] = dm.base[w];
+        uint32_t at = 0;
+        for (uint32_t q = 0; q < dm.nseg; ++q) {
+            const uint64_t bits = (others >> at) & fmask(dm.width[q]);
+            kset_word<W>(k, dm.word[q], kword<W>(k, dm.word[q]) | (bits << dm.shift[q]));
+            at += dm.width[q];
+        }
+        fput<W>(k, dm.low, s & fmask(dm.nd));
+        const uint64_t dst = before + row_wave[j * kWaves + wave] + lane_pre[j];
+        store_key<W>(kout, cap_out, dst, k);
+        aout[dst] = v[j];
+    }
+}
+
+// {marks = 0, groups, WHT groups = 1} after the kept total.
+__global__ void dense_init_kernel(uint64_t* __restrict__ tail, uint64_t groups) {
+    tail[0] = 0;
+    tail[1] = groups;
+    tail[2] = 1;
+}
+
+// The dense-domain transform, when it applies: returns false (state untouched)
+// when the domain is too large or too sparsely filled, or the state holds
+// duplicate keys.
+template <int W>
+static bool hadamard_dense(State& st, const KeyStats& ks, sparq_field_t low, uint32_t nd, double eps) {
+    const uint64_t n = st.n;
+    if (eps <= 0.0 || n < (uint64_t(1) << 14) || ks.vary.size() != st.W || ks.ones.size() != st.W) return false;
+    const uint64_t lowmask = fmask_host(nd) << low.shift;
+    uint32_t t_o = 0;
+    for (uint32_t w = 0; w < st.W; ++w) t_o += (uint32_t)__builtin_popcountll(ks.vary[w] & (w == low.word ? ~lowmask : ~0ull));
+    const uint32_t T = t_o + nd;
+    if (t_o == 0 || T > kDenseMaxLog || T < 12 || n * 8 < (uint64_t(1) << T)) return false;
+    DenseMap dm{};
+    dm.nd = nd;
+    dm.low = low;
+    for (int w = (int)st.W - 1; w >= 0; --w) {   // least significant word first, bit 0 up
+        const uint64_t o = ks.vary[w] & ((uint32_t)w == low.word ? ~lowmask : ~0ull);
+        for (int p = 0; p < 64; ++p) {
+            if (!((o >> p) & 1)) continue;
+            if (dm.nseg > 0 && dm.word[dm.nseg - 1] == w && dm.shift[dm.nseg - 1] + dm.width[dm.nseg - 1] == p) {
+                ++dm.width[dm.nseg - 1];
+                continue;
+            }
+            dm.word[dm.nseg] = (uint8_t)w;
+            dm.shift[dm.nseg] = (uint8_t)p;
+            dm.width[dm.nseg] = 1;
+            ++dm.nseg;
+        }
+    }
+    for (uint32_t w = 0; w < st.W; ++w)
+        dm.base[w] = ks.ones[w] & ~ks.vary[w] & (w == low.word ? ~lowmask : ~0ull);
+    const uint64_t slots = uint64_t(1) << T, groups = uint64_t(1) << t_o;
+    const uint32_t tiles = (uint32_t)(slots / kDenseTile);
+    double2* A = dalloc<double2>(slots, st.stream);
+    uint8_t* mark = dalloc<uint8_t>(slots, st.stream);
+    uint8_t* cls = dalloc<uint8_t>(groups, st.stream);
+    uint32_t* kept = dalloc<uint32_t>(tiles, st.stream);
+    uint64_t* offs = dalloc<uint64_t>(tiles + 4, st.stream);   // [tiles] kept total, [tiles + 1] marks, [tiles + 2, + 4) {groups, WHT groups}
+    uint64_t* gcount = offs + tiles + 2;
+    SPQ_HIP(hipMemsetAsync(A, 0, slots * sizeof(double2), st.stream));
+    SPQ_HIP(hipMemsetAsync(mark, 0, slots, st.stream));
+    SPQ_HIP(hipMemsetAsync(cls, kDirect, groups, st.stream));
+    dense_init_kernel<<<1, 1, 0, st.stream>>>(offs + tiles + 1, groups);
+    SPQ_HIP(hipGetLastError());
+    {
+        SPQ_PROF(st.stream, "dense_scatter_kernel", n * (8.0 * W + 16 + 16 + 2));
+        dense_scatter_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, dm, A, mark, cls);
+    }
+    SPQ_HIP(hipGetLastError());
+    launch_wht(st, A, gcount, groups, nd, cls, (double)slots * 32.0);
+    {
+        SPQ_PROF(st.stream, "dense_count_kernel", (double)slots * 17.0);
+        dense_count_kernel<<<tiles, kBlock, 0, st.stream>>>(A, mark, cls, nd, eps, kept, (unsigned long long*)(offs + tiles + 1));
+    }
+    SPQ_HIP(hipGetLastError());
+    exclusive_scan_u32(st, kept, offs, tiles, offs + tiles);
+    uint64_t info[2];   // kept total, marks
+    read_back(st, offs + tiles, sizeof(info), info);
+    const bool ok = info[1] == n;
+    if (ok) {
+        const uint64_t total = info[0];
+        const uint64_t cap_out = std::max<uint64_t>(total, 1);
+        uint64_t* kout;
+        double2* aout;
+        alloc_arrays(st, st.W, cap_out, &kout, &aout);
+        {
+            SPQ_PROF(st.stream, "dense_emit_kernel", (double)slots * 16.0 + (double)total * (8.0 * W + 16));
+            dense_emit_kernel<W><<<tiles, kBlock, 0, st.stream>>>(A, cls, dm, eps, offs, kout, aout, cap_out);
+        }
+        SPQ_HIP(hipGetLastError());
+        install_arrays(st, st.W, kout, aout, total, cap_out);
+    }
+    dfree(A, st.stream);
+    dfree(mark, st.stream);
+    dfree(cls, st.stream);
+    dfree(kept, st.stream);
+    dfree(offs, st.stream);
+    return ok;
+}
+
 template <int W>
 static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, const double2* dense,
                           bool allow_learned = true, bool unit_basis = false) {
@@ -407,6 +635,7 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     if (n > 1) {
         const SizerQuery q{n, st.W, nd, dense ? 2u : (full ? 1u : 0u), low.word, low.shift};
         ks = key_stats(st, &low, allow_learned ? &q : nullptr);
+        if (!dense && g_dense_transform && hadamard_dense<W>(st, ks, low, nd, eps)) return;
         if (!ks.sorted) sort_with_stats(st, ks, &low);
         single = true;
         for (uint32_t w = 0; w < st.W; ++w) {

@@ -140,6 +140,9 @@ void sort_by_plan(State& st, const std::vector<Digit>& plan, int radix_bits = 8,
 void set_radix_bits(int bits);
 // 0: prefix passes + LDS-local chunk sort where it applies, 1: LSD passes only.
 void set_sort_mode(int mode);
+// Hadamard_Int on a small, well filled key domain through a dense amplitude
+// array (on by default); off: always the grouped path.
+void set_dense_transform(bool on);
 // out[6]: sorts by the prefix + local path, their entries, local attempts that
 // overflowed a chunk, sorts by the LSD passes, their entries, and the local
 // sorts that ran no prefix pass (the state was already sorted by the prefix);
@@ -163,6 +166,7 @@ struct SizerQuery {
 };
 struct KeyStats {
     std::vector<uint64_t> vary;
+    std::vector<uint64_t> ones;        // OR of the keys (the constant bits' values where vary is 0)
     bool sorted = true;
     bool last_sorted = true;           // the `last` field alone is non-decreasing
     // per word, the first differing bits of adjacent pairs out of (others, last)

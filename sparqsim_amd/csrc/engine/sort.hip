@@ -222,6 +222,7 @@ KeyStats key_stats(State& st, const sparq_field_t* last, const SizerQuery* q) {
     dfree(partial, st.stream);
     dfree(out, st.stream);
     for (uint32_t w = 0; w < st.W; ++w) ks.vary[w] = h[w] ^ h[st.W + w];
+    ks.ones.assign(h.begin(), h.begin() + st.W);
     ks.sorted = h[2 * st.W] == 0;
     ks.last_sorted = last != nullptr && h[2 * st.W + 1] == 0;
     ks.disorder.assign(h.begin() + 2 * st.W + 2, h.begin() + 3 * st.W + 2);

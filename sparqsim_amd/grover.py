@@ -50,24 +50,44 @@ class GroverOracle:
 
 
 class DiffusionOperator:
-    """HPH: Hadamard_Int_Full, phase flip of |0>, Hadamard_Int_Full (grover.cpp:64-69)."""
+    """HPH: Hadamard_Int_Full, phase flip of |0>, Hadamard_Int_Full (grover.cpp:64-69).
+    With `digits`, the Hadamards act on the low `digits` address bits only
+    (Hadamard_Int(addr, digits); none for 0): the diffusion of a search over the
+    first 2^digits addresses, whose upper address bits stay 0."""
 
-    def __init__(self, mod, addr: str):
-        self.h = mod.Hadamard_Int_Full(addr)
+    def __init__(self, mod, addr: str, digits: int | None = None):
+        if digits is None:
+            self.hs = [mod.Hadamard_Int_Full(addr)]
+        else:
+            self.hs = [mod.Hadamard_Int(addr, digits)] if digits > 0 else []
         self.flip = mod.ZeroConditionalPhaseFlip([addr])
 
     def __call__(self, st):
-        self.h(st)
+        for h in self.hs:
+            h(st)
         self.flip(st)
-        self.h(st)
+        for h in self.hs:
+            h(st)
+
+
+class ClassDiffusion(DiffusionOperator):
+    """The diffusion of a batch whose trajectories search address ranges of
+    different widths: register `cls` holds each trajectory's width c, and the
+    Hadamard on the low c address bits is conditioned on cls == c, one launch
+    per width present.  The phase flip of addr == 0 is shared (upper address
+    bits are 0 in every trajectory)."""
+
+    def __init__(self, mod, addr: str, widths):
+        self.hs = [mod.Hadamard_Int(addr, c).conditioned_by_value("cls", c) for c in sorted(set(widths)) if c > 0]
+        self.flip = mod.ZeroConditionalPhaseFlip([addr])
 
 
 class GroverOperator:
     """One Grover iteration: oracle then diffusion (grover.cpp:71-76)."""
 
-    def __init__(self, mod, qram, addr: str, data: str, search: str):
+    def __init__(self, mod, qram, addr: str, data: str, search: str, diffusion: DiffusionOperator | None = None):
         self.oracle = GroverOracle(mod, qram, addr, data, search)
-        self.diffusion = DiffusionOperator(mod, addr)
+        self.diffusion = diffusion or DiffusionOperator(mod, addr)
 
     def __call__(self, st):
         self.oracle(st)
@@ -119,6 +139,13 @@ def grover_success_amplitude(address_bits: int, k: int) -> float:
 
 
 # ---------------------------------------------------------------- noisy trajectories
+def uneven_widths(n_traj: int, address_bits: int) -> list[int]:
+    """Search widths of the uneven batch: trajectory t searches the first
+    2^(t mod (address_bits + 1)) addresses, so its state holds 1 to
+    2^address_bits entries."""
+    return [t % (address_bits + 1) for t in range(n_traj)]
+
+
 class NoisyGrover:
     """n_traj Grover trajectories on (addr, data, search) of the given widths, each with
     its own seeded Pauli X/Z insertions after every iteration.
@@ -126,14 +153,25 @@ class NoisyGrover:
     `run_batched()` keeps all trajectories in one state keyed by `traj`;
     `run_plain(t)` runs trajectory t alone in its own state.  Both start from
     |0>: search := target (Init_Unsafe), Hadamard_Int_Full(addr), then
-    `iterations` x (GroverOperator, noise)."""
+    `iterations` x (GroverOperator, noise).
+
+    `widths` (one per trajectory, 0..address_bits) makes the batch uneven:
+    trajectory t searches only the first 2^widths[t] addresses (Hadamards on
+    the low widths[t] address bits, noise on those bits only), so its state
+    holds 2^widths[t] entries; the batch carries a register `cls` with each
+    trajectory's width for the per-width diffusion."""
 
     def __init__(self, mod, n_traj: int, address_bits: int, data_bits: int, iterations: int, p: float,
-                 seed_offset: int = GROVER_SEED_OFFSET):
+                 seed_offset: int = GROVER_SEED_OFFSET, widths=None):
         self.mod, self.n_traj, self.A, self.D = mod, n_traj, address_bits, data_bits
         self.iterations, self.p = iterations, p
         self.memory, self.target, self.marked = grover_memory(address_bits, data_bits, seed_offset)
+        self.widths = None if widths is None else [int(c) for c in widths]
+        if self.widths is not None and (len(self.widths) != n_traj or not all(0 <= c <= address_bits for c in self.widths)):
+            raise ValueError("widths needs one value in [0, address_bits] per trajectory")
         self.events = noise_events(n_traj, iterations, address_bits, p, seed_offset)
+        if self.widths is not None:
+            self.events = [[e for e in ev if e[1] < self.widths[e[0]]] for ev in self.events]
         self.traj_bits = max(1, (n_traj - 1).bit_length())
         self.qram = None
 
@@ -143,6 +181,8 @@ class NoisyGrover:
         st = m.SparseState()
         if batched:
             m.AddRegister("traj", m.UnsignedInteger, self.traj_bits)(st)
+            if self.widths is not None:
+                m.AddRegister("cls", m.UnsignedInteger, max(1, self.A.bit_length()))(st)
         m.AddRegister("addr", m.UnsignedInteger, self.A)(st)
         m.AddRegister("data", m.UnsignedInteger, self.D)(st)
         m.AddRegister("search", m.UnsignedInteger, self.D)(st)
@@ -156,12 +196,21 @@ class NoisyGrover:
     def run_batched(self):
         m = self.mod
         st = self._registers(batched=True)
-        vals = np.zeros((self.n_traj, 4), dtype=np.uint64)
+        uneven = self.widths is not None
+        vals = np.zeros((self.n_traj, 5 if uneven else 4), dtype=np.uint64)
         vals[:, 0] = np.arange(self.n_traj, dtype=np.uint64)
+        if uneven:
+            vals[:, 1] = np.asarray(self.widths, dtype=np.uint64)
         st.load(vals, np.ones(self.n_traj, dtype=complex))
         m.Init_Unsafe("search", self.target)(st)
-        m.Hadamard_Int_Full("addr")(st)
-        g = GroverOperator(m, self.qram, "addr", "data", "search")
+        if uneven:
+            diffusion = ClassDiffusion(m, "addr", self.widths)
+            for h in diffusion.hs:
+                h(st)
+        else:
+            diffusion = None
+            m.Hadamard_Int_Full("addr")(st)
+        g = GroverOperator(m, self.qram, "addr", "data", "search", diffusion)
         for k in range(self.iterations):
             g(st)
             for t, i, kind in self.events[k]:
@@ -177,8 +226,13 @@ class NoisyGrover:
         m = self.mod
         st = m.SparseState()
         m.Init_Unsafe("search", self.target)(st)
-        m.Hadamard_Int_Full("addr")(st)
-        g = GroverOperator(m, self.qram, "addr", "data", "search")
+        diffusion = None if self.widths is None else DiffusionOperator(m, "addr", self.widths[t])
+        if diffusion is None:
+            m.Hadamard_Int_Full("addr")(st)
+        else:
+            for h in diffusion.hs:
+                h(st)
+        g = GroverOperator(m, self.qram, "addr", "data", "search", diffusion)
         for k in range(self.iterations):
             g(st)
             for tt, i, kind in self.events[k]:

@@ -51,6 +51,53 @@ def test_config1_plain_circuit(ps, oracle, w):
     assert_same_state(eg, eo, exact_amps=True, what="config1")
 
 
+def test_config1_16bit_first_steps(ps, oracle):
+    """Configuration 1 at its own 16-bit width, steps 1-3 (H(a); b += a;
+    d ^= M[a]: 65 536 entries) against the oracle, amplitudes bit-exact.  The
+    literal final H(a) makes 2^32 entries; it is checked by its closed form at
+    12 bits below."""
+    w = 16
+    rng = np.random.default_rng(0x5EED2026 + 0)
+    mem = [int(x) for x in rng.integers(0, 1 << w, size=1 << w)]
+
+    def build(mod):
+        st = regs(mod, [("a", "UnsignedInteger", w), ("b", "UnsignedInteger", w), ("d", "UnsignedInteger", w)])
+        q = mod.QRAMCircuit_qutrit(w, w, mem)
+        mod.Hadamard_Int("a", w)(st)
+        mod.Add_UInt_UInt_InPlace("a", "b")(st)
+        mod.QRAMLoad(q, "a", "d")(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert eg[1].shape[0] == 1 << w
+    assert_same_state(eg, eo, exact_amps=True, what="config1 steps 1-3 at 16 bits")
+
+
+def test_config1_12bit_full_circuit_closed_form(ps):
+    """The whole configuration-1 circuit at 12 bits (2^24 entries after the final
+    H(a)) against its closed form: every (a', b) pair once, d = M[b], amplitude
+    2^-12 (-1)^popcount(a' & b) (b holds the first superposition's a)."""
+    w = 12
+    rng = np.random.default_rng(0x5EED2026 + 0)
+    mem = rng.integers(0, 1 << w, size=1 << w).astype(np.uint64)
+    ps.System.clear()
+    st = regs(ps, [("a", "UnsignedInteger", w), ("b", "UnsignedInteger", w), ("d", "UnsignedInteger", w)])
+    q = ps.QRAMCircuit_qutrit(w, w, [int(x) for x in mem])
+    ps.Hadamard_Int("a", w)(st)
+    ps.Add_UInt_UInt_InPlace("a", "b")(st)
+    ps.QRAMLoad(q, "a", "d")(st)
+    ps.Hadamard_Int("a", w)(st)
+    ps.ClearZero(1e-14)(st)
+    _, vals, amps = st.export()
+    assert vals.shape[0] == 1 << (2 * w)
+    a, b, d = vals[:, 0], vals[:, 1], vals[:, 2]
+    pair = np.sort(b * (1 << w) + a)
+    assert np.array_equal(pair, np.arange(1 << (2 * w), dtype=np.uint64))
+    assert np.array_equal(d, mem[b])
+    sign = 1.0 - 2.0 * (np.bitwise_count(a & b) & 1)
+    assert np.max(np.abs(amps - sign * 2.0 ** -w)) <= 1e-12 * 2.0 ** -w
+
+
 @pytest.mark.parametrize("A", [6, 12, 16])
 def test_config2_qram_interference(ps, oracle, A):
     """H(addr); data ^= M[addr]; 32 Z on data bits; uncompute; H(addr)."""

@@ -13,14 +13,37 @@ import pytest
 
 from parity import assert_same_state, canonical
 from sparqsim_amd.grover import (NoisyGrover, grover_memory, grover_success_amplitude,
-                                 noise_events)
+                                 noise_events, uneven_widths)
 
 
-def trajectory_of(export, t):
-    """Entries of trajectory t of a batched export, with the traj column dropped."""
+def trajectory_of(export, t, skip=1):
+    """Entries of trajectory t of a batched export, with the traj column (and,
+    for an uneven batch, skip=2, the cls column) dropped."""
     ids, vals, amps = export
     sel = vals[:, 0] == t
-    return [i - 1 for i in ids[1:]], vals[sel][:, 1:], amps[sel]
+    return [i - skip for i in ids[skip:]], vals[sel][:, skip:], amps[sel]
+
+
+def check_uneven_closed_form(g, export, trajectories, atol=1e-12):
+    """Noise-free trajectories of an uneven batch: 2^c entries for width c; a
+    search range holding the marked address gives sin((2K+1) theta_c) there and
+    cos((2K+1) theta_c)/sqrt(2^c - 1) elsewhere (sin theta_c = 2^(-c/2)); a
+    range without it keeps every |amplitude| at 2^(-c/2)."""
+    K = g.iterations
+    for t in trajectories:
+        c = g.widths[t]
+        _, v, a = trajectory_of(export, t, skip=2)
+        assert len(a) == 1 << c, (t, c, len(a))
+        assert np.all(v[:, 0] < (1 << c)) and np.all(v[:, 1] == 0) and np.all(v[:, 2] == g.target), t
+        if g.marked < (1 << c) and c > 0:
+            theta = math.asin(2.0 ** (-c / 2))
+            s, co = math.sin((2 * K + 1) * theta), math.cos((2 * K + 1) * theta)
+            marked = v[:, 0] == g.marked
+            assert np.allclose(a[marked], s, atol=atol, rtol=0), t
+            if c > 0 and (1 << c) > 1:
+                assert np.allclose(np.abs(a[~marked]), abs(co) / math.sqrt((1 << c) - 1), atol=atol, rtol=0), t
+        else:
+            assert np.allclose(np.abs(a), 2.0 ** (-c / 2), atol=atol, rtol=0), t
 
 
 # ---------------------------------------------------------------- inputs
@@ -71,6 +94,31 @@ def test_oracle_batched_equals_plain(oracle):
     g.new_plain_registers()
     for t in range(6):
         assert_same_state(trajectory_of(eb, t), g.run_plain(t).export(), exact_amps=True, what=f"traj {t}")
+
+
+def test_oracle_uneven_batched_equals_plain(oracle):
+    """An uneven batch (search widths 0..5 over 8 trajectories): the batched run
+    equals each trajectory run alone, entry for entry."""
+    g = NoisyGrover(oracle, 8, 5, 5, 3, 0.1, widths=uneven_widths(8, 5))
+    assert g.widths == [0, 1, 2, 3, 4, 5, 0, 1]
+    assert all(i < g.widths[t] for ev in g.events for (t, i, _) in ev)
+    assert sum(len(e) for e in g.events) > 0
+    eb = g.run_batched().export()
+    g.new_plain_registers()
+    for t in range(8):
+        assert_same_state(trajectory_of(eb, t, skip=2), g.run_plain(t).export(), exact_amps=True, what=f"traj {t}")
+
+
+def test_oracle_uneven_closed_form(oracle):
+    """Noise-free uneven batch with the marked address inside some search
+    ranges and outside others: entry counts 1..2^A and the closed forms."""
+    A = 6
+    g = NoisyGrover(oracle, 2 * (A + 1), A, A, 3, 0.0, widths=uneven_widths(2 * (A + 1), A))
+    g.marked = 5                    # in range for widths >= 3, out of range below
+    g.memory = [w if i != 5 else g.target for i, w in enumerate(g.memory)]
+    g.memory = [w ^ 1 if (w == g.target and i != 5) else w for i, w in enumerate(g.memory)]
+    eb = g.run_batched().export()
+    check_uneven_closed_form(g, eb, range(2 * (A + 1)))
 
 
 def test_oracle_noise_changes_the_state(oracle):
@@ -153,3 +201,35 @@ def test_engine_batched_full_size_properties(ps):
         marked = v[:, 0] == g.marked
         assert np.allclose(a[marked], s, atol=1e-12, rtol=0), t
         assert np.allclose(a[~marked], c, atol=1e-12, rtol=0), t
+
+
+@pytest.mark.gpu
+def test_engine_uneven_batched_vs_oracle(ps, oracle):
+    """The uneven batch on the engine (one conditioned Hadamard per width in each
+    diffusion) against the oracle's per-trajectory runs."""
+    T, A, K, p = 14, 6, 3, 0.08
+    ge = NoisyGrover(ps, T, A, A, K, p, widths=uneven_widths(T, A))
+    eb = ge.run_batched().export()
+    go = NoisyGrover(oracle, T, A, A, K, p, widths=uneven_widths(T, A))
+    go.new_plain_registers()
+    for t in range(T):
+        assert_same_state(trajectory_of(eb, t, skip=2), go.run_plain(t).export(), what=f"traj {t}")
+
+
+@pytest.mark.gpu
+def test_engine_uneven_full_size_properties(ps):
+    """The benchmarked uneven batch (1 024 trajectories searching 2^0 .. 2^14
+    addresses, 40 iterations, p = 1e-3): every trajectory keeps norm 1 and every
+    trajectory without a noise event holds its closed form."""
+    T, A, K, p = 1024, 14, 40, 1e-3
+    g = NoisyGrover(ps, T, A, A, K, p, widths=uneven_widths(T, A))
+    export = g.run_batched().export()
+    traj = export[1][:, 0].astype(np.int64)
+    norms = np.bincount(traj, weights=np.abs(export[2]) ** 2, minlength=T)
+    assert np.all(np.abs(norms - 1.0) <= 1e-10), float(np.max(np.abs(norms - 1.0)))
+    counts = np.bincount(traj, minlength=T)
+    assert counts.min() == 1 and counts.max() == 1 << A
+    touched = {t for it in g.events for (t, _, _) in it}
+    clean = [t for t in range(T) if t not in touched]
+    assert len(clean) > 300
+    check_uneven_closed_form(g, export, clean[:96])
