@@ -172,6 +172,7 @@ def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
         g.run_batched().size()
         t_learned = time.perf_counter() - t0
         sizer.enable(False)
+    captured = grover_captured_timing(g, iterations)
     # each iteration applies 12 reference operators to every entry of every trajectory
     entry_ops = 12 * iterations * n_final
     return {
@@ -183,8 +184,32 @@ def grover_leg(ps, n_traj, address_bits, iterations, p, plain_traj):
         "plain_s": t_plain, "plain_sample_trajectories": plain_traj,
         "batched_speedup_vs_plain": t_plain / t_batched,
         "batched_learned_speedup_vs_plain": (t_plain / t_learned) if t_learned else None,
+        "captured": captured,
+        "captured_speedup_vs_batched": t_batched / captured["replays_s"],
         "uneven": grover_uneven_leg(ps, n_traj, address_bits, iterations, p),
     }
+
+
+def grover_captured_timing(g, iterations):
+    """The batch with every iteration replayed from one captured HIP graph
+    (oracle, diffusion, this iteration's noise from a device table): the
+    replays are queued back to back with no host synchronisation; setup is the
+    one-off planning and recording of the sequence."""
+    st, seq = g.capture_batched()               # warm-up
+    seq.launch(iterations)
+    seq.sync()
+    del seq, st
+    t0 = time.perf_counter()
+    st, seq = g.capture_batched()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    seq.launch(iterations)
+    seq.sync()
+    t_replays = time.perf_counter() - t0
+    out = {"setup_s": t_setup, "replays_s": t_replays, "ms_per_iteration": t_replays / iterations * 1e3,
+           "graph_nodes_per_iteration": seq.nodes, "capacity": seq.capacity, "entries_final": st.size()}
+    del seq, st
+    return out
 
 
 def grover_uneven_leg(ps, n_traj, address_bits, iterations, p):
@@ -212,11 +237,14 @@ def grover_uneven_leg(ps, n_traj, address_bits, iterations, p):
         t0 = time.perf_counter()
         g.run_plain(t).size()
         t_plain += (time.perf_counter() - t0) * widths.count(c)
+    captured = grover_captured_timing(g, iterations)
     return {"workload": f"{n_traj} trajectories searching 2^0..2^{address_bits} addresses (t mod {address_bits + 1} bits)",
             "entries_final": n_final, "entries_per_trajectory_min": int(counts.min()),
             "entries_per_trajectory_max": int(counts.max()), "batched_s": t_batched,
             "batched_entries_per_s": 12 * iterations * n_final / t_batched, "plain_s": t_plain,
-            "batched_speedup_vs_plain": t_plain / t_batched}
+            "batched_speedup_vs_plain": t_plain / t_batched,
+            "captured": captured, "captured_speedup_vs_batched": t_batched / captured["replays_s"],
+            "captured_speedup_vs_plain": t_plain / captured["replays_s"]}
 
 
 def qda_sparse_leg(ps, nqubit, steps):

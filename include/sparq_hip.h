@@ -141,6 +141,15 @@ enum sparq_pw_opcode {
     SPARQ_PW_TRIDIAG_LOAD,        /* f3 ^= M(f0 + 4*f1 + f2)                      TridiagOracle1 (:16-34) */
     SPARQ_PW_TRIDIAG_COL2SLOT,    /* column -> slot                               TridiagOracle2 (:36-61) */
     SPARQ_PW_TRIDIAG_SLOT2COL,    /* slot -> column                               TridiagOracle2::dag (:63-88) */
+    /* No counterpart in the reference: */
+    SPARQ_PW_ASSERT_ZERO,         /* f0 must be 0 (else the capture's flags get kCapNonzeroRemoved); f0 = 0.
+                                     RemoveRegister's uniformity check (system_operations.cpp:303-322) inside a
+                                     captured sequence; the engine supplies the flags word, only valid there */
+    SPARQ_PW_PAULI_TABLE,         /* seeded Pauli noise from a device table of uint64 words: k = table[0]
+                                     (iteration), s = f0 (segment, e.g. trajectory id); when k < c1 and
+                                     s < c0: f1 ^= table[1 + 2(k*c0 + s)], then amp *= d0 for each set bit of
+                                     f1 & table[2 + 2(k*c0 + s)], ascending -- the X_Bool / Z_Bool insertions
+                                     of one iteration of one segment (basic_gates.cpp:11-55, 383-400) */
     SPARQ_PW_OPCODE_END
 };
 
@@ -192,6 +201,12 @@ int sparq_state_stream(sparq_state_t st, void** stream);
 int sparq_table_create(const void* host_words, uint64_t nwords, uint32_t word_bytes, sparq_table_t* out);
 int sparq_table_destroy(sparq_table_t t);
 int sparq_table_device_ptr(sparq_table_t t, const void** ptr);
+/* Overwrite words [first, first+count) of a table from host memory (synchronous). */
+int sparq_table_write(sparq_table_t t, uint64_t first, const void* host_words, uint64_t count);
+/* Word 0 of a uint64 table += 1, ordered on the state's stream: the iteration
+ * counter of SPARQ_PW_PAULI_TABLE.  Capturable (inside a captured sequence it
+ * advances once per replay).  No counterpart in the reference. */
+int sparq_table_advance(sparq_state_t st, sparq_table_t t);
 
 /* ------------------------------------------------------------ operators */
 /* Fused per-entry program (all key-rewrite and diagonal operators). */
@@ -300,6 +315,48 @@ int sparq_cond_norm2(sparq_state_t st, const sparq_cond_t* conds, uint32_t ncond
 int sparq_state_read_range(sparq_state_t st, uint64_t first, uint64_t count, uint64_t* keys, double* amps);
 /* TestRemovable helper (debugger.cpp:191-212): *uniform = 1 when every entry has the same value. */
 int sparq_field_uniform(sparq_state_t st, sparq_field_t f, int* uniform, uint64_t* value);
+
+/* ------------------------------------------------------------ captured sequences
+ * No counterpart in the reference, whose callers loop over operators with a
+ * host synchronisation after every branching one.  A fixed operator sequence
+ * is recorded once on a state and replayed as a HIP graph:
+ *   sparq_capture_plan(st)     snapshot the state's keys; the caller then runs
+ *                              the sequence once: nothing is launched, each
+ *                              branching operator plans its transform from the
+ *                              snapshot (segments: the distinct values of the
+ *                              key bits that vary outside its target among the
+ *                              entries its conditions select);
+ *   sparq_capture_record(st)   allocate what the plans need, grow the state's
+ *                              arrays to the capacity they imply and start
+ *                              capturing the state's stream; the caller runs
+ *                              the same sequence again;
+ *   sparq_capture_end(st, &g)  stop capturing; g replays the sequence.
+ * Inside a capture only sparq_pointwise (any program), sparq_hadamard_int /
+ * sparq_hadamard_int_full (positive ClearZero threshold; conditions allowed),
+ * sparq_table_advance, sparq_state_size / sparq_state_words and the capture
+ * calls are accepted; anything else fails with SPARQ_INVALID (then call
+ * sparq_capture_abort).  Recorded kernels read the entry count from device
+ * memory and stride over it: replays need no host synchronisation.
+ * sparq_graph_launch queues `times` replays on the state's stream; the next
+ * call on the state reads the entry count and the violation flags, and fails
+ * with SPARQ_RUNTIME if a replay broke an assumption of the plan (key bits
+ * outside a branching operator's segments or constant bits, duplicate keys,
+ * more entries than the capacity, a removed register not zero); the state's
+ * content is then undefined.  A graph is bound to the state's arrays: after an
+ * operation that reallocates them (an eager branching operator, a layout
+ * change) sparq_graph_launch fails with SPARQ_INVALID.  Destroy a graph before
+ * its state. */
+typedef struct sparq_graph_s* sparq_graph_t;
+int sparq_capture_plan(sparq_state_t st);
+int sparq_capture_record(sparq_state_t st);
+int sparq_capture_end(sparq_state_t st, sparq_graph_t* out);
+int sparq_capture_abort(sparq_state_t st);
+/* 0: no capture, 1: planning, 2: recording. */
+int sparq_capture_phase(sparq_state_t st, int* phase);
+int sparq_graph_launch(sparq_graph_t g, uint32_t times);
+/* Entry capacity of the captured sequence and the graph's node count. */
+int sparq_graph_info(sparq_graph_t g, uint64_t* capacity, uint64_t* nodes);
+int sparq_graph_destroy(sparq_graph_t g);
 
 /* ------------------------------------------------------------ diagnostics */
 /* Message of the last failing call on this thread. */

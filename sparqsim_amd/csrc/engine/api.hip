@@ -4,6 +4,7 @@
 #include <cmath>
 #include <limits>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -19,6 +20,20 @@ struct sparq_table_s {
     void* dev = nullptr;
     uint64_t nwords = 0;
     uint32_t word_bytes = 0;
+};
+
+// A captured sequence: the graph, its capture (device memory, plans) and the
+// state's arrays it was recorded on.
+struct sparq_graph_s {
+    sparq_state_t state = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    Capture* cap = nullptr;
+    uint64_t* keys = nullptr;
+    double2* amps = nullptr;
+    uint64_t arr_cap = 0;
+    uint32_t W = 0;
+    size_t nodes = 0;
 };
 
 static thread_local std::string g_last_error;
@@ -99,14 +114,48 @@ static void recycle_resources(State& st) {
     g_res_pool.push_back(r);
 }
 
-// Every access through S may change the state: it clears basis_amp_one.
+// Replays of a captured sequence leave the entry count on the device: the
+// first access after them reads it with the violation flags.  A violated
+// plan assumption leaves the content undefined (the count stays within the
+// arrays) and is reported once.
+static void settle(State& st) {
+    if (!st.pending_words) return;
+    uint64_t w[2];
+    read_back(st, st.pending_words, sizeof(w), w);
+    uint64_t* words = const_cast<uint64_t*>(st.pending_words);
+    st.pending_words = nullptr;
+    st.n = w[kCapN];
+    if (w[kCapFlags]) {
+        SPQ_HIP(hipMemsetAsync(words + kCapFlags, 0, sizeof(uint64_t), st.stream));
+        std::string why;
+        if (w[kCapFlags] & kCapOutsidePlan) why += " an entry outside the planned segments or constant bits;";
+        if (w[kCapFlags] & kCapDuplicate) why += " duplicate keys;";
+        if (w[kCapFlags] & kCapOverflow) why += " more entries than the captured capacity;";
+        if (w[kCapFlags] & kCapNonzeroRemoved) why += " a removed register was not zero;";
+        throw Error(SPARQ_RUNTIME, "a replay of the captured sequence broke its plan (state content undefined):" + why);
+    }
+}
+
+// Every access through S may change the state: it clears basis_amp_one.  No
+// access but the capture-aware ones (S_cap) is allowed while a sequence is
+// being captured on the state.
 static State& S(sparq_state_t h) {
     if (!h) throw Error(SPARQ_INVALID, "null state handle");
+    if (h->st.capture) throw Error(SPARQ_INVALID, "not available while a sequence is being captured on this state");
+    settle(h->st);
     h->st.basis_amp_one = false;
     return h->st;
 }
 static State& S_keep_basis(sparq_state_t h) {
     if (!h) throw Error(SPARQ_INVALID, "null state handle");
+    if (h->st.capture) throw Error(SPARQ_INVALID, "not available while a sequence is being captured on this state");
+    settle(h->st);
+    return h->st;
+}
+static State& S_cap(sparq_state_t h) {
+    if (!h) throw Error(SPARQ_INVALID, "null state handle");
+    settle(h->st);
+    h->st.basis_amp_one = false;
     return h->st;
 }
 
@@ -219,6 +268,7 @@ int sparq_state_destroy(sparq_state_t h) {
     return guarded([&] {
         if (!h) return;
         State& st = h->st;
+        capture_abort(st);
         release(st);
         recycle_resources(st);
         delete h;
@@ -242,13 +292,14 @@ int sparq_state_clone(sparq_state_t src, sparq_state_t* out) {
 }
 
 int sparq_state_words(sparq_state_t h, uint32_t* words) {
-    return guarded([&] { *words = S(h).W; });
+    return guarded([&] { *words = S_cap(h).W; });
 }
 
 int sparq_state_size(sparq_state_t h, uint64_t* n) {
     return guarded([&] {
-        State& st = S(h);
-        SPQ_HIP(hipStreamSynchronize(st.stream));
+        State& st = S_cap(h);
+        // while a sequence is captured the count of the plan's starting state
+        if (!st.capture) SPQ_HIP(hipStreamSynchronize(st.stream));
         *n = st.n;
     });
 }
@@ -450,7 +501,7 @@ int sparq_pointwise(sparq_state_t h, const sparq_pw_op_t* ops, uint32_t nops, co
     return guarded([&] {
         SPQ_REQUIRE(nops == 0 || ops, "null op list");
         SPQ_REQUIRE(nconds == 0 || conds, "null condition list");
-        pointwise(S(h), ops, nops, conds, nconds);
+        pointwise(S_cap(h), ops, nops, conds, nconds);
     });
 }
 
@@ -607,6 +658,10 @@ int sparq_hadamard_int(sparq_state_t h, sparq_field_t target, uint32_t n_digits,
                        double eps) {
     return guarded([&] {
         require_conditions_off_target(target, n_digits, conds, ncond);
+        if (h && h->st.capture) {
+            hadamard_captured(S_cap(h), target, n_digits, eps, conds, ncond);
+            return;
+        }
         State& st = S_keep_basis(h);
         const bool unit = st.basis_amp_one && ncond == 0;
         st.basis_amp_one = false;
@@ -617,6 +672,10 @@ int sparq_hadamard_int(sparq_state_t h, sparq_field_t target, uint32_t n_digits,
 int sparq_hadamard_int_full(sparq_state_t h, sparq_field_t target, const sparq_cond_t* conds, uint32_t ncond, double eps) {
     return guarded([&] {
         require_conditions_off_target(target, target.width, conds, ncond);
+        if (h && h->st.capture) {   // duplicate keys are flagged by the captured transform itself
+            hadamard_captured(S_cap(h), target, target.width, eps, conds, ncond);
+            return;
+        }
         State& st = S(h);
         // CheckDuplicateKey precedes the transform (hadamard.cpp:183)
         if (st.n > 1) {
@@ -730,6 +789,123 @@ int sparq_field_uniform(sparq_state_t h, sparq_field_t f, int* uniform, uint64_t
         bool u = field_uniform(st, f, &v);
         if (uniform) *uniform = u ? 1 : 0;
         if (value) *value = v;
+    });
+}
+
+int sparq_capture_plan(sparq_state_t h) {
+    return guarded([&] { capture_plan_begin(S(h)); });
+}
+
+int sparq_capture_record(sparq_state_t h) {
+    return guarded([&] {
+        State& st = S_cap(h);
+        try {
+            capture_record_begin(st);
+        } catch (...) {
+            capture_abort(st);
+            throw;
+        }
+    });
+}
+
+int sparq_capture_end(sparq_state_t h, sparq_graph_t* out) {
+    return guarded([&] {
+        SPQ_REQUIRE(out != nullptr, "null output");
+        State& st = S_cap(h);
+        hipGraph_t graph = nullptr;
+        std::unique_ptr<Capture> cap(capture_end(st, &graph));
+        auto g = std::make_unique<sparq_graph_s>();
+        g->state = h;
+        g->graph = graph;
+        g->cap = cap.get();
+        const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            throw Error(SPARQ_RUNTIME, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(e));
+        }
+        (void)hipGraphGetNodes(graph, nullptr, &g->nodes);
+        g->keys = st.keys;
+        g->amps = st.amps;
+        g->arr_cap = st.cap;
+        g->W = st.W;
+        cap.release();
+        *out = g.release();
+    });
+}
+
+int sparq_capture_abort(sparq_state_t h) {
+    return guarded([&] {
+        SPQ_REQUIRE(h != nullptr, "null state handle");
+        capture_abort(h->st);
+    });
+}
+
+int sparq_capture_phase(sparq_state_t h, int* phase) {
+    return guarded([&] {
+        SPQ_REQUIRE(h && phase, "null argument");
+        *phase = h->st.capture ? h->st.capture->phase : 0;
+    });
+}
+
+int sparq_graph_launch(sparq_graph_t g, uint32_t times) {
+    return guarded([&] {
+        SPQ_REQUIRE(g && g->state, "null graph");
+        State& st = g->state->st;
+        SPQ_REQUIRE(!st.capture, "a sequence is being captured on the graph's state");
+        SPQ_REQUIRE(st.keys == g->keys && st.amps == g->amps && st.cap == g->arr_cap && st.W == g->W,
+                    "the state's arrays changed since the sequence was captured");
+        if (st.pending_words != g->cap->words) {
+            settle(st);
+            SPQ_REQUIRE(st.n <= g->cap->capacity, "the state holds more entries than the captured capacity");
+            capture_set_count(st, g->cap->words);
+        }
+        st.basis_amp_one = false;
+        for (uint32_t t = 0; t < times; ++t) SPQ_HIP(hipGraphLaunch(g->exec, st.stream));
+        st.pending_words = g->cap->words;
+    });
+}
+
+int sparq_graph_info(sparq_graph_t g, uint64_t* capacity, uint64_t* nodes) {
+    return guarded([&] {
+        SPQ_REQUIRE(g != nullptr, "null graph");
+        if (capacity) *capacity = g->cap->capacity;
+        if (nodes) *nodes = g->nodes;
+    });
+}
+
+int sparq_graph_destroy(sparq_graph_t g) {
+    return guarded([&] {
+        if (!g) return;
+        // replays may still run on the state's stream: the device memory they use goes last
+        (void)hipDeviceSynchronize();
+        if (g->state && g->state->st.pending_words == g->cap->words) {
+            State& st = g->state->st;
+            uint64_t w[2] = {0, 0};
+            if (hipMemcpy(w, st.pending_words, sizeof(w), hipMemcpyDeviceToHost) == hipSuccess) st.n = std::min(w[kCapN], st.cap);
+            st.pending_words = nullptr;
+        }
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g->cap;
+        delete g;
+    });
+}
+
+int sparq_table_advance(sparq_state_t h, sparq_table_t t) {
+    return guarded([&] {
+        SPQ_REQUIRE(t && t->word_bytes == 8 && t->nwords >= 1, "the table needs uint64 words");
+        table_advance(S_cap(h), (uint64_t*)t->dev);
+    });
+}
+
+int sparq_table_write(sparq_table_t t, uint64_t first, const void* host_words, uint64_t count) {
+    return guarded([&] {
+        SPQ_REQUIRE(t && (host_words || count == 0), "null argument");
+        SPQ_REQUIRE(first <= t->nwords && count <= t->nwords - first, "table write out of range");
+        if (count) {
+            SPQ_HIP(hipDeviceSynchronize());   // after queued work that reads the table
+            SPQ_HIP(hipMemcpy((char*)t->dev + first * t->word_bytes, host_words, count * t->word_bytes, hipMemcpyHostToDevice));
+        }
     });
 }
 

@@ -9,6 +9,8 @@
 // WHT pass); pair ops: (8W+16) read + (8W+16) written per output entry.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <memory>
 
 #include "blockscan.hpp"
 #include "engine.hpp"
@@ -1011,6 +1013,457 @@ void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_par
     if (p.kind == SPARQ_PAIR_CONDROT_TABLE) SPQ_REQUIRE(p.table != nullptr && p.in.width <= 24, "table-driven rotation needs a table");
     if (st.n == 0) return;
     SPQ_DISPATCH_W(st.W, pair_impl, st, target, bit, p, cl);
+}
+
+// ---------------------------------------------------------------- captured transform
+// Hadamard_Int inside a captured sequence: the dense-domain transform with a
+// plan fixed when the sequence was planned.  The satisfying entries' others
+// value (the key bits outside the low target bits that vary across the planned
+// state, packed in canonical order) maps through a lookup table to a segment
+// number: one segment per value present, numbered in ascending order, segment
+// g owning slots [g << nd, (g + 1) << nd) -- for a batch of trajectories, one
+// segment per trajectory.  Every kernel reads the entry count from the device
+// (grid-stride, persistent grids); nothing is read back or allocated.  The
+// output goes back into the state's own arrays (the scatter has consumed the
+// input): the transformed entries in slot order (canonical order), then the
+// entries that fail the conditions, in their order.  What the plan assumed
+// (constant bits, segments present, no duplicate keys, the capacity) is
+// checked on the device into the capture's flags word.
+struct CapHadamard {
+    sparq_field_t target{}, low{};
+    uint32_t nd = 0;
+    double eps = 0.0;
+    uint32_t ncond = 0;
+    sparq_cond_t conds[SPARQ_MAX_PROGRAM_CONDS] = {};
+    DenseMap dm{};                          // others bits (runs); base = the constant bits' values
+    uint64_t cmask[SPARQ_MAX_WORDS] = {};   // bits outside the others and the low target bits
+    uint32_t t_o = 0;
+    uint64_t G = 0, slots = 0, slots_pad = 0, rest = 0;
+    std::vector<uint32_t> lut;              // 2^t_o: segment number, or ~0u
+    std::vector<uint64_t> gkey;             // [G][W]: the segment's key, low target bits zero
+    std::vector<uint8_t> cls;               // slots_pad >> nd: kWht for segments, kDirect for padding
+    uint32_t* d_lut = nullptr;
+    uint64_t* d_gkey = nullptr;
+    uint8_t* d_cls = nullptr;
+};
+
+struct KeyMask {
+    uint64_t m[SPARQ_MAX_WORDS];
+};
+
+constexpr int kCapTile = 4096;              // entries per tile of the scatter and rest kernels
+constexpr int kCapItems = kCapTile / kBlock;
+constexpr uint32_t kCapMaxOthers = 24;      // lookup table of at most 2^24 segment numbers
+
+static bool host_cond_ok(const uint64_t* k, const sparq_cond_t& c) {
+    const uint64_t v = (k[c.field.word] >> c.field.shift) & fmask_host(c.field.width);
+    switch (c.kind) {
+        case SPARQ_COND_NONZERO: return v != 0;
+        case SPARQ_COND_ALL_ONES: return v == fmask_host(c.field.width);
+        case SPARQ_COND_BIT: return (v >> c.value) & 1;
+        case SPARQ_COND_GE: return v >= c.value;
+        case SPARQ_COND_LE: return v <= c.value;
+        default: return v == c.value;
+    }
+}
+
+// Zero the dense domain and the op's counters.
+__global__ void __launch_bounds__(kBlock) cap_fill_kernel(double2* __restrict__ A, uint8_t* __restrict__ mark, uint64_t slots,
+                                                     uint64_t* __restrict__ words) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * kBlock) {
+        A[i] = make_double2(0.0, 0.0);
+        mark[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        words[kCapSat] = 0;
+        words[kCapMarks] = 0;
+        words[kCapRest] = 0;
+    }
+}
+
+// Entries failing the conditions, per tile of kCapTile entries (tiles past the
+// entry count give 0).
+template <int W>
+__global__ void __launch_bounds__(kBlock) cap_rest_count_kernel(const uint64_t* __restrict__ keys, uint64_t cap,
+                                                           const uint64_t* __restrict__ words, CondList cl,
+                                                           uint32_t* __restrict__ rest_cnt, uint64_t ntiles) {
+    __shared__ uint32_t lds[kBlock / 64];
+    const uint64_t n = words[kCapN];
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint32_t c = 0;
+#pragma unroll 4
+        for (int j = 0; j < kCapItems; ++j) {
+            const uint64_t i = tile * kCapTile + (uint64_t)j * kBlock + threadIdx.x;
+            if (i >= n) break;
+            uint64_t k[W];
+            load_key<W>(keys, cap, i, k);
+            bool ok = true;
+            for (uint32_t q = 0; q < cl.n; ++q) ok = ok && cond_ok<W>(k, cl.c[q]);
+            c += ok ? 0 : 1;
+        }
+        c = wave_sum(c);
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
+            rest_cnt[tile] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// Satisfying entries to their slot (and mark); the others, in order, to the
+// rest arrays at rest_off[tile] (when the op has conditions).
+template <int W>
+__global__ void __launch_bounds__(kBlock) cap_scatter_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps,
+                                                        uint64_t cap, uint64_t* __restrict__ words, DenseMap dm,
+                                                        KeyMask cmask_in, const uint32_t* __restrict__ lut,
+                                                        CondList cl, double2* __restrict__ A, uint8_t* __restrict__ mark,
+                                                        uint64_t* __restrict__ rest_keys, double2* __restrict__ rest_amps,
+                                                        uint64_t rest_cap, const uint64_t* __restrict__ rest_off, uint64_t ntiles) {
+    constexpr int kWaves = kBlock / 64;
+    __shared__ uint32_t row_wave[kCapItems * kWaves];
+    __shared__ uint32_t sat_lds[kWaves];
+    const uint64_t n = words[kCapN];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint64_t cmask[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) cmask[w] = cmask_in.m[w];
+    uint32_t sat_count = 0, bad = 0;
+    for (uint64_t tile = blockIdx.x; tile * kCapTile < n && tile < ntiles; tile += gridDim.x) {
+        uint32_t rest_flags = 0, lane_pre[kCapItems];
+#pragma unroll
+        for (int j = 0; j < kCapItems; ++j) {
+            const uint64_t i = tile * kCapTile + (uint64_t)j * kBlock + threadIdx.x;
+            bool rest = false;
+            if (i < n) {
+                uint64_t k[W];
+                load_key<W>(keys, cap, i, k);
+                bool ok = true;
+                for (uint32_t q = 0; q < cl.n; ++q) ok = ok && cond_ok<W>(k, cl.c[q]);
+                if (ok) {
+                    bool in_plan = true;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) in_plan = in_plan && ((k[w] & cmask[w]) == dm.base[w]);
+                    const uint32_t g = in_plan ? lut[dense_others<W>(k, dm)] : ~0u;
+                    if (g != ~0u) {
+                        const uint64_t slot = ((uint64_t)g << dm.nd) | fget<W>(k, dm.low);
+                        A[slot] = amps[i];
+                        mark[slot] = 1;
+                        ++sat_count;
+                    } else {
+                        bad = 1;
+                    }
+                } else {
+                    rest = true;
+                }
+            }
+            const uint64_t m = __ballot(rest);
+            rest_flags |= uint32_t(rest) << j;
+            lane_pre[j] = (uint32_t)__popcll(m & lanemask_lt());
+            if (lane == 0) row_wave[j * kWaves + wave] = (uint32_t)__popcll(m);
+        }
+        if (rest_keys) {
+            __syncthreads();
+            if (threadIdx.x < 64) {   // exclusive scan of the (row, wave) counts in entry order
+                const uint32_t x = row_wave[threadIdx.x];
+                const uint32_t inc = wave_incl_scan(x);
+                row_wave[threadIdx.x] = inc - x;
+            }
+            __syncthreads();
+            const uint64_t before = rest_off[tile];
+#pragma unroll
+            for (int j = 0; j < kCapItems; ++j) {
+                if (!((rest_flags >> j) & 1)) continue;
+                const uint64_t i = tile * kCapTile + (uint64_t)j * kBlock + threadIdx.x;
+                const uint64_t dst = before + row_wave[j * kWaves + wave] + lane_pre[j];
+                if (dst >= rest_cap) continue;
+#pragma unroll
+                for (int w = 0; w < W; ++w) rest_keys[(uint64_t)w * rest_cap + dst] = keys[(uint64_t)w * cap + i];
+                rest_amps[dst] = amps[i];
+            }
+            __syncthreads();
+        }
+    }
+    sat_count = wave_sum(sat_count);
+    if (lane == 0) sat_lds[wave] = sat_count;
+    if (__any(bad) && lane == 0) atomicOr((unsigned long long*)(words + kCapFlags), (unsigned long long)kCapOutsidePlan);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kWaves; ++w) t += sat_lds[w];
+        if (t) atomicAdd((unsigned long long*)(words + kCapSat), (unsigned long long)t);
+    }
+}
+
+// The op's output size: kept slots + rest entries, bounded by the capacity;
+// duplicate keys (fewer marks than satisfying entries) and overflow flagged.
+__global__ void cap_finish_kernel(uint64_t* __restrict__ words, const uint64_t* __restrict__ kept_total, uint64_t capacity) {
+    const uint64_t kept = kept_total ? kept_total[0] : 0;
+    if (words[kCapMarks] != words[kCapSat]) words[kCapFlags] |= kCapDuplicate;
+    uint64_t total = kept + words[kCapRest];
+    if (total > capacity) {
+        words[kCapFlags] |= kCapOverflow;
+        total = capacity;
+    }
+    words[kCapN] = total;
+}
+
+// Kept slots in slot order: key = segment key with the low target bits set.
+template <int W>
+__global__ void __launch_bounds__(kBlock) cap_emit_kernel(const double2* __restrict__ A, const uint8_t* __restrict__ cls, uint32_t nd,
+                                                     sparq_field_t low, double eps, const uint64_t* __restrict__ offs,
+                                                     const uint64_t* __restrict__ gkey, uint64_t* __restrict__ kout,
+                                                     double2* __restrict__ aout, uint64_t cap_out, uint64_t capacity) {
+    constexpr int kWaves = kBlock / 64;
+    __shared__ uint32_t row_wave[kDenseItems * kWaves];
+    const uint64_t tile = (uint64_t)blockIdx.x * kDenseTile;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double2 v[kDenseItems];
+    uint32_t flags = 0, lane_pre[kDenseItems];
+#pragma unroll
+    for (int j = 0; j < kDenseItems; ++j) {
+        const uint64_t s = tile + (uint64_t)j * kBlock + threadIdx.x;
+        const bool present = cls[s >> nd] == kWht;
+        v[j] = present ? A[s] : make_double2(0.0, 0.0);
+        const bool keep = present && !(cabs2(v[j]) < eps);
+        const uint64_t m = __ballot(keep);
+        flags |= uint32_t(keep) << j;
+        lane_pre[j] = (uint32_t)__popcll(m & lanemask_lt());
+        if (lane == 0) row_wave[j * kWaves + wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t x = threadIdx.x < kDenseItems * kWaves ? row_wave[threadIdx.x] : 0;
+        const uint32_t inc = wave_incl_scan(x);
+        if (threadIdx.x < kDenseItems * kWaves) row_wave[threadIdx.x] = inc - x;
+    }
+    __syncthreads();
+    const uint64_t before = offs[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kDenseItems; ++j) {
+        if (!((flags >> j) & 1)) continue;
+        const uint64_t s = tile + (uint64_t)j * kBlock + threadIdx.x;
+        const uint64_t dst = before + row_wave[j * kWaves + wave] + lane_pre[j];
+        if (dst >= capacity) continue;
+        const uint64_t g = s >> nd;
+        uint64_t k[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) k[w] = gkey[g * W + w];
+        fput<W>(k, low, s & fmask(nd));
+        store_key<W>(kout, cap_out, dst, k);
+        aout[dst] = v[j];
+    }
+}
+
+// The rest entries after the kept ones.
+template <int W>
+__global__ void __launch_bounds__(kBlock) cap_rest_copy_kernel(const uint64_t* __restrict__ rest_keys, const double2* __restrict__ rest_amps,
+                                                          uint64_t rest_cap, const uint64_t* __restrict__ words,
+                                                          const uint64_t* __restrict__ kept_total, uint64_t* __restrict__ kout,
+                                                          double2* __restrict__ aout, uint64_t cap_out, uint64_t capacity) {
+    const uint64_t r = words[kCapRest], kept = kept_total ? kept_total[0] : 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < r; i += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t dst = kept + i;
+        if (dst >= capacity || i >= rest_cap) break;
+#pragma unroll
+        for (int w = 0; w < W; ++w) kout[(uint64_t)w * cap_out + dst] = rest_keys[(uint64_t)w * rest_cap + i];
+        aout[dst] = rest_amps[i];
+    }
+}
+
+// Phase 1: the plan of one op from the snapshot.
+static CapHadamard* plan_hadamard(const Capture& c, sparq_field_t target, uint32_t nd, double eps, const sparq_cond_t* conds,
+                                  uint32_t ncond) {
+    const uint32_t W = c.W;
+    auto* p = new CapHadamard();
+    std::unique_ptr<CapHadamard> guard(p);
+    p->target = target;
+    p->low = sparq_field_t{target.word, target.shift, (uint8_t)nd, 0};
+    p->nd = nd;
+    p->eps = eps;
+    p->ncond = ncond;
+    for (uint32_t q = 0; q < ncond; ++q) p->conds[q] = conds[q];
+    const uint64_t lowmask = fmask_host(nd) << target.shift;
+    // the key bits that vary among the entries the op transforms, and their OR
+    std::vector<uint64_t> k(W), any(W, 0), all(W, ~0ull);
+    uint64_t sat = 0;
+    auto satisfies = [&](uint64_t i) {
+        for (uint32_t w = 0; w < W; ++w) k[w] = c.snap[(uint64_t)w * c.n0 + i];
+        bool ok = true;
+        for (uint32_t q = 0; q < ncond && ok; ++q) ok = host_cond_ok(k.data(), conds[q]);
+        return ok;
+    };
+    for (uint64_t i = 0; i < c.n0; ++i) {
+        if (!satisfies(i)) continue;
+        ++sat;
+        for (uint32_t w = 0; w < W; ++w) {
+            any[w] |= k[w];
+            all[w] &= k[w];
+        }
+    }
+    std::vector<uint64_t> vary(W, 0);
+    for (uint32_t w = 0; w < W && sat; ++w) vary[w] = any[w] ^ all[w];
+    DenseMap& dm = p->dm;
+    dm.nd = nd;
+    dm.low = p->low;
+    for (int w = (int)W - 1; w >= 0; --w) {   // least significant word first, bit 0 up
+        const uint64_t o = vary[w] & ((uint32_t)w == target.word ? ~lowmask : ~0ull);
+        for (int b = 0; b < 64; ++b) {
+            if (!((o >> b) & 1)) continue;
+            ++p->t_o;
+            SPQ_REQUIRE(p->t_o <= kCapMaxOthers, "captured Hadamard_Int: more than 24 key bits vary outside its target");
+            if (dm.nseg > 0 && dm.word[dm.nseg - 1] == w && dm.shift[dm.nseg - 1] + dm.width[dm.nseg - 1] == b) {
+                ++dm.width[dm.nseg - 1];
+                continue;
+            }
+            dm.word[dm.nseg] = (uint8_t)w;
+            dm.shift[dm.nseg] = (uint8_t)b;
+            dm.width[dm.nseg] = 1;
+            ++dm.nseg;
+        }
+    }
+    for (uint32_t w = 0; w < W; ++w) {
+        p->cmask[w] = ~vary[w] & (w == target.word ? ~lowmask : ~0ull);
+        dm.base[w] = any[w] & p->cmask[w];
+    }
+    auto others = [&](const uint64_t* k) {
+        uint64_t v = 0;
+        uint32_t at = 0;
+        for (uint32_t s = 0; s < dm.nseg; ++s) {
+            v |= ((k[dm.word[s]] >> dm.shift[s]) & fmask_host(dm.width[s])) << at;
+            at += dm.width[s];
+        }
+        return v;
+    };
+    p->lut.assign(size_t(1) << p->t_o, 0u);
+    for (uint64_t i = 0; i < c.n0; ++i)
+        if (satisfies(i)) p->lut[others(k.data())] = 1;
+    for (uint64_t o = 0; o < p->lut.size(); ++o) {
+        if (!p->lut[o]) {
+            p->lut[o] = ~0u;
+            continue;
+        }
+        p->lut[o] = (uint32_t)p->G++;
+        for (uint32_t w = 0; w < W; ++w) p->gkey.push_back(dm.base[w]);
+        uint64_t* gk = p->gkey.data() + (p->G - 1) * W;
+        uint32_t at = 0;
+        for (uint32_t s = 0; s < dm.nseg; ++s) {
+            gk[dm.word[s]] |= ((o >> at) & fmask_host(dm.width[s])) << dm.shift[s];
+            at += dm.width[s];
+        }
+    }
+    SPQ_REQUIRE(p->G <= (uint64_t(1) << (32 - std::min<uint32_t>(nd, 32))) && nd <= 30,
+                "captured Hadamard_Int: domain beyond 2^32 slots");
+    p->slots = p->G << nd;
+    p->slots_pad = (p->slots + kDenseTile - 1) / kDenseTile * kDenseTile;
+    SPQ_REQUIRE(p->slots_pad <= (uint64_t(1) << 32), "captured Hadamard_Int: domain beyond 2^32 slots");
+    p->cls.assign(p->slots_pad >> nd, (uint8_t)kDirect);
+    for (uint64_t g = 0; g < p->G; ++g) p->cls[g] = kWht;
+    p->rest = c.n0 - sat;
+    return guard.release();
+}
+
+uint64_t capture_plan_bytes(const Capture& c) {
+    uint64_t b = 0;
+    for (const CapHadamard* p : c.hs) {
+        b += (p->lut.size() * 4 + 255) / 256 * 256;
+        b += (p->gkey.size() * 8 + 255) / 256 * 256;
+        b += (p->cls.size() + 255) / 256 * 256;
+    }
+    return b;
+}
+
+void capture_plan_upload(Capture& c, char*& at) {
+    auto put = [&](const void* src, size_t bytes) {
+        void* dst = at;
+        if (bytes) SPQ_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        at += (bytes + 255) / 256 * 256;
+        return dst;
+    };
+    for (CapHadamard* p : c.hs) {
+        p->d_lut = (uint32_t*)put(p->lut.data(), p->lut.size() * 4);
+        p->d_gkey = (uint64_t*)put(p->gkey.data(), p->gkey.size() * 8);
+        p->d_cls = (uint8_t*)put(p->cls.data(), p->cls.size());
+    }
+}
+
+uint64_t capture_plan_slots(const CapHadamard& p) { return p.slots; }
+uint64_t capture_plan_slots_padded(const CapHadamard& p) { return p.slots_pad; }
+uint64_t capture_plan_rest(const CapHadamard& p) { return p.rest; }
+bool capture_plan_has_conditions(const CapHadamard& p) { return p.ncond > 0; }
+
+void capture_plan_free(Capture& c) {
+    for (CapHadamard* p : c.hs) delete p;
+    c.hs.clear();
+}
+
+// Phase 2: the op's kernels on the capturing stream.
+template <int W>
+static void record_hadamard(State& st, const CapHadamard& p) {
+    Capture& c = *st.capture;
+    hipStream_t s = st.stream;
+    uint64_t* words = c.words;
+    CondList cl{};
+    cl.n = p.ncond;
+    for (uint32_t q = 0; q < p.ncond; ++q) cl.c[q] = p.conds[q];
+    const bool rest = p.ncond > 0;
+    const uint64_t ntiles = (c.capacity + kCapTile - 1) / kCapTile;
+    cap_fill_kernel<<<persistent_grid(std::max<uint64_t>(p.slots_pad, 1), kBlock), kBlock, 0, s>>>(c.A, c.mark, p.slots_pad, words);
+    SPQ_HIP(hipGetLastError());
+    if (rest) {
+        cap_rest_count_kernel<W><<<persistent_grid(c.capacity, kCapTile), kBlock, 0, s>>>(st.keys, st.cap, words, cl, c.rest_cnt, ntiles);
+        SPQ_HIP(hipGetLastError());
+        exclusive_scan_u32_scratch(s, c.rest_cnt, c.rest_off, ntiles, words + kCapRest, c.scan_scratch, c.scan_words);
+    }
+    KeyMask cm{};
+    for (uint32_t w = 0; w < st.W; ++w) cm.m[w] = p.cmask[w];
+    cap_scatter_kernel<W><<<persistent_grid(c.capacity, kCapTile), kBlock, 0, s>>>(
+        st.keys, st.amps, st.cap, words, p.dm, cm, p.d_lut, cl, c.A, c.mark, rest ? c.rest_keys : nullptr,
+        c.rest_amps, c.capacity, c.rest_off, ntiles);
+    SPQ_HIP(hipGetLastError());
+    const uint64_t stiles = p.slots_pad / kDenseTile;
+    if (p.G > 0) {
+        launch_wht(st, c.A, nullptr, p.G, p.nd, nullptr, (double)p.slots * 32.0);
+        dense_count_kernel<<<(unsigned)stiles, kBlock, 0, s>>>(c.A, c.mark, p.d_cls, p.nd, p.eps, c.kept,
+                                                               (unsigned long long*)(words + kCapMarks));
+        SPQ_HIP(hipGetLastError());
+        exclusive_scan_u32_scratch(s, c.kept, c.offs, stiles, c.offs + stiles, c.scan_scratch, c.scan_words);
+    }
+    const uint64_t* kept_total = p.G > 0 ? c.offs + stiles : nullptr;
+    cap_finish_kernel<<<1, 1, 0, s>>>(words, kept_total, c.capacity);
+    SPQ_HIP(hipGetLastError());
+    if (p.G > 0) {
+        cap_emit_kernel<W><<<(unsigned)stiles, kBlock, 0, s>>>(c.A, p.d_cls, p.nd, p.low, p.eps, c.offs, p.d_gkey, st.keys, st.amps, st.cap,
+                                                             c.capacity);
+        SPQ_HIP(hipGetLastError());
+    }
+    if (rest) {
+        cap_rest_copy_kernel<W><<<persistent_grid(c.capacity, kBlock), kBlock, 0, s>>>(c.rest_keys, c.rest_amps, c.capacity, words,
+                                                                                     kept_total, st.keys, st.amps, st.cap, c.capacity);
+        SPQ_HIP(hipGetLastError());
+    }
+}
+
+void hadamard_captured(State& st, sparq_field_t target, uint32_t nd, double eps, const sparq_cond_t* conds, uint32_t ncond) {
+    Capture& c = *st.capture;
+    SPQ_REQUIRE(target.word < st.W && target.width >= 1 && (uint32_t)target.shift + target.width <= 64, "bad target field");
+    SPQ_REQUIRE(nd >= 1 && nd <= target.width, "n_digits must be in [1, width]");
+    SPQ_REQUIRE(eps > 0.0, "a captured Hadamard_Int needs a positive ClearZero threshold");
+    SPQ_REQUIRE(ncond <= SPARQ_MAX_PROGRAM_CONDS, "too many conditions");
+    if (c.phase == 1) {
+        c.hs.push_back(plan_hadamard(c, target, nd, eps, conds, ncond));
+        return;
+    }
+    SPQ_REQUIRE(c.next_h < c.hs.size(), "the captured sequence runs more branching operators than its plan");
+    const CapHadamard& p = *c.hs[c.next_h++];
+    bool same = std::memcmp(&p.target, &target, sizeof(target)) == 0 && p.nd == nd && p.eps == eps && p.ncond == ncond;
+    for (uint32_t q = 0; same && q < ncond; ++q)
+        same = p.conds[q].kind == conds[q].kind && std::memcmp(&p.conds[q].field, &conds[q].field, sizeof(sparq_field_t)) == 0 &&
+               p.conds[q].value == conds[q].value;
+    SPQ_REQUIRE(same, "the captured sequence differs from its plan");
+    SPQ_DISPATCH_W(st.W, record_hadamard, st, p);
 }
 
 }  // namespace sparq

@@ -68,6 +68,8 @@ void ensure_device();
 constexpr int kBlock = 256;           // threads per workgroup (4 waves of 64)
 constexpr int kWave = 64;
 
+struct Capture;
+
 struct State {
     uint32_t W = 1;
     uint32_t Wcap = 0;          // key words the allocation holds when more than W (0: exactly W)
@@ -84,6 +86,11 @@ struct State {
     // device arrays have been handed to a caller, who may write them directly.
     bool basis_amp_one = false;
     bool arrays_escaped = false;
+    // Graph capture in progress on this state (capture.hip), or null.
+    Capture* capture = nullptr;
+    // Replays of a captured sequence are queued and n is on the device until
+    // the next access settles it: {entry count, violation flags} (capture.hip).
+    const uint64_t* pending_words = nullptr;
 };
 
 // ---- memory (caching device allocator)
@@ -244,6 +251,75 @@ void tile_abs2(State& st, uint64_t tile, double* host_out, uint64_t ntiles);
 double cond_abs2(State& st, const sparq_cond_t* conds, uint32_t ncond);
 void scale_amps(State& st, double s);
 bool field_uniform(State& st, sparq_field_t f, uint64_t* value);
+
+// ---- graph capture (capture.hip)
+// A fixed operator sequence is recorded once on a state and replayed as a HIP
+// graph.  Phase 1 (plan) runs the sequence without launching anything: each
+// branching operator plans its transform from a host snapshot of the state
+// (the key bits that vary, the segments its conditions select).  Phase 2
+// (capture) runs it again while the state's stream is capturing: every kernel
+// reads the entry count from device memory and grid-strides over it, so no
+// size crosses to the host and nothing is allocated.  What the plan assumed is
+// checked on the device into a flags word, read when the replays settle.
+enum CaptureFlag : uint64_t {
+    kCapOutsidePlan = 1,     // an entry of a branching op fell outside its planned segments or constant bits
+    kCapDuplicate = 2,       // duplicate keys in a branching op's domain
+    kCapOverflow = 4,        // more entries than the captured capacity
+    kCapNonzeroRemoved = 8,  // a removed register was not zero
+};
+// Device words of a capture: entry count, flags, then per-branching-op counters.
+enum CaptureWord { kCapN = 0, kCapFlags = 1, kCapSat = 2, kCapMarks = 3, kCapRest = 4, kCapWords = 8 };
+struct CapHadamard;   // one planned Hadamard_Int (branch.hip)
+struct Capture {
+    int phase = 1;                        // 1 plan, 2 capture
+    uint32_t W = 1;
+    uint64_t n0 = 0;                      // entries when the plan began
+    std::vector<uint64_t> snap;           // phase 1: the state's keys, word-major [W][n0]
+    std::vector<CapHadamard*> hs;         // planned branching ops, in sequence order
+    size_t next_h = 0;                    // phase 2: the next planned op to record
+    uint64_t capacity = 0;                // entries the sequence may hold (the state's arrays hold as many)
+    // phase 2 device memory: one allocation, owned by the graph once captured
+    char* arena = nullptr;
+    uint64_t* words = nullptr;            // kCapWords words (CaptureWord)
+    double2* A = nullptr;                 // dense amplitudes of the largest planned domain
+    uint8_t* mark = nullptr;
+    uint32_t* kept = nullptr;             // kept slots per slot tile
+    uint64_t* offs = nullptr;             // their exclusive scan (+ total)
+    uint32_t* rest_cnt = nullptr;         // entries failing a branching op's conditions, per entry tile
+    uint64_t* rest_off = nullptr;
+    uint64_t* rest_keys = nullptr;        // [W][capacity]
+    double2* rest_amps = nullptr;
+    uint64_t* scan_scratch = nullptr;
+    uint64_t scan_words = 0;
+    uint64_t max_slots = 0;               // largest planned domain (slots, padded to whole slot tiles)
+    bool any_conditions = false;
+    ~Capture();
+};
+inline uint64_t* capture_words(State& st) { return st.capture ? st.capture->words : nullptr; }
+// Branching ops in a capture (branch.hip): plan (phase 1) or record (phase 2).
+void hadamard_captured(State& st, sparq_field_t target, uint32_t nd, double eps, const sparq_cond_t* conds, uint32_t ncond);
+// Device memory and tables of the planned ops (phase 1 -> 2), sizes in bytes.
+uint64_t capture_plan_bytes(const Capture& c);
+void capture_plan_upload(Capture& c, char*& at);
+void capture_plan_free(Capture& c);
+uint64_t capture_plan_slots(const CapHadamard& p);          // planned domain (segments << digits)
+uint64_t capture_plan_slots_padded(const CapHadamard& p);   // ... in whole slot tiles
+uint64_t capture_plan_rest(const CapHadamard& p);           // planned entries failing its conditions
+bool capture_plan_has_conditions(const CapHadamard& p);
+// Lifecycle (capture.hip): plan, record (starts stream capture), end (returns
+// the capture, which the graph keeps), abort.
+void capture_plan_begin(State& st);
+void capture_record_begin(State& st);
+Capture* capture_end(State& st, hipGraph_t* graph);
+void capture_abort(State& st);
+// Queue "entry count := st.n" on the state's stream (before replays).
+void capture_set_count(State& st, uint64_t* words);
+// word += 1 on the state's stream (capturable; nothing while planning).
+void table_advance(State& st, uint64_t* word);
+// Grid of a persistent grid-stride kernel over up to `cap` items of `per_block`.
+unsigned persistent_grid(uint64_t cap, unsigned per_block);
+// Graph capture is active somewhere in the process (profiling events stay out of it).
+bool capture_active();
 
 // ---- profiling (prof.hip)
 bool prof_enabled();

@@ -83,6 +83,26 @@ struct Tridiag {
     }
 };
 
+// One multiplication by the phase per set bit of `bits`, lowest first, as one
+// Phase_Bool per digit does (basic_gates.cpp:11-31).  For a phase (-1, s), Z's
+// (cos pi, sin pi), a*(-1, s) = (-a.x - a.y*s, a.x*s - a.y) is the same IEEE
+// result as the general complex product (x*(-1) is exact and x + (-y) ==
+// x - y) with two fewer multiplications.
+__device__ __forceinline__ void phase_per_bit(double2& amp, uint64_t bits, const double2 ph) {
+    if (ph.x == -1.0) {
+        const double sn = ph.y;
+        while (bits) {
+            amp = make_double2(-amp.x - amp.y * sn, amp.x * sn - amp.y);
+            bits &= bits - 1;
+        }
+    } else {
+        while (bits) {
+            amp = cmul(amp, ph);
+            bits &= bits - 1;
+        }
+    }
+}
+
 // LIGHT: the heavy opcodes (is_heavy_opcode) compile to nothing, so a program
 // of light opcodes runs with far fewer registers per lane.
 template <int W, bool LIGHT>
@@ -301,25 +321,19 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
                 fput<W>(k, f3, k1 ^ (hit ? row_addr + (k2 - base) : 0));
             }
             break;
-        case SPARQ_PW_PHASE_BITS: {
-            // One multiplication by the phase per set bit, as one Phase_Bool per
-            // digit does (basic_gates.cpp:11-31).  For a phase (-1, s), Z's
-            // (cos pi, sin pi), a*(-1, s) = (-a.x - a.y*s, a.x*s - a.y) is the
-            // same IEEE result as the general complex product (x*(-1) is exact
-            // and x + (-y) == x - y) with two fewer multiplications.
-            uint64_t bits = fget<W>(k, f0) & op.c[0];
-            const double2 ph = make_double2(op.d[0], op.d[1]);
-            if (ph.x == -1.0) {
-                const double sn = ph.y;
-                while (bits) {
-                    amp = make_double2(-amp.x - amp.y * sn, amp.x * sn - amp.y);
-                    bits &= bits - 1;
-                }
-            } else {
-                while (bits) {
-                    amp = cmul(amp, ph);
-                    bits &= bits - 1;
-                }
+        case SPARQ_PW_PHASE_BITS: phase_per_bit(amp, fget<W>(k, f0) & op.c[0], make_double2(op.d[0], op.d[1])); break;
+        case SPARQ_PW_ASSERT_ZERO:
+            if (fget<W>(k, f0) != 0) atomicOr((unsigned long long*)op.table, (unsigned long long)kCapNonzeroRemoved);
+            fput<W>(k, f0, 0);
+            break;
+        case SPARQ_PW_PAULI_TABLE: {
+            const uint64_t* t = (const uint64_t*)op.table;
+            const uint64_t it = t[0], s = fget<W>(k, f0);
+            if (it < op.c[1] && s < op.c[0]) {
+                const uint64_t row = 1 + 2 * (it * op.c[0] + s);
+                const uint64_t v = fget<W>(k, f1) ^ t[row];
+                fput<W>(k, f1, v);
+                phase_per_bit(amp, v & t[row + 1], make_double2(op.d[0], op.d[1]));
             }
             break;
         }
@@ -376,15 +390,19 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
 template <int W>
 constexpr int kPwItems() { return (W == 3 || W == 4) ? 2 : 1; }
 
-template <int W, bool AMP, bool LIGHT>
+// DEV: the entry count is read from n_dev and the grid strides over it (a
+// persistent grid; captured sequences); otherwise one round over n.
+template <int W, bool AMP, bool LIGHT, bool DEV>
 __global__ void __launch_bounds__(kBlock) pointwise_kernel(uint64_t* __restrict__ keys, double2* __restrict__ amps, uint64_t cap,
-                                                      uint64_t n, const PwProgram prog) {
+                                                      uint64_t n_arg, const uint64_t* __restrict__ n_dev, const PwProgram prog) {
     // kPwItems(W) entries per thread (striped by the block size).  One entry
     // per thread for one- and two-word keys: the interpreter's registers for
     // several entries cut the waves per SIMD more than the extra gathers in
     // flight gain (23 vs 27 us for the 2^20 QRAM-load + 32-phase program).
     constexpr int kI = kPwItems<W>();
-    const uint64_t base = (uint64_t)blockIdx.x * kBlock * kI + threadIdx.x;
+    const uint64_t n = DEV ? n_dev[0] : n_arg;
+    for (uint64_t blk = blockIdx.x; blk * kBlock * kI < n; blk += gridDim.x) {
+    const uint64_t base = blk * kBlock * kI + threadIdx.x;
     uint64_t k[kI][W];
     double2 a[kI];
 #pragma unroll
@@ -415,6 +433,8 @@ __global__ void __launch_bounds__(kBlock) pointwise_kernel(uint64_t* __restrict_
         store_key<W>(keys, cap, i, k[it]);
         if (AMP) amps[i] = a[it];
     }
+    if (!DEV) break;
+    }
 }
 
 // Opcodes with 64-bit division, square roots, arc cosines or table searches:
@@ -434,25 +454,35 @@ static bool is_heavy_opcode(uint32_t opcode) {
 static bool touches_amp(uint32_t opcode) {
     switch (opcode) {
         case SPARQ_PW_PHASE_BITS: case SPARQ_PW_RZ: case SPARQ_PW_Y: case SPARQ_PW_ROT_DIAG:
-        case SPARQ_PW_ROT_OFFDIAG: case SPARQ_PW_NEG_AMP: case SPARQ_PW_MUL_AMP:
+        case SPARQ_PW_ROT_OFFDIAG: case SPARQ_PW_NEG_AMP: case SPARQ_PW_MUL_AMP: case SPARQ_PW_PAULI_TABLE:
             return true;
         default:
             return false;
     }
 }
 
-template <int W>
-static void launch_pointwise(State& st, const PwProgram& prog, bool amp, bool light) {
-    unsigned g = grid_for(st.n, kBlock * kPwItems<W>());
-    SPQ_PROF(st.stream, "pointwise_kernel", st.n * (16.0 * W + (amp ? 32.0 : 0.0)));
+template <int W, bool DEV>
+static void launch_pointwise_impl(State& st, const PwProgram& prog, bool amp, bool light, unsigned g, const uint64_t* n_dev) {
     if (light) {
-        if (amp) pointwise_kernel<W, true, true><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
-        else pointwise_kernel<W, false, true><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
+        if (amp) pointwise_kernel<W, true, true, DEV><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
+        else pointwise_kernel<W, false, true, DEV><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
     } else {
-        if (amp) pointwise_kernel<W, true, false><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
-        else pointwise_kernel<W, false, false><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, prog);
+        if (amp) pointwise_kernel<W, true, false, DEV><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
+        else pointwise_kernel<W, false, false, DEV><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
     }
     SPQ_HIP(hipGetLastError());
+}
+
+template <int W>
+static void launch_pointwise(State& st, const PwProgram& prog, bool amp, bool light) {
+    if (st.capture) {   // recording a captured sequence: the count is on the device
+        const uint64_t* n_dev = capture_words(st) + kCapN;
+        launch_pointwise_impl<W, true>(st, prog, amp, light, persistent_grid(st.capture->capacity, kBlock * kPwItems<W>()), n_dev);
+        return;
+    }
+    unsigned g = grid_for(st.n, kBlock * kPwItems<W>());
+    SPQ_PROF(st.stream, "pointwise_kernel", st.n * (16.0 * W + (amp ? 32.0 : 0.0)));
+    launch_pointwise_impl<W, false>(st, prog, amp, light, g, nullptr);
 }
 
 static void check_field(const State& st, sparq_field_t f) {
@@ -479,6 +509,11 @@ void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_c
             SPQ_REQUIRE(kw >= 1 && kw <= 64, "tridiagonal oracle: bad column width");
         }
         if (op.opcode == SPARQ_PW_XOR_QW_ANGLE) SPQ_REQUIRE(op.c[0] >= 2 && op.c[0] <= 63, "walk angle: data size out of range");
+        if (op.opcode == SPARQ_PW_ASSERT_ZERO) SPQ_REQUIRE(st.capture, "the zero assertion runs only inside a captured sequence");
+        if (op.opcode == SPARQ_PW_PAULI_TABLE) {
+            SPQ_REQUIRE(op.table != nullptr, "Pauli noise without table");
+            SPQ_REQUIRE(op.c[0] >= 1 && op.c[1] >= 1, "Pauli noise table needs rows and iterations");
+        }
         if (op.opcode == SPARQ_PW_QRAM_LOAD || op.opcode == SPARQ_PW_XOR_BSEARCH) {
             SPQ_REQUIRE(op.table != nullptr, "QRAM load without table");
             SPQ_REQUIRE(op.table_word_bytes == 1 || op.table_word_bytes == 2 || op.table_word_bytes == 4 ||
@@ -487,12 +522,14 @@ void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_c
         amp = amp || touches_amp(op.opcode);
         light = light && !is_heavy_opcode(op.opcode);
         prog.ops[o] = op;
+        if (op.opcode == SPARQ_PW_ASSERT_ZERO && capture_words(st)) prog.ops[o].table = capture_words(st) + kCapFlags;
     }
     for (uint32_t c = 0; c < nconds; ++c) {
         check_field(st, conds[c].field);
         prog.conds[c] = conds[c];
     }
-    if (st.n == 0 || nops == 0) return;
+    if (st.capture && st.capture->phase == 1) return;   // planning: nothing runs
+    if ((st.n == 0 && !st.capture) || nops == 0) return;
     SPQ_DISPATCH_W(st.W, launch_pointwise, st, prog, amp, light);
 }
 

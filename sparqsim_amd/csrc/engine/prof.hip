@@ -52,7 +52,7 @@ void drain() {
 }
 }  // namespace
 
-bool prof_enabled() { return g_on; }
+bool prof_enabled() { return g_on && !capture_active(); }
 
 void prof_begin(hipStream_t s, const char* name, double bytes) {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -274,6 +274,30 @@ PYBIND11_MODULE(_core, m) {
         .def(py::init<const QRAMCircuit_qutrit*, size_t, size_t>(), py::keep_alive<1, 2>());
     m.attr("QRAMLoadFast") = m.attr("QRAMLoad");
 
+    // captured sequences and the table-driven noise channel (no counterpart in the reference)
+    py::class_<CapturedSequence>(m, "CapturedSequence")
+        .def(py::init([](SparseState& st, py::object body) {
+                 return std::make_unique<CapturedSequence>(
+                     st, [&body](SparseState& s) { body(py::cast(&s, py::return_value_policy::reference)); });
+             }),
+             py::arg("state"), py::arg("body"), py::keep_alive<1, 2>())
+        .def("launch", &CapturedSequence::launch, py::arg("times") = 1, Release())
+        .def("sync", &CapturedSequence::sync, Release())
+        .def_property_readonly("capacity", &CapturedSequence::capacity)
+        .def_property_readonly("nodes", &CapturedSequence::nodes);
+    py::class_<PauliNoiseTable, std::shared_ptr<PauliNoiseTable>>(m, "PauliNoiseTable")
+        .def(py::init<size_t, size_t, const std::vector<uint64_t>&, const std::vector<uint64_t>&>(), py::arg("rows"),
+             py::arg("iterations"), py::arg("x_masks"), py::arg("z_masks"))
+        .def("set_iteration", &PauliNoiseTable::set_iteration, py::arg("k"), Release())
+        .def_readonly("rows", &PauliNoiseTable::rows)
+        .def_readonly("iterations", &PauliNoiseTable::iterations);
+    pw_class<PauliNoise>(m, "PauliNoise")
+        .def(py::init<std::string_view, std::string_view, const PauliNoiseTable&>(), py::arg("segment"), py::arg("target"),
+             py::arg("table"))
+        .def(py::init<size_t, size_t, const PauliNoiseTable&>(), py::arg("segment"), py::arg("target"), py::arg("table"));
+    py::class_<AdvancePauliNoise, BaseOperator, std::shared_ptr<AdvancePauliNoise>>(m, "AdvancePauliNoise")
+        .def(py::init<const PauliNoiseTable&>(), py::arg("table"));
+
     // branching
     auto hi = py::class_<Hadamard_Int, SelfAdjointOperator, std::shared_ptr<Hadamard_Int>>(m, "Hadamard_Int")
         .def(py::init<std::string_view, size_t>(), py::arg("reg_in"), py::arg("n_digits"))

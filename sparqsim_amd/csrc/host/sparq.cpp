@@ -1103,6 +1103,94 @@ void TestRemovable::operator()(SparseState& state) const {   // debugger.cpp:191
     if (!uni) throw std::runtime_error("Removing failed: register " + System::name_of(register_id) + " is not uniform");
 }
 
+// ------------------------------------------------------------ captured sequences
+// Same key width and the same field for every active register (the register
+// table keeps removed entries, so the id ranges may differ).
+static bool same_active_fields(const Layout& a, const Layout& b) {
+    if (a.words != b.words) return false;
+    const size_t n = std::max(a.field.size(), b.field.size());
+    for (size_t i = 0; i < n; ++i) {
+        const sparq_field_t fa = i < a.field.size() ? a.field[i] : sparq_field_t{0, 0, 0, 0};
+        const sparq_field_t fb = i < b.field.size() ? b.field[i] : sparq_field_t{0, 0, 0, 0};
+        if (fa.width == 0 && fb.width == 0) continue;
+        if (std::memcmp(&fa, &fb, sizeof(sparq_field_t)) != 0) return false;
+    }
+    return true;
+}
+
+CapturedSequence::CapturedSequence(SparseState& state, const std::function<void(SparseState&)>& body) : st_(&state) {
+    const Layout before = state.sync();
+    state.flush();
+    check(sparq_capture_plan(state.handle()));
+    auto run = [&](const char* what) {
+        body(state);
+        state.flush();
+        if (!same_active_fields(state.sync(), before))
+            throw std::invalid_argument(std::string("CapturedSequence: the sequence must leave the register layout as it found it (") +
+                                        what + ")");
+    };
+    try {
+        run("planning");
+        check(sparq_capture_record(state.handle()));
+        run("recording");
+        check(sparq_capture_end(state.handle(), &g_));
+    } catch (...) {
+        sparq_capture_abort(state.handle());
+        throw;
+    }
+}
+CapturedSequence::~CapturedSequence() {
+    if (g_) sparq_graph_destroy(g_);
+}
+void CapturedSequence::launch(size_t times) {
+    st_->flush();
+    check(sparq_graph_launch(g_, (uint32_t)times));
+}
+void CapturedSequence::sync() { (void)st_->size(); }
+uint64_t CapturedSequence::capacity() const {
+    uint64_t c = 0;
+    check(sparq_graph_info(g_, &c, nullptr));
+    return c;
+}
+uint64_t CapturedSequence::nodes() const {
+    uint64_t n = 0;
+    check(sparq_graph_info(g_, nullptr, &n));
+    return n;
+}
+
+PauliNoiseTable::PauliNoiseTable(size_t r, size_t it, const std::vector<uint64_t>& x, const std::vector<uint64_t>& z)
+    : rows(r), iterations(it) {
+    if (r == 0 || it == 0) throw std::invalid_argument("PauliNoiseTable: rows and iterations must be positive");
+    if (x.size() != r * it || z.size() != r * it) throw std::invalid_argument("PauliNoiseTable: need rows * iterations masks");
+    std::vector<uint64_t> words(1 + 2 * r * it, 0);
+    for (size_t i = 0; i < r * it; ++i) {
+        words[1 + 2 * i] = x[i];
+        words[2 + 2 * i] = z[i];
+    }
+    sparq_table_t t = nullptr;
+    check(sparq_table_create(words.data(), words.size(), 8, &t));
+    table = std::shared_ptr<sparq_table_s>(t, [](sparq_table_s* p) { sparq_table_destroy(p); });
+}
+void PauliNoiseTable::set_iteration(uint64_t k) { check(sparq_table_write(table.get(), 0, &k, 1)); }
+
+PauliNoise::PauliNoise(size_t segment, size_t target, const PauliNoiseTable& t) : keep(t.table) {
+    require_ids({segment, target});
+    set_op(*this, "PauliNoise", SPARQ_PW_PAULI_TABLE, {segment, target});
+    c[0] = t.rows;
+    c[1] = t.iterations;
+    d[0] = std::cos(pi);   // Z_Bool's phase (Phase_Bool with lambda = pi)
+    d[1] = std::sin(pi);
+    const void* ptr = nullptr;
+    check(sparq_table_device_ptr(t.table.get(), &ptr));
+    table = ptr;
+    table_word_bytes = 8;
+    has_dag = false;
+}
+void AdvancePauliNoise::operator()(SparseState& state) const {
+    state.flush();   // after the noise it follows
+    check(sparq_table_advance(state.handle(), table.get()));
+}
+
 AddRegister::AddRegister(std::string_view n, StateStorageType t, size_t s) : register_name(n), type(t), size(s) {
     if (size > 64) throw std::invalid_argument("register wider than 64 bits");   // system_operations.cpp:235-245
 }
@@ -1122,12 +1210,24 @@ size_t AddRegisterWithHadamard::operator()(SparseState& state) const {   // syst
 void RemoveRegister::operator()(SparseState& state) const {   // system_operations.cpp:303-322
     const Layout& L = state.sync();
     if (register_id >= System::name_register_map.size()) throw std::runtime_error("Register not found.");
-    state.flush();
-    int uni = 1;
-    uint64_t v = 0;
-    check(sparq_field_uniform(state.handle(), L.at(register_id), &uni, &v));
-    if (!uni) throw std::runtime_error("Removing failed: register " + System::name_of(register_id) + " is not uniform");
-    if (v == 0) state.mark_zero(register_id);   // its bits need no clearing when the field is dropped
+    int phase = 0;
+    check(sparq_capture_phase(state.handle(), &phase));
+    if (phase) {
+        // Inside a captured sequence the register must be zero: checked (and
+        // cleared) on the device instead of read back.
+        sparq_pw_op_t op{};
+        op.opcode = SPARQ_PW_ASSERT_ZERO;
+        op.f[0] = L.at(register_id);
+        state.push_pointwise(op, {});
+        state.mark_zero(register_id);
+    } else {
+        state.flush();
+        int uni = 1;
+        uint64_t v = 0;
+        check(sparq_field_uniform(state.handle(), L.at(register_id), &uni, &v));
+        if (!uni) throw std::runtime_error("Removing failed: register " + System::name_of(register_id) + " is not uniform");
+        if (v == 0) state.mark_zero(register_id);   // its bits need no clearing when the field is dropped
+    }
     System::remove_register(register_id);
     state.sync();
 }

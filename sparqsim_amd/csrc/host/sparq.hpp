@@ -276,6 +276,65 @@ struct PointwiseOperator : BaseOperator, Controllable {
     void emit(SparseState& state, bool dagger) const;
 };
 
+// ------------------------------------------------------------ captured sequences
+// A fixed operator sequence recorded once on one state and replayed as a HIP
+// graph with no host synchronisation (no counterpart in the reference, whose
+// callers loop over operators and wait after every branching one).  The body
+// runs twice: once to plan each branching operator from a snapshot of the
+// state, once to record the kernels.  It must leave the register layout as it
+// found it and use only per-entry operators, Hadamard_Int / Hadamard_Int_Full,
+// register add/remove that keeps fields in place, PauliNoise and
+// AdvancePauliNoise.  Replays assume what the plan saw (the key bits that vary
+// outside each Hadamard's target, its segments, no duplicate keys); a replay
+// that breaks it is reported by the next access to the state.  The state must
+// outlive the sequence, and its arrays must not be reallocated between
+// launches (an eager branching operator does).
+class CapturedSequence {
+public:
+    CapturedSequence(SparseState& state, const std::function<void(SparseState&)>& body);
+    ~CapturedSequence();
+    CapturedSequence(const CapturedSequence&) = delete;
+    CapturedSequence& operator=(const CapturedSequence&) = delete;
+    void launch(size_t times = 1);   // queue replays; no synchronisation
+    void sync();                     // wait for them and check their assumptions
+    uint64_t capacity() const;       // entries the sequence may hold
+    uint64_t nodes() const;          // kernels (graph nodes) per replay
+    SparseState& state() { return *st_; }
+
+private:
+    SparseState* st_;
+    sparq_graph_t g_ = nullptr;
+};
+
+// Seeded Pauli X/Z insertions on one register, per (iteration, segment), held
+// in a device table so that a noise channel whose events change every
+// iteration replays inside one captured sequence.  x and z are [iteration][row]
+// bit masks; the table's iteration counter starts at 0.
+struct PauliNoiseTable {
+    std::shared_ptr<struct sparq_table_s> table;
+    size_t rows = 0, iterations = 0;
+    PauliNoiseTable(size_t rows, size_t iterations, const std::vector<uint64_t>& x, const std::vector<uint64_t>& z);
+    void set_iteration(uint64_t k);
+};
+// For the entries whose `segment` register holds row s < rows: `target` ^= the
+// X mask of (current iteration, s), then a Z (phase pi, as Z_Bool) per set bit
+// of the Z mask on the flipped value, lowest bit first: the same amplitudes as
+// X_Bool(target, i) then Z_Bool(target, i) for the drawn i in ascending order,
+// each conditioned on segment == s.
+struct PauliNoise : PointwiseOperator {
+    std::shared_ptr<struct sparq_table_s> keep;
+    PauliNoise(std::string_view segment, std::string_view target, const PauliNoiseTable& t)
+        : PauliNoise(System::get(segment), System::get(target), t) {}
+    PauliNoise(size_t segment, size_t target, const PauliNoiseTable& t);
+};
+// The table's iteration counter += 1, ordered on the state's stream.
+struct AdvancePauliNoise : BaseOperator {
+    SPQ_BOTH_VIEWS
+    std::shared_ptr<struct sparq_table_s> table;
+    explicit AdvancePauliNoise(const PauliNoiseTable& t) : table(t.table) {}
+    void operator()(SparseState& state) const override;
+};
+
 struct QRAMCircuit_qutrit {
     size_t address_size, data_size;
     std::vector<uint64_t> memory;

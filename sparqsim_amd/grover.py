@@ -193,7 +193,8 @@ class NoisyGrover:
         m = self.mod
         return m.X_Bool("addr", i) if kind == "X" else m.Z_Bool("addr", i)
 
-    def run_batched(self):
+    def _start_batched(self):
+        """The batch state after the first Hadamard, and the Grover iteration operator."""
         m = self.mod
         st = self._registers(batched=True)
         uneven = self.widths is not None
@@ -210,11 +211,53 @@ class NoisyGrover:
         else:
             diffusion = None
             m.Hadamard_Int_Full("addr")(st)
-        g = GroverOperator(m, self.qram, "addr", "data", "search", diffusion)
+        return st, GroverOperator(m, self.qram, "addr", "data", "search", diffusion)
+
+    def run_batched(self):
+        st, g = self._start_batched()
         for k in range(self.iterations):
             g(st)
             for t, i, kind in self.events[k]:
                 self._noise_op(i, kind).conditioned_by_value("traj", t)(st)
+        return st
+
+    def pauli_masks(self):
+        """The noise events as [iteration][trajectory] X and Z bit masks on addr
+        (row-major lists for PauliNoiseTable)."""
+        x = np.zeros((self.iterations, self.n_traj), dtype=np.uint64)
+        z = np.zeros((self.iterations, self.n_traj), dtype=np.uint64)
+        for k, ev in enumerate(self.events):
+            for t, i, kind in ev:
+                (x if kind == "X" else z)[k, t] |= np.uint64(1 << i)
+        return x.ravel().tolist(), z.ravel().tolist()
+
+    def capture_batched(self):
+        """The batch state after the first Hadamard and a CapturedSequence of one
+        whole iteration: the Grover operator, then this iteration's noise from a
+        device table of per-(iteration, trajectory) Pauli masks, then the
+        table's iteration counter.  The noise is the same X_Bool / Z_Bool
+        insertions run_batched applies one conditioned operator at a time."""
+        m = self.mod
+        st, g = self._start_batched()
+        x, z = self.pauli_masks()
+        self.noise_table = m.PauliNoiseTable(self.n_traj, self.iterations, x, z)
+        noise = m.PauliNoise("traj", "addr", self.noise_table)
+        advance = m.AdvancePauliNoise(self.noise_table)
+
+        def iteration(s):
+            g(s)
+            noise(s)
+            advance(s)
+
+        return st, m.CapturedSequence(st, iteration)
+
+    def run_captured(self):
+        """run_batched with every iteration replayed from one captured graph: no
+        host synchronisation between iterations."""
+        st, seq = self.capture_batched()
+        seq.launch(self.iterations)
+        seq.sync()
+        del seq
         return st
 
     def new_plain_registers(self):
