@@ -59,7 +59,10 @@ enum sparq_cond_kind {
     SPARQ_COND_BIT = 2,      /* bit `value` of field is 1          (conditioned_by_bit)      */
     SPARQ_COND_VALUE = 3,    /* field == value                     (conditioned_by_value)    */
     SPARQ_COND_GE = 4,       /* field >= value      (PartialTraceSelectRange lower bound, partial_trace.cpp:187-225) */
-    SPARQ_COND_LE = 5        /* field <= value      (PartialTraceSelectRange upper bound) */
+    SPARQ_COND_LE = 5,       /* field <= value      (PartialTraceSelectRange upper bound) */
+    SPARQ_COND_BIT_CLEAR = 6 /* bit `value` of field is 0: a control on |0> (the reference
+                                reaches it by flipping the bit around a conditioned_by_bit
+                                operator, PySparQ/pysparq/rir.py _with_conditions) */
 };
 typedef struct {
     uint32_t kind;

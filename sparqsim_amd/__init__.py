@@ -27,3 +27,7 @@ from ._core import System, SparseState  # noqa: E402
 ENGINE_LIBRARY = _os.path.join(_here, "libsparq_hip.so")
 
 __all__ = [name for name in dir(_core) if not name.startswith("_")]
+
+from .rir import RIRError, RIRResult, load_rir, run_rir, run_rir_file  # noqa: E402
+
+__all__ += ["RIRError", "RIRResult", "load_rir", "run_rir", "run_rir_file"]

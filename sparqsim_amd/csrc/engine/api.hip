@@ -647,8 +647,9 @@ static void require_conditions_off_target(sparq_field_t target, uint32_t n_digit
     for (uint32_t c = 0; c < ncond; ++c) {
         const sparq_field_t f = conds[c].field;
         // a bit condition reads one bit of its register, the others the whole field
-        const uint32_t flo = conds[c].kind == SPARQ_COND_BIT ? f.shift + (uint32_t)conds[c].value : f.shift;
-        const uint32_t fhi = conds[c].kind == SPARQ_COND_BIT ? flo + 1 : (uint32_t)f.shift + f.width;
+        const bool one_bit = conds[c].kind == SPARQ_COND_BIT || conds[c].kind == SPARQ_COND_BIT_CLEAR;
+        const uint32_t flo = one_bit ? f.shift + (uint32_t)conds[c].value : f.shift;
+        const uint32_t fhi = one_bit ? flo + 1 : (uint32_t)f.shift + f.width;
         const bool overlap = f.word == target.word && flo < hi && fhi > lo;
         SPQ_REQUIRE(!overlap, "a condition of a group transform reads the transformed bits of its target register");
     }

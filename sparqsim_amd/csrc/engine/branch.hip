@@ -1061,6 +1061,7 @@ static bool host_cond_ok(const uint64_t* k, const sparq_cond_t& c) {
         case SPARQ_COND_NONZERO: return v != 0;
         case SPARQ_COND_ALL_ONES: return v == fmask_host(c.field.width);
         case SPARQ_COND_BIT: return (v >> c.value) & 1;
+        case SPARQ_COND_BIT_CLEAR: return !((v >> c.value) & 1);
         case SPARQ_COND_GE: return v >= c.value;
         case SPARQ_COND_LE: return v <= c.value;
         default: return v == c.value;

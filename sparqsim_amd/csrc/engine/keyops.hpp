@@ -88,6 +88,7 @@ __device__ __forceinline__ bool cond_ok(const uint64_t (&k)[W], const sparq_cond
         case SPARQ_COND_NONZERO: return v != 0;
         case SPARQ_COND_ALL_ONES: return v == fmask(c.field.width);
         case SPARQ_COND_BIT: return (v >> c.value) & 1;
+        case SPARQ_COND_BIT_CLEAR: return !((v >> c.value) & 1);
         case SPARQ_COND_GE: return v >= c.value;
         case SPARQ_COND_LE: return v <= c.value;
         default: return v == c.value;

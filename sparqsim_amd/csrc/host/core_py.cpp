@@ -33,6 +33,7 @@ void bind_controllable(py::class_<T, Extra...>& c) {
     c.def_property_readonly("condition_variable_nonzeros", [](T& o) { return o.condition_variable_nonzeros; })
         .def_property_readonly("condition_variable_all_ones", [](T& o) { return o.condition_variable_all_ones; })
         .def_property_readonly("condition_variable_by_bit", [](T& o) { return o.condition_variable_by_bit; })
+        .def_property_readonly("condition_variable_by_clear_bit", [](T& o) { return o.condition_variable_by_clear_bit; })
         .def_property_readonly("condition_variable_by_value", [](T& o) { return o.condition_variable_by_value; })
         .def("conditioned_by_nonzeros", [](T& o, py::object c) -> T& {
             o.condition_variable_nonzeros.clear();
@@ -58,6 +59,18 @@ void bind_controllable(py::class_<T, Extra...>& c) {
             }
             return o;
         }, py::arg("cond"), py::arg("pos") = py::none(), py::return_value_policy::reference_internal)
+        .def("conditioned_by_clear_bit", [](T& o, py::object c, py::object pos) -> T& {
+            o.condition_variable_by_clear_bit.clear();
+            if (pos.is_none()) {
+                for (auto it : c) {
+                    auto t = it.cast<py::tuple>();
+                    o.condition_variable_by_clear_bit.emplace_back(resolve(t[0]), t[1].cast<size_t>());
+                }
+            } else {
+                o.condition_variable_by_clear_bit.emplace_back(resolve(c), pos.cast<size_t>());
+            }
+            return o;
+        }, py::arg("cond"), py::arg("pos") = py::none(), py::return_value_policy::reference_internal)
         .def("conditioned_by_value", [](T& o, py::object c, py::object val) -> T& {
             o.condition_variable_by_value.clear();
             if (val.is_none()) {
@@ -73,6 +86,7 @@ void bind_controllable(py::class_<T, Extra...>& c) {
         .def("clear_control_nonzeros", [](T& o) { o.clear_control_nonzeros(); })
         .def("clear_control_all_ones", [](T& o) { o.clear_control_all_ones(); })
         .def("clear_control_by_bit", [](T& o) { o.clear_control_by_bit(); })
+        .def("clear_control_by_clear_bit", [](T& o) { o.clear_control_by_clear_bit(); })
         .def("clear_control_by_value", [](T& o) { o.clear_control_by_value(); });
 }
 
@@ -136,6 +150,21 @@ PYBIND11_MODULE(_core, m) {
         .def(py::init<const SparseState&>(), py::arg("other"))
         .def("clone", [](const SparseState& s) { return SparseState(s); })
         .def("size", [](SparseState& s) { return s.size(); }, Release())
+        // The register's value when every entry holds the same one, else None
+        // (one device reduction; no counterpart in the reference).
+        .def("uniform_value", [](SparseState& s, py::object reg) -> py::object {
+            const size_t id = resolve(reg);
+            int uni = 1;
+            uint64_t v = 0;
+            {
+                py::gil_scoped_release r;
+                const Layout& L = s.sync();
+                s.flush();
+                check(sparq_field_uniform(s.handle(), L.at(id), &uni, &v));
+            }
+            if (!uni) return py::none();
+            return py::int_(v);
+        }, py::arg("reg"))
         .def("empty", [](SparseState& s) { return s.empty(); }, Release())
         .def("key_words", [](SparseState& s) { return s.sync().words; })
         // basis_states (PySparQ/core.cpp:182): the entries as System objects,

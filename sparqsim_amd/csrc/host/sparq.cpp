@@ -507,6 +507,10 @@ std::vector<sparq_cond_t> Controllable::conditions(const Layout& L) const {   //
         if (p.second >= 64) throw std::invalid_argument("condition bit out of range");
         out.push_back(sparq_cond_t{SPARQ_COND_BIT, L.at(p.first), p.second});
     }
+    for (auto& p : condition_variable_by_clear_bit) {
+        if (p.second >= 64) throw std::invalid_argument("condition bit out of range");
+        out.push_back(sparq_cond_t{SPARQ_COND_BIT_CLEAR, L.at(p.first), p.second});
+    }
     for (size_t id : condition_variable_nonzeros) out.push_back(sparq_cond_t{SPARQ_COND_NONZERO, L.at(id), 0});
     for (auto& p : condition_variable_by_value) out.push_back(sparq_cond_t{SPARQ_COND_VALUE, L.at(p.first), p.second});
     return out;

@@ -191,14 +191,18 @@ struct Controllable {
     std::vector<size_t> condition_variable_nonzeros;
     std::vector<size_t> condition_variable_all_ones;
     std::vector<std::pair<size_t, size_t>> condition_variable_by_bit;
+    // bit == 0 (no counterpart in the reference's Controllable; see SPARQ_COND_BIT_CLEAR)
+    std::vector<std::pair<size_t, size_t>> condition_variable_by_clear_bit;
     std::vector<std::pair<size_t, uint64_t>> condition_variable_by_value;
     bool has_condition() const {
         return !condition_variable_nonzeros.empty() || !condition_variable_all_ones.empty() ||
-               !condition_variable_by_bit.empty() || !condition_variable_by_value.empty();
+               !condition_variable_by_bit.empty() || !condition_variable_by_value.empty() ||
+               !condition_variable_by_clear_bit.empty();
     }
     void clear_control_nonzeros() { condition_variable_nonzeros.clear(); }
     void clear_control_all_ones() { condition_variable_all_ones.clear(); }
     void clear_control_by_bit() { condition_variable_by_bit.clear(); }
+    void clear_control_by_clear_bit() { condition_variable_by_clear_bit.clear(); }
     void clear_control_by_value() { condition_variable_by_value.clear(); }
     // Resolve to engine predicates for the given layout.
     std::vector<sparq_cond_t> conditions(const Layout& L) const;
@@ -206,6 +210,7 @@ struct Controllable {
         t.condition_variable_nonzeros = condition_variable_nonzeros;
         t.condition_variable_all_ones = condition_variable_all_ones;
         t.condition_variable_by_bit = condition_variable_by_bit;
+        t.condition_variable_by_clear_bit = condition_variable_by_clear_bit;
         t.condition_variable_by_value = condition_variable_by_value;
     }
 };
