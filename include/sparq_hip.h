@@ -430,6 +430,15 @@ int sparq_set_sort_mode(int mode);
  * canonical key order); 0 always takes the grouped path.  Same states either
  * way (amplitudes to rounding). */
 int sparq_set_dense_transform(int on);
+/* Entries per thread of the fused pointwise kernel: 1, 2 or 4 forces a shape
+ * (4 only for keys of at most two words), 0 (default) lets the launch-shape
+ * autotuner pick per class (key words, amplitude access, interpreter, log2 N)
+ * from timed first calls.  Results do not depend on it. */
+int sparq_set_pointwise_items(int items);
+/* The autotuners' measurements and choices (pointwise launch shape, radix digit
+ * width) as a JSON object in buf (NUL-terminated, truncated to len);
+ * *needed receives the full length + 1. */
+int sparq_autotune_report(char* buf, size_t len, size_t* needed);
 /* Sort-path counters since the last reset (no counterpart in the reference):
  * out[0] sorts by the prefix + local path, out[1] their entries, out[2] local
  * attempts that overflowed a chunk, out[3] sorts by the LSD passes, out[4]

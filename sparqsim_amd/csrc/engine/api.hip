@@ -424,6 +424,25 @@ int sparq_set_sort_mode(int mode) {
     });
 }
 
+int sparq_set_pointwise_items(int items) {
+    return guarded([&] {
+        SPQ_REQUIRE(items == 0 || items == 1 || items == 2 || items == 4, "entries per thread must be 0 (autotuned), 1, 2 or 4");
+        set_pointwise_items(items);
+    });
+}
+
+int sparq_autotune_report(char* buf, size_t len, size_t* needed) {
+    return guarded([&] {
+        const std::string r = "{\"pointwise\":" + pointwise_tuning_report() + ",\"radix\":" + radix_tuning_report() + "}";
+        if (needed) *needed = r.size() + 1;
+        if (buf && len) {
+            const size_t k = std::min(len - 1, r.size());
+            std::memcpy(buf, r.data(), k);
+            buf[k] = 0;
+        }
+    });
+}
+
 int sparq_set_dense_transform(int on) {
     return guarded([&] { set_dense_transform(on != 0); });
 }

@@ -150,6 +150,11 @@ void set_sort_mode(int mode);
 // Hadamard_Int on a small, well filled key domain through a dense amplitude
 // array (on by default); off: always the grouped path.
 void set_dense_transform(bool on);
+// Entries per thread of the pointwise kernel: 1, 2 or 4 forced, 0 autotuned.
+void set_pointwise_items(int items);
+// The autotuners' measurements and choices as JSON arrays.
+std::string pointwise_tuning_report();
+std::string radix_tuning_report();
 // out[6]: sorts by the prefix + local path, their entries, local attempts that
 // overflowed a chunk, sorts by the LSD passes, their entries, and the local
 // sorts that ran no prefix pass (the state was already sorted by the prefix);

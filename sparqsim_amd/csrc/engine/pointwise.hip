@@ -3,6 +3,10 @@
 // HBM bytes per entry: 2*8*W (+32 when any op touches the amplitude)
 // (+ the table word for each QRAM load, served mostly from L2/MALL).
 #include <cmath>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "engine.hpp"
 #include "keyops.hpp"
@@ -387,19 +391,23 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
     }
 }
 
+// Entries per thread when the launch-shape autotuner has no measurement (and
+// for captured sequences): one for one- and two-word keys, where the
+// interpreter's registers for several entries cut the waves per SIMD more than
+// the extra loads in flight gain (23 vs 27 us for the 2^20 QRAM-load +
+// 32-phase program with the full interpreter), two for three and four words.
 template <int W>
 constexpr int kPwItems() { return (W == 3 || W == 4) ? 2 : 1; }
+// Entries per thread the autotuner may try.
+constexpr bool pw_items_allowed(int W, int items) { return items == 1 || items == 2 || (items == 4 && W <= 2); }
 
 // DEV: the entry count is read from n_dev and the grid strides over it (a
 // persistent grid; captured sequences); otherwise one round over n.
-template <int W, bool AMP, bool LIGHT, bool DEV>
+template <int W, bool AMP, bool LIGHT, bool DEV, int ITEMS>
 __global__ void __launch_bounds__(kBlock) pointwise_kernel(uint64_t* __restrict__ keys, double2* __restrict__ amps, uint64_t cap,
                                                       uint64_t n_arg, const uint64_t* __restrict__ n_dev, const PwProgram prog) {
-    // kPwItems(W) entries per thread (striped by the block size).  One entry
-    // per thread for one- and two-word keys: the interpreter's registers for
-    // several entries cut the waves per SIMD more than the extra gathers in
-    // flight gain (23 vs 27 us for the 2^20 QRAM-load + 32-phase program).
-    constexpr int kI = kPwItems<W>();
+    // ITEMS entries per thread, striped by the block size.
+    constexpr int kI = ITEMS;
     const uint64_t n = DEV ? n_dev[0] : n_arg;
     for (uint64_t blk = blockIdx.x; blk * kBlock * kI < n; blk += gridDim.x) {
     const uint64_t base = blk * kBlock * kI + threadIdx.x;
@@ -461,28 +469,137 @@ static bool touches_amp(uint32_t opcode) {
     }
 }
 
-template <int W, bool DEV>
-static void launch_pointwise_impl(State& st, const PwProgram& prog, bool amp, bool light, unsigned g, const uint64_t* n_dev) {
+template <int W, bool DEV, int ITEMS>
+static void launch_pointwise_items(State& st, const PwProgram& prog, bool amp, bool light, unsigned g, const uint64_t* n_dev) {
     if (light) {
-        if (amp) pointwise_kernel<W, true, true, DEV><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
-        else pointwise_kernel<W, false, true, DEV><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
+        if (amp) pointwise_kernel<W, true, true, DEV, ITEMS><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
+        else pointwise_kernel<W, false, true, DEV, ITEMS><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
     } else {
-        if (amp) pointwise_kernel<W, true, false, DEV><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
-        else pointwise_kernel<W, false, false, DEV><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
+        if (amp) pointwise_kernel<W, true, false, DEV, ITEMS><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
+        else pointwise_kernel<W, false, false, DEV, ITEMS><<<g, kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, st.n, n_dev, prog);
     }
     SPQ_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- launch-shape autotuner
+// Entries per thread (1, 2 or 4) are chosen per class (key words, amplitude
+// access, light / full interpreter, log2 N) from N >= 2^16 on: the first two
+// calls of a class run the default shape unmeasured (first allocations, cold
+// caches), the next ones cycle twice through the candidates with the kernel
+// alone timed by HIP events and normalised per entry, keeping each
+// candidate's faster sample; later calls take the fastest, the default unless
+// another is at least 5 % faster.  Every call does its real work, whichever
+// shape runs.
+namespace {
+constexpr int kPwCandidates[3] = {1, 2, 4};
+constexpr int kPwWarmCalls = 2, kPwRounds = 2;
+struct PwTune {
+    int calls = 0;
+    float ms[3] = {-1.f, -1.f, -1.f};   // ms per 2^20 entries, the faster sample
+    int best = -1;
+};
+std::mutex g_pw_mu;
+std::map<uint32_t, PwTune> g_pw_tune;
+int g_pw_forced = 0;
+}  // namespace
+
+void set_pointwise_items(int items) {
+    std::lock_guard<std::mutex> lock(g_pw_mu);
+    g_pw_forced = items;
+}
+
+std::string pointwise_tuning_report() {
+    std::lock_guard<std::mutex> lock(g_pw_mu);
+    std::string out = "[";
+    for (auto& kv : g_pw_tune) {
+        const uint32_t k = kv.first;
+        char buf[256];
+        snprintf(buf, sizeof(buf), "%s{\"words\":%u,\"amp\":%u,\"light\":%u,\"log2_n\":%u,\"items\":%d,\"ms_per_2p20\":[%.5f,%.5f,%.5f]}",
+                 out.size() > 1 ? "," : "", k & 15, (k >> 4) & 1, (k >> 5) & 1, k >> 8,
+                 kv.second.best >= 0 ? kPwCandidates[kv.second.best] : 0, kv.second.ms[0], kv.second.ms[1], kv.second.ms[2]);
+        out += buf;
+    }
+    return out + "]";
+}
+
+template <int W, bool DEV>
+static void launch_pointwise_impl(State& st, const PwProgram& prog, bool amp, bool light, int items, const uint64_t* n_dev,
+                                  uint64_t span) {
+    const unsigned g = DEV ? persistent_grid(span, kBlock * items) : grid_for(span, kBlock * items);
+    if (items == 4) {
+        if constexpr (pw_items_allowed(W, 4)) {
+            launch_pointwise_items<W, DEV, 4>(st, prog, amp, light, g, n_dev);
+            return;
+        }
+    }
+    if (items == 2) launch_pointwise_items<W, DEV, 2>(st, prog, amp, light, g, n_dev);
+    else launch_pointwise_items<W, DEV, 1>(st, prog, amp, light, g, n_dev);
 }
 
 template <int W>
 static void launch_pointwise(State& st, const PwProgram& prog, bool amp, bool light) {
     if (st.capture) {   // recording a captured sequence: the count is on the device
         const uint64_t* n_dev = capture_words(st) + kCapN;
-        launch_pointwise_impl<W, true>(st, prog, amp, light, persistent_grid(st.capture->capacity, kBlock * kPwItems<W>()), n_dev);
+        launch_pointwise_impl<W, true>(st, prog, amp, light, kPwItems<W>(), n_dev, st.capture->capacity);
         return;
     }
-    unsigned g = grid_for(st.n, kBlock * kPwItems<W>());
+    int items = kPwItems<W>(), measure = -1;
+    {
+        std::lock_guard<std::mutex> lock(g_pw_mu);
+        if (g_pw_forced) {
+            items = pw_items_allowed(W, g_pw_forced) ? g_pw_forced : kPwItems<W>();
+        } else if (st.n >= (uint64_t(1) << 16)) {
+            int lg = 0;
+            while ((uint64_t(1) << (lg + 1)) <= st.n && lg < 62) ++lg;
+            const uint32_t key = (uint32_t)W | (amp ? 16u : 0u) | (light ? 32u : 0u) | ((uint32_t)lg << 8);
+            PwTune& t = g_pw_tune[key];
+            const int ncand = pw_items_allowed(W, 4) ? 3 : 2;
+            if (t.calls < kPwWarmCalls) {
+                ++t.calls;
+            } else if (t.calls < kPwWarmCalls + kPwRounds * ncand) {
+                measure = (t.calls - kPwWarmCalls) % ncand;
+                items = kPwCandidates[measure];
+                ++t.calls;
+            } else if (t.best >= 0) {
+                items = kPwCandidates[t.best];
+            }
+        }
+    }
     SPQ_PROF(st.stream, "pointwise_kernel", st.n * (16.0 * W + (amp ? 32.0 : 0.0)));
-    launch_pointwise_impl<W, false>(st, prog, amp, light, g, nullptr);
+    if (measure < 0) {
+        launch_pointwise_impl<W, false>(st, prog, amp, light, items, nullptr, st.n);
+        return;
+    }
+    // events of this call only: other states may be timed concurrently
+    hipEvent_t ev[2];
+    SPQ_HIP(hipEventCreate(&ev[0]));
+    SPQ_HIP(hipEventCreate(&ev[1]));
+    SPQ_HIP(hipEventRecord(ev[0], st.stream));
+    launch_pointwise_impl<W, false>(st, prog, amp, light, items, nullptr, st.n);
+    SPQ_HIP(hipEventRecord(ev[1], st.stream));
+    SPQ_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    SPQ_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    SPQ_HIP(hipEventDestroy(ev[0]));
+    SPQ_HIP(hipEventDestroy(ev[1]));
+    int lg = 0;
+    while ((uint64_t(1) << (lg + 1)) <= st.n && lg < 62) ++lg;
+    const uint32_t key = (uint32_t)W | (amp ? 16u : 0u) | (light ? 32u : 0u) | ((uint32_t)lg << 8);
+    std::lock_guard<std::mutex> lock(g_pw_mu);
+    PwTune& t = g_pw_tune[key];
+    const float per = std::max(ms, 1e-6f) / (float)((double)st.n / 1048576.0);
+    t.ms[measure] = t.ms[measure] > 0.f ? std::min(t.ms[measure], per) : per;
+    const int ncand = pw_items_allowed(W, 4) ? 3 : 2;
+    bool all = t.calls >= kPwWarmCalls + kPwRounds * ncand;
+    for (int c = 0; c < ncand; ++c) all = all && t.ms[c] > 0.f;
+    if (all) {   // a call of another thread still measuring may refine a sample afterwards
+        int def = 0;
+        while (kPwCandidates[def] != kPwItems<W>()) ++def;
+        int best = def;
+        for (int c = 0; c < ncand; ++c)
+            if (c != def && t.ms[c] < 0.95f * t.ms[def] && t.ms[c] < t.ms[best]) best = c;
+        t.best = best;
+    }
 }
 
 static void check_field(const State& st, sparq_field_t f) {

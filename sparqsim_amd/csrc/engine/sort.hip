@@ -314,6 +314,20 @@ void set_radix_bits(int bits) {
     g_forced_bits = bits;
 }
 
+std::string radix_tuning_report() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    std::string out = "[";
+    for (int lg = 0; lg < 64 && g_tune_init; ++lg) {
+        if (g_tune_phase[lg] == 0) continue;
+        const float c8 = g_ms_per_pass[0][lg], c11 = g_ms_per_pass[1][lg];
+        char buf[160];
+        snprintf(buf, sizeof(buf), "%s{\"log2_n\":%d,\"ms_per_pass_per_2p20\":[%.5f,%.5f],\"done\":%d}", out.size() > 1 ? "," : "",
+                 lg, c8, c11, g_tune_phase[lg] >= 4 ? 1 : 0);
+        out += buf;
+    }
+    return out + "]";
+}
+
 // Sort by one of two plans of the same ordering (8- and 11-bit digits), the
 // digit width picked by the autotuner.
 static void sort_with_plans(State& st, const std::vector<Digit>& p8, const std::vector<Digit>& p11) {

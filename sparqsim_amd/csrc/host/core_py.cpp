@@ -607,4 +607,14 @@ PYBIND11_MODULE(_core, m) {
           "Sort strategy: 0 prefix passes + LDS chunk sort where it applies (default), 1 LSD passes only.");
     m.def("set_radix_bits", [](int bits) { check(sparq_set_radix_bits(bits)); }, py::arg("bits"),
           "Digit width of the radix sort: 8 or 11 (forced) or 0 (autotuned, the default).");
+    m.def("set_pointwise_items", [](int items) { check(sparq_set_pointwise_items(items)); }, py::arg("items"),
+          "Entries per thread of the fused pointwise kernel: 1, 2 or 4 (forced) or 0 (autotuned, the default).");
+    m.def("autotune_report", []() {
+        size_t need = 0;
+        check(sparq_autotune_report(nullptr, 0, &need));
+        std::string buf(need, '\0');
+        check(sparq_autotune_report(&buf[0], need, &need));
+        buf.resize(need ? need - 1 : 0);
+        return py::module_::import("json").attr("loads")(buf);
+    }, "The autotuners' measurements and choices: pointwise launch shape per class, radix digit width per size class.");
 }
