@@ -4,6 +4,7 @@
 // (+ the table word for each QRAM load, served mostly from L2/MALL).
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -13,10 +14,20 @@
 
 namespace sparq {
 
+// A QRAM table word read before the program's ops run (PwProgram::pre).
+struct PwPrefetch {
+    const void* table;
+    uint64_t mask;
+    sparq_field_t f;
+    uint32_t bytes;
+};
+constexpr int kPwPrefetch = 2;
+
 struct PwProgram {
-    uint32_t nops, nconds;
+    uint32_t nops, nconds, npre, pad;
     sparq_pw_op_t ops[SPARQ_MAX_PROGRAM_OPS];
     sparq_cond_t conds[SPARQ_MAX_PROGRAM_CONDS];
+    PwPrefetch pre[kPwPrefetch];
 };
 
 // Fixed-point encoding of x in [0,1) into w bits by truncation; identical to the
@@ -109,8 +120,10 @@ __device__ __forceinline__ void phase_per_bit(double2& amp, uint64_t bits, const
 
 // LIGHT: the heavy opcodes (is_heavy_opcode) compile to nothing, so a program
 // of light opcodes runs with far fewer registers per lane.
+// pre0 / pre1: the program's prefetched table words (an op with pad = s + 1
+// reads word s instead of loading it).
 template <int W, bool LIGHT>
-__device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W], double2& amp) {
+__device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W], double2& amp, uint64_t pre0, uint64_t pre1) {
     const sparq_field_t f0 = op.f[0], f1 = op.f[1], f2 = op.f[2], f3 = op.f[3];
     switch (op.opcode) {
         case SPARQ_PW_FLIP_BIT: fput<W>(k, f0, fget<W>(k, f0) ^ (uint64_t(1) << op.c[0])); break;
@@ -262,8 +275,11 @@ __device__ __forceinline__ void run_op(const sparq_pw_op_t& op, uint64_t (&k)[W]
             }
             break;
         case SPARQ_PW_QRAM_LOAD: {
-            uint64_t a = fget<W>(k, f0) & fmask((uint32_t)op.c[0]);
-            fput<W>(k, f1, fget<W>(k, f1) ^ table_word(op.table, op.table_word_bytes, a));
+            uint64_t w;
+            if (op.pad == 1) w = pre0;
+            else if (op.pad == 2) w = pre1;
+            else w = table_word(op.table, op.table_word_bytes, fget<W>(k, f0) & fmask((uint32_t)op.c[0]));
+            fput<W>(k, f1, fget<W>(k, f1) ^ w);
             break;
         }
         case SPARQ_PW_XOR_AFFINE: {
@@ -425,13 +441,20 @@ __global__ void __launch_bounds__(kBlock) pointwise_kernel(uint64_t* __restrict_
             for (int w = 0; w < W; ++w) k[it][w] = 0;
         }
     }
+    // prefetched table words: every load of the entry in flight together
+    uint64_t p0[kI], p1[kI];
+#pragma unroll
+    for (int it = 0; it < kI; ++it) {
+        p0[it] = prog.npre > 0 ? table_word(prog.pre[0].table, prog.pre[0].bytes, fget<W>(k[it], prog.pre[0].f) & prog.pre[0].mask) : 0;
+        p1[it] = prog.npre > 1 ? table_word(prog.pre[1].table, prog.pre[1].bytes, fget<W>(k[it], prog.pre[1].f) & prog.pre[1].mask) : 0;
+    }
     for (uint32_t o = 0; o < prog.nops; ++o) {
         const sparq_pw_op_t& op = prog.ops[o];
 #pragma unroll
         for (int it = 0; it < kI; ++it) {
             bool ok = true;
             for (uint32_t c = 0; c < op.ncond; ++c) ok = ok && cond_ok<W>(k[it], prog.conds[op.cond_first + c]);
-            if (ok) run_op<W, LIGHT>(op, k[it], a[it]);
+            if (ok) run_op<W, LIGHT>(op, k[it], a[it], p0[it], p1[it]);
         }
     }
 #pragma unroll
@@ -456,6 +479,52 @@ static bool is_heavy_opcode(uint32_t opcode) {
             return true;
         default:
             return false;
+    }
+}
+
+// Whether an op may write its operand field f (conservatively: every operand
+// field of an op not known to write less).
+static bool may_write_field(const sparq_pw_op_t& op, int f) {
+    switch (op.opcode) {
+        case SPARQ_PW_PHASE_BITS: case SPARQ_PW_RZ: case SPARQ_PW_ROT_DIAG: case SPARQ_PW_NEG_AMP: case SPARQ_PW_MUL_AMP:
+            return false;
+        case SPARQ_PW_QRAM_LOAD:
+            return f == 1;
+        default:
+            return op.f[f].width != 0;
+    }
+}
+static bool fields_overlap(sparq_field_t a, sparq_field_t b) {
+    return a.word == b.word && a.shift < (uint32_t)b.shift + b.width && b.shift < (uint32_t)a.shift + a.width;
+}
+
+// QRAM loads whose address bits no earlier op of the program may write read
+// their table word before the first op runs (up to kPwPrefetch distinct
+// words; loads of the same word share one): every gather of an entry is then
+// in flight together instead of one after another.
+static void plan_prefetch(PwProgram& prog) {
+    prog.npre = 0;
+    for (uint32_t o = 0; o < prog.nops; ++o) {
+        sparq_pw_op_t& op = prog.ops[o];
+        op.pad = 0;
+        if (op.opcode != SPARQ_PW_QRAM_LOAD) continue;
+        bool stable = true;
+        for (uint32_t e = 0; e < o && stable; ++e)
+            for (int f = 0; f < 4 && stable; ++f)
+                if (may_write_field(prog.ops[e], f) && fields_overlap(prog.ops[e].f[f], op.f[0])) stable = false;
+        if (!stable) continue;
+        const uint64_t mask = fmask_host((uint32_t)op.c[0]);
+        uint32_t slot = prog.npre;
+        for (uint32_t s = 0; s < prog.npre; ++s)
+            if (prog.pre[s].table == op.table && prog.pre[s].bytes == op.table_word_bytes && prog.pre[s].mask == mask &&
+                std::memcmp(&prog.pre[s].f, &op.f[0], sizeof(sparq_field_t)) == 0)
+                slot = s;
+        if (slot == (uint32_t)kPwPrefetch) continue;
+        if (slot == prog.npre) {
+            prog.pre[slot] = PwPrefetch{op.table, mask, op.f[0], op.table_word_bytes};
+            ++prog.npre;
+        }
+        op.pad = slot + 1;
     }
 }
 
@@ -645,6 +714,7 @@ void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_c
         check_field(st, conds[c].field);
         prog.conds[c] = conds[c];
     }
+    plan_prefetch(prog);
     if (st.capture && st.capture->phase == 1) return;   // planning: nothing runs
     if ((st.n == 0 && !st.capture) || nops == 0) return;
     SPQ_DISPATCH_W(st.W, launch_pointwise, st, prog, amp, light);

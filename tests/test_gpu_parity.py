@@ -273,6 +273,40 @@ def test_qram_load_conditioned(ps, oracle):
     assert_same_state(eg, eo, exact_amps=True, what="qram")
 
 
+@pytest.mark.parametrize("case", ["address_rewritten", "three_tables", "shared_word", "phases_between"])
+def test_qram_load_fused_programs(ps, oracle, case):
+    """QRAM loads fused into one per-entry program: a load whose address an
+    earlier op rewrites, three tables at once (two table words are read ahead
+    of the ops, the third in place), two loads of one word, and phases between
+    a load and its uncompute (the QRAM-interference pattern)."""
+    rng = np.random.default_rng(9)
+    m1 = [int(x) for x in rng.integers(0, 1 << 9, size=1 << 7)]
+    m2 = [int(x) for x in rng.integers(0, 1 << 6, size=1 << 6)]
+    m3 = [int(x) for x in rng.integers(0, 1 << 7, size=1 << 6)]
+
+    def build(mod):
+        st = loaded(mod, SPEC3, 3000, seed=6)
+        q1 = mod.QRAMCircuit_qutrit(7, 9, m1)
+        q2 = mod.QRAMCircuit_qutrit(6, 6, m2)
+        q3 = mod.QRAMCircuit_qutrit(6, 7, m3)
+        if case == "address_rewritten":
+            ops = [mod.QRAMLoad(q1, "a", "b"), mod.Add_ConstUInt_InPlace("a", 5), mod.QRAMLoad(q1, "a", "b"),
+                   mod.X_Bool("a", 2), mod.QRAMLoad(q1, "a", "b").conditioned_by_nonzeros("f")]
+        elif case == "three_tables":
+            ops = [mod.QRAMLoad(q1, "a", "b"), mod.QRAMLoad(q2, "c", "s"), mod.QRAMLoad(q3, "c", "a"),
+                   mod.QRAMLoad(q1, "a", "b")]
+        elif case == "shared_word":
+            ops = [mod.QRAMLoad(q1, "a", "b"), mod.Z_Bool("b", 3), mod.QRAMLoad(q1, "a", "b"), mod.QRAMLoad(q1, "a", "b")]
+        else:
+            ops = [mod.QRAMLoad(q1, "a", "b")] + [mod.Z_Bool("b", i) for i in range(9)] + [mod.QRAMLoad(q1, "a", "b")]
+        for o in ops:
+            o(st)
+        return st
+
+    eg, eo = run_both(ps, oracle, build)
+    assert_same_state(eg, eo, exact_amps=True, what=case)
+
+
 # ---------------------------------------------------------------- branching
 @pytest.mark.parametrize("nd", [1, 3, 6])
 def test_hadamard_int_mixed_groups(ps, oracle, nd):
