@@ -593,9 +593,11 @@ static bool hadamard_dense(State& st, const KeyStats& ks, sparq_field_t low, uin
         dense_scatter_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, dm, A, mark, cls);
     }
     SPQ_HIP(hipGetLastError());
-    launch_wht(st, A, gcount, groups, nd, cls, (double)slots * 32.0);
+    // profiled bytes: 32 per entry, a lower bound (the slots of occupied groups
+    // move; which groups are occupied is known only on the device)
+    launch_wht(st, A, gcount, groups, nd, cls, (double)n * 32.0);
     {
-        SPQ_PROF(st.stream, "dense_count_kernel", (double)slots * 17.0);
+        SPQ_PROF(st.stream, "dense_count_kernel", (double)n * 17.0);   // lower bound, as the butterflies
         dense_count_kernel<<<tiles, kBlock, 0, st.stream>>>(A, mark, cls, nd, eps, kept, (unsigned long long*)(offs + tiles + 1));
     }
     SPQ_HIP(hipGetLastError());
@@ -610,7 +612,7 @@ static bool hadamard_dense(State& st, const KeyStats& ks, sparq_field_t low, uin
         double2* aout;
         alloc_arrays(st, st.W, cap_out, &kout, &aout);
         {
-            SPQ_PROF(st.stream, "dense_emit_kernel", (double)slots * 16.0 + (double)total * (8.0 * W + 16));
+            SPQ_PROF(st.stream, "dense_emit_kernel", (double)n * 16.0 + (double)total * (8.0 * W + 16));   // lower bound
             dense_emit_kernel<W><<<tiles, kBlock, 0, st.stream>>>(A, cls, dm, eps, offs, kout, aout, cap_out);
         }
         SPQ_HIP(hipGetLastError());

@@ -97,10 +97,12 @@ class BlockEncodingHs:
     def apply(self, st):
         m = self.mod
         a1, a2, a3, a4, main = self.a1, self.a2, self.a3, self.a4, self.main_reg
-        # self-adjoint state preparation of b = ones/sqrt(n)
+        # state preparation of b: a Hadamard (self-adjoint) for b = ones/sqrt(n),
+        # or an object whose `dag` is its adjoint
         enc_b = self.enc_b if self.enc_b is not None else m.Hadamard_Int_Full(main)
+        enc_b_dag = getattr(enc_b, "dag", enc_b)
         m.Hadamard_Bool(a3)(st)
-        enc_b(st)                              # enc_b.dag
+        enc_b_dag(st)
         m.X_Bool(a1, 0)(st)
         m.Reflection_Bool(main, True).conditioned_by_all_ones([a1, a3, a4])(st)
         m.X_Bool(a1, 0)(st)
@@ -118,7 +120,7 @@ class BlockEncodingHs:
         m.X_Bool(a4, 0)(st)
         m.Rot_Bool(a2, 0, self.R_s).conditioned_by_all_ones(a4)(st)
         m.X_Bool(a4, 0)(st)
-        enc_b(st)                              # enc_b.dag
+        enc_b_dag(st)
         m.X_Bool(a1, 0)(st)
         m.Reflection_Bool(main, True).conditioned_by_all_ones([a1, a3, a4])(st)
         m.X_Bool(a1, 0)(st)

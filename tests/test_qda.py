@@ -18,7 +18,8 @@ def run_golden(mod, name, beta, max_points=None):
     # The reference checks the same walk over two block encodings of A (tridiagonal
     # direct oracles and Block_Encoding_via_QRAM, CorrectnessTest_QDA_CompareList.inl:
     # 165-202); their lists agree to 3e-15 (test_golden_lists_of_both_encodings_agree),
-    # so the tridiagonal walk is held to both.  The QRAM encoding itself is not built.
+    # so the tridiagonal walk is held to both (the QRAM walk to its own lists in
+    # tests/test_qda_qram.py).
     refs = [GOLDEN["lists"][name], GOLDEN["lists"][name.replace("_Tri", "_QRAM")]]
     ref = refs[0]
     a, b, kappa, steps = qda.qda_schedule(3, 0.01, 2.0, beta)
