@@ -159,13 +159,34 @@ static State& S_cap(sparq_state_t h) {
     return h->st;
 }
 
+// One lane per entry: all W key words and the amplitude of entry b+i of src
+// go to entry at+i of dst.  One launch instead of W+1 copies.
+template <int W>
+__global__ void __launch_bounds__(kBlock) copy_entries_kernel(uint64_t* __restrict__ dk, double2* __restrict__ da, uint64_t dcap,
+                                                              const uint64_t* __restrict__ sk, const double2* __restrict__ sa,
+                                                              uint64_t scap, uint64_t cnt) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < cnt; i += stride) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) dk[(uint64_t)w * dcap + i] = sk[(uint64_t)w * scap + i];
+        da[i] = sa[i];
+    }
+}
+
+template <int W>
+static void launch_copy_entries(State& dst, uint64_t at, const State& src, uint64_t b, uint64_t cnt) {
+    SPQ_PROF(dst.stream, "copy_entries", (double)cnt * 2.0 * (8.0 * W + 16.0));
+    copy_entries_kernel<W><<<std::min<unsigned>(grid_for(cnt, kBlock), 8192), kBlock, 0, dst.stream>>>(
+        dst.keys + at, dst.amps + at, dst.cap, src.keys + b, src.amps + b, src.cap, cnt);
+    SPQ_HIP(hipGetLastError());
+}
+
 // Copy entries [b, b+cnt) of src into dst at position at (same W).
 static void copy_entries(State& dst, uint64_t at, const State& src, uint64_t b, uint64_t cnt) {
     if (!cnt) return;
-    for (uint32_t w = 0; w < src.W; ++w)
-        SPQ_HIP(hipMemcpyAsync(dst.keys + (uint64_t)w * dst.cap + at, src.keys + (uint64_t)w * src.cap + b, cnt * 8,
-                               hipMemcpyDeviceToDevice, dst.stream));
-    SPQ_HIP(hipMemcpyAsync(dst.amps + at, src.amps + b, cnt * 16, hipMemcpyDeviceToDevice, dst.stream));
+    if (src.W != dst.W) throw Error(SPARQ_RUNTIME, "copy_entries between states of different key widths");
+    if (at + cnt > dst.cap || b + cnt > src.cap) throw Error(SPARQ_RUNTIME, "copy_entries out of range");
+    SPQ_DISPATCH_W(src.W, launch_copy_entries, dst, at, src, b, cnt);
 }
 
 static State sub_state(State& parent, uint64_t b, uint64_t cnt) {
@@ -189,35 +210,62 @@ static void release(State& s) {
     s.amps = nullptr;
 }
 
+// Append the entries of `tail` behind those of st: in place when st's arrays
+// have room, else into fresh arrays.
+static void append_entries(State& st, const State& tail) {
+    if (!tail.n) return;
+    const uint64_t total = st.n + tail.n;
+    if (total <= st.cap) {
+        copy_entries(st, st.n, tail, 0, tail.n);
+        st.n = total;
+        return;
+    }
+    uint64_t* k;
+    double2* a;
+    alloc_arrays(st, st.W, total, &k, &a);
+    State out = st;
+    out.keys = k;
+    out.amps = a;
+    out.cap = total;
+    copy_entries(out, 0, st, 0, st.n);
+    copy_entries(out, st.n, tail, 0, tail.n);
+    install_arrays(st, st.W, k, a, total, total);
+}
+
 // Run `op` on the entries satisfying `conds`; the rest are kept (and ClearZero'd,
-// as the reference's ClearZero covers the whole state).
+// as the reference's ClearZero covers the whole state).  After the partition
+// the rest moves to arrays of its own, the op runs on the state itself holding
+// only the satisfying entries, and the rest is appended behind its result.
 template <class F>
 static void with_conditions(State& st, const sparq_cond_t* conds, uint32_t ncond, double eps, F op) {
     if (ncond == 0) {
         op(st);
         return;
     }
-    uint64_t S_ = partition_by_conditions(st, conds, ncond);
+    const uint64_t S_ = partition_by_conditions(st, conds, ncond);
     if (S_ == st.n) {
         op(st);
         return;
     }
-    State act = sub_state(st, 0, S_);
     State rest = sub_state(st, S_, st.n - S_);
-    if (act.n) op(act);
-    clear_zero(rest, eps);
-    uint64_t total = act.n + rest.n;
-    uint64_t* k;
-    double2* a;
-    alloc_arrays(st, st.W, std::max<uint64_t>(total, 1), &k, &a);
-    State out = st;
-    out.keys = k;
-    out.amps = a;
-    out.cap = std::max<uint64_t>(total, 1);
-    copy_entries(out, 0, act, 0, act.n);
-    copy_entries(out, act.n, rest, 0, rest.n);
-    install_arrays(st, st.W, k, a, total, out.cap);
-    release(act);
+    st.n = S_;
+    bool appended = false;
+    try {
+        if (S_) op(st);
+        clear_zero(rest, eps);
+        append_entries(st, rest);
+        appended = true;
+    } catch (...) {
+        // keep the rest's entries behind whatever the failed op left
+        if (!appended) {
+            try {
+                append_entries(st, rest);
+            } catch (...) {
+            }
+        }
+        release(rest);
+        throw;
+    }
     release(rest);
 }
 

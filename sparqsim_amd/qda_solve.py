@@ -25,7 +25,7 @@ from . import qda
 
 def solve(ps, nqubit=20, steprate=1.1, p=1.3, alpha=2.2666666666564934, beta=-1.0, kappa=0.0, log_every=250,
           tol=1e-6, max_steps=10 ** 9, cz_eps=1e-14, ancilla_order=("anc_4", "anc_3", "anc_2", "anc_1"), log=print,
-          stop_after=None, step_hook=None):
+          stop_after=None, step_hook=None, time_limit=None):
     n = 1 << nqubit
     kappa_analytic = qda.analytic_kappa(alpha, beta, n)
     kappa = kappa if kappa > 0 else kappa_analytic
@@ -41,6 +41,7 @@ def solve(ps, nqubit=20, steprate=1.1, p=1.3, alpha=2.2666666666564934, beta=-1.
     curve, peak, termination = [], 0, "fixed_step_budget"
     t0 = time.perf_counter()
     t_extract = 0.0
+    last_walk = (0.0, 0)
     try:
         st = qda.setup_poiseuille(ps, nqubit, ancilla_order)
         clear = ps.ClearZero(cz_eps)
@@ -52,19 +53,27 @@ def solve(ps, nqubit=20, steprate=1.1, p=1.3, alpha=2.2666666666564934, beta=-1.
                                  alpha, beta)(st)
             clear(st)
             norm(st)
-            if (i + 1) % log_every == 0 or i == steps - 1 or (stop_after is not None and i + 1 == stop_after):
+            out_of_time = time_limit is not None and time.perf_counter() - t0 - t_extract > time_limit
+            if (i + 1) % log_every == 0 or i == steps - 1 or (stop_after is not None and i + 1 == stop_after) or out_of_time:
                 size = st.size()
                 peak = max(peak, size)
                 te = time.perf_counter()
+                walk_now = te - t0 - t_extract
+                interval_ms = (walk_now - last_walk[0]) / (i + 1 - last_walk[1]) * 1e3
+                last_walk = (walk_now, i + 1)
                 out, _ = qda.get_output(st, ps)
                 m = qda.solution_metrics(out, n, alpha, beta, xref)
                 t_extract += time.perf_counter() - te
-                m.update(step=i + 1, entries=size, wall_s=time.perf_counter() - t0)
+                m.update(step=i + 1, entries=size, wall_s=time.perf_counter() - t0, interval_ms_per_step=interval_ms)
                 curve.append(m)
                 log(f"step {i + 1:7d}/{steps} res_dir={m['res_dir']:.6e} res_opt={m['res_opt']:.6e} "
-                    f"succP={m['success_prob']:.4f} fid={m['fidelity']:.6f} entries={size} wall={m['wall_s']:.1f}s")
+                    f"succP={m['success_prob']:.4f} fid={m['fidelity']:.6f} entries={size} wall={m['wall_s']:.1f}s "
+                    f"ms/step={interval_ms:.1f}")
                 if not math.isnan(m["res_dir"]) and m["res_dir"] < tol:
                     termination = f"residual_below_tol({tol:g})"
+                    break
+                if out_of_time:
+                    termination = f"time_limit({time_limit:g}s)"
                     break
     finally:
         ps.set_clearzero_default_eps(old_eps)
@@ -114,6 +123,8 @@ def main():
     ap.add_argument("--max-steps", type=int, default=10 ** 9)
     ap.add_argument("--cz-eps", type=float, default=1e-14)
     ap.add_argument("--stop-after", type=int, default=None, help="run only the first N steps of the schedule")
+    ap.add_argument("--time-limit", type=float, default=None,
+                    help="stop after the first step that ends past this many seconds of walk time (extraction excluded)")
     ap.add_argument("--learned", action="store_true", help="size branching operators with the learned sizer")
     ap.add_argument("--json", default=None, help="write the summary (with the curve) to this file")
     ap.add_argument("--kernel-profile", default=None,
@@ -135,12 +146,13 @@ def main():
     with prof or contextlib.nullcontext():
         res = solve(ps, args.nqubit, args.steprate, args.p, args.alpha, args.beta, args.kappa, args.log_every, args.tol,
                     args.max_steps, args.cz_eps, log=lambda s: print(s, flush=True), stop_after=args.stop_after,
-                    step_hook=hook)
+                    step_hook=hook, time_limit=args.time_limit)
     if prof:
         prof.write(args.kernel_profile, steps=res["steps_run"] - args.profile_from, first_step=args.profile_from,
                    peak_logged_entries=res["peak_logged_entries"])
     if args.learned:
         res["sizer"] = sizer.stats()
+    res["max_system_size"] = ps.System.max_system_size()
     summary = {k: v for k, v in res.items() if k != "curve"}
     print(json.dumps(summary), flush=True)
     if args.outdir:
