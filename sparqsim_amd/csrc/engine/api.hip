@@ -125,6 +125,7 @@ static void settle(State& st) {
     uint64_t* words = const_cast<uint64_t*>(st.pending_words);
     st.pending_words = nullptr;
     st.n = w[kCapN];
+    st.pair_bit = -1;
     if (w[kCapFlags]) {
         SPQ_HIP(hipMemsetAsync(words + kCapFlags, 0, sizeof(uint64_t), st.stream));
         std::string why;
@@ -136,28 +137,25 @@ static void settle(State& st) {
     }
 }
 
-// Every access through S may change the state: it clears basis_amp_one.  No
-// access but the capture-aware ones (S_cap) is allowed while a sequence is
-// being captured on the state.
-static State& S(sparq_state_t h) {
+// Every access through S may change the state: it clears basis_amp_one and
+// pair_bit.  No access but the capture-aware ones (kAllowCapture) is allowed
+// while a sequence is being captured on the state.  kKeepPairs is for the
+// accesses that keep the pair grouping or decide about it themselves (size,
+// pointwise programs, ClearZero, Normalize, the pair operators).
+enum : unsigned { kAllowCapture = 1, kKeepBasis = 2, kKeepPairs = 4 };
+static State& access(sparq_state_t h, unsigned how) {
     if (!h) throw Error(SPARQ_INVALID, "null state handle");
-    if (h->st.capture) throw Error(SPARQ_INVALID, "not available while a sequence is being captured on this state");
+    if (h->st.capture && !(how & kAllowCapture))
+        throw Error(SPARQ_INVALID, "not available while a sequence is being captured on this state");
     settle(h->st);
-    h->st.basis_amp_one = false;
+    if (!(how & kKeepBasis)) h->st.basis_amp_one = false;
+    if (!(how & kKeepPairs) || h->st.capture || h->st.arrays_escaped) h->st.pair_bit = -1;
     return h->st;
 }
-static State& S_keep_basis(sparq_state_t h) {
-    if (!h) throw Error(SPARQ_INVALID, "null state handle");
-    if (h->st.capture) throw Error(SPARQ_INVALID, "not available while a sequence is being captured on this state");
-    settle(h->st);
-    return h->st;
-}
-static State& S_cap(sparq_state_t h) {
-    if (!h) throw Error(SPARQ_INVALID, "null state handle");
-    settle(h->st);
-    h->st.basis_amp_one = false;
-    return h->st;
-}
+static State& S(sparq_state_t h) { return access(h, 0); }
+static State& S_keep_basis(sparq_state_t h) { return access(h, kKeepBasis); }
+static State& S_cap(sparq_state_t h) { return access(h, kAllowCapture); }
+static State& S_pairs(sparq_state_t h) { return access(h, kKeepPairs); }
 
 // One lane per entry: all W key words and the amplitude of entry b+i of src
 // go to entry at+i of dst.  One launch instead of W+1 copies.
@@ -340,12 +338,12 @@ int sparq_state_clone(sparq_state_t src, sparq_state_t* out) {
 }
 
 int sparq_state_words(sparq_state_t h, uint32_t* words) {
-    return guarded([&] { *words = S_cap(h).W; });
+    return guarded([&] { *words = access(h, kAllowCapture | kKeepPairs).W; });
 }
 
 int sparq_state_size(sparq_state_t h, uint64_t* n) {
     return guarded([&] {
-        State& st = S_cap(h);
+        State& st = access(h, kAllowCapture | kKeepPairs);
         // while a sequence is captured the count of the plan's starting state
         if (!st.capture) SPQ_HIP(hipStreamSynchronize(st.stream));
         *n = st.n;
@@ -568,12 +566,12 @@ int sparq_pointwise(sparq_state_t h, const sparq_pw_op_t* ops, uint32_t nops, co
     return guarded([&] {
         SPQ_REQUIRE(nops == 0 || ops, "null op list");
         SPQ_REQUIRE(nconds == 0 || conds, "null condition list");
-        pointwise(S_cap(h), ops, nops, conds, nconds);
+        pointwise(access(h, kAllowCapture | kKeepPairs), ops, nops, conds, nconds);
     });
 }
 
 int sparq_clear_zero(sparq_state_t h, double eps) {
-    return guarded([&] { clear_zero(S(h), eps); });
+    return guarded([&] { clear_zero(S_pairs(h), eps); });
 }
 
 int sparq_sort(sparq_state_t h, const sparq_field_t* last) {
@@ -745,7 +743,13 @@ int sparq_hadamard_int_full(sparq_state_t h, sparq_field_t target, const sparq_c
             return;
         }
         State& st = S(h);
-        // CheckDuplicateKey precedes the transform (hadamard.cpp:183)
+        // CheckDuplicateKey precedes the transform (hadamard.cpp:183).  Without
+        // conditions the transform checks on its way (before it changes
+        // anything); with them the whole state is checked first.
+        if (ncond == 0) {
+            hadamard_int(st, target, target.width, true, eps, false, true);
+            return;
+        }
         if (st.n > 1) {
             // any total order of the keys makes duplicates adjacent; sorting with the
             // target least significant also prepares the grouping sort below
@@ -771,7 +775,7 @@ int sparq_pair_op(sparq_state_t h, sparq_field_t target, uint32_t bit, const spa
     return guarded([&] {
         SPQ_REQUIRE(params != nullptr, "null params");
         SPQ_REQUIRE(ncond <= SPARQ_MAX_PROGRAM_CONDS, "too many conditions");
-        State& st = S(h);
+        State& st = S_pairs(h);
         // Conditions that do not read the target bit are evaluated inside the
         // pair kernels (partners agree on them; failing entries pass through);
         // otherwise the state is partitioned first.
@@ -790,10 +794,12 @@ int sparq_pair_op(sparq_state_t h, sparq_field_t target, uint32_t bit, const spa
             clear_zero(st, eps);
             return;
         }
+        st.pair_bit = -1;
         with_conditions(st, conds, ncond, eps, [&](State& s) {
             pair_op(s, target, bit, *params, CondList{});
             clear_zero(s, eps);
         });
+        st.pair_bit = -1;   // the rest is appended behind the pairs
     });
 }
 
@@ -803,7 +809,7 @@ int sparq_replicate(sparq_state_t h, sparq_field_t target) {
 
 int sparq_normalize(sparq_state_t h, double* norm2) {
     return guarded([&] {
-        State& st = S(h);
+        State& st = S_pairs(h);
         double s = sum_abs2(st);
         if (norm2) *norm2 = s;
         double f = 1.0 / std::sqrt(s);
@@ -845,13 +851,13 @@ int sparq_state_read_range(sparq_state_t h, uint64_t first, uint64_t count, uint
 int sparq_norm2(sparq_state_t h, double* norm2) {
     return guarded([&] {
         SPQ_REQUIRE(norm2 != nullptr, "null argument");
-        *norm2 = sum_abs2(S(h));
+        *norm2 = sum_abs2(S_pairs(h));
     });
 }
 
 int sparq_field_uniform(sparq_state_t h, sparq_field_t f, int* uniform, uint64_t* value) {
     return guarded([&] {
-        State& st = S(h);
+        State& st = S_pairs(h);
         SPQ_REQUIRE(f.word < st.W, "bad field");
         uint64_t v = 0;
         bool u = field_uniform(st, f, &v);
@@ -928,6 +934,7 @@ int sparq_graph_launch(sparq_graph_t g, uint32_t times) {
             capture_set_count(st, g->cap->words);
         }
         st.basis_amp_one = false;
+        st.pair_bit = -1;
         for (uint32_t t = 0; t < times; ++t) SPQ_HIP(hipGraphLaunch(g->exec, st.stream));
         st.pending_words = g->cap->words;
     });

@@ -628,7 +628,7 @@ static bool hadamard_dense(State& st, const KeyStats& ks, sparq_field_t low, uin
 
 template <int W>
 static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, const double2* dense,
-                          bool allow_learned = true, bool unit_basis = false) {
+                          bool allow_learned = true, bool unit_basis = false, bool check_dups = false) {
     const uint64_t n = st.n;
     const uint64_t F = uint64_t(1) << nd;
     sparq_field_t low{target.word, target.shift, (uint8_t)nd, 0};
@@ -639,8 +639,11 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     if (n > 1) {
         const SizerQuery q{n, st.W, nd, dense ? 2u : (full ? 1u : 0u), low.word, low.shift};
         ks = key_stats(st, &low, allow_learned ? &q : nullptr);
+        // a completed dense-domain transform saw every entry on its own slot:
+        // the keys were distinct
         if (!dense && g_dense_transform && hadamard_dense<W>(st, ks, low, nd, eps)) return;
         if (!ks.sorted) sort_with_stats(st, ks, &low);
+        if (check_dups && has_sorted_duplicates(st)) throw Error(SPARQ_RUNTIME, "Hadamard_Int_Full: duplicate key in state");
         single = true;
         for (uint32_t w = 0; w < st.W; ++w) {
             uint64_t v = ks.vary[w];
@@ -809,12 +812,13 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     }
 }
 
-void hadamard_int(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, bool unit_basis) {
+void hadamard_int(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, bool unit_basis, bool check_dups) {
     SPQ_REQUIRE(target.word < st.W && target.width >= 1 && (uint32_t)target.shift + target.width <= 64, "bad target field");
     SPQ_REQUIRE(nd >= 1 && nd <= target.width, "n_digits must be in [1, width]");
     SPQ_REQUIRE(nd <= 40, "n_digits too large for a sparse expansion");
     if (st.n == 0) return;
-    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, nd, full, eps, (const double2*)nullptr, true, unit_basis && st.n == 1);
+    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, nd, full, eps, (const double2*)nullptr, true, unit_basis && st.n == 1,
+                   check_dups);
 }
 
 void group_unitary(State& st, sparq_field_t target, const double* mat, double eps) {
@@ -982,7 +986,11 @@ template <int W>
 static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl) {
     const uint64_t n = st.n;
     sparq_field_t b{target.word, (uint8_t)(target.shift + bit), 1, 0};
-    sort_state(st, &b);
+    // partners adjacent, bit-clear first: sorted with the bit least significant,
+    // or already grouped so across it
+    const int32_t pair_bit = (int32_t)(b.word * 64 + b.shift);
+    if (st.pair_bit != pair_bit) sort_state(st, &b);
+    st.pair_bit = -1;
     const uint64_t bm = uint64_t(1) << b.shift;
     uint32_t* cnt = dalloc<uint32_t>(n, st.stream);
     uint64_t* off = dalloc<uint64_t>(n + 1, st.stream);
@@ -1005,6 +1013,9 @@ static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq
     install_arrays(st, st.W, kout, aout, M, M);
     dfree(cnt, st.stream);
     dfree(off, st.stream);
+    // every output pair is written as (bit clear, bit set) at adjacent slots;
+    // entries failing the conditions pass through beside their partners
+    st.pair_bit = pair_bit;
 }
 
 void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl) {

@@ -86,6 +86,14 @@ struct State {
     // device arrays have been handed to a caller, who may write them directly.
     bool basis_amp_one = false;
     bool arrays_escaped = false;
+    // Entries are grouped in pairs across key bit pair_bit (word * 64 + shift;
+    // -1: not known): wherever a key and the key with that bit flipped are both
+    // present they are adjacent, the one with the bit clear first.  Set by the
+    // pair operators (whose output is laid out so); kept by accesses that only
+    // drop entries in order, change amplitudes, or change keys without reading
+    // or writing that bit; cleared by every other access (api.hip's accessors),
+    // and never kept once the arrays have been handed to a caller.
+    int32_t pair_bit = -1;
     // Graph capture in progress on this state (capture.hip), or null.
     Capture* capture = nullptr;
     // Replays of a captured sequence are queued and n is on the device until
@@ -229,9 +237,11 @@ uint64_t select_by_conditions(State& st, const sparq_cond_t* conds, uint32_t nco
 
 // ---- branching (branch.hip)
 // unit_basis: st is one entry of amplitude exactly (1, 0) (lets the one-entry
-// expansion decide its ClearZero on the host).
+// expansion decide its ClearZero on the host).  check_dups: throw on duplicate
+// keys before changing anything (the dense-domain path finds them by its slot
+// marks, the grouping path after its sort).
 void hadamard_int(State& st, sparq_field_t target, uint32_t n_digits, bool full_variant, double eps,
-                  bool unit_basis = false);
+                  bool unit_basis = false, bool check_dups = false);
 void replicate(State& st, sparq_field_t f);
 void group_unitary(State& st, sparq_field_t target, const double* mat, double eps);
 // Conditions as a kernel argument (AND of all).

@@ -528,6 +528,26 @@ static void plan_prefetch(PwProgram& prog) {
     }
 }
 
+// Whether a program keeps the entries' grouping in pairs across key bit
+// `bit` (word * 64 + shift): every op that may change keys reads and writes
+// only fields, and evaluates only conditions, that leave the bit out.  Such
+// an op maps a key and its partner to a key and its partner, in place and in
+// the same order; ops that change amplitudes alone keep it whatever they read.
+static bool keeps_pairs(const PwProgram& prog, int32_t bit) {
+    const sparq_field_t b{(uint8_t)(bit / 64), (uint8_t)(bit % 64), 1, 0};
+    for (uint32_t o = 0; o < prog.nops; ++o) {
+        const sparq_pw_op_t& op = prog.ops[o];
+        bool keys_change = false;
+        for (int f = 0; f < 4; ++f) keys_change = keys_change || may_write_field(op, f);
+        if (!keys_change) continue;
+        for (int f = 0; f < 4; ++f)
+            if (op.f[f].width && fields_overlap(op.f[f], b)) return false;
+        for (uint32_t c = 0; c < op.ncond; ++c)
+            if (fields_overlap(prog.conds[op.cond_first + c].field, b)) return false;
+    }
+    return true;
+}
+
 static bool touches_amp(uint32_t opcode) {
     switch (opcode) {
         case SPARQ_PW_PHASE_BITS: case SPARQ_PW_RZ: case SPARQ_PW_Y: case SPARQ_PW_ROT_DIAG:
@@ -715,6 +735,7 @@ void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_c
         prog.conds[c] = conds[c];
     }
     plan_prefetch(prog);
+    if (st.pair_bit >= 0 && (st.capture || !keeps_pairs(prog, st.pair_bit))) st.pair_bit = -1;
     if (st.capture && st.capture->phase == 1) return;   // planning: nothing runs
     if ((st.n == 0 && !st.capture) || nops == 0) return;
     SPQ_DISPATCH_W(st.W, launch_pointwise, st, prog, amp, light);

@@ -117,3 +117,29 @@ def test_duplicate_keys_fall_back(ps):
     (dense, dn), (grouped, _) = _both_paths(ps, spec, vals, amps, lambda st: ps.Hadamard_Int("main", 13)(st))
     assert "dense_scatter_kernel" in dn and "group_start_kernel" in dn
     assert_same_state(dense, grouped, exact_amps=True, what="duplicate keys")
+
+
+@pytest.mark.parametrize("fill,cond", [(0.9, False), (0.05, False), (0.9, True)])
+def test_full_variant_rejects_duplicate_keys(ps, fill, cond):
+    """Hadamard_Int_Full checks for duplicate keys before it changes anything
+    (CheckDuplicateKey, hadamard.cpp:183): through the dense-domain path (its
+    slot marks), the grouped path (after its sort) and, with a condition, on
+    the whole state first.  The state keeps its entries."""
+    spec = [("main", 11), ("anc", 3)]
+    rng = np.random.default_rng(8)
+    vals, amps = _random_rows(rng, spec, 1 << 14, fill=fill)
+    vals = np.concatenate([vals, vals[3:4]])
+    amps = np.concatenate([amps, [0.5 - 0.25j]])
+    ps.System.clear()
+    st = _state(ps, spec, vals, amps)
+    op = ps.Hadamard_Int_Full("main")
+    if cond:
+        op = op.conditioned_by_nonzeros("anc")
+    with pytest.raises(RuntimeError, match="duplicate"):
+        op(st)
+    assert st.size() == len(vals)
+    # without the repeated row the same transform goes through
+    ps.System.clear()
+    st = _state(ps, spec, vals[:-1], amps[:-1])
+    ps.Hadamard_Int_Full("main")(st)
+    assert st.size() > 0

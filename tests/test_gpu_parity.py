@@ -400,6 +400,44 @@ def test_hadamard_bool_and_rotations(ps, oracle):
     assert_same_state(eg, eo, what="pair ops")
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_pair_ops_between_pointwise_programs(ps, oracle, seed):
+    """A pair operator leaves partners across its bit adjacent; the next pair
+    operator on that bit skips its grouping sort when only ops that neither read
+    nor write the bit ran in between (flips and adds of other registers, phases
+    and reflections reading it, ClearZero, Normalize).  Random sequences mixing
+    those with ops that do read or write the bit (a flip of it, a flip
+    conditioned on it, an add into a register holding it, a Hadamard on another
+    bit), compared with the oracle after every operator."""
+    spec = [("a", "Boolean", 1), ("b", "UnsignedInteger", 3), ("c", "Boolean", 1), ("d", "UnsignedInteger", 4)]
+    mat = [complex(0.6, 0.1), complex(-0.3, 0.2), complex(0.5, -0.4), complex(0.2, 0.7)]
+    ops = [
+        lambda m: m.Hadamard_Bool("c"),
+        lambda m: m.Hadamard_Bool("c").conditioned_by_all_ones("a"),
+        lambda m: m.Rot_Bool("c", 0, mat).conditioned_by_all_ones("a"),
+        lambda m: m.Rot_Bool("c", 0, mat),
+        lambda m: m.X_Bool("a", 0),                                     # keeps the grouping on c
+        lambda m: m.Add_UInt_UInt_InPlace("b", "d"),                    # keeps
+        lambda m: m.Reflection_Bool("c", True),                          # amplitude only: keeps
+        lambda m: m.GlobalPhase(complex(0.0, 1.0)),
+        lambda m: m.X_Bool("a", 0).conditioned_by_all_ones("c"),        # reads c
+        lambda m: m.X_Bool("c", 0),                                     # writes c
+        lambda m: m.Hadamard_Bool("a"),                                 # pairs across another bit
+        lambda m: m.ClearZero(1e-14),
+        lambda m: m.Normalize(),
+    ]
+    rng = np.random.default_rng(0xA11 + seed)
+    seq = [int(x) for x in rng.integers(0, len(ops), size=24)]
+    sts = {}
+    for mod in (ps, oracle):
+        mod.System.clear()
+        sts[mod.__name__] = loaded(mod, spec, 90, seed=50 + seed)
+    for step, k in enumerate(seq):
+        for mod in (ps, oracle):
+            ops[k](mod)(sts[mod.__name__])
+        assert_same_state(sts["sparqsim_amd"].export(), sts["_oracle"].export(), what=f"seed {seed} step {step} op {k}")
+
+
 def test_condrot_general(ps, oracle):
     spec = [("r", "UnsignedInteger", 5), ("q", "Boolean", 1), ("z", "UnsignedInteger", 4)]
 
