@@ -426,10 +426,23 @@ int sparq_set_sort_mode(int mode);
 /* Dense-domain Hadamard (no counterpart in the reference): with on != 0 (the
  * default) Hadamard_Int / Hadamard_Int_Full on a state whose low target bits
  * and other varying key bits span at most 2^26 slots, filled to at least a
- * quarter, run on a dense amplitude array (no grouping sort; the result is in
- * canonical key order); 0 always takes the grouped path.  Same states either
- * way (amplitudes to rounding). */
+ * quarter, run on a dense amplitude array (no grouping sort); 0 always takes
+ * the grouped path.  Same states either way (amplitudes to rounding). */
 int sparq_set_dense_transform(int on);
+/* Output order of Hadamard_Int / Hadamard_Int_Full (no counterpart in the
+ * reference): with on != 0 (the default), when every varying key bit outside
+ * the transformed bits lies below them, the output is written in canonical
+ * (target-major) order, so that a following sort finds it presorted; 0 keeps
+ * the group-major order of the transform.  Same states either way. */
+int sparq_set_canonical_output(int on);
+/* Conditioned group unitaries (Rot_GeneralUnitary / Rot_GeneralStatePrep with
+ * conditions; no counterpart in the reference): with on != 0 the groups whose
+ * entries fail the conditions pass through inside the transform (the whole
+ * state is grouped, the output stays in one group order); 0 (the default)
+ * partitions the state first, groups and transforms only the satisfying
+ * entries and appends the others behind them, which measured faster on the
+ * QDA walk.  Same states either way. */
+int sparq_set_group_pass_through(int on);
 /* Entries per thread of the fused pointwise kernel: 1, 2 or 4 forces a shape
  * (4 only for keys of at most two words), 0 (default) lets the launch-shape
  * autotuner pick per class (key words, amplitude access, interpreter, log2 N)

@@ -493,6 +493,15 @@ int sparq_set_dense_transform(int on) {
     return guarded([&] { set_dense_transform(on != 0); });
 }
 
+int sparq_set_canonical_output(int on) {
+    return guarded([&] { set_canonical_output(on != 0); });
+}
+
+static bool g_group_pass = false;
+int sparq_set_group_pass_through(int on) {
+    return guarded([&] { g_group_pass = on != 0; });
+}
+
 int sparq_sort_stats(uint64_t* out, uint32_t count, int reset) {
     return guarded([&] {
         SPQ_REQUIRE(out && count >= 6, "sparq_sort_stats needs room for 6 counters");
@@ -765,8 +774,23 @@ int sparq_group_unitary(sparq_state_t h, sparq_field_t target, const double* mat
     return guarded([&] {
         SPQ_REQUIRE(mat != nullptr, "null matrix");
         require_conditions_off_target(target, target.width, conds, ncond);
+        SPQ_REQUIRE(ncond <= SPARQ_MAX_PROGRAM_CONDS, "too many conditions");
         State& st = S(h);
-        with_conditions(st, conds, ncond, eps, [&](State& s) { group_unitary(s, target, mat, eps); });
+        // Partition first (the default): only the entries that satisfy the
+        // conditions are grouped and transformed.
+        if (ncond && !g_group_pass) {
+            with_conditions(st, conds, ncond, eps, [&](State& s) { group_unitary(s, target, mat, eps); });
+            return;
+        }
+        // The conditions read no target bit, so a group's members agree on
+        // them: groups that fail pass through inside the transform.
+        CondList cl{};
+        cl.n = ncond;
+        for (uint32_t c = 0; c < ncond; ++c) {
+            SPQ_REQUIRE(conds[c].field.word < st.W, "condition field out of range");
+            cl.c[c] = conds[c];
+        }
+        group_unitary(st, target, mat, eps, &cl);
     });
 }
 

@@ -18,10 +18,15 @@
 
 namespace sparq {
 
-enum GroupClass : uint8_t { kDirect = 0, kSparseSum = 1, kWht = 2, kDense = 3 };
+// kPass: a group of a conditioned group unitary whose entries fail the
+// conditions (all members agree: the conditions read no target bit); its
+// members pass through unchanged.
+enum GroupClass : uint8_t { kDirect = 0, kSparseSum = 1, kWht = 2, kDense = 3, kPass = 4 };
 
 static bool g_dense_transform = true;   // the dense-domain Hadamard path (below) where it applies
+static bool g_canonical = true;         // transform outputs in canonical (target-major) order where that applies
 void set_dense_transform(bool on) { g_dense_transform = on; }
+void set_canonical_output(bool on) { g_canonical = on; }
 
 // ---------------------------------------------------------------- groups
 template <int W>
@@ -77,6 +82,29 @@ __global__ void __launch_bounds__(kBlock) classify_kernel(const uint64_t* __rest
     }
 }
 
+// Groups whose head fails the conditions pass through (kPass).
+template <int W>
+__global__ void __launch_bounds__(kBlock) pass_classify_kernel(const uint64_t* __restrict__ keys, uint64_t cap,
+                                                          const uint64_t* __restrict__ start, const uint64_t* __restrict__ G_dev,
+                                                          const CondList cl, uint8_t* __restrict__ cls) {
+    const uint64_t G = G_dev[0];
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < G; g += (uint64_t)gridDim.x * kBlock) {
+        uint64_t k[W];
+        load_key<W>(keys, cap, start[g], k);
+        bool ok = true;
+        for (uint32_t c = 0; c < cl.n; ++c) ok = ok && cond_ok<W>(k, cl.c[c]);
+        if (!ok) cls[g] = kPass;
+    }
+}
+
+// Outputs of each group: its members when it passes through, else 2^nd.
+__global__ void __launch_bounds__(kBlock) group_out_size_kernel(const uint64_t* __restrict__ start, const uint8_t* __restrict__ cls,
+                                                           uint64_t G, uint32_t nd, uint32_t* __restrict__ size) {
+    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= G) return;
+    size[g] = cls[g] == kPass ? (uint32_t)(start[g + 1] - start[g]) : (1u << nd);
+}
+
 constexpr int kExpandItems = 4;   // outputs per thread (strided by the block size)
 
 // Largest output (bytes of keys + amplitudes) a learned capacity may allocate.
@@ -93,12 +121,18 @@ static double learned_budget_bytes() {
 // DIRECT and SPARSE_SUM amplitudes are produced here; WHT blocks start at 0.
 // The group count G is read from the device; the output holds G_cap groups
 // and nothing is written when G > G_cap (a too small predicted capacity).
+// With group offsets `goff` (a conditioned group unitary) group g's outputs
+// start at goff[g] instead: 2^nd of them, or its members unchanged (kPass).
+// With tG = G (no WHT groups, no offsets) output o holds j = o / G of group
+// g = o mod G: the transposed order, canonical when every others bit lies
+// below the target bits.
 template <int W>
 __global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
                                                         const uint64_t* __restrict__ start, const uint8_t* __restrict__ cls, uint32_t nd,
                                                         sparq_field_t low, double extra, const uint64_t* __restrict__ G_dev, uint64_t G_cap,
                                                         uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out,
-                                                        const double2* __restrict__ dense) {
+                                                        const double2* __restrict__ dense, const uint64_t* __restrict__ goff,
+                                                        uint64_t tG) {
     const uint64_t G = G_dev[0];
     if (G > G_cap) return;
     const uint64_t M = G << nd;
@@ -106,14 +140,37 @@ __global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __r
     for (int it = 0; it < kExpandItems; ++it) {
     const uint64_t o = ((uint64_t)blockIdx.x * kExpandItems + it) * kBlock + threadIdx.x;
     if (o >= M) return;
-    const uint64_t g = o >> nd, j = o & ((uint64_t(1) << nd) - 1);
+    uint64_t g, j;
+    if (tG) {
+        if (M <= 0xffffffffull) {
+            g = (uint32_t)o % (uint32_t)tG;
+            j = (uint32_t)o / (uint32_t)tG;
+        } else {
+            g = o % tG;
+            j = o / tG;
+        }
+    } else {
+        g = o >> nd;
+        j = o & ((uint64_t(1) << nd) - 1);
+    }
     const uint64_t h = start[g];
+    const uint8_t c = cls[g];
+    if (c == kPass) {   // members j, j + 2^nd, ... to their place in the group's output
+        const uint64_t e = start[g + 1];
+        for (uint64_t m = h + j; m < e; m += uint64_t(1) << nd) {
+            uint64_t km[W];
+            load_key<W>(keys, cap, m, km);
+            store_key<W>(kout, cap_out, goff[g] + (m - h), km);
+            aout[goff[g] + (m - h)] = amps[m];
+        }
+        continue;
+    }
+    const uint64_t out = goff ? goff[g] + j : o;
     uint64_t k[W];
     load_key<W>(keys, cap, h, k);
     const uint64_t vlow = fget<W>(k, low);
     fput<W>(k, low, j);
-    store_key<W>(kout, cap_out, o, k);
-    const uint8_t c = cls[g];
+    store_key<W>(kout, cap_out, out, k);
     double2 r = make_double2(0.0, 0.0);
     if (c == kDirect) {
         // amp * 2^{-n/2}, then * -1 on odd parity (hadamard.cpp:30-39)
@@ -139,7 +196,7 @@ __global__ void __launch_bounds__(kBlock) expand_init_kernel(const uint64_t* __r
             r = cadd(r, cmul(dense[j * F + fget<W>(km, low)], amps[m]));
         }
     }
-    aout[o] = r;
+    aout[out] = r;
     }
 }
 
@@ -412,6 +469,7 @@ static void launch_wht(State& st, double2* a, const uint64_t* counts, uint64_t G
 // each slot it writes; a state with duplicate keys (fewer marks than
 // entries) keeps the grouped path.
 constexpr uint32_t kDenseMaxLog = 28;   // 2^28 slots: 4 GiB of amplitudes
+constexpr uint32_t kTransposeMaxLog = 8;   // widest others index emitted in transposed (canonical) order
 constexpr int kDenseTile = 4096;        // slots per workgroup of the count / emit kernels
 constexpr int kDenseItems = kDenseTile / kBlock;
 
@@ -537,11 +595,216 @@ __global__ void __launch_bounds__(kBlock) dense_emit_kernel(const double2* __res
     }
 }
 
-// {marks = 0, groups, WHT groups = 1} after the kept total.
+// ---- transposed (canonical) emit order
+// A group-major array of Gp = 2^tg groups of 2^nd slots (group g at
+// [g << nd, (g + 1) << nd)) read in the transposed order: position
+// p = (j << tg) | g holds slot (g << nd) | j.  When every varying key bit
+// outside the low target bits lies below them, this order is canonical.
+// A tile of kTxTile positions covers every group at C = kTxTile >> tg
+// consecutive j: its amplitudes come in as C-long row segments (coalesced)
+// through LDS and leave in position order.  Groups g >= G, or whose class is
+// not kWht when `cls` is given, are empty.
+constexpr int kTxTile = 2048;
+constexpr int kTxItems = kTxTile / kBlock;
+
+// Kept slots counted in slot order (streaming), per tile of kDenseTile slots
+// (`tile_kept`, optional) and per chunk of C = 2^lc slots of one group
+// (`chunk_cnt`, [Gp][2^nd / C], optional): tile t of the transposed order
+// holds chunk t of every group, so its count is a column sum of chunk_cnt.
+// Totals of kept slots and of marks (optional) go to totals[0], totals[1].
+// Groups g >= G, or whose class is not kWht when `cls` is given, are empty.
+// Needs nd >= 12: a tile of slots lies in one group.
+__global__ void __launch_bounds__(kBlock) chunk_count_kernel(const double2* __restrict__ A, const uint8_t* __restrict__ cls,
+                                                        const uint8_t* __restrict__ mark, uint64_t G, uint32_t nd, uint32_t lc,
+                                                        double eps, uint32_t* __restrict__ tile_kept,
+                                                        uint32_t* __restrict__ chunk_cnt, unsigned long long* __restrict__ totals) {
+    __shared__ uint32_t cc[kDenseTile / 8];
+    __shared__ uint32_t red[2][kBlock / 64];
+    const uint64_t s0 = (uint64_t)blockIdx.x * kDenseTile;
+    const uint64_t g = s0 >> nd;
+    const uint32_t nch = chunk_cnt ? (uint32_t)kDenseTile >> lc : 0;   // <= kDenseTile / 8: lc >= 3
+    for (uint32_t x = threadIdx.x; x < nch; x += kBlock) cc[x] = 0;
+    __syncthreads();
+    const bool present = g < G && (!cls || cls[g] == kWht);
+    uint32_t c = 0, m = 0;
+    if (present) {
+#pragma unroll
+        for (int it = 0; it < kDenseItems; ++it) {
+            const uint32_t off = (uint32_t)it * kBlock + threadIdx.x;
+            const bool keep = !(cabs2(A[s0 + off]) < eps);
+            if (chunk_cnt && keep) atomicAdd(&cc[off >> lc], 1u);
+            c += keep;
+            if (mark) m += mark[s0 + off];
+        }
+    }
+    c = wave_sum(c);
+    m = wave_sum(m);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = c;
+        red[1][threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tc = 0, tm = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            tc += red[0][w];
+            tm += red[1][w];
+        }
+        if (tile_kept) tile_kept[blockIdx.x] = tc;
+        if (tc) atomicAdd(&totals[0], (unsigned long long)tc);
+        if (mark && tm) atomicAdd(&totals[1], (unsigned long long)tm);
+    }
+    if (chunk_cnt) {
+        const uint64_t T = (uint64_t(1) << nd) >> lc;
+        const uint64_t t0 = (s0 & ((uint64_t(1) << nd) - 1)) >> lc;
+        for (uint32_t x = threadIdx.x; x < nch; x += kBlock) chunk_cnt[g * T + t0 + x] = cc[x];
+    }
+}
+
+// Tile t of the transposed order: the sum over the Gp groups of chunk t.
+__global__ void __launch_bounds__(kBlock) chunk_column_kernel(const uint32_t* __restrict__ chunk_cnt, uint64_t Gp, uint64_t T,
+                                                         uint32_t* __restrict__ tile_cnt) {
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= T) return;
+    uint32_t sum = 0;
+    for (uint64_t g = 0; g < Gp; ++g) sum += chunk_cnt[g * T + t];
+    tile_cnt[t] = sum;
+}
+
+// Kept positions in position order.  The tile's amplitudes (and, with KEYS,
+// the keys from K, group-major like A) come in as row segments through LDS;
+// without KEYS the keys are rebuilt from the slot by the dense map.
+constexpr uint32_t kTxMaxKeyWords = 2;   // LDS: 36 KiB of amplitudes + 18 KiB per key word
+template <int W, bool KEYS>
+__global__ void __launch_bounds__(kBlock) tx_emit_kernel(const double2* __restrict__ A, const uint8_t* __restrict__ cls,
+                                                    const uint64_t* __restrict__ K, uint64_t cap_in, DenseMap dm, uint64_t G,
+                                                    uint32_t tg, uint32_t nd, double eps, const uint64_t* __restrict__ offs,
+                                                    uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out) {
+    constexpr int kWaves = kBlock / 64;
+    constexpr int kPadded = kTxTile + 256;   // row g at g * (C + 1): C + 1 odd, so a column read spreads over the banks
+    __shared__ double2 la[kPadded];
+    __shared__ uint64_t lk[KEYS ? W : 1][KEYS ? kPadded : 1];
+    __shared__ uint32_t row_wave[kTxItems * kWaves];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t lc = (uint32_t)__builtin_ctz(kTxTile) - tg;
+    const uint32_t C = 1u << lc;
+    const uint64_t j0 = (uint64_t)blockIdx.x << lc;
+    const uint64_t F = uint64_t(1) << nd;
+#pragma unroll
+    for (int r = 0; r < kTxItems; ++r) {   // row segments in
+        const uint32_t q = (uint32_t)r * kBlock + threadIdx.x;
+        const uint32_t g = q >> lc, jj = q & (C - 1);
+        const bool ok = g < G && j0 + jj < F && (!cls || cls[g] == kWht);
+        const uint64_t slot = ((uint64_t)g << nd) | (j0 + jj);
+        la[g * (C + 1) + jj] = ok ? A[slot] : make_double2(0.0, 0.0);
+        if (KEYS && ok)
+#pragma unroll
+            for (int w = 0; w < W; ++w) lk[w][g * (C + 1) + jj] = K[(uint64_t)w * cap_in + slot];
+    }
+    __syncthreads();
+    double2 v[kTxItems];
+    uint32_t flags = 0, lane_pre[kTxItems];
+#pragma unroll
+    for (int r = 0; r < kTxItems; ++r) {   // position order out
+        const uint32_t x = (uint32_t)r * kBlock + threadIdx.x;
+        const uint32_t g = x & ((1u << tg) - 1), jj = x >> tg;
+        const bool present = g < G && j0 + jj < F && (!cls || cls[g] == kWht);
+        v[r] = la[g * (C + 1) + jj];
+        const bool keep = present && !(cabs2(v[r]) < eps);
+        const uint64_t m = __ballot(keep);
+        flags |= uint32_t(keep) << r;
+        lane_pre[r] = (uint32_t)__popcll(m & lanemask_lt());
+        if (lane == 0) row_wave[r * kWaves + wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {   // exclusive scan of the (row, wave) counts in position order
+        const uint32_t x = threadIdx.x < kTxItems * kWaves ? row_wave[threadIdx.x] : 0;
+        const uint32_t inc = wave_incl_scan(x);
+        if (threadIdx.x < kTxItems * kWaves) row_wave[threadIdx.x] = inc - x;
+    }
+    __syncthreads();
+    const uint64_t before = offs[blockIdx.x];
+#pragma unroll
+    for (int r = 0; r < kTxItems; ++r) {
+        if (!((flags >> r) & 1)) continue;
+        const uint32_t x = (uint32_t)r * kBlock + threadIdx.x;
+        const uint32_t g = x & ((1u << tg) - 1), jj = x >> tg;
+        const uint64_t dst = before + row_wave[r * kWaves + wave] + lane_pre[r];
+        uint64_t k[W];
+        if (KEYS) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) k[w] = lk[KEYS ? w : 0][KEYS ? g * (C + 1) + jj : 0];
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) k[w] = dm.base[w];
+            uint32_t at = 0;
+            for (uint32_t q = 0; q < dm.nseg; ++q) {
+                const uint64_t bits = ((uint64_t)g >> at) & fmask(dm.width[q]);
+                kset_word<W>(k, dm.word[q], kword<W>(k, dm.word[q]) | (bits << dm.shift[q]));
+                at += dm.width[q];
+            }
+            fput<W>(k, dm.low, j0 + jj);
+        }
+        store_key<W>(kout, cap_out, dst, k);
+        aout[dst] = v[r];
+    }
+}
+
+// Tiles of the transposed order over Gp = 2^tg groups of 2^nd slots.
+static uint64_t tx_tiles(uint32_t tg, uint32_t nd) {
+    const uint64_t P = uint64_t(1) << (tg + nd);
+    return (P + kTxTile - 1) / kTxTile;
+}
+
+// The state holds a group-major expansion of G groups of 2^nd entries:
+// ClearZero it into the transposed order.
+template <int W>
+static void transposed_clear_zero(State& st, uint64_t G, uint32_t nd, double eps) {
+    if constexpr (W > (int)kTxMaxKeyWords) {
+        clear_zero(st, eps);
+        return;
+    }
+    uint32_t tg = 0;
+    while ((uint64_t(1) << tg) < G) ++tg;
+    const uint32_t lc = (uint32_t)__builtin_ctz(kTxTile) - tg;
+    const uint64_t tiles = tx_tiles(tg, nd);                 // = 2^nd / C
+    const uint64_t slot_tiles = (G << nd) / kDenseTile;      // nd >= 12
+    uint32_t* chunk = dalloc<uint32_t>(G * tiles, st.stream);
+    uint32_t* kept = dalloc<uint32_t>(tiles, st.stream);
+    uint64_t* offs = dalloc<uint64_t>(tiles + 3, st.stream);   // [tiles] total, [tiles + 1, + 3) atomic totals
+    SPQ_HIP(hipMemsetAsync(offs + tiles + 1, 0, 2 * sizeof(uint64_t), st.stream));
+    {
+        SPQ_PROF(st.stream, "chunk_count_kernel", (double)(G << nd) * 16.0);
+        chunk_count_kernel<<<(unsigned)slot_tiles, kBlock, 0, st.stream>>>(st.amps, nullptr, nullptr, G, nd, lc, eps, nullptr, chunk,
+                                                                           (unsigned long long*)(offs + tiles + 1));
+    }
+    SPQ_HIP(hipGetLastError());
+    chunk_column_kernel<<<grid_for(tiles, kBlock), kBlock, 0, st.stream>>>(chunk, G, tiles, kept);
+    SPQ_HIP(hipGetLastError());
+    exclusive_scan_u32(st, kept, offs, tiles, offs + tiles);
+    const uint64_t total = read_u64(st, offs + tiles);
+    dfree(chunk, st.stream);
+    const uint64_t cap_out = std::max<uint64_t>(total, 1);
+    uint64_t* kout;
+    double2* aout;
+    alloc_arrays(st, st.W, cap_out, &kout, &aout);
+    {
+        SPQ_PROF(st.stream, "tx_emit_kernel", (double)(G << nd) * 16.0 + (double)total * (8.0 * W + 16));
+        tx_emit_kernel<W, (W <= (int)kTxMaxKeyWords)><<<(unsigned)tiles, kBlock, 0, st.stream>>>(st.amps, nullptr, st.keys, st.cap, DenseMap{}, G, tg, nd, eps,
+                                                                           offs, kout, aout, cap_out);
+    }
+    SPQ_HIP(hipGetLastError());
+    install_arrays(st, st.W, kout, aout, total, cap_out);
+    dfree(kept, st.stream);
+    dfree(offs, st.stream);
+}
+
+// {kept total = 0, marks = 0, groups, WHT groups = 1}.
 __global__ void dense_init_kernel(uint64_t* __restrict__ tail, uint64_t groups) {
     tail[0] = 0;
-    tail[1] = groups;
-    tail[2] = 1;
+    tail[1] = 0;
+    tail[2] = groups;
+    tail[3] = 1;
 }
 
 // The dense-domain transform, when it applies: returns false (state untouched)
@@ -575,18 +838,30 @@ static bool hadamard_dense(State& st, const KeyStats& ks, sparq_field_t low, uin
     }
     for (uint32_t w = 0; w < st.W; ++w)
         dm.base[w] = ks.ones[w] & ~ks.vary[w] & (w == low.word ? ~lowmask : ~0ull);
+    // The output may go in canonical order (low target value major): when
+    // every others bit lies below the low target bits and the others index is
+    // narrow enough for a tile of the transposed order to cover every group.
+    // It does when the output is not much smaller than the input (a spread
+    // state, which the next sort would otherwise reorder in full).
+    bool tx_ok = t_o <= kTransposeMaxLog && nd >= 12 && g_canonical;
+    for (uint32_t q = 0; q < dm.nseg; ++q)
+        tx_ok = tx_ok && (dm.word[q] > low.word || (dm.word[q] == low.word && dm.shift[q] + dm.width[q] <= low.shift));
     const uint64_t slots = uint64_t(1) << T, groups = uint64_t(1) << t_o;
     const uint32_t tiles = (uint32_t)(slots / kDenseTile);
+    const uint32_t lc = tx_ok ? (uint32_t)__builtin_ctz(kTxTile) - t_o : 0;
+    const uint64_t txt = tx_ok ? tx_tiles(t_o, nd) : 0;   // tiles of the transposed order
     double2* A = dalloc<double2>(slots, st.stream);
     uint8_t* mark = dalloc<uint8_t>(slots, st.stream);
     uint8_t* cls = dalloc<uint8_t>(groups, st.stream);
-    uint32_t* kept = dalloc<uint32_t>(tiles, st.stream);
-    uint64_t* offs = dalloc<uint64_t>(tiles + 4, st.stream);   // [tiles] kept total, [tiles + 1] marks, [tiles + 2, + 4) {groups, WHT groups}
-    uint64_t* gcount = offs + tiles + 2;
+    uint32_t* kept = dalloc<uint32_t>(std::max<uint64_t>(tiles, txt), st.stream);
+    uint32_t* chunk = tx_ok ? dalloc<uint32_t>(groups * txt, st.stream) : nullptr;
+    uint64_t* offs = dalloc<uint64_t>(std::max<uint64_t>(tiles, txt) + 1, st.stream);
+    uint64_t* tail = dalloc<uint64_t>(4, st.stream);   // {kept total, marks, groups, WHT groups}
+    uint64_t* gcount = tail + 2;
     SPQ_HIP(hipMemsetAsync(A, 0, slots * sizeof(double2), st.stream));
     SPQ_HIP(hipMemsetAsync(mark, 0, slots, st.stream));
     SPQ_HIP(hipMemsetAsync(cls, kDirect, groups, st.stream));
-    dense_init_kernel<<<1, 1, 0, st.stream>>>(offs + tiles + 1, groups);
+    dense_init_kernel<<<1, 1, 0, st.stream>>>(tail, groups);
     SPQ_HIP(hipGetLastError());
     {
         SPQ_PROF(st.stream, "dense_scatter_kernel", n * (8.0 * W + 16 + 16 + 2));
@@ -596,28 +871,50 @@ static bool hadamard_dense(State& st, const KeyStats& ks, sparq_field_t low, uin
     // profiled bytes: 32 per entry, a lower bound (the slots of occupied groups
     // move; which groups are occupied is known only on the device)
     launch_wht(st, A, gcount, groups, nd, cls, (double)n * 32.0);
-    {
-        SPQ_PROF(st.stream, "dense_count_kernel", (double)n * 17.0);   // lower bound, as the butterflies
-        dense_count_kernel<<<tiles, kBlock, 0, st.stream>>>(A, mark, cls, nd, eps, kept, (unsigned long long*)(offs + tiles + 1));
+    // counts per tile of slots (and per chunk for the transposed order); a
+    // tile lies in one group from nd = 12 on, else the per-slot class lookup
+    const bool chunked = nd >= 12;
+    if (chunked) {
+        SPQ_PROF(st.stream, "chunk_count_kernel", (double)n * 17.0);   // lower bound, as the butterflies
+        chunk_count_kernel<<<tiles, kBlock, 0, st.stream>>>(A, cls, mark, groups, nd, lc, eps, kept, chunk, (unsigned long long*)tail);
+    } else {
+        {
+            SPQ_PROF(st.stream, "dense_count_kernel", (double)n * 17.0);
+            dense_count_kernel<<<tiles, kBlock, 0, st.stream>>>(A, mark, cls, nd, eps, kept, (unsigned long long*)(tail + 1));
+        }
+        SPQ_HIP(hipGetLastError());
+        exclusive_scan_u32(st, kept, offs, tiles, tail);
     }
     SPQ_HIP(hipGetLastError());
-    exclusive_scan_u32(st, kept, offs, tiles, offs + tiles);
     uint64_t info[2];   // kept total, marks
-    read_back(st, offs + tiles, sizeof(info), info);
+    read_back(st, tail, sizeof(info), info);
     const bool ok = info[1] == n;
     if (ok) {
         const uint64_t total = info[0];
+        const bool tx = tx_ok && total * 4 >= n;
+        if (tx) {
+            chunk_column_kernel<<<grid_for(txt, kBlock), kBlock, 0, st.stream>>>(chunk, groups, txt, kept);
+            SPQ_HIP(hipGetLastError());
+        }
+        const uint64_t et = tx ? txt : tiles;
+        if (chunked) exclusive_scan_u32(st, kept, offs, et, offs + et);
         const uint64_t cap_out = std::max<uint64_t>(total, 1);
         uint64_t* kout;
         double2* aout;
         alloc_arrays(st, st.W, cap_out, &kout, &aout);
         {
-            SPQ_PROF(st.stream, "dense_emit_kernel", (double)n * 16.0 + (double)total * (8.0 * W + 16));   // lower bound
-            dense_emit_kernel<W><<<tiles, kBlock, 0, st.stream>>>(A, cls, dm, eps, offs, kout, aout, cap_out);
+            SPQ_PROF(st.stream, tx ? "tx_emit_kernel" : "dense_emit_kernel", (double)n * 16.0 + (double)total * (8.0 * W + 16));   // lower bound
+            if (tx)
+                tx_emit_kernel<W, false><<<(unsigned)txt, kBlock, 0, st.stream>>>(A, cls, nullptr, 0, dm, groups, t_o, nd, eps, offs, kout,
+                                                                                  aout, cap_out);
+            else
+                dense_emit_kernel<W><<<tiles, kBlock, 0, st.stream>>>(A, cls, dm, eps, offs, kout, aout, cap_out);
         }
         SPQ_HIP(hipGetLastError());
         install_arrays(st, st.W, kout, aout, total, cap_out);
     }
+    if (chunk) dfree(chunk, st.stream);
+    dfree(tail, st.stream);
     dfree(A, st.stream);
     dfree(mark, st.stream);
     dfree(cls, st.stream);
@@ -628,13 +925,18 @@ static bool hadamard_dense(State& st, const KeyStats& ks, sparq_field_t low, uin
 
 template <int W>
 static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool full, double eps, const double2* dense,
-                          bool allow_learned = true, bool unit_basis = false, bool check_dups = false) {
+                          bool allow_learned = true, bool unit_basis = false, bool check_dups = false,
+                          const CondList* pass = nullptr) {
     const uint64_t n = st.n;
     const uint64_t F = uint64_t(1) << nd;
     sparq_field_t low{target.word, target.shift, (uint8_t)nd, 0};
     // Sort by (everything else, low target bits) unless already in that order;
     // when no bit outside the low target bits varies there is a single group.
-    bool single = (n == 1);
+    // Conditions (group unitaries only) take the general path, which finds the
+    // groups that pass through.
+    if (pass && pass->n == 0) pass = nullptr;
+    bool single = (n == 1) && !pass;
+    bool others_below = true;   // every varying bit outside the low target bits lies below them
     KeyStats ks;
     if (n > 1) {
         const SizerQuery q{n, st.W, nd, dense ? 2u : (full ? 1u : 0u), low.word, low.shift};
@@ -644,11 +946,14 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
         if (!dense && g_dense_transform && hadamard_dense<W>(st, ks, low, nd, eps)) return;
         if (!ks.sorted) sort_with_stats(st, ks, &low);
         if (check_dups && has_sorted_duplicates(st)) throw Error(SPARQ_RUNTIME, "Hadamard_Int_Full: duplicate key in state");
-        single = true;
+        single = !pass;
         for (uint32_t w = 0; w < st.W; ++w) {
             uint64_t v = ks.vary[w];
             if (w == low.word) v &= ~(fmask_host(nd) << low.shift);
             if (v) single = false;
+            // an others bit above the low target bits: group-major order is
+            // not canonical transposed
+            if (v && (w < low.word || (w == low.word && (v >> low.shift) != 0))) others_below = false;
         }
     }
     uint32_t* head = nullptr;
@@ -723,9 +1028,14 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
         }
         SPQ_HIP(hipGetLastError());
         SPQ_HIP(hipMemcpyAsync(counts, pos + n, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, st.stream));
+        if (pass) {
+            pass_classify_kernel<W><<<std::min<unsigned>(grid_for(n, kBlock), 2048), kBlock, 0, st.stream>>>(st.keys, st.cap, start,
+                                                                                                       pos + n, *pass, cls);
+            SPQ_HIP(hipGetLastError());
+        }
         // With the sizer active the output is sized by the predicted group count and
         // the exact count stays on the device until the final readback.
-        if (ks.have_prediction) {
+        if (ks.have_prediction && !pass) {
             // The prediction refines the exact bound G <= min(n, 2^(bits varying
             // outside the low target bits)) and never exceeds it.
             int vo = 0;
@@ -754,17 +1064,35 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
         }
     }
     SPQ_REQUIRE(G_cap <= (~0ull >> nd), "Hadamard output size overflows");
+    // no butterflies: the outputs can be written straight in the transposed
+    // (canonical) order
+    const uint64_t tG = (g_canonical && !pass && !learned && n_wht == 0 && others_below && G > 1) ? G : 0;
+    // conditioned: groups that pass through keep their members; the output
+    // stays in group order
+    uint64_t* goff = nullptr;
+    uint64_t M_out = G_cap << nd;
+    if (pass) {
+        uint32_t* gsize = dalloc<uint32_t>(G, st.stream);
+        goff = dalloc<uint64_t>(G + 1, st.stream);
+        group_out_size_kernel<<<grid_for(G, kBlock), kBlock, 0, st.stream>>>(start, cls, G, nd, gsize);
+        SPQ_HIP(hipGetLastError());
+        exclusive_scan_u32(st, gsize, goff, G, goff + G);
+        M_out = read_u64(st, goff + G);
+        dfree(gsize, st.stream);
+    }
     const uint64_t M_cap = G_cap << nd;
     uint64_t* kout;
     double2* aout;
-    alloc_arrays(st, st.W, M_cap, &kout, &aout);
+    alloc_arrays(st, st.W, std::max<uint64_t>(M_out, 1), &kout, &aout);
     const double extra = 1.0 / std::sqrt((double)F);
     {
-        SPQ_PROF(st.stream, "expand_init_kernel", M_cap * (8.0 * W + 16));
+        SPQ_PROF(st.stream, "expand_init_kernel", M_out * (8.0 * W + 16));
         expand_init_kernel<W><<<grid_for(M_cap, kBlock * kExpandItems), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, start, cls, nd, low, extra, counts,
-                                                                             G_cap, kout, aout, M_cap, dense);
+                                                                             G_cap, kout, aout, std::max<uint64_t>(M_out, 1), dense, goff,
+                                                                             tG);
     }
     SPQ_HIP(hipGetLastError());
+    if (goff) dfree(goff, st.stream);
     if (n_wht || learned) {
         {
             SPQ_PROF(st.stream, "expand_scatter_kernel", n * (8.0 * W + 16));
@@ -780,8 +1108,14 @@ static void hadamard_impl(State& st, sparq_field_t target, uint32_t nd, bool ful
     dfree(cls, st.stream);
     if (!learned) {
         dfree(counts, st.stream);
-        install_arrays(st, st.W, kout, aout, G << nd, M_cap);
-        clear_zero(st, eps);
+        install_arrays(st, st.W, kout, aout, pass ? M_out : G << nd, std::max<uint64_t>(M_out, 1));
+        // output in canonical order when the groups' others bits lie below
+        // the target bits (the next sort then finds it sorted)
+        if (g_canonical && !tG && !pass && others_below && G > 1 && G <= (uint64_t(1) << kTransposeMaxLog) && st.W <= kTxMaxKeyWords &&
+            nd >= 12)
+            transposed_clear_zero<W>(st, G, nd, eps);
+        else
+            clear_zero(st, eps);
         return;
     }
     // One readback for the exact group count, the overflow check and ClearZero.
@@ -821,7 +1155,7 @@ void hadamard_int(State& st, sparq_field_t target, uint32_t nd, bool full, doubl
                    check_dups);
 }
 
-void group_unitary(State& st, sparq_field_t target, const double* mat, double eps) {
+void group_unitary(State& st, sparq_field_t target, const double* mat, double eps, const CondList* cl) {
     SPQ_REQUIRE(target.word < st.W && target.width >= 1 && target.width <= 6 && (uint32_t)target.shift + target.width <= 64,
                 "group unitary needs a register of 1..6 bits");
     if (st.n == 0) return;
@@ -829,7 +1163,7 @@ void group_unitary(State& st, sparq_field_t target, const double* mat, double ep
     double2* dense = dalloc<double2>(F * F, st.stream);
     SPQ_HIP(hipMemcpyAsync(dense, mat, F * F * sizeof(double2), hipMemcpyHostToDevice, st.stream));
     SPQ_HIP(hipStreamSynchronize(st.stream));   // the host matrix may be released after return
-    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, (uint32_t)target.width, false, eps, (const double2*)dense);
+    SPQ_DISPATCH_W(st.W, hadamard_impl, st, target, (uint32_t)target.width, false, eps, (const double2*)dense, true, false, false, cl);
     dfree(dense, st.stream);
 }
 

@@ -158,6 +158,7 @@ void set_sort_mode(int mode);
 // Hadamard_Int on a small, well filled key domain through a dense amplitude
 // array (on by default); off: always the grouped path.
 void set_dense_transform(bool on);
+void set_canonical_output(bool on);
 // Entries per thread of the pointwise kernel: 1, 2 or 4 forced, 0 autotuned.
 void set_pointwise_items(int items);
 // The autotuners' measurements and choices as JSON arrays.
@@ -243,7 +244,10 @@ uint64_t select_by_conditions(State& st, const sparq_cond_t* conds, uint32_t nco
 void hadamard_int(State& st, sparq_field_t target, uint32_t n_digits, bool full_variant, double eps,
                   bool unit_basis = false, bool check_dups = false);
 void replicate(State& st, sparq_field_t f);
-void group_unitary(State& st, sparq_field_t target, const double* mat, double eps);
+// With conditions (which read no target bit), groups whose entries fail them
+// pass through unchanged, in place in the group-ordered output.
+struct CondList;
+void group_unitary(State& st, sparq_field_t target, const double* mat, double eps, const CondList* cl = nullptr);
 // Conditions as a kernel argument (AND of all).
 struct CondList {
     uint32_t n;

@@ -603,6 +603,12 @@ PYBIND11_MODULE(_core, m) {
         py::arg("reset") = false, "Sorts by path since the last reset: prefix + local chunk sort, overflowed local attempts, LSD passes, chunk sorts of states already sorted by their prefix.");
     m.def("set_dense_transform", [](bool on) { check(sparq_set_dense_transform(on ? 1 : 0)); }, py::arg("on"),
           "Hadamard_Int on a small, well filled key domain through a dense amplitude array (default on).");
+    m.def("set_canonical_output", [](bool on) { check(sparq_set_canonical_output(on ? 1 : 0)); }, py::arg("on"),
+          "Hadamard outputs in canonical (target-major) order where every other varying bit lies below the target "
+          "(default on).");
+    m.def("set_group_pass_through", [](bool on) { check(sparq_set_group_pass_through(on ? 1 : 0)); }, py::arg("on"),
+          "Conditioned group unitaries pass failing groups through inside the transform (on) or partition the state "
+          "first (off, the default).");
     m.def("set_sort_mode", [](int mode) { check(sparq_set_sort_mode(mode)); }, py::arg("mode"),
           "Sort strategy: 0 prefix passes + LDS chunk sort where it applies (default), 1 LSD passes only.");
     m.def("set_radix_bits", [](int bits) { check(sparq_set_radix_bits(bits)); }, py::arg("bits"),

@@ -5,8 +5,10 @@ the final Hadamard is one entry and the state fans out to 2^32 entries
 Too large for the oracle or a host copy: checked on the device by its closed
 form -- the entry count, the norm (exactly 1: every |a|^2 is 2^-32), and
 sampled ranges of entries, each of which must be (a', b, M[b]) at position
-b * 2^16 + a' with amplitude exactly (-1)^popcount(a' & b) 2^-16 (the first H
-gives 2^-8, the one-entry group multiplies by 2^-8)."""
+a' * 2^16 + b (canonical order: a is the most significant register, and the
+one-entry groups' outputs are written target-major) with amplitude exactly
+(-1)^popcount(a' & b) 2^-16 (the first H gives 2^-8, the one-entry group
+multiplies by 2^-8)."""
 import ctypes
 import time
 
@@ -55,8 +57,8 @@ def test_config1_literal_2p32_final_step(ps):
         a = (keys >> np.uint64(shifts["a"])) & np.uint64(0xFFFF)
         b = (keys >> np.uint64(shifts["b"])) & np.uint64(0xFFFF)
         d = (keys >> np.uint64(shifts["d"])) & np.uint64(0xFFFF)
-        assert np.array_equal(a, pos & np.uint64(0xFFFF)), first
-        assert np.array_equal(b, pos >> np.uint64(16)), first
+        assert np.array_equal(a, pos >> np.uint64(16)), first
+        assert np.array_equal(b, pos & np.uint64(0xFFFF)), first
         assert np.array_equal(d, mem[b.astype(np.int64)]), first
         assert np.all(keys & np.uint64(0xFFFF) == 0), first
         sign = 1.0 - 2.0 * (np.bitwise_count(a & b) & 1)

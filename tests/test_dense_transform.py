@@ -143,3 +143,100 @@ def test_full_variant_rejects_duplicate_keys(ps, fill, cond):
     st = _state(ps, spec, vals[:-1], amps[:-1])
     ps.Hadamard_Int_Full("main")(st)
     assert st.size() > 0
+
+
+@pytest.mark.parametrize("fill,kernel", [(0.9, "tx_emit_kernel"), (0.05, "tx_emit_kernel")])
+def test_output_in_canonical_order(ps, oracle, fill, kernel):
+    """With every other varying bit below the target, Hadamard_Int_Full emits
+    in canonical order (target value major): through the dense-domain path
+    (fill 0.9) and through the grouped expansion (fill 0.05, a few groups).
+    The rows come back sorted, and equal the oracle's."""
+    spec = [("main", 12), ("anc", 3)]
+    rng = np.random.default_rng(9)
+    vals, amps = _random_rows(rng, spec, 1 << 14, fill=fill)
+    ps.System.clear()
+    st = _state(ps, spec, vals, amps)
+    eg, names = _kernels(ps, lambda: (ps.Hadamard_Int_Full("main")(st), st.export())[1])
+    assert kernel in names and (("dense_scatter_kernel" in names) == (fill > 0.5))
+    oracle.System.clear()
+    so = _state(oracle, spec, vals, amps)
+    oracle.Hadamard_Int_Full("main")(so)
+    assert_same_state(eg, so.export(), what=f"fill {fill}")
+    v = np.asarray(eg[1], dtype=np.uint64)
+    order = np.lexsort(tuple(v[:, k] for k in reversed(range(v.shape[1]))))
+    assert np.array_equal(order, np.arange(len(order))), "output not in canonical order"
+
+
+def test_concentrating_transform_keeps_slot_order(ps, oracle):
+    """A transform whose output is far smaller than its input (uniform rows of
+    main per anc value go to main = 0) takes the dense path with the slot-order
+    emit; the result equals the oracle's."""
+    spec = [("main", 12), ("anc", 3)]
+    rows = np.array([[m, a] for a in (1, 4, 6, 7) for m in range(1 << 12)], dtype=np.uint64)
+    amps = np.repeat(np.array([0.5, -0.25j, 0.125 + 0.5j, 0.3]), 1 << 12) / np.sqrt(1 << 12)
+    amps = amps / np.linalg.norm(amps)
+    ps.System.clear()
+    st = _state(ps, spec, rows, amps)
+    eg, names = _kernels(ps, lambda: (ps.Hadamard_Int_Full("main")(st), st.export())[1])
+    assert "dense_scatter_kernel" in names and "dense_emit_kernel" in names
+    assert len(eg[2]) == 4
+    oracle.System.clear()
+    so = _state(oracle, spec, rows, amps)
+    oracle.Hadamard_Int_Full("main")(so)
+    assert_same_state(eg, so.export(), what="concentrating transform")
+
+
+@pytest.mark.parametrize("full", [True, False])
+def test_expansion_without_butterflies_in_canonical_order(ps, oracle, full):
+    """Groups of one entry (Hadamard_Int: direct expansion) or of a few
+    (Hadamard_Int_Full: sparse sums) need no butterflies: their outputs are
+    written straight in canonical order.  Rows come back sorted and equal the
+    oracle's."""
+    spec = [("main", 12), ("anc", 3)]
+    rng = np.random.default_rng(10)
+    rows = []
+    for a in range(8):
+        for m in rng.choice(1 << 12, size=1 if not full else 1 + a % 3, replace=False):
+            rows.append([int(m), a])
+    rows = np.array(rows, dtype=np.uint64)
+    amps = rng.standard_normal(len(rows)) + 1j * rng.standard_normal(len(rows))
+    amps /= np.linalg.norm(amps)
+
+    def op(mod, st):
+        if full:
+            mod.Hadamard_Int_Full("main")(st)
+        else:
+            mod.Hadamard_Int("main", 12)(st)
+
+    ps.System.clear()
+    st = _state(ps, spec, rows, amps)
+    eg, names = _kernels(ps, lambda: (op(ps, st), st.export())[1])
+    assert "expand_init_kernel" in names and "tx_emit_kernel" not in names and "wht_first_pass_kernel" not in names
+    oracle.System.clear()
+    so = _state(oracle, spec, rows, amps)
+    op(oracle, so)
+    assert_same_state(eg, so.export(), what=f"full={full}")
+    v = np.asarray(eg[1], dtype=np.uint64)
+    order = np.lexsort(tuple(v[:, k] for k in reversed(range(v.shape[1]))))
+    assert np.array_equal(order, np.arange(len(order))), "output not in canonical order"
+
+
+@pytest.mark.parametrize("fill", [0.9, 0.05])
+def test_canonical_output_switch_keeps_the_state(ps, fill):
+    """set_canonical_output(False) keeps the group-major order; the state is the
+    same entry for entry (amplitudes bit-identical)."""
+    spec = [("main", 12), ("anc", 3)]
+    rng = np.random.default_rng(12)
+    vals, amps = _random_rows(rng, spec, 1 << 14, fill=fill)
+    out = []
+    for on in (True, False):
+        ps.set_canonical_output(on)
+        try:
+            ps.System.clear()
+            st = _state(ps, spec, vals, amps)
+            ps.Hadamard_Int_Full("main")(st)
+            ps.Hadamard_Int("anc", 3)(st)
+            out.append(st.export())
+        finally:
+            ps.set_canonical_output(True)
+    assert_same_state(out[0], out[1], exact_amps=True, what=f"canonical on/off, fill {fill}")

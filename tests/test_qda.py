@@ -160,3 +160,47 @@ def test_poiseuille_2p20_40_steps_matches_published_residual(ps):
     r = qda_solve.solve(ps, nqubit=20, max_steps=40, cz_eps=1e-16, log=lambda s: None)
     assert abs(r["final_res_dir"] - 5.178906e-3) < 1e-8, r["final_res_dir"]
     assert r["final_fidelity"] > 0.9999985
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pass_through", [True, False])
+@pytest.mark.parametrize("n_entries", [1, 7, 400])
+def test_conditioned_group_unitary_pass_through(ps, oracle, n_entries, pass_through):
+    """Conditioned Rot_GeneralStatePrep / Rot_GeneralUnitary: groups whose
+    entries fail the conditions pass through unchanged inside the transform.
+    Conditions of every kind on registers beside the target (all ones, a value,
+    zero, nonzero, one bit set, two at once), groups where every entry
+    fails or passes, a single entry, and the state after every operator
+    against the oracle; also with the partition-first path (the default)."""
+    rng = np.random.default_rng(11 + n_entries)
+    q, _ = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))
+    rows = [[complex(x) for x in r] for r in q]
+    prep = [0.5, 0.3 + 0.2j, 0.1, 0.7]
+    ops = [
+        lambda m: m.Rot_GeneralStatePrep("u", prep).conditioned_by_all_ones("c"),
+        lambda m: m.Rot_GeneralStatePrep("u", prep).conditioned_by_value("m", 3),
+        lambda m: m.Rot_GeneralUnitary("u", rows).conditioned_by_nonzeros("m"),
+        lambda m: m.Rot_GeneralStatePrep("u", prep).conditioned_by_bit("m", 2),
+        lambda m: m.Rot_GeneralUnitary("u", rows).conditioned_by_all_ones(["c", "o"]),
+        lambda m: m.Rot_GeneralStatePrep("u", prep).conditioned_by_value("m", 31),   # no entry passes
+        lambda m: m.X_Bool("c", 0),
+        lambda m: m.Rot_GeneralStatePrep("u", prep).conditioned_by_value("o", 0),
+        lambda m: m.Rot_GeneralStatePrep("u", prep).dag,
+    ]
+    sts = {}
+    for mod in (ps, oracle):
+        mod.System.clear()
+        st = mod.SparseState()
+        for name, w in (("m", 5), ("u", 2), ("o", 1), ("c", 1)):
+            mod.AddRegister(name, mod.UnsignedInteger, w)(st)
+        vals, amps = random_state(mod, n_entries, seed=90 + n_entries)
+        st.load(vals, amps)
+        sts[mod.__name__] = st
+    ps.set_group_pass_through(pass_through)
+    try:
+        for step, op in enumerate(ops):
+            for mod in (ps, oracle):
+                op(mod)(sts[mod.__name__])
+            assert_same_state(sts["sparqsim_amd"].export(), sts["_oracle"].export(), what=f"n={n_entries} op {step}")
+    finally:
+        ps.set_group_pass_through(False)
