@@ -172,13 +172,15 @@ def test_short_ordering_passes_only(ps, n):
     assert stats["local_sorts"] == 1 and stats["lsd_sorts"] == 0, stats
 
 
-def test_presorted_prefix_skips_passes(ps):
+@pytest.mark.parametrize("hi_bits", [20, 14, 12])
+def test_presorted_prefix_skips_passes(ps, hi_bits):
     """Keys sorted by `hi` but not within equal `hi`: the key statistics find
-    the 20-bit sorted prefix and the chunk sort alone orders the state."""
+    the sorted prefix and the chunk sort alone orders the state: buckets of a
+    few entries (20-bit prefix), ~24 (14 bits) and ~100 (12 bits)."""
     n = 400_000
-    widths = [("hi", 20), ("lo", 12)]
+    widths = [("hi", hi_bits), ("lo", 12)]
     rng = np.random.default_rng(21)
-    vals = np.stack([np.sort(rng.integers(0, 1 << 20, n, dtype=np.uint64)),
+    vals = np.stack([np.sort(rng.integers(0, 1 << hi_bits, n, dtype=np.uint64)),
                      rng.integers(0, 1 << 12, n, dtype=np.uint64)], axis=1)
     amps = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     out, stats = _sort_stats_of(ps, widths, vals, amps)
