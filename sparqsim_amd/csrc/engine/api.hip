@@ -814,15 +814,11 @@ int sparq_pair_op(sparq_state_t h, sparq_field_t target, uint32_t bit, const spa
             if (f.word == target.word && tb >= f.shift && tb < (uint32_t)f.shift + f.width) inline_ok = false;
         }
         if (inline_ok) {
-            pair_op(st, target, bit, *params, cl);
-            clear_zero(st, eps);
+            pair_op(st, target, bit, *params, cl, eps);
             return;
         }
         st.pair_bit = -1;
-        with_conditions(st, conds, ncond, eps, [&](State& s) {
-            pair_op(s, target, bit, *params, CondList{});
-            clear_zero(s, eps);
-        });
+        with_conditions(st, conds, ncond, eps, [&](State& s) { pair_op(s, target, bit, *params, CondList{}, eps); });
         st.pair_bit = -1;   // the rest is appended behind the pairs
     });
 }

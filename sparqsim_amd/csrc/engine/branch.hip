@@ -1205,60 +1205,39 @@ void replicate(State& st, sparq_field_t f) {
 }
 
 // ---------------------------------------------------------------- pair ops
+// Entries sorted with the target bit least significant (or known to hold their
+// partners adjacent, bit clear first): a pair (i, i + 1) differs only in that
+// bit.  Both kernels evaluate the same device functions, so the count and the
+// write agree bit for bit; ClearZero (|a|^2 not below eps kept, as
+// compact.hip) is applied on the way: only kept outputs are counted and
+// written, the bit-clear output of a pair before its bit-set one.
+
+// Whether entry i (key k) has its partner adjacent; `one`: i's bit is set.
 template <int W>
-__global__ void __launch_bounds__(kBlock) pair_count_kernel(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, uint32_t word,
-                                                       uint64_t bm, const CondList cl, uint32_t* __restrict__ cnt) {
-    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint64_t k[W];
-    load_key<W>(keys, cap, i, k);
-    bool ok = true;
-    for (uint32_t c = 0; c < cl.n; ++c) ok = ok && cond_ok<W>(k, cl.c[c]);
-    if (!ok) {            // passes through unchanged
-        cnt[i] = 1;
-        return;
-    }
-    const bool one = kword<W>(k, word) & bm;
-    bool paired = false;
-    uint64_t j = one ? i - 1 : i + 1;
-    if ((one && i > 0) || (!one && i + 1 < n)) {
-        paired = true;
+__device__ __forceinline__ bool pair_partner(const uint64_t* __restrict__ keys, uint64_t cap, uint64_t n, uint64_t i,
+                                             const uint64_t (&k)[W], uint32_t word, uint64_t bm, bool& one) {
+    one = kword<W>(k, word) & bm;
+    const uint64_t j = one ? i - 1 : i + 1;
+    if (!((one && i > 0) || (!one && i + 1 < n))) return false;
+    bool paired = true;
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            uint64_t expect = (w == (int)word) ? (k[w] ^ bm) : k[w];
-            paired = paired && (keys[(uint64_t)w * cap + j] == expect);
-        }
+    for (int w = 0; w < W; ++w) {
+        const uint64_t expect = (w == (int)word) ? (k[w] ^ bm) : k[w];
+        paired = paired && (keys[(uint64_t)w * cap + j] == expect);
     }
-    cnt[i] = paired ? 1 : 2;
+    return paired;
 }
 
+// The two outputs (|bit clear>, |bit set>) of entry i, the bit-clear partner
+// of a pair or an unpaired entry.
 template <int W>
-__global__ void __launch_bounds__(kBlock) pair_apply_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
-                                                       uint64_t n, uint32_t word, uint64_t bm, const CondList cl,
-                                                       const uint32_t* __restrict__ cnt, const uint64_t* __restrict__ off,
-                                                       const sparq_pair_params_t p, uint64_t* __restrict__ kout,
-                                                       double2* __restrict__ aout, uint64_t cap_out) {
-    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint64_t k[W];
-    load_key<W>(keys, cap, i, k);
-    bool ok = true;
-    for (uint32_t c = 0; c < cl.n; ++c) ok = ok && cond_ok<W>(k, cl.c[c]);
-    if (!ok) {            // passes through unchanged
-        store_key<W>(kout, cap_out, off[i], k);
-        aout[off[i]] = amps[i];
-        return;
-    }
-    const bool one = kword<W>(k, word) & bm;
-    const bool paired = cnt[i] == 1;
-    if (paired && one) return;               // handled by the zero partner
-    const uint64_t o = off[i];
+__device__ __forceinline__ void pair_values(const double2* __restrict__ amps, uint64_t i, bool paired, bool one,
+                                            const uint64_t (&k)[W], const sparq_pair_params_t& p, double2& z2, double2& o2) {
     double2 z, on;                            // amplitudes of the |0> and |1> branches
     const double2 zero = make_double2(0.0, 0.0);
     if (paired) { z = amps[i]; on = amps[i + 1]; }
     else if (one) { z = zero; on = amps[i]; }
     else { z = amps[i]; on = zero; }
-    double2 z2, o2;
     switch (p.kind) {
         case SPARQ_PAIR_HADAMARD_BOOL:
             if (paired) {
@@ -1304,20 +1283,88 @@ __global__ void __launch_bounds__(kBlock) pair_apply_kernel(const uint64_t* __re
             break;
         }
     }
-    uint64_t kz[W], ko[W];
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-        kz[w] = (w == (int)word) ? (k[w] & ~bm) : k[w];
-        ko[w] = (w == (int)word) ? (k[w] | bm) : k[w];
+}
+
+// Kept outputs of entry i: (kept bit-clear output, kept bit-set output), or
+// for an entry failing the conditions whether it stays; the bit-set partner
+// of a pair contributes nothing (its pair's bit-clear partner writes both).
+template <int W>
+__device__ __forceinline__ void pair_entry(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap, uint64_t n,
+                                           uint64_t i, uint32_t word, uint64_t bm, const CondList& cl, const sparq_pair_params_t& p,
+                                           double eps, uint64_t (&k)[W], bool& pass, bool& keep_z, bool& keep_o, double2& z2,
+                                           double2& o2) {
+    load_key<W>(keys, cap, i, k);
+    bool ok = true;
+    for (uint32_t c = 0; c < cl.n; ++c) ok = ok && cond_ok<W>(k, cl.c[c]);
+    pass = !ok;
+    if (pass) {                                   // passes through unchanged, ClearZero'd
+        z2 = amps[i];
+        keep_z = !(cabs2(z2) < eps);
+        keep_o = false;
+        return;
     }
-    store_key<W>(kout, cap_out, o, kz);
-    store_key<W>(kout, cap_out, o + 1, ko);
-    aout[o] = z2;
-    aout[o + 1] = o2;
+    bool one;
+    const bool paired = pair_partner<W>(keys, cap, n, i, k, word, bm, one);
+    if (paired && one) {
+        keep_z = keep_o = false;
+        return;
+    }
+    pair_values<W>(amps, i, paired, one, k, p, z2, o2);
+    keep_z = !(cabs2(z2) < eps);
+    keep_o = !(cabs2(o2) < eps);
 }
 
 template <int W>
-static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl) {
+__global__ void __launch_bounds__(kBlock) pair_count_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                                       uint64_t n, uint32_t word, uint64_t bm, const CondList cl,
+                                                       const sparq_pair_params_t p, double eps, uint32_t* __restrict__ cnt) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k[W];
+    bool pass, kz, ko;
+    double2 z2, o2;
+    pair_entry<W>(keys, amps, cap, n, i, word, bm, cl, p, eps, k, pass, kz, ko, z2, o2);
+    cnt[i] = (uint32_t)kz + (uint32_t)ko;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) pair_apply_kernel(const uint64_t* __restrict__ keys, const double2* __restrict__ amps, uint64_t cap,
+                                                       uint64_t n, uint32_t word, uint64_t bm, const CondList cl,
+                                                       const uint64_t* __restrict__ off, const sparq_pair_params_t p, double eps,
+                                                       uint64_t* __restrict__ kout, double2* __restrict__ aout, uint64_t cap_out) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k[W];
+    bool pass, kz, ko;
+    double2 z2, o2;
+    pair_entry<W>(keys, amps, cap, n, i, word, bm, cl, p, eps, k, pass, kz, ko, z2, o2);
+    uint64_t o = off[i];
+    if (pass) {
+        if (kz) {
+            store_key<W>(kout, cap_out, o, k);
+            aout[o] = z2;
+        }
+        return;
+    }
+    if (kz) {
+        uint64_t kk[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) kk[w] = (w == (int)word) ? (k[w] & ~bm) : k[w];
+        store_key<W>(kout, cap_out, o, kk);
+        aout[o] = z2;
+        ++o;
+    }
+    if (ko) {
+        uint64_t kk[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) kk[w] = (w == (int)word) ? (k[w] | bm) : k[w];
+        store_key<W>(kout, cap_out, o, kk);
+        aout[o] = o2;
+    }
+}
+
+template <int W>
+static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl, double eps) {
     const uint64_t n = st.n;
     sparq_field_t b{target.word, (uint8_t)(target.shift + bit), 1, 0};
     // partners adjacent, bit-clear first: sorted with the bit least significant,
@@ -1329,37 +1376,38 @@ static void pair_impl(State& st, sparq_field_t target, uint32_t bit, const sparq
     uint32_t* cnt = dalloc<uint32_t>(n, st.stream);
     uint64_t* off = dalloc<uint64_t>(n + 1, st.stream);
     {
-        SPQ_PROF(st.stream, "pair_count_kernel", n * (8.0 * W + 4));
-        pair_count_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.cap, n, b.word, bm, cl, cnt);
+        SPQ_PROF(st.stream, "pair_count_kernel", n * (8.0 * W + 16 + 4));
+        pair_count_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, b.word, bm, cl, p, eps, cnt);
     }
     SPQ_HIP(hipGetLastError());
     exclusive_scan_u32(st, cnt, off, n, off + n);
     const uint64_t M = read_u64(st, off + n);
     uint64_t* kout;
     double2* aout;
-    alloc_arrays(st, st.W, M, &kout, &aout);
+    alloc_arrays(st, st.W, std::max<uint64_t>(M, 1), &kout, &aout);
     {
-        SPQ_PROF(st.stream, "pair_apply_kernel", n * (8.0 * W + 28) + M * (8.0 * W + 16));
-        pair_apply_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, b.word, bm, cl, cnt, off, p, kout,
-                                                                         aout, M);
+        SPQ_PROF(st.stream, "pair_apply_kernel", n * (8.0 * W + 16 + 8) + M * (8.0 * W + 16));
+        pair_apply_kernel<W><<<grid_for(n, kBlock), kBlock, 0, st.stream>>>(st.keys, st.amps, st.cap, n, b.word, bm, cl, off, p, eps, kout,
+                                                                         aout, std::max<uint64_t>(M, 1));
     }
     SPQ_HIP(hipGetLastError());
-    install_arrays(st, st.W, kout, aout, M, M);
+    install_arrays(st, st.W, kout, aout, M, std::max<uint64_t>(M, 1));
     dfree(cnt, st.stream);
     dfree(off, st.stream);
-    // every output pair is written as (bit clear, bit set) at adjacent slots;
-    // entries failing the conditions pass through beside their partners
+    // every kept output pair is written as (bit clear, bit set) at adjacent
+    // slots; entries failing the conditions pass through beside their
+    // partners; ClearZero only drops entries
     st.pair_bit = pair_bit;
 }
 
-void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl) {
+void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl, double eps) {
     SPQ_REQUIRE(target.word < st.W && bit < target.width, "bad target bit");
     SPQ_REQUIRE(p.kind <= SPARQ_PAIR_ROT_GENERAL, "unknown pair kind");
     if (p.kind == SPARQ_PAIR_CONDROT_RATIONAL || p.kind == SPARQ_PAIR_CONDROT_TABLE)
         SPQ_REQUIRE(p.in.word < st.W && p.in.width >= 1, "bad input register");
     if (p.kind == SPARQ_PAIR_CONDROT_TABLE) SPQ_REQUIRE(p.table != nullptr && p.in.width <= 24, "table-driven rotation needs a table");
     if (st.n == 0) return;
-    SPQ_DISPATCH_W(st.W, pair_impl, st, target, bit, p, cl);
+    SPQ_DISPATCH_W(st.W, pair_impl, st, target, bit, p, cl, eps);
 }
 
 // ---------------------------------------------------------------- captured transform

@@ -254,8 +254,10 @@ struct CondList {
     sparq_cond_t c[SPARQ_MAX_PROGRAM_CONDS];
 };
 // With conditions, entries that fail them pass through unchanged; the
-// conditions must not read the target bit (the caller checks).
-void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl);
+// conditions must not read the target bit (the caller checks).  The result is
+// ClearZero'd with eps (every entry, as the reference's ClearZero after the
+// operator).
+void pair_op(State& st, sparq_field_t target, uint32_t bit, const sparq_pair_params_t& p, const CondList& cl, double eps);
 
 // ---- pointwise (pointwise.hip)
 void pointwise(State& st, const sparq_pw_op_t* ops, uint32_t nops, const sparq_cond_t* conds, uint32_t nconds);
