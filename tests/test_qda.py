@@ -204,3 +204,19 @@ def test_conditioned_group_unitary_pass_through(ps, oracle, n_entries, pass_thro
             assert_same_state(sts["sparqsim_amd"].export(), sts["_oracle"].export(), what=f"n={n_entries} op {step}")
     finally:
         ps.set_group_pass_through(False)
+
+
+def test_solver_time_limit_and_logging(oracle):
+    """qda_solve's loop on the oracle at dim 8: a time limit stops it cleanly at
+    the first step boundary past the limit (termination names the limit, the
+    last curve point is that step), and every curve point carries its
+    interval time per step; a step cap logs every log_every steps and at the
+    cap; the ClearZero threshold the solver sets is restored."""
+    from sparqsim_amd import qda_solve
+
+    r = qda_solve.solve(oracle, nqubit=3, log_every=2, cz_eps=1e-15, time_limit=0.0, log=lambda s: None)
+    assert r["termination"] == "time_limit(0s)" and r["steps_run"] == 1 and r["curve"][-1]["step"] == 1
+    assert r["curve"][-1]["interval_ms_per_step"] > 0
+    r = qda_solve.solve(oracle, nqubit=3, log_every=4, cz_eps=1e-15, stop_after=6, log=lambda s: None)
+    assert r["termination"] == "fixed_step_budget" and [c["step"] for c in r["curve"]] == [4, 6]
+    assert oracle.get_clearzero_default_eps() == 1e-14   # the solver's threshold is restored
