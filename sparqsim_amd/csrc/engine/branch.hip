@@ -464,8 +464,10 @@ static void launch_wht(State& st, double2* a, const uint64_t* counts, uint64_t G
 // packing the varying bits outside the low target bits in canonical order.
 // The array is group-major like the grouped path's output, so its butterfly
 // passes run unchanged, on the groups that hold an entry (absent groups stay
-// zero and are skipped).  The kept entries are written in slot order, keys
-// rebuilt from the slot and the bits every key shares.  The scatter marks
+// zero and are skipped).  The kept entries are written in slot order, or in
+// the transposed (canonical) order below when that applies and the output is
+// not concentrated, keys rebuilt from the slot and the bits every key
+// shares.  The scatter marks
 // each slot it writes; a state with duplicate keys (fewer marks than
 // entries) keeps the grouped path.
 constexpr uint32_t kDenseMaxLog = 28;   // 2^28 slots: 4 GiB of amplitudes
