@@ -384,23 +384,35 @@ struct FieldMoves {
     sparq_field_t to[64];
 };
 
+// kRelayoutItems entries per lane (striped by the block size), all loads
+// issued before the first store: a key word is 8 bytes, so one entry per lane
+// keeps too few bytes in flight to cover the memory latency.
+constexpr int kRelayoutItems = 4;
 template <int WI, int WO>
 __global__ void __launch_bounds__(kBlock) relayout_kernel(const uint64_t* __restrict__ kin, uint64_t cap_in, uint64_t* __restrict__ kout,
                                                      uint64_t cap_out, uint64_t n, FieldMoves mv) {
-    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint64_t a[WI];
-    load_key<WI>(kin, cap_in, i, a);
-    uint64_t b[WO];
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kRelayoutItems) + threadIdx.x;
+    uint64_t a[kRelayoutItems][WI];
 #pragma unroll
-    for (int w = 0; w < WO; ++w) b[w] = 0;
-    for (uint32_t f = 0; f < mv.nf; ++f) fput<WO>(b, mv.to[f], fget<WI>(a, mv.from[f]));
-    store_key<WO>(kout, cap_out, i, b);
+    for (int it = 0; it < kRelayoutItems; ++it) {
+        const uint64_t i = base + (uint64_t)it * kBlock;
+        load_key<WI>(kin, cap_in, i < n ? i : n - 1, a[it]);   // clamped: in bounds, used only when i < n
+    }
+#pragma unroll
+    for (int it = 0; it < kRelayoutItems; ++it) {
+        const uint64_t i = base + (uint64_t)it * kBlock;
+        if (i >= n) break;
+        uint64_t b[WO];
+#pragma unroll
+        for (int w = 0; w < WO; ++w) b[w] = 0;
+        for (uint32_t f = 0; f < mv.nf; ++f) fput<WO>(b, mv.to[f], fget<WI>(a[it], mv.from[f]));
+        store_key<WO>(kout, cap_out, i, b);
+    }
 }
 
 template <int WI>
 static void launch_relayout_in(State& st, uint32_t WO, uint64_t* kout, uint64_t cap_out, const FieldMoves& mv) {
-    unsigned g = grid_for(st.n, kBlock);
+    unsigned g = grid_for(st.n, kBlock * kRelayoutItems);
 #define SPQ_RL(WOV)                                                                                          \
     case WOV: {                                                                                              \
         SPQ_PROF(st.stream, "relayout_kernel", st.n * 8.0 * (WI + WOV));                                     \
